@@ -1,0 +1,256 @@
+/*
+ * pf.h — C ABI of the MI355X-native particle solver (libpf.so).
+ *
+ * The ABI is the drop-in boundary for threejs-particle-fluids' per-substep
+ * solver: each entry point replaces one reference interface, cited as
+ * path:line in the reference repository. Plain pointers and sizes only: no
+ * device pointers, streams or torch types cross it. Every call returns 0 on
+ * success or a PF_E* code; pf_last_error() (thread-local) then holds the
+ * message, worded as the reference's "<Owner>: <reason>" errors so a host
+ * facade can rethrow it unchanged.
+ *
+ * Particle data crosses the ABI in original particle order and in the
+ * reference's layouts: vec4 xyzw float32 per particle for snapshots
+ * (particles.ts:27-38), xyz triples for uploads (ParticleInit, particles.ts:11-24).
+ * Inside the library particles are kept cell-sorted; readbacks undo that.
+ */
+#ifndef PF_H
+#define PF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_EINVAL 1       /* invalid argument (reference throws synchronously) */
+#define PF_EDEVICE 2      /* HIP runtime failure */
+#define PF_ESTATE 3       /* object disposed / wrong order of calls */
+#define PF_ENOMEM 4       /* device allocation failed */
+
+typedef struct pf_context_s pf_context;
+typedef struct pf_particles_s pf_particles;
+typedef struct pf_grid_s pf_grid;
+typedef struct pf_fluid_s pf_fluid;
+typedef struct pf_primitives_s pf_primitives;
+typedef struct pf_softbody_s pf_softbody;
+typedef struct pf_loop_s pf_loop;
+typedef struct pf_batch_s pf_batch;
+
+/* ---- errors / version ---------------------------------------------------- */
+const char* pf_last_error(void);
+uint32_t pf_abi_version(void);
+
+/* ---- context: one GPU, one HIP stream ------------------------------------
+ * Replaces the WebGPU renderer the reference threads through every object
+ * (createParticleRenderer, src/core/renderer.ts:16-33). */
+int pf_context_create(int device, pf_context** out);
+int pf_context_destroy(pf_context* ctx);
+int pf_context_synchronize(pf_context* ctx);
+/* Wall time of the stream between begin and end, from HIP events. */
+int pf_context_timer_begin(pf_context* ctx);
+int pf_context_timer_end(pf_context* ctx, float* ms);
+/* Per-kernel timing: while on, every launch is bracketed by HIP events on
+ * the library's stream; entries aggregate launches by kernel name. */
+int pf_context_set_profiling(pf_context* ctx, int on);
+int pf_context_profile_count(pf_context* ctx, uint32_t* n);
+int pf_context_profile_entry(pf_context* ctx, uint32_t index, char* name, uint32_t name_cap,
+                             uint32_t* launches, double* total_ms);
+int pf_context_profile_reset(pf_context* ctx);
+
+/* ---- ParticleSystem (src/core/particles.ts:52-258) ----------------------- */
+typedef struct pf_snapshot {
+  float* positions;           /* 4*capacity, nullable */
+  float* predicted_positions; /* 4*capacity, nullable */
+  float* velocities;          /* 4*capacity, nullable */
+  float* inv_mass;            /* capacity, nullable */
+  float* rotation;            /* 4*capacity, nullable */
+  float* predicted_rotation;  /* 4*capacity, nullable */
+  float* angular_velocity;    /* 4*capacity, nullable */
+} pf_snapshot;
+
+/* new ParticleSystem(renderer, capacity, particleRadius) (particles.ts:81-113) */
+int pf_particles_create(pf_context* ctx, uint32_t capacity, double particle_radius,
+                        pf_particles** out);
+/* dispose() (particles.ts:234-257); idempotent. */
+int pf_particles_destroy(pf_particles* ps);
+/* uploadParticles(data, start) (particles.ts:121-160): xyz triples; velocity
+ * nullable (0), inv_mass nullable (1), collision_group nullable (0). */
+int pf_particles_upload(pf_particles* ps, uint32_t start, uint32_t count, const float* positions,
+                        const float* velocities, const float* inv_mass,
+                        const uint32_t* collision_group);
+/* setInvMass(range, invMass) (particles.ts:163-173) */
+int pf_particles_set_inv_mass(pf_particles* ps, uint32_t start, uint32_t count, double inv_mass);
+/* setCollisionGroup(range, group) (particles.ts:176-186) */
+int pf_particles_set_collision_group(pf_particles* ps, uint32_t start, uint32_t count,
+                                     uint32_t group);
+/* readback() -> ParticleSnapshot (particles.ts:192-227) */
+int pf_particles_readback(pf_particles* ps, const pf_snapshot* out);
+int pf_particles_info(pf_particles* ps, uint32_t* capacity, double* particle_radius);
+
+/* ---- HashGrid (src/core/hashGrid/HashGrid.ts:80-286) ---------------------
+ * hash_table_size 0 = min(nextPow2(2*capacity), 2^20) (HashGrid.ts:121-124).
+ * Any power of two up to 2^30 is accepted (the reference caps at 2^20,
+ * HashGrid.ts:136-142, because of its two-level scan). origin nullable. */
+int pf_grid_create(pf_particles* ps, double cell_size, uint32_t hash_table_size,
+                   const double origin[3], pf_grid** out);
+/* A grid over its own position buffer (for sorting points that are not a
+ * ParticleSystem); positions uploaded with pf_grid_upload_positions. */
+int pf_grid_create_standalone(pf_context* ctx, uint32_t capacity, double cell_size,
+                              uint32_t hash_table_size, const double origin[3], pf_grid** out);
+int pf_grid_upload_positions(pf_grid* grid, const float* positions4, uint32_t count);
+/* rebuild() (HashGrid.ts:219-222); sort_positions=0 skips the sorted copy of
+ * the positions (sortedPositions.ts:31-39) when only the sort is wanted. */
+int pf_grid_rebuild(pf_grid* grid);
+int pf_grid_set_sorted_positions(pf_grid* grid, int enabled);
+/* readback() -> HashGridSnapshot (HashGrid.ts:230-249). counts/cell_start/
+ * cell_end hold hashTableSizePadded entries; each pointer nullable. */
+int pf_grid_readback(pf_grid* grid, uint32_t* cell_index, uint32_t* counts, uint32_t* cell_start,
+                     uint32_t* cell_end, uint32_t* sorted_indices);
+int pf_grid_readback_sorted_keys(pf_grid* grid, uint32_t* keys);
+int pf_grid_readback_sorted_positions(pf_grid* grid, float* positions4);
+/* readbackOverflow() (HashGrid.ts:256-260) */
+int pf_grid_readback_overflow(pf_grid* grid, int* overflow);
+int pf_grid_info(pf_grid* grid, uint32_t* capacity, uint32_t* hash_table_size,
+                 uint32_t* hash_table_size_padded);
+/* hashOrigin setter (HashGrid.ts:205-215) */
+int pf_grid_set_origin(pf_grid* grid, const double origin[3]);
+/* emitForEachNeighbor (query.ts:24-70) over a built standalone grid, as the
+ * reference's hash-grid tests use it: per query point, the number of
+ * candidates visited (own index excluded) and how many lie within radius. */
+int pf_grid_query_counts(pf_grid* grid, const float* query4, uint32_t nquery, double radius,
+                         uint32_t* candidates, uint32_t* within);
+int pf_grid_destroy(pf_grid* grid);
+
+/* ---- FluidSystem (src/fluids/FluidSystem.ts:71-322) ---------------------- */
+#define PF_FLUID_VISCOSITY 1u
+#define PF_FLUID_VORTICITY 2u
+#define PF_FLUID_SURFACE_TENSION 4u
+#define PF_FLUID_ADHESION 8u
+typedef struct pf_fluid_options {
+  uint32_t range_start, range_count; /* range_count 0 = every particle */
+  double rest_density;               /* <= 0: default 1000 */
+  double particle_spacing;           /* <= 0: default 2 * particle radius */
+  double smoothing_radius;           /* <= 0: default 2 * spacing */
+  double compliance;                 /* < 0: default 1e-4 */
+  uint32_t present;                  /* PF_FLUID_* bits: effects given in the options */
+  double viscosity, vorticity, surface_tension, adhesion;
+} pf_fluid_options;
+
+int pf_fluid_create(pf_particles* ps, const pf_fluid_options* options, pf_fluid** out);
+/* viscosity / vorticity / surfaceTension / adhesion setters (FluidSystem.ts:129-159) */
+int pf_fluid_set(pf_fluid* fluid, uint32_t effect, double value);
+int pf_fluid_get(pf_fluid* fluid, uint32_t effect, double* value);
+/* addBoundary(range, {dynamic}) (FluidSystem.ts:171-183) */
+int pf_fluid_add_boundary(pf_fluid* fluid, uint32_t start, uint32_t count, int dynamic);
+/* density buffer in original particle order (FluidSystem.ts:81) */
+int pf_fluid_readback_density(pf_fluid* fluid, float* density);
+/* readbackOverflow() (FluidSystem.ts:190-192) */
+int pf_fluid_readback_overflow(pf_fluid* fluid, int* overflow);
+/* mass, smoothing radius, particle spacing, rest density (FluidSystem.ts:75-84) */
+int pf_fluid_info(pf_fluid* fluid, double* mass, double* smoothing_radius,
+                  double* particle_spacing, double* rest_density);
+int pf_fluid_destroy(pf_fluid* fluid);
+
+/* ---- PrimitiveSet (src/core/collision/PrimitiveSet.ts:121-518) -----------
+ * mu_s / mu_k < 0 take the defaults 0.5 / 0.4 (PrimitiveSet.ts:529-544). */
+int pf_primitives_create(pf_particles* ps, uint32_t capacity, pf_primitives** out);
+int pf_primitives_add_plane(pf_primitives* set, const double normal[3], const double point[3],
+                            double mu_s, double mu_k, uint32_t* slot);
+int pf_primitives_add_sphere(pf_primitives* set, const double center[3], double radius,
+                             int invert, double mu_s, double mu_k, uint32_t* slot);
+/* rotation: quaternion xyzw, nullable (identity) */
+int pf_primitives_add_box(pf_primitives* set, const double center[3], const double half_extents[3],
+                          const double rotation[4], int invert, double mu_s, double mu_k,
+                          uint32_t* slot);
+int pf_primitives_add_capsule(pf_primitives* set, const double a[3], const double b[3],
+                              double radius, double mu_s, double mu_k, uint32_t* slot);
+/* setSphere(slot, center, radius, velocity?) (PrimitiveSet.ts:241-256) */
+int pf_primitives_set_sphere(pf_primitives* set, uint32_t slot, const double center[3],
+                             double radius, const double velocity[3]);
+/* attach()+update() for a host-driven pose: place the primitive's reference
+ * point at `position` with orientation `rotation` (xyzw); with dt > 0 the
+ * surface velocity and spin follow from the previous pose
+ * (PrimitiveSet.ts:211-285). The first call only records the pose. */
+int pf_primitives_set_pose(pf_primitives* set, uint32_t slot, const double position[3],
+                           const double rotation[4], double dt);
+int pf_primitives_count(pf_primitives* set, uint32_t* count);
+int pf_primitives_destroy(pf_primitives* set);
+
+/* ---- SoftbodySystem, global shape matching (src/softbody/SoftbodySystem.ts) */
+typedef struct pf_softbody_def {
+  uint32_t start, count;   /* particle range of the body */
+  uint32_t surface_count;  /* 0xffffffff = all */
+  double compliance;       /* s^2/kg, >= 0 */
+  const float* rest_positions; /* xyz triples, nullable = uploaded positions */
+} pf_softbody_def;
+int pf_softbody_create(pf_particles* ps, const pf_softbody_def* bodies, uint32_t nbodies,
+                       pf_softbody** out);
+/* setCompliance(index, compliance) (SoftbodySystem.ts:220-224) */
+int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double compliance);
+/* bodyCenters (vec4 per body) and bodyRotations (3 vec4 rows per body) */
+int pf_softbody_readback_frames(pf_softbody* sb, float* centers4, float* rotations12);
+int pf_softbody_destroy(pf_softbody* sb);
+
+/* ---- SimLoop (src/core/loop.ts:92-324) ----------------------------------- */
+#define PF_MATERIAL_FLUID 1u
+#define PF_MATERIAL_SOFTBODY 2u
+typedef struct pf_material {
+  uint32_t kind;   /* PF_MATERIAL_* */
+  void* handle;    /* pf_fluid* or pf_softbody* */
+} pf_material;
+typedef struct pf_loop_options {
+  uint32_t substeps;      /* 0 = default 4 */
+  uint32_t iterations;    /* 0xffffffff = default 2 */
+  double gravity[3];
+  int has_hash_origin;
+  double hash_origin[3];
+} pf_loop_options;
+typedef struct pf_overflow {
+  int positions, velocities, grid, contacts, neighbors, impulses, long_bucket;
+} pf_overflow;
+
+int pf_loop_create(pf_particles* ps, const pf_loop_options* options, const pf_material* materials,
+                   uint32_t nmaterials, pf_primitives* const* colliders, uint32_t ncolliders,
+                   pf_loop** out);
+/* step(dt) (loop.ts:267-279): enqueues every substep and returns. */
+int pf_loop_step(pf_loop* loop, double dt);
+int pf_loop_synchronize(pf_loop* loop);
+/* readbackOverflow() (loop.ts:286-295) */
+int pf_loop_readback_overflow(pf_loop* loop, pf_overflow* out);
+int pf_loop_set_substeps(pf_loop* loop, uint32_t substeps);
+int pf_loop_set_gravity(pf_loop* loop, const double gravity[3]);
+int pf_loop_info(pf_loop* loop, uint32_t* substeps, uint32_t* iterations, double* cell_size,
+                 uint32_t* hash_table_size, uint32_t* kernels_per_step);
+/* The loop's grid (SimLoop.hashGrid, loop.ts:104-106); owned by the loop. */
+int pf_loop_grid(pf_loop* loop, pf_grid** out);
+/* Capture each step's launches in a HIP graph and replay it (1) or launch
+ * them one by one (0, default). */
+int pf_loop_set_graph(pf_loop* loop, int enabled);
+int pf_loop_destroy(pf_loop* loop);
+
+/* ---- Batched form: many independent loops advanced by one launch sequence.
+ * No counterpart in the reference. All loops must share substeps and
+ * iterations; every launch covers the work tiles of every scene. */
+int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t nloops, pf_batch** out);
+int pf_batch_step(pf_batch* batch, double dt);
+int pf_batch_set_graph(pf_batch* batch, int enabled);
+int pf_batch_destroy(pf_batch* batch);
+
+/* ---- Scene construction (src/simulation/layout.ts:7-42, Simulation.ts:957-1104) */
+/* fillBox(box, spacing): lattice points, y -> z -> x order. out may be null to count. */
+int pf_layout_fill_box(const double box_min[3], const double box_max[3], double spacing,
+                       float* out_xyz, uint32_t capacity, uint32_t* count);
+/* radiusForBudget(volume, area, clothSpacing, budget) (layout.ts:27-42) */
+double pf_layout_radius_for_budget(double volume, double area, double cloth_spacing,
+                                   uint32_t budget);
+/* Simulation.fitBudget for box fluids in a container (Simulation.ts:957-1002 with
+ * layout 1044-1104): radius and particle count of the best fitting layout. */
+int pf_layout_fit_budget(uint32_t budget, const double container[6], const double* fluid_boxes,
+                         uint32_t nboxes, double* radius, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PF_H */

@@ -1,0 +1,209 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or as the timed CPU baseline.
+It restates the reference's algorithm (see pf_oracle.c / orc_sim.h for the
+file:line of each pass).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+P = C.POINTER
+u32, i32, f32, f64, vp = C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_void_p
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    L.orc_part1by2.argtypes = [u32]
+    L.orc_part1by2.restype = u32
+    L.orc_morton.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+    L.orc_morton.restype = u32
+    L.orc_grid_build.argtypes = [P(f32), u32, f32, P(f32), u32, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32)]
+    L.orc_grid_build.restype = i32
+    L.orc_grid_query_counts.argtypes = [P(f32), u32, f32, P(f32), u32, P(f32), u32, f32, P(u32), P(u32)]
+    L.orc_grid_query_counts.restype = None
+    for suf in ("f32", "f64"):
+        g = lambda n: getattr(L, f"orc_{n}_{suf}")
+        g("scene_create").argtypes = [u32, f64]
+        g("scene_create").restype = vp
+        g("scene_destroy").argtypes = [vp]
+        g("scene_destroy").restype = None
+        g("scene_set_particles").argtypes = [vp, P(f32), P(f32), P(f32)]
+        g("scene_set_particles").restype = None
+        g("scene_set_loop").argtypes = [vp, u32, u32, P(f64), f64, u32, P(f64)]
+        g("scene_set_loop").restype = None
+        g("scene_add_fluid").argtypes = [vp, u32, u32, f64, f64, f64, f64, u32, f64, f64, f64, f64]
+        g("scene_add_fluid").restype = i32
+        g("scene_add_prim").argtypes = [vp, u32, u32, P(f64), P(f64), P(f64), P(f64), P(f64)]
+        g("scene_add_prim").restype = i32
+        g("scene_add_body").argtypes = [vp, u32, u32, f64, P(f32), P(f32)]
+        g("scene_add_body").restype = i32
+        g("scene_step").argtypes = [vp, f64]
+        g("scene_step").restype = None
+        g("scene_get").argtypes = [vp, P(f64), P(f64), P(f64), P(f64)]
+        g("scene_get").restype = None
+        g("scene_flags").argtypes = [vp, P(i32)]
+        g("scene_flags").restype = None
+        g("scene_body_frame").argtypes = [vp, u32, P(f64), P(f64)]
+        g("scene_body_frame").restype = None
+        g("polar").argtypes = [P(f64), P(f64)]
+        g("polar").restype = None
+        g("scene_neighbors").argtypes = [vp, u32, P(u32), P(u32)]
+        g("scene_neighbors").restype = None
+    L.orc_poly6_f32.argtypes = [f64, f64]
+    L.orc_poly6_f32.restype = f64
+    L.orc_spiky_f32.argtypes = [P(f64), f64, P(f64)]
+    L.orc_spiky_f32.restype = None
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(P(t))
+
+
+def morton(cx, cy, cz):
+    return lib().orc_morton(int(cx), int(cy), int(cz))
+
+
+def part1by2(n):
+    return lib().orc_part1by2(int(n))
+
+
+def grid_build(pos4, cell, H, origin=(0.0, 0.0, 0.0)):
+    """Stable counting sort: dict of cellIndex, counts, cellStart, cellEnd,
+    sortedIndices, sortedKeys, overflow (counts etc. padded to 1024)."""
+    pos = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+    n = pos.shape[0]
+    Hp = (H + 1023) // 1024 * 1024
+    out = dict(cellIndex=np.zeros(n, np.uint32), counts=np.zeros(Hp, np.uint32), cellStart=np.zeros(Hp, np.uint32),
+               cellEnd=np.zeros(Hp, np.uint32), sortedIndices=np.zeros(n, np.uint32), sortedKeys=np.zeros(n, np.uint32))
+    o = np.asarray(origin, np.float32)
+    over = lib().orc_grid_build(_p(pos, f32), n, f32(cell), _p(o, f32), H, _p(out["cellIndex"], u32), _p(out["counts"], u32),
+                                _p(out["cellStart"], u32), _p(out["cellEnd"], u32), _p(out["sortedIndices"], u32),
+                                _p(out["sortedKeys"], u32))
+    out["overflow"] = bool(over)
+    return out
+
+
+def grid_query_counts(pos4, cell, H, query4, radius, origin=(0.0, 0.0, 0.0)):
+    pos = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+    q = np.ascontiguousarray(query4, np.float32).reshape(-1, 4)
+    o = np.asarray(origin, np.float32)
+    cand = np.zeros(q.shape[0], np.uint32)
+    within = np.zeros(q.shape[0], np.uint32)
+    lib().orc_grid_query_counts(_p(pos, f32), pos.shape[0], f32(cell), _p(o, f32), H, _p(q, f32), q.shape[0], f32(radius),
+                                _p(cand, u32), _p(within, u32))
+    return cand, within
+
+
+def polar(a, precision="f32"):
+    a9 = np.ascontiguousarray(np.asarray(a, np.float64).reshape(9))
+    r9 = np.zeros(9, np.float64)
+    getattr(lib(), f"orc_polar_{precision}")(_p(a9, f64), _p(r9, f64))
+    return r9.reshape(3, 3)
+
+
+class Scene:
+    """One scene stepped on the CPU, in FP32 ("f32") or FP64 ("f64")."""
+
+    def __init__(self, n, radius, precision="f32"):
+        self.suf = precision
+        self.L = lib()
+        self.n = int(n)
+        self.h = self._f("scene_create")(self.n, float(radius))
+
+    def _f(self, name):
+        return getattr(self.L, f"orc_{name}_{self.suf}")
+
+    def set_particles(self, x3, v3=None, w=None):
+        x = np.ascontiguousarray(x3, np.float32).reshape(-1)
+        v = None if v3 is None else np.ascontiguousarray(v3, np.float32).reshape(-1)
+        ww = None if w is None else np.ascontiguousarray(w, np.float32)
+        self._f("scene_set_particles")(self.h, _p(x, f32), _p(v, f32), _p(ww, f32))
+
+    def set_loop(self, substeps, iterations, gravity, cell, H, origin=(0.0, 0.0, 0.0)):
+        g = np.asarray(gravity, np.float64)
+        o = np.asarray(origin, np.float64)
+        self._f("scene_set_loop")(self.h, int(substeps), int(iterations), _p(g, f64), float(cell), int(H), _p(o, f64))
+
+    def add_fluid(self, start, count, rho0, spacing, h, compliance=1e-4, viscosity=None, vorticity=None,
+                  surface_tension=None, adhesion=None):
+        eff = 0
+        eff |= 1 if viscosity is not None else 0
+        eff |= 2 if vorticity is not None else 0
+        eff |= 4 if surface_tension is not None else 0
+        eff |= 8 if adhesion is not None else 0
+        rc = self._f("scene_add_fluid")(self.h, int(start), int(count), float(rho0), float(spacing), float(h),
+                                        float(compliance), eff, float(viscosity or 0), float(vorticity or 0),
+                                        float(surface_tension or 0), float(adhesion or 0))
+        assert rc == 0
+
+    def add_prim(self, kind, flags, d0, d1, vel, rot=(0, 0, 0, 1), spin=(0, 0, 0)):
+        arr = [np.asarray(a, np.float64) for a in (d0, d1, vel, rot, spin)]
+        rc = self._f("scene_add_prim")(self.h, int(kind), int(flags), *[_p(a, f64) for a in arr])
+        assert rc == 0
+
+    def add_plane(self, normal, point, mus=0.5, muk=0.4):
+        n = np.asarray(normal, np.float64)
+        n = n * (1.0 / np.sqrt((n * n).sum()))
+        self.add_prim(0, 0, [n[0], n[1], n[2], 0], [point[0], point[1], point[2], mus], [0, 0, 0, muk])
+
+    def add_body(self, start, count, compliance, rest3, weights):
+        r = np.ascontiguousarray(rest3, np.float32).reshape(-1)
+        w = np.ascontiguousarray(weights, np.float32)
+        rc = self._f("scene_add_body")(self.h, int(start), int(count), float(compliance), _p(r, f32), _p(w, f32))
+        assert rc == 0
+
+    def step(self, dt):
+        self._f("scene_step")(self.h, float(dt))
+
+    def get(self):
+        x = np.zeros((self.n, 3))
+        xs = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        rho = np.zeros(self.n)
+        self._f("scene_get")(self.h, _p(x, f64), _p(xs, f64), _p(v, f64), _p(rho, f64))
+        return dict(x=x, xs=xs, v=v, rho=rho)
+
+    def flags(self):
+        f = np.zeros(5, np.int32)
+        self._f("scene_flags")(self.h, _p(f, i32))
+        return dict(grid=bool(f[0]), neighbors=bool(f[1]), positions=bool(f[2]), velocities=bool(f[3]), impulses=bool(f[4]))
+
+    def body_frame(self, b):
+        c = np.zeros(3)
+        r = np.zeros(9)
+        self._f("scene_body_frame")(self.h, int(b), _p(c, f64), _p(r, f64))
+        return c, r.reshape(3, 3)
+
+    def neighbors(self, f=0):
+        nl = np.zeros((self.n, 64), np.uint32)
+        cnt = np.zeros(self.n, np.uint32)
+        self._f("scene_neighbors")(self.h, int(f), _p(nl, u32), _p(cnt, u32))
+        return nl, cnt
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._f("scene_destroy")(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -1,0 +1,739 @@
+/*
+ * CPU restatement of one SimLoop substep schedule, included twice by
+ * pf_oracle.c: with REAL=float (the FP32 mirror) and REAL=double (FP64).
+ * TEST INFRASTRUCTURE ONLY: it checks the GPU path, it is never the product.
+ *
+ * Each function cites the reference pass it restates. Particles stay in
+ * their original order; the neighbour order is the one a stable counting
+ * sort gives (27 buckets dz->dy->dx, ascending particle index inside a
+ * bucket), which is one of the orders the reference's atomic scatter may
+ * produce (sort.ts:187-192).
+ *
+ * Expected macros: REAL, SFX(name) (symbol suffixing), R(x) (literal),
+ * RSQRT, RFLOOR, RFMAX, RFMIN, RFABS, RSIN, RCOS, RRINT.
+ */
+
+typedef struct {
+  uint32_t start, count, effects;
+  REAL rho0, pv, mass, h, h2, poly6, spiky, compliance;
+  REAL gamma, visc, vort, adhesion;
+  REAL coh_coef, coh_h6_64, adh_coef, adh_m4_over_h, adh_2h;
+  REAL imp_scale, imp_inv;
+  uint32_t* nl;     /* [64 n] row-major: nl[i*64 + k] */
+  uint32_t* nlc;    /* [n] */
+  REAL* normal;     /* [3n] */
+  REAL* lambda;     /* [n] */
+  REAL* dx;         /* [3n] */
+  REAL* omega;      /* [3n] */
+  REAL* omlen;      /* [n] */
+  REAL* eta;        /* [3n] */
+  REAL* dv;         /* [3n] */
+} SFX(fluid);
+
+typedef struct {
+  uint32_t kind, flags;
+  REAL d0[4], d1[4], vel[4], rot[4], spin[3];
+} SFX(prim);
+
+typedef struct {
+  uint32_t n;
+  REAL radius;
+  REAL *x, *xs, *v, *w, *rho, *bvol; /* x, xs, v: [3n] */
+  /* grid */
+  REAL cell, origin[3];
+  uint32_t H;
+  uint32_t *key, *start, *perm; /* start: [H+1] */
+  /* materials */
+  uint32_t nfluids;
+  SFX(fluid) fluid[4];
+  uint32_t nprim;
+  SFX(prim) prim[32];
+  REAL* plam;      /* [2 * 32 * n] */
+  int32_t *pacc, *vacc, *iacc; /* [3n] fixed-point accumulators */
+  REAL pos_scale, pos_inv, vel_scale, vel_inv;
+  /* soft bodies (global) */
+  uint32_t nbodies;
+  uint32_t bstart[4096], bcount[4096];
+  REAL bcomp[4096];
+  REAL *rest, *weights, *slam; /* rest/slam: [3n] */
+  REAL centers[4096][3];
+  REAL rot[4096][9];
+  /* loop */
+  uint32_t substeps, iterations;
+  REAL g[3], dt;
+  REAL clock[64];
+  /* flags */
+  int flag_grid, flag_nl, flag_pos, flag_vel, flag_imp;
+} SFX(scene);
+
+/* --------------------------------------------------------------------- */
+/* small vector helpers, operand order as the GPU kernels                */
+/* --------------------------------------------------------------------- */
+static inline REAL SFX(dot)(const REAL a[3], const REAL b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline void SFX(cross)(const REAL a[3], const REAL b[3], REAL o[3]) {
+  REAL t0 = a[1] * b[2] - a[2] * b[1];
+  REAL t1 = a[2] * b[0] - a[0] * b[2];
+  REAL t2 = a[0] * b[1] - a[1] * b[0];
+  o[0] = t0; o[1] = t1; o[2] = t2;
+}
+
+/* kernels.ts:64-69 */
+static inline REAL SFX(poly6)(REAL r2, REAL h2, REAL coef) {
+  REAL d = RFMAX(h2 - r2, R(0.0));
+  return ((d * d) * d) * coef;
+}
+
+/* kernels.ts:89-107 */
+static inline void SFX(spiky)(const REAL r[3], REAL h, REAL coef, REAL g[3]) {
+  REAL rsq = SFX(dot)(r, r);
+  REAL rl = RSQRT(rsq);
+  REAL t = RFMAX(h - rl, R(0.0));
+  REAL rs = RFMAX(rl, (REAL)1e-20);
+  REAL mag = -((t * t) * coef);
+  g[0] = (r[0] / rs) * mag;
+  g[1] = (r[1] / rs) * mag;
+  g[2] = (r[2] / rs) * mag;
+}
+
+/* accumulator.ts:62-68: i32(round(v * scale)), half-to-even, saturating */
+static inline int32_t SFX(ticks)(REAL v, REAL scale) {
+  REAL t = RRINT(v * scale);
+  if (!(t == t)) return 0;
+  if (t >= (REAL)2147483648.0) return 2147483647;
+  if (t < (REAL)-2147483648.0) return (int32_t)(-2147483647 - 1);
+  return (int32_t)t;
+}
+static inline int32_t SFX(wadd)(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+/* accumulator.ts:101-131: returns 1 when saturated */
+static inline int SFX(acc_change)(const int32_t* t, REAL inv, REAL c[3]) {
+  REAL tx = (REAL)t[0], ty = (REAL)t[1], tz = (REAL)t[2];
+  REAL m = RFMAX(RFMAX(RFABS(tx), RFABS(ty)), RFABS(tz));
+  c[0] = tx * inv; c[1] = ty * inv; c[2] = tz * inv;
+  return m >= (REAL)1073741824.0;
+}
+
+/* --------------------------------------------------------------------- */
+/* grid (cellIndex.ts:113-170, sort.ts, stable)                           */
+/* --------------------------------------------------------------------- */
+static int32_t SFX(bin_coord)(REAL p, REAL o, REAL cell, int* over) {
+  REAL rel = (p - o) / cell;
+  int32_t raw = orc_f2i_sat((double)RFLOOR(rel));
+  int32_t c = raw < -(1 << 30) ? -(1 << 30) : (raw > (1 << 30) - 1 ? (1 << 30) - 1 : raw);
+  if (raw != c || c < -512 || c >= 512) *over = 1;
+  return c;
+}
+
+static void SFX(grid_build)(SFX(scene)* s) {
+  uint32_t n = s->n, H = s->H;
+  uint32_t* counts = (uint32_t*)calloc(H, sizeof(uint32_t));
+  int over = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const REAL* p = s->xs + 3 * (size_t)i;
+    int32_t cx = SFX(bin_coord)(p[0], s->origin[0], s->cell, &over);
+    int32_t cy = SFX(bin_coord)(p[1], s->origin[1], s->cell, &over);
+    int32_t cz = SFX(bin_coord)(p[2], s->origin[2], s->cell, &over);
+    uint32_t b = orc_morton(cx, cy, cz) & (H - 1);
+    s->key[i] = b;
+    counts[b]++;
+  }
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < H; ++b) {
+    s->start[b] = run;
+    run += counts[b];
+  }
+  s->start[H] = run;
+  for (uint32_t b = 0; b < H; ++b) counts[b] = s->start[b];
+  for (uint32_t i = 0; i < n; ++i) s->perm[counts[s->key[i]]++] = i;
+  free(counts);
+  s->flag_grid = over;
+}
+
+/* neighborList.ts:45-72 over query.ts:24-70 */
+static void SFX(neighbors)(SFX(scene)* s, SFX(fluid)* F) {
+  uint32_t H = s->H;
+  int over = 0;
+  for (uint32_t i = F->start; i < F->start + F->count; ++i) {
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    int32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = orc_f2i_sat((double)RFLOOR((xi[a] - s->origin[a]) / s->cell));
+    uint32_t visited[27], nv = 0, cnt = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          uint32_t b = orc_morton((int32_t)((uint32_t)q[0] + (uint32_t)dx), (int32_t)((uint32_t)q[1] + (uint32_t)dy),
+                                  (int32_t)((uint32_t)q[2] + (uint32_t)dz)) & (H - 1);
+          int seen = 0;
+          for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
+          visited[nv++] = b;
+          if (seen) continue;
+          for (uint32_t t = s->start[b]; t < s->start[b + 1]; ++t) {
+            uint32_t j = s->perm[t];
+            const REAL* xj = s->xs + 3 * (size_t)j;
+            REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            if (SFX(dot)(d, d) < F->h2) {
+              if (cnt < 64) F->nl[(size_t)i * 64 + cnt++] = j;
+              else over = 1;
+            }
+          }
+        }
+    F->nlc[i] = cnt;
+  }
+  s->flag_nl |= over;
+}
+
+static inline REAL SFX(nvol)(const SFX(scene)* s, const SFX(fluid)* F, uint32_t j) {
+  return s->bvol[j] > R(0.0) ? s->bvol[j] : F->pv;
+}
+static inline REAL SFX(nmass)(const SFX(scene)* s, const SFX(fluid)* F, uint32_t j) {
+  if (s->bvol[j] > R(0.0)) return F->rho0 * s->bvol[j];
+  return s->w[j] > R(0.0) ? R(1.0) / s->w[j] : R(0.0);
+}
+
+/* cohesion.ts:25-39 */
+static void SFX(color_normal)(SFX(scene)* s, SFX(fluid)* F) {
+  for (uint32_t i = F->start; i < F->start + F->count; ++i) {
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    REAL sum[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+      uint32_t j = F->nl[(size_t)i * 64 + k];
+      const REAL* xj = s->xs + 3 * (size_t)j;
+      REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
+      SFX(spiky)(d, F->h, F->spiky, g);
+      REAL V = SFX(nvol)(s, F, j);
+      sum[0] = sum[0] + g[0] * V; sum[1] = sum[1] + g[1] * V; sum[2] = sum[2] + g[2] * V;
+    }
+    for (int a = 0; a < 3; ++a) F->normal[3 * (size_t)i + a] = sum[a] * F->h;
+  }
+}
+
+/* cohesion.ts:46-55 */
+static inline REAL SFX(coh_spline)(const SFX(fluid)* F, REAL r) {
+  REAL gap = RFMAX(F->h - r, R(0.0));
+  REAL shared = ((gap * gap) * gap) * ((r * r) * r);
+  REAL value = (r * R(2.0) > F->h) ? shared : (shared * R(2.0) - F->coh_h6_64);
+  return (r > R(0.0) && r <= F->h) ? value * F->coh_coef : R(0.0);
+}
+
+/* adhesion.ts:16-27 */
+static inline REAL SFX(adh_spline)(const SFX(fluid)* F, REAL r) {
+  REAL root = RSQRT(RSQRT(RFMAX(((r * r) * F->adh_m4_over_h + r * R(6.0)) - F->adh_2h, R(0.0))));
+  return (r * R(2.0) > F->h && r <= F->h) ? root * F->adh_coef : R(0.0);
+}
+
+/* cohesion.ts:63-103 (pairs j > i inside the range, scattered both ways),
+ * adhesion.ts:34-60, then the impulse apply (FluidSystem.ts:253-258). */
+static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
+  int st = (F->effects & 4u) != 0, adh = (F->effects & 8u) != 0;
+  uint32_t end = F->start + F->count;
+  for (uint32_t i = F->start; i < end; ++i) {
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+      uint32_t j = F->nl[(size_t)i * 64 + k];
+      const REAL* xj = s->xs + 3 * (size_t)j;
+      if (st && j > i && j < end) {
+        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        REAL r = RSQRT(SFX(dot)(off, off));
+        if (!(r < (REAL)1e-20)) {
+          REAL corr = (F->rho0 * R(2.0)) / RFMAX(s->rho[i] + s->rho[j], R(1.0));
+          REAL cs = -(((F->gamma * F->mass) * F->mass) * SFX(coh_spline)(F, r));
+          REAL gm = F->gamma * F->mass;
+          for (int a = 0; a < 3; ++a) {
+            REAL coh = (off[a] / r) * cs;
+            REAL curv = -((F->normal[3 * (size_t)i + a] - F->normal[3 * (size_t)j + a]) * gm);
+            REAL dv = (((coh + curv) * corr) / F->mass) * s->dt;
+            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)(dv, F->imp_scale));
+            s->iacc[3 * (size_t)j + a] = SFX(wadd)(s->iacc[3 * (size_t)j + a], SFX(ticks)(-dv, F->imp_scale));
+          }
+        }
+      }
+      if (adh && s->bvol[j] > R(0.0)) {
+        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        REAL r = RSQRT(SFX(dot)(off, off));
+        if (!(r < (REAL)1e-20)) {
+          REAL fs = -(((F->adhesion * F->mass) * (F->rho0 * s->bvol[j])) * SFX(adh_spline)(F, r));
+          for (int a = 0; a < 3; ++a) {
+            REAL f = (off[a] / r) * fs;
+            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)((f / F->mass) * s->dt, F->imp_scale));
+          }
+        }
+      }
+    }
+  }
+  for (uint32_t i = F->start; i < end; ++i) {
+    REAL c[3];
+    if (SFX(acc_change)(s->iacc + 3 * (size_t)i, F->imp_inv, c)) s->flag_imp = 1;
+    for (int a = 0; a < 3; ++a) {
+      s->v[3 * (size_t)i + a] = s->v[3 * (size_t)i + a] + c[a];
+      s->xs[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a] + c[a] * s->dt;
+      s->iacc[3 * (size_t)i + a] = 0;
+    }
+  }
+}
+
+/* lambda.ts:30-71 */
+static void SFX(lambda_pass)(SFX(scene)* s, SFX(fluid)* F) {
+  for (uint32_t i = F->start; i < F->start + F->count; ++i) {
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    REAL rho = 0, gss = 0, gs[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+      uint32_t j = F->nl[(size_t)i * 64 + k];
+      const REAL* xj = s->xs + 3 * (size_t)j;
+      REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
+      REAL mj = SFX(nmass)(s, F, j);
+      rho = rho + mj * SFX(poly6)(SFX(dot)(d, d), F->h2, F->poly6);
+      SFX(spiky)(d, F->h, F->spiky, g);
+      g[0] = g[0] * mj; g[1] = g[1] * mj; g[2] = g[2] * mj;
+      gs[0] = gs[0] + g[0]; gs[1] = gs[1] + g[1]; gs[2] = gs[2] + g[2];
+      gss = gss + SFX(dot)(g, g) * s->w[j];
+    }
+    s->rho[i] = rho;
+    REAL inv = R(1.0) / F->rho0;
+    REAL C = rho * inv - R(1.0);
+    REAL lam = 0;
+    if (!(C <= R(0.0))) {
+      REAL gn = (SFX(dot)(gs, gs) * s->w[i] + gss) * (inv * inv);
+      REAL at = F->compliance / (s->dt * s->dt);
+      lam = -C / RFMAX(gn + at, (REAL)1e-20);
+    }
+    F->lambda[i] = lam;
+  }
+}
+
+/* positionDelta.ts:26-85 (deltaX pass, then the separate apply) */
+static void SFX(delta_pass)(SFX(scene)* s, SFX(fluid)* F) {
+  uint32_t end = F->start + F->count;
+  for (uint32_t i = F->start; i < end; ++i) {
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    REAL li = F->lambda[i], sum[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+      uint32_t j = F->nl[(size_t)i * 64 + k];
+      const REAL* xj = s->xs + 3 * (size_t)j;
+      REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
+      SFX(spiky)(d, F->h, F->spiky, g);
+      REAL lj = (j >= F->start && j < end) ? F->lambda[j] : R(0.0);
+      REAL sc = (li + lj) * SFX(nmass)(s, F, j);
+      sum[0] = sum[0] + g[0] * sc; sum[1] = sum[1] + g[1] * sc; sum[2] = sum[2] + g[2] * sc;
+    }
+    for (int a = 0; a < 3; ++a) F->dx[3 * (size_t)i + a] = (sum[a] / F->rho0) * s->w[i];
+  }
+  for (uint32_t i = F->start; i < end; ++i)
+    for (int a = 0; a < 3; ++a) s->xs[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a] + F->dx[3 * (size_t)i + a];
+}
+
+/* velocity.ts:36-119 */
+static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
+  int vort = (F->effects & 2u) != 0, visc = (F->effects & 1u) != 0;
+  if (!vort && !visc) return;
+  uint32_t end = F->start + F->count;
+  for (uint32_t i = F->start; i < end; ++i) {
+    const REAL* xi = s->x + 3 * (size_t)i;
+    const REAL* vi = s->v + 3 * (size_t)i;
+    REAL curl[3] = {0, 0, 0}, sm[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+      uint32_t j = F->nl[(size_t)i * 64 + k];
+      const REAL* xj = s->x + 3 * (size_t)j;
+      const REAL* vj = s->v + 3 * (size_t)j;
+      REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      REAL V = SFX(nvol)(s, F, j);
+      if (vort) {
+        REAL g[3], dv[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]}, c[3];
+        SFX(spiky)(off, F->h, F->spiky, g);
+        SFX(cross)(dv, g, c);
+        curl[0] = curl[0] + c[0] * V; curl[1] = curl[1] + c[1] * V; curl[2] = curl[2] + c[2] * V;
+      }
+      if (visc) {
+        REAL W = SFX(poly6)(SFX(dot)(off, off), F->h2, F->poly6);
+        for (int a = 0; a < 3; ++a) sm[a] = sm[a] + ((vj[a] - vi[a]) * W) * V;
+      }
+    }
+    if (vort) {
+      for (int a = 0; a < 3; ++a) F->omega[3 * (size_t)i + a] = curl[a];
+      F->omlen[i] = RSQRT(SFX(dot)(curl, curl));
+    }
+    if (visc) for (int a = 0; a < 3; ++a) F->dv[3 * (size_t)i + a] = sm[a] * F->visc;
+  }
+  if (vort) {
+    for (uint32_t i = F->start; i < end; ++i) {
+      const REAL* xi = s->x + 3 * (size_t)i;
+      REAL oi = F->omlen[i], eta[3] = {0, 0, 0};
+      for (uint32_t k = 0; k < F->nlc[i]; ++k) {
+        uint32_t j = F->nl[(size_t)i * 64 + k];
+        if (s->bvol[j] > R(0.0)) continue;
+        const REAL* xj = s->x + 3 * (size_t)j;
+        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
+        SFX(spiky)(off, F->h, F->spiky, g);
+        REAL oj = (j >= F->start && j < end) ? F->omlen[j] : R(0.0);
+        REAL sc = (oj - oi) * F->pv;
+        eta[0] = eta[0] + g[0] * sc; eta[1] = eta[1] + g[1] * sc; eta[2] = eta[2] + g[2] * sc;
+      }
+      for (int a = 0; a < 3; ++a) F->eta[3 * (size_t)i + a] = eta[a];
+    }
+  }
+  for (uint32_t i = F->start; i < end; ++i) {
+    REAL ch[3] = {0, 0, 0};
+    if (vort) {
+      REAL* eta = F->eta + 3 * (size_t)i;
+      REAL el = RSQRT(SFX(dot)(eta, eta));
+      if (el >= (REAL)1e-12) {
+        REAL nrm[3] = {eta[0] / el, eta[1] / el, eta[2] / el}, f[3];
+        SFX(cross)(nrm, F->omega + 3 * (size_t)i, f);
+        for (int a = 0; a < 3; ++a) ch[a] = ch[a] + (f[a] * F->vort) * s->dt;
+      }
+    }
+    if (visc) for (int a = 0; a < 3; ++a) ch[a] = ch[a] + F->dv[3 * (size_t)i + a];
+    for (int a = 0; a < 3; ++a) s->v[3 * (size_t)i + a] = s->v[3 * (size_t)i + a] + ch[a];
+  }
+}
+
+/* --------------------------------------------------------------------- */
+/* colliders (primitives.ts:85-242, collider.ts:53-118)                  */
+/* --------------------------------------------------------------------- */
+static void SFX(qrot)(const REAL q[4], const REAL v[3], REAL o[3]) {
+  REAL c[3], t[3], c2[3];
+  SFX(cross)(q, v, c);
+  t[0] = R(2.0) * c[0]; t[1] = R(2.0) * c[1]; t[2] = R(2.0) * c[2];
+  SFX(cross)(q, t, c2);
+  for (int a = 0; a < 3; ++a) o[a] = (v[a] + t[a] * q[3]) + c2[a];
+}
+
+static void SFX(pivot)(const SFX(prim)* P, REAL o[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (P->kind == 0) o[a] = P->d1[a];
+    else if (P->kind == 3) o[a] = (P->d0[a] + P->d1[a]) * R(0.5);
+    else o[a] = P->d0[a];
+  }
+}
+
+static void SFX(sdf)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL* phi_out, REAL grad[3]) {
+  REAL t = (P->flags & 2u) ? rewind : R(0.0);
+  REAL shifted[3], pv[3], x[3], turn[4], rel[3], rr[3];
+  for (int a = 0; a < 3; ++a) shifted[a] = xw[a] + P->vel[a] * t;
+  SFX(pivot)(P, pv);
+  {
+    REAL rate = RSQRT(SFX(dot)(P->spin, P->spin));
+    REAL half = (rate * t) * R(0.5);
+    REAL den = RFMAX(rate, (REAL)1e-8);
+    REAL sn = RSIN(half);
+    for (int a = 0; a < 3; ++a) turn[a] = (P->spin[a] / den) * sn;
+    turn[3] = RCOS(half);
+  }
+  for (int a = 0; a < 3; ++a) rel[a] = shifted[a] - pv[a];
+  SFX(qrot)(turn, rel, rr);
+  for (int a = 0; a < 3; ++a) x[a] = pv[a] + rr[a];
+  REAL phi = 0, g[3] = {0, 0, 0};
+  if (P->kind == 0) {
+    REAL d[3] = {x[0] - P->d1[0], x[1] - P->d1[1], x[2] - P->d1[2]};
+    phi = SFX(dot)(P->d0, d);
+    g[0] = P->d0[0]; g[1] = P->d0[1]; g[2] = P->d0[2];
+  } else if (P->kind == 1) {
+    REAL d[3] = {x[0] - P->d0[0], x[1] - P->d0[1], x[2] - P->d0[2]};
+    REAL len = RSQRT(SFX(dot)(d, d));
+    phi = len - P->d0[3];
+    REAL sl = RFMAX(len, (REAL)1e-8);
+    for (int a = 0; a < 3; ++a) g[a] = d[a] / sl;
+  } else if (P->kind == 2) {
+    REAL conj[4] = {-P->rot[0], -P->rot[1], -P->rot[2], P->rot[3]};
+    REAL off[3] = {x[0] - P->d0[0], x[1] - P->d0[1], x[2] - P->d0[2]}, xl[3], q[3], qp[3], sg[3], lg[3];
+    SFX(qrot)(conj, off, xl);
+    for (int a = 0; a < 3; ++a) {
+      q[a] = RFABS(xl[a]) - P->d1[a];
+      qp[a] = RFMAX(q[a], R(0.0));
+      sg[a] = xl[a] >= R(0.0) ? R(1.0) : R(-1.0);
+    }
+    REAL outside = RSQRT(SFX(dot)(qp, qp));
+    REAL inside = RFMIN(RFMAX(RFMAX(q[0], q[1]), q[2]), R(0.0));
+    phi = outside + inside;
+    int domx = q[0] >= q[1] && q[0] >= q[2];
+    int domy = q[1] >= q[0] && q[1] >= q[2];
+    if (outside > (REAL)1e-8) {
+      REAL ol = RFMAX(outside, (REAL)1e-8);
+      for (int a = 0; a < 3; ++a) lg[a] = (qp[a] * sg[a]) / ol;
+    } else {
+      lg[0] = domx ? sg[0] : R(0.0);
+      lg[1] = (!domx && domy) ? sg[1] : R(0.0);
+      lg[2] = (!domx && !domy) ? sg[2] : R(0.0);
+    }
+    SFX(qrot)(P->rot, lg, g);
+  } else {
+    REAL ab[3], ax[3], cl[3], d[3];
+    for (int a = 0; a < 3; ++a) { ab[a] = P->d1[a] - P->d0[a]; ax[a] = x[a] - P->d0[a]; }
+    REAL ll = RFMAX(SFX(dot)(ab, ab), (REAL)1e-8);
+    REAL tt = RFMIN(RFMAX(SFX(dot)(ax, ab) / ll, R(0.0)), R(1.0));
+    for (int a = 0; a < 3; ++a) { cl[a] = P->d0[a] + ab[a] * tt; d[a] = x[a] - cl[a]; }
+    REAL len = RSQRT(SFX(dot)(d, d));
+    phi = len - P->d0[3];
+    REAL sl = RFMAX(len, (REAL)1e-8);
+    for (int a = 0; a < 3; ++a) g[a] = d[a] / sl;
+  }
+  if (P->flags & 1u) {
+    phi = -phi;
+    g[0] = -g[0]; g[1] = -g[1]; g[2] = -g[2];
+  }
+  *phi_out = phi;
+  REAL un[4] = {-turn[0], -turn[1], -turn[2], turn[3]};
+  SFX(qrot)(un, g, grad);
+}
+
+static void SFX(surface_velocity)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL o[3]) {
+  REAL pv[3], arm[3], c[3];
+  SFX(pivot)(P, pv);
+  REAL t = (P->flags & 2u) ? rewind : R(0.0);
+  for (int a = 0; a < 3; ++a) arm[a] = (xw[a] - pv[a]) + P->vel[a] * t;
+  SFX(cross)(P->spin, arm, c);
+  for (int a = 0; a < 3; ++a) o[a] = P->vel[a] + c[a];
+}
+
+/* PrimitiveSet.ts:336-364 then the positions accumulator apply (loop.ts:222) */
+static void SFX(walls_solve)(SFX(scene)* s, REAL clock) {
+  uint32_t n = s->n;
+  for (uint32_t p = 0; p < n; ++p) {
+    REAL w = s->w[p];
+    if (!(w > R(0.0))) continue;
+    const REAL* xs = s->xs + 3 * (size_t)p;
+    REAL disp[3] = {xs[0] - s->x[3 * (size_t)p], xs[1] - s->x[3 * (size_t)p + 1], xs[2] - s->x[3 * (size_t)p + 2]};
+    for (uint32_t k = 0; k < s->nprim; ++k) {
+      REAL phi, nrm[3];
+      SFX(sdf)(&s->prim[k], xs, clock, &phi, nrm);
+      if (phi < s->radius) {
+        REAL cv[3], cd[3], slip[3], tang[3];
+        SFX(surface_velocity)(&s->prim[k], xs, clock, cv);
+        for (int a = 0; a < 3; ++a) cd[a] = cv[a] * s->dt;
+        REAL depth = s->radius - phi;
+        REAL* lam = s->plam + ((size_t)p * 32 + k) * 2;
+        REAL ln = lam[0] + depth / w;
+        lam[0] = ln;
+        for (int a = 0; a < 3; ++a) slip[a] = disp[a] - cd[a];
+        REAL sd = SFX(dot)(slip, nrm);
+        for (int a = 0; a < 3; ++a) tang[a] = slip[a] - nrm[a] * sd;
+        REAL tl = RSQRT(SFX(dot)(tang, tang));
+        REAL lt = lam[1];
+        int sticks = tl > (REAL)1e-10 && (tl / w) <= (s->prim[k].d1[3] * ln - lt);
+        for (int a = 0; a < 3; ++a) {
+          REAL push = nrm[a] * depth;
+          REAL corr = sticks ? push - tang[a] : push;
+          s->pacc[3 * (size_t)p + a] = SFX(wadd)(s->pacc[3 * (size_t)p + a], SFX(ticks)(corr, s->pos_scale));
+        }
+        lam[1] = lt + (sticks ? tl / w : R(0.0));
+      }
+    }
+  }
+  for (uint32_t p = 0; p < n; ++p) {
+    REAL c[3];
+    if (SFX(acc_change)(s->pacc + 3 * (size_t)p, s->pos_inv, c)) s->flag_pos = 1;
+    for (int a = 0; a < 3; ++a) {
+      s->xs[3 * (size_t)p + a] = s->xs[3 * (size_t)p + a] + c[a];
+      s->pacc[3 * (size_t)p + a] = 0;
+    }
+  }
+}
+
+/* PrimitiveSet.ts:366-391 then the velocities accumulator apply (loop.ts:226-230) */
+static void SFX(walls_friction)(SFX(scene)* s, REAL clock) {
+  uint32_t n = s->n;
+  for (uint32_t p = 0; p < n; ++p) {
+    REAL w = s->w[p];
+    if (!(w > R(0.0))) continue;
+    const REAL* xs = s->xs + 3 * (size_t)p;
+    const REAL* v = s->v + 3 * (size_t)p;
+    for (uint32_t k = 0; k < s->nprim; ++k) {
+      REAL ln = s->plam[((size_t)p * 32 + k) * 2];
+      if (!(ln > R(0.0))) continue;
+      REAL phi, g[3];
+      SFX(sdf)(&s->prim[k], xs, clock, &phi, g);
+      REAL gl = RSQRT(SFX(dot)(g, g));
+      if (!(gl > (REAL)1e-6)) continue;
+      REAL nrm[3] = {g[0] / gl, g[1] / gl, g[2] / gl}, cv[3], rel[3], vt[3];
+      SFX(surface_velocity)(&s->prim[k], xs, clock, cv);
+      for (int a = 0; a < 3; ++a) rel[a] = v[a] - cv[a];
+      REAL nd = SFX(dot)(nrm, rel);
+      for (int a = 0; a < 3; ++a) vt[a] = rel[a] - nrm[a] * nd;
+      REAL vl = RSQRT(SFX(dot)(vt, vt));
+      if (vl > (REAL)1e-6) {
+        REAL change = RFMIN(((s->prim[k].vel[3] * ln) * w) / s->dt, vl);
+        for (int a = 0; a < 3; ++a) {
+          REAL d = -((vt[a] / vl) * change);
+          s->vacc[3 * (size_t)p + a] = SFX(wadd)(s->vacc[3 * (size_t)p + a], SFX(ticks)(d, s->vel_scale));
+        }
+      }
+    }
+  }
+  for (uint32_t p = 0; p < n; ++p) {
+    REAL c[3];
+    if (SFX(acc_change)(s->vacc + 3 * (size_t)p, s->vel_inv, c)) s->flag_vel = 1;
+    for (int a = 0; a < 3; ++a) {
+      s->v[3 * (size_t)p + a] = s->v[3 * (size_t)p + a] + c[a];
+      s->vacc[3 * (size_t)p + a] = 0;
+    }
+  }
+}
+
+/* --------------------------------------------------------------------- */
+/* global shape matching (shapeMatch.ts:53-351, polarDecomp.ts:34-264)   */
+/* --------------------------------------------------------------------- */
+static void SFX(polar)(const REAL a[9], REAL r[9]) {
+#define A(i, j) a[3 * (i) + (j)]
+  REAL m00 = (A(0, 0) * A(0, 0) + A(1, 0) * A(1, 0)) + A(2, 0) * A(2, 0);
+  REAL m01 = (A(0, 0) * A(0, 1) + A(1, 0) * A(1, 1)) + A(2, 0) * A(2, 1);
+  REAL m02 = (A(0, 0) * A(0, 2) + A(1, 0) * A(1, 2)) + A(2, 0) * A(2, 2);
+  REAL m11 = (A(0, 1) * A(0, 1) + A(1, 1) * A(1, 1)) + A(2, 1) * A(2, 1);
+  REAL m12 = (A(0, 1) * A(0, 2) + A(1, 1) * A(1, 2)) + A(2, 1) * A(2, 2);
+  REAL m22 = (A(0, 2) * A(0, 2) + A(1, 2) * A(1, 2)) + A(2, 2) * A(2, 2);
+  REAL q[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  REAL eps = (REAL)1e-12;
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    for (int plane = 0; plane < 3; ++plane) {
+      int P = plane == 2 ? 1 : 0, Qi = plane == 0 ? 1 : 2;
+      REAL* mpq = plane == 0 ? &m01 : (plane == 1 ? &m02 : &m12);
+      REAL* mpp = P == 0 ? &m00 : &m11;
+      REAL* mqq = Qi == 1 ? &m11 : &m22;
+      if (!(RFABS(*mpq) > eps)) continue;
+      REAL apq = *mpq, app = *mpp, aqq = *mqq;
+      REAL tau = (aqq - app) / (apq * R(2.0));
+      REAL sg = tau >= R(0.0) ? R(1.0) : R(-1.0);
+      REAL ta = RFABS(tau);
+      REAL t = sg / (ta + RSQRT(ta * ta + R(1.0)));
+      REAL cc = R(1.0) / RSQRT(t * t + R(1.0));
+      REAL ss = t * cc;
+      /* the two other off-diagonals of the rotated plane */
+      REAL *u, *v;
+      if (plane == 0) { u = &m02; v = &m12; }
+      else if (plane == 1) { u = &m01; v = &m12; }
+      else { u = &m01; v = &m02; }
+      REAL ou = *u, ov = *v;
+      *mpp = app - t * apq;
+      *mqq = aqq + t * apq;
+      *mpq = R(0.0);
+      *u = cc * ou - ss * ov;
+      *v = ss * ou + cc * ov;
+      for (int i = 0; i < 3; ++i) {
+        REAL qp = q[i][P], qq = q[i][Qi];
+        q[i][P] = cc * qp - ss * qq;
+        q[i][Qi] = ss * qp + cc * qq;
+      }
+    }
+  }
+  REAL il[3] = {R(1.0) / RSQRT(RFMAX(m00, (REAL)1e-20)), R(1.0) / RSQRT(RFMAX(m11, (REAL)1e-20)),
+                R(1.0) / RSQRT(RFMAX(m22, (REAL)1e-20))};
+  REAL si[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = i; j < 3; ++j) {
+      REAL v = ((q[i][0] * q[j][0]) * il[0] + (q[i][1] * q[j][1]) * il[1]) + (q[i][2] * q[j][2]) * il[2];
+      si[i][j] = v;
+      si[j][i] = v;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[3 * i + j] = (A(i, 0) * si[0][j] + A(i, 1) * si[1][j]) + A(i, 2) * si[2][j];
+#undef A
+}
+
+/* Tree fold of 256 per-thread partials, pairing i with i + s (shapeMatch.ts:88-101). */
+static void SFX(tree)(REAL (*v)[13]) {
+  for (int s = 128; s > 0; s >>= 1)
+    for (int t = 0; t < s; ++t)
+      for (int k = 0; k < 13; ++k) v[t][k] = v[t][k] + v[t + s][k];
+}
+
+static void SFX(shape_match)(SFX(scene)* s, int first) {
+  static REAL part[256][13];
+  for (uint32_t b = 0; b < s->nbodies; ++b) {
+    uint32_t st = s->bstart[b], cnt = s->bcount[b];
+    for (int t = 0; t < 256; ++t) {
+      for (int k = 0; k < 13; ++k) part[t][k] = 0;
+      for (uint32_t i = (uint32_t)t; i < cnt; i += 256) {
+        uint32_t id = st + i;
+        const REAL* x = s->xs + 3 * (size_t)id;
+        REAL m = s->weights[id];
+        const REAL* r = s->rest + 3 * (size_t)id;
+        part[t][0] = part[t][0] + x[0] * m;
+        part[t][1] = part[t][1] + x[1] * m;
+        part[t][2] = part[t][2] + x[2] * m;
+        part[t][3] = part[t][3] + m;
+        REAL xm[3] = {x[0] * m, x[1] * m, x[2] * m};
+        for (int row = 0; row < 3; ++row)
+          for (int col = 0; col < 3; ++col) part[t][4 + 3 * row + col] = part[t][4 + 3 * row + col] + r[col] * xm[row];
+      }
+    }
+    SFX(tree)(part);
+    REAL c[3] = {part[0][0] / part[0][3], part[0][1] / part[0][3], part[0][2] / part[0][3]};
+    REAL A[9], Rm[9];
+    for (int k = 0; k < 9; ++k) A[k] = part[0][4 + k];
+    SFX(polar)(A, Rm);
+    for (int a = 0; a < 3; ++a) s->centers[b][a] = c[a];
+    for (int k = 0; k < 9; ++k) s->rot[b][k] = Rm[k];
+    REAL at = s->bcomp[b] / (s->dt * s->dt);
+    for (uint32_t i = 0; i < cnt; ++i) {
+      uint32_t id = st + i;
+      const REAL* r = s->rest + 3 * (size_t)id;
+      REAL* x = s->xs + 3 * (size_t)id;
+      REAL* lam = s->slam + 3 * (size_t)id;
+      REAL w = s->w[id];
+      REAL den = w + at;
+      for (int a = 0; a < 3; ++a) {
+        REAL goal = ((Rm[3 * a] * r[0] + Rm[3 * a + 1] * r[1]) + Rm[3 * a + 2] * r[2]) + c[a];
+        REAL C = x[a] - goal;
+        REAL lo = first ? R(0.0) : lam[a];
+        REAL dl = ((-C) - lo * at) / den;
+        lam[a] = lo + dl;
+        x[a] = x[a] + dl * w;
+      }
+    }
+  }
+}
+
+/* --------------------------------------------------------------------- */
+/* one step (loop.ts:175-279)                                            */
+/* --------------------------------------------------------------------- */
+static void SFX(step)(SFX(scene)* s, double dt_step) {
+  uint32_t n = s->n;
+  s->dt = (REAL)(dt_step / (double)s->substeps);
+  {
+    REAL clock = s->dt * ((REAL)s->substeps - R(1.0));
+    for (uint32_t k = 0; k < s->substeps; ++k) {
+      s->clock[k] = clock;
+      clock = RFMAX(clock - s->dt, R(0.0));
+    }
+  }
+  s->flag_pos = 0;
+  s->flag_vel = 0;
+  for (uint32_t k = 0; k < s->substeps; ++k) {
+    /* predict (integrate.ts:25-34) */
+    for (uint32_t i = 0; i < n; ++i) {
+      if (!(s->w[i] > R(0.0))) continue;
+      for (int a = 0; a < 3; ++a) {
+        REAL vel = s->v[3 * (size_t)i + a] + s->g[a] * s->dt;
+        s->v[3 * (size_t)i + a] = vel;
+        s->xs[3 * (size_t)i + a] = s->x[3 * (size_t)i + a] + vel * s->dt;
+      }
+    }
+    s->flag_nl = 0;
+    if (s->nfluids) SFX(grid_build)(s);
+    memset(s->plam, 0, sizeof(REAL) * 64 * (size_t)n);
+    for (uint32_t f = 0; f < s->nfluids; ++f) SFX(neighbors)(s, &s->fluid[f]);
+    for (uint32_t f = 0; f < s->nfluids; ++f) {
+      SFX(fluid)* F = &s->fluid[f];
+      if (F->effects & 4u) SFX(color_normal)(s, F);
+      if (F->effects & (4u | 8u)) SFX(impulses)(s, F);
+    }
+    for (uint32_t it = 0; it < s->iterations; ++it) {
+      for (uint32_t f = 0; f < s->nfluids; ++f) {
+        SFX(lambda_pass)(s, &s->fluid[f]);
+        SFX(delta_pass)(s, &s->fluid[f]);
+      }
+      if (s->nbodies) SFX(shape_match)(s, it == 0);
+      if (s->nprim) SFX(walls_solve)(s, s->clock[k]);
+    }
+    /* advect (integrate.ts:36-44) */
+    for (uint32_t i = 0; i < n; ++i) {
+      if (!(s->w[i] > R(0.0))) continue;
+      for (int a = 0; a < 3; ++a) {
+        s->v[3 * (size_t)i + a] = (s->xs[3 * (size_t)i + a] - s->x[3 * (size_t)i + a]) / s->dt;
+        s->x[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a];
+      }
+    }
+    for (uint32_t f = 0; f < s->nfluids; ++f) SFX(velocity_passes)(s, &s->fluid[f]);
+    if (s->nprim && s->iterations > 0) SFX(walls_friction)(s, s->clock[k]);
+  }
+}

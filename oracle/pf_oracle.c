@@ -1,0 +1,366 @@
+/*
+ * pf_oracle.c — CPU oracle for the MI355X particle solver.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load this library, and only as the checker
+ * or the timed CPU baseline, never as a product path.
+ *
+ * It restates the reference's per-substep solver on the CPU:
+ *   - Morton bucket key and stable counting sort
+ *     (src/core/hashGrid/mortonHash.ts:39-79, cellIndex.ts:113-170, sort.ts:55-195);
+ *   - the 27-bucket neighbour walk (query.ts:24-70) and the 64-neighbour list
+ *     (neighborList.ts:45-72);
+ *   - the full SimLoop substep for fluids, primitive colliders and global
+ *     shape matching (orc_sim.h), in FP32 (operand order of the GPU kernels)
+ *     and in FP64.
+ * The reference itself is TypeScript/WGSL that cannot run here (no tsc,
+ * three.js or WebGPU), so nothing under /root/reference is compiled.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+/* integer part (mortonHash.ts, cellIndex.ts)                              */
+/* ---------------------------------------------------------------------- */
+uint32_t orc_part1by2(uint32_t n) {
+  n &= 0x000003ffu;
+  n = (n | (n << 16)) & 0xff0000ffu;
+  n = (n | (n << 8)) & 0x0300f00fu;
+  n = (n | (n << 4)) & 0x030c30c3u;
+  n = (n | (n << 2)) & 0x09249249u;
+  return n;
+}
+
+uint32_t orc_morton(int32_t cx, int32_t cy, int32_t cz) {
+  uint32_t ux = (uint32_t)cx + 512u, uy = (uint32_t)cy + 512u, uz = (uint32_t)cz + 512u;
+  return orc_part1by2(ux) | (orc_part1by2(uy) << 1) | (orc_part1by2(uz) << 2);
+}
+
+/* WGSL f32 -> i32 conversion saturates; NaN is taken as 0. */
+int32_t orc_f2i_sat(double f) {
+  if (!(f == f)) return 0;
+  if (f >= 2147483648.0) return 2147483647;
+  if (f < -2147483648.0) return (int32_t)(-2147483647 - 1);
+  return (int32_t)f;
+}
+
+/* Bucket of one FP32 position: (p - o) / cell in FP32, floor, saturate,
+ * clamp to +-2^30, Morton, mask (cellIndex.ts:113-170). */
+uint32_t orc_bucket(const float p[3], const float o[3], float cell, uint32_t H, int* over) {
+  int32_t c[3];
+  for (int a = 0; a < 3; ++a) {
+    float rel = (p[a] - o[a]) / cell;
+    int32_t raw = orc_f2i_sat((double)floorf(rel));
+    int32_t cc = raw < -(1 << 30) ? -(1 << 30) : (raw > (1 << 30) - 1 ? (1 << 30) - 1 : raw);
+    if (raw != cc || cc < -512 || cc >= 512) *over = 1;
+    c[a] = cc;
+  }
+  return orc_morton(c[0], c[1], c[2]) & (H - 1);
+}
+
+/* HashGrid rebuild (HashGrid.ts:195-204) as a stable counting sort.
+ * counts/cell_start/cell_end hold Hpad = ceil(H/1024)*1024 entries. */
+int orc_grid_build(const float* pos4, uint32_t n, float cell, const float origin[3], uint32_t H,
+                   uint32_t* cell_index, uint32_t* counts, uint32_t* cell_start, uint32_t* cell_end,
+                   uint32_t* sorted_indices, uint32_t* sorted_keys) {
+  int over = 0;
+  uint32_t Hpad = (uint32_t)(((uint64_t)H + 1023) / 1024 * 1024);
+  uint32_t* cnt = (uint32_t*)calloc(Hpad, sizeof(uint32_t));
+  uint32_t* key = cell_index ? cell_index : (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  for (uint32_t i = 0; i < n; ++i) {
+    key[i] = orc_bucket(pos4 + 4 * (size_t)i, origin, cell, H, &over);
+    cnt[key[i]]++;
+  }
+  uint32_t* st = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)Hpad + 1));
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < Hpad; ++b) {
+    st[b] = run;
+    run += cnt[b];
+  }
+  st[Hpad] = run;
+  if (counts) memcpy(counts, cnt, sizeof(uint32_t) * Hpad);
+  if (cell_start) memcpy(cell_start, st, sizeof(uint32_t) * Hpad);
+  if (cell_end)
+    for (uint32_t b = 0; b < Hpad; ++b) cell_end[b] = st[b] + cnt[b];
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t slot = st[key[i]]++;
+    if (sorted_indices) sorted_indices[slot] = i;
+    if (sorted_keys) sorted_keys[slot] = key[i];
+  }
+  free(st);
+  free(cnt);
+  if (!cell_index) free(key);
+  return over;
+}
+
+/* emitForEachNeighbor (query.ts:24-70): for each query point, candidates
+ * visited in the 27 buckets (own index excluded) and those within r. */
+void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const float origin[3], uint32_t H,
+                           const float* query4, uint32_t nq, float radius, uint32_t* cand,
+                           uint32_t* within) {
+  uint32_t Hpad = (uint32_t)(((uint64_t)H + 1023) / 1024 * 1024);
+  uint32_t* cs = (uint32_t*)malloc(sizeof(uint32_t) * Hpad);
+  uint32_t* ce = (uint32_t*)malloc(sizeof(uint32_t) * Hpad);
+  uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  orc_grid_build(pos4, n, cell, origin, H, NULL, NULL, cs, ce, perm, NULL);
+  float r2 = radius * radius;
+  for (uint32_t i = 0; i < nq; ++i) {
+    const float* x = query4 + 4 * (size_t)i;
+    int32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = orc_f2i_sat((double)floorf((x[a] - origin[a]) / cell));
+    uint32_t visited[27], nv = 0, nc = 0, nw = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          uint32_t b = orc_morton((int32_t)((uint32_t)q[0] + (uint32_t)dx), (int32_t)((uint32_t)q[1] + (uint32_t)dy),
+                                  (int32_t)((uint32_t)q[2] + (uint32_t)dz)) & (H - 1);
+          int seen = 0;
+          for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
+          visited[nv++] = b;
+          if (seen) continue;
+          for (uint32_t t = cs[b]; t < ce[b]; ++t) {
+            uint32_t j = perm[t];
+            if (j == i) continue;
+            ++nc;
+            const float* y = pos4 + 4 * (size_t)j;
+            float ddx = x[0] - y[0], ddy = x[1] - y[1], ddz = x[2] - y[2];
+            if ((ddx * ddx + ddy * ddy) + ddz * ddz < r2) ++nw;
+          }
+        }
+    if (cand) cand[i] = nc;
+    if (within) within[i] = nw;
+  }
+  free(cs);
+  free(ce);
+  free(perm);
+}
+
+/* ---------------------------------------------------------------------- */
+/* FP32 mirror and FP64 restatement of the substep                        */
+/* ---------------------------------------------------------------------- */
+#define REAL float
+#define SFX(name) name##_f32
+#define R(x) ((float)(x))
+#define RSQRT sqrtf
+#define RFLOOR floorf
+#define RFMAX fmaxf
+#define RFMIN fminf
+#define RFABS fabsf
+#define RSIN sinf
+#define RCOS cosf
+#define RRINT rintf
+#include "orc_sim.h"
+#undef REAL
+#undef SFX
+#undef R
+#undef RSQRT
+#undef RFLOOR
+#undef RFMAX
+#undef RFMIN
+#undef RFABS
+#undef RSIN
+#undef RCOS
+#undef RRINT
+
+#define REAL double
+#define SFX(name) name##_f64
+#define R(x) ((double)(x))
+#define RSQRT sqrt
+#define RFLOOR floor
+#define RFMAX fmax
+#define RFMIN fmin
+#define RFABS fabs
+#define RSIN sin
+#define RCOS cos
+#define RRINT rint
+#include "orc_sim.h"
+#undef REAL
+#undef SFX
+#undef R
+
+/* ---------------------------------------------------------------------- */
+/* exported scene API (same entry points for both precisions)              */
+/* ---------------------------------------------------------------------- */
+#define ORC_SCENE_API(SUF, T)                                                                      \
+  void* orc_scene_create_##SUF(uint32_t n, double radius) {                                        \
+    scene_##SUF* s = (scene_##SUF*)calloc(1, sizeof(scene_##SUF));                                 \
+    s->n = n;                                                                                      \
+    s->radius = (T)radius;                                                                         \
+    s->x = (T*)calloc(3 * (size_t)n, sizeof(T));                                                   \
+    s->xs = (T*)calloc(3 * (size_t)n, sizeof(T));                                                  \
+    s->v = (T*)calloc(3 * (size_t)n, sizeof(T));                                                   \
+    s->w = (T*)calloc(n, sizeof(T));                                                               \
+    s->rho = (T*)calloc(n, sizeof(T));                                                             \
+    s->bvol = (T*)calloc(n, sizeof(T));                                                            \
+    s->key = (uint32_t*)calloc(n, sizeof(uint32_t));                                               \
+    s->perm = (uint32_t*)calloc(n, sizeof(uint32_t));                                              \
+    s->plam = (T*)calloc(64 * (size_t)n, sizeof(T));                                               \
+    s->pacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
+    s->vacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
+    s->iacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
+    s->rest = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
+    s->weights = (T*)calloc(n, sizeof(T));                                                         \
+    s->slam = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
+    s->substeps = 4;                                                                               \
+    s->iterations = 2;                                                                             \
+    s->g[1] = (T)-9.81;                                                                            \
+    s->pos_scale = (T)floor(1073741824.0 / 10.0);                                                  \
+    s->pos_inv = (T)(1.0 / floor(1073741824.0 / 10.0));                                            \
+    s->vel_scale = (T)floor(1073741824.0 / 50.0);                                                  \
+    s->vel_inv = (T)(1.0 / floor(1073741824.0 / 50.0));                                            \
+    return s;                                                                                      \
+  }                                                                                                \
+  void orc_scene_destroy_##SUF(void* p) {                                                          \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (!s) return;                                                                                \
+    free(s->x); free(s->xs); free(s->v); free(s->w); free(s->rho); free(s->bvol);                  \
+    free(s->key); free(s->perm); free(s->start); free(s->plam); free(s->pacc); free(s->vacc);      \
+    free(s->iacc); free(s->rest); free(s->weights); free(s->slam);                                 \
+    for (uint32_t f = 0; f < s->nfluids; ++f) {                                                    \
+      fluid_##SUF* F = &s->fluid[f];                                                               \
+      free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
+      free(F->omega); free(F->omlen); free(F->eta); free(F->dv);                                   \
+    }                                                                                              \
+    free(s);                                                                                       \
+  }                                                                                                \
+  /* positions/velocities: xyz triples (float); inv_mass per particle */                          \
+  void orc_scene_set_particles_##SUF(void* p, const float* x3, const float* v3, const float* w) {   \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    for (size_t k = 0; k < 3 * (size_t)s->n; ++k) {                                                \
+      s->x[k] = (T)x3[k];                                                                          \
+      s->xs[k] = (T)x3[k];                                                                         \
+      s->v[k] = v3 ? (T)v3[k] : (T)0;                                                              \
+    }                                                                                              \
+    for (uint32_t i = 0; i < s->n; ++i) s->w[i] = w ? (T)w[i] : (T)1;                              \
+  }                                                                                                \
+  void orc_scene_set_loop_##SUF(void* p, uint32_t substeps, uint32_t iterations, const double g[3], \
+                                double cell, uint32_t H, const double origin[3]) {                 \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    s->substeps = substeps;                                                                        \
+    s->iterations = iterations;                                                                    \
+    for (int a = 0; a < 3; ++a) {                                                                  \
+      s->g[a] = (T)g[a];                                                                           \
+      s->origin[a] = origin ? (T)origin[a] : (T)0;                                                 \
+    }                                                                                              \
+    s->cell = (T)cell;                                                                             \
+    s->H = H;                                                                                      \
+    free(s->start);                                                                                \
+    s->start = (uint32_t*)calloc((size_t)H + 1, sizeof(uint32_t));                                 \
+  }                                                                                                \
+  /* FluidSystem.ts:90-127 + build 200-309; sets inv mass and density of the range */             \
+  int orc_scene_add_fluid_##SUF(void* p, uint32_t start, uint32_t count, double rho0,              \
+                                double spacing, double h, double compliance, uint32_t effects,     \
+                                double visc, double vort, double st, double adh) {                 \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (s->nfluids >= 4) return -1;                                                                \
+    fluid_##SUF* F = &s->fluid[s->nfluids++];                                                      \
+    double mass = rho0 * spacing * spacing * spacing;                                              \
+    F->start = start; F->count = count; F->effects = effects;                                      \
+    F->rho0 = (T)rho0; F->pv = (T)(spacing * spacing * spacing); F->mass = (T)mass;                \
+    F->h = (T)h; F->h2 = (T)(h * h);                                                               \
+    F->poly6 = (T)(315.0 / (64.0 * M_PI * pow(h, 9)));                                              \
+    F->spiky = (T)(45.0 / (M_PI * pow(h, 6)));                                                      \
+    F->compliance = (T)compliance;                                                                 \
+    F->gamma = (T)st; F->visc = (T)visc; F->vort = (T)vort; F->adhesion = (T)adh;                  \
+    F->coh_coef = (T)(32.0 / (M_PI * pow(h, 9)));                                                   \
+    F->coh_h6_64 = (T)(pow(h, 6) / 64.0);                                                           \
+    F->adh_coef = (T)(0.007 / pow(h, 3.25));                                                        \
+    F->adh_m4_over_h = (T)(-4.0 / h);                                                               \
+    F->adh_2h = (T)(2.0 * h);                                                                       \
+    F->imp_scale = (T)floor(1073741824.0 / 50.0);                                                  \
+    F->imp_inv = (T)(1.0 / floor(1073741824.0 / 50.0));                                           \
+    size_t n = s->n;                                                                               \
+    F->nl = (uint32_t*)calloc(64 * n, sizeof(uint32_t));                                           \
+    F->nlc = (uint32_t*)calloc(n, sizeof(uint32_t));                                               \
+    F->normal = (T*)calloc(3 * n, sizeof(T));                                                      \
+    F->lambda = (T*)calloc(n, sizeof(T));                                                          \
+    F->dx = (T*)calloc(3 * n, sizeof(T));                                                          \
+    F->omega = (T*)calloc(3 * n, sizeof(T));                                                       \
+    F->omlen = (T*)calloc(n, sizeof(T));                                                           \
+    F->eta = (T*)calloc(3 * n, sizeof(T));                                                         \
+    F->dv = (T*)calloc(3 * n, sizeof(T));                                                          \
+    /* invMass = 1 / mass stored as f32 (FluidSystem.ts:118) */                                   \
+    float im = (float)(1.0 / mass);                                                                \
+    for (uint32_t i = start; i < start + count; ++i) {                                             \
+      s->w[i] = (T)im;                                                                             \
+      s->rho[i] = (T)(float)rho0;                                                                  \
+    }                                                                                              \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* PrimitiveSet add* after the host-side double setup; data as uploaded (f32 values) */         \
+  int orc_scene_add_prim_##SUF(void* p, uint32_t kind, uint32_t flags, const double d0[4],         \
+                               const double d1[4], const double vel[4], const double rot[4],       \
+                               const double spin[3]) {                                             \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (s->nprim >= 32) return -1;                                                                 \
+    prim_##SUF* P = &s->prim[s->nprim++];                                                          \
+    P->kind = kind; P->flags = flags;                                                              \
+    for (int a = 0; a < 4; ++a) {                                                                  \
+      P->d0[a] = (T)(float)d0[a]; P->d1[a] = (T)(float)d1[a];                                      \
+      P->vel[a] = (T)(float)vel[a]; P->rot[a] = (T)(float)rot[a];                                  \
+    }                                                                                              \
+    for (int a = 0; a < 3; ++a) P->spin[a] = (T)(float)spin[a];                                    \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* one global shape-matching body; rest offsets and weights as SoftbodySystem computes them */  \
+  int orc_scene_add_body_##SUF(void* p, uint32_t start, uint32_t count, double compliance,        \
+                               const float* rest3, const float* weights) {                         \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (s->nbodies >= 4096) return -1;                                                             \
+    uint32_t b = s->nbodies++;                                                                     \
+    s->bstart[b] = start; s->bcount[b] = count; s->bcomp[b] = (T)(float)compliance;                \
+    for (uint32_t i = 0; i < count; ++i) {                                                         \
+      for (int a = 0; a < 3; ++a) s->rest[3 * (size_t)(start + i) + a] = (T)rest3[3 * (size_t)i + a]; \
+      s->weights[start + i] = (T)weights[i];                                                       \
+    }                                                                                              \
+    return 0;                                                                                      \
+  }                                                                                                \
+  void orc_scene_step_##SUF(void* p, double dt) { step_##SUF((scene_##SUF*)p, dt); }               \
+  /* state out as double: x, x*, v as xyz triples, density */                                     \
+  void orc_scene_get_##SUF(void* p, double* x3, double* xs3, double* v3, double* rho) {           \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    for (size_t k = 0; k < 3 * (size_t)s->n; ++k) {                                                \
+      if (x3) x3[k] = (double)s->x[k];                                                             \
+      if (xs3) xs3[k] = (double)s->xs[k];                                                          \
+      if (v3) v3[k] = (double)s->v[k];                                                             \
+    }                                                                                              \
+    if (rho) for (uint32_t i = 0; i < s->n; ++i) rho[i] = (double)s->rho[i];                       \
+  }                                                                                                \
+  void orc_scene_flags_##SUF(void* p, int* out5) {                                                 \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    out5[0] = s->flag_grid; out5[1] = s->flag_nl; out5[2] = s->flag_pos;                           \
+    out5[3] = s->flag_vel; out5[4] = s->flag_imp;                                                  \
+  }                                                                                                \
+  void orc_scene_body_frame_##SUF(void* p, uint32_t b, double c[3], double r9[9]) {               \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    for (int a = 0; a < 3; ++a) c[a] = (double)s->centers[b][a];                                   \
+    for (int k = 0; k < 9; ++k) r9[k] = (double)s->rot[b][k];                                      \
+  }                                                                                                \
+  void orc_polar_##SUF(const double a9[9], double r9[9]) {                                         \
+    T a[9], r[9];                                                                                  \
+    for (int k = 0; k < 9; ++k) a[k] = (T)a9[k];                                                   \
+    polar_##SUF(a, r);                                                                             \
+    for (int k = 0; k < 9; ++k) r9[k] = (double)r[k];                                              \
+  }                                                                                                \
+  /* neighbour lists of the last substep, in walk order (row-major, 64 per particle) */           \
+  void orc_scene_neighbors_##SUF(void* p, uint32_t f, uint32_t* nl, uint32_t* cnt) {              \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    fluid_##SUF* F = &s->fluid[f];                                                                 \
+    memcpy(nl, F->nl, sizeof(uint32_t) * 64 * (size_t)s->n);                                       \
+    memcpy(cnt, F->nlc, sizeof(uint32_t) * (size_t)s->n);                                          \
+  }
+
+ORC_SCENE_API(f32, float)
+ORC_SCENE_API(f64, double)
+
+/* SPH closed forms for the known-answer tests (kernels.ts:64-107). */
+double orc_poly6_f32(double r2, double h) {
+  return (double)poly6_f32((float)r2, (float)(h * h), (float)(315.0 / (64.0 * M_PI * pow(h, 9))));
+}
+void orc_spiky_f32(const double r[3], double h, double out[3]) {
+  float rr[3] = {(float)r[0], (float)r[1], (float)r[2]}, g[3];
+  spiky_f32(rr, (float)h, (float)(45.0 / (M_PI * pow(h, 6))), g);
+  for (int a = 0; a < 3; ++a) out[a] = g[a];
+}

@@ -1,0 +1,112 @@
+"""GPU hash-grid counting sort vs the oracle, bit-exact (keys, counts, offsets, sorted keys, permutation)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import particle_fluids as pf
+from particle_fluids import scenes
+
+pytestmark = pytest.mark.gpu
+
+N_SORT = 4_194_304
+CELL = 0.01
+H_SORT = 1 << 22
+
+
+def gpu_sort(pos, cell, H, sorted_positions=True):
+    g = pf.HashGrid.standalone(len(pos), cell, H)
+    g.setSortedPositions(sorted_positions)
+    g.uploadPositions(pos)
+    g.rebuild()
+    snap = g.readback()
+    keys = g.readbackSortedKeys()
+    spos = g.readbackSortedPositions() if sorted_positions else None
+    over = g.readbackOverflow()
+    g.dispose()
+    return snap, keys, spos, over
+
+
+@pytest.mark.parametrize("dist", ["uniform", "blobs"])
+def test_sort_4m_bit_exact(dist):
+    print(f"[grid] sort {dist} N={N_SORT} H={H_SORT}", flush=True)
+    pos = scenes.sort_points(N_SORT, dist)
+    ref = orc.grid_build(pos, CELL, H_SORT)
+    snap, keys, spos, over = gpu_sort(pos, CELL, H_SORT)
+    assert np.array_equal(snap.cellIndex, ref["cellIndex"])
+    assert np.array_equal(snap.counts, ref["counts"])
+    assert np.array_equal(snap.cellStart, ref["cellStart"])
+    assert np.array_equal(snap.cellEnd, ref["cellEnd"])
+    assert np.array_equal(keys, ref["sortedKeys"])
+    assert np.array_equal(snap.sortedIndices, ref["sortedIndices"])
+    assert np.array_equal(spos, pos[ref["sortedIndices"]])
+    assert over == ref["overflow"] is False
+
+
+def test_sort_small_tables_and_ragged_sizes():
+    # capacity not a multiple of the 256-thread tile, tables below one scan tile
+    for n, H in [(1, 1), (5, 16), (257, 64), (1000, 1024), (4099, 4096), (70001, 1 << 17)]:
+        pos = scenes.sort_points(max(n, 64), "uniform", seed=n)[:n]
+        ref = orc.grid_build(pos, 0.05, H)
+        snap, keys, _, _ = gpu_sort(pos, 0.05, H)
+        assert np.array_equal(snap.cellIndex, ref["cellIndex"]), (n, H)
+        assert np.array_equal(snap.counts, ref["counts"]), (n, H)
+        assert np.array_equal(snap.cellStart, ref["cellStart"]), (n, H)
+        assert np.array_equal(snap.sortedIndices, ref["sortedIndices"]), (n, H)
+
+
+def test_sort_dense_bucket_and_repeated_rebuilds():
+    # every point in one cell: a 3000-long bucket, then 50 rebuilds identical (determinism tier 1)
+    pos = np.zeros((3000, 4), np.float32)
+    pos[:, :3] = 0.005 + scenes.lcg_uniform(9, 9000).reshape(3000, 3).astype(np.float32) * 0.004
+    ref = orc.grid_build(pos, 0.01, 4096)
+    g = pf.HashGrid.standalone(3000, 0.01, 4096)
+    g.uploadPositions(pos)
+    first = None
+    for _ in range(50):
+        g.rebuild()
+        s = g.readback()
+        if first is None:
+            first = s
+            assert np.array_equal(s.cellStart, ref["cellStart"])
+        assert np.array_equal(s.cellStart, first.cellStart)
+        assert np.array_equal(s.counts, first.counts)
+        assert np.array_equal(np.sort(s.sortedIndices), np.arange(3000))
+    g.dispose()
+
+
+def test_collision_rate_curve_on_gpu():
+    # hashgrid-collision-rate.gpu.test.ts:31-38 measured curve, through the GPU grid.
+    n, cell = 16, 0.05
+    g3 = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    pos = np.zeros((n ** 3, 4), np.float32)
+    pos[:, :3] = (g3.reshape(-1, 3)[:, ::-1] * cell + cell / 2).astype(np.float32)
+    true = 0
+    for d in [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1) if (a, b, c) != (0, 0, 0)]:
+        ok = np.ones(n ** 3, bool)
+        idx = g3.reshape(-1, 3)
+        for ax in range(3):
+            ok &= (idx[:, ax] + d[ax] >= 0) & (idx[:, ax] + d[ax] < n)
+        true += int(ok.sum())
+    for H, rate in {1024: 370.0, 2048: 132.8, 4096: 14.2, 8192: 9.3, 16384: 4.5, 32768: 0.0}.items():
+        g = pf.HashGrid.standalone(len(pos), cell, H)
+        g.uploadPositions(pos)
+        g.rebuild()
+        cand, _ = g.queryCounts(pos, cell)
+        g.dispose()
+        measured = (int(cand.sum()) - true) / true * 100
+        assert round(measured, 1) == pytest.approx(rate, abs=0.051), (H, measured)
+
+
+def test_overflow_flag_and_particle_grid():
+    ps = pf.ParticleSystem(2, 0.02)
+    ps.uploadParticles(np.array([[(512 + 0.5) * 0.1, 0, 0], [0.05, 0.05, 0.05]], np.float32))
+    g = pf.HashGrid(ps, 0.1)
+    g.rebuild()
+    assert g.readbackOverflow()
+    ps.uploadParticles(np.array([[(511 + 0.5) * 0.1, 0, 0]], np.float32))
+    g.rebuild()
+    assert not g.readbackOverflow()
+    snap = g.readback()
+    assert snap.cellIndex[0] == orc.morton(511, 0, 0) & (g.hashTableSize - 1)
+    g.dispose()
+    ps.dispose()

@@ -1,0 +1,151 @@
+"""GPU PBF substep vs the CPU oracle (FP32 mirror and FP64), on the dam break."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import particle_fluids as pf
+from helpers import oracle_dam_break, rel_err
+from particle_fluids import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative (inf-norm) tolerance of one FP32 substep
+
+
+def gpu_state(sc):
+    snap = sc["ps"].readback()
+    rho = sc["fluid"].readbackDensity()
+    return dict(x=snap.positions[:, :3], xs=snap.predictedPositions[:, :3], v=snap.velocities[:, :3], rho=rho)
+
+
+def compare(g, o):
+    return {k: rel_err(g[k], o[k]) for k in ("x", "xs", "v", "rho")}
+
+
+@pytest.mark.parametrize("substeps,steps", [(1, 1), (2, 1), (2, 3)])
+def test_substeps_match_fp32_oracle(substeps, steps):
+    print(f"[pbf] C1 substeps={substeps} steps={steps}", flush=True)
+    sc = scenes.build_dam_break(5000, substeps=substeps, iterations=4)
+    pos, r = sc["positions"], sc["radius"]
+    o = oracle_dam_break(pos, r, substeps, 4, "f32")
+    for _ in range(steps):
+        sc["loop"].step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+    err = compare(gpu_state(sc), o.get())
+    print("[pbf] rel err vs FP32 oracle", err, flush=True)
+    for k, e in err.items():
+        assert e <= TOL, (k, e)
+
+
+def test_one_substep_vs_fp64_oracle():
+    sc = scenes.build_dam_break(5000, substeps=1, iterations=4)
+    o = oracle_dam_break(sc["positions"], sc["radius"], 1, 4, "f64")
+    sc["loop"].step(1.0 / 120.0)
+    o.step(1.0 / 120.0)
+    err = compare(gpu_state(sc), o.get())
+    print("[pbf] rel err vs FP64 oracle", err, flush=True)
+    assert err["x"] <= TOL and err["xs"] <= TOL and err["rho"] <= TOL
+    # velocity = (x* - x) / dt amplifies FP32 position rounding
+    assert err["v"] <= 1e-3
+
+
+def diagnostics(x, v, rho, m):
+    ke = 0.5 * m * (v ** 2).sum()
+    com = x.mean(0)
+    resid = np.maximum(rho / 1000.0 - 1.0, 0.0).mean()
+    return ke, com, resid
+
+
+def test_200_steps_diagnostics_agree():
+    sc = scenes.build_dam_break(5000, substeps=2, iterations=4)
+    o = oracle_dam_break(sc["positions"], sc["radius"], 2, 4, "f32")
+    for _ in range(200):
+        sc["loop"].step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+    g, oo = gpu_state(sc), o.get()
+    m = sc["fluid"].mass
+    kg, cg, rg = diagnostics(g["x"], g["v"], g["rho"], m)
+    ko, co, ro = diagnostics(oo["x"], oo["v"], oo["rho"], m)
+    print(f"[pbf] 200 steps KE {kg:.6e}/{ko:.6e} COM {cg}/{co} resid {rg:.4e}/{ro:.4e}", flush=True)
+    assert abs(kg - ko) <= 0.01 * abs(ko) + 1e-9
+    assert np.abs(cg - co).max() <= 0.01 * np.abs(co).max()
+    assert abs(rg - ro) <= 0.01 * max(ro, 1e-3)
+    assert not any(sc["loop"].readbackOverflow()[:3])
+
+
+def test_graph_replay_matches_eager():
+    a = scenes.build_dam_break(5000, substeps=2, iterations=4)
+    b = scenes.build_dam_break(5000, substeps=2, iterations=4, use_graph=True)
+    for k in range(6):
+        a["loop"].step(1.0 / 60.0)
+        b["loop"].step(1.0 / 60.0)
+        ga, gb = gpu_state(a), gpu_state(b)
+        for key in ga:
+            assert np.array_equal(ga[key], gb[key]), (k, key)
+
+
+def test_odd_substeps_and_three_iterations():
+    # descriptor variants swap every step when the flip count is odd
+    sc = scenes.build_dam_break(5000, substeps=3, iterations=3)
+    o = oracle_dam_break(sc["positions"], sc["radius"], 3, 3, "f32")
+    for _ in range(3):
+        sc["loop"].step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+    err = compare(gpu_state(sc), o.get())
+    assert max(err.values()) <= TOL, err
+
+
+def test_batch_matches_each_scene_alone():
+    budgets = [3000, 5000, 8000]
+    alone = [scenes.build_dam_break(b, substeps=2, iterations=4, seed=b) for b in budgets]
+    mixed = [scenes.build_dam_break(b, substeps=2, iterations=4, seed=b) for b in budgets]
+    batch = pf.Batch([s["loop"] for s in mixed])
+    for _ in range(3):
+        for s in alone:
+            s["loop"].step(1.0 / 60.0)
+        batch.step(1.0 / 60.0)
+    for s, t in zip(alone, mixed):
+        ga, gb = gpu_state(s), gpu_state(t)
+        for key in ga:
+            assert np.array_equal(ga[key], gb[key]), key
+
+
+def test_softbody_global_matches_oracle():
+    # 8 cubes of 6^3 particles dropped on the floor, global shape matching + planes.
+    r = 0.01
+    cubes = []
+    for k in range(8):
+        g = np.stack(np.meshgrid(*[np.arange(6)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        cubes.append((g * 2 * r + np.array([-0.4 + 0.1 * k, 0.05 + 0.02 * k, 0.0])).astype(np.float32))
+    pos = np.concatenate(cubes)
+    n = len(pos)
+    ps = pf.ParticleSystem(n, r)
+    ps.uploadParticles(pos, invMass=np.full(n, 2.0, np.float32))
+    bodies = [dict(range=(216 * k, 216), compliance=1e-6 * (k + 1)) for k in range(8)]
+    sb = pf.SoftbodySystem(ps, bodies)
+    walls = pf.PrimitiveSet(ps)
+    walls.addPlane((0, 1, 0), (0, 0, 0))
+    loop = pf.SimLoop(ps, substeps=2, iterations=3, materials=[sb], colliders=[walls])
+    o = orc.Scene(n, r, "f32")
+    o.set_particles(pos, None, np.full(n, 2.0, np.float32))
+    o.set_loop(2, 3, (0.0, -9.81, 0.0), 0.0, 1)
+    o.add_plane((0, 1, 0), (0, 0, 0))
+    for k in range(8):
+        # centroid as SoftbodySystem sums it: sequentially, in double (SoftbodySystem.ts:374-386)
+        c = np.zeros(3)
+        for p in cubes[k].astype(np.float64):
+            c += p * 1.0
+        c /= 216.0
+        o.add_body(216 * k, 216, 1e-6 * (k + 1), (cubes[k] - c).astype(np.float32), np.ones(216, np.float32))
+    for _ in range(20):
+        loop.step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+    snap = ps.readback()
+    st = o.get()
+    ex = rel_err(snap.positions[:, :3], st["x"])
+    ev = rel_err(snap.velocities[:, :3], st["v"])
+    print(f"[softbody] rel err x {ex:.3e} v {ev:.3e}", flush=True)
+    assert ex <= TOL and ev <= 1e-4
+    c_gpu, r_gpu = sb.readbackFrames()
+    c_o, r_o = o.body_frame(3)
+    assert np.abs(c_gpu[3] - c_o).max() < 1e-5

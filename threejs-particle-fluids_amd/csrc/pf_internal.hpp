@@ -1,0 +1,255 @@
+// Internal device-side layout and math shared by every HIP kernel of the
+// particle solver (gfx950 / CDNA4, wave64).
+//
+// Particle state lives in HBM as cell-sorted structure-of-arrays: after every
+// substep's counting sort each particle sits at the slot of its hash bucket,
+// and `ids[slot]` records which original particle it is. Kernels read scene
+// parameters and buffer pointers from a `SceneDev` descriptor in device
+// memory, selected per workgroup through a tile list, so one launch can
+// advance one scene (plain form) or many independent scenes (batched form).
+//
+// Floating-point convention: every kernel is compiled with
+// -ffp-contract=off and evaluates expressions in the operand order of the
+// reference TSL passes (cited per function), with IEEE division and sqrt.
+// The CPU oracle follows the same order, so single-substep results agree
+// bit-for-bit when the neighbour order agrees (it does: stable sort by id).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PF_MAX_FLUIDS 4
+#define PF_MAX_NEIGHBORS 64          // neighborList.ts:14
+#define PF_MAX_BOUNDARIES 8
+#define PF_MAX_SUBSTEPS 64
+#define PF_MORTON_BIAS 512           // mortonHash.ts:25
+#define PF_CELL_COORD_MAX (1 << 30)  // cellIndex.ts:35
+#define PF_SCAN_TILE 4096            // buckets per scan workgroup (256 x 16)
+#define PF_BLOCK 256
+#define PF_STABLE_SEGMENT_LIMIT 2048 // buckets longer than this keep atomic order (flagged)
+
+// Overflow / status flag words, one u32 each in SceneDev::flags.
+enum {
+  PF_FLAG_GRID = 0,        // cell coordinate outside the Morton range (cellIndex.ts:150-169)
+  PF_FLAG_NEIGHBORS = 1,   // a fluid neighbour list was truncated at 64 (neighborList.ts:63-66)
+  PF_FLAG_POSITIONS = 2,   // position accumulator saturated (accumulator.ts:114-120)
+  PF_FLAG_VELOCITIES = 3,  // velocity accumulator saturated
+  PF_FLAG_IMPULSES = 4,    // surface-tension/adhesion accumulator saturated
+  PF_FLAG_LONG_BUCKET = 5, // a bucket exceeded the stable-rank limit (order not canonical)
+  PF_FLAG_REACTION = 6,    // boundary reaction accumulator saturated
+  PF_NUM_FLAGS = 8
+};
+
+// Fluid effect bits (FluidSystem.ts:31-54: optional effects compiled in only when given).
+enum {
+  PF_FX_VISCOSITY = 1u << 0,
+  PF_FX_VORTICITY = 1u << 1,
+  PF_FX_SURFACE_TENSION = 1u << 2,
+  PF_FX_ADHESION = 1u << 3,
+};
+
+// Primitive kinds / flags (primitives.ts:7-20).
+enum { PF_KIND_PLANE = 0, PF_KIND_SPHERE = 1, PF_KIND_BOX = 2, PF_KIND_CAPSULE = 3 };
+enum { PF_PRIM_INVERT = 1u << 0, PF_PRIM_SWEEP = 1u << 1 };
+
+struct FluidDev {
+  uint32_t start, count;   // particle id range of the fluid
+  uint32_t effects;        // PF_FX_* bits
+  uint32_t nboundary;      // boundary ranges (FluidSystem.addBoundary)
+  uint32_t bstart[PF_MAX_BOUNDARIES], bcount[PF_MAX_BOUNDARIES];
+  uint32_t bdynamic;       // bit b: boundary b recomputed every substep
+  float rho0, pv, mass;    // rest density, spacing^3, rho0*spacing^3 (f32 uniforms)
+  float h, h2, poly6, spiky; // kernels.ts:22-47 (coefficients computed in double)
+  float compliance;
+  float gamma, visc, vort, adhesion;
+  float coh_coef, coh_h6_64;        // cohesion.ts:46-55 constants
+  float adh_coef, adh_m4_over_h, adh_2h; // adhesion.ts:16-27 constants
+  float imp_scale, imp_inv;         // Accumulator(50) for impulses
+  float rea_scale, rea_inv;         // Accumulator(10) for boundary reaction
+  uint32_t* nl;        // [64 * n] neighbour slots, column-major k*n + slot
+  uint32_t* nl_count;  // [n]
+  float4* normal;      // [n] colour-field normal
+  float4* omega;       // [n] vorticity (xyz) and |omega| (w)
+  float4* dv;          // [n] XSPH velocity change
+  int* reaction;       // [3n] fixed-point boundary reaction (only with boundaries)
+};
+
+// Global shape matching of one scene's soft bodies, in original-id order
+// (SoftbodySystem.ts:105-199).
+struct SoftDev {
+  uint32_t nbodies;
+  const uint32_t* body_start;
+  const uint32_t* body_count;
+  const float* body_compliance;
+  const float4* rest;   // [n] rest offset from the body's rest centre, w = compliance
+  const float* weights; // [n] mass weight relative to the body mean (massWeights)
+  float4* lambda;       // [n] XPBD multiplier per particle
+  float4* centers;      // [nbodies]
+  float4* rotations;    // [3 * nbodies] rows of R
+};
+
+struct SceneDev {
+  uint32_t n;          // particle capacity
+  uint32_t H;          // hash table size (power of two)
+  uint32_t hash_mask;
+  uint32_t nscan_tiles;
+  float cell, ox, oy, oz;  // grid cell size and hash origin
+  float radius;            // particle radius (collider contact distance)
+  float dt;                // substep length (f32 uniform)
+  float gx, gy, gz;        // gravity
+  uint32_t nfluids;
+  uint32_t has_boundary;   // any particle has boundaryVolume > 0
+  // permuted particle state, double buffered by substep parity
+  float4* x[2];
+  float4* v[2];
+  float* w[2];             // inverse mass
+  uint32_t* id[2];         // original particle id at each slot
+  float* rho[2];           // fluid density (persists: surface tension reads the previous one)
+  float* bvol[2];          // boundary volume (Akinci), 0 for non-boundary particles
+  uint32_t* grp[2];        // collision group
+  // per-substep buffers
+  float4* xs[2];           // predicted positions ping-pong; .w carries the fluid lambda
+  float* mass;             // 1/w or 0, per slot
+  uint32_t* slot_of;       // original id -> slot
+  // counting sort
+  uint32_t* key;           // [n] bucket per pre-sort slot
+  uint32_t* rank;          // [n] atomic arrival rank inside the bucket
+  uint32_t* counts;        // [H] histogram, left zeroed by the scan
+  uint32_t* start;         // [H+1] exclusive prefix (cellStart); start[b+1] is cellEnd
+  unsigned long long* scan_status; // [nscan_tiles] decoupled look-back words
+  uint32_t* scan_ctl;      // [2] tiles finished this scan, scan epoch
+  uint32_t* tmp_src;       // [n] pre-sort slot at each bucket position (arrival order)
+  uint32_t* tmp_id;        // [n] original id at each bucket position (arrival order)
+  uint32_t* tmp_key;       // [n] bucket at each bucket position
+  uint32_t* flags;         // [PF_NUM_FLAGS]
+  // colliders (one PrimitiveSet per scene)
+  uint32_t nprim, prim_cap;
+  const uint32_t* prim_packed; // (kind << 16) | flags
+  const float4* prim_d0;
+  const float4* prim_d1;       // .w = muS
+  const float4* prim_vel;      // .w = muK
+  const float4* prim_rot;      // quaternion xyzw
+  const float4* prim_spin;     // angular velocity xyz
+  float* prim_lambda;          // [2 * prim_cap * n] (lambda_n, lambda_t) per slot/primitive
+  uint32_t* prim_touch;        // [n] bit p set when primitive p touched the slot this substep
+  float pos_scale, pos_inv, vel_scale, vel_inv; // Accumulator(10), Accumulator(50)
+  float clock[PF_MAX_SUBSTEPS];                 // sweep clock per substep (PrimitiveSet.ts:393-398)
+  FluidDev fluid[PF_MAX_FLUIDS];
+  SoftDev soft;
+};
+
+// One workgroup's share of a per-particle kernel: scene index and first slot.
+struct Tile {
+  uint32_t scene;
+  uint32_t first;
+};
+
+// ---------------------------------------------------------------------------
+// Integer helpers (bit-exact with the reference's WGSL semantics)
+// ---------------------------------------------------------------------------
+
+// Spread the low 10 bits of n to every third bit (mortonHash.ts:39-50).
+__host__ __device__ __forceinline__ uint32_t pf_part1by2(uint32_t n) {
+  n &= 0x000003ffu;
+  n = (n | (n << 16)) & 0xff0000ffu;
+  n = (n | (n << 8)) & 0x0300f00fu;
+  n = (n | (n << 4)) & 0x030c30c3u;
+  n = (n | (n << 2)) & 0x09249249u;
+  return n;
+}
+
+// Unmasked Morton code of signed cell coordinates (mortonHash.ts:71-79).
+// The bias add is done in unsigned arithmetic: two's-complement wrap, as WGSL.
+__host__ __device__ __forceinline__ uint32_t pf_morton(int cx, int cy, int cz) {
+  const uint32_t ux = (uint32_t)cx + (uint32_t)PF_MORTON_BIAS;
+  const uint32_t uy = (uint32_t)cy + (uint32_t)PF_MORTON_BIAS;
+  const uint32_t uz = (uint32_t)cz + (uint32_t)PF_MORTON_BIAS;
+  return pf_part1by2(ux) | (pf_part1by2(uy) << 1) | (pf_part1by2(uz) << 2);
+}
+
+// Saturating f32 -> i32 (WGSL conversion semantics; NaN maps to 0).
+__host__ __device__ __forceinline__ int pf_f2i_sat(float f) {
+  if (!(f == f)) return 0;
+  if (f >= 2147483648.0f) return 2147483647;
+  if (f < -2147483648.0f) return (-2147483647 - 1);
+  return (int)f;
+}
+
+__host__ __device__ __forceinline__ int pf_clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Cell coordinate for binning: FP32 subtract, then divide, floor, saturate,
+// clamp to +-2^30 (cellIndex.ts:113-140). Returns 1 in *over when the clamp
+// fired or the coordinate is outside the Morton range (cellIndex.ts:150-169).
+__device__ __forceinline__ int pf_bin_coord(float p, float o, float cell, uint32_t* over) {
+  const float rel = (p - o) / cell;
+  const int raw = pf_f2i_sat(floorf(rel));
+  const int c = pf_clampi(raw, -PF_CELL_COORD_MAX, PF_CELL_COORD_MAX - 1);
+  if (raw != c || c < -PF_MORTON_BIAS || c >= PF_MORTON_BIAS) *over = 1u;
+  return c;
+}
+
+// Query cell coordinate: floor + conversion without clamp (query.ts:30-33).
+__device__ __forceinline__ int pf_query_coord(float p, float o, float cell) {
+  return pf_f2i_sat(floorf((p - o) / cell));
+}
+
+// ---------------------------------------------------------------------------
+// SPH kernels (kernels.ts:64-107), FP32, operand order as the reference
+// ---------------------------------------------------------------------------
+
+// W = max(h^2 - r^2, 0)^3 * poly6Coef, evaluated ((d*d)*d)*coef.
+__device__ __forceinline__ float pf_poly6(float r2, float h2, float poly6) {
+  const float d = fmaxf(h2 - r2, 0.0f);
+  return ((d * d) * d) * poly6;
+}
+
+// grad W = (r_vec / max(|r|, 1e-20)) * (-(t*t*spikyCoef)), t = max(h - |r|, 0).
+__device__ __forceinline__ float3 pf_spiky_grad(float3 r, float h, float spiky) {
+  const float rsq = (r.x * r.x + r.y * r.y) + r.z * r.z;
+  const float rl = sqrtf(rsq);
+  const float t = fmaxf(h - rl, 0.0f);
+  const float rs = fmaxf(rl, 1e-20f);
+  const float mag = -((t * t) * spiky);
+  return make_float3((r.x / rs) * mag, (r.y / rs) * mag, (r.z / rs) * mag);
+}
+
+__device__ __forceinline__ float pf_dot3(float3 a, float3 b) {
+  return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+__device__ __forceinline__ float3 pf_sub3(float3 a, float3 b) {
+  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ __forceinline__ float3 pf_add3(float3 a, float3 b) {
+  return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+__device__ __forceinline__ float3 pf_mul3(float3 a, float s) {
+  return make_float3(a.x * s, a.y * s, a.z * s);
+}
+__device__ __forceinline__ float3 pf_cross3(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float pf_len3(float3 a) { return sqrtf(pf_dot3(a, a)); }
+__device__ __forceinline__ float3 pf_xyz(float4 a) { return make_float3(a.x, a.y, a.z); }
+
+// Fixed-point accumulator add (accumulator.ts:62-68): i32(round(v * scale)),
+// round-half-to-even, saturating conversion. Sums wrap like WGSL atomicAdd.
+__device__ __forceinline__ int pf_ticks(float v, float scale) {
+  return pf_f2i_sat(rintf(v * scale));
+}
+__device__ __forceinline__ int pf_wrap_add(int a, int b) {
+  return (int)((uint32_t)a + (uint32_t)b);
+}
+
+// XCD-aware mapping: gfx950 places consecutive workgroups on different XCDs
+// round-robin. Give each XCD a contiguous run of tiles so neighbour reads of
+// spatially adjacent (cell-sorted) particles hit the same XCD's L2.
+__device__ __forceinline__ uint32_t pf_xcd_remap(uint32_t bid, uint32_t nblocks) {
+  const uint32_t nxcd = 8;
+  if (nblocks < 2 * nxcd) return bid;
+  const uint32_t per = nblocks / nxcd;      // full runs
+  const uint32_t full = per * nxcd;
+  if (bid >= full) return bid;              // remainder keeps identity
+  return (bid % nxcd) * per + bid / nxcd;
+}

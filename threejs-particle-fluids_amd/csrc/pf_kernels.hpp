@@ -1,0 +1,49 @@
+// Host-side launchers of the device kernels (one per kernel family).
+#pragma once
+#include "pf_internal.hpp"
+
+namespace pf {
+
+// grid.hip
+void launch_hash(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
+                 int mode, int parity, int xs_prev, int reset_step_flags);
+void launch_scan(hipStream_t st, const SceneDev* scenes, const uint2* stiles, uint32_t nstiles,
+                 void* ctl);
+void launch_place(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
+                  int mode, int parity);
+void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
+                 int mode, int parity, int xs_prev, int xs_out, uint32_t* out_perm,
+                 uint32_t* out_skey, float4* out_spos);
+
+// fluid.hip
+void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                      int p, int c, bool normal);
+void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                     int p, int c, bool surface_tension, bool adhesion);
+void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                   int p, int c);
+void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                  int p, int c, bool walls, bool advect, int first_iteration, int substep);
+void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c,
+                  bool advect, int first_iteration, int substep);
+void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c);
+void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                     int p, bool vort, bool visc);
+void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                      int p, int c, bool vort, bool visc, bool friction, int substep,
+                      int touch_valid);
+void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,
+                     int c, int substep);
+
+// softbody.hip
+void launch_shape_match(hipStream_t st, const SceneDev* sc, const uint2* bodies, uint32_t nbodies,
+                        int p, int c, int first_iteration, int pull);
+
+// util kernels (runtime.hip)
+void launch_gather_by_id(hipStream_t st, const SceneDev* sc, int p, int c, uint32_t n, int what,
+                         void* out);
+void launch_scatter_upload(hipStream_t st, const SceneDev* sc, int p, int c, uint32_t start,
+                           uint32_t count, const float4* pos, const float4* vel, const float* inv_mass,
+                           const uint32_t* group, uint32_t mask);
+
+}  // namespace pf

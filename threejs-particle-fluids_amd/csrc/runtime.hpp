@@ -1,0 +1,254 @@
+// Host-side objects behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pf.h"
+#include "pf_internal.hpp"
+#include "pf_kernels.hpp"
+
+namespace pf {
+
+void set_error(const char* fmt, ...);
+
+struct HipError {
+  int code;
+};
+
+struct PfError {
+  int code;
+};
+
+#define PF_HIP(expr)                                                               \
+  do {                                                                             \
+    hipError_t e__ = (expr);                                                       \
+    if (e__ != hipSuccess) {                                                       \
+      ::pf::set_error("HIP: %s failed: %s", #expr, hipGetErrorString(e__));       \
+      throw ::pf::HipError{PF_EDEVICE};                                            \
+    }                                                                              \
+  } while (0)
+
+// Throw an argument error with the reference's "<Owner>: <reason>" wording.
+[[noreturn]] void fail(int code, const char* fmt, ...);
+
+template <typename T>
+T* dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  PF_HIP(hipMemset(p, 0, n * sizeof(T)));
+  // Library streams are non-blocking: finish the clear before any of them runs.
+  PF_HIP(hipDeviceSynchronize());
+  return static_cast<T*>(p);
+}
+void dfree(void* p);
+
+struct KernelStat {
+  uint32_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct Context;
+
+// Brackets a launch with events when the context is profiling.
+struct LaunchScope {
+  Context* ctx;
+  const char* name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LaunchScope(Context* c, const char* n);
+  ~LaunchScope();
+};
+#define PF_LAUNCH(ctx, name) ::pf::LaunchScope pf_scope__(ctx, name)
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool profiling = false;
+  bool capturing = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, KernelStat> stats;
+  void* scan_ctl = nullptr;  // ScanCtlDev for launches on this stream
+  hipEvent_t get_event();
+  void collect();
+};
+
+// Per-scene device state shared by particles, grid, loop.
+struct Particles {
+  Context* ctx = nullptr;
+  uint32_t n = 0;
+  double radius = 0;
+  bool disposed = false;
+  bool stepped = false;  // state is cell-sorted (slot != id)
+  SceneDev host{};       // host copy of the descriptor (buffer pairs unswapped)
+  SceneDev* dev = nullptr;  // 4 device variants: bit 1 swaps state pairs, bit 0 swaps xs
+  int variant = 0;       // variant whose [0] buffers hold the current state / prediction
+  bool dirty = true;
+  SceneDev desc(int v) const;
+  SceneDev* cur() const { return dev + variant; }
+  std::vector<float> cpu_inv_mass;   // host mirror of inverse masses (original order)
+  std::vector<float> cpu_positions;  // host mirror of uploaded positions (xyz)
+  std::vector<uint32_t> cpu_groups;
+  struct pf_loop_s* loop = nullptr;  // attached loop, if any
+  void upload_desc();
+};
+
+struct Grid;
+struct Fluid;
+struct Primitives;
+struct Softbody;
+struct Loop;
+
+}  // namespace pf
+
+struct pf_context_s : pf::Context {};
+struct pf_particles_s : pf::Particles {};
+
+struct pf_grid_s {
+  pf::Context* ctx = nullptr;
+  pf_particles_s* ps = nullptr;  // null for standalone grids
+  bool owned_by_loop = false;
+  uint32_t n = 0, H = 0, Hpad = 0;
+  double cell = 0, origin[3] = {0, 0, 0};
+  bool sorted_positions = true;
+  bool built = false;
+  SceneDev host{};
+  SceneDev* dev = nullptr;
+  Tile* tiles = nullptr;
+  uint32_t ntiles = 0;
+  uint2* stiles = nullptr;
+  uint32_t nstiles = 0;
+  float4* positions = nullptr;   // [n] input positions (original order)
+  uint32_t* perm = nullptr;      // [n] sorted indices
+  uint32_t* skey = nullptr;      // [n] sorted keys
+  float4* spos = nullptr;        // [n] sorted positions
+  bool disposed = false;
+};
+
+struct pf_fluid_s {
+  pf_particles_s* ps = nullptr;
+  uint32_t start = 0, count = 0;
+  double rest_density = 1000, spacing = 0, h = 0, compliance = 1e-4, mass = 0;
+  uint32_t present = 0;
+  double viscosity = 0, vorticity = 0, surface_tension = 0, adhesion = 0;
+  struct Boundary { uint32_t start, count; bool dynamic; };
+  std::vector<Boundary> boundaries;
+  pf_loop_s* loop = nullptr;
+  int slot = -1;  // index in SceneDev::fluid
+  uint32_t* nl = nullptr;
+  uint32_t* nl_count = nullptr;
+  float4* normal = nullptr;
+  float4* omega = nullptr;
+  float4* dv = nullptr;
+  int* reaction = nullptr;
+  bool disposed = false;
+};
+
+struct pf_primitives_s {
+  pf_particles_s* ps = nullptr;
+  uint32_t reserved = 0;
+  struct Prim {
+    uint32_t kind, flags;
+    double d0[4], d1[4], vel[4], spin[3], rot[4];
+    bool has_pose = false;
+    double prev_pos[3], prev_rot[4], offset[4], direction[3];
+  };
+  std::vector<Prim> prims;
+  uint32_t capacity = 0;  // fixed once a loop is built
+  pf_loop_s* loop = nullptr;
+  uint32_t* packed = nullptr;
+  float4 *d0 = nullptr, *d1 = nullptr, *vel = nullptr, *rot = nullptr, *spin = nullptr;
+  bool dirty = true;
+  bool disposed = false;
+  void upload();
+};
+
+struct pf_softbody_s {
+  pf_particles_s* ps = nullptr;
+  struct Body { uint32_t start, count, surface_count; double compliance; };
+  std::vector<Body> bodies;
+  std::vector<float> rest;     // 4n (id order), w = compliance
+  std::vector<float> weights;  // n
+  uint32_t* d_start = nullptr;
+  uint32_t* d_count = nullptr;
+  float* d_compliance = nullptr;
+  float4* d_rest = nullptr;
+  float* d_weights = nullptr;
+  float4* d_lambda = nullptr;
+  float4* d_centers = nullptr;
+  float4* d_rotations = nullptr;
+  pf_loop_s* loop = nullptr;
+  bool disposed = false;
+};
+
+// A launch plan: a uniform substep schedule over one or more scenes.
+struct PlanShape {
+  uint32_t substeps = 4, iterations = 2;
+  uint32_t nfluids = 0;
+  uint32_t fx[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of effects per fluid slot
+  bool any_boundary = false;
+  bool walls = false;       // any scene has primitives
+  bool soft = false;        // any scene has soft bodies
+};
+
+struct pf_loop_s {
+  pf_particles_s* ps = nullptr;
+  pf::Context* ctx = nullptr;
+  uint32_t substeps = 4, iterations = 2;
+  double gravity[3] = {0, -9.81, 0};
+  std::vector<pf_fluid_s*> fluids;
+  pf_softbody_s* soft = nullptr;
+  pf_primitives_s* prims = nullptr;
+  pf_grid_s grid;  // the loop's grid view (owned_by_loop)
+  Tile* tiles = nullptr;
+  uint32_t ntiles = 0;
+  uint2* stiles = nullptr;
+  uint32_t nstiles = 0;
+  uint2* btiles = nullptr;
+  uint32_t nbtiles = 0;
+  double last_dt = -1;
+  uint32_t last_flag_half = 0;
+  uint64_t steps = 0;
+  bool use_graph = false;
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // by descriptor variant
+  bool graph_valid[4] = {false, false, false, false};
+  double graph_dt = -1;
+  uint32_t kernels_per_step = 0;
+  bool disposed = false;
+};
+
+struct pf_batch_s {
+  pf::Context* ctx = nullptr;
+  std::vector<pf_loop_s*> loops;
+  SceneDev* dev = nullptr;
+  Tile* tiles = nullptr;
+  uint32_t ntiles = 0;
+  uint2* stiles = nullptr;
+  uint32_t nstiles = 0;
+  uint2* btiles = nullptr;
+  uint32_t nbtiles = 0;
+  void* scan_ctl = nullptr;
+  PlanShape shape;
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  bool graph_valid = false;
+  bool desc_dirty = true;
+};
+
+namespace pf {
+// The substep schedule, shared by loops and batches (sim.hip).
+uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& shape,
+                      const Tile* tiles, uint32_t ntiles, const uint2* stiles, uint32_t nstiles,
+                      const uint2* btiles, uint32_t nbtiles, void* scan_ctl);
+// Swap descriptor halves after a step with an odd number of state flips.
+void finish_step_host(Particles* ps, const PlanShape& shape);
+}  // namespace pf

@@ -1,0 +1,1362 @@
+// FluidSystem, PrimitiveSet, SoftbodySystem, SimLoop and the batched form
+// behind the C ABI, plus the substep schedule shared by loops and batches,
+// and the Simulation scene-layout helpers.
+#include <algorithm>
+#include <cmath>
+
+#include "runtime.hpp"
+
+namespace pf {
+void particles_scatter(pf_particles* ps, uint32_t start, uint32_t count, const std::vector<float4>* pos,
+                       const std::vector<float4>* vel, const std::vector<float>* f,
+                       const std::vector<uint32_t>* grp, uint32_t mask);
+void particles_gather(pf_particles* ps, int what, void* host, size_t elem);
+
+// Predict without a grid (integrate.ts:25-34), for scenes with no neighbour
+// queries: the state keeps its order and the prediction goes to xs[c ^ 1].
+__global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __restrict__ tiles,
+                          uint32_t ntiles, int p, int c, int reset_step_flags) {
+  const Tile t = tiles[blockIdx.x];
+  const SceneDev& S = scenes[t.scene];
+  const uint32_t s = t.first + threadIdx.x;
+  if (reset_step_flags && t.first == 0 && threadIdx.x == 0) {
+    S.flags[PF_FLAG_POSITIONS] = 0u;
+    S.flags[PF_FLAG_VELOCITIES] = 0u;
+  }
+  if (s >= S.n) return;
+  const float w = S.w[p][s];
+  float4 xs = S.xs[c][s];
+  if (w > 0.0f) {
+    const float4 v = S.v[p][s];
+    const float vx = v.x + S.gx * S.dt, vy = v.y + S.gy * S.dt, vz = v.z + S.gz * S.dt;
+    S.v[p][s] = make_float4(vx, vy, vz, 0.0f);
+    const float4 x = S.x[p][s];
+    xs = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
+  }
+  S.xs[c ^ 1][s] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+}
+
+// The substep schedule (loop.ts:175-241, FluidSystem.ts:200-309), fused as
+// described in fluid.hip. Scenes start every step at state parity 0 and
+// prediction buffer 0; the caller swaps descriptor variants afterwards.
+// Returns the number of kernel launches.
+uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh, const Tile* tiles,
+                      uint32_t ntiles, const uint2* stiles, uint32_t nstiles, const uint2* btiles,
+                      uint32_t nbtiles, void* scan_ctl) {
+  hipStream_t st = ctx->stream;
+  uint32_t nk = 0;
+  int c = 0;
+  const bool grid = sh.nfluids > 0;
+  enum LastOp { NONE, DELTA, WALLS, SOFT };
+  LastOp last = NONE;
+  if (sh.walls) last = (sh.soft || sh.nfluids == 0) ? WALLS : DELTA;
+  else if (sh.soft) last = SOFT;
+  else if (sh.nfluids > 0) last = DELTA;
+  for (uint32_t k = 0; k < sh.substeps; ++k) {
+    int p = 0;
+    if (grid) {
+      p = (int)(k & 1u);
+      { PF_LAUNCH(ctx, "hash_predict"); launch_hash(st, scenes, tiles, ntiles, 1, p, c, k == 0); ++nk; }
+      { PF_LAUNCH(ctx, "scan"); launch_scan(st, scenes, stiles, nstiles, scan_ctl); ++nk; }
+      { PF_LAUNCH(ctx, "place"); launch_place(st, scenes, tiles, ntiles, 1, p); ++nk; }
+      { PF_LAUNCH(ctx, "rank_permute"); launch_rank(st, scenes, tiles, ntiles, 1, p, c, c ^ 1, nullptr, nullptr, nullptr); ++nk; }
+      c ^= 1;
+      p ^= 1;
+    } else {
+      PF_LAUNCH(ctx, "predict");
+      hipLaunchKernelGGL(k_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, 0, c, k == 0);
+      ++nk;
+      c ^= 1;
+    }
+    for (uint32_t f = 0; f < sh.nfluids; ++f) {
+      PF_LAUNCH(ctx, "neighbors");
+      launch_neighbors(st, scenes, tiles, ntiles, (int)f, p, c, (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0);
+      ++nk;
+    }
+    for (uint32_t f = 0; f < sh.nfluids; ++f) {
+      const bool stx = (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0;
+      const bool adh = (sh.fx[f] & PF_FX_ADHESION) != 0;
+      if (stx || adh) {
+        PF_LAUNCH(ctx, "impulses");
+        launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary);
+        ++nk;
+        c ^= 1;
+      }
+    }
+    for (uint32_t it = 0; it < sh.iterations; ++it) {
+      const bool lastit = it + 1 == sh.iterations;
+      for (uint32_t f = 0; f < sh.nfluids; ++f) {
+        const bool lastf = f + 1 == sh.nfluids;
+        { PF_LAUNCH(ctx, "lambda"); launch_lambda(st, scenes, tiles, ntiles, (int)f, p, c); ++nk; }
+        const bool fuse_walls = lastf && last == DELTA && sh.walls;
+        const bool fuse_adv = lastf && lastit && last == DELTA;
+        {
+          PF_LAUNCH(ctx, fuse_walls ? "delta_walls" : "delta");
+          launch_delta(st, scenes, tiles, ntiles, (int)f, p, c, fuse_walls, fuse_adv, it == 0, (int)k);
+          ++nk;
+        }
+        c ^= 1;
+      }
+      if (sh.soft) {
+        PF_LAUNCH(ctx, "shape_match");
+        launch_shape_match(st, scenes, btiles, nbtiles, p, c, it == 0, 1);
+        ++nk;
+      }
+      if (sh.walls && last == WALLS) {
+        PF_LAUNCH(ctx, "walls");
+        launch_walls(st, scenes, tiles, ntiles, p, c, lastit, it == 0, (int)k);
+        ++nk;
+      }
+    }
+    if (sh.iterations == 0 || last == SOFT || last == NONE) {
+      PF_LAUNCH(ctx, "advect");
+      launch_advect(st, scenes, tiles, ntiles, p, c);
+      ++nk;
+    }
+    const int touch_valid = sh.iterations > 0 ? 1 : 0;
+    bool friction_done = !sh.walls;
+    for (uint32_t f = 0; f < sh.nfluids; ++f) {
+      const bool vort = (sh.fx[f] & PF_FX_VORTICITY) != 0;
+      const bool visc = (sh.fx[f] & PF_FX_VISCOSITY) != 0;
+      const bool lastf = f + 1 == sh.nfluids;
+      if (vort || visc) {
+        PF_LAUNCH(ctx, "velocity");
+        launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc);
+        ++nk;
+      }
+      const bool fric = lastf && sh.walls;
+      if (vort || visc || fric) {
+        PF_LAUNCH(ctx, fric ? "vorticity_friction" : "vorticity");
+        launch_vorticity(st, scenes, tiles, ntiles, (int)f, p, c, vort, visc, fric, (int)k, touch_valid);
+        ++nk;
+        if (fric) friction_done = true;
+      }
+    }
+    if (!friction_done) {
+      PF_LAUNCH(ctx, "friction");
+      if (touch_valid) launch_friction(st, scenes, tiles, ntiles, p, c, (int)k);
+      ++nk;
+    }
+  }
+  return nk;
+}
+
+// Parity bookkeeping of one step: returns the descriptor variant flips.
+int step_variant_flips(const PlanShape& sh) {
+  int state = 0, xs = 0;
+  const bool grid = sh.nfluids > 0;
+  for (uint32_t k = 0; k < sh.substeps; ++k) {
+    if (grid) state ^= 1;
+    xs ^= 1;  // hash/predict
+    for (uint32_t f = 0; f < sh.nfluids; ++f)
+      if (sh.fx[f] & (PF_FX_SURFACE_TENSION | PF_FX_ADHESION)) xs ^= 1;
+    for (uint32_t it = 0; it < sh.iterations; ++it)
+      for (uint32_t f = 0; f < sh.nfluids; ++f) xs ^= 1;
+  }
+  return (state << 1) | xs;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+#define PF_API_BEGIN try {
+#define PF_API_END                                    \
+  }                                                   \
+  catch (const pf::PfError& e) { return e.code; }     \
+  catch (const pf::HipError& e) { return e.code; }    \
+  catch (const std::bad_alloc&) {                     \
+    pf::set_error("host allocation failed");          \
+    return PF_ENOMEM;                                 \
+  }                                                   \
+  catch (const std::exception& e) {                   \
+    pf::set_error("%s", e.what());                    \
+    return PF_EINVAL;                                 \
+  }                                                   \
+  return PF_OK;
+
+static void alive(pf_particles* ps, const char* owner) {
+  if (!ps) fail(PF_EINVAL, "%s: null handle", owner);
+  if (ps->disposed) fail(PF_ESTATE, "%s: the ParticleSystem has been disposed", owner);
+}
+
+static void range_ok(pf_particles* ps, uint32_t start, uint32_t count, const char* owner) {
+  if ((uint64_t)start + count > ps->n)
+    fail(PF_EINVAL, "%s: invalid particle range start=%u count=%u (capacity %u)", owner, start, count, ps->n);
+}
+
+// ---------------------------------------------------------------------------
+// Quaternion helpers (double, host) for PrimitiveSet poses
+// ---------------------------------------------------------------------------
+struct Q {
+  double x, y, z, w;
+};
+static Q qmul(Q a, Q b) {
+  return Q{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+           a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+static Q qinv(Q a) { return Q{-a.x, -a.y, -a.z, a.w}; }
+static void qapply(Q q, const double v[3], double out[3]) {
+  const double ix = q.w * v[0] + q.y * v[2] - q.z * v[1];
+  const double iy = q.w * v[1] + q.z * v[0] - q.x * v[2];
+  const double iz = q.w * v[2] + q.x * v[1] - q.y * v[0];
+  const double iw = -q.x * v[0] - q.y * v[1] - q.z * v[2];
+  out[0] = ix * q.w + iw * -q.x + iy * -q.z - iz * -q.y;
+  out[1] = iy * q.w + iw * -q.y + iz * -q.x - ix * -q.z;
+  out[2] = iz * q.w + iw * -q.z + ix * -q.y - iy * -q.x;
+}
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// FluidSystem
+// ---------------------------------------------------------------------------
+int pf_fluid_create(pf_particles* ps, const pf_fluid_options* o, pf_fluid** out) {
+  PF_API_BEGIN
+  alive(ps, "FluidSystem");
+  if (!out) fail(PF_EINVAL, "FluidSystem: null output");
+  pf_fluid_options def{};
+  def.compliance = -1;
+  const pf_fluid_options& opt = o ? *o : def;
+  uint32_t start = opt.range_start, count = opt.range_count;
+  if (count == 0) { start = 0; count = ps->n; }
+  range_ok(ps, start, count, "FluidSystem");
+  const double rho0 = opt.rest_density > 0 ? opt.rest_density : 1000.0;
+  if (!std::isfinite(rho0)) fail(PF_EINVAL, "FluidSystem: restDensity must be positive, got %g", rho0);
+  const double spacing = opt.particle_spacing > 0 ? opt.particle_spacing : 2 * ps->radius;
+  if (!std::isfinite(spacing)) fail(PF_EINVAL, "FluidSystem: particleSpacing must be positive, got %g", spacing);
+  const double h = opt.smoothing_radius > 0 ? opt.smoothing_radius : 2 * spacing;
+  if (!std::isfinite(h)) fail(PF_EINVAL, "FluidSystem: smoothingRadius must be positive, got %g", h);
+  const double compliance = opt.compliance >= 0 ? opt.compliance : 1e-4;
+  if (!std::isfinite(compliance)) fail(PF_EINVAL, "FluidSystem: compliance must be ≥ 0, got %g", compliance);
+  const double vals[4] = {opt.viscosity, opt.vorticity, opt.surface_tension, opt.adhesion};
+  const char* names[4] = {"viscosity", "vorticity", "surfaceTension", "adhesion"};
+  for (int k = 0; k < 4; ++k)
+    if ((opt.present >> k) & 1u)
+      if (!std::isfinite(vals[k])) fail(PF_EINVAL, "FluidSystem: %s must be finite, got %g", names[k], vals[k]);
+  auto* f = new pf_fluid_s();
+  f->ps = ps;
+  f->start = start;
+  f->count = count;
+  f->rest_density = rho0;
+  f->spacing = spacing;
+  f->h = h;
+  f->compliance = compliance;
+  f->mass = rho0 * spacing * spacing * spacing;  // FluidSystem.ts:109 (restDensity * spacing ** 3)
+  f->present = opt.present & 15u;
+  f->viscosity = opt.viscosity;
+  f->vorticity = opt.vorticity;
+  f->surface_tension = opt.surface_tension;
+  f->adhesion = opt.adhesion;
+  // The fluid owns its particles' mass (FluidSystem.ts:117-118) and its
+  // density starts at the rest density (FluidSystem.ts:121-126).
+  std::vector<float> im(count, (float)(1.0 / f->mass));
+  for (uint32_t k = 0; k < count; ++k) ps->cpu_inv_mass[start + k] = im[k];
+  particles_scatter(ps, start, count, nullptr, nullptr, &im, nullptr, 4u);
+  std::vector<float> rho(count, (float)rho0);
+  particles_scatter(ps, start, count, nullptr, nullptr, &rho, nullptr, 16u);
+  *out = f;
+  PF_API_END
+}
+
+static double* fluid_field(pf_fluid* f, uint32_t effect) {
+  switch (effect) {
+    case PF_FLUID_VISCOSITY: return &f->viscosity;
+    case PF_FLUID_VORTICITY: return &f->vorticity;
+    case PF_FLUID_SURFACE_TENSION: return &f->surface_tension;
+    case PF_FLUID_ADHESION: return &f->adhesion;
+  }
+  return nullptr;
+}
+
+static const char* effect_name(uint32_t e) {
+  switch (e) {
+    case PF_FLUID_VISCOSITY: return "viscosity";
+    case PF_FLUID_VORTICITY: return "vorticity";
+    case PF_FLUID_SURFACE_TENSION: return "surfaceTension";
+    case PF_FLUID_ADHESION: return "adhesion";
+  }
+  return "?";
+}
+
+static void fluid_write_params(pf_fluid* f);
+
+int pf_fluid_set(pf_fluid* f, uint32_t effect, double value) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  double* fld = fluid_field(f, effect);
+  if (!fld) fail(PF_EINVAL, "FluidSystem: unknown effect %u", effect);
+  if (!(f->present & effect))
+    fail(PF_EINVAL, "FluidSystem: pass `%s` in the options to enable it before changing it", effect_name(effect));
+  if (!std::isfinite(value)) fail(PF_EINVAL, "FluidSystem: %s must be finite, got %g", effect_name(effect), value);
+  *fld = value;
+  if (f->loop) fluid_write_params(f);
+  PF_API_END
+}
+
+int pf_fluid_get(pf_fluid* f, uint32_t effect, double* value) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  double* fld = fluid_field(f, effect);
+  if (!fld) fail(PF_EINVAL, "FluidSystem: unknown effect %u", effect);
+  if (!(f->present & effect))
+    fail(PF_EINVAL, "FluidSystem: pass `%s` in the options to enable it before changing it", effect_name(effect));
+  *value = *fld;
+  PF_API_END
+}
+
+int pf_fluid_add_boundary(pf_fluid* f, uint32_t start, uint32_t count, int dynamic) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (f->loop) fail(PF_EINVAL, "FluidSystem.addBoundary: add boundaries before creating the SimLoop");
+  range_ok(f->ps, start, count, "FluidSystem.addBoundary");
+  auto overlaps = [](uint32_t a0, uint32_t ac, uint32_t b0, uint32_t bc) { return a0 < b0 + bc && b0 < a0 + ac; };
+  if (overlaps(start, count, f->start, f->count)) fail(PF_EINVAL, "FluidSystem.addBoundary: a boundary cannot overlap the fluid");
+  for (auto& b : f->boundaries)
+    if (overlaps(start, count, b.start, b.count)) fail(PF_EINVAL, "FluidSystem.addBoundary: boundaries cannot overlap each other");
+  if (f->boundaries.size() >= PF_MAX_BOUNDARIES) fail(PF_EINVAL, "FluidSystem.addBoundary: at most %d boundaries", PF_MAX_BOUNDARIES);
+  f->boundaries.push_back({start, count, dynamic != 0});
+  PF_API_END
+}
+
+int pf_fluid_readback_density(pf_fluid* f, float* density) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (f->ps->loop) PF_HIP(hipStreamSynchronize(f->ps->ctx->stream));
+  std::vector<float> all(f->ps->n);
+  particles_gather(f->ps, 4 /* density */, all.data(), sizeof(float));
+  std::memcpy(density, all.data(), sizeof(float) * f->ps->n);
+  PF_API_END
+}
+
+int pf_fluid_readback_overflow(pf_fluid* f, int* overflow) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (!f->loop) fail(PF_EINVAL, "FluidSystem: add the fluid to a SimLoop first");
+  PF_HIP(hipStreamSynchronize(f->ps->ctx->stream));
+  uint32_t fl[2 * PF_NUM_FLAGS];
+  PF_HIP(hipMemcpy(fl, f->ps->host.flags, sizeof fl, hipMemcpyDeviceToHost));
+  *overflow = fl[f->loop->last_flag_half * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0;
+  PF_API_END
+}
+
+int pf_fluid_info(pf_fluid* f, double* mass, double* h, double* spacing, double* rho0) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (mass) *mass = f->mass;
+  if (h) *h = f->h;
+  if (spacing) *spacing = f->spacing;
+  if (rho0) *rho0 = f->rest_density;
+  PF_API_END
+}
+
+int pf_fluid_destroy(pf_fluid* f) {
+  PF_API_BEGIN
+  if (!f || f->disposed) return PF_OK;
+  f->disposed = true;
+  dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv); dfree(f->reaction);
+  f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr; f->reaction = nullptr;
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// PrimitiveSet
+// ---------------------------------------------------------------------------
+int pf_primitives_create(pf_particles* ps, uint32_t capacity, pf_primitives** out) {
+  PF_API_BEGIN
+  alive(ps, "PrimitiveSet");
+  auto* s = new pf_primitives_s();
+  s->ps = ps;
+  s->reserved = capacity;
+  *out = s;
+  PF_API_END
+}
+
+static void friction_defaults(double& mus, double& muk, const char* owner) {
+  if (mus < 0) mus = 0.5;
+  if (muk < 0) muk = 0.4;
+  if (!(mus >= 0) || !(muk >= 0) || !std::isfinite(mus) || !std::isfinite(muk))
+    fail(PF_EINVAL, "%s: friction coefficients must be non-negative (got muS=%g, muK=%g)", owner, mus, muk);
+}
+
+static uint32_t prim_add(pf_primitives* s, uint32_t kind, uint32_t flags, const double d0[4], const double d1[4],
+                         double mus, double muk, const double* rot) {
+  if (!s || s->disposed) fail(PF_ESTATE, "PrimitiveSet has been disposed");
+  const uint32_t cap = s->loop ? s->capacity : (s->reserved ? s->reserved : 0);
+  if (cap && s->prims.size() >= cap)
+    fail(PF_EINVAL, "PrimitiveSet: capacity %u is full. Pass a larger `capacity` to reserve slots for primitives added after the simulation starts.", cap);
+  if (s->prims.size() >= 32) fail(PF_EINVAL, "PrimitiveSet: at most 32 primitives per set");
+  friction_defaults(mus, muk, "PrimitiveSet");
+  pf_primitives_s::Prim p{};
+  p.kind = kind;
+  p.flags = flags;
+  for (int a = 0; a < 4; ++a) { p.d0[a] = d0[a]; p.d1[a] = d1[a]; }
+  p.d1[3] = mus;
+  p.vel[0] = p.vel[1] = p.vel[2] = 0;
+  p.vel[3] = muk;
+  p.spin[0] = p.spin[1] = p.spin[2] = 0;
+  p.rot[0] = p.rot[1] = p.rot[2] = 0;
+  p.rot[3] = 1;
+  if (rot) for (int a = 0; a < 4; ++a) p.rot[a] = rot[a];
+  s->prims.push_back(p);
+  s->dirty = true;
+  return (uint32_t)s->prims.size() - 1;
+}
+
+int pf_primitives_add_plane(pf_primitives* s, const double n[3], const double pt[3], double mus, double muk,
+                            uint32_t* slot) {
+  PF_API_BEGIN
+  const double l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (!(l2 > 0)) fail(PF_EINVAL, "PrimitiveSet.addPlane: normal must be non-zero");
+  // Vector3.normalize(): multiply by 1 / length.
+  const double inv = 1.0 / std::sqrt(l2);
+  const double d0[4] = {n[0] * inv, n[1] * inv, n[2] * inv, 0};
+  const double d1[4] = {pt[0], pt[1], pt[2], 0};
+  const uint32_t k = prim_add(s, PF_KIND_PLANE, 0, d0, d1, mus, muk, nullptr);
+  if (slot) *slot = k;
+  PF_API_END
+}
+
+int pf_primitives_add_sphere(pf_primitives* s, const double c[3], double r, int invert, double mus, double muk,
+                             uint32_t* slot) {
+  PF_API_BEGIN
+  if (!(r > 0) || !std::isfinite(r)) fail(PF_EINVAL, "PrimitiveSet.addSphere: radius must be positive, got %g", r);
+  const double d0[4] = {c[0], c[1], c[2], r};
+  const double d1[4] = {0, 0, 0, 0};
+  const uint32_t k = prim_add(s, PF_KIND_SPHERE, invert ? PF_PRIM_INVERT : 0u, d0, d1, mus, muk, nullptr);
+  if (slot) *slot = k;
+  PF_API_END
+}
+
+int pf_primitives_add_box(pf_primitives* s, const double c[3], const double he[3], const double rot[4], int invert,
+                          double mus, double muk, uint32_t* slot) {
+  PF_API_BEGIN
+  if (!(he[0] > 0 && he[1] > 0 && he[2] > 0)) fail(PF_EINVAL, "PrimitiveSet.addBox: halfExtents must be positive");
+  double q[4] = {0, 0, 0, 1};
+  if (rot) {
+    const double l = std::sqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2] + rot[3] * rot[3]);
+    if (!(l > 0) || !std::isfinite(l)) fail(PF_EINVAL, "PrimitiveSet.addBox: rotation must be a finite, non-zero quaternion");
+    for (int a = 0; a < 4; ++a) q[a] = rot[a] / l;
+  }
+  const double d0[4] = {c[0], c[1], c[2], 0};
+  const double d1[4] = {he[0], he[1], he[2], 0};
+  const uint32_t k = prim_add(s, PF_KIND_BOX, invert ? PF_PRIM_INVERT : 0u, d0, d1, mus, muk, q);
+  if (slot) *slot = k;
+  PF_API_END
+}
+
+int pf_primitives_add_capsule(pf_primitives* s, const double a[3], const double b[3], double r, double mus,
+                              double muk, uint32_t* slot) {
+  PF_API_BEGIN
+  if (!(r > 0) || !std::isfinite(r)) fail(PF_EINVAL, "PrimitiveSet.addCapsule: radius must be positive, got %g", r);
+  const double d0[4] = {a[0], a[1], a[2], r};
+  const double d1[4] = {b[0], b[1], b[2], 0};
+  const uint32_t k = prim_add(s, PF_KIND_CAPSULE, 0u, d0, d1, mus, muk, nullptr);
+  if (slot) *slot = k;
+  PF_API_END
+}
+
+static pf_primitives_s::Prim& prim_at(pf_primitives* s, uint32_t slot) {
+  if (!s || s->disposed) fail(PF_ESTATE, "PrimitiveSet has been disposed");
+  if (slot >= s->prims.size()) fail(PF_EINVAL, "PrimitiveSet: no primitive in slot %u", slot);
+  return s->prims[slot];
+}
+
+int pf_primitives_set_sphere(pf_primitives* s, uint32_t slot, const double c[3], double r, const double v[3]) {
+  PF_API_BEGIN
+  auto& p = prim_at(s, slot);
+  if (p.kind != PF_KIND_SPHERE) fail(PF_EINVAL, "PrimitiveSet.setSphere: slot %u is not a sphere", slot);
+  if (!(r > 0) || !std::isfinite(r)) fail(PF_EINVAL, "PrimitiveSet.setSphere: radius must be positive, got %g", r);
+  p.d0[0] = c[0]; p.d0[1] = c[1]; p.d0[2] = c[2]; p.d0[3] = r;
+  p.flags |= PF_PRIM_SWEEP;
+  p.vel[0] = v ? v[0] : 0; p.vel[1] = v ? v[1] : 0; p.vel[2] = v ? v[2] : 0;
+  s->dirty = true;
+  PF_API_END
+}
+
+// place() (PrimitiveSet.ts:469-486)
+static void prim_place(pf_primitives_s::Prim& p, const double pos[3], Q rot) {
+  const Q r = qmul(rot, Q{p.offset[0], p.offset[1], p.offset[2], p.offset[3]});
+  p.rot[0] = r.x; p.rot[1] = r.y; p.rot[2] = r.z; p.rot[3] = r.w;
+  double d[3];
+  qapply(r, p.direction, d);
+  if (p.kind == PF_KIND_PLANE) {
+    p.d0[0] = d[0]; p.d0[1] = d[1]; p.d0[2] = d[2];
+    p.d1[0] = pos[0]; p.d1[1] = pos[1]; p.d1[2] = pos[2];
+  } else if (p.kind == PF_KIND_CAPSULE) {
+    for (int a = 0; a < 3; ++a) { p.d0[a] = pos[a] - d[a]; p.d1[a] = pos[a] + d[a]; }
+  } else {
+    p.d0[0] = pos[0]; p.d0[1] = pos[1]; p.d0[2] = pos[2];
+  }
+}
+
+int pf_primitives_set_pose(pf_primitives* s, uint32_t slot, const double pos[3], const double rot[4], double dt) {
+  PF_API_BEGIN
+  auto& p = prim_at(s, slot);
+  const Q q = rot ? Q{rot[0], rot[1], rot[2], rot[3]} : Q{0, 0, 0, 1};
+  if (!p.has_pose) {
+    // attach() (PrimitiveSet.ts:211-234)
+    if (p.kind == PF_KIND_PLANE) { p.direction[0] = p.d0[0]; p.direction[1] = p.d0[1]; p.direction[2] = p.d0[2]; }
+    else if (p.kind == PF_KIND_CAPSULE) for (int a = 0; a < 3; ++a) p.direction[a] = (p.d1[a] - p.d0[a]) * 0.5;
+    else p.direction[0] = p.direction[1] = p.direction[2] = 0;
+    const Q off = p.kind == PF_KIND_BOX ? Q{0, 0, 0, 1} : qinv(q);
+    p.offset[0] = off.x; p.offset[1] = off.y; p.offset[2] = off.z; p.offset[3] = off.w;
+    p.flags |= PF_PRIM_SWEEP;
+    p.has_pose = true;
+  } else if (dt > 0) {
+    // update(dt) (PrimitiveSet.ts:259-285)
+    for (int a = 0; a < 3; ++a) p.vel[a] = (pos[a] - p.prev_pos[a]) / dt;
+    Q turn = qmul(q, qinv(Q{p.prev_rot[0], p.prev_rot[1], p.prev_rot[2], p.prev_rot[3]}));
+    if (turn.w < 0) turn = Q{-turn.x, -turn.y, -turn.z, -turn.w};
+    const double sn = std::hypot(turn.x, std::hypot(turn.y, turn.z));
+    const double rate = sn > 1e-12 ? (2 * std::atan2(sn, turn.w)) / (sn * dt) : 0;
+    p.spin[0] = turn.x * rate; p.spin[1] = turn.y * rate; p.spin[2] = turn.z * rate;
+  }
+  prim_place(p, pos, q);
+  for (int a = 0; a < 3; ++a) p.prev_pos[a] = pos[a];
+  p.prev_rot[0] = q.x; p.prev_rot[1] = q.y; p.prev_rot[2] = q.z; p.prev_rot[3] = q.w;
+  s->dirty = true;
+  PF_API_END
+}
+
+int pf_primitives_count(pf_primitives* s, uint32_t* count) {
+  PF_API_BEGIN
+  if (!s || s->disposed) fail(PF_ESTATE, "PrimitiveSet has been disposed");
+  *count = (uint32_t)s->prims.size();
+  PF_API_END
+}
+
+int pf_primitives_destroy(pf_primitives* s) {
+  PF_API_BEGIN
+  if (!s || s->disposed) return PF_OK;
+  s->disposed = true;
+  dfree(s->packed); dfree(s->d0); dfree(s->d1); dfree(s->vel); dfree(s->rot); dfree(s->spin);
+  s->packed = nullptr; s->d0 = s->d1 = s->vel = s->rot = s->spin = nullptr;
+  PF_API_END
+}
+
+}  // extern "C"
+
+void pf_primitives_s::upload() {
+  if (!dirty || !packed) return;
+  const uint32_t n = (uint32_t)prims.size();
+  std::vector<uint32_t> pk(capacity, 0);
+  std::vector<float4> a(capacity), b(capacity), v(capacity), r(capacity), sp(capacity);
+  for (uint32_t i = 0; i < n; ++i) {
+    const Prim& p = prims[i];
+    pk[i] = (p.kind << 16) | p.flags;
+    a[i] = make_float4((float)p.d0[0], (float)p.d0[1], (float)p.d0[2], (float)p.d0[3]);
+    b[i] = make_float4((float)p.d1[0], (float)p.d1[1], (float)p.d1[2], (float)p.d1[3]);
+    v[i] = make_float4((float)p.vel[0], (float)p.vel[1], (float)p.vel[2], (float)p.vel[3]);
+    r[i] = make_float4((float)p.rot[0], (float)p.rot[1], (float)p.rot[2], (float)p.rot[3]);
+    sp[i] = make_float4((float)p.spin[0], (float)p.spin[1], (float)p.spin[2], 0.f);
+  }
+  PF_HIP(hipStreamSynchronize(ps->ctx->stream));
+  PF_HIP(hipMemcpy(packed, pk.data(), capacity * 4, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(d0, a.data(), capacity * 16, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(d1, b.data(), capacity * 16, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(vel, v.data(), capacity * 16, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(rot, r.data(), capacity * 16, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(spin, sp.data(), capacity * 16, hipMemcpyHostToDevice));
+  if (ps->host.nprim != n) {
+    ps->host.nprim = n;
+    ps->dirty = true;
+  }
+  dirty = false;
+}
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// SoftbodySystem (global shape matching)
+// ---------------------------------------------------------------------------
+// Smallest eigenvalue of a symmetric 3x3 by Jacobi (SoftbodySystem.ts:424-486).
+static double smallest_eigenvalue(double a00, double a01, double a02, double a11, double a12, double a22) {
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    const double off = std::fabs(a01) + std::fabs(a02) + std::fabs(a12);
+    if (off < 1e-14) break;
+    const int pivot = (std::fabs(a01) >= std::fabs(a02) && std::fabs(a01) >= std::fabs(a12)) ? 0
+                      : (std::fabs(a02) >= std::fabs(a12) ? 1 : 2);
+    const double apq = pivot == 0 ? a01 : pivot == 1 ? a02 : a12;
+    const double app = pivot == 2 ? a11 : a00;
+    const double aqq = pivot == 0 ? a11 : a22;
+    if (std::fabs(apq) < 1e-20) break;
+    const double theta = (aqq - app) / (2 * apq);
+    const double sg = theta > 0 ? 1.0 : (theta < 0 ? -1.0 : 1.0);
+    const double t = sg / (std::fabs(theta) + std::sqrt(1 + theta * theta));
+    const double c = 1 / std::sqrt(1 + t * t);
+    const double s = t * c;
+    if (pivot == 0) {
+      const double n02 = c * a02 - s * a12, n12 = s * a02 + c * a12;
+      a00 = app - t * apq; a11 = aqq + t * apq; a01 = 0; a02 = n02; a12 = n12;
+    } else if (pivot == 1) {
+      const double n01 = c * a01 - s * a12, n12 = s * a01 + c * a12;
+      a00 = app - t * apq; a22 = aqq + t * apq; a02 = 0; a01 = n01; a12 = n12;
+    } else {
+      const double n01 = c * a01 - s * a02, n02 = s * a01 + c * a02;
+      a11 = app - t * apq; a22 = aqq + t * apq; a12 = 0; a01 = n01; a02 = n02;
+    }
+  }
+  return std::min(a00, std::min(a11, a22));
+}
+
+int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t nb, pf_softbody** out) {
+  PF_API_BEGIN
+  alive(ps, "SoftbodySystem");
+  if (nb == 0 || !defs) fail(PF_EINVAL, "SoftbodySystem: at least one body is required");
+  auto sb = std::make_unique<pf_softbody_s>();
+  sb->ps = ps;
+  const uint32_t n = ps->n;
+  sb->rest.assign(4 * (size_t)n, 0.0f);
+  sb->weights.assign(n, 0.0f);
+  std::vector<uint8_t> used(n, 0);
+  for (uint32_t b = 0; b < nb; ++b) {
+    const pf_softbody_def& d = defs[b];
+    if (d.count == 0) fail(PF_EINVAL, "SoftbodySystem: body %u has no particles", b);
+    char owner[64];
+    std::snprintf(owner, sizeof owner, "SoftbodySystem body %u", b);
+    range_ok(ps, d.start, d.count, owner);
+    for (uint32_t i = d.start; i < d.start + d.count; ++i) {
+      if (used[i]) fail(PF_EINVAL, "SoftbodySystem: particle %u belongs to two bodies", i);
+      used[i] = 1;
+    }
+    std::vector<float> restp(3 * (size_t)d.count);
+    for (uint32_t k = 0; k < 3 * d.count; ++k)
+      restp[k] = d.rest_positions ? d.rest_positions[k] : ps->cpu_positions[3 * (size_t)d.start + k];
+    const uint32_t surface = d.surface_count == 0xffffffffu ? d.count : d.surface_count;
+    if (surface > d.count) fail(PF_EINVAL, "SoftbodySystem: body %u surfaceCount %u is out of range", b, surface);
+    if (!(d.compliance >= 0) || !std::isfinite(d.compliance))
+      fail(PF_EINVAL, "SoftbodySystem: compliance must be ≥ 0, got %g", d.compliance);
+    // massWeights (SoftbodySystem.ts:366-372)
+    std::vector<double> masses(d.count);
+    std::vector<double> dyn;
+    for (uint32_t i = 0; i < d.count; ++i) {
+      const float w = ps->cpu_inv_mass[d.start + i];
+      masses[i] = w > 0 ? 1.0 / (double)w : 0.0;
+      if (masses[i] > 0) dyn.push_back(masses[i]);
+    }
+    bool equal = true;
+    for (double m : dyn) equal = equal && (m == dyn[0]);
+    std::vector<float> wts(d.count, 1.0f);
+    if (!equal) {
+      double sum = 0;
+      for (double m : dyn) sum += m;
+      const double mean = sum / (double)dyn.size();
+      for (uint32_t i = 0; i < d.count; ++i) wts[i] = (float)(masses[i] > 0 ? masses[i] / mean : 1.0);
+    }
+    // centroid (SoftbodySystem.ts:374-386)
+    double cx = 0, cy = 0, cz = 0, tot = 0;
+    for (uint32_t i = 0; i < d.count; ++i) {
+      cx += (double)restp[3 * i] * wts[i];
+      cy += (double)restp[3 * i + 1] * wts[i];
+      cz += (double)restp[3 * i + 2] * wts[i];
+      tot += wts[i];
+    }
+    cx /= tot; cy /= tot; cz /= tot;
+    // assertVolumetric (SoftbodySystem.ts:392-421)
+    double xx = 0, xy = 0, xz = 0, yy = 0, yz = 0, zz = 0;
+    for (uint32_t i = 0; i < d.count; ++i) {
+      const double x = restp[3 * i] - cx, y = restp[3 * i + 1] - cy, z = restp[3 * i + 2] - cz;
+      xx += x * x; xy += x * y; xz += x * z; yy += y * y; yz += y * z; zz += z * z;
+    }
+    const double smallest = std::sqrt(std::max(0.0, smallest_eigenvalue(xx, xy, xz, yy, yz, zz)));
+    if (smallest < ps->radius * 1e-3)
+      fail(PF_EINVAL, "SoftbodySystem: body %u is flat; shape matching needs a 3D particle cloud", b);
+    for (uint32_t i = 0; i < d.count; ++i) {
+      const size_t g = d.start + i;
+      sb->rest[4 * g + 0] = (float)(restp[3 * i] - cx);
+      sb->rest[4 * g + 1] = (float)(restp[3 * i + 1] - cy);
+      sb->rest[4 * g + 2] = (float)(restp[3 * i + 2] - cz);
+      sb->rest[4 * g + 3] = (float)d.compliance;
+      sb->weights[g] = wts[i];
+    }
+    sb->bodies.push_back({d.start, d.count, surface, d.compliance});
+  }
+  const uint32_t B = nb;
+  std::vector<uint32_t> bs(B), bc(B);
+  std::vector<float> cmp(B);
+  for (uint32_t b = 0; b < B; ++b) { bs[b] = sb->bodies[b].start; bc[b] = sb->bodies[b].count; cmp[b] = (float)sb->bodies[b].compliance; }
+  sb->d_start = dalloc<uint32_t>(B);
+  sb->d_count = dalloc<uint32_t>(B);
+  sb->d_compliance = dalloc<float>(B);
+  sb->d_rest = dalloc<float4>(n);
+  sb->d_weights = dalloc<float>(n);
+  sb->d_lambda = dalloc<float4>(n);
+  sb->d_centers = dalloc<float4>(B);
+  sb->d_rotations = dalloc<float4>(3 * (size_t)B);
+  PF_HIP(hipMemcpy(sb->d_start, bs.data(), B * 4, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(sb->d_count, bc.data(), B * 4, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(sb->d_compliance, cmp.data(), B * 4, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(sb->d_rest, sb->rest.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(sb->d_weights, sb->weights.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  // Identity rotation until the first step (SoftbodySystem.ts:147-151).
+  std::vector<float4> rot(3 * (size_t)B);
+  for (uint32_t b = 0; b < B; ++b) {
+    rot[3 * b] = make_float4(1, 0, 0, 0); rot[3 * b + 1] = make_float4(0, 1, 0, 0); rot[3 * b + 2] = make_float4(0, 0, 1, 0);
+  }
+  PF_HIP(hipMemcpy(sb->d_rotations, rot.data(), rot.size() * 16, hipMemcpyHostToDevice));
+  *out = sb.release();
+  PF_API_END
+}
+
+int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double compliance) {
+  PF_API_BEGIN
+  if (!sb || sb->disposed) fail(PF_ESTATE, "SoftbodySystem: disposed");
+  if (index >= sb->bodies.size()) fail(PF_EINVAL, "SoftbodySystem: no body %u (have %zu)", index, sb->bodies.size());
+  if (!(compliance >= 0) || !std::isfinite(compliance)) fail(PF_EINVAL, "SoftbodySystem: compliance must be ≥ 0, got %g", compliance);
+  sb->bodies[index].compliance = compliance;
+  const float c = (float)compliance;
+  PF_HIP(hipStreamSynchronize(sb->ps->ctx->stream));
+  PF_HIP(hipMemcpy(sb->d_compliance + index, &c, 4, hipMemcpyHostToDevice));
+  const auto& bd = sb->bodies[index];
+  for (uint32_t i = bd.start; i < bd.start + bd.count; ++i) sb->rest[4 * (size_t)i + 3] = c;
+  PF_HIP(hipMemcpy(sb->d_rest + bd.start, sb->rest.data() + 4 * (size_t)bd.start, (size_t)bd.count * 16, hipMemcpyHostToDevice));
+  PF_API_END
+}
+
+int pf_softbody_readback_frames(pf_softbody* sb, float* centers4, float* rotations12) {
+  PF_API_BEGIN
+  if (!sb || sb->disposed) fail(PF_ESTATE, "SoftbodySystem: disposed");
+  PF_HIP(hipStreamSynchronize(sb->ps->ctx->stream));
+  const size_t B = sb->bodies.size();
+  if (centers4) PF_HIP(hipMemcpy(centers4, sb->d_centers, B * 16, hipMemcpyDeviceToHost));
+  if (rotations12) PF_HIP(hipMemcpy(rotations12, sb->d_rotations, B * 48, hipMemcpyDeviceToHost));
+  PF_API_END
+}
+
+int pf_softbody_destroy(pf_softbody* sb) {
+  PF_API_BEGIN
+  if (!sb || sb->disposed) return PF_OK;
+  sb->disposed = true;
+  dfree(sb->d_start); dfree(sb->d_count); dfree(sb->d_compliance); dfree(sb->d_rest); dfree(sb->d_weights);
+  dfree(sb->d_lambda); dfree(sb->d_centers); dfree(sb->d_rotations);
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// SimLoop
+// ---------------------------------------------------------------------------
+static void fluid_write_params(pf_fluid* f) {
+  pf_particles* ps = f->ps;
+  FluidDev& F = ps->host.fluid[f->slot];
+  F.start = f->start;
+  F.count = f->count;
+  uint32_t fx = 0;
+  if (f->present & PF_FLUID_VISCOSITY) fx |= PF_FX_VISCOSITY;
+  if (f->present & PF_FLUID_VORTICITY) fx |= PF_FX_VORTICITY;
+  if (f->present & PF_FLUID_SURFACE_TENSION) fx |= PF_FX_SURFACE_TENSION;
+  if (f->present & PF_FLUID_ADHESION) fx |= PF_FX_ADHESION;
+  F.effects = fx;
+  F.rho0 = (float)f->rest_density;
+  F.pv = (float)(f->spacing * f->spacing * f->spacing);
+  F.mass = (float)f->mass;
+  const double h = f->h;
+  // kernels.ts:33-35: coefficients in double, uploaded as f32.
+  F.h = (float)h;
+  F.h2 = (float)(h * h);
+  F.poly6 = (float)(315.0 / (64.0 * M_PI * std::pow(h, 9)));
+  F.spiky = (float)(45.0 / (M_PI * std::pow(h, 6)));
+  F.compliance = (float)f->compliance;
+  F.gamma = (float)f->surface_tension;
+  F.visc = (float)f->viscosity;
+  F.vort = (float)f->vorticity;
+  F.adhesion = (float)f->adhesion;
+  // cohesion.ts:47, 53; adhesion.ts:17-24
+  F.coh_coef = (float)(32.0 / (M_PI * std::pow(h, 9)));
+  F.coh_h6_64 = (float)(std::pow(h, 6) / 64.0);
+  F.adh_coef = (float)(0.007 / std::pow(h, 3.25));
+  F.adh_m4_over_h = (float)(-4.0 / h);
+  F.adh_2h = (float)(2.0 * h);
+  // Accumulator(50) for impulses, Accumulator(10) for reactions (accumulator.ts:55-56).
+  const double s50 = std::floor(1073741824.0 / 50.0), s10 = std::floor(1073741824.0 / 10.0);
+  F.imp_scale = (float)s50;
+  F.imp_inv = (float)(1.0 / s50);
+  F.rea_scale = (float)s10;
+  F.rea_inv = (float)(1.0 / s10);
+  F.nboundary = (uint32_t)f->boundaries.size();
+  for (uint32_t b = 0; b < F.nboundary; ++b) {
+    F.bstart[b] = f->boundaries[b].start;
+    F.bcount[b] = f->boundaries[b].count;
+    if (f->boundaries[b].dynamic) F.bdynamic |= 1u << b;
+  }
+  F.nl = f->nl;
+  F.nl_count = f->nl_count;
+  F.normal = f->normal;
+  F.omega = f->omega;
+  F.dv = f->dv;
+  F.reaction = f->reaction;
+  ps->dirty = true;
+}
+
+static PlanShape loop_shape(const pf_loop* L) {
+  PlanShape sh;
+  sh.substeps = L->substeps;
+  sh.iterations = L->iterations;
+  sh.nfluids = (uint32_t)L->fluids.size();
+  for (uint32_t f = 0; f < sh.nfluids; ++f) sh.fx[f] = L->ps->host.fluid[f].effects;
+  sh.walls = L->prims && !L->prims->prims.empty();
+  sh.soft = L->soft != nullptr;
+  sh.any_boundary = L->ps->host.has_boundary != 0;
+  return sh;
+}
+
+static void loop_set_dt(pf_loop* L, double dt) {
+  SceneDev& S = L->ps->host;
+  const float dtf = (float)(dt / (double)L->substeps);
+  // Sweep clock per substep: frameStart sets dt*(substeps-1), every substep
+  // end subtracts dt, floored at 0 (PrimitiveSet.ts:393-398).
+  float clock = dtf * ((float)L->substeps - 1.0f);
+  float tab[PF_MAX_SUBSTEPS];
+  for (uint32_t k = 0; k < L->substeps; ++k) {
+    tab[k] = clock;
+    clock = std::max(clock - dtf, 0.0f);
+  }
+  if (S.dt != dtf || std::memcmp(S.clock, tab, sizeof(float) * L->substeps) != 0) {
+    S.dt = dtf;
+    std::memcpy(S.clock, tab, sizeof(float) * L->substeps);
+    L->ps->dirty = true;
+  }
+}
+
+int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_material* mats, uint32_t nmats,
+                   pf_primitives* const* colliders, uint32_t ncol, pf_loop** out) {
+  PF_API_BEGIN
+  alive(ps, "SimLoop");
+  if (ps->loop) fail(PF_EINVAL, "SimLoop: the ParticleSystem already has a SimLoop");
+  const uint32_t substeps = opt && opt->substeps ? opt->substeps : 4;
+  const uint32_t iterations = opt && opt->iterations != 0xffffffffu ? opt->iterations : 2;
+  if (substeps < 1 || substeps > PF_MAX_SUBSTEPS) fail(PF_EINVAL, "SimLoop: substeps must be an integer ≥ 1 (at most %d here), got %u", PF_MAX_SUBSTEPS, substeps);
+  auto L = std::make_unique<pf_loop_s>();
+  L->ps = ps;
+  L->ctx = ps->ctx;
+  L->substeps = substeps;
+  L->iterations = iterations;
+  if (opt) for (int a = 0; a < 3; ++a) L->gravity[a] = opt->gravity[a];
+  double neighbor_radius = 0;
+  for (uint32_t m = 0; m < nmats; ++m) {
+    if (mats[m].kind == PF_MATERIAL_FLUID) {
+      auto* f = static_cast<pf_fluid*>(mats[m].handle);
+      if (!f || f->disposed) fail(PF_EINVAL, "SimLoop: a material has been disposed");
+      if (f->ps != ps) fail(PF_EINVAL, "SimLoop: every material must be built for the same ParticleSystem");
+      if (f->loop) fail(PF_EINVAL, "SimLoop: a fluid belongs to another SimLoop");
+      if (L->soft) fail(PF_EINVAL, "SimLoop: list fluids before the soft-body system");
+      if (L->fluids.size() >= PF_MAX_FLUIDS) fail(PF_EINVAL, "SimLoop: at most %d fluids per loop", PF_MAX_FLUIDS);
+      if (!f->boundaries.empty())
+        fail(PF_EINVAL, "FluidSystem.addBoundary: fluid boundaries are not supported by this build");
+      L->fluids.push_back(f);
+      neighbor_radius = std::max(neighbor_radius, f->h);
+    } else if (mats[m].kind == PF_MATERIAL_SOFTBODY) {
+      auto* sb = static_cast<pf_softbody*>(mats[m].handle);
+      if (!sb || sb->disposed) fail(PF_EINVAL, "SimLoop: a material has been disposed");
+      if (sb->ps != ps) fail(PF_EINVAL, "SimLoop: every material must be built for the same ParticleSystem");
+      if (L->soft) fail(PF_EINVAL, "SimLoop: one SoftbodySystem per loop");
+      L->soft = sb;
+    } else {
+      fail(PF_EINVAL, "SimLoop: unknown material kind %u", mats[m].kind);
+    }
+  }
+  if (ncol > 1) fail(PF_EINVAL, "SimLoop: one PrimitiveSet per loop in this build");
+  for (uint32_t k = 0; k < ncol; ++k) {
+    pf_primitives* s = colliders[k];
+    if (!s || s->disposed) fail(PF_EINVAL, "SimLoop: a collider has been disposed");
+    if (s->ps != ps) fail(PF_EINVAL, "SimLoop: every collider must be built for the same ParticleSystem");
+    if (s->loop) fail(PF_EINVAL, "SimLoop: the collider belongs to another SimLoop");
+    L->prims = s;
+  }
+  SceneDev& S = ps->host;
+  const uint32_t n = ps->n;
+  S.gx = (float)L->gravity[0]; S.gy = (float)L->gravity[1]; S.gz = (float)L->gravity[2];
+  S.radius = (float)ps->radius;
+  // Grid sized for the widest neighbour query (loop.ts:151-162).
+  if (neighbor_radius > 0) {
+    pf_grid_s& g = L->grid;
+    g.ctx = ps->ctx;
+    g.ps = ps;
+    g.owned_by_loop = true;
+    g.n = n;
+    g.H = std::min<uint32_t>([&] { uint64_t p = 1; while (p < 2ull * n) p <<= 1; return (uint32_t)p; }(), 1u << 20);
+    g.Hpad = (g.H + 1023) / 1024 * 1024;
+    g.cell = neighbor_radius;
+    if (opt && opt->has_hash_origin) for (int a = 0; a < 3; ++a) g.origin[a] = opt->hash_origin[a];
+    S.H = g.H;
+    S.hash_mask = g.H - 1;
+    S.nscan_tiles = (g.H + PF_SCAN_TILE - 1) / PF_SCAN_TILE;
+    S.cell = (float)neighbor_radius;
+    S.ox = (float)g.origin[0]; S.oy = (float)g.origin[1]; S.oz = (float)g.origin[2];
+    S.key = dalloc<uint32_t>(n);
+    S.rank = dalloc<uint32_t>(n);
+    S.counts = dalloc<uint32_t>(g.H);
+    S.start = dalloc<uint32_t>((size_t)g.H + 1);
+    S.scan_status = dalloc<unsigned long long>(S.nscan_tiles);
+    S.scan_ctl = dalloc<uint32_t>(2);
+    S.tmp_src = dalloc<uint32_t>(n);
+    S.tmp_id = dalloc<uint32_t>(n);
+    S.tmp_key = dalloc<uint32_t>(n);
+    g.host = S;
+  }
+  S.nfluids = (uint32_t)L->fluids.size();
+  for (uint32_t k = 0; k < S.nfluids; ++k) {
+    pf_fluid* f = L->fluids[k];
+    f->slot = (int)k;
+    f->loop = L.get();
+    f->nl = dalloc<uint32_t>((size_t)PF_MAX_NEIGHBORS * n);
+    f->nl_count = dalloc<uint32_t>(n);
+    if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
+    if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
+    if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
+    fluid_write_params(f);
+  }
+  const double s10 = std::floor(1073741824.0 / 10.0), s50 = std::floor(1073741824.0 / 50.0);
+  S.pos_scale = (float)s10;
+  S.pos_inv = (float)(1.0 / s10);
+  S.vel_scale = (float)s50;
+  S.vel_inv = (float)(1.0 / s50);
+  if (L->prims) {
+    pf_primitives* s = L->prims;
+    s->capacity = std::max<uint32_t>(1, s->reserved ? s->reserved : (uint32_t)s->prims.size());
+    if (s->capacity > 32) fail(PF_EINVAL, "PrimitiveSet: at most 32 primitives per set");
+    s->packed = dalloc<uint32_t>(s->capacity);
+    s->d0 = dalloc<float4>(s->capacity);
+    s->d1 = dalloc<float4>(s->capacity);
+    s->vel = dalloc<float4>(s->capacity);
+    s->rot = dalloc<float4>(s->capacity);
+    s->spin = dalloc<float4>(s->capacity);
+    s->loop = L.get();
+    S.prim_cap = s->capacity;
+    S.prim_packed = s->packed; S.prim_d0 = s->d0; S.prim_d1 = s->d1; S.prim_vel = s->vel; S.prim_rot = s->rot; S.prim_spin = s->spin;
+    S.prim_lambda = dalloc<float>(2 * (size_t)s->capacity * n);
+    S.prim_touch = dalloc<uint32_t>(n);
+    s->dirty = true;
+    s->upload();
+  }
+  if (L->soft) {
+    pf_softbody* sb = L->soft;
+    sb->loop = L.get();
+    SoftDev& B = S.soft;
+    B.nbodies = (uint32_t)sb->bodies.size();
+    B.body_start = sb->d_start; B.body_count = sb->d_count; B.body_compliance = sb->d_compliance;
+    B.rest = sb->d_rest; B.weights = sb->d_weights; B.lambda = sb->d_lambda;
+    B.centers = sb->d_centers; B.rotations = sb->d_rotations;
+  }
+  // Work tiles.
+  L->ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
+  std::vector<Tile> tl(L->ntiles);
+  for (uint32_t i = 0; i < L->ntiles; ++i) tl[i] = Tile{0u, i * PF_BLOCK};
+  L->tiles = dalloc<Tile>(L->ntiles);
+  PF_HIP(hipMemcpy(L->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  L->nstiles = S.nscan_tiles;
+  if (L->nstiles) {
+    std::vector<uint2> stl(L->nstiles);
+    for (uint32_t i = 0; i < L->nstiles; ++i) stl[i] = make_uint2(0u, i);
+    L->stiles = dalloc<uint2>(L->nstiles);
+    PF_HIP(hipMemcpy(L->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  L->nbtiles = S.soft.nbodies;
+  if (L->nbtiles) {
+    std::vector<uint2> btl(L->nbtiles);
+    for (uint32_t i = 0; i < L->nbtiles; ++i) btl[i] = make_uint2(0u, i);
+    L->btiles = dalloc<uint2>(L->nbtiles);
+    PF_HIP(hipMemcpy(L->btiles, btl.data(), btl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  ps->loop = L.get();
+  ps->dirty = true;
+  ps->upload_desc();
+  *out = L.release();
+  PF_API_END
+}
+
+static void loop_alive(pf_loop* L, const char* what) {
+  if (!L) fail(PF_EINVAL, "%s: null handle", what);
+  if (L->disposed) fail(PF_ESTATE, "SimLoop: the loop has been disposed");
+  if (L->ps->disposed) fail(PF_ESTATE, "SimLoop: the ParticleSystem has been disposed");
+}
+
+static void loop_invalidate_graphs(pf_loop* L) {
+  for (int v = 0; v < 4; ++v) {
+    if (L->graph[v]) (void)hipGraphExecDestroy(L->graph[v]);
+    L->graph[v] = nullptr;
+    L->graph_valid[v] = false;
+  }
+}
+
+int pf_loop_step(pf_loop* L, double dt) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.step");
+  if (!(dt > 0) || !std::isfinite(dt)) fail(PF_EINVAL, "SimLoop: step dt must be a positive finite number, got %g", dt);
+  pf_particles* ps = L->ps;
+  Context* ctx = L->ctx;
+  if (L->prims) {
+    const uint32_t before = ps->host.nprim;
+    L->prims->upload();
+    if (ps->host.nprim != before) loop_invalidate_graphs(L);
+  }
+  loop_set_dt(L, dt);
+  ps->upload_desc();
+  const PlanShape sh = loop_shape(L);
+  const int v = ps->variant;
+  if (L->use_graph) {
+    if (!L->graph_valid[v]) {
+      hipGraph_t g;
+      PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      ctx->capturing = true;
+      try {
+        L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles,
+                                           L->btiles, L->nbtiles, ctx->scan_ctl);
+      } catch (...) {
+        ctx->capturing = false;
+        hipGraph_t dummy;
+        (void)hipStreamEndCapture(ctx->stream, &dummy);
+        throw;
+      }
+      ctx->capturing = false;
+      PF_HIP(hipStreamEndCapture(ctx->stream, &g));
+      PF_HIP(hipGraphInstantiate(&L->graph[v], g, nullptr, nullptr, 0));
+      PF_HIP(hipGraphDestroy(g));
+      L->graph_valid[v] = true;
+    }
+    PF_HIP(hipGraphLaunch(L->graph[v], ctx->stream));
+  } else {
+    L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles, L->btiles,
+                                       L->nbtiles, ctx->scan_ctl);
+  }
+  PF_HIP(hipGetLastError());
+  ps->variant ^= step_variant_flips(sh);
+  ps->stepped = true;
+  L->last_flag_half = (L->substeps - 1) & 1u;
+  L->steps += 1;
+  PF_API_END
+}
+
+int pf_loop_synchronize(pf_loop* L) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop");
+  PF_HIP(hipStreamSynchronize(L->ctx->stream));
+  PF_API_END
+}
+
+int pf_loop_readback_overflow(pf_loop* L, pf_overflow* o) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.readbackOverflow");
+  PF_HIP(hipStreamSynchronize(L->ctx->stream));
+  uint32_t fl[2 * PF_NUM_FLAGS];
+  PF_HIP(hipMemcpy(fl, L->ps->host.flags, sizeof fl, hipMemcpyDeviceToHost));
+  const uint32_t* last = fl + L->last_flag_half * PF_NUM_FLAGS;
+  std::memset(o, 0, sizeof *o);
+  o->positions = fl[PF_FLAG_POSITIONS] != 0;
+  o->velocities = fl[PF_FLAG_VELOCITIES] != 0;
+  o->grid = last[PF_FLAG_GRID] != 0;
+  o->contacts = 0;
+  o->neighbors = last[PF_FLAG_NEIGHBORS] != 0;
+  o->impulses = fl[PF_FLAG_IMPULSES] != 0;
+  o->long_bucket = last[PF_FLAG_LONG_BUCKET] != 0;
+  PF_API_END
+}
+
+int pf_loop_set_substeps(pf_loop* L, uint32_t substeps) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.substeps");
+  if (substeps < 1 || substeps > PF_MAX_SUBSTEPS) fail(PF_EINVAL, "SimLoop: substeps must be an integer ≥ 1, got %u", substeps);
+  if (substeps != L->substeps) {
+    L->substeps = substeps;
+    L->last_dt = -1;
+    loop_invalidate_graphs(L);
+  }
+  PF_API_END
+}
+
+int pf_loop_set_gravity(pf_loop* L, const double g[3]) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.gravity");
+  for (int a = 0; a < 3; ++a) L->gravity[a] = g[a];
+  SceneDev& S = L->ps->host;
+  const float gx = (float)g[0], gy = (float)g[1], gz = (float)g[2];
+  if (S.gx != gx || S.gy != gy || S.gz != gz) {
+    S.gx = gx; S.gy = gy; S.gz = gz;
+    L->ps->dirty = true;
+  }
+  PF_API_END
+}
+
+int pf_loop_info(pf_loop* L, uint32_t* substeps, uint32_t* iterations, double* cell, uint32_t* H,
+                 uint32_t* kernels_per_step) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop");
+  if (substeps) *substeps = L->substeps;
+  if (iterations) *iterations = L->iterations;
+  if (cell) *cell = L->grid.cell;
+  if (H) *H = L->grid.H;
+  if (kernels_per_step) *kernels_per_step = L->kernels_per_step;
+  PF_API_END
+}
+
+int pf_loop_grid(pf_loop* L, pf_grid** out) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.hashGrid");
+  if (!L->grid.owned_by_loop) fail(PF_EINVAL, "SimLoop: no material needs a neighbour grid");
+  L->grid.host = L->ps->host;
+  *out = &L->grid;
+  PF_API_END
+}
+
+int pf_loop_set_graph(pf_loop* L, int enabled) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop");
+  if ((enabled != 0) != L->use_graph) loop_invalidate_graphs(L);
+  L->use_graph = enabled != 0;
+  PF_API_END
+}
+
+int pf_loop_destroy(pf_loop* L) {
+  PF_API_BEGIN
+  if (!L || L->disposed) return PF_OK;
+  (void)hipStreamSynchronize(L->ctx->stream);
+  L->disposed = true;
+  loop_invalidate_graphs(L);
+  SceneDev& S = L->ps->host;
+  dfree(S.key); dfree(S.rank); dfree(S.counts); dfree(S.start); dfree(S.scan_status); dfree(S.scan_ctl);
+  dfree(S.tmp_src); dfree(S.tmp_id); dfree(S.tmp_key); dfree(S.prim_lambda); dfree(S.prim_touch);
+  S.key = S.rank = S.counts = S.start = S.tmp_src = S.tmp_id = S.tmp_key = nullptr;
+  S.scan_status = nullptr; S.scan_ctl = nullptr; S.prim_lambda = nullptr; S.prim_touch = nullptr;
+  dfree(L->tiles); dfree(L->stiles); dfree(L->btiles);
+  for (auto* f : L->fluids) {
+    f->loop = nullptr;
+    dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv);
+    f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr;
+  }
+  if (L->prims) L->prims->loop = nullptr;
+  if (L->soft) L->soft->loop = nullptr;
+  L->ps->loop = nullptr;
+  L->ps->dirty = true;
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// Batched form
+// ---------------------------------------------------------------------------
+int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch** out) {
+  PF_API_BEGIN
+  if (!ctx || !loops || n == 0) fail(PF_EINVAL, "Batch: at least one loop is required");
+  auto B = std::make_unique<pf_batch_s>();
+  B->ctx = ctx;
+  PlanShape sh;
+  sh.substeps = loops[0]->substeps;
+  sh.iterations = loops[0]->iterations;
+  std::vector<Tile> tl;
+  std::vector<uint2> stl, btl;
+  for (uint32_t s = 0; s < n; ++s) {
+    pf_loop* L = loops[s];
+    loop_alive(L, "Batch");
+    if (L->ctx != ctx) fail(PF_EINVAL, "Batch: every loop must use the batch's context");
+    if (L->substeps != sh.substeps || L->iterations != sh.iterations)
+      fail(PF_EINVAL, "Batch: every loop must share substeps and iterations");
+    for (uint32_t t = 0; t < s; ++t)
+      if (loops[t] == L) fail(PF_EINVAL, "Batch: a loop appears twice");
+    const PlanShape ls = loop_shape(L);
+    sh.nfluids = std::max(sh.nfluids, ls.nfluids);
+    for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) sh.fx[f] |= ls.fx[f];
+    sh.walls = sh.walls || ls.walls;
+    sh.soft = sh.soft || ls.soft;
+    sh.any_boundary = sh.any_boundary || ls.any_boundary;
+    for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{s, i * PF_BLOCK});
+    for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(s, i));
+    for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(s, i));
+    B->loops.push_back(L);
+  }
+  if (sh.nfluids > 0)
+    for (auto* L : B->loops)
+      if (L->fluids.empty()) fail(PF_EINVAL, "Batch: mixing scenes with and without fluid is not supported");
+  B->shape = sh;
+  B->ntiles = (uint32_t)tl.size();
+  B->nstiles = (uint32_t)stl.size();
+  B->nbtiles = (uint32_t)btl.size();
+  B->tiles = dalloc<Tile>(B->ntiles);
+  PF_HIP(hipMemcpy(B->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  if (B->nstiles) {
+    B->stiles = dalloc<uint2>(B->nstiles);
+    PF_HIP(hipMemcpy(B->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  if (B->nbtiles) {
+    B->btiles = dalloc<uint2>(B->nbtiles);
+    PF_HIP(hipMemcpy(B->btiles, btl.data(), btl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  B->dev = dalloc<SceneDev>(n);
+  B->scan_ctl = dalloc<uint32_t>(4);
+  *out = B.release();
+  PF_API_END
+}
+
+int pf_batch_step(pf_batch* B, double dt) {
+  PF_API_BEGIN
+  if (!B) fail(PF_EINVAL, "Batch: null handle");
+  if (!(dt > 0) || !std::isfinite(dt)) fail(PF_EINVAL, "Batch: step dt must be a positive finite number, got %g", dt);
+  Context* ctx = B->ctx;
+  bool changed = false;
+  for (auto* L : B->loops) {
+    loop_alive(L, "Batch");
+    if (L->prims) L->prims->upload();
+    loop_set_dt(L, dt);
+    changed = changed || L->ps->dirty;
+  }
+  if (changed || B->desc_dirty) {
+    PF_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<SceneDev> descs;
+    for (auto* L : B->loops) {
+      L->ps->upload_desc();
+      descs.push_back(L->ps->desc(L->ps->variant));
+    }
+    PF_HIP(hipMemcpy(B->dev, descs.data(), descs.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
+    B->desc_dirty = false;
+  }
+  if (B->use_graph) {
+    if (!B->graph_valid) {
+      hipGraph_t g;
+      PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      ctx->capturing = true;
+      try {
+        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->scan_ctl);
+      } catch (...) {
+        ctx->capturing = false;
+        hipGraph_t dummy;
+        (void)hipStreamEndCapture(ctx->stream, &dummy);
+        throw;
+      }
+      ctx->capturing = false;
+      PF_HIP(hipStreamEndCapture(ctx->stream, &g));
+      PF_HIP(hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0));
+      PF_HIP(hipGraphDestroy(g));
+      B->graph_valid = true;
+    }
+    PF_HIP(hipGraphLaunch(B->graph, ctx->stream));
+  } else {
+    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->scan_ctl);
+  }
+  PF_HIP(hipGetLastError());
+  const int flips = step_variant_flips(B->shape);
+  for (auto* L : B->loops) {
+    L->ps->variant ^= flips;
+    L->ps->stepped = true;
+    L->last_flag_half = (L->substeps - 1) & 1u;
+    L->steps += 1;
+  }
+  // The graph reads the descriptor array at run time; swapped buffer pairs
+  // only need the array re-uploaded before the next step.
+  if (flips) B->desc_dirty = true;
+  PF_API_END
+}
+
+int pf_batch_set_graph(pf_batch* B, int enabled) {
+  PF_API_BEGIN
+  if (!B) fail(PF_EINVAL, "Batch: null handle");
+  B->use_graph = enabled != 0;
+  if (B->graph) { (void)hipGraphExecDestroy(B->graph); B->graph = nullptr; }
+  B->graph_valid = false;
+  PF_API_END
+}
+
+int pf_batch_destroy(pf_batch* B) {
+  PF_API_BEGIN
+  if (!B) return PF_OK;
+  (void)hipStreamSynchronize(B->ctx->stream);
+  if (B->graph) (void)hipGraphExecDestroy(B->graph);
+  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->scan_ctl);
+  delete B;
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// Scene layout (Simulation.ts:957-1104, layout.ts:7-42)
+// ---------------------------------------------------------------------------
+static uint32_t fill_box(const double mn[3], const double mx[3], double sp, float* out, uint32_t cap) {
+  uint32_t k = 0;
+  if (!(mn[0] <= mx[0] && mn[1] <= mx[1] && mn[2] <= mx[2])) return 0;  // empty box
+  for (double y = mn[1] + sp / 2; y < mx[1]; y += sp)
+    for (double z = mn[2] + sp / 2; z < mx[2]; z += sp)
+      for (double x = mn[0] + sp / 2; x < mx[0]; x += sp) {
+        if (out && k < cap) {
+          out[3 * (size_t)k] = (float)x;
+          out[3 * (size_t)k + 1] = (float)y;
+          out[3 * (size_t)k + 2] = (float)z;
+        }
+        ++k;
+      }
+  return k;
+}
+
+int pf_layout_fill_box(const double mn[3], const double mx[3], double spacing, float* out, uint32_t cap,
+                       uint32_t* count) {
+  PF_API_BEGIN
+  if (!(spacing > 0)) fail(PF_EINVAL, "fillBox: spacing must be positive, got %g", spacing);
+  const uint32_t k = fill_box(mn, mx, spacing, out, cap);
+  if (count) *count = k;
+  PF_API_END
+}
+
+double pf_layout_radius_for_budget(double volume, double area, double cloth_spacing, uint32_t budget) {
+  auto count = [&](double r) { return volume / std::pow(2 * r, 3) + area / std::pow(cloth_spacing * r, 2); };
+  double low = 1e-5, high = 10;
+  for (int i = 0; i < 80; ++i) {
+    const double mid = std::sqrt(low * high);
+    if (count(mid) > budget) low = mid;
+    else high = mid;
+  }
+  return high;
+}
+
+// Box3.intersect and expandByScalar(-r), then fillBox at spacing 2r.
+static uint32_t layout_count(const double* container, const double* boxes, uint32_t nb, double r) {
+  uint32_t total = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    double mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = boxes[6 * b + a];
+      mx[a] = boxes[6 * b + 3 + a];
+      if (container) {
+        mn[a] = std::max(mn[a], container[a] + r);
+        mx[a] = std::min(mx[a], container[3 + a] - r);
+      }
+    }
+    total += fill_box(mn, mx, 2 * r, nullptr, 0);
+  }
+  return total;
+}
+
+int pf_layout_fit_budget(uint32_t budget, const double container[6], const double* boxes, uint32_t nb,
+                         double* radius, uint32_t* count) {
+  PF_API_BEGIN
+  if (budget == 0) fail(PF_EINVAL, "Simulation: particles must be a positive integer, got %u", budget);
+  if (!boxes || nb == 0) fail(PF_EINVAL, "Simulation: add a fluid, smoke, soft body, or cloth before stepping");
+  double volume = 0;
+  for (uint32_t b = 0; b < nb; ++b) {
+    double sz[3];
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) {
+      double lo = boxes[6 * b + a], hi = boxes[6 * b + 3 + a];
+      if (container) { lo = std::max(lo, container[a]); hi = std::min(hi, container[3 + a]); }
+      sz[a] = hi - lo;
+      if (hi < lo) empty = true;
+    }
+    if (!empty) volume += sz[0] * sz[1] * sz[2];
+  }
+  const double guess = pf_layout_radius_for_budget(volume, 0, 2.2, budget);
+  double r = guess;
+  bool have = false;
+  double best_r = 0;
+  uint32_t best_n = 0;
+  for (int pass = 0; pass < 12; ++pass) {
+    const uint32_t c = layout_count(container, boxes, nb, r);
+    if (c <= budget && (!have || c > best_n)) { have = true; best_r = r; best_n = c; }
+    if (c <= budget && c >= 0.97 * budget) break;
+    r *= std::cbrt(std::max<double>(c, 1) / (0.985 * budget));
+  }
+  for (int pass = 0; !have && pass < 100; ++pass) {
+    r *= 1.05;
+    const uint32_t c = layout_count(container, boxes, nb, r);
+    if (c <= budget) { have = true; best_r = r; best_n = c; }
+  }
+  if (!have) fail(PF_EINVAL, "Simulation: the scene can't fit in %u particles", budget);
+  if (radius) *radius = best_r;
+  if (count) *count = best_n;
+  PF_API_END
+}
+
+}  // extern "C"

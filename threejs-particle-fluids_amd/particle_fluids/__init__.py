@@ -1,0 +1,551 @@
+"""Python host mirror of threejs-particle-fluids' solver API over libpf.so.
+
+Class, method and option names follow the reference (src/core/*,
+src/fluids/FluidSystem.ts, src/softbody/SoftbodySystem.ts,
+src/simulation/Simulation.ts) so code written against it reads the same;
+errors carry the reference's messages. Every call goes to the HIP library:
+there is no CPU fallback.
+"""
+import ctypes as C
+import math
+from collections import namedtuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import PFError, call, load
+
+__all__ = [
+    "PFError", "Context", "ParticleSystem", "HashGrid", "FluidSystem", "PrimitiveSet",
+    "SoftbodySystem", "SimLoop", "Batch", "Simulation", "fill_box", "fit_budget",
+    "radius_for_budget",
+]
+
+P = C.POINTER
+f32p = P(C.c_float)
+u32p = P(C.c_uint32)
+f64p = P(C.c_double)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def _d4(v):
+    return (C.c_double * 4)(*[float(x) for x in v])
+
+
+def _range(r, default_count):
+    if r is None:
+        return 0, default_count
+    if isinstance(r, dict):
+        return int(r["start"]), int(r["count"])
+    return int(r[0]), int(r[1])
+
+
+class Context:
+    """One GPU and one HIP stream (replaces the WebGPU renderer handle)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        call("pf_context_create", int(device), C.byref(h))
+        self.handle = h
+
+    def synchronize(self):
+        call("pf_context_synchronize", self.handle)
+
+    def timer_begin(self):
+        call("pf_context_timer_begin", self.handle)
+
+    def timer_end(self):
+        ms = C.c_float()
+        call("pf_context_timer_end", self.handle, C.byref(ms))
+        return ms.value
+
+    def set_profiling(self, on):
+        call("pf_context_set_profiling", self.handle, 1 if on else 0)
+
+    def profile(self):
+        """{kernel name: (launches, total ms)} since the last reset."""
+        n = C.c_uint32()
+        call("pf_context_profile_count", self.handle, C.byref(n))
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n.value):
+            launches = C.c_uint32()
+            total = C.c_double()
+            call("pf_context_profile_entry", self.handle, i, buf, 128, C.byref(launches), C.byref(total))
+            out[buf.value.decode()] = (launches.value, total.value)
+        return out
+
+    def profile_reset(self):
+        call("pf_context_profile_reset", self.handle)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_context_destroy", self.handle)
+            self.handle = None
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+ParticleSnapshot = namedtuple(
+    "ParticleSnapshot",
+    "capacity positions predictedPositions velocities invMass rotation predictedRotation angularVelocity",
+)
+HashGridSnapshot = namedtuple(
+    "HashGridSnapshot",
+    "capacity hashTableSize hashTableSizePadded cellIndex counts cellStart cellEnd sortedIndices",
+)
+
+
+class ParticleSystem:
+    """src/core/particles.ts:52-258. Positions are (N, 3) arrays."""
+
+    def __init__(self, capacity, particleRadius, context=None):
+        self.context = context or default_context()
+        h = C.c_void_p()
+        call("pf_particles_create", self.context.handle, int(capacity), float(particleRadius), C.byref(h))
+        self.handle = h
+        self.capacity = int(capacity)
+        self.particleRadius = float(particleRadius)
+
+    def uploadParticles(self, positions, velocities=None, invMass=None, collisionGroup=None, start=0):
+        pos = _f32(positions).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if velocities is None else _f32(velocities).reshape(-1, 3)
+        im = None if invMass is None else _f32(np.broadcast_to(invMass, (n,)))
+        grp = None if collisionGroup is None else np.ascontiguousarray(np.broadcast_to(collisionGroup, (n,)), dtype=np.uint32)
+        call("pf_particles_upload", self.handle, int(start), n, _ptr(pos, f32p), _ptr(vel, f32p), _ptr(im, f32p), _ptr(grp, u32p))
+
+    def setInvMass(self, range, invMass):
+        s, c = _range(range, self.capacity)
+        call("pf_particles_set_inv_mass", self.handle, s, c, float(invMass))
+
+    def setCollisionGroup(self, range, group):
+        s, c = _range(range, self.capacity)
+        call("pf_particles_set_collision_group", self.handle, s, c, int(group))
+
+    def readback(self):
+        n = self.capacity
+        arrs = [np.zeros((n, 4), np.float32) for _ in range(3)] + [np.zeros(n, np.float32)] + [np.zeros((n, 4), np.float32) for _ in range(3)]
+        snap = _lib.Snapshot(*[_ptr(a, f32p) for a in arrs])
+        call("pf_particles_readback", self.handle, C.byref(snap))
+        return ParticleSnapshot(n, *arrs)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_particles_destroy", self.handle)
+            self.handle = None
+
+    @property
+    def disposed(self):
+        return self.handle is None
+
+
+class HashGrid:
+    """src/core/hashGrid/HashGrid.ts:80-286.
+
+    ``HashGrid(particles, cellSize, ...)`` bins a ParticleSystem's predicted
+    positions; ``HashGrid.standalone(capacity, cellSize, ...)`` sorts points
+    uploaded with :meth:`uploadPositions`.
+    """
+
+    def __init__(self, particles, cellSize, hashTableSize=None, hashOrigin=None, _handle=None, _owner=None):
+        self.particles = particles
+        self._owned = _owner is not None
+        self._owner = _owner
+        if _handle is not None:
+            self.handle = _handle
+        else:
+            h = C.c_void_p()
+            origin = None if hashOrigin is None else _d3(hashOrigin)
+            call("pf_grid_create", particles.handle, float(cellSize), int(hashTableSize or 0), origin, C.byref(h))
+            self.handle = h
+        self._info()
+
+    @classmethod
+    def standalone(cls, capacity, cellSize, hashTableSize=None, hashOrigin=None, context=None):
+        ctx = context or default_context()
+        h = C.c_void_p()
+        origin = None if hashOrigin is None else _d3(hashOrigin)
+        call("pf_grid_create_standalone", ctx.handle, int(capacity), float(cellSize), int(hashTableSize or 0), origin, C.byref(h))
+        g = cls.__new__(cls)
+        g.particles = None
+        g._owned = False
+        g._owner = None
+        g.handle = h
+        g._info()
+        return g
+
+    def _info(self):
+        cap, H, Hp = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        call("pf_grid_info", self.handle, C.byref(cap), C.byref(H), C.byref(Hp))
+        self.capacity, self.hashTableSize, self.hashTableSizePadded = cap.value, H.value, Hp.value
+
+    def uploadPositions(self, positions):
+        p = _f32(positions)
+        p = p.reshape(-1, p.shape[-1] if p.ndim == 2 else 4)
+        if p.shape[1] == 3:
+            p = np.concatenate([p, np.zeros((p.shape[0], 1), np.float32)], axis=1)
+        p = _f32(p)
+        call("pf_grid_upload_positions", self.handle, _ptr(p, f32p), p.shape[0])
+
+    def setSortedPositions(self, enabled):
+        call("pf_grid_set_sorted_positions", self.handle, 1 if enabled else 0)
+
+    def rebuild(self):
+        call("pf_grid_rebuild", self.handle)
+
+    def readback(self):
+        n, Hp = self.capacity, self.hashTableSizePadded
+        ci = np.zeros(n, np.uint32)
+        cnt = np.zeros(Hp, np.uint32)
+        cs = np.zeros(Hp, np.uint32)
+        ce = np.zeros(Hp, np.uint32)
+        si = np.zeros(n, np.uint32)
+        call("pf_grid_readback", self.handle, _ptr(ci, u32p), _ptr(cnt, u32p), _ptr(cs, u32p), _ptr(ce, u32p), _ptr(si, u32p))
+        return HashGridSnapshot(n, self.hashTableSize, Hp, ci, cnt, cs, ce, si)
+
+    def readbackSortedKeys(self):
+        k = np.zeros(self.capacity, np.uint32)
+        call("pf_grid_readback_sorted_keys", self.handle, _ptr(k, u32p))
+        return k
+
+    def readbackSortedPositions(self):
+        p = np.zeros((self.capacity, 4), np.float32)
+        call("pf_grid_readback_sorted_positions", self.handle, _ptr(p, f32p))
+        return p
+
+    def readbackOverflow(self):
+        o = C.c_int()
+        call("pf_grid_readback_overflow", self.handle, C.byref(o))
+        return bool(o.value)
+
+    def queryCounts(self, points, radius):
+        q = _f32(points).reshape(-1, 4)
+        cand = np.zeros(q.shape[0], np.uint32)
+        within = np.zeros(q.shape[0], np.uint32)
+        call("pf_grid_query_counts", self.handle, _ptr(q, f32p), q.shape[0], float(radius), _ptr(cand, u32p), _ptr(within, u32p))
+        return cand, within
+
+    @property
+    def hashOrigin(self):
+        raise AttributeError("write-only here; use setHashOrigin")
+
+    def setHashOrigin(self, origin):
+        call("pf_grid_set_origin", self.handle, _d3(origin))
+
+    def dispose(self):
+        if self.handle and not self._owned:
+            call("pf_grid_destroy", self.handle)
+        self.handle = None
+
+
+_FX = {"viscosity": 1, "vorticity": 2, "surfaceTension": 4, "adhesion": 8}
+
+
+class FluidSystem:
+    """src/fluids/FluidSystem.ts:71-322."""
+
+    def __init__(self, particles, range=None, restDensity=None, particleSpacing=None, smoothingRadius=None,
+                 compliance=None, viscosity=None, vorticity=None, surfaceTension=None, adhesion=None):
+        self.particles = particles
+        s, c = _range(range, 0)
+        o = _lib.FluidOptions()
+        o.range_start, o.range_count = s, c
+        o.rest_density = float(restDensity) if restDensity is not None else 0.0
+        o.particle_spacing = float(particleSpacing) if particleSpacing is not None else 0.0
+        o.smoothing_radius = float(smoothingRadius) if smoothingRadius is not None else 0.0
+        o.compliance = float(compliance) if compliance is not None else -1.0
+        present = 0
+        for name, val in (("viscosity", viscosity), ("vorticity", vorticity), ("surfaceTension", surfaceTension), ("adhesion", adhesion)):
+            if val is not None:
+                present |= _FX[name]
+        o.present = present
+        o.viscosity = float(viscosity or 0.0)
+        o.vorticity = float(vorticity or 0.0)
+        o.surface_tension = float(surfaceTension or 0.0)
+        o.adhesion = float(adhesion or 0.0)
+        h = C.c_void_p()
+        call("pf_fluid_create", particles.handle, C.byref(o), C.byref(h))
+        self.handle = h
+        m, hh, sp, rho = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        call("pf_fluid_info", h, C.byref(m), C.byref(hh), C.byref(sp), C.byref(rho))
+        self.mass, self.smoothingRadius, self.particleSpacing, self.restDensity = m.value, hh.value, sp.value, rho.value
+        self.neighborRadius = self.smoothingRadius
+        self.range = {"start": s, "count": c or particles.capacity}
+
+    def _get(self, key):
+        v = C.c_double()
+        call("pf_fluid_get", self.handle, _FX[key], C.byref(v))
+        return v.value
+
+    def _set(self, key, value):
+        call("pf_fluid_set", self.handle, _FX[key], float(value))
+
+    viscosity = property(lambda self: self._get("viscosity"), lambda self, v: self._set("viscosity", v))
+    vorticity = property(lambda self: self._get("vorticity"), lambda self, v: self._set("vorticity", v))
+    surfaceTension = property(lambda self: self._get("surfaceTension"), lambda self, v: self._set("surfaceTension", v))
+    adhesion = property(lambda self: self._get("adhesion"), lambda self, v: self._set("adhesion", v))
+
+    def addBoundary(self, range, dynamic=True):
+        s, c = _range(range, 0)
+        call("pf_fluid_add_boundary", self.handle, s, c, 1 if dynamic else 0)
+
+    def readbackDensity(self):
+        d = np.zeros(self.particles.capacity, np.float32)
+        call("pf_fluid_readback_density", self.handle, _ptr(d, f32p))
+        return d
+
+    def readbackOverflow(self):
+        o = C.c_int()
+        call("pf_fluid_readback_overflow", self.handle, C.byref(o))
+        return bool(o.value)
+
+    def material(self):
+        return _lib.Material(1, self.handle)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_fluid_destroy", self.handle)
+            self.handle = None
+
+
+class PrimitiveSet:
+    """src/core/collision/PrimitiveSet.ts:121-518."""
+
+    def __init__(self, particles, capacity=None):
+        self.particles = particles
+        h = C.c_void_p()
+        call("pf_primitives_create", particles.handle, int(capacity or 0), C.byref(h))
+        self.handle = h
+
+    def addPlane(self, normal, point, muS=None, muK=None):
+        slot = C.c_uint32()
+        call("pf_primitives_add_plane", self.handle, _d3(normal), _d3(point), -1.0 if muS is None else float(muS), -1.0 if muK is None else float(muK), C.byref(slot))
+        return slot.value
+
+    def addSphere(self, center, radius, invert=False, muS=None, muK=None):
+        slot = C.c_uint32()
+        call("pf_primitives_add_sphere", self.handle, _d3(center), float(radius), 1 if invert else 0, -1.0 if muS is None else float(muS), -1.0 if muK is None else float(muK), C.byref(slot))
+        return slot.value
+
+    def addBox(self, center, halfExtents, rotation=None, invert=False, muS=None, muK=None):
+        slot = C.c_uint32()
+        call("pf_primitives_add_box", self.handle, _d3(center), _d3(halfExtents), None if rotation is None else _d4(rotation), 1 if invert else 0, -1.0 if muS is None else float(muS), -1.0 if muK is None else float(muK), C.byref(slot))
+        return slot.value
+
+    def addCapsule(self, a, b, radius, muS=None, muK=None):
+        slot = C.c_uint32()
+        call("pf_primitives_add_capsule", self.handle, _d3(a), _d3(b), float(radius), -1.0 if muS is None else float(muS), -1.0 if muK is None else float(muK), C.byref(slot))
+        return slot.value
+
+    def setSphere(self, slot, center, radius, velocity=None):
+        call("pf_primitives_set_sphere", self.handle, int(slot), _d3(center), float(radius), None if velocity is None else _d3(velocity))
+
+    def setPose(self, slot, position, rotation=(0, 0, 0, 1), dt=0.0):
+        call("pf_primitives_set_pose", self.handle, int(slot), _d3(position), _d4(rotation), float(dt))
+
+    @property
+    def count(self):
+        n = C.c_uint32()
+        call("pf_primitives_count", self.handle, C.byref(n))
+        return n.value
+
+    def dispose(self):
+        if self.handle:
+            call("pf_primitives_destroy", self.handle)
+            self.handle = None
+
+
+class SoftbodySystem:
+    """src/softbody/SoftbodySystem.ts:105-359, global shape matching.
+
+    bodies: list of dicts with ``range`` (start, count), optional
+    ``compliance``, ``surfaceCount`` and ``restPositions`` ((count, 3)).
+    """
+
+    def __init__(self, particles, bodies, shapeMatching="global"):
+        if shapeMatching != "global":
+            raise PFError(1, "SoftbodySystem: local shape matching is not available in this build")
+        self.particles = particles
+        defs = (_lib.SoftbodyDef * len(bodies))()
+        self._keep = []
+        for k, b in enumerate(bodies):
+            s, c = _range(b["range"], 0)
+            defs[k].start, defs[k].count = s, c
+            sc = b.get("surfaceCount")
+            defs[k].surface_count = 0xFFFFFFFF if sc is None else int(sc)
+            defs[k].compliance = float(b.get("compliance", 0.0))
+            rp = b.get("restPositions")
+            if rp is not None:
+                arr = _f32(rp).reshape(-1)
+                self._keep.append(arr)
+                defs[k].rest_positions = _ptr(arr, f32p)
+        h = C.c_void_p()
+        call("pf_softbody_create", particles.handle, defs, len(bodies), C.byref(h))
+        self.handle = h
+        self.nbodies = len(bodies)
+
+    def setCompliance(self, index, compliance):
+        call("pf_softbody_set_compliance", self.handle, int(index), float(compliance))
+
+    def readbackFrames(self):
+        c = np.zeros((self.nbodies, 4), np.float32)
+        r = np.zeros((self.nbodies, 3, 4), np.float32)
+        call("pf_softbody_readback_frames", self.handle, _ptr(c, f32p), _ptr(r, f32p))
+        return c[:, :3], r[:, :, :3]
+
+    def material(self):
+        return _lib.Material(2, self.handle)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_softbody_destroy", self.handle)
+            self.handle = None
+
+
+SimLoopOverflow = namedtuple("SimLoopOverflow", "positions velocities grid contacts neighbors impulses long_bucket")
+
+
+class SimLoop:
+    """src/core/loop.ts:92-324."""
+
+    def __init__(self, particles, substeps=4, iterations=2, gravity=(0.0, -9.81, 0.0), materials=(),
+                 colliders=(), hashOrigin=None, contact=None):
+        if contact:
+            raise PFError(1, "SimLoop: particle contacts are not available in this build")
+        self.particles = particles
+        o = _lib.LoopOptions()
+        o.substeps = int(substeps)
+        o.iterations = int(iterations)
+        for a in range(3):
+            o.gravity[a] = float(gravity[a])
+        if hashOrigin is not None:
+            o.has_hash_origin = 1
+            for a in range(3):
+                o.hash_origin[a] = float(hashOrigin[a])
+        mats = (_lib.Material * max(1, len(materials)))(*[m.material() for m in materials])
+        cols = (C.c_void_p * max(1, len(colliders)))(*[c.handle for c in colliders])
+        h = C.c_void_p()
+        call("pf_loop_create", particles.handle, C.byref(o), mats, len(materials), cols, len(colliders), C.byref(h))
+        self.handle = h
+        self.materials = list(materials)
+        self.colliders = list(colliders)
+        self._grid = None
+
+    def step(self, dt):
+        call("pf_loop_step", self.handle, float(dt))
+
+    def synchronize(self):
+        call("pf_loop_synchronize", self.handle)
+
+    def readbackOverflow(self):
+        o = _lib.Overflow()
+        call("pf_loop_readback_overflow", self.handle, C.byref(o))
+        return SimLoopOverflow(*(bool(getattr(o, f)) for f in SimLoopOverflow._fields))
+
+    def info(self):
+        s, it, cell, H, k = C.c_uint32(), C.c_uint32(), C.c_double(), C.c_uint32(), C.c_uint32()
+        call("pf_loop_info", self.handle, C.byref(s), C.byref(it), C.byref(cell), C.byref(H), C.byref(k))
+        return {"substeps": s.value, "iterations": it.value, "cellSize": cell.value, "hashTableSize": H.value, "kernelsPerStep": k.value}
+
+    @property
+    def substeps(self):
+        return self.info()["substeps"]
+
+    @substeps.setter
+    def substeps(self, value):
+        call("pf_loop_set_substeps", self.handle, int(value))
+
+    @property
+    def iterations(self):
+        return self.info()["iterations"]
+
+    def setGravity(self, g):
+        call("pf_loop_set_gravity", self.handle, _d3(g))
+
+    def useGraph(self, enabled=True):
+        call("pf_loop_set_graph", self.handle, 1 if enabled else 0)
+
+    @property
+    def hashGrid(self):
+        if self._grid is None:
+            h = C.c_void_p()
+            call("pf_loop_grid", self.handle, C.byref(h))
+            self._grid = HashGrid(self.particles, 0, _handle=h, _owner=self)
+        return self._grid
+
+    def dispose(self):
+        if self.handle:
+            call("pf_loop_destroy", self.handle)
+            self.handle = None
+
+
+class Batch:
+    """Many independent SimLoops advanced by one launch sequence (no reference counterpart)."""
+
+    def __init__(self, loops, context=None):
+        ctx = context or default_context()
+        arr = (C.c_void_p * len(loops))(*[L.handle for L in loops])
+        h = C.c_void_p()
+        call("pf_batch_create", ctx.handle, arr, len(loops), C.byref(h))
+        self.handle = h
+        self.loops = list(loops)
+
+    def step(self, dt):
+        call("pf_batch_step", self.handle, float(dt))
+
+    def useGraph(self, enabled=True):
+        call("pf_batch_set_graph", self.handle, 1 if enabled else 0)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_batch_destroy", self.handle)
+            self.handle = None
+
+
+# ---------------------------------------------------------------------------
+# Scene layout (src/simulation/layout.ts, Simulation.ts:957-1104)
+# ---------------------------------------------------------------------------
+def fill_box(box_min, box_max, spacing):
+    """fillBox(box, spacing) (layout.ts:7-14): (N, 3) float32 lattice, y -> z -> x."""
+    n = C.c_uint32()
+    call("pf_layout_fill_box", _d3(box_min), _d3(box_max), float(spacing), None, 0, C.byref(n))
+    out = np.zeros((n.value, 3), np.float32)
+    call("pf_layout_fill_box", _d3(box_min), _d3(box_max), float(spacing), _ptr(out, f32p), n.value, C.byref(n))
+    return out
+
+
+def radius_for_budget(volume, area, cloth_spacing, budget):
+    load()
+    return _lib._lib.pf_layout_radius_for_budget(float(volume), float(area), float(cloth_spacing), int(budget))
+
+
+def fit_budget(budget, container, boxes):
+    """Simulation.fitBudget for box fluids: (radius, particle count)."""
+    cont = None if container is None else (C.c_double * 6)(*[float(v) for v in container])
+    flat = (C.c_double * (6 * len(boxes)))(*[float(v) for b in boxes for v in b])
+    r, n = C.c_double(), C.c_uint32()
+    call("pf_layout_fit_budget", int(budget), cont, flat, len(boxes), C.byref(r), C.byref(n))
+    return r.value, n.value
+
+
+from .simulation import Simulation  # noqa: E402

@@ -1,0 +1,103 @@
+"""Workload scenes: the dam break and the isolated sort inputs.
+
+Dam break (examples/fluid.ts geometry): container [-0.5,0.5]x[0,0.8]x[-0.3,0.3],
+fluid box [-0.5,-0.1]x[0,0.5]x[-0.3,0.3], particles sized by Simulation's
+budget fit, Simulation's default fluid effects (Simulation.ts:348-352) and
+five container planes (Simulation.ts:777-787). The lattice gets a seeded
++-1 % spacing jitter from the reference tests' LCG
+(tests/analytical/hashgrid/hashgrid-random-stress.gpu.test.ts:21-27).
+"""
+import numpy as np
+
+from . import FluidSystem, ParticleSystem, PrimitiveSet, SimLoop, fill_box, fit_budget
+
+CONTAINER = (-0.5, 0.0, -0.3, 0.5, 0.8, 0.3)
+FLUID_BOX = (-0.5, 0.0, -0.3, -0.1, 0.5, 0.3)
+FLUID_DEFAULTS = dict(viscosity=0.01, surfaceTension=0.1, vorticity=0.02, adhesion=0.1)
+PLANES = (((0, 1, 0), "min"), ((1, 0, 0), "min"), ((-1, 0, 0), "max"), ((0, 0, 1), "min"), ((0, 0, -1), "max"))
+
+
+def lcg_uniform(seed, n):
+    """First n draws u = state / 2^32 of the LCG state = imul(state, 1664525) +
+    1013904223 (mod 2^32), vectorised by composing the affine recurrence."""
+    # x_{k+1} = a x_k + c mod 2^32; x_k = A_k x_0 + C_k with (A_k, C_k) composed by doubling.
+    a, c, mod = 1664525, 1013904223, 1 << 32
+    x0 = (seed & 0xFFFFFFFF) or 1
+    out = np.empty(n, np.uint64)
+    block = 4096
+    # coefficients for steps 1..block
+    A = np.empty(block, np.uint64)
+    Cc = np.empty(block, np.uint64)
+    ak, ck = 1, 0
+    for k in range(block):
+        ak, ck = (ak * a) % mod, (ck * a + c) % mod
+        A[k], Cc[k] = ak, ck
+    x = x0
+    done = 0
+    while done < n:
+        m = min(block, n - done)
+        vals = (A[:m] * np.uint64(x) + Cc[:m]) % np.uint64(mod)
+        out[done:done + m] = vals
+        x = int(vals[-1])
+        done += m
+    return out.astype(np.float64) / 4294967296.0
+
+
+def dam_break_layout(budget):
+    """(positions (N,3) float32, particle radius) of Simulation's fitted layout."""
+    r, n = fit_budget(budget, CONTAINER, [FLUID_BOX])
+    shrunk = [CONTAINER[i] + r for i in range(3)] + [CONTAINER[3 + i] - r for i in range(3)]
+    lo = [max(FLUID_BOX[i], shrunk[i]) for i in range(3)]
+    hi = [min(FLUID_BOX[3 + i], shrunk[3 + i]) for i in range(3)]
+    pos = fill_box(lo, hi, 2 * r)
+    assert len(pos) == n
+    return pos, r
+
+
+def jitter(positions, spacing, seed=0xC0FFEE, fraction=0.01):
+    """Add a uniform +-fraction*spacing offset per axis, LCG draws in particle order."""
+    u = lcg_uniform(seed, positions.size).reshape(positions.shape)
+    return (positions.astype(np.float64) + (2.0 * u - 1.0) * fraction * spacing).astype(np.float32)
+
+
+def build_dam_break(budget, substeps=2, iterations=4, seed=0xC0FFEE, context=None, use_graph=False):
+    """The dam break scene on the GPU: dict of ps, fluid, walls, loop, positions, radius."""
+    pos, r = dam_break_layout(budget)
+    pos = jitter(pos, 2 * r, seed)
+    ps = ParticleSystem(len(pos), r, context=context)
+    ps.uploadParticles(pos)
+    fluid = FluidSystem(ps, **FLUID_DEFAULTS)
+    walls = PrimitiveSet(ps)
+    mn, mx = CONTAINER[:3], CONTAINER[3:]
+    for n, which in PLANES:
+        walls.addPlane(n, mn if which == "min" else mx)
+    loop = SimLoop(ps, substeps=substeps, iterations=iterations, materials=[fluid], colliders=[walls])
+    if use_graph:
+        loop.useGraph(True)
+    return dict(ps=ps, fluid=fluid, walls=walls, loop=loop, positions=pos, radius=r)
+
+
+def sort_points(n, distribution, seed=0x5EED):
+    """Isolated-sort inputs in [0,1)^3: 'uniform' or 64 Gaussian 'blobs' (sigma 0.03)."""
+    if distribution == "uniform":
+        u = lcg_uniform(seed, 3 * n).reshape(n, 3)
+        pts = u
+    elif distribution == "blobs":
+        rng = np.random.default_rng(seed)
+        centers = rng.random((64, 3))
+        per = n // 64
+        out = []
+        for b in range(64):
+            pts_b = np.empty((0, 3))
+            need = per if b < 63 else n - per * 63
+            while len(pts_b) < need:
+                cand = centers[b] + 0.03 * rng.standard_normal((2 * need, 3))
+                ok = np.all((cand >= 0.0) & (cand < 1.0), axis=1)
+                pts_b = np.concatenate([pts_b, cand[ok]])
+            out.append(pts_b[:need])
+        pts = np.concatenate(out)
+    else:
+        raise ValueError(distribution)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = pts.astype(np.float32)
+    return pos
