@@ -1,0 +1,54 @@
+"""The Node N-API addon over the C ABI: loads in Node (CPU) and drives the GPU path (gpu)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import particle_fluids as pf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JS = os.path.join(ROOT, "threejs-particle-fluids_amd", "js")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(os.path.join(JS, "pf_node.node")),
+                                reason="node or the built addon is missing")
+
+
+def node(script):
+    return subprocess.run([NODE, "-e", script], cwd=JS, capture_output=True, text=True, timeout=120)
+
+
+def test_addon_loads_and_reports_layout():
+    r = node("const pf=require('./index.js');"
+             "const f=pf.native.layoutFitBudget(5000,[-0.5,0,-0.3,0.5,0.8,0.3],[-0.5,0,-0.3,-0.1,0.5,0.3]);"
+             "console.log(JSON.stringify({abi:pf.native.abiVersion(),n:f.count,r:f.radius,k:Object.keys(pf.native).length}))")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["abi"] == 1 and out["n"] == 4998 and out["k"] >= 48
+    assert out["r"] == pf.fit_budget(5000, (-0.5, 0, -0.3, 0.5, 0.8, 0.3), [(-0.5, 0, -0.3, -0.1, 0.5, 0.3)])[0]
+
+
+def test_addon_errors_carry_reference_messages():
+    r = node("const pf=require('./index.js'); try { new pf.Simulation({particles: 10, particleRadius: 0.1}); }"
+             " catch (e) { console.log(e.message); }")
+    assert "give exactly one of `particles` or `particleRadius`" in r.stdout
+
+
+@pytest.mark.gpu
+def test_js_simulation_matches_python_simulation(tmp_path):
+    out = tmp_path / "pos.bin"
+    r = subprocess.run([NODE, os.path.join(JS, "test", "dam_break.js"), str(out), "2000", "5"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    js_pos = np.fromfile(out, np.float32).reshape(-1, 4)
+    sim = pf.Simulation(particles=2000, container=(-0.5, 0, -0.3, 0.5, 0.8, 0.3), substeps=2, iterations=4)
+    sim.addFluid((-0.5, 0, -0.3, -0.1, 0.5, 0.3))
+    for _ in range(5):
+        sim.step(1.0 / 60.0)
+    py_pos = sim.particleSystem.readback().positions
+    assert js_pos.shape == py_pos.shape
+    assert np.array_equal(js_pos, py_pos)
+    sim.dispose()

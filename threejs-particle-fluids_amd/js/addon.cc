@@ -1,0 +1,705 @@
+// Node N-API addon over the C ABI of libpf.so (include/pf.h).
+//
+// Thin by design: every export converts JS values to plain pointers and
+// sizes, calls one pf_* entry point, and throws a JS Error carrying
+// pf_last_error() on failure, so index.js can mirror the reference's classes.
+// Handles cross as napi externals. loopStepAsync enqueues the step on the
+// library's stream from the JS thread and resolves its Promise from a worker
+// thread that waits on the stream (reference: SimLoop.step returns a Promise,
+// src/core/loop.ts:267-279).
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pf.h"
+
+namespace {
+
+#define CHECK_NAPI(expr)                                                     \
+  do {                                                                       \
+    if ((expr) != napi_ok) {                                                 \
+      napi_throw_error(env, nullptr, "pf addon: N-API call failed: " #expr); \
+      return nullptr;                                                        \
+    }                                                                        \
+  } while (0)
+
+bool fail_if(napi_env env, int rc) {
+  if (rc == PF_OK) return false;
+  napi_throw_error(env, nullptr, pf_last_error());
+  return true;
+}
+
+#define PF_CALL(expr)                 \
+  do {                                \
+    if (fail_if(env, (expr))) return nullptr; \
+  } while (0)
+
+struct Args {
+  napi_value argv[8];
+  size_t argc = 8;
+};
+
+Args get_args(napi_env env, napi_callback_info info) {
+  Args a;
+  napi_get_cb_info(env, info, &a.argc, a.argv, nullptr, nullptr);
+  for (size_t i = a.argc; i < 8; ++i) napi_get_undefined(env, &a.argv[i]);
+  return a;
+}
+
+bool is_nullish(napi_env env, napi_value v) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  return t == napi_undefined || t == napi_null;
+}
+
+void* handle(napi_env env, napi_value v) {
+  if (is_nullish(env, v)) return nullptr;
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return p;
+}
+
+napi_value make_handle(napi_env env, void* p) {
+  napi_value v;
+  napi_create_external(env, p, nullptr, nullptr, &v);
+  return v;
+}
+
+double num(napi_env env, napi_value v, double dflt = 0) {
+  if (is_nullish(env, v)) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+uint32_t u32(napi_env env, napi_value v, uint32_t dflt = 0) {
+  if (is_nullish(env, v)) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return (uint32_t)d;
+}
+
+// Pointer into a TypedArray (null for null/undefined); *count = element count.
+template <typename T>
+T* typed(napi_env env, napi_value v, size_t* count = nullptr) {
+  if (is_nullish(env, v)) {
+    if (count) *count = 0;
+    return nullptr;
+  }
+  napi_typedarray_type type;
+  size_t length = 0;
+  void* data = nullptr;
+  napi_value buf;
+  size_t offset = 0;
+  if (napi_get_typedarray_info(env, v, &type, &length, &data, &buf, &offset) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "pf addon: expected a TypedArray");
+    return nullptr;
+  }
+  if (count) *count = length;
+  return static_cast<T*>(data);
+}
+
+// A JS array of numbers into doubles.
+std::vector<double> doubles(napi_env env, napi_value v) {
+  std::vector<double> out;
+  if (is_nullish(env, v)) return out;
+  uint32_t n = 0;
+  napi_get_array_length(env, v, &n);
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value e;
+    napi_get_element(env, v, i, &e);
+    out.push_back(num(env, e));
+  }
+  return out;
+}
+
+napi_value f32_array(napi_env env, size_t n, float** data) {
+  napi_value ab, arr;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, n * sizeof(float), &p, &ab);
+  napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr);
+  *data = static_cast<float*>(p);
+  return arr;
+}
+
+napi_value u32_array(napi_env env, size_t n, uint32_t** data) {
+  napi_value ab, arr;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, n * sizeof(uint32_t), &p, &ab);
+  napi_create_typedarray(env, napi_uint32_array, n, ab, 0, &arr);
+  *data = static_cast<uint32_t*>(p);
+  return arr;
+}
+
+napi_value number(napi_env env, double d) {
+  napi_value v;
+  napi_create_double(env, d, &v);
+  return v;
+}
+
+napi_value boolean(napi_env env, bool b) {
+  napi_value v;
+  napi_get_boolean(env, b, &v);
+  return v;
+}
+
+void set(napi_env env, napi_value obj, const char* key, napi_value v) { napi_set_named_property(env, obj, key, v); }
+
+napi_value prop(napi_env env, napi_value obj, const char* key) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (!has) {
+    napi_get_undefined(env, &v);
+    return v;
+  }
+  napi_get_named_property(env, obj, key, &v);
+  return v;
+}
+
+// ---------------------------------------------------------------- context
+napi_value ContextCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_context* c = nullptr;
+  PF_CALL(pf_context_create((int)num(env, a.argv[0]), &c));
+  return make_handle(env, c);
+}
+napi_value ContextDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_context_destroy((pf_context*)handle(env, a.argv[0])));
+  return nullptr;
+}
+napi_value ContextSynchronize(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_context_synchronize((pf_context*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- particles
+napi_value ParticlesCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_particles* p = nullptr;
+  PF_CALL(pf_particles_create((pf_context*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2]), &p));
+  return make_handle(env, p);
+}
+napi_value ParticlesDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_particles_destroy((pf_particles*)handle(env, a.argv[0])));
+  return nullptr;
+}
+// (ps, start, positions3, velocities3|null, invMass|null, groups|null)
+napi_value ParticlesUpload(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  size_t n3 = 0;
+  float* pos = typed<float>(env, a.argv[2], &n3);
+  PF_CALL(pf_particles_upload((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]), (uint32_t)(n3 / 3), pos,
+                              typed<float>(env, a.argv[3]), typed<float>(env, a.argv[4]),
+                              typed<uint32_t>(env, a.argv[5])));
+  return nullptr;
+}
+napi_value ParticlesSetInvMass(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_particles_set_inv_mass((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]), u32(env, a.argv[2]),
+                                    num(env, a.argv[3])));
+  return nullptr;
+}
+napi_value ParticlesSetCollisionGroup(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_particles_set_collision_group((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]),
+                                           u32(env, a.argv[2]), u32(env, a.argv[3])));
+  return nullptr;
+}
+napi_value ParticlesReadback(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_particles* ps = (pf_particles*)handle(env, a.argv[0]);
+  uint32_t cap = 0;
+  double radius = 0;
+  PF_CALL(pf_particles_info(ps, &cap, &radius));
+  pf_snapshot s{};
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "capacity", number(env, cap));
+  set(env, obj, "positions", f32_array(env, 4 * (size_t)cap, &s.positions));
+  set(env, obj, "predictedPositions", f32_array(env, 4 * (size_t)cap, &s.predicted_positions));
+  set(env, obj, "velocities", f32_array(env, 4 * (size_t)cap, &s.velocities));
+  set(env, obj, "invMass", f32_array(env, cap, &s.inv_mass));
+  set(env, obj, "rotation", f32_array(env, 4 * (size_t)cap, &s.rotation));
+  set(env, obj, "predictedRotation", f32_array(env, 4 * (size_t)cap, &s.predicted_rotation));
+  set(env, obj, "angularVelocity", f32_array(env, 4 * (size_t)cap, &s.angular_velocity));
+  PF_CALL(pf_particles_readback(ps, &s));
+  return obj;
+}
+
+// ---------------------------------------------------------------- grid
+napi_value GridCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  std::vector<double> o = doubles(env, a.argv[3]);
+  pf_grid* g = nullptr;
+  PF_CALL(pf_grid_create((pf_particles*)handle(env, a.argv[0]), num(env, a.argv[1]), u32(env, a.argv[2]),
+                         o.size() == 3 ? o.data() : nullptr, &g));
+  return make_handle(env, g);
+}
+napi_value GridRebuild(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_grid_rebuild((pf_grid*)handle(env, a.argv[0])));
+  return nullptr;
+}
+napi_value GridReadback(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_grid* g = (pf_grid*)handle(env, a.argv[0]);
+  uint32_t cap = 0, H = 0, Hp = 0;
+  PF_CALL(pf_grid_info(g, &cap, &H, &Hp));
+  uint32_t *ci, *cn, *cs, *ce, *si;
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "capacity", number(env, cap));
+  set(env, obj, "hashTableSize", number(env, H));
+  set(env, obj, "hashTableSizePadded", number(env, Hp));
+  set(env, obj, "cellIndex", u32_array(env, cap, &ci));
+  set(env, obj, "counts", u32_array(env, Hp, &cn));
+  set(env, obj, "cellStart", u32_array(env, Hp, &cs));
+  set(env, obj, "cellEnd", u32_array(env, Hp, &ce));
+  set(env, obj, "sortedIndices", u32_array(env, cap, &si));
+  PF_CALL(pf_grid_readback(g, ci, cn, cs, ce, si));
+  return obj;
+}
+napi_value GridReadbackOverflow(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  int o = 0;
+  PF_CALL(pf_grid_readback_overflow((pf_grid*)handle(env, a.argv[0]), &o));
+  return boolean(env, o != 0);
+}
+napi_value GridInfo(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t cap = 0, H = 0, Hp = 0;
+  PF_CALL(pf_grid_info((pf_grid*)handle(env, a.argv[0]), &cap, &H, &Hp));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "capacity", number(env, cap));
+  set(env, obj, "hashTableSize", number(env, H));
+  set(env, obj, "hashTableSizePadded", number(env, Hp));
+  return obj;
+}
+napi_value GridSetOrigin(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  std::vector<double> o = doubles(env, a.argv[1]);
+  if (o.size() != 3) {
+    napi_throw_type_error(env, nullptr, "HashGrid: hashOrigin needs three numbers");
+    return nullptr;
+  }
+  PF_CALL(pf_grid_set_origin((pf_grid*)handle(env, a.argv[0]), o.data()));
+  return nullptr;
+}
+napi_value GridDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_grid_destroy((pf_grid*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- fluid
+napi_value FluidCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  napi_value o = a.argv[1];
+  pf_fluid_options opt{};
+  opt.range_start = u32(env, prop(env, o, "rangeStart"));
+  opt.range_count = u32(env, prop(env, o, "rangeCount"));
+  opt.rest_density = num(env, prop(env, o, "restDensity"), 0);
+  opt.particle_spacing = num(env, prop(env, o, "particleSpacing"), 0);
+  opt.smoothing_radius = num(env, prop(env, o, "smoothingRadius"), 0);
+  opt.compliance = num(env, prop(env, o, "compliance"), -1);
+  const char* keys[4] = {"viscosity", "vorticity", "surfaceTension", "adhesion"};
+  double* vals[4] = {&opt.viscosity, &opt.vorticity, &opt.surface_tension, &opt.adhesion};
+  for (int k = 0; k < 4; ++k) {
+    napi_value v = prop(env, o, keys[k]);
+    if (!is_nullish(env, v)) {
+      opt.present |= 1u << k;
+      *vals[k] = num(env, v);
+    }
+  }
+  pf_fluid* f = nullptr;
+  PF_CALL(pf_fluid_create((pf_particles*)handle(env, a.argv[0]), &opt, &f));
+  return make_handle(env, f);
+}
+napi_value FluidSet(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_fluid_set((pf_fluid*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2])));
+  return nullptr;
+}
+napi_value FluidGet(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  double v = 0;
+  PF_CALL(pf_fluid_get((pf_fluid*)handle(env, a.argv[0]), u32(env, a.argv[1]), &v));
+  return number(env, v);
+}
+napi_value FluidAddBoundary(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  bool dyn = true;
+  if (!is_nullish(env, a.argv[3])) napi_get_value_bool(env, a.argv[3], &dyn);
+  PF_CALL(pf_fluid_add_boundary((pf_fluid*)handle(env, a.argv[0]), u32(env, a.argv[1]), u32(env, a.argv[2]), dyn ? 1 : 0));
+  return nullptr;
+}
+napi_value FluidReadbackDensity(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_fluid* f = (pf_fluid*)handle(env, a.argv[0]);
+  uint32_t cap = u32(env, a.argv[1]);
+  float* d;
+  napi_value arr = f32_array(env, cap, &d);
+  PF_CALL(pf_fluid_readback_density(f, d));
+  return arr;
+}
+napi_value FluidReadbackOverflow(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  int o = 0;
+  PF_CALL(pf_fluid_readback_overflow((pf_fluid*)handle(env, a.argv[0]), &o));
+  return boolean(env, o != 0);
+}
+napi_value FluidInfo(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  double m, h, sp, rho;
+  PF_CALL(pf_fluid_info((pf_fluid*)handle(env, a.argv[0]), &m, &h, &sp, &rho));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "mass", number(env, m));
+  set(env, obj, "smoothingRadius", number(env, h));
+  set(env, obj, "particleSpacing", number(env, sp));
+  set(env, obj, "restDensity", number(env, rho));
+  return obj;
+}
+napi_value FluidDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_fluid_destroy((pf_fluid*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- primitives
+napi_value PrimitivesCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_primitives* s = nullptr;
+  PF_CALL(pf_primitives_create((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]), &s));
+  return make_handle(env, s);
+}
+// (set, kind, ...) kind: 'plane' n p muS muK | 'sphere' c r invert muS muK |
+// 'box' c half rot invert muS muK | 'capsule' a b r muS muK
+napi_value PrimitivesAdd(napi_env env, napi_callback_info info) {
+  size_t argc = 8;
+  napi_value v[8];
+  napi_get_cb_info(env, info, &argc, v, nullptr, nullptr);
+  for (size_t i = argc; i < 8; ++i) napi_get_undefined(env, &v[i]);
+  pf_primitives* s = (pf_primitives*)handle(env, v[0]);
+  char kind[16] = {0};
+  size_t len = 0;
+  napi_get_value_string_utf8(env, v[1], kind, sizeof kind, &len);
+  uint32_t slot = 0;
+  std::string k(kind);
+  if (k == "plane") {
+    auto n = doubles(env, v[2]), p = doubles(env, v[3]);
+    PF_CALL(pf_primitives_add_plane(s, n.data(), p.data(), num(env, v[4], -1), num(env, v[5], -1), &slot));
+  } else if (k == "sphere") {
+    auto c = doubles(env, v[2]);
+    bool inv = false;
+    if (!is_nullish(env, v[4])) napi_get_value_bool(env, v[4], &inv);
+    PF_CALL(pf_primitives_add_sphere(s, c.data(), num(env, v[3]), inv, num(env, v[5], -1), num(env, v[6], -1), &slot));
+  } else if (k == "box") {
+    auto c = doubles(env, v[2]), h = doubles(env, v[3]), q = doubles(env, v[4]);
+    bool inv = false;
+    if (!is_nullish(env, v[5])) napi_get_value_bool(env, v[5], &inv);
+    PF_CALL(pf_primitives_add_box(s, c.data(), h.data(), q.size() == 4 ? q.data() : nullptr, inv, num(env, v[6], -1),
+                                  num(env, v[7], -1), &slot));
+  } else if (k == "capsule") {
+    auto p0 = doubles(env, v[2]), p1 = doubles(env, v[3]);
+    PF_CALL(pf_primitives_add_capsule(s, p0.data(), p1.data(), num(env, v[4]), num(env, v[5], -1), num(env, v[6], -1), &slot));
+  } else {
+    napi_throw_error(env, nullptr, "PrimitiveSet: unknown primitive kind");
+    return nullptr;
+  }
+  return number(env, slot);
+}
+napi_value PrimitivesSetSphere(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto c = doubles(env, a.argv[2]), vel = doubles(env, a.argv[4]);
+  PF_CALL(pf_primitives_set_sphere((pf_primitives*)handle(env, a.argv[0]), u32(env, a.argv[1]), c.data(),
+                                   num(env, a.argv[3]), vel.size() == 3 ? vel.data() : nullptr));
+  return nullptr;
+}
+napi_value PrimitivesSetPose(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto p = doubles(env, a.argv[2]), q = doubles(env, a.argv[3]);
+  PF_CALL(pf_primitives_set_pose((pf_primitives*)handle(env, a.argv[0]), u32(env, a.argv[1]), p.data(),
+                                 q.size() == 4 ? q.data() : nullptr, num(env, a.argv[4])));
+  return nullptr;
+}
+napi_value PrimitivesCount(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t n = 0;
+  PF_CALL(pf_primitives_count((pf_primitives*)handle(env, a.argv[0]), &n));
+  return number(env, n);
+}
+napi_value PrimitivesDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_primitives_destroy((pf_primitives*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- softbody
+napi_value SoftbodyCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t nb = 0;
+  napi_get_array_length(env, a.argv[1], &nb);
+  std::vector<pf_softbody_def> defs(nb);
+  for (uint32_t b = 0; b < nb; ++b) {
+    napi_value o;
+    napi_get_element(env, a.argv[1], b, &o);
+    defs[b].start = u32(env, prop(env, o, "start"));
+    defs[b].count = u32(env, prop(env, o, "count"));
+    napi_value sc = prop(env, o, "surfaceCount");
+    defs[b].surface_count = is_nullish(env, sc) ? 0xffffffffu : u32(env, sc);
+    defs[b].compliance = num(env, prop(env, o, "compliance"), 0);
+    defs[b].rest_positions = typed<float>(env, prop(env, o, "restPositions"));
+  }
+  pf_softbody* sb = nullptr;
+  PF_CALL(pf_softbody_create((pf_particles*)handle(env, a.argv[0]), defs.data(), nb, &sb));
+  return make_handle(env, sb);
+}
+napi_value SoftbodySetCompliance(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_softbody_set_compliance((pf_softbody*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2])));
+  return nullptr;
+}
+napi_value SoftbodyReadbackFrames(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t nb = u32(env, a.argv[1]);
+  float *c, *r;
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "bodyCenters", f32_array(env, 4 * (size_t)nb, &c));
+  set(env, obj, "bodyRotations", f32_array(env, 12 * (size_t)nb, &r));
+  PF_CALL(pf_softbody_readback_frames((pf_softbody*)handle(env, a.argv[0]), c, r));
+  return obj;
+}
+napi_value SoftbodyDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_softbody_destroy((pf_softbody*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- loop
+// (ps, {substeps, iterations, gravity, hashOrigin}, [{kind, handle}], [colliders])
+napi_value LoopCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_loop_options o{};
+  napi_value opts = a.argv[1];
+  o.substeps = u32(env, prop(env, opts, "substeps"), 0);
+  napi_value it = prop(env, opts, "iterations");
+  o.iterations = is_nullish(env, it) ? 0xffffffffu : u32(env, it);
+  auto g = doubles(env, prop(env, opts, "gravity"));
+  if (g.size() == 3) {
+    o.gravity[0] = g[0]; o.gravity[1] = g[1]; o.gravity[2] = g[2];
+  } else {
+    o.gravity[1] = -9.81;
+  }
+  auto ho = doubles(env, prop(env, opts, "hashOrigin"));
+  if (ho.size() == 3) {
+    o.has_hash_origin = 1;
+    o.hash_origin[0] = ho[0]; o.hash_origin[1] = ho[1]; o.hash_origin[2] = ho[2];
+  }
+  uint32_t nm = 0, nc = 0;
+  if (!is_nullish(env, a.argv[2])) napi_get_array_length(env, a.argv[2], &nm);
+  if (!is_nullish(env, a.argv[3])) napi_get_array_length(env, a.argv[3], &nc);
+  std::vector<pf_material> mats(nm);
+  for (uint32_t i = 0; i < nm; ++i) {
+    napi_value m;
+    napi_get_element(env, a.argv[2], i, &m);
+    mats[i].kind = u32(env, prop(env, m, "kind"));
+    mats[i].handle = handle(env, prop(env, m, "handle"));
+  }
+  std::vector<pf_primitives*> cols(nc);
+  for (uint32_t i = 0; i < nc; ++i) {
+    napi_value c;
+    napi_get_element(env, a.argv[3], i, &c);
+    cols[i] = (pf_primitives*)handle(env, c);
+  }
+  pf_loop* L = nullptr;
+  PF_CALL(pf_loop_create((pf_particles*)handle(env, a.argv[0]), &o, mats.data(), nm, cols.data(), nc, &L));
+  return make_handle(env, L);
+}
+napi_value LoopStep(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_loop_step((pf_loop*)handle(env, a.argv[0]), num(env, a.argv[1])));
+  return nullptr;
+}
+
+struct StepWork {
+  pf_loop* loop;
+  napi_deferred deferred;
+  napi_async_work work;
+  int rc;
+  std::string error;
+};
+
+void step_execute(napi_env, void* data) {
+  StepWork* w = static_cast<StepWork*>(data);
+  w->rc = pf_loop_synchronize(w->loop);
+  if (w->rc != PF_OK) w->error = pf_last_error();
+}
+
+void step_complete(napi_env env, napi_status, void* data) {
+  StepWork* w = static_cast<StepWork*>(data);
+  if (w->rc == PF_OK) {
+    napi_value u;
+    napi_get_undefined(env, &u);
+    napi_resolve_deferred(env, w->deferred, u);
+  } else {
+    napi_value msg, err;
+    napi_create_string_utf8(env, w->error.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  }
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// Enqueue a step and return a Promise resolved when the GPU has finished it.
+napi_value LoopStepAsync(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_loop* L = (pf_loop*)handle(env, a.argv[0]);
+  PF_CALL(pf_loop_step(L, num(env, a.argv[1])));
+  auto* w = new StepWork{L, nullptr, nullptr, PF_OK, {}};
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
+  napi_create_string_utf8(env, "pf_loop_step", NAPI_AUTO_LENGTH, &name);
+  CHECK_NAPI(napi_create_async_work(env, nullptr, name, step_execute, step_complete, w, &w->work));
+  CHECK_NAPI(napi_queue_async_work(env, w->work));
+  return promise;
+}
+napi_value LoopSynchronize(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_loop_synchronize((pf_loop*)handle(env, a.argv[0])));
+  return nullptr;
+}
+napi_value LoopReadbackOverflow(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_overflow o{};
+  PF_CALL(pf_loop_readback_overflow((pf_loop*)handle(env, a.argv[0]), &o));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "positions", boolean(env, o.positions));
+  set(env, obj, "velocities", boolean(env, o.velocities));
+  set(env, obj, "grid", boolean(env, o.grid));
+  set(env, obj, "contacts", boolean(env, o.contacts));
+  return obj;
+}
+napi_value LoopSetSubsteps(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_loop_set_substeps((pf_loop*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value LoopSetGravity(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto g = doubles(env, a.argv[1]);
+  if (g.size() != 3) {
+    napi_throw_type_error(env, nullptr, "SimLoop: gravity needs three numbers");
+    return nullptr;
+  }
+  PF_CALL(pf_loop_set_gravity((pf_loop*)handle(env, a.argv[0]), g.data()));
+  return nullptr;
+}
+napi_value LoopSetGraph(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  bool on = false;
+  napi_get_value_bool(env, a.argv[1], &on);
+  PF_CALL(pf_loop_set_graph((pf_loop*)handle(env, a.argv[0]), on ? 1 : 0));
+  return nullptr;
+}
+napi_value LoopInfo(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t s, it, H, k;
+  double cell;
+  PF_CALL(pf_loop_info((pf_loop*)handle(env, a.argv[0]), &s, &it, &cell, &H, &k));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "substeps", number(env, s));
+  set(env, obj, "iterations", number(env, it));
+  set(env, obj, "cellSize", number(env, cell));
+  set(env, obj, "hashTableSize", number(env, H));
+  set(env, obj, "kernelsPerStep", number(env, k));
+  return obj;
+}
+napi_value LoopGrid(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_grid* g = nullptr;
+  PF_CALL(pf_loop_grid((pf_loop*)handle(env, a.argv[0]), &g));
+  return make_handle(env, g);
+}
+napi_value LoopDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_loop_destroy((pf_loop*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- layout
+napi_value LayoutFillBox(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto mn = doubles(env, a.argv[0]), mx = doubles(env, a.argv[1]);
+  double sp = num(env, a.argv[2]);
+  uint32_t n = 0;
+  PF_CALL(pf_layout_fill_box(mn.data(), mx.data(), sp, nullptr, 0, &n));
+  float* d;
+  napi_value arr = f32_array(env, 3 * (size_t)n, &d);
+  PF_CALL(pf_layout_fill_box(mn.data(), mx.data(), sp, d, n, &n));
+  return arr;
+}
+napi_value LayoutFitBudget(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto c = doubles(env, a.argv[1]), boxes = doubles(env, a.argv[2]);
+  double r = 0;
+  uint32_t n = 0;
+  PF_CALL(pf_layout_fit_budget(u32(env, a.argv[0]), c.size() == 6 ? c.data() : nullptr, boxes.data(),
+                               (uint32_t)(boxes.size() / 6), &r, &n));
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "radius", number(env, r));
+  set(env, obj, "count", number(env, n));
+  return obj;
+}
+napi_value AbiVersion(napi_env env, napi_callback_info) { return number(env, pf_abi_version()); }
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct Fn {
+    const char* name;
+    napi_callback cb;
+  } fns[] = {
+      {"contextCreate", ContextCreate}, {"contextDestroy", ContextDestroy}, {"contextSynchronize", ContextSynchronize},
+      {"particlesCreate", ParticlesCreate}, {"particlesDestroy", ParticlesDestroy},
+      {"particlesUpload", ParticlesUpload}, {"particlesSetInvMass", ParticlesSetInvMass},
+      {"particlesSetCollisionGroup", ParticlesSetCollisionGroup}, {"particlesReadback", ParticlesReadback},
+      {"gridCreate", GridCreate}, {"gridRebuild", GridRebuild}, {"gridReadback", GridReadback},
+      {"gridReadbackOverflow", GridReadbackOverflow}, {"gridInfo", GridInfo}, {"gridSetOrigin", GridSetOrigin},
+      {"gridDestroy", GridDestroy},
+      {"fluidCreate", FluidCreate}, {"fluidSet", FluidSet}, {"fluidGet", FluidGet},
+      {"fluidAddBoundary", FluidAddBoundary}, {"fluidReadbackDensity", FluidReadbackDensity},
+      {"fluidReadbackOverflow", FluidReadbackOverflow}, {"fluidInfo", FluidInfo}, {"fluidDestroy", FluidDestroy},
+      {"primitivesCreate", PrimitivesCreate}, {"primitivesAdd", PrimitivesAdd},
+      {"primitivesSetSphere", PrimitivesSetSphere}, {"primitivesSetPose", PrimitivesSetPose},
+      {"primitivesCount", PrimitivesCount}, {"primitivesDestroy", PrimitivesDestroy},
+      {"softbodyCreate", SoftbodyCreate}, {"softbodySetCompliance", SoftbodySetCompliance},
+      {"softbodyReadbackFrames", SoftbodyReadbackFrames}, {"softbodyDestroy", SoftbodyDestroy},
+      {"loopCreate", LoopCreate}, {"loopStep", LoopStep}, {"loopStepAsync", LoopStepAsync},
+      {"loopSynchronize", LoopSynchronize}, {"loopReadbackOverflow", LoopReadbackOverflow},
+      {"loopSetSubsteps", LoopSetSubsteps}, {"loopSetGravity", LoopSetGravity}, {"loopSetGraph", LoopSetGraph},
+      {"loopInfo", LoopInfo}, {"loopGrid", LoopGrid}, {"loopDestroy", LoopDestroy},
+      {"layoutFillBox", LayoutFillBox}, {"layoutFitBudget", LayoutFitBudget}, {"abiVersion", AbiVersion},
+  };
+  for (const Fn& f : fns) {
+    napi_value fn;
+    napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);
+    napi_set_named_property(env, exports, f.name, fn);
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,454 @@
+'use strict';
+// JS facade of the native solver, mirroring threejs-particle-fluids' classes
+// (src/core/particles.ts, hashGrid/HashGrid.ts, collision/PrimitiveSet.ts,
+// loop.ts, src/fluids/FluidSystem.ts, src/softbody/SoftbodySystem.ts,
+// src/simulation/Simulation.ts) over the N-API addon. Vectors may be
+// three.js-like {x, y, z} objects or [x, y, z] arrays; boxes {min, max}.
+// GPU work is enqueued on the library's HIP stream; step() and readbacks
+// return Promises like the reference's.
+
+const path = require('path');
+
+const native = require(path.join(__dirname, 'pf_node.node'));
+
+const vec = (v) => (Array.isArray(v) ? [v[0], v[1], v[2]] : [v.x, v.y, v.z]);
+const quat = (q) => (q ? (Array.isArray(q) ? [q[0], q[1], q[2], q[3]] : [q.x, q.y, q.z, q.w]) : undefined);
+const def = (v, d) => (v === undefined || v === null ? d : v);
+const box6 = (b) => [...vec(b.min), ...vec(b.max)];
+
+class Context {
+  constructor(device = 0) {
+    this.handle = native.contextCreate(device);
+  }
+  synchronize() {
+    native.contextSynchronize(this.handle);
+  }
+  dispose() {
+    if (this.handle) native.contextDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+let defaultContext = null;
+function getDefaultContext() {
+  if (!defaultContext) defaultContext = new Context(0);
+  return defaultContext;
+}
+
+class ParticleSystem {
+  constructor(capacity, particleRadius, context = getDefaultContext()) {
+    this.context = context;
+    this.handle = native.particlesCreate(context.handle, capacity, particleRadius);
+    this.capacity = capacity;
+    this.particleRadius = particleRadius;
+    this.isDisposed = false;
+  }
+  /** uploadParticles(data: ParticleInit[], start = 0) (particles.ts:121-160). */
+  uploadParticles(data, start = 0) {
+    const n = data.length;
+    const pos = new Float32Array(3 * n);
+    const vel = new Float32Array(3 * n);
+    const inv = new Float32Array(n);
+    const grp = new Uint32Array(n);
+    data.forEach((p, i) => {
+      pos.set(p.position, 3 * i);
+      if (p.velocity) vel.set(p.velocity, 3 * i);
+      inv[i] = def(p.invMass, 1);
+      grp[i] = def(p.collisionGroup, 0);
+    });
+    native.particlesUpload(this.handle, start, pos, vel, inv, grp);
+  }
+  /** Bulk form: xyz triples in Float32Arrays. */
+  uploadArrays(positions, start = 0, velocities = null, invMass = null, groups = null) {
+    native.particlesUpload(this.handle, start, positions, velocities, invMass, groups);
+  }
+  setInvMass(range, invMass) {
+    native.particlesSetInvMass(this.handle, range.start, range.count, invMass);
+  }
+  setCollisionGroup(range, group) {
+    native.particlesSetCollisionGroup(this.handle, range.start, range.count, group);
+  }
+  async readback() {
+    return native.particlesReadback(this.handle);
+  }
+  get disposed() {
+    return this.isDisposed;
+  }
+  dispose() {
+    if (this.isDisposed) return;
+    this.isDisposed = true;
+    native.particlesDestroy(this.handle);
+  }
+}
+
+class HashGrid {
+  constructor(particles, options, handle = null) {
+    this.particles = particles;
+    this.owned = handle !== null;
+    this.handle =
+      handle !== null
+        ? handle
+        : native.gridCreate(
+        particles.handle,
+        options.cellSize,
+        def(options.hashTableSize, 0),
+        options.hashOrigin ? vec(options.hashOrigin) : null,
+      );
+    const info = native.gridInfo(this.handle);
+    this.hashTableSize = info.hashTableSize;
+    this.hashTableSizePadded = info.hashTableSizePadded;
+    this.cellSize = options ? options.cellSize : undefined;
+  }
+  async rebuild() {
+    native.gridRebuild(this.handle);
+  }
+  async readback() {
+    return native.gridReadback(this.handle);
+  }
+  async readbackOverflow() {
+    return native.gridReadbackOverflow(this.handle);
+  }
+  setHashOrigin(origin) {
+    native.gridSetOrigin(this.handle, vec(origin));
+  }
+  dispose() {
+    if (!this.owned && this.handle) native.gridDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+const EFFECT = { viscosity: 1, vorticity: 2, surfaceTension: 4, adhesion: 8 };
+
+class FluidSystem {
+  constructor(particles, options = {}) {
+    this.particles = particles;
+    const range = def(options.range, { start: 0, count: particles.capacity });
+    this.range = range;
+    this.handle = native.fluidCreate(particles.handle, {
+      rangeStart: range.start,
+      rangeCount: range.count,
+      restDensity: options.restDensity,
+      particleSpacing: options.particleSpacing,
+      smoothingRadius: options.smoothingRadius,
+      compliance: options.compliance,
+      viscosity: options.viscosity,
+      vorticity: options.vorticity,
+      surfaceTension: options.surfaceTension,
+      adhesion: options.adhesion,
+    });
+    const info = native.fluidInfo(this.handle);
+    this.mass = info.mass;
+    this.smoothingRadius = info.smoothingRadius;
+    this.particleSpacing = info.particleSpacing;
+    this.restDensity = info.restDensity;
+    this.neighborRadius = this.smoothingRadius;
+  }
+  get viscosity() { return native.fluidGet(this.handle, EFFECT.viscosity); }
+  set viscosity(v) { native.fluidSet(this.handle, EFFECT.viscosity, v); }
+  get vorticity() { return native.fluidGet(this.handle, EFFECT.vorticity); }
+  set vorticity(v) { native.fluidSet(this.handle, EFFECT.vorticity, v); }
+  get surfaceTension() { return native.fluidGet(this.handle, EFFECT.surfaceTension); }
+  set surfaceTension(v) { native.fluidSet(this.handle, EFFECT.surfaceTension, v); }
+  get adhesion() { return native.fluidGet(this.handle, EFFECT.adhesion); }
+  set adhesion(v) { native.fluidSet(this.handle, EFFECT.adhesion, v); }
+  addBoundary(range, options = {}) {
+    native.fluidAddBoundary(this.handle, range.start, range.count, def(options.dynamic, true));
+  }
+  async readbackOverflow() {
+    return native.fluidReadbackOverflow(this.handle);
+  }
+  async readbackDensity() {
+    return native.fluidReadbackDensity(this.handle, this.particles.capacity);
+  }
+  get material() {
+    return { kind: 1, handle: this.handle };
+  }
+  dispose() {
+    if (this.handle) native.fluidDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+class PrimitiveSet {
+  constructor(particles, options = {}) {
+    this.particles = particles;
+    this.handle = native.primitivesCreate(particles.handle, def(options.capacity, 0));
+  }
+  addPlane(normal, point, options = {}) {
+    return native.primitivesAdd(this.handle, 'plane', vec(normal), vec(point), options.muS, options.muK);
+  }
+  addSphere(center, radius, options = {}) {
+    return native.primitivesAdd(this.handle, 'sphere', vec(center), radius, !!options.invert, options.muS, options.muK);
+  }
+  addBox(center, halfExtents, options = {}) {
+    return native.primitivesAdd(this.handle, 'box', vec(center), vec(halfExtents), quat(options.rotation),
+      !!options.invert, options.muS, options.muK);
+  }
+  addCapsule(a, b, radius, options = {}) {
+    return native.primitivesAdd(this.handle, 'capsule', vec(a), vec(b), radius, options.muS, options.muK);
+  }
+  setSphere(slot, center, radius, velocity) {
+    native.primitivesSetSphere(this.handle, slot, vec(center), radius, velocity ? vec(velocity) : null);
+  }
+  /** attach()+update() for a pose the caller drives each step. */
+  setPose(slot, position, rotation, dt = 0) {
+    native.primitivesSetPose(this.handle, slot, vec(position), def(quat(rotation), [0, 0, 0, 1]), dt);
+  }
+  get count() {
+    return native.primitivesCount(this.handle);
+  }
+  dispose() {
+    if (this.handle) native.primitivesDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+class SoftbodySystem {
+  constructor(particles, options) {
+    if ((def(options.shapeMatching, 'global')) !== 'global') {
+      throw new Error('SoftbodySystem: local shape matching is not available in this build');
+    }
+    this.particles = particles;
+    this.bodies = options.bodies;
+    this.handle = native.softbodyCreate(
+      particles.handle,
+      options.bodies.map((b) => ({
+        start: b.range.start,
+        count: b.range.count,
+        surfaceCount: b.surfaceCount,
+        compliance: def(b.compliance, 0),
+        restPositions: b.restPositions,
+      })),
+    );
+  }
+  setCompliance(index, compliance) {
+    native.softbodySetCompliance(this.handle, index, compliance);
+  }
+  async readbackFrames() {
+    return native.softbodyReadbackFrames(this.handle, this.bodies.length);
+  }
+  get material() {
+    return { kind: 2, handle: this.handle };
+  }
+  dispose() {
+    if (this.handle) native.softbodyDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+class SimLoop {
+  constructor(particles, options = {}) {
+    if (options.contact) throw new Error('SimLoop: particle contacts are not available in this build');
+    this.particles = particles;
+    this.handle = native.loopCreate(
+      particles.handle,
+      {
+        substeps: options.substeps,
+        iterations: options.iterations,
+        gravity: options.gravity ? vec(options.gravity) : undefined,
+        hashOrigin: options.hashOrigin ? vec(options.hashOrigin) : undefined,
+      },
+      (def(options.materials, [])).map((m) => m.material),
+      (def(options.colliders, [])).map((c) => c.handle),
+    );
+    this.stepping = null;
+  }
+  /** step(dt) (loop.ts:267-279): resolves when the GPU has finished the step. */
+  step(dt) {
+    return native.loopStepAsync(this.handle, dt);
+  }
+  get substeps() { return native.loopInfo(this.handle).substeps; }
+  set substeps(v) { native.loopSetSubsteps(this.handle, v); }
+  get iterations() { return native.loopInfo(this.handle).iterations; }
+  setGravity(g) {
+    native.loopSetGravity(this.handle, vec(g));
+  }
+  useGraph(enabled = true) {
+    native.loopSetGraph(this.handle, !!enabled);
+  }
+  get hashGrid() {
+    return new HashGrid(this.particles, null, native.loopGrid(this.handle));
+  }
+  async readbackOverflow() {
+    return native.loopReadbackOverflow(this.handle);
+  }
+  dispose() {
+    if (this.handle) native.loopDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** new Simulation({particles | particleRadius, container, closed, gravity, substeps, iterations}). */
+class Simulation {
+  constructor(options) {
+    const { particles, particleRadius } = options;
+    if ((particles === undefined) === (particleRadius === undefined)) {
+      throw new Error('Simulation: give exactly one of `particles` or `particleRadius`');
+    }
+    if (particles !== undefined && !(Number.isInteger(particles) && particles > 0)) {
+      throw new Error(`Simulation: particles must be a positive integer, got ${particles}`);
+    }
+    this.options = options;
+    this.radius = def(particleRadius, 0);
+    this.gravityValue = options.gravity ? vec(options.gravity) : [0, -9.81, 0];
+    this.fluids = [];
+    this.colliderDefs = [];
+    this.started = null;
+    this.stepping = null;
+  }
+  addFluid(options) {
+    if (this.started) throw new Error('Simulation: addFluid must happen before the first step() or start()');
+    if (!options.box) throw new Error('addFluid: give either `box` or `mesh`');
+    const entry = {
+      box: box6(options.box),
+      viscosity: def(options.viscosity, 0.01),
+      surfaceTension: def(options.surfaceTension, 0.1),
+      vorticity: def(options.vorticity, 0.02),
+      adhesion: def(options.adhesion, 0.1),
+    };
+    this.fluids.push(entry);
+    return entry;
+  }
+  addFloor(options = {}) {
+    this.colliderDefs.push({ kind: 'floor', height: def(options.height, 0), friction: def(options.friction, 0.5) });
+  }
+  addSphere(options) {
+    this.colliderDefs.push({ kind: 'sphere', center: vec(def(options.center, [0, 0, 0])), radius: options.radius,
+      friction: def(options.friction, 0.5) });
+  }
+  addBox(options) {
+    const s = vec(options.size);
+    this.colliderDefs.push({ kind: 'box', center: vec(def(options.center, [0, 0, 0])), half: s.map((v) => v * 0.5),
+      rotation: quat(options.rotation), friction: def(options.friction, 0.5) });
+  }
+  addCapsule(options) {
+    this.colliderDefs.push({ kind: 'capsule', a: vec(options.start), b: vec(options.end), radius: options.radius,
+      friction: def(options.friction, 0.5) });
+  }
+  get gravity() {
+    return this.gravityValue;
+  }
+  get particleRadius() {
+    return this.radius;
+  }
+  get particleCount() {
+    return this.system ? this.system.capacity : 0;
+  }
+  get particleSystem() {
+    if (!this.system) throw new Error('Simulation.particleSystem is created on the first step');
+    return this.system;
+  }
+  get loop() {
+    if (!this.simLoop) throw new Error('Simulation.loop is created on the first step');
+    return this.simLoop;
+  }
+  start() {
+    if (!this.started) {
+      this.started = Promise.resolve().then(() => this.build()).catch((error) => {
+        this.started = null;
+        throw error;
+      });
+    }
+    return this.started;
+  }
+  step(dt = 1 / 60) {
+    if (!this.stepping) {
+      this.stepping = this.start()
+        .then(() => {
+          this.simLoop.setGravity(this.gravityValue);
+          return this.simLoop.step(dt);
+        })
+        .finally(() => {
+          this.stepping = null;
+        });
+    }
+    return this.stepping;
+  }
+  build() {
+    if (!this.fluids.length) throw new Error('Simulation: add a fluid, smoke, soft body, or cloth before stepping');
+    if (this.colliderDefs.some((d) => d.kind !== 'floor') && this.options.particles !== undefined) {
+      // obstacles change the fitted layout; the Python host implements clearOf for them
+      throw new Error('Simulation: budget fitting with obstacles is not available in the JS facade');
+    }
+    const container = this.options.container ? box6(this.options.container) : null;
+    let r = this.radius;
+    if (this.options.particles !== undefined) {
+      const fit = native.layoutFitBudget(this.options.particles, container, this.fluids.flatMap((f) => f.box));
+      r = fit.radius;
+    }
+    this.radius = r;
+    const parts = [];
+    const ranges = [];
+    let total = 0;
+    for (const f of this.fluids) {
+      let b = f.box;
+      if (container) {
+        b = [Math.max(b[0], container[0] + r), Math.max(b[1], container[1] + r), Math.max(b[2], container[2] + r),
+          Math.min(b[3], container[3] - r), Math.min(b[4], container[4] - r), Math.min(b[5], container[5] - r)];
+      }
+      const pts = native.layoutFillBox(b.slice(0, 3), b.slice(3), 2 * r);
+      if (pts.length === 0) throw new Error('addFluid: the fluid has no room; check its box and the container');
+      parts.push(pts);
+      ranges.push({ start: total, count: pts.length / 3 });
+      total += pts.length / 3;
+    }
+    const positions = new Float32Array(3 * total);
+    let off = 0;
+    for (const p of parts) {
+      positions.set(p, off);
+      off += p.length;
+    }
+    this.system = new ParticleSystem(total, r);
+    this.system.uploadArrays(positions, 0);
+    this.fluidSystems = this.fluids.map((f, i) => new FluidSystem(this.system, {
+      range: ranges[i], viscosity: f.viscosity, surfaceTension: f.surfaceTension, vorticity: f.vorticity,
+      adhesion: f.adhesion,
+    }));
+    this.walls = new PrimitiveSet(this.system);
+    if (container) {
+      const mn = container.slice(0, 3);
+      const mx = container.slice(3);
+      this.walls.addPlane([0, 1, 0], mn);
+      this.walls.addPlane([1, 0, 0], mn);
+      this.walls.addPlane([-1, 0, 0], mx);
+      this.walls.addPlane([0, 0, 1], mn);
+      this.walls.addPlane([0, 0, -1], mx);
+      if (this.options.closed) this.walls.addPlane([0, -1, 0], mx);
+    }
+    for (const d of this.colliderDefs) {
+      const muS = d.friction;
+      const muK = d.friction * 0.8;
+      if (d.kind === 'floor') this.walls.addPlane([0, 1, 0], [0, d.height, 0], { muS, muK });
+      if (d.kind === 'sphere') this.walls.addSphere(d.center, d.radius, { muS, muK });
+      if (d.kind === 'box') this.walls.addBox(d.center, d.half, { rotation: d.rotation, muS, muK });
+      if (d.kind === 'capsule') this.walls.addCapsule(d.a, d.b, d.radius, { muS, muK });
+    }
+    const substeps = def(this.options.substeps, Math.min(24, Math.ceil(4 * Math.max(1, 0.018 / r))));
+    this.simLoop = new SimLoop(this.system, {
+      substeps,
+      iterations: def(this.options.iterations, 2),
+      gravity: this.gravityValue,
+      materials: this.fluidSystems,
+      colliders: this.walls.count > 0 ? [this.walls] : [],
+    });
+  }
+  dispose() {
+    if (this.simLoop) this.simLoop.dispose();
+    for (const f of this.fluidSystems || []) f.dispose();
+    if (this.walls) this.walls.dispose();
+    if (this.system) this.system.dispose();
+  }
+}
+
+module.exports = {
+  native,
+  Context,
+  ParticleSystem,
+  HashGrid,
+  FluidSystem,
+  PrimitiveSet,
+  SoftbodySystem,
+  SimLoop,
+  Simulation,
+  getDefaultContext,
+};
