@@ -66,47 +66,79 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const float3 xi = pf_xyz(xi4);
-  const int qx = pf_query_coord(xi.x, S.ox, S.cell);
-  const int qy = pf_query_coord(xi.y, S.oy, S.cell);
-  const int qz = pf_query_coord(xi.z, S.oz, S.cell);
-  const bool dedup = S.H < 64u;  // below 64 buckets two stencil cells can share a bucket
+  const float rx = (xi.x - S.ox) / S.cell, ry = (xi.y - S.oy) / S.cell, rz = (xi.z - S.oz) / S.cell;
+  const int qx = pf_f2i_sat(floorf(rx));
+  const int qy = pf_f2i_sat(floorf(ry));
+  const int qz = pf_f2i_sat(floorf(rz));
+  // Below 64 buckets two stencil cells can share a bucket: walk with the
+  // reference's dedup and without culling.
+  const bool small_table = S.H < 64u;
+  // Distance from the particle to the faces of its own cell, to skip stencil
+  // cells that no particle within h can occupy (a wide margin covers the
+  // rounding of the cell assignment).
+  const float lim = F.h + 1e-4f * S.cell;
+  const float lim2 = lim * lim;
+  float lo[3], hi[3];
+  {
+    const float f[3] = {rx - floorf(rx), ry - floorf(ry), rz - floorf(rz)};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = f[a] * S.cell;
+      hi[a] = (1.0f - f[a]) * S.cell;
+    }
+  }
   uint32_t visited[27];
   uint32_t nvisited = 0;
   uint32_t count = 0;
+  uint32_t last_key = 0xffffffffu;
   bool over = false;
   float3 nsum = make_float3(0.f, 0.f, 0.f);
   const uint32_t n = S.n;
   uint32_t* __restrict__ nl = F.nl;
   const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
-  for (int dz = -1; dz <= 1; ++dz) {
-    for (int dy = -1; dy <= 1; ++dy) {
-      for (int dx = -1; dx <= 1; ++dx) {
+  for (int dz = -1; dz <= 1 && !over; ++dz) {
+    const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
+    for (int dy = -1; dy <= 1 && !over; ++dy) {
+      const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
+      for (int dx = -1; dx <= 1 && !over; ++dx) {
+        const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
         const uint32_t b =
             pf_morton((int)((uint32_t)qx + (uint32_t)dx), (int)((uint32_t)qy + (uint32_t)dy),
                       (int)((uint32_t)qz + (uint32_t)dz)) &
             S.hash_mask;
-        if (dedup) {
+        if (small_table) {
           bool seen = false;
           for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == b);
           visited[nvisited++] = b;
           if (seen) continue;
+        } else if ((ex * ex + ey * ey) + ez * ez > lim2) {
+          continue;
         }
+        const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
         const uint32_t st = S.start[b];
         const uint32_t en = S.start[b + 1];
-        for (uint32_t j = st; j < en; ++j) {
-          const float4 xj = xs[j];
-          const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-          const float r2 = pf_dot3(d, d);
-          if (r2 < F.h2) {
-            if (count < PF_MAX_NEIGHBORS) {
-              nl[(size_t)count * n + s] = j;
-              ++count;
-              if (want_normal) {
-                const float3 g = pf_spiky_grad(d, F.h, F.spiky);
-                nsum = pf_add3(nsum, pf_mul3(g, neighbor_volume(S, F, p, j)));
+        for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
+          float4 xj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xj[u] = (j0 + u < en) ? xs[j0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t j = j0 + u;
+            const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+            const float r2 = pf_dot3(d, d);
+            if (j < en && r2 < F.h2 && !over) {
+              if (count < PF_MAX_NEIGHBORS) {
+                nl[(size_t)count * n + s] = j;
+                ++count;
+                last_key = (stencil << 27) | j;
+                if (want_normal) {
+                  const float3 g = pf_spiky_grad(d, F.h, F.spiky);
+                  nsum = pf_add3(nsum, pf_mul3(g, neighbor_volume(S, F, p, j)));
+                }
+              } else {
+                // The list is full and another neighbour exists: it overflows,
+                // nothing later in the walk can enter it.
+                over = true;
               }
-            } else {
-              over = true;
             }
           }
         }
@@ -114,6 +146,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     }
   }
   F.nl_count[s] = count;
+  // Walk position of the last kept neighbour of a truncated list; the
+  // surface-tension pass uses it to know which partners this list dropped.
+  F.nl_last[s] = (over && !small_table) ? last_key : 0xffffffffu;
   if (want_normal) F.normal[s] = make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
   // Per-substep flags live in the half of the substep's pre-sort parity.
   if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
@@ -203,10 +238,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
         }
       } else if (idj < id && idj >= F.start) {
         // The pair belongs to j's thread in the reference; it exists there iff
-        // this slot is in j's (possibly truncated) list.
-        bool listed = F.nl_count[j] < PF_MAX_NEIGHBORS;
+        // this slot is in j's (possibly truncated) list. Lists keep their first
+        // 64 neighbours in walk order, so compare this slot's walk key in j's
+        // walk with the key of j's last kept neighbour.
+        const uint32_t last = F.nl_last[j];
+        bool listed = last == 0xffffffffu;
         if (!listed) {
-          for (uint32_t m = 0; m < PF_MAX_NEIGHBORS; ++m) listed |= (nl[(size_t)m * n + j] == s);
+          if (S.H < 64u) {
+            for (uint32_t m = 0; m < PF_MAX_NEIGHBORS; ++m) listed |= (nl[(size_t)m * n + j] == s);
+          } else {
+            const float4 xj4 = xs[j];
+            const int ddx = pf_query_coord(xi.x, S.ox, S.cell) - pf_query_coord(xj4.x, S.ox, S.cell);
+            const int ddy = pf_query_coord(xi.y, S.oy, S.cell) - pf_query_coord(xj4.y, S.oy, S.cell);
+            const int ddz = pf_query_coord(xi.z, S.oz, S.cell) - pf_query_coord(xj4.z, S.oz, S.cell);
+            const uint32_t key = ((uint32_t)((ddz + 1) * 9 + (ddy + 1) * 3 + (ddx + 1)) << 27) | s;
+            listed = key <= last;
+          }
         }
         float3 dv;
         if (listed &&
@@ -262,16 +309,29 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
   const uint32_t* __restrict__ nl = F.nl;
   float rho = 0.0f, gss = 0.0f;
   float3 gs = make_float3(0.f, 0.f, 0.f);
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = nl[(size_t)k * n + s];
-    const float4 xj = xs[j];
-    const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-    const float mj = neighbor_mass(S, F, p, j);
-    const float wj = S.w[p][j];
-    rho = rho + mj * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
-    const float3 g = pf_mul3(pf_spiky_grad(d, F.h, F.spiky), mj);
-    gs = pf_add3(gs, g);
-    gss = gss + pf_dot3(g, g) * wj;
+  // Four neighbours' loads in flight, then the sums in list order.
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    uint32_t jj[4];
+    float4 xj[4];
+    float mj[4], wj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xj[u] = xs[jj[u]];
+      mj[u] = neighbor_mass(S, F, p, jj[u]);
+      wj[u] = S.w[p][jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u < cnt) {
+        const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+        rho = rho + mj[u] * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
+        const float3 g = pf_mul3(pf_spiky_grad(d, F.h, F.spiky), mj[u]);
+        gs = pf_add3(gs, g);
+        gss = gss + pf_dot3(g, g) * wj[u];
+      }
+    }
   }
   S.rho[p][s] = rho;
   const float inv_rho0 = 1.0f / F.rho0;
@@ -308,12 +368,25 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     const uint32_t* __restrict__ nl = F.nl;
     const float li = xi4.w;
     float3 sum = make_float3(0.f, 0.f, 0.f);
-    for (uint32_t k = 0; k < cnt; ++k) {
-      const uint32_t j = nl[(size_t)k * n + s];
-      const float4 xj = xs[j];
-      const float3 g = pf_spiky_grad(make_float3(xn.x - xj.x, xn.y - xj.y, xn.z - xj.z), F.h, F.spiky);
-      const float sc = (li + xj.w) * neighbor_mass(S, F, p, j);
-      sum = pf_add3(sum, pf_mul3(g, sc));
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t jj[4];
+      float4 xj[4];
+      float mj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = xs[jj[u]];
+        mj[u] = neighbor_mass(S, F, p, jj[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u < cnt) {
+          const float3 g = pf_spiky_grad(make_float3(xn.x - xj[u].x, xn.y - xj[u].y, xn.z - xj[u].z), F.h, F.spiky);
+          const float sc = (li + xj[u].w) * mj[u];
+          sum = pf_add3(sum, pf_mul3(g, sc));
+        }
+      }
     }
     xn = make_float3(xn.x + (sum.x / F.rho0) * w, xn.y + (sum.y / F.rho0) * w,
                      xn.z + (sum.z / F.rho0) * w);
@@ -409,14 +482,27 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
   const uint32_t* __restrict__ nl = F.nl;
   float3 curl = make_float3(0.f, 0.f, 0.f);
   float3 sm = make_float3(0.f, 0.f, 0.f);
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = nl[(size_t)k * n + s];
-    const float4 xj = X[j];
-    const float3 vj = pf_xyz(V[j]);
-    const float3 off = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-    const float vol = neighbor_volume(S, F, p, j);
-    if (vort) curl = pf_add3(curl, pf_mul3(pf_cross3(pf_sub3(vi, vj), pf_spiky_grad(off, F.h, F.spiky)), vol));
-    if (visc) sm = pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol));
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    uint32_t jj[4];
+    float4 xj[4], vj4[4];
+    float vol[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xj[u] = X[jj[u]];
+      vj4[u] = V[jj[u]];
+      vol[u] = neighbor_volume(S, F, p, jj[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u < cnt) {
+        const float3 vj = pf_xyz(vj4[u]);
+        const float3 off = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+        if (vort) curl = pf_add3(curl, pf_mul3(pf_cross3(pf_sub3(vi, vj), pf_spiky_grad(off, F.h, F.spiky)), vol[u]));
+        if (visc) sm = pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol[u]));
+      }
+    }
   }
   if (vort) F.omega[s] = make_float4(curl.x, curl.y, curl.z, pf_len3(curl));
   if (visc) F.dv[s] = make_float4(sm.x * F.visc, sm.y * F.visc, sm.z * F.visc, 0.0f);
@@ -446,12 +532,26 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
       const uint32_t cnt = F.nl_count[s];
       const uint32_t* __restrict__ nl = F.nl;
       float3 eta = make_float3(0.f, 0.f, 0.f);
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t j = nl[(size_t)k * n + s];
-        if (S.has_boundary && S.bvol[p][j] > 0.0f) continue;
-        const float4 xj = X[j];
-        const float3 g = pf_spiky_grad(make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z), F.h, F.spiky);
-        eta = pf_add3(eta, pf_mul3(g, (F.omega[j].w - oi) * F.pv));
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t jj[4];
+        float4 xj[4];
+        float oj[4];
+        bool skip[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = X[jj[u]];
+          oj[u] = F.omega[jj[u]].w;
+          skip[u] = (k0 + u >= cnt) || (S.has_boundary && S.bvol[p][jj[u]] > 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (!skip[u]) {
+            const float3 g = pf_spiky_grad(make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z), F.h, F.spiky);
+            eta = pf_add3(eta, pf_mul3(g, (oj[u] - oi) * F.pv));
+          }
+        }
       }
       const float el = pf_len3(eta);
       if (el >= 1e-12f) {

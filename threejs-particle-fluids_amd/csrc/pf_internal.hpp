@@ -68,6 +68,7 @@ struct FluidDev {
   float rea_scale, rea_inv;         // Accumulator(10) for boundary reaction
   uint32_t* nl;        // [64 * n] neighbour slots, column-major k*n + slot
   uint32_t* nl_count;  // [n]
+  uint32_t* nl_last;   // [n] walk key of the last kept neighbour when the list overflowed, else ~0
   float4* normal;      // [n] colour-field normal
   float4* omega;       // [n] vorticity (xyz) and |omega| (w)
   float4* dv;          // [n] XSPH velocity change

@@ -146,6 +146,7 @@ struct pf_fluid_s {
   int slot = -1;  // index in SceneDev::fluid
   uint32_t* nl = nullptr;
   uint32_t* nl_count = nullptr;
+  uint32_t* nl_last = nullptr;
   float4* normal = nullptr;
   float4* omega = nullptr;
   float4* dv = nullptr;

@@ -354,8 +354,8 @@ int pf_fluid_destroy(pf_fluid* f) {
   PF_API_BEGIN
   if (!f || f->disposed) return PF_OK;
   f->disposed = true;
-  dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv); dfree(f->reaction);
-  f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr; f->reaction = nullptr;
+  dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv); dfree(f->reaction);
+  f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr; f->reaction = nullptr;
   PF_API_END
 }
 
@@ -781,6 +781,7 @@ static void fluid_write_params(pf_fluid* f) {
   }
   F.nl = f->nl;
   F.nl_count = f->nl_count;
+  F.nl_last = f->nl_last;
   F.normal = f->normal;
   F.omega = f->omega;
   F.dv = f->dv;
@@ -901,6 +902,7 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     f->loop = L.get();
     f->nl = dalloc<uint32_t>((size_t)PF_MAX_NEIGHBORS * n);
     f->nl_count = dalloc<uint32_t>(n);
+    f->nl_last = dalloc<uint32_t>(n);
     if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
@@ -1120,8 +1122,8 @@ int pf_loop_destroy(pf_loop* L) {
   dfree(L->tiles); dfree(L->stiles); dfree(L->btiles);
   for (auto* f : L->fluids) {
     f->loop = nullptr;
-    dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv);
-    f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr;
+    dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv);
+    f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr;
   }
   if (L->prims) L->prims->loop = nullptr;
   if (L->soft) L->soft->loop = nullptr;
