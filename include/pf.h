@@ -151,6 +151,12 @@ int pf_fluid_readback_overflow(pf_fluid* fluid, int* overflow);
 /* mass, smoothing radius, particle spacing, rest density (FluidSystem.ts:75-84) */
 int pf_fluid_info(pf_fluid* fluid, double* mass, double* smoothing_radius,
                   double* particle_spacing, double* rest_density);
+/* No counterpart in the reference (a tuning knob of this implementation):
+ * the most particles a 256-slot tile may stage in LDS for the neighbour
+ * passes (at most 2048, the default). Tiles whose neighbour buckets hold
+ * more gather from global memory; 0 makes every tile do so. Results do not
+ * depend on it. */
+int pf_fluid_set_tile_union_limit(pf_fluid* fluid, uint32_t max_particles);
 int pf_fluid_destroy(pf_fluid* fluid);
 
 /* ---- PrimitiveSet (src/core/collision/PrimitiveSet.ts:121-518) -----------

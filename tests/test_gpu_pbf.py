@@ -84,6 +84,22 @@ def test_graph_replay_matches_eager():
             assert np.array_equal(ga[key], gb[key]), (k, key)
 
 
+@pytest.mark.parametrize("limit", [0, 600])
+def test_tile_union_limit_gives_identical_results(limit):
+    # 0: every tile gathers neighbours from global memory; 600: tiles mix
+    # LDS-staged and global gathers. Both must equal the default (all staged).
+    print(f"[pbf] tile union limit {limit}", flush=True)
+    a = scenes.build_dam_break(5000, substeps=2, iterations=4)
+    b = scenes.build_dam_break(5000, substeps=2, iterations=4)
+    b["fluid"].setTileUnionLimit(limit)
+    for k in range(8):
+        a["loop"].step(1.0 / 60.0)
+        b["loop"].step(1.0 / 60.0)
+        ga, gb = gpu_state(a), gpu_state(b)
+        for key in ga:
+            assert np.array_equal(ga[key], gb[key]), (k, key)
+
+
 def test_odd_substeps_and_three_iterations():
     # descriptor variants swap every step when the flip count is odd
     sc = scenes.build_dam_break(5000, substeps=3, iterations=3)

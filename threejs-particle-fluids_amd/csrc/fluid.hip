@@ -51,107 +51,399 @@ static __device__ __forceinline__ void flag_set(const SceneDev& S, int parity, i
 }
 
 // ---------------------------------------------------------------------------
+// Block helpers
+// ---------------------------------------------------------------------------
+static __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  return v;
+}
+
+// Exclusive prefix of one value per thread over the block; *total gets the
+// sum. sh holds PF_BLOCK / 64 words. Every thread of the block must call it.
+static __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh,
+                                                                uint32_t* total) {
+  const uint32_t inc = wave_inclusive_scan(v);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 63) sh[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < PF_BLOCK / 64; ++w) {
+    const uint32_t x = sh[w];
+    if (w < wave) before += x;
+    all += x;
+  }
+  *total = all;
+  __syncthreads();
+  return before + inc - v;
+}
+
+// Hash-set slot of a bucket (Fibonacci hashing into PF_TU_HASH slots).
+static __device__ __forceinline__ uint32_t tu_hash(uint32_t b) {
+  return (b * 0x9E3779B1u) >> (32 - 10);
+}
+static_assert(PF_TU_HASH == 1024, "tu_hash assumes 1024 slots");
+
+static __device__ __forceinline__ float3 sel3(bool ok, float3 a, float3 b) {
+  return make_float3(ok ? a.x : b.x, ok ? a.y : b.y, ok ? a.z : b.z);
+}
+
+// Colour-field normal n_i = h * sum_j V_j grad W (cohesion.ts:25-39) over a
+// finished global-slot list, in list order.
+static __device__ __forceinline__ float4 color_normal_global(const SceneDev& S, const FluidDev& F, int p,
+                                                             const float4* __restrict__ xs, float3 xi,
+                                                             uint32_t s, uint32_t count) {
+  const uint32_t n = S.n;
+  float3 nsum = make_float3(0.f, 0.f, 0.f);
+  for (uint32_t k0 = 0; k0 < count; k0 += 4) {
+    uint32_t jj[4];
+    float4 xj[4];
+    float vol[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < count) ? F.nl[(size_t)(k0 + u) * n + s] : s;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xj[u] = xs[jj[u]];
+      vol[u] = neighbor_volume(S, F, p, jj[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+      nsum = sel3(k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
+    }
+  }
+  return make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
+}
+
+// Sorted slots of the tile's union entries threadIdx.x + i * PF_BLOCK, all
+// loads in flight at once; entries past U read slot 0 (callers skip them).
+#define PF_TU_PER (PF_TU_MAX / PF_BLOCK)
+static __device__ __forceinline__ void tu_load_slots(const uint32_t* __restrict__ slots, uint32_t U,
+                                                     uint32_t js[PF_TU_PER]) {
+#pragma unroll
+  for (int i = 0; i < PF_TU_PER; ++i) {
+    const uint32_t u = threadIdx.x + i * PF_BLOCK;
+    js[i] = u < U ? slots[u] : 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Neighbour list (+ colour-field normal when surface tension is on)
+//
+// Each 256-slot tile first gathers the set of buckets its fluid particles
+// walk (all 27 stencil buckets of each distinct query cell) in an LDS hash
+// set, stages those buckets' particles in LDS in one pass (the tile union),
+// and walks them there. The list keeps the reference's order and cap (walk
+// order dz->dy->dx, slot order inside a bucket, first 64 within h) and stores
+// 16-bit union indices, two per word. The colour-field normal is summed over
+// the finished list afterwards, in list order, so the divergent walk carries
+// only the distance test. A tile whose union does not fit (more than PF_TU_MAX
+// particles or PF_TU_HASH_LIMIT buckets), and every tile of a table smaller
+// than 64 buckets (where stencil buckets can coincide), walks global memory
+// and stores global slots instead. Both walks give the same list.
 // ---------------------------------------------------------------------------
 template <bool NORMAL>
 __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c) {
+  __shared__ uint32_t h_key[PF_TU_HASH];    // bucket + 1, 0 = empty
+  __shared__ uint32_t h_off[PF_TU_HASH];    // (union offset << 16) | bucket count
+  __shared__ uint32_t h_start[PF_TU_HASH];  // first sorted slot of the bucket
+  // The staged union; before staging the same words hold each thread's query
+  // cell and activity for the leader test.
+  __shared__ float4 u_x[PF_TU_MAX];
+  int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(&u_x[0]);
+  uint32_t* a_sh = reinterpret_cast<uint32_t*>(&u_x[0]) + 3 * PF_BLOCK;
+  __shared__ uint32_t ctl[4 + PF_BLOCK / 64];  // distinct buckets, give-up flag, -, -, scan
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  if (!in_range(S.id[p][s], F.start, F.count)) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t s = t.first + tid;
+  const uint32_t tile = t.first / PF_BLOCK;
+  const uint32_t n = S.n;
+  const bool active = s < n && in_range(S.id[p][s], F.start, F.count);
   const float4* __restrict__ xs = S.xs[c];
-  const float4 xi4 = xs[s];
-  const float3 xi = pf_xyz(xi4);
-  const float rx = (xi.x - S.ox) / S.cell, ry = (xi.y - S.oy) / S.cell, rz = (xi.z - S.oz) / S.cell;
-  const int qx = pf_f2i_sat(floorf(rx));
-  const int qy = pf_f2i_sat(floorf(ry));
-  const int qz = pf_f2i_sat(floorf(rz));
-  // Below 64 buckets two stencil cells can share a bucket: walk with the
-  // reference's dedup and without culling.
-  const bool small_table = S.H < 64u;
+  float3 xi = make_float3(0.f, 0.f, 0.f);
+  int q[3] = {0, 0, 0};
   // Distance from the particle to the faces of its own cell, to skip stencil
   // cells that no particle within h can occupy (a wide margin covers the
   // rounding of the cell assignment).
-  const float lim = F.h + 1e-4f * S.cell;
-  const float lim2 = lim * lim;
-  float lo[3], hi[3];
-  {
-    const float f[3] = {rx - floorf(rx), ry - floorf(ry), rz - floorf(rz)};
+  float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
+  if (active) {
+    xi = pf_xyz(xs[s]);
+    const float r[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
-      lo[a] = f[a] * S.cell;
-      hi[a] = (1.0f - f[a]) * S.cell;
+      const float fl = floorf(r[a]);
+      q[a] = pf_f2i_sat(fl);
+      lo[a] = (r[a] - fl) * S.cell;
+      hi[a] = (1.0f - (r[a] - fl)) * S.cell;
     }
   }
-  uint32_t visited[27];
-  uint32_t nvisited = 0;
+  const float lim = F.h + 1e-4f * S.cell;
+  const float lim2 = lim * lim;
+  const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
   uint32_t count = 0;
   uint32_t last_key = 0xffffffffu;
   bool over = false;
-  float3 nsum = make_float3(0.f, 0.f, 0.f);
-  const uint32_t n = S.n;
-  uint32_t* __restrict__ nl = F.nl;
-  const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
-  for (int dz = -1; dz <= 1 && !over; ++dz) {
-    const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
-    for (int dy = -1; dy <= 1 && !over; ++dy) {
-      const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
-      for (int dx = -1; dx <= 1 && !over; ++dx) {
-        const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
-        const uint32_t b =
-            pf_morton((int)((uint32_t)qx + (uint32_t)dx), (int)((uint32_t)qy + (uint32_t)dy),
-                      (int)((uint32_t)qz + (uint32_t)dz)) &
-            S.hash_mask;
-        if (small_table) {
+
+  // Tables below 64 buckets: global walk with the reference's bucket dedup.
+  if (S.H < 64u) {
+    if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
+    if (!active) return;
+    uint32_t visited[27];
+    uint32_t nvisited = 0;
+    for (int dz = -1; dz <= 1 && !over; ++dz)
+      for (int dy = -1; dy <= 1 && !over; ++dy)
+        for (int dx = -1; dx <= 1 && !over; ++dx) {
+          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
           bool seen = false;
           for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == b);
           visited[nvisited++] = b;
           if (seen) continue;
-        } else if ((ex * ex + ey * ey) + ez * ez > lim2) {
-          continue;
-        }
-        const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
-        const uint32_t st = S.start[b];
-        const uint32_t en = S.start[b + 1];
-        for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
-          float4 xj[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) xj[u] = (j0 + u < en) ? xs[j0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t j = j0 + u;
-            const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-            const float r2 = pf_dot3(d, d);
-            if (j < en && r2 < F.h2 && !over) {
+          for (uint32_t j = S.start[b]; j < S.start[b + 1] && !over; ++j) {
+            const float4 xj = xs[j];
+            const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+            if (pf_dot3(d, d) < F.h2) {
               if (count < PF_MAX_NEIGHBORS) {
-                nl[(size_t)count * n + s] = j;
+                F.nl[(size_t)count * n + s] = j;
                 ++count;
-                last_key = (stencil << 27) | j;
-                if (want_normal) {
-                  const float3 g = pf_spiky_grad(d, F.h, F.spiky);
-                  nsum = pf_add3(nsum, pf_mul3(g, neighbor_volume(S, F, p, j)));
-                }
               } else {
-                // The list is full and another neighbour exists: it overflows,
-                // nothing later in the walk can enter it.
                 over = true;
+              }
+            }
+          }
+        }
+    F.nl_count[s] = count;
+    F.nl_last[s] = 0xffffffffu;
+    if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
+    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+    return;
+  }
+
+  // 1. Bucket set: the first particle of each run of equal query cells inserts
+  // its 27 stencil buckets.
+  for (uint32_t i = tid; i < PF_TU_HASH; i += PF_BLOCK) h_key[i] = 0u;
+  if (tid < 4) ctl[tid] = 0u;
+  q_sh[0][tid] = q[0];
+  q_sh[1][tid] = q[1];
+  q_sh[2][tid] = q[2];
+  a_sh[tid] = active ? 1 : 0;
+  __syncthreads();
+  const bool leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] ||
+                                 q_sh[1][tid - 1] != q[1] || q_sh[2][tid - 1] != q[2]);
+  if (leader) {
+    volatile uint32_t* give_up = &ctl[1];
+    for (int d = 0; d < 27 && !*give_up; ++d) {
+      const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+      const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                   (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+      uint32_t h = tu_hash(b);
+      bool placed = false;
+      for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
+        const uint32_t prev = atomicCAS(&h_key[h], 0u, b + 1u);
+        if (prev == 0u) {
+          if (atomicAdd(&ctl[0], 1u) + 1u > PF_TU_HASH_LIMIT) *give_up = 1u;
+          placed = true;
+          break;
+        }
+        if (prev == b + 1u) {
+          placed = true;
+          break;
+        }
+        h = (h + 1u) & (PF_TU_HASH - 1u);
+      }
+      if (!placed) *give_up = 1u;
+    }
+  }
+  __syncthreads();
+  bool global_walk = ctl[1] != 0u;
+  uint32_t U = 0;
+  if (!global_walk) {
+    // 2. Union layout: buckets in hash-slot order, offsets by a block scan.
+    constexpr int PER = PF_TU_HASH / PF_BLOCK;
+    uint32_t cn[PER], st[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t key = h_key[tid * PER + i];
+      cn[i] = 0u;
+      st[i] = 0u;
+      if (key) {
+        st[i] = S.start[key - 1u];
+        cn[i] = S.start[key] - st[i];
+      }
+      sum += cn[i];
+    }
+    uint32_t off = block_exclusive_scan(sum, ctl + 4, &U);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      h_off[tid * PER + i] = (min(off, 0xffffu) << 16) | min(cn[i], 0xffffu);
+      h_start[tid * PER + i] = st[i];
+      off += cn[i];
+    }
+    global_walk = U > F.tu_limit;
+    __syncthreads();
+  }
+
+  if (global_walk) {
+    if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
+    if (!active) return;
+    uint32_t* __restrict__ nl = F.nl;
+    for (int dz = -1; dz <= 1 && !over; ++dz) {
+      const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
+      for (int dy = -1; dy <= 1 && !over; ++dy) {
+        const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
+        for (int dx = -1; dx <= 1 && !over; ++dx) {
+          const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
+          if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
+          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+          const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+          const uint32_t st = S.start[b];
+          const uint32_t en = S.start[b + 1];
+          for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
+            float4 xj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xj[u] = (j0 + u < en) ? xs[j0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t j = j0 + u;
+              const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+              if (j < en && pf_dot3(d, d) < F.h2 && !over) {
+                if (count < PF_MAX_NEIGHBORS) {
+                  nl[(size_t)count * n + s] = j;
+                  ++count;
+                  last_key = (stencil << 27) | j;
+                } else {
+                  // The list is full and another neighbour exists: it
+                  // overflows, nothing later in the walk can enter it.
+                  over = true;
+                }
               }
             }
           }
         }
       }
     }
+    F.nl_count[s] = count;
+    // Walk position of the last kept neighbour of a truncated list; the
+    // surface-tension pass uses it to know which partners this list dropped.
+    F.nl_last[s] = over ? last_key : 0xffffffffu;
+    if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
+    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+    return;
   }
-  F.nl_count[s] = count;
-  // Walk position of the last kept neighbour of a truncated list; the
-  // surface-tension pass uses it to know which partners this list dropped.
-  F.nl_last[s] = (over && !small_table) ? last_key : 0xffffffffu;
-  if (want_normal) F.normal[s] = make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
-  // Per-substep flags live in the half of the substep's pre-sort parity.
-  if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+
+  // 3. Stage the union: entry u belongs to the last hash slot whose offset is
+  // not above u.
+  {
+    uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+    uint32_t js[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = tid + i * PF_BLOCK;
+      // fixed-depth search over the PF_TU_HASH offsets
+      uint32_t a = 0;
+#pragma unroll
+      for (uint32_t step = PF_TU_HASH / 2; step; step >>= 1)
+        if ((h_off[a + step] >> 16) <= u) a += step;
+      js[i] = u < U ? h_start[a] + (u - (h_off[a] >> 16)) : 0u;
+    }
+    float4 xv[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xs[js[i]];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = tid + i * PF_BLOCK;
+      if (u < U) {
+        u_x[u] = xv[i];
+        slots[u] = js[i];
+      }
+    }
+  }
+  __syncthreads();
+
+  // 4. Walk the union. Completed entry pairs go straight to their list row.
+  uint32_t* __restrict__ nl16 = F.nl16;
+  if (active) {
+    uint32_t pend = 0;
+    for (int dz = -1; dz <= 1 && !over; ++dz) {
+      const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
+      for (int dy = -1; dy <= 1 && !over; ++dy) {
+        const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
+        for (int dx = -1; dx <= 1 && !over; ++dx) {
+          const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
+          if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
+          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+          uint32_t h = tu_hash(b);
+          for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
+          if (h_key[h] != b + 1u) continue;  // unreachable: the cell's leader inserted b
+          const uint32_t ho = h_off[h];
+          const uint32_t off = ho >> 16, cn = ho & 0xffffu;
+          const uint32_t st = h_start[h];
+          const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+          for (uint32_t i0 = 0; i0 < cn && !over; i0 += 4) {
+            float4 xj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xj[u] = (i0 + u < cn) ? u_x[off + i0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t i = i0 + u;
+              const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+              if (i < cn && pf_dot3(d, d) < F.h2 && !over) {
+                if (count < PF_MAX_NEIGHBORS) {
+                  const uint32_t local = off + i;
+                  if (count & 1u) nl16[(size_t)(count >> 1) * n + s] = pend | (local << 16);
+                  else pend = local;
+                  ++count;
+                  last_key = (stencil << 27) | (st + i);
+                } else {
+                  over = true;
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (count & 1u) nl16[(size_t)(count >> 1) * n + s] = pend;
+    F.nl_count[s] = count;
+    F.nl_last[s] = over ? last_key : 0xffffffffu;
+    if (want_normal) {
+      // Colour-field normal over the finished list, in list order.
+      const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+      float3 nsum = make_float3(0.f, 0.f, 0.f);
+      for (uint32_t k0 = 0; k0 < count; k0 += 4) {
+        const uint32_t w0 = nl16[(size_t)(k0 >> 1) * n + s];
+        const uint32_t w1 = nl16[(size_t)min((k0 >> 1) + 1u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 1)) * n + s];
+        const uint32_t idx[4] = {w0 & (PF_TU_MAX - 1), (w0 >> 16) & (PF_TU_MAX - 1), w1 & (PF_TU_MAX - 1),
+                                 (w1 >> 16) & (PF_TU_MAX - 1)};
+        float4 xj[4];
+        float vol[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = u_x[idx[u]];
+          vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+          nsum = sel3(k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
+        }
+      }
+      F.normal[s] = make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
+    }
+    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+  }
+  if (tid == 0) F.tu_size[tile] = U;
 }
 
 // ---------------------------------------------------------------------------
@@ -225,8 +517,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     rhoi = S.rho[p][s];
   }
   const uint32_t fend = F.start + F.count;
+  const uint32_t U = F.tu_size[t.first / PF_BLOCK];
+  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = nl[(size_t)k * n + s];
+    const uint32_t j = (U == PF_TU_GLOBAL)
+                           ? nl[(size_t)k * n + s]
+                           : uslots[(F.nl16[(size_t)(k >> 1) * n + s] >> ((k & 1u) * 16)) & (PF_TU_MAX - 1)];
     if (st_on) {
       const uint32_t idj = S.id[p][j];
       if (idj > id && idj < fend) {
@@ -291,46 +587,147 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Neighbour-list cursors. Both walk the list four entries at a time with the
+// next four entries' list words already loaded; rows are clamped so those
+// loads stay in bounds past the end of the list, and entries past the count
+// resolve to a harmless index whose terms the callers discard.
+// ---------------------------------------------------------------------------
+struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
+  const uint32_t* __restrict__ col;
+  size_t n;
+  uint32_t w0, w1;
+  __device__ __forceinline__ Nl16Cursor(const uint32_t* nl16, uint32_t nn, uint32_t s)
+      : col(nl16 + s), n(nn) {
+    w0 = col[0];
+    w1 = col[n];
+  }
+  // Union indices of entries k0..k0+3; prefetches those of k0+4..k0+7.
+  __device__ __forceinline__ void next(uint32_t k0, uint32_t idx[4]) {
+    const uint32_t c0 = w0, c1 = w1;
+    const uint32_t r = min((k0 >> 1) + 2u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 2));
+    w0 = col[(size_t)r * n];
+    w1 = col[(size_t)(r + 1) * n];
+    idx[0] = c0 & (PF_TU_MAX - 1);
+    idx[1] = (c0 >> 16) & (PF_TU_MAX - 1);
+    idx[2] = c1 & (PF_TU_MAX - 1);
+    idx[3] = (c1 >> 16) & (PF_TU_MAX - 1);
+  }
+};
+
+struct NlCursor {  // global-gather tile: sorted slots
+  const uint32_t* __restrict__ col;
+  size_t n;
+  uint32_t self, cnt;
+  uint32_t w[4];
+  __device__ __forceinline__ NlCursor(const uint32_t* nl, uint32_t nn, uint32_t s, uint32_t count)
+      : col(nl + s), n(nn), self(s), cnt(count) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)u * n];
+  }
+  // Slots of entries k0..k0+3 (the own slot past the count).
+  __device__ __forceinline__ void next(uint32_t k0, uint32_t jj[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? w[u] : self;
+    const uint32_t r = min(k0 + 4u, (uint32_t)(PF_MAX_NEIGHBORS - 4));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)(r + u) * n];
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Density and lambda; lambda is written to xs[c].w for the correction pass
 // ---------------------------------------------------------------------------
+
+// One neighbour's terms of the density and lambda sums (lambda.ts:40-58),
+// added when ok.
+static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, float3 xi, float4 xj,
+                                                   float mj, float wj, float& rho, float3& gs,
+                                                   float& gss) {
+  const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+  const float r1 = rho + mj * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
+  const float3 g = pf_mul3(pf_spiky_grad(d, F.h, F.spiky), mj);
+  const float3 gs1 = pf_add3(gs, g);
+  const float gss1 = gss + pf_dot3(g, g) * wj;
+  rho = ok ? r1 : rho;
+  gs = sel3(ok, gs1, gs);
+  gss = ok ? gss1 : gss;
+}
+
 __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, uint32_t ntiles,
                                                      int fi, int p, int c) {
+  __shared__ float4 u_x[PF_TU_MAX];  // position, neighbour mass
+  __shared__ float u_w[PF_TU_MAX];   // inverse mass
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
   const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  if (!in_range(S.id[p][s], F.start, F.count)) return;
-  float4* __restrict__ xs = S.xs[c];
-  const float3 xi = pf_xyz(xs[s]);
+  const uint32_t tile = t.first / PF_BLOCK;
   const uint32_t n = S.n;
+  float4* __restrict__ xs = S.xs[c];
+  const uint32_t U = F.tu_size[tile];
+  const bool staged = U != PF_TU_GLOBAL;
+  if (staged) {
+    uint32_t js[PF_TU_PER];
+    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+    float4 xv[PF_TU_PER];
+    float mv[PF_TU_PER], wv[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      xv[i] = xs[js[i]];
+      mv[i] = S.mass[js[i]];
+      wv[i] = S.w[p][js[i]];
+    }
+    if (S.has_boundary) {
+#pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = threadIdx.x + i * PF_BLOCK;
+      if (u < U) {
+        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, mv[i]);
+        u_w[u] = wv[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (s >= n || !in_range(S.id[p][s], F.start, F.count)) return;
+  const float3 xi = pf_xyz(xs[s]);
   const uint32_t cnt = F.nl_count[s];
-  const uint32_t* __restrict__ nl = F.nl;
   float rho = 0.0f, gss = 0.0f;
   float3 gs = make_float3(0.f, 0.f, 0.f);
   // Four neighbours' loads in flight, then the sums in list order.
-  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-    uint32_t jj[4];
-    float4 xj[4];
-    float mj[4], wj[4];
+  if (staged) {
+    Nl16Cursor cur(F.nl16, n, s);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t idx[4];
+      cur.next(k0, idx);
+      float4 xj[4];
+      float wj[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      xj[u] = xs[jj[u]];
-      mj[u] = neighbor_mass(S, F, p, jj[u]);
-      wj[u] = S.w[p][jj[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (k0 + u < cnt) {
-        const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-        rho = rho + mj[u] * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
-        const float3 g = pf_mul3(pf_spiky_grad(d, F.h, F.spiky), mj[u]);
-        gs = pf_add3(gs, g);
-        gss = gss + pf_dot3(g, g) * wj[u];
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = u_x[idx[u]];
+        wj[u] = u_w[idx[u]];
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lambda_term(F, k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
+    }
+  } else {
+    NlCursor cur(F.nl, n, s, cnt);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t jj[4];
+      cur.next(k0, jj);
+      float4 xj[4];
+      float mj[4], wj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = xs[jj[u]];
+        mj[u] = neighbor_mass(S, F, p, jj[u]);
+        wj[u] = S.w[p][jj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lambda_term(F, k0 + u < cnt, xi, xj[u], mj[u], wj[u], rho, gs, gss);
     }
   }
   S.rho[p][s] = rho;
@@ -348,44 +745,91 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
 // ---------------------------------------------------------------------------
 // Position correction (+ container contact, accumulator apply, advect)
 // ---------------------------------------------------------------------------
+
+// One neighbour's term of the correction sum (positionDelta.ts:44-52), added when ok.
+static __device__ __forceinline__ void delta_term(const FluidDev& F, bool ok, float3 xn, float li,
+                                                  float4 xj, float mj, float3& sum) {
+  const float3 g = pf_spiky_grad(make_float3(xn.x - xj.x, xn.y - xj.y, xn.z - xj.z), F.h, F.spiky);
+  sum = sel3(ok, pf_add3(sum, pf_mul3(g, (li + xj.w) * mj)), sum);
+}
+
 template <bool WALLS, bool ADVECT>
 __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, uint32_t ntiles,
                                                     int fi, int p, int c, int first_iteration,
                                                     int substep) {
+  __shared__ float4 u_x[PF_TU_MAX];  // position, lambda
+  __shared__ float u_m[PF_TU_MAX];   // neighbour mass
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
   const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
+  const uint32_t tile = t.first / PF_BLOCK;
+  const uint32_t n = S.n;
   const float4* __restrict__ xs = S.xs[c];
+  const uint32_t U = F.tu_size[tile];
+  const bool staged = U != PF_TU_GLOBAL;
+  if (staged) {
+    uint32_t js[PF_TU_PER];
+    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+    float4 xv[PF_TU_PER];
+    float mv[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      xv[i] = xs[js[i]];
+      mv[i] = S.mass[js[i]];
+    }
+    if (S.has_boundary) {
+#pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = threadIdx.x + i * PF_BLOCK;
+      if (u < U) {
+        u_x[u] = xv[i];
+        u_m[u] = mv[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (s >= n) return;
   const float4 xi4 = xs[s];
   float3 xn = pf_xyz(xi4);
   const float w = S.w[p][s];
   if (in_range(S.id[p][s], F.start, F.count)) {
-    const uint32_t n = S.n;
     const uint32_t cnt = F.nl_count[s];
-    const uint32_t* __restrict__ nl = F.nl;
     const float li = xi4.w;
     float3 sum = make_float3(0.f, 0.f, 0.f);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-      uint32_t jj[4];
-      float4 xj[4];
-      float mj[4];
+    if (staged) {
+      Nl16Cursor cur(F.nl16, n, s);
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t idx[4];
+        cur.next(k0, idx);
+        float4 xj[4];
+        float mj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = xs[jj[u]];
-        mj[u] = neighbor_mass(S, F, p, jj[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (k0 + u < cnt) {
-          const float3 g = pf_spiky_grad(make_float3(xn.x - xj[u].x, xn.y - xj[u].y, xn.z - xj[u].z), F.h, F.spiky);
-          const float sc = (li + xj[u].w) * mj[u];
-          sum = pf_add3(sum, pf_mul3(g, sc));
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = u_x[idx[u]];
+          mj[u] = u_m[idx[u]];
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+      }
+    } else {
+      NlCursor cur(F.nl, n, s, cnt);
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t jj[4];
+        cur.next(k0, jj);
+        float4 xj[4];
+        float mj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = xs[jj[u]];
+          mj[u] = neighbor_mass(S, F, p, jj[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
       }
     }
     xn = make_float3(xn.x + (sum.x / F.rho0) * w, xn.y + (sum.y / F.rho0) * w,
@@ -454,17 +898,62 @@ __global__ __launch_bounds__(PF_BLOCK) void k_advect(const SceneDev* __restrict_
 // ---------------------------------------------------------------------------
 // Velocity passes
 // ---------------------------------------------------------------------------
+// One neighbour's vorticity and XSPH terms (velocity.ts:45-62), added when ok.
+template <bool VORT, bool VISC>
+static __device__ __forceinline__ void velocity_term(const FluidDev& F, bool ok, bool vort, bool visc,
+                                                     float3 xi, float3 vi, float4 xj, float3 vj,
+                                                     float vol, float3& curl, float3& sm) {
+  const float3 off = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+  if (VORT && vort)
+    curl = sel3(ok, pf_add3(curl, pf_mul3(pf_cross3(pf_sub3(vi, vj), pf_spiky_grad(off, F.h, F.spiky)), vol)), curl);
+  if (VISC && visc)
+    sm = sel3(ok, pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol)), sm);
+}
+
 template <bool VORT, bool VISC>
 __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p) {
+  __shared__ float4 u_x[PF_TU_MAX];  // position, neighbour volume
+  __shared__ float4 u_v[PF_TU_MAX];  // velocity
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
   const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
+  const uint32_t tile = t.first / PF_BLOCK;
+  const uint32_t n = S.n;
   const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
   const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
+  const float4* __restrict__ X = S.x[p];
+  const float4* __restrict__ V = S.v[p];
+  const uint32_t U = F.tu_size[tile];
+  const bool staged = (vort || visc) && U != PF_TU_GLOBAL;
+  if (staged) {
+    uint32_t js[PF_TU_PER];
+    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+    float4 xv[PF_TU_PER], vv[PF_TU_PER];
+    float ov[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      xv[i] = X[js[i]];
+      vv[i] = V[js[i]];
+      ov[i] = F.pv;
+    }
+    if (S.has_boundary) {
+#pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) ov[i] = neighbor_volume(S, F, p, js[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = threadIdx.x + i * PF_BLOCK;
+      if (u < U) {
+        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
+        u_v[u] = vv[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (s >= n) return;
   if (!in_range(S.id[p][s], F.start, F.count)) {
     // |omega| of particles outside the fluid reads as zero in the gradient
     // pass (the reference's per-fluid buffer is zero there); slots change
@@ -473,39 +962,53 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
     return;
   }
   if (!(vort || visc)) return;
-  const float4* __restrict__ X = S.x[p];
-  const float4* __restrict__ V = S.v[p];
   const float3 xi = pf_xyz(X[s]);
   const float3 vi = pf_xyz(V[s]);
-  const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
-  const uint32_t* __restrict__ nl = F.nl;
   float3 curl = make_float3(0.f, 0.f, 0.f);
   float3 sm = make_float3(0.f, 0.f, 0.f);
-  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-    uint32_t jj[4];
-    float4 xj[4], vj4[4];
-    float vol[4];
+  if (staged) {
+    Nl16Cursor cur(F.nl16, n, s);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t idx[4];
+      cur.next(k0, idx);
+      float4 xj[4], vj4[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      xj[u] = X[jj[u]];
-      vj4[u] = V[jj[u]];
-      vol[u] = neighbor_volume(S, F, p, jj[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (k0 + u < cnt) {
-        const float3 vj = pf_xyz(vj4[u]);
-        const float3 off = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-        if (vort) curl = pf_add3(curl, pf_mul3(pf_cross3(pf_sub3(vi, vj), pf_spiky_grad(off, F.h, F.spiky)), vol[u]));
-        if (visc) sm = pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol[u]));
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = u_x[idx[u]];
+        vj4[u] = u_v[idx[u]];
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        velocity_term<VORT, VISC>(F, k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), xj[u].w, curl, sm);
+    }
+  } else {
+    NlCursor cur(F.nl, n, s, cnt);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t jj[4];
+      cur.next(k0, jj);
+      float4 xj[4], vj4[4];
+      float vol[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = X[jj[u]];
+        vj4[u] = V[jj[u]];
+        vol[u] = neighbor_volume(S, F, p, jj[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        velocity_term<VORT, VISC>(F, k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), vol[u], curl, sm);
     }
   }
   if (vort) F.omega[s] = make_float4(curl.x, curl.y, curl.z, pf_len3(curl));
   if (visc) F.dv[s] = make_float4(sm.x * F.visc, sm.y * F.visc, sm.z * F.visc, 0.0f);
+}
+
+// One neighbour's term of the vorticity gradient (velocity.ts:80-88), added when ok.
+static __device__ __forceinline__ void eta_term(const FluidDev& F, bool ok, float3 xi, float oi,
+                                                float4 xj, float oj, float3& eta) {
+  const float3 g = pf_spiky_grad(make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z), F.h, F.spiky);
+  eta = sel3(ok, pf_add3(eta, pf_mul3(g, (oj - oi) * F.pv)), eta);
 }
 
 template <bool VORT, bool VISC, bool FRICTION>
@@ -513,44 +1016,86 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
                                                         int substep, int touch_valid) {
+  __shared__ float4 u_x[PF_TU_MAX];  // position, |omega|
+  __shared__ uint8_t u_skip[PF_TU_MAX]; // boundary particle (left out of the gradient)
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
   const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  float4 v = S.v[p][s];
+  const uint32_t tile = t.first / PF_BLOCK;
+  const uint32_t n = S.n;
   const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
   const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
+  const float4* __restrict__ X = S.x[p];
+  const uint32_t U = F.tu_size[tile];
+  const bool staged = vort && U != PF_TU_GLOBAL;
+  if (staged) {
+    uint32_t js[PF_TU_PER];
+    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+    float4 xv[PF_TU_PER];
+    float ov[PF_TU_PER];
+    uint8_t kv[PF_TU_PER];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      xv[i] = X[js[i]];
+      ov[i] = F.omega[js[i]].w;
+      kv[i] = 0;
+    }
+    if (S.has_boundary) {
+#pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) kv[i] = S.bvol[p][js[i]] > 0.0f ? 1 : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = threadIdx.x + i * PF_BLOCK;
+      if (u < U) {
+        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
+        u_skip[u] = kv[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (s >= n) return;
+  float4 v = S.v[p][s];
   if ((vort || visc) && in_range(S.id[p][s], F.start, F.count)) {
     float3 change = make_float3(0.f, 0.f, 0.f);
     if (vort) {
-      const float4* __restrict__ X = S.x[p];
       const float3 xi = pf_xyz(X[s]);
       const float4 oi4 = F.omega[s];
       const float oi = oi4.w;
-      const uint32_t n = S.n;
       const uint32_t cnt = F.nl_count[s];
-      const uint32_t* __restrict__ nl = F.nl;
       float3 eta = make_float3(0.f, 0.f, 0.f);
-      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-        uint32_t jj[4];
-        float4 xj[4];
-        float oj[4];
-        bool skip[4];
+      if (staged) {
+        Nl16Cursor cur(F.nl16, n, s);
+        for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+          uint32_t idx[4];
+          cur.next(k0, idx);
+          float4 xj[4];
+          bool ok[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? nl[(size_t)(k0 + u) * n + s] : s;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          xj[u] = X[jj[u]];
-          oj[u] = F.omega[jj[u]].w;
-          skip[u] = (k0 + u >= cnt) || (S.has_boundary && S.bvol[p][jj[u]] > 0.0f);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (!skip[u]) {
-            const float3 g = pf_spiky_grad(make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z), F.h, F.spiky);
-            eta = pf_add3(eta, pf_mul3(g, (oj[u] - oi) * F.pv));
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = u_x[idx[u]];
+            ok[u] = (k0 + u < cnt) && !u_skip[idx[u]];
           }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], xj[u].w, eta);
+        }
+      } else {
+        NlCursor cur(F.nl, n, s, cnt);
+        for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+          uint32_t jj[4];
+          cur.next(k0, jj);
+          float4 xj[4];
+          float oj[4];
+          bool ok[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = X[jj[u]];
+            oj[u] = F.omega[jj[u]].w;
+            ok[u] = (k0 + u < cnt) && !(S.has_boundary && S.bvol[p][jj[u]] > 0.0f);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], oj[u], eta);
         }
       }
       const float el = pf_len3(eta);

@@ -27,6 +27,12 @@
 #define PF_SCAN_TILE 4096            // buckets per scan workgroup (256 x 16)
 #define PF_BLOCK 256
 #define PF_STABLE_SEGMENT_LIMIT 2048 // buckets longer than this keep atomic order (flagged)
+// Tile union: the particles of every bucket that the fluid particles of one
+// 256-slot tile walk, staged in LDS for the neighbour passes.
+#define PF_TU_MAX 2048               // particles a tile union may hold (16-bit list entries)
+#define PF_TU_HASH 1024              // bucket hash-set slots per tile (power of two)
+#define PF_TU_HASH_LIMIT 768         // distinct buckets beyond which the tile uses global gathers
+#define PF_TU_GLOBAL 0xffffffffu     // tu_size value of a tile whose list holds global slots
 
 // Overflow / status flag words, one u32 each in SceneDev::flags.
 enum {
@@ -66,7 +72,11 @@ struct FluidDev {
   float adh_coef, adh_m4_over_h, adh_2h; // adhesion.ts:16-27 constants
   float imp_scale, imp_inv;         // Accumulator(50) for impulses
   float rea_scale, rea_inv;         // Accumulator(10) for boundary reaction
-  uint32_t* nl;        // [64 * n] neighbour slots, column-major k*n + slot
+  uint32_t* nl;        // [64 * n] neighbour slots, column-major k*n + slot (global-gather tiles)
+  uint32_t* nl16;      // [32 * n] pairs of 16-bit tile-union indices, column-major (k/2)*n + slot
+  uint32_t tu_limit;   // largest union a tile stages (<= PF_TU_MAX)
+  uint32_t* tu_size;   // [tiles] union size of each 256-slot tile, or PF_TU_GLOBAL
+  uint32_t* tu_slots;  // [tiles * PF_TU_MAX] sorted slot of each union entry
   uint32_t* nl_count;  // [n]
   uint32_t* nl_last;   // [n] walk key of the last kept neighbour when the list overflowed, else ~0
   float4* normal;      // [n] colour-field normal

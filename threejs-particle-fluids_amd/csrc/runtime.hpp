@@ -145,6 +145,10 @@ struct pf_fluid_s {
   pf_loop_s* loop = nullptr;
   int slot = -1;  // index in SceneDev::fluid
   uint32_t* nl = nullptr;
+  uint32_t* nl16 = nullptr;
+  uint32_t tu_limit = PF_TU_MAX;
+  uint32_t* tu_size = nullptr;
+  uint32_t* tu_slots = nullptr;
   uint32_t* nl_count = nullptr;
   uint32_t* nl_last = nullptr;
   float4* normal = nullptr;

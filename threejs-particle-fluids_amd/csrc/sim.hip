@@ -350,11 +350,22 @@ int pf_fluid_info(pf_fluid* f, double* mass, double* h, double* spacing, double*
   PF_API_END
 }
 
+int pf_fluid_set_tile_union_limit(pf_fluid* f, uint32_t max_particles) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (max_particles > PF_TU_MAX) fail(PF_EINVAL, "FluidSystem: the tile union limit is at most %u, got %u", PF_TU_MAX, max_particles);
+  f->tu_limit = max_particles;
+  if (f->loop) fluid_write_params(f);
+  PF_API_END
+}
+
 int pf_fluid_destroy(pf_fluid* f) {
   PF_API_BEGIN
   if (!f || f->disposed) return PF_OK;
   f->disposed = true;
   dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv); dfree(f->reaction);
+  dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
+  f->nl16 = f->tu_size = f->tu_slots = nullptr;
   f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr; f->reaction = nullptr;
   PF_API_END
 }
@@ -782,6 +793,10 @@ static void fluid_write_params(pf_fluid* f) {
   F.nl = f->nl;
   F.nl_count = f->nl_count;
   F.nl_last = f->nl_last;
+  F.nl16 = f->nl16;
+  F.tu_limit = f->tu_limit;
+  F.tu_size = f->tu_size;
+  F.tu_slots = f->tu_slots;
   F.normal = f->normal;
   F.omega = f->omega;
   F.dv = f->dv;
@@ -903,6 +918,10 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     f->nl = dalloc<uint32_t>((size_t)PF_MAX_NEIGHBORS * n);
     f->nl_count = dalloc<uint32_t>(n);
     f->nl_last = dalloc<uint32_t>(n);
+    const size_t ntiles = ((size_t)n + PF_BLOCK - 1) / PF_BLOCK;
+    f->nl16 = dalloc<uint32_t>((size_t)(PF_MAX_NEIGHBORS / 2) * n);
+    f->tu_size = dalloc<uint32_t>(ntiles);
+    f->tu_slots = dalloc<uint32_t>(ntiles * PF_TU_MAX);
     if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
@@ -1123,6 +1142,8 @@ int pf_loop_destroy(pf_loop* L) {
   for (auto* f : L->fluids) {
     f->loop = nullptr;
     dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv);
+    dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
+    f->nl16 = f->tu_size = f->tu_slots = nullptr;
     f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr;
   }
   if (L->prims) L->prims->loop = nullptr;

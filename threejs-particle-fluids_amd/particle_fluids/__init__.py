@@ -319,6 +319,11 @@ class FluidSystem:
         call("pf_fluid_readback_overflow", self.handle, C.byref(o))
         return bool(o.value)
 
+    def setTileUnionLimit(self, maxParticles):
+        """LDS staging limit per 256-slot tile (no reference counterpart;
+        results do not depend on it; 0 gathers every tile from global memory)."""
+        call("pf_fluid_set_tile_union_limit", self.handle, int(maxParticles))
+
     def material(self):
         return _lib.Material(1, self.handle)
 

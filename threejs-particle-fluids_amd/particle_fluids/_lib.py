@@ -130,6 +130,7 @@ SIGNATURES = {
     "pf_fluid_readback_density": [vp, P(f32)],
     "pf_fluid_readback_overflow": [vp, P(i32)],
     "pf_fluid_info": [vp, P(f64), P(f64), P(f64), P(f64)],
+    "pf_fluid_set_tile_union_limit": [vp, u32],
     "pf_fluid_destroy": [vp],
     "pf_primitives_create": [vp, u32, P(vp)],
     "pf_primitives_add_plane": [vp, P(f64), P(f64), f64, f64, P(u32)],
