@@ -47,7 +47,7 @@ static __device__ __forceinline__ float neighbor_volume(const SceneDev& S, const
 }
 
 static __device__ __forceinline__ void flag_set(const SceneDev& S, int parity, int f) {
-  atomicOr(S.flags + parity * PF_NUM_FLAGS + f, 1u);
+  pf_flag_raise(S.flags + parity * PF_NUM_FLAGS + f);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   if (global_walk) {
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
     if (!active) return;
-    uint32_t* __restrict__ nl = F.nl;
+    uint32_t* row = F.nl + s;  // the row of the next entry
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
       for (int dy = -1; dy <= 1 && !over; ++dy) {
@@ -311,22 +311,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
             float4 xj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = (j0 + u < en) ? xs[j0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
+            for (int u = 0; u < 4; ++u) xj[u] = xs[min(j0 + u, en - 1u)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const uint32_t j = j0 + u;
               const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              if (j < en && pf_dot3(d, d) < F.h2 && !over) {
-                if (count < PF_MAX_NEIGHBORS) {
-                  nl[(size_t)count * n + s] = j;
-                  ++count;
-                  last_key = (stencil << 27) | j;
-                } else {
-                  // The list is full and another neighbour exists: it
-                  // overflows, nothing later in the walk can enter it.
-                  over = true;
-                }
+              const bool acc = j < en && pf_dot3(d, d) < F.h2;
+              const bool keep = acc && count < PF_MAX_NEIGHBORS;
+              // The list is full and another neighbour exists: it overflows,
+              // nothing later in the walk can enter it.
+              over = over || (acc && count >= PF_MAX_NEIGHBORS);
+              if (keep) {
+                *row = j;
+                row += n;
               }
+              last_key = keep ? ((stencil << 27) | j) : last_key;
+              count += keep ? 1u : 0u;
             }
           }
         }
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   uint32_t* __restrict__ nl16 = F.nl16;
   if (active) {
     uint32_t pend = 0;
+    uint32_t* row = nl16 + s;  // the row of the next entry pair
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
       for (int dy = -1; dy <= 1 && !over; ++dy) {
@@ -390,31 +391,34 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           const uint32_t off = ho >> 16, cn = ho & 0xffffu;
           const uint32_t st = h_start[h];
           const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+          // Branch-free append: only a completed entry pair stores.
           for (uint32_t i0 = 0; i0 < cn && !over; i0 += 4) {
             float4 xj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = (i0 + u < cn) ? u_x[off + i0 + u] : make_float4(3.0e38f, 0.f, 0.f, 0.f);
+            for (int u = 0; u < 4; ++u) xj[u] = u_x[off + min(i0 + u, cn - 1u)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const uint32_t i = i0 + u;
               const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              if (i < cn && pf_dot3(d, d) < F.h2 && !over) {
-                if (count < PF_MAX_NEIGHBORS) {
-                  const uint32_t local = off + i;
-                  if (count & 1u) nl16[(size_t)(count >> 1) * n + s] = pend | (local << 16);
-                  else pend = local;
-                  ++count;
-                  last_key = (stencil << 27) | (st + i);
-                } else {
-                  over = true;
-                }
+              const bool acc = i < cn && pf_dot3(d, d) < F.h2;
+              const bool keep = acc && count < PF_MAX_NEIGHBORS;
+              // The list is full and another neighbour exists: it overflows,
+              // nothing later in the walk can enter it.
+              over = over || (acc && count >= PF_MAX_NEIGHBORS);
+              const uint32_t local = off + i;
+              if (keep && (count & 1u)) {
+                *row = pend | (local << 16);
+                row += n;
               }
+              pend = (keep && !(count & 1u)) ? local : pend;
+              last_key = keep ? ((stencil << 27) | (st + i)) : last_key;
+              count += keep ? 1u : 0u;
             }
           }
         }
       }
     }
-    if (count & 1u) nl16[(size_t)(count >> 1) * n + s] = pend;
+    if (count & 1u) *row = pend;
     F.nl_count[s] = count;
     F.nl_last[s] = over ? last_key : 0xffffffffu;
     if (want_normal) {

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_hash(const SceneDev* __restrict__ 
   }
   const uint32_t hb = __shfl(base, (int)head_lane);
   if (valid) S.rank[i] = hb + (lane - head_lane);
-  if (__ballot(over != 0) && lane == 0) atomicOr(flag_word(S, MODE == 1 ? parity : 0, PF_FLAG_GRID), 1u);
+  if (over) pf_flag_raise(flag_word(S, MODE == 1 ? parity : 0, PF_FLAG_GRID));
 }
 
 // ---------------------------------------------------------------------------
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_rank(const SceneDev* __restrict__ 
     dest = st + r;
   } else {
     dest = s;
-    atomicOr(S.flags + (MODE == 1 ? parity : 0) * PF_NUM_FLAGS + PF_FLAG_LONG_BUCKET, 1u);
+    pf_flag_raise(S.flags + (MODE == 1 ? parity : 0) * PF_NUM_FLAGS + PF_FLAG_LONG_BUCKET);
   }
   if (MODE == 0) {
     out_perm[dest] = src;

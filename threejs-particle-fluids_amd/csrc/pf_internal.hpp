@@ -253,6 +253,16 @@ __device__ __forceinline__ int pf_wrap_add(int a, int b) {
   return (int)((uint32_t)a + (uint32_t)b);
 }
 
+// Raise a status flag word shared by every particle of a scene: one atomic
+// per wave (from its lowest active lane), and none once the word reads set,
+// so a flag raised by most particles does not serialise on one address.
+__device__ __forceinline__ void pf_flag_raise(uint32_t* word) {
+  const unsigned long long active = __ballot(1);
+  if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)active) - 1)) {
+    if (__atomic_load_n(word, __ATOMIC_RELAXED) == 0u) atomicOr(word, 1u);
+  }
+}
+
 // XCD-aware mapping: gfx950 places consecutive workgroups on different XCDs
 // round-robin. Give each XCD a contiguous run of tiles so neighbour reads of
 // spatially adjacent (cell-sorted) particles hit the same XCD's L2.
