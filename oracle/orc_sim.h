@@ -15,7 +15,7 @@
 
 typedef struct {
   uint32_t start, count, effects;
-  REAL rho0, pv, mass, h, h2, poly6, spiky, compliance;
+  REAL rho0, pv, mass, inv_mass, h, h2, poly6, spiky, compliance;
   REAL gamma, visc, vort, adhesion;
   REAL coh_coef, coh_h6_64, adh_coef, adh_m4_over_h, adh_2h;
   REAL imp_scale, imp_inv;
@@ -85,16 +85,18 @@ static inline REAL SFX(poly6)(REAL r2, REAL h2, REAL coef) {
   return ((d * d) * d) * coef;
 }
 
-/* kernels.ts:89-107 */
+/* kernels.ts:89-107. The direction r / max(|r|, 1e-20) takes one rounded
+ * reciprocal and three products (the reference's WGSL divides per component
+ * with 2.5-ulp division; this form stays within that). */
 static inline void SFX(spiky)(const REAL r[3], REAL h, REAL coef, REAL g[3]) {
   REAL rsq = SFX(dot)(r, r);
   REAL rl = RSQRT(rsq);
   REAL t = RFMAX(h - rl, R(0.0));
-  REAL rs = RFMAX(rl, (REAL)1e-20);
+  REAL inv = R(1.0) / RFMAX(rl, (REAL)1e-20);
   REAL mag = -((t * t) * coef);
-  g[0] = (r[0] / rs) * mag;
-  g[1] = (r[1] / rs) * mag;
-  g[2] = (r[2] / rs) * mag;
+  g[0] = (r[0] * inv) * mag;
+  g[1] = (r[1] * inv) * mag;
+  g[2] = (r[2] * inv) * mag;
 }
 
 /* accumulator.ts:62-68: i32(round(v * scale)), half-to-even, saturating */
@@ -240,10 +242,11 @@ static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
           REAL corr = (F->rho0 * R(2.0)) / RFMAX(s->rho[i] + s->rho[j], R(1.0));
           REAL cs = -(((F->gamma * F->mass) * F->mass) * SFX(coh_spline)(F, r));
           REAL gm = F->gamma * F->mass;
+          REAL ir = R(1.0) / r;
           for (int a = 0; a < 3; ++a) {
-            REAL coh = (off[a] / r) * cs;
+            REAL coh = (off[a] * ir) * cs;
             REAL curv = -((F->normal[3 * (size_t)i + a] - F->normal[3 * (size_t)j + a]) * gm);
-            REAL dv = (((coh + curv) * corr) / F->mass) * s->dt;
+            REAL dv = (((coh + curv) * corr) * F->inv_mass) * s->dt;
             s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)(dv, F->imp_scale));
             s->iacc[3 * (size_t)j + a] = SFX(wadd)(s->iacc[3 * (size_t)j + a], SFX(ticks)(-dv, F->imp_scale));
           }
@@ -254,9 +257,10 @@ static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
         REAL r = RSQRT(SFX(dot)(off, off));
         if (!(r < (REAL)1e-20)) {
           REAL fs = -(((F->adhesion * F->mass) * (F->rho0 * s->bvol[j])) * SFX(adh_spline)(F, r));
+          REAL ir = R(1.0) / r;
           for (int a = 0; a < 3; ++a) {
-            REAL f = (off[a] / r) * fs;
-            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)((f / F->mass) * s->dt, F->imp_scale));
+            REAL f = (off[a] * ir) * fs;
+            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)((f * F->inv_mass) * s->dt, F->imp_scale));
           }
         }
       }

@@ -258,7 +258,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     fluid_##SUF* F = &s->fluid[s->nfluids++];                                                      \
     double mass = rho0 * spacing * spacing * spacing;                                              \
     F->start = start; F->count = count; F->effects = effects;                                      \
-    F->rho0 = (T)rho0; F->pv = (T)(spacing * spacing * spacing); F->mass = (T)mass;                \
+    F->rho0 = (T)rho0; F->pv = (T)(spacing * spacing * spacing); F->mass = (T)mass; F->inv_mass = (T)1 / F->mass;                \
     F->h = (T)h; F->h2 = (T)(h * h);                                                               \
     F->poly6 = (T)(315.0 / (64.0 * M_PI * pow(h, 9)));                                              \
     F->spiky = (T)(45.0 / (M_PI * pow(h, 6)));                                                      \

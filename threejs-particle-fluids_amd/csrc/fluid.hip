@@ -472,11 +472,12 @@ static __device__ __forceinline__ bool pair_dv(const FluidDev& F, float dt, floa
   if (r < 1e-20f) return false;
   const float corr = (F.rho0 * 2.0f) / fmaxf(rhoa + rhob, 1.0f);
   const float cs = -(((F.gamma * F.mass) * F.mass) * cohesion_spline(F, r));
-  const float3 coh = make_float3((off.x / r) * cs, (off.y / r) * cs, (off.z / r) * cs);
+  const float ir = 1.0f / r;
+  const float3 coh = make_float3((off.x * ir) * cs, (off.y * ir) * cs, (off.z * ir) * cs);
   const float gm = F.gamma * F.mass;
   const float3 curv = make_float3(-((na.x - nb.x) * gm), -((na.y - nb.y) * gm), -((na.z - nb.z) * gm));
-  *dv = make_float3((((coh.x + curv.x) * corr) / F.mass) * dt, (((coh.y + curv.y) * corr) / F.mass) * dt,
-                    (((coh.z + curv.z) * corr) / F.mass) * dt);
+  *dv = make_float3((((coh.x + curv.x) * corr) * F.inv_mass) * dt, (((coh.y + curv.y) * corr) * F.inv_mass) * dt,
+                    (((coh.z + curv.z) * corr) * F.inv_mass) * dt);
   return true;
 }
 
@@ -571,10 +572,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
         const float r = pf_len3(off);
         if (!(r < 1e-20f)) {
           const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
-          const float3 f = make_float3((off.x / r) * fs, (off.y / r) * fs, (off.z / r) * fs);
-          tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x / F.mass) * S.dt, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y / F.mass) * S.dt, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z / F.mass) * S.dt, F.imp_scale));
+          const float ir = 1.0f / r;
+          const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
+          tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
+          tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
+          tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
         }
       }
     }

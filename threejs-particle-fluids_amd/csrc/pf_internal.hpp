@@ -65,6 +65,7 @@ struct FluidDev {
   uint32_t bstart[PF_MAX_BOUNDARIES], bcount[PF_MAX_BOUNDARIES];
   uint32_t bdynamic;       // bit b: boundary b recomputed every substep
   float rho0, pv, mass;    // rest density, spacing^3, rho0*spacing^3 (f32 uniforms)
+  float inv_mass;          // 1 / mass, rounded once
   float h, h2, poly6, spiky; // kernels.ts:22-47 (coefficients computed in double)
   float compliance;
   float gamma, visc, vort, adhesion;
@@ -216,14 +217,16 @@ __device__ __forceinline__ float pf_poly6(float r2, float h2, float poly6) {
   return ((d * d) * d) * poly6;
 }
 
-// grad W = (r_vec / max(|r|, 1e-20)) * (-(t*t*spikyCoef)), t = max(h - |r|, 0).
+// grad W = (r_vec / max(|r|, 1e-20)) * (-(t*t*spikyCoef)), t = max(h - |r|, 0);
+// the direction takes one rounded reciprocal and three products (within the
+// reference's 2.5-ulp WGSL division).
 __device__ __forceinline__ float3 pf_spiky_grad(float3 r, float h, float spiky) {
   const float rsq = (r.x * r.x + r.y * r.y) + r.z * r.z;
   const float rl = sqrtf(rsq);
   const float t = fmaxf(h - rl, 0.0f);
-  const float rs = fmaxf(rl, 1e-20f);
+  const float inv = 1.0f / fmaxf(rl, 1e-20f);
   const float mag = -((t * t) * spiky);
-  return make_float3((r.x / rs) * mag, (r.y / rs) * mag, (r.z / rs) * mag);
+  return make_float3((r.x * inv) * mag, (r.y * inv) * mag, (r.z * inv) * mag);
 }
 
 __device__ __forceinline__ float pf_dot3(float3 a, float3 b) {

@@ -761,6 +761,7 @@ static void fluid_write_params(pf_fluid* f) {
   F.rho0 = (float)f->rest_density;
   F.pv = (float)(f->spacing * f->spacing * f->spacing);
   F.mass = (float)f->mass;
+  F.inv_mass = 1.0f / F.mass;
   const double h = f->h;
   // kernels.ts:33-35: coefficients in double, uploaded as f32.
   F.h = (float)h;
