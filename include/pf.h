@@ -87,6 +87,9 @@ int pf_particles_set_collision_group(pf_particles* ps, uint32_t start, uint32_t 
                                      uint32_t group);
 /* readback() -> ParticleSnapshot (particles.ts:192-227) */
 int pf_particles_readback(pf_particles* ps, const pf_snapshot* out);
+/* the boundaryVolume buffer (particles.ts:101; written by
+ * buildBoundaryVolumeKernel, boundaryVolume.ts:26-52), in particle order */
+int pf_particles_readback_boundary_volume(pf_particles* ps, float* volume);
 int pf_particles_info(pf_particles* ps, uint32_t* capacity, double* particle_radius);
 
 /* ---- HashGrid (src/core/hashGrid/HashGrid.ts:80-286) ---------------------

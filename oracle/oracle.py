@@ -55,6 +55,10 @@ def lib():
         g("scene_add_prim").restype = i32
         g("scene_add_body").argtypes = [vp, u32, u32, f64, P(f32), P(f32)]
         g("scene_add_body").restype = i32
+        g("scene_add_boundary").argtypes = [vp, u32, u32, u32, i32]
+        g("scene_add_boundary").restype = i32
+        g("scene_get_bvol").argtypes = [vp, P(f64)]
+        g("scene_get_bvol").restype = None
         g("scene_step").argtypes = [vp, f64]
         g("scene_step").restype = None
         g("scene_get").argtypes = [vp, P(f64), P(f64), P(f64), P(f64)]
@@ -171,6 +175,15 @@ class Scene:
         w = np.ascontiguousarray(weights, np.float32)
         rc = self._f("scene_add_body")(self.h, int(start), int(count), float(compliance), _p(r, f32), _p(w, f32))
         assert rc == 0
+
+    def add_boundary(self, fluid, start, count, dynamic=True):
+        rc = self._f("scene_add_boundary")(self.h, int(fluid), int(start), int(count), 1 if dynamic else 0)
+        assert rc == 0
+
+    def boundary_volumes(self):
+        out = np.zeros(self.n)
+        self._f("scene_get_bvol")(self.h, _p(out, f64))
+        return out
 
     def step(self, dt):
         self._f("scene_step")(self.h, float(dt))

@@ -19,6 +19,9 @@ typedef struct {
   REAL gamma, visc, vort, adhesion;
   REAL coh_coef, coh_h6_64, adh_coef, adh_m4_over_h, adh_2h;
   REAL imp_scale, imp_inv;
+  REAL rea_scale, rea_inv;
+  uint32_t nb;                                 /* boundary ranges (addBoundary) */
+  uint32_t bstart[8], bcount[8], bdyn[8];
   uint32_t* nl;     /* [64 n] row-major: nl[i*64 + k] */
   uint32_t* nlc;    /* [n] */
   REAL* normal;     /* [3n] */
@@ -49,7 +52,8 @@ typedef struct {
   uint32_t nprim;
   SFX(prim) prim[32];
   REAL* plam;      /* [2 * 32 * n] */
-  int32_t *pacc, *vacc, *iacc; /* [3n] fixed-point accumulators */
+  int32_t *pacc, *vacc, *iacc, *racc; /* [3n] fixed-point accumulators */
+  int initialized;                     /* first step ran (static boundary volumes) */
   REAL pos_scale, pos_inv, vel_scale, vel_inv;
   /* soft bodies (global) */
   uint32_t nbodies;
@@ -63,7 +67,7 @@ typedef struct {
   REAL g[3], dt;
   REAL clock[64];
   /* flags */
-  int flag_grid, flag_nl, flag_pos, flag_vel, flag_imp;
+  int flag_grid, flag_nl, flag_pos, flag_vel, flag_imp, flag_rea;
 } SFX(scene);
 
 /* --------------------------------------------------------------------- */
@@ -194,6 +198,47 @@ static inline REAL SFX(nmass)(const SFX(scene)* s, const SFX(fluid)* F, uint32_t
   return s->w[j] > R(0.0) ? R(1.0) / s->w[j] : R(0.0);
 }
 
+/* boundaryVolume.ts:26-52: V_i = 1 / max(sum_k W(x_i - x_k), 1e-12) over the
+ * boundary's own range, walking the grid (query.ts:24-70, bucket dedup).
+ * pos: the positions the walk reads (committed for a static boundary's
+ * one-time pass, predicted for a dynamic one). */
+static void SFX(boundary_volume)(SFX(scene)* s, const SFX(fluid)* F, uint32_t b, const REAL* pos) {
+  uint32_t H = s->H, lo = F->bstart[b], hi = F->bstart[b] + F->bcount[b];
+  for (uint32_t i = lo; i < hi; ++i) {
+    const REAL* xi = pos + 3 * (size_t)i;
+    int32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = orc_f2i_sat((double)RFLOOR((xi[a] - s->origin[a]) / s->cell));
+    uint32_t visited[27], nv = 0;
+    REAL sum = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          uint32_t bk = orc_morton((int32_t)((uint32_t)q[0] + (uint32_t)dx), (int32_t)((uint32_t)q[1] + (uint32_t)dy),
+                                   (int32_t)((uint32_t)q[2] + (uint32_t)dz)) & (H - 1);
+          int seen = 0;
+          for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == bk;
+          visited[nv++] = bk;
+          if (seen) continue;
+          for (uint32_t t = s->start[bk]; t < s->start[bk + 1]; ++t) {
+            uint32_t j = s->perm[t];
+            if (j < lo || j >= hi) continue;
+            const REAL* xj = pos + 3 * (size_t)j;
+            REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            REAL r2 = SFX(dot)(d, d);
+            if (r2 >= F->h2) continue;
+            sum = sum + SFX(poly6)(r2, F->h2, F->poly6);
+          }
+        }
+    s->bvol[i] = R(1.0) / RFMAX(sum, (REAL)1e-12);
+  }
+}
+
+static inline int SFX(in_boundary)(const SFX(fluid)* F, uint32_t j) {
+  for (uint32_t b = 0; b < F->nb; ++b)
+    if (j >= F->bstart[b] && j < F->bstart[b] + F->bcount[b]) return 1;
+  return 0;
+}
+
 /* cohesion.ts:25-39 */
 static void SFX(color_normal)(SFX(scene)* s, SFX(fluid)* F) {
   for (uint32_t i = F->start; i < F->start + F->count; ++i) {
@@ -320,11 +365,33 @@ static void SFX(delta_pass)(SFX(scene)* s, SFX(fluid)* F) {
       REAL lj = (j >= F->start && j < end) ? F->lambda[j] : R(0.0);
       REAL sc = (li + lj) * SFX(nmass)(s, F, j);
       sum[0] = sum[0] + g[0] * sc; sum[1] = sum[1] + g[1] * sc; sum[2] = sum[2] + g[2] * sc;
+      /* reaction on dynamic boundary neighbours (positionDelta.ts:54-69) */
+      if (F->nb && s->bvol[j] > R(0.0) && s->w[j] > R(0.0) && li != R(0.0) && SFX(in_boundary)(F, j)) {
+        REAL rs = ((-(li / F->rho0)) * (F->rho0 * s->bvol[j])) * s->w[j];
+        for (int a = 0; a < 3; ++a)
+          s->racc[3 * (size_t)j + a] = SFX(wadd)(s->racc[3 * (size_t)j + a], SFX(ticks)(g[a] * rs, F->rea_scale));
+      }
     }
     for (int a = 0; a < 3; ++a) F->dx[3 * (size_t)i + a] = (sum[a] / F->rho0) * s->w[i];
   }
   for (uint32_t i = F->start; i < end; ++i)
     for (int a = 0; a < 3; ++a) s->xs[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a] + F->dx[3 * (size_t)i + a];
+  if (F->nb) {
+    /* reaction apply over the covering range, then clear (accumulator.ts:101-131) */
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t b = 0; b < F->nb; ++b) {
+      if (F->bstart[b] < lo) lo = F->bstart[b];
+      if (F->bstart[b] + F->bcount[b] > hi) hi = F->bstart[b] + F->bcount[b];
+    }
+    for (uint32_t j = lo; j < hi; ++j) {
+      REAL c[3];
+      s->flag_rea |= SFX(acc_change)(s->racc + 3 * (size_t)j, F->rea_inv, c);
+      for (int a = 0; a < 3; ++a) {
+        s->xs[3 * (size_t)j + a] = s->xs[3 * (size_t)j + a] + c[a];
+        s->racc[3 * (size_t)j + a] = 0;
+      }
+    }
+  }
 }
 
 /* velocity.ts:36-119 */
@@ -702,6 +769,23 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
   }
   s->flag_pos = 0;
   s->flag_vel = 0;
+  if (!s->initialized) {
+    /* first step: static boundary volumes from committed positions, over a
+     * grid of the current predictions (loop.ts:236, 276-277) */
+    int any_static = 0;
+    for (uint32_t f = 0; f < s->nfluids; ++f)
+      for (uint32_t b = 0; b < s->fluid[f].nb; ++b) any_static |= !s->fluid[f].bdyn[b];
+    if (any_static) {
+      for (uint32_t i = 0; i < n; ++i)
+        if (s->w[i] > R(0.0))
+          for (int a = 0; a < 3; ++a) s->xs[3 * (size_t)i + a] = s->x[3 * (size_t)i + a];
+      SFX(grid_build)(s);
+      for (uint32_t f = 0; f < s->nfluids; ++f)
+        for (uint32_t b = 0; b < s->fluid[f].nb; ++b)
+          if (!s->fluid[f].bdyn[b]) SFX(boundary_volume)(s, &s->fluid[f], b, s->x);
+    }
+    s->initialized = 1;
+  }
   for (uint32_t k = 0; k < s->substeps; ++k) {
     /* predict (integrate.ts:25-34) */
     for (uint32_t i = 0; i < n; ++i) {
@@ -714,6 +798,9 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
     }
     s->flag_nl = 0;
     if (s->nfluids) SFX(grid_build)(s);
+    for (uint32_t f = 0; f < s->nfluids; ++f)
+      for (uint32_t b = 0; b < s->fluid[f].nb; ++b)
+        if (s->fluid[f].bdyn[b]) SFX(boundary_volume)(s, &s->fluid[f], b, s->xs);
     memset(s->plam, 0, sizeof(REAL) * 64 * (size_t)n);
     for (uint32_t f = 0; f < s->nfluids; ++f) SFX(neighbors)(s, &s->fluid[f]);
     for (uint32_t f = 0; f < s->nfluids; ++f) {

@@ -200,6 +200,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     s->pacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
     s->vacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
     s->iacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
+    s->racc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));                                    \
     s->rest = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
     s->weights = (T*)calloc(n, sizeof(T));                                                         \
     s->slam = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
@@ -217,7 +218,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     if (!s) return;                                                                                \
     free(s->x); free(s->xs); free(s->v); free(s->w); free(s->rho); free(s->bvol);                  \
     free(s->key); free(s->perm); free(s->start); free(s->plam); free(s->pacc); free(s->vacc);      \
-    free(s->iacc); free(s->rest); free(s->weights); free(s->slam);                                 \
+    free(s->iacc); free(s->racc); free(s->rest); free(s->weights); free(s->slam);                  \
     for (uint32_t f = 0; f < s->nfluids; ++f) {                                                    \
       fluid_##SUF* F = &s->fluid[f];                                                               \
       free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
@@ -271,6 +272,8 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     F->adh_2h = (T)(2.0 * h);                                                                       \
     F->imp_scale = (T)floor(1073741824.0 / 50.0);                                                  \
     F->imp_inv = (T)(1.0 / floor(1073741824.0 / 50.0));                                           \
+    F->rea_scale = (T)floor(1073741824.0 / 10.0);                                                  \
+    F->rea_inv = (T)(1.0 / floor(1073741824.0 / 10.0));                                            \
     size_t n = s->n;                                                                               \
     F->nl = (uint32_t*)calloc(64 * n, sizeof(uint32_t));                                           \
     F->nlc = (uint32_t*)calloc(n, sizeof(uint32_t));                                               \
@@ -288,6 +291,20 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
       s->rho[i] = (T)(float)rho0;                                                                  \
     }                                                                                              \
     return 0;                                                                                      \
+  }                                                                                                \
+  /* FluidSystem.addBoundary (FluidSystem.ts:171-183): range and dynamic flag of fluid f */        \
+  int orc_scene_add_boundary_##SUF(void* p, uint32_t f, uint32_t start, uint32_t count, int dyn) { \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (f >= s->nfluids || s->fluid[f].nb >= 8) return -1;                                         \
+    fluid_##SUF* F = &s->fluid[f];                                                                 \
+    F->bstart[F->nb] = start; F->bcount[F->nb] = count; F->bdyn[F->nb] = dyn ? 1u : 0u;            \
+    F->nb++;                                                                                       \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* boundary volumes, in particle order */                                                        \
+  void orc_scene_get_bvol_##SUF(void* p, double* out) {                                            \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    for (uint32_t i = 0; i < s->n; ++i) out[i] = (double)s->bvol[i];                               \
   }                                                                                                \
   /* PrimitiveSet add* after the host-side double setup; data as uploaded (f32 values) */         \
   int orc_scene_add_prim_##SUF(void* p, uint32_t kind, uint32_t flags, const double d0[4],         \

@@ -34,3 +34,43 @@ def rel_err(a, b):
     d = np.abs(a - b).max()
     s = np.abs(b).max()
     return float(d / s) if s > 0 else float(d)
+
+
+def boundary_layout(radius=0.01, plate_w=0.0):
+    """A fluid block above a two-layer plate of boundary particles.
+
+    Returns (positions (N,3) f32, inv_mass (N,) f32, n_fluid, n_plate). The
+    fluid's inverse mass is set by the fluid itself; plate particles get
+    plate_w (0 pins them)."""
+    s = 2.0 * radius
+    fluid = [(x, y, z) for x in np.arange(-0.19, 0.19 + 1e-9, s) for y in np.arange(0.09, 0.27 + 1e-9, s)
+             for z in np.arange(-0.09, 0.09 + 1e-9, s)]
+    plate = [(x, y, z) for x in np.arange(-0.29, 0.29 + 1e-9, s) for y in (0.045, 0.065)
+             for z in np.arange(-0.15, 0.15 + 1e-9, s)]
+    pos = np.array(fluid + plate, np.float32)
+    w = np.zeros(len(pos), np.float32)
+    w[len(fluid):] = plate_w
+    return pos, w, len(fluid), len(plate)
+
+
+def oracle_boundary_scene(pos, w, nf, nb, radius, dynamic, substeps=2, iterations=4, precision="f32",
+                          gravity=(0.0, -9.81, 0.0), effects=None, planes=True):
+    """The boundary scene in the CPU oracle (fluid [0, nf), boundary [nf, nf + nb))."""
+    n = len(pos)
+    spacing = 2 * radius
+    h = 2 * spacing
+    H = 1
+    while H < 2 * n:
+        H <<= 1
+    fx = dict(scenes.FLUID_DEFAULTS) if effects is None else effects
+    s = orc.Scene(n, radius, precision)
+    s.set_particles(pos, w=w)
+    s.set_loop(substeps, iterations, gravity, h, min(H, 1 << 20))
+    s.add_fluid(0, nf, 1000.0, spacing, h, 1e-4, viscosity=fx.get("viscosity"), vorticity=fx.get("vorticity"),
+                surface_tension=fx.get("surfaceTension"), adhesion=fx.get("adhesion"))
+    s.add_boundary(0, nf, nb, dynamic)
+    if planes:
+        mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
+        for nrm, which in scenes.PLANES:
+            s.add_plane(nrm, mn if which == "min" else mx)
+    return s

@@ -752,11 +752,31 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
 // Position correction (+ container contact, accumulator apply, advect)
 // ---------------------------------------------------------------------------
 
-// One neighbour's term of the correction sum (positionDelta.ts:44-52), added when ok.
-static __device__ __forceinline__ void delta_term(const FluidDev& F, bool ok, float3 xn, float li,
-                                                  float4 xj, float mj, float3& sum) {
+// One neighbour's term of the correction sum (positionDelta.ts:44-52), added
+// when ok; returns the kernel gradient for the boundary reaction.
+static __device__ __forceinline__ float3 delta_term(const FluidDev& F, bool ok, float3 xn, float li,
+                                                    float4 xj, float mj, float3& sum) {
   const float3 g = pf_spiky_grad(make_float3(xn.x - xj.x, xn.y - xj.y, xn.z - xj.z), F.h, F.spiky);
   sum = sel3(ok, pf_add3(sum, pf_mul3(g, (li + xj.w) * mj)), sum);
+  return g;
+}
+
+// The matching push on a dynamic boundary neighbour j of this fluid
+// (positionDelta.ts:54-69): -(lambda_i / rho0) * psi_j * w_j * grad W, into
+// the reaction accumulator (Accumulator(10)) of slot j.
+static __device__ __forceinline__ void reaction_add(const SceneDev& S, const FluidDev& F, int p, uint32_t j,
+                                                    float li, float3 g) {
+  const float bj = S.bvol[p][j];
+  const float wj = S.w[p][j];
+  if (!(bj > 0.0f && wj > 0.0f)) return;
+  const uint32_t idj = S.id[p][j];
+  bool inb = false;
+  for (uint32_t b = 0; b < F.nboundary; ++b) inb = inb || in_range(idj, F.bstart[b], F.bcount[b]);
+  if (!inb) return;
+  const float rs = ((-(li / F.rho0)) * (F.rho0 * bj)) * wj;
+  atomicAdd(F.reaction + 3 * (size_t)j + 0, pf_ticks(g.x * rs, F.rea_scale));
+  atomicAdd(F.reaction + 3 * (size_t)j + 1, pf_ticks(g.y * rs, F.rea_scale));
+  atomicAdd(F.reaction + 3 * (size_t)j + 2, pf_ticks(g.z * rs, F.rea_scale));
 }
 
 template <bool WALLS, bool ADVECT>
@@ -806,6 +826,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
   if (in_range(S.id[p][s], F.start, F.count)) {
     const uint32_t cnt = F.nl_count[s];
     const float li = xi4.w;
+    const bool react = F.nboundary != 0u && li != 0.0f;
+    const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
     float3 sum = make_float3(0.f, 0.f, 0.f);
     if (staged) {
       Nl16Cursor cur(F.nl16, n, s);
@@ -820,7 +842,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
           mj[u] = u_m[idx[u]];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+        for (int u = 0; u < 4; ++u) {
+          const float3 g = delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+          if (react && k0 + u < cnt) reaction_add(S, F, p, uslots[idx[u]], li, g);
+        }
       }
     } else {
       NlCursor cur(F.nl, n, s, cnt);
@@ -835,7 +860,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
           mj[u] = neighbor_mass(S, F, p, jj[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+        for (int u = 0; u < 4; ++u) {
+          const float3 g = delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+          if (react && k0 + u < cnt) reaction_add(S, F, p, jj[u], li, g);
+        }
       }
     }
     xn = make_float3(xn.x + (sum.x / F.rho0) * w, xn.y + (sum.y / F.rho0) * w,
@@ -857,6 +885,79 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     }
   }
   S.xs[c ^ 1][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+}
+
+// Apply and clear the boundary reaction over the span covering the fluid's
+// boundary ranges (accumulator.ts:101-131 over coveringRange, FluidSystem.ts:280-287).
+__global__ __launch_bounds__(PF_BLOCK) void k_reaction_apply(const SceneDev* __restrict__ scenes,
+                                                             const Tile* __restrict__ tiles,
+                                                             uint32_t ntiles, int fi, int p, int c) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || F.nboundary == 0u) return;
+  const uint32_t id = S.id[p][s];
+  if (!(id >= F.rcover_lo && id < F.rcover_hi)) return;
+  int* acc = F.reaction + 3 * (size_t)s;
+  const int tk[3] = {acc[0], acc[1], acc[2]};
+  bool sat;
+  const float3 ch = ticks_to_change(tk, F.rea_inv, &sat);
+  if (sat) flag_set(S, 0, PF_FLAG_REACTION);
+  float4 x = S.xs[c][s];
+  S.xs[c][s] = make_float4(x.x + ch.x, x.y + ch.y, x.z + ch.z, x.w);
+  acc[0] = 0;
+  acc[1] = 0;
+  acc[2] = 0;
+}
+
+// Akinci boundary volume V_i = 1 / max(sum_k W(x_i - x_k), 1e-12) over the
+// boundary's own range (boundaryVolume.ts:26-52), for the slots of this
+// fluid's dynamic boundaries (predicted positions, every substep) or static
+// boundaries (committed positions, once). The walk is the reference's
+// 27-bucket query with bucket dedup (query.ts:24-70).
+__global__ __launch_bounds__(PF_BLOCK) void k_boundary_volume(const SceneDev* __restrict__ scenes,
+                                                              const Tile* __restrict__ tiles,
+                                                              uint32_t ntiles, int fi, int p, int c,
+                                                              int dynamic) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || F.nboundary == 0u) return;
+  const uint32_t id = S.id[p][s];
+  int b = -1;
+  for (uint32_t k = 0; k < F.nboundary; ++k)
+    if (in_range(id, F.bstart[k], F.bcount[k]) && (int)((F.bdynamic >> k) & 1u) == dynamic) b = (int)k;
+  if (b < 0) return;
+  const uint32_t lo = F.bstart[b], cnt = F.bcount[b];
+  const float4* __restrict__ pos = dynamic ? S.xs[c] : S.x[p];
+  const float3 xi = pf_xyz(pos[s]);
+  const int qx = pf_query_coord(xi.x, S.ox, S.cell);
+  const int qy = pf_query_coord(xi.y, S.oy, S.cell);
+  const int qz = pf_query_coord(xi.z, S.oz, S.cell);
+  uint32_t visited[27];
+  uint32_t nvisited = 0;
+  float sum = 0.0f;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const uint32_t bk = pf_morton((int)((uint32_t)qx + (uint32_t)dx), (int)((uint32_t)qy + (uint32_t)dy),
+                                      (int)((uint32_t)qz + (uint32_t)dz)) & S.hash_mask;
+        bool seen = false;
+        for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == bk);
+        visited[nvisited++] = bk;
+        if (seen) continue;
+        for (uint32_t j = S.start[bk]; j < S.start[bk + 1]; ++j) {
+          if (!in_range(S.id[p][j], lo, cnt)) continue;
+          const float3 xj = pf_xyz(pos[j]);
+          const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+          const float r2 = pf_dot3(d, d);
+          if (r2 >= F.h2) continue;
+          sum = sum + pf_poly6(r2, F.h2, F.poly6);
+        }
+      }
+  S.bvol[p][s] = 1.0f / fmaxf(sum, 1e-12f);
 }
 
 // Container contact + position accumulator apply (+ advect), in place.
@@ -1189,6 +1290,18 @@ void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t n
   if (!ntiles) return;
   if (advect) hipLaunchKernelGGL(k_walls<true>, PF_GRID, sc, tl, ntiles, p, c, first_iteration, substep);
   else hipLaunchKernelGGL(k_walls<false>, PF_GRID, sc, tl, ntiles, p, c, first_iteration, substep);
+}
+
+void launch_reaction_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                           int c) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_reaction_apply, PF_GRID, sc, tl, ntiles, fi, p, c);
+}
+
+void launch_boundary_volume(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                            int c, int dynamic) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_boundary_volume, PF_GRID, sc, tl, ntiles, fi, p, c, dynamic);
 }
 
 void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c) {

@@ -64,6 +64,7 @@ struct FluidDev {
   uint32_t nboundary;      // boundary ranges (FluidSystem.addBoundary)
   uint32_t bstart[PF_MAX_BOUNDARIES], bcount[PF_MAX_BOUNDARIES];
   uint32_t bdynamic;       // bit b: boundary b recomputed every substep
+  uint32_t rcover_lo, rcover_hi; // id span covering every boundary range (reaction apply)
   float rho0, pv, mass;    // rest density, spacing^3, rho0*spacing^3 (f32 uniforms)
   float inv_mass;          // 1 / mass, rounded once
   float h, h2, poly6, spiky; // kernels.ts:22-47 (coefficients computed in double)

@@ -27,6 +27,10 @@ void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t n
 void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c,
                   bool advect, int first_iteration, int substep);
 void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c);
+void launch_reaction_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                           int c);
+void launch_boundary_volume(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                            int c, int dynamic);
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, bool vort, bool visc);
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,

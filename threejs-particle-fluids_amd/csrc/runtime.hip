@@ -101,7 +101,7 @@ void Particles::upload_desc() {
 // Utility kernels: original order <-> cell-sorted order
 // ---------------------------------------------------------------------------
 enum GatherWhat {
-  G_POS = 0, G_PRED = 1, G_VEL = 2, G_INVMASS = 3, G_DENSITY = 4, G_KEY = 5, G_IDS = 6
+  G_POS = 0, G_PRED = 1, G_VEL = 2, G_INVMASS = 3, G_DENSITY = 4, G_KEY = 5, G_IDS = 6, G_BVOL = 7
 };
 
 __global__ void k_gather_by_id(const SceneDev* __restrict__ sc, int p, int c, uint32_t n, int what,
@@ -122,6 +122,7 @@ __global__ void k_gather_by_id(const SceneDev* __restrict__ sc, int p, int c, ui
     case G_DENSITY: reinterpret_cast<float*>(out)[id] = S.rho[p][s]; break;
     case G_KEY: reinterpret_cast<uint32_t*>(out)[id] = S.tmp_key[s]; break;
     case G_IDS: reinterpret_cast<uint32_t*>(out)[s] = id; break;
+    case G_BVOL: reinterpret_cast<float*>(out)[id] = S.bvol[p][s]; break;
   }
 }
 
@@ -510,6 +511,15 @@ int pf_particles_readback(pf_particles* ps, const pf_snapshot* out) {
     for (uint32_t i = 0; i < ps->n; ++i) { q[4 * i] = 0; q[4 * i + 1] = 0; q[4 * i + 2] = 0; q[4 * i + 3] = 1; }
   }
   if (out->angular_velocity) std::memset(out->angular_velocity, 0, sizeof(float) * 4 * (size_t)ps->n);
+  PF_API_END
+}
+
+int pf_particles_readback_boundary_volume(pf_particles* ps, float* volume) {
+  PF_API_BEGIN
+  check_alive(ps, "ParticleSystem.readback");
+  PF_REQUIRE(volume, "ParticleSystem.readback");
+  if (ps->loop) PF_HIP(hipStreamSynchronize(ps->ctx->stream));
+  particles_gather(ps, G_BVOL, volume, sizeof(float));
   PF_API_END
 }
 

@@ -201,6 +201,8 @@ struct PlanShape {
   uint32_t nfluids = 0;
   uint32_t fx[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of effects per fluid slot
   bool any_boundary = false;
+  uint32_t nb[PF_MAX_FLUIDS] = {0, 0, 0, 0};    // boundary ranges per fluid slot (max over scenes)
+  uint32_t bdyn[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of dynamic-boundary bits per fluid slot
   bool walls = false;       // any scene has primitives
   bool soft = false;        // any scene has soft bodies
 };
@@ -220,6 +222,7 @@ struct pf_loop_s {
   uint32_t nstiles = 0;
   uint2* btiles = nullptr;
   uint32_t nbtiles = 0;
+  bool static_boundaries = false;  // a fluid has a static boundary (volumes computed on the first step)
   double last_dt = -1;
   uint32_t last_flag_half = 0;
   uint64_t steps = 0;

@@ -48,10 +48,16 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
   int c = 0;
   const bool grid = sh.nfluids > 0;
   enum LastOp { NONE, DELTA, WALLS, SOFT };
+  // The last fluid's correction pass can carry the container contact and the
+  // advect only when nothing runs after it inside an iteration (soft bodies,
+  // a boundary reaction apply).
+  bool reaction = false;
+  for (uint32_t f = 0; f < sh.nfluids; ++f) reaction = reaction || sh.nb[f] > 0;
+  const bool delta_can_end = !sh.soft && !reaction && sh.nfluids > 0;
   LastOp last = NONE;
-  if (sh.walls) last = (sh.soft || sh.nfluids == 0) ? WALLS : DELTA;
+  if (sh.walls) last = delta_can_end ? DELTA : WALLS;
   else if (sh.soft) last = SOFT;
-  else if (sh.nfluids > 0) last = DELTA;
+  else if (delta_can_end) last = DELTA;
   for (uint32_t k = 0; k < sh.substeps; ++k) {
     int p = 0;
     if (grid) {
@@ -67,6 +73,13 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       hipLaunchKernelGGL(k_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, 0, c, k == 0);
       ++nk;
       c ^= 1;
+    }
+    for (uint32_t f = 0; f < sh.nfluids; ++f) {
+      if (sh.bdyn[f]) {
+        PF_LAUNCH(ctx, "boundary_volume");
+        launch_boundary_volume(st, scenes, tiles, ntiles, (int)f, p, c, 1);
+        ++nk;
+      }
     }
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       PF_LAUNCH(ctx, "neighbors");
@@ -96,6 +109,11 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
           ++nk;
         }
         c ^= 1;
+        if (sh.nb[f]) {
+          PF_LAUNCH(ctx, "reaction_apply");
+          launch_reaction_apply(st, scenes, tiles, ntiles, (int)f, p, c);
+          ++nk;
+        }
       }
       if (sh.soft) {
         PF_LAUNCH(ctx, "shape_match");
@@ -786,7 +804,12 @@ static void fluid_write_params(pf_fluid* f) {
   F.rea_scale = (float)s10;
   F.rea_inv = (float)(1.0 / s10);
   F.nboundary = (uint32_t)f->boundaries.size();
+  F.bdynamic = 0;
+  F.rcover_lo = 0xffffffffu;
+  F.rcover_hi = 0;
   for (uint32_t b = 0; b < F.nboundary; ++b) {
+    F.rcover_lo = std::min(F.rcover_lo, f->boundaries[b].start);
+    F.rcover_hi = std::max(F.rcover_hi, f->boundaries[b].start + f->boundaries[b].count);
     F.bstart[b] = f->boundaries[b].start;
     F.bcount[b] = f->boundaries[b].count;
     if (f->boundaries[b].dynamic) F.bdynamic |= 1u << b;
@@ -814,6 +837,10 @@ static PlanShape loop_shape(const pf_loop* L) {
   sh.walls = L->prims && !L->prims->prims.empty();
   sh.soft = L->soft != nullptr;
   sh.any_boundary = L->ps->host.has_boundary != 0;
+  for (uint32_t f = 0; f < sh.nfluids; ++f) {
+    sh.nb[f] = L->ps->host.fluid[f].nboundary;
+    sh.bdyn[f] = L->ps->host.fluid[f].bdynamic;
+  }
   return sh;
 }
 
@@ -858,8 +885,6 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
       if (f->loop) fail(PF_EINVAL, "SimLoop: a fluid belongs to another SimLoop");
       if (L->soft) fail(PF_EINVAL, "SimLoop: list fluids before the soft-body system");
       if (L->fluids.size() >= PF_MAX_FLUIDS) fail(PF_EINVAL, "SimLoop: at most %d fluids per loop", PF_MAX_FLUIDS);
-      if (!f->boundaries.empty())
-        fail(PF_EINVAL, "FluidSystem.addBoundary: fluid boundaries are not supported by this build");
       L->fluids.push_back(f);
       neighbor_radius = std::max(neighbor_radius, f->h);
     } else if (mats[m].kind == PF_MATERIAL_SOFTBODY) {
@@ -926,6 +951,11 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
+    if (!f->boundaries.empty()) {
+      f->reaction = dalloc<int>(3 * (size_t)n);
+      S.has_boundary = 1u;
+      for (auto& b : f->boundaries) L->static_boundaries = L->static_boundaries || !b.dynamic;
+    }
     fluid_write_params(f);
   }
   const double s10 = std::floor(1073741824.0 / 10.0), s50 = std::floor(1073741824.0 / 50.0);
@@ -1001,6 +1031,33 @@ static void loop_invalidate_graphs(pf_loop* L) {
   }
 }
 
+// First step of a loop with static boundaries (loop.ts:236, 276-277): sort
+// the committed positions (a prediction over dt = 0 leaves them in place)
+// and compute the static boundary volumes from them, once.
+static void loop_static_boundaries(pf_loop* L) {
+  pf_particles* ps = L->ps;
+  Context* ctx = L->ctx;
+  hipStream_t st = ctx->stream;
+  SceneDev& S = ps->host;
+  const float dt_keep = S.dt;
+  S.dt = 0.0f;
+  ps->dirty = true;
+  ps->upload_desc();
+  const SceneDev* d = ps->cur();
+  launch_hash(st, d, L->tiles, L->ntiles, 1, 0, 0, 0);
+  launch_scan(st, d, L->stiles, L->nstiles, ctx->scan_ctl);
+  launch_place(st, d, L->tiles, L->ntiles, 1, 0);
+  launch_rank(st, d, L->tiles, L->ntiles, 1, 0, 0, 1, nullptr, nullptr, nullptr);
+  for (uint32_t f = 0; f < (uint32_t)L->fluids.size(); ++f)
+    if (S.fluid[f].nboundary) launch_boundary_volume(st, d, L->tiles, L->ntiles, (int)f, 1, 1, 0);
+  PF_HIP(hipGetLastError());
+  PF_HIP(hipStreamSynchronize(st));
+  // The sorted state is the loop's state from here: swap both pairs.
+  ps->variant ^= 3;
+  S.dt = dt_keep;
+  ps->dirty = true;
+}
+
 int pf_loop_step(pf_loop* L, double dt) {
   PF_API_BEGIN
   loop_alive(L, "SimLoop.step");
@@ -1013,6 +1070,7 @@ int pf_loop_step(pf_loop* L, double dt) {
     if (ps->host.nprim != before) loop_invalidate_graphs(L);
   }
   loop_set_dt(L, dt);
+  if (L->static_boundaries && L->steps == 0) loop_static_boundaries(L);
   ps->upload_desc();
   const PlanShape sh = loop_shape(L);
   const int v = ps->variant;
@@ -1181,6 +1239,10 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
     sh.walls = sh.walls || ls.walls;
     sh.soft = sh.soft || ls.soft;
     sh.any_boundary = sh.any_boundary || ls.any_boundary;
+    for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
+      sh.nb[f] = std::max(sh.nb[f], ls.nb[f]);
+      sh.bdyn[f] |= ls.bdyn[f];
+    }
     for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{s, i * PF_BLOCK});
     for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(s, i));
     for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(s, i));
@@ -1219,6 +1281,7 @@ int pf_batch_step(pf_batch* B, double dt) {
     loop_alive(L, "Batch");
     if (L->prims) L->prims->upload();
     loop_set_dt(L, dt);
+    if (L->static_boundaries && L->steps == 0) loop_static_boundaries(L);
     changed = changed || L->ps->dirty;
   }
   if (changed || B->desc_dirty) {

@@ -149,6 +149,12 @@ class ParticleSystem:
         call("pf_particles_readback", self.handle, C.byref(snap))
         return ParticleSnapshot(n, *arrs)
 
+    def readbackBoundaryVolume(self):
+        """The boundaryVolume buffer (Akinci volumes of boundary particles, else 0)."""
+        v = np.zeros(self.capacity, np.float32)
+        call("pf_particles_readback_boundary_volume", self.handle, _ptr(v, f32p))
+        return v
+
     def dispose(self):
         if self.handle:
             call("pf_particles_destroy", self.handle)

@@ -109,6 +109,7 @@ SIGNATURES = {
     "pf_particles_set_inv_mass": [vp, u32, u32, f64],
     "pf_particles_set_collision_group": [vp, u32, u32, u32],
     "pf_particles_readback": [vp, P(Snapshot)],
+    "pf_particles_readback_boundary_volume": [vp, P(f32)],
     "pf_particles_info": [vp, P(u32), P(f64)],
     "pf_grid_create": [vp, f64, u32, P(f64), P(vp)],
     "pf_grid_create_standalone": [vp, u32, f64, u32, P(f64), P(vp)],
