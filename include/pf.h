@@ -35,6 +35,7 @@ typedef struct pf_grid_s pf_grid;
 typedef struct pf_fluid_s pf_fluid;
 typedef struct pf_primitives_s pf_primitives;
 typedef struct pf_softbody_s pf_softbody;
+typedef struct pf_sdf_collider_s pf_sdf_collider;
 typedef struct pf_loop_s pf_loop;
 typedef struct pf_batch_s pf_batch;
 
@@ -202,6 +203,31 @@ int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double complianc
 int pf_softbody_readback_frames(pf_softbody* sb, float* centers4, float* rotations12);
 int pf_softbody_destroy(pf_softbody* sb);
 
+/* ---- SDFCollider (src/core/collision/SDFCollider.ts:68-311) --------------
+ * data: one distance per voxel, x fastest, then y, then z (SDFData, 32-39);
+ * stored as half floats converted like three's DataUtils.toHalfFloat
+ * (clamped to +-65504, mantissa truncated) and sampled trilinearly with
+ * clamp-to-edge addressing (sdf.ts:107-146). mu_s / mu_k < 0 take the
+ * defaults 0.5 / 0.4; rotation (xyzw) is normalised, nullable = identity. */
+typedef struct pf_sdf_options {
+  double mu_s, mu_k;
+  double position[3];
+  const double* rotation;
+  double scale;      /* <= 0: 1 */
+  double thickness;  /* extra contact distance beyond the particle radius */
+} pf_sdf_options;
+int pf_sdf_collider_create(pf_particles* ps, const float* data, const uint32_t resolution[3],
+                           const double origin[3], const double voxel_size[3],
+                           const pf_sdf_options* options, pf_sdf_collider** out);
+/* setPosition / setRotation / setScale (SDFCollider.ts:186-212) */
+int pf_sdf_collider_set_position(pf_sdf_collider* c, const double position[3]);
+int pf_sdf_collider_set_rotation(pf_sdf_collider* c, const double rotation[4]);
+int pf_sdf_collider_set_scale(pf_sdf_collider* c, double scale);
+/* muS / muK / thickness setters (SDFCollider.ts:150-172) */
+int pf_sdf_collider_set_friction(pf_sdf_collider* c, double mu_s, double mu_k);
+int pf_sdf_collider_set_thickness(pf_sdf_collider* c, double thickness);
+int pf_sdf_collider_destroy(pf_sdf_collider* c);
+
 /* ---- SimLoop (src/core/loop.ts:92-324) ----------------------------------- */
 #define PF_MATERIAL_FLUID 1u
 #define PF_MATERIAL_SOFTBODY 2u
@@ -209,6 +235,12 @@ typedef struct pf_material {
   uint32_t kind;   /* PF_MATERIAL_* */
   void* handle;    /* pf_fluid* or pf_softbody* */
 } pf_material;
+#define PF_COLLIDER_PRIMITIVES 1u
+#define PF_COLLIDER_SDF 2u
+typedef struct pf_collider {
+  uint32_t kind;   /* PF_COLLIDER_* */
+  void* handle;    /* pf_primitives* or pf_sdf_collider* */
+} pf_collider;
 typedef struct pf_loop_options {
   uint32_t substeps;      /* 0 = default 4 */
   uint32_t iterations;    /* 0xffffffff = default 2 */
@@ -220,8 +252,10 @@ typedef struct pf_overflow {
   int positions, velocities, grid, contacts, neighbors, impulses, long_bucket;
 } pf_overflow;
 
+/* new SimLoop(particles, {substeps, iterations, gravity, materials, colliders})
+ * (loop.ts:117-241): at most one PrimitiveSet and four SDFColliders. */
 int pf_loop_create(pf_particles* ps, const pf_loop_options* options, const pf_material* materials,
-                   uint32_t nmaterials, pf_primitives* const* colliders, uint32_t ncolliders,
+                   uint32_t nmaterials, const pf_collider* colliders, uint32_t ncolliders,
                    pf_loop** out);
 /* step(dt) (loop.ts:267-279): enqueues every substep and returns. */
 int pf_loop_step(pf_loop* loop, double dt);

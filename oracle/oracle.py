@@ -39,6 +39,10 @@ def lib():
     L.orc_grid_build.restype = i32
     L.orc_grid_query_counts.argtypes = [P(f32), u32, f32, P(f32), u32, P(f32), u32, f32, P(u32), P(u32)]
     L.orc_grid_query_counts.restype = None
+    L.orc_float_to_half.argtypes = [f32]
+    L.orc_float_to_half.restype = C.c_uint16
+    L.orc_half_to_double.argtypes = [C.c_uint16]
+    L.orc_half_to_double.restype = f64
     for suf in ("f32", "f64"):
         g = lambda n: getattr(L, f"orc_{n}_{suf}")
         g("scene_create").argtypes = [u32, f64]
@@ -55,6 +59,10 @@ def lib():
         g("scene_add_prim").restype = i32
         g("scene_add_body").argtypes = [vp, u32, u32, f64, P(f32), P(f32)]
         g("scene_add_body").restype = i32
+        g("scene_add_sdf").argtypes = [vp, P(f32), P(u32), P(f64), P(f64), P(f64), P(f64), f64, f64, f64, f64]
+        g("scene_add_sdf").restype = i32
+        g("scene_sdf_sample").argtypes = [vp, u32, P(f64), P(f64), P(f64)]
+        g("scene_sdf_sample").restype = None
         g("scene_add_boundary").argtypes = [vp, u32, u32, u32, i32]
         g("scene_add_boundary").restype = i32
         g("scene_get_bvol").argtypes = [vp, P(f64)]
@@ -118,6 +126,15 @@ def grid_query_counts(pos4, cell, H, query4, radius, origin=(0.0, 0.0, 0.0)):
     return cand, within
 
 
+def float_to_half(v):
+    """Half-float bits of v as three's DataUtils.toHalfFloat gives them."""
+    return int(lib().orc_float_to_half(float(v)))
+
+
+def half_to_float(h):
+    return float(lib().orc_half_to_double(int(h)))
+
+
 def polar(a, precision="f32"):
     a9 = np.ascontiguousarray(np.asarray(a, np.float64).reshape(9))
     r9 = np.zeros(9, np.float64)
@@ -175,6 +192,24 @@ class Scene:
         w = np.ascontiguousarray(weights, np.float32)
         rc = self._f("scene_add_body")(self.h, int(start), int(count), float(compliance), _p(r, f32), _p(w, f32))
         assert rc == 0
+
+    def add_sdf(self, data, resolution, origin, voxel_size, position=(0.0, 0.0, 0.0), rotation=None, scale=1.0,
+                thickness=0.0, mus=0.5, muk=0.4):
+        """An SDF collider; rotation is a row-major 3x3 matrix (identity if None)."""
+        d = np.ascontiguousarray(data, np.float32).reshape(-1)
+        res = np.asarray(resolution, np.uint32)
+        rot = np.eye(3) if rotation is None else np.asarray(rotation, np.float64).reshape(3, 3)
+        arr = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (origin, voxel_size, position, rot)]
+        rc = self._f("scene_add_sdf")(self.h, _p(d, f32), _p(res, u32), *[_p(a, f64) for a in arr], float(scale),
+                                      float(thickness), float(mus), float(muk))
+        assert rc == 0
+
+    def sdf_sample(self, k, x):
+        xx = np.asarray(x, np.float64)
+        phi = np.zeros(1)
+        g = np.zeros(3)
+        self._f("scene_sdf_sample")(self.h, int(k), _p(xx, f64), _p(phi, f64), _p(g, f64))
+        return float(phi[0]), g
 
     def add_boundary(self, fluid, start, count, dynamic=True):
         rc = self._f("scene_add_boundary")(self.h, int(fluid), int(start), int(count), 1 if dynamic else 0)

@@ -39,6 +39,14 @@ typedef struct {
 } SFX(prim);
 
 typedef struct {
+  uint16_t* grid; /* half floats, x fastest */
+  int nx, ny, nz;
+  REAL res[3], inv_res[3], origin[3], voxel[3], inv_voxel[3], pos[3];
+  REAL rot[9], inv_rot[9]; /* row-major */
+  REAL scale, inv_scale, thickness, mus, muk;
+} SFX(sdfc);
+
+typedef struct {
   uint32_t n;
   REAL radius;
   REAL *x, *xs, *v, *w, *rho, *bvol; /* x, xs, v: [3n] */
@@ -52,6 +60,9 @@ typedef struct {
   uint32_t nprim;
   SFX(prim) prim[32];
   REAL* plam;      /* [2 * 32 * n] */
+  uint32_t nsdf;
+  SFX(sdfc) sdf[4];
+  REAL* sdflam;    /* [2 * 4 * n] */
   int32_t *pacc, *vacc, *iacc, *racc; /* [3n] fixed-point accumulators */
   int initialized;                     /* first step ran (static boundary volumes) */
   REAL pos_scale, pos_inv, vel_scale, vel_inv;
@@ -558,6 +569,104 @@ static void SFX(surface_velocity)(const SFX(prim)* P, const REAL xw[3], REAL rew
 }
 
 /* PrimitiveSet.ts:336-364 then the positions accumulator apply (loop.ts:222) */
+/* half-float voxel (i, j, k) */
+static inline REAL SFX(texel)(const SFX(sdfc)* D, int i, int j, int k) {
+  return (REAL)orc_half_to_double(D->grid[((size_t)k * D->ny + j) * D->nx + i]);
+}
+static inline REAL SFX(lerp)(REAL a, REAL b, REAL f) { return a + (b - a) * f; }
+static inline int SFX(clampi)(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* Linear filtering at normalised uv with clamp-to-edge addressing: texel
+ * coordinate t = uv * n - 0.5, weights from its fraction, lerp x, y, z. */
+static REAL SFX(tex)(const SFX(sdfc)* D, REAL u, REAL v, REAL w) {
+  REAL t[3] = {u * D->res[0] - R(0.5), v * D->res[1] - R(0.5), w * D->res[2] - R(0.5)};
+  int n[3] = {D->nx, D->ny, D->nz}, i0[3], i1[3];
+  REAL f[3];
+  for (int a = 0; a < 3; ++a) {
+    REAL fl = RFLOOR(t[a]);
+    f[a] = t[a] - fl;
+    int ii = SFX(clampi)(orc_f2i_sat((double)fl), -1, n[a] - 1);
+    i0[a] = ii < 0 ? 0 : ii;
+    i1[a] = ii + 1 > n[a] - 1 ? n[a] - 1 : ii + 1;
+  }
+  REAL c00 = SFX(lerp)(SFX(texel)(D, i0[0], i0[1], i0[2]), SFX(texel)(D, i1[0], i0[1], i0[2]), f[0]);
+  REAL c10 = SFX(lerp)(SFX(texel)(D, i0[0], i1[1], i0[2]), SFX(texel)(D, i1[0], i1[1], i0[2]), f[0]);
+  REAL c01 = SFX(lerp)(SFX(texel)(D, i0[0], i0[1], i1[2]), SFX(texel)(D, i1[0], i0[1], i1[2]), f[0]);
+  REAL c11 = SFX(lerp)(SFX(texel)(D, i0[0], i1[1], i1[2]), SFX(texel)(D, i1[0], i1[1], i1[2]), f[0]);
+  return SFX(lerp)(SFX(lerp)(c00, c10, f[1]), SFX(lerp)(c01, c11, f[1]), f[2]);
+}
+
+/* emitSampleSdf (sdf.ts:107-146): world -> local -> uv, trilinear phi,
+ * 6-tap central-difference gradient, back to world. */
+static void SFX(sdf_sample)(const SFX(sdfc)* D, const REAL x[3], REAL* phi, REAL g[3]) {
+  REAL wo[3] = {x[0] - D->pos[0], x[1] - D->pos[1], x[2] - D->pos[2]}, uv[3];
+  for (int r = 0; r < 3; ++r) {
+    REAL xl = ((D->inv_rot[3 * r] * wo[0] + D->inv_rot[3 * r + 1] * wo[1]) + D->inv_rot[3 * r + 2] * wo[2]) * D->inv_scale;
+    uv[r] = ((xl - D->origin[r]) * D->inv_voxel[r]) / D->res[r];
+  }
+  REAL pl = SFX(tex)(D, uv[0], uv[1], uv[2]);
+  REAL gl[3];
+  gl[0] = (SFX(tex)(D, uv[0] + D->inv_res[0], uv[1], uv[2]) - SFX(tex)(D, uv[0] - D->inv_res[0], uv[1], uv[2])) / (R(2.0) * D->voxel[0]);
+  gl[1] = (SFX(tex)(D, uv[0], uv[1] + D->inv_res[1], uv[2]) - SFX(tex)(D, uv[0], uv[1] - D->inv_res[1], uv[2])) / (R(2.0) * D->voxel[1]);
+  gl[2] = (SFX(tex)(D, uv[0], uv[1], uv[2] + D->inv_res[2]) - SFX(tex)(D, uv[0], uv[1], uv[2] - D->inv_res[2])) / (R(2.0) * D->voxel[2]);
+  *phi = pl * D->scale;
+  for (int r = 0; r < 3; ++r) g[r] = (D->rot[3 * r] * gl[0] + D->rot[3 * r + 1] * gl[1]) + D->rot[3 * r + 2] * gl[2];
+}
+
+/* SDFCollider solve (SDFCollider.ts:237-269 + collider.ts:53-89), static
+ * collider: ticks into the positions accumulator of particle p. */
+static void SFX(sdf_solve)(SFX(scene)* s, uint32_t p, REAL w, const REAL* xs, const REAL disp[3]) {
+  for (uint32_t k = 0; k < s->nsdf; ++k) {
+    const SFX(sdfc)* D = &s->sdf[k];
+    REAL phi, g[3];
+    SFX(sdf_sample)(D, xs, &phi, g);
+    REAL gl = RSQRT(SFX(dot)(g, g));
+    if (!(gl > (REAL)1e-6)) continue;
+    REAL nrm[3] = {g[0] / gl, g[1] / gl, g[2] / gl};
+    REAL reach = s->radius + D->thickness;
+    if (!(phi < reach)) continue;
+    REAL depth = reach - phi;
+    REAL* lam = s->sdflam + ((size_t)p * 4 + k) * 2;
+    REAL ln = lam[0] + depth / w;
+    lam[0] = ln;
+    REAL sd = SFX(dot)(disp, nrm), tang[3];
+    for (int a = 0; a < 3; ++a) tang[a] = disp[a] - nrm[a] * sd;
+    REAL tl = RSQRT(SFX(dot)(tang, tang));
+    REAL lt = lam[1];
+    int sticks = tl > (REAL)1e-10 && (tl / w) <= (D->mus * ln - lt);
+    for (int a = 0; a < 3; ++a) {
+      REAL push = nrm[a] * depth;
+      REAL corr = sticks ? push - tang[a] : push;
+      s->pacc[3 * (size_t)p + a] = SFX(wadd)(s->pacc[3 * (size_t)p + a], SFX(ticks)(corr, s->pos_scale));
+    }
+    lam[1] = lt + (sticks ? tl / w : R(0.0));
+  }
+}
+
+/* SDFCollider friction (SDFCollider.ts:271-291 + collider.ts:96-118) */
+static void SFX(sdf_friction)(SFX(scene)* s, uint32_t p, REAL w, const REAL* xs, const REAL* v) {
+  for (uint32_t k = 0; k < s->nsdf; ++k) {
+    const SFX(sdfc)* D = &s->sdf[k];
+    REAL ln = s->sdflam[((size_t)p * 4 + k) * 2];
+    if (!(ln > R(0.0))) continue;
+    REAL phi, g[3];
+    SFX(sdf_sample)(D, xs, &phi, g);
+    REAL gl = RSQRT(SFX(dot)(g, g));
+    if (!(gl > (REAL)1e-6)) continue;
+    REAL nrm[3] = {g[0] / gl, g[1] / gl, g[2] / gl}, vt[3];
+    REAL nd = SFX(dot)(nrm, v);
+    for (int a = 0; a < 3; ++a) vt[a] = v[a] - nrm[a] * nd;
+    REAL vl = RSQRT(SFX(dot)(vt, vt));
+    if (vl > (REAL)1e-6) {
+      REAL change = RFMIN(((D->muk * ln) * w) / s->dt, vl);
+      for (int a = 0; a < 3; ++a) {
+        REAL d = -((vt[a] / vl) * change);
+        s->vacc[3 * (size_t)p + a] = SFX(wadd)(s->vacc[3 * (size_t)p + a], SFX(ticks)(d, s->vel_scale));
+      }
+    }
+  }
+}
+
 static void SFX(walls_solve)(SFX(scene)* s, REAL clock) {
   uint32_t n = s->n;
   for (uint32_t p = 0; p < n; ++p) {
@@ -590,6 +699,7 @@ static void SFX(walls_solve)(SFX(scene)* s, REAL clock) {
         lam[1] = lt + (sticks ? tl / w : R(0.0));
       }
     }
+    if (s->nsdf) SFX(sdf_solve)(s, p, w, xs, disp);
   }
   for (uint32_t p = 0; p < n; ++p) {
     REAL c[3];
@@ -630,6 +740,7 @@ static void SFX(walls_friction)(SFX(scene)* s, REAL clock) {
         }
       }
     }
+    if (s->nsdf) SFX(sdf_friction)(s, p, w, xs, v);
   }
   for (uint32_t p = 0; p < n; ++p) {
     REAL c[3];
@@ -802,6 +913,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       for (uint32_t b = 0; b < s->fluid[f].nb; ++b)
         if (s->fluid[f].bdyn[b]) SFX(boundary_volume)(s, &s->fluid[f], b, s->xs);
     memset(s->plam, 0, sizeof(REAL) * 64 * (size_t)n);
+    if (s->sdflam) memset(s->sdflam, 0, sizeof(REAL) * 8 * (size_t)n);
     for (uint32_t f = 0; f < s->nfluids; ++f) SFX(neighbors)(s, &s->fluid[f]);
     for (uint32_t f = 0; f < s->nfluids; ++f) {
       SFX(fluid)* F = &s->fluid[f];
@@ -814,7 +926,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
         SFX(delta_pass)(s, &s->fluid[f]);
       }
       if (s->nbodies) SFX(shape_match)(s, it == 0);
-      if (s->nprim) SFX(walls_solve)(s, s->clock[k]);
+      if (s->nprim || s->nsdf) SFX(walls_solve)(s, s->clock[k]);
     }
     /* advect (integrate.ts:36-44) */
     for (uint32_t i = 0; i < n; ++i) {
@@ -825,6 +937,6 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       }
     }
     for (uint32_t f = 0; f < s->nfluids; ++f) SFX(velocity_passes)(s, &s->fluid[f]);
-    if (s->nprim && s->iterations > 0) SFX(walls_friction)(s, s->clock[k]);
+    if ((s->nprim || s->nsdf) && s->iterations > 0) SFX(walls_friction)(s, s->clock[k]);
   }
 }

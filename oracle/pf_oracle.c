@@ -46,6 +46,55 @@ int32_t orc_f2i_sat(double f) {
   return (int32_t)f;
 }
 
+/* ---------------------------------------------------------------------- */
+/* half floats (SDFCollider.ts:97-101)                                     */
+/* ---------------------------------------------------------------------- */
+/* three@0.184.0 DataUtils.toHalfFloat (not vendored in the reference): the
+ * value is clamped to +-65504, then converted with the base/shift tables of
+ * van der Zijp's "Fast Half Float Conversions" (2008), which truncate the
+ * mantissa: half = base[sign|exponent] + (mantissa >> shift[sign|exponent]). */
+static uint16_t orc_half_base[512];
+static uint8_t orc_half_shift[512];
+static int orc_half_ready = 0;
+
+static void orc_half_tables(void) {
+  for (int i = 0; i < 256; ++i) {
+    int e = i - 127;
+    uint16_t b;
+    uint8_t sh;
+    if (e < -27) { b = 0x0000; sh = 24; }
+    else if (e < -14) { b = (uint16_t)(0x0400 >> (-e - 14)); sh = (uint8_t)(-e - 1); }
+    else if (e <= 15) { b = (uint16_t)((e + 15) << 10); sh = 13; }
+    else if (e < 128) { b = 0x7c00; sh = 24; }
+    else { b = 0x7c00; sh = 13; }
+    orc_half_base[i] = b;
+    orc_half_base[i | 0x100] = (uint16_t)(b | 0x8000);
+    orc_half_shift[i] = sh;
+    orc_half_shift[i | 0x100] = sh;
+  }
+  orc_half_ready = 1;
+}
+
+uint16_t orc_float_to_half(float v) {
+  if (!orc_half_ready) orc_half_tables();
+  if (v > 65504.0f) v = 65504.0f;
+  if (v < -65504.0f) v = -65504.0f;
+  uint32_t f;
+  memcpy(&f, &v, 4);
+  uint32_t e = (f >> 23) & 0x1ffu;
+  return (uint16_t)(orc_half_base[e] + ((f & 0x007fffffu) >> orc_half_shift[e]));
+}
+
+/* exact value of a half float */
+double orc_half_to_double(uint16_t h) {
+  int sgn = h >> 15, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+  double v;
+  if (e == 0) v = ldexp((double)m, -24);
+  else if (e == 31) v = m ? NAN : INFINITY;
+  else v = ldexp((double)(m | 0x400), e - 25);
+  return sgn ? -v : v;
+}
+
 /* Bucket of one FP32 position: (p - o) / cell in FP32, floor, saturate,
  * clamp to +-2^30, Morton, mask (cellIndex.ts:113-170). */
 uint32_t orc_bucket(const float p[3], const float o[3], float cell, uint32_t H, int* over) {
@@ -218,7 +267,8 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     if (!s) return;                                                                                \
     free(s->x); free(s->xs); free(s->v); free(s->w); free(s->rho); free(s->bvol);                  \
     free(s->key); free(s->perm); free(s->start); free(s->plam); free(s->pacc); free(s->vacc);      \
-    free(s->iacc); free(s->racc); free(s->rest); free(s->weights); free(s->slam);                  \
+    free(s->iacc); free(s->racc); free(s->rest); free(s->weights); free(s->slam); free(s->sdflam); \
+    for (uint32_t k = 0; k < s->nsdf; ++k) free(s->sdf[k].grid);                                   \
     for (uint32_t f = 0; f < s->nfluids; ++f) {                                                    \
       fluid_##SUF* F = &s->fluid[f];                                                               \
       free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
@@ -300,6 +350,44 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     F->bstart[F->nb] = start; F->bcount[F->nb] = count; F->bdyn[F->nb] = dyn ? 1u : 0u;            \
     F->nb++;                                                                                       \
     return 0;                                                                                      \
+  }                                                                                                \
+  /* SDFCollider (SDFCollider.ts:68-311): voxels as float (converted here), placement   \
+   * and friction as the f32 uniforms the reference uploads. rot: row-major 3x3. */     \
+  int orc_scene_add_sdf_##SUF(void* p, const float* data, const uint32_t res[3], const double origin[3], \
+                              const double voxel[3], const double pos[3], const double rot9[9],        \
+                              double scale, double thickness, double mus, double muk) {                \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (s->nsdf >= 4) return -1;                                                                   \
+    sdfc_##SUF* D = &s->sdf[s->nsdf++];                                                             \
+    size_t nv = (size_t)res[0] * res[1] * res[2];                                                  \
+    D->grid = (uint16_t*)malloc(nv * sizeof(uint16_t));                                            \
+    for (size_t k = 0; k < nv; ++k) D->grid[k] = orc_float_to_half(data[k]);                        \
+    D->nx = (int)res[0]; D->ny = (int)res[1]; D->nz = (int)res[2];                                 \
+    for (int a = 0; a < 3; ++a) {                                                                  \
+      D->res[a] = (T)(float)res[a];                                                                \
+      D->inv_res[a] = (T)(float)(1.0 / (double)res[a]);                                            \
+      D->origin[a] = (T)(float)origin[a];                                                          \
+      D->voxel[a] = (T)(float)voxel[a];                                                            \
+      D->inv_voxel[a] = (T)(float)(1.0 / voxel[a]);                                                \
+      D->pos[a] = (T)(float)pos[a];                                                                \
+    }                                                                                              \
+    for (int r = 0; r < 3; ++r)                                                                    \
+      for (int c = 0; c < 3; ++c) {                                                                \
+        D->rot[3 * r + c] = (T)(float)rot9[3 * r + c];                                             \
+        D->inv_rot[3 * r + c] = (T)(float)rot9[3 * c + r];                                         \
+      }                                                                                            \
+    D->scale = (T)(float)scale; D->inv_scale = (T)(float)(1.0 / scale);                            \
+    D->thickness = (T)(float)thickness; D->mus = (T)(float)mus; D->muk = (T)(float)muk;            \
+    if (!s->sdflam) s->sdflam = (T*)calloc(2 * 4 * (size_t)s->n, sizeof(T));                       \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* phi and gradient of SDF collider k at a world point (sdf.ts:107-146) */                       \
+  void orc_scene_sdf_sample_##SUF(void* p, uint32_t k, const double x[3], double* phi, double g[3]) { \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    T xx[3] = {(T)(float)x[0], (T)(float)x[1], (T)(float)x[2]}, ph, gg[3];                         \
+    sdf_sample_##SUF(&s->sdf[k], xx, &ph, gg);                                                     \
+    *phi = (double)ph;                                                                             \
+    for (int a = 0; a < 3; ++a) g[a] = (double)gg[a];                                              \
   }                                                                                                \
   /* boundary volumes, in particle order */                                                        \
   void orc_scene_get_bvol_##SUF(void* p, double* out) {                                            \

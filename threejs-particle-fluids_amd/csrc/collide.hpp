@@ -186,6 +186,131 @@ __device__ __forceinline__ void walls_friction(const SceneDev& S, uint32_t s, fl
   }
 }
 
+// ---------------------------------------------------------------------------
+// SDF colliders (SDFCollider.ts:68-311, sdf.ts:107-146)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pf_half_to_float(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+
+__device__ __forceinline__ float sdf_texel(const SdfDev& D, int i, int j, int k) {
+  return pf_half_to_float(D.grid[((size_t)k * (uint32_t)D.ny + (uint32_t)j) * (uint32_t)D.nx + (uint32_t)i]);
+}
+
+__device__ __forceinline__ float sdf_lerp(float a, float b, float f) { return a + (b - a) * f; }
+
+// Linear filtering at normalised uv with clamp-to-edge addressing: texel
+// coordinate t = uv * n - 0.5, weights from its fraction, lerp x, y, z.
+__device__ __forceinline__ float sdf_tex(const SdfDev& D, float u, float v, float w) {
+  const float t[3] = {u * D.res[0] - 0.5f, v * D.res[1] - 0.5f, w * D.res[2] - 0.5f};
+  const int n[3] = {D.nx, D.ny, D.nz};
+  int i0[3], i1[3];
+  float f[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float fl = floorf(t[a]);
+    f[a] = t[a] - fl;
+    const int ii = pf_clampi(pf_f2i_sat(fl), -1, n[a] - 1);
+    i0[a] = ii < 0 ? 0 : ii;
+    i1[a] = ii + 1 > n[a] - 1 ? n[a] - 1 : ii + 1;
+  }
+  const float c00 = sdf_lerp(sdf_texel(D, i0[0], i0[1], i0[2]), sdf_texel(D, i1[0], i0[1], i0[2]), f[0]);
+  const float c10 = sdf_lerp(sdf_texel(D, i0[0], i1[1], i0[2]), sdf_texel(D, i1[0], i1[1], i0[2]), f[0]);
+  const float c01 = sdf_lerp(sdf_texel(D, i0[0], i0[1], i1[2]), sdf_texel(D, i1[0], i0[1], i1[2]), f[0]);
+  const float c11 = sdf_lerp(sdf_texel(D, i0[0], i1[1], i1[2]), sdf_texel(D, i1[0], i1[1], i1[2]), f[0]);
+  return sdf_lerp(sdf_lerp(c00, c10, f[1]), sdf_lerp(c01, c11, f[1]), f[2]);
+}
+
+// emitSampleSdf: world -> local -> uv, trilinear phi, 6-tap central-difference
+// gradient, back to world.
+__device__ __forceinline__ void sdf_sample(const SdfDev& D, float3 x, float* phi, float3* grad) {
+  const float wo[3] = {x.x - D.pos[0], x.y - D.pos[1], x.z - D.pos[2]};
+  float uv[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float xl = ((D.inv_rot[3 * r] * wo[0] + D.inv_rot[3 * r + 1] * wo[1]) + D.inv_rot[3 * r + 2] * wo[2]) * D.inv_scale;
+    uv[r] = ((xl - D.origin[r]) * D.inv_voxel[r]) / D.res[r];
+  }
+  const float pl = sdf_tex(D, uv[0], uv[1], uv[2]);
+  float gl[3];
+  gl[0] = (sdf_tex(D, uv[0] + D.inv_res[0], uv[1], uv[2]) - sdf_tex(D, uv[0] - D.inv_res[0], uv[1], uv[2])) / (2.0f * D.voxel[0]);
+  gl[1] = (sdf_tex(D, uv[0], uv[1] + D.inv_res[1], uv[2]) - sdf_tex(D, uv[0], uv[1] - D.inv_res[1], uv[2])) / (2.0f * D.voxel[1]);
+  gl[2] = (sdf_tex(D, uv[0], uv[1], uv[2] + D.inv_res[2]) - sdf_tex(D, uv[0], uv[1], uv[2] - D.inv_res[2])) / (2.0f * D.voxel[2]);
+  *phi = pl * D.scale;
+  *grad = make_float3((D.rot[0] * gl[0] + D.rot[1] * gl[1]) + D.rot[2] * gl[2],
+                      (D.rot[3] * gl[0] + D.rot[4] * gl[1]) + D.rot[5] * gl[2],
+                      (D.rot[6] * gl[0] + D.rot[7] * gl[1]) + D.rot[8] * gl[2]);
+}
+
+// One XPBD contact iteration against every SDF collider of the scene
+// (SDFCollider.ts:237-269, collider.ts:53-89; static colliders). Ticks go to
+// the positions accumulator; (lambda_n, lambda_t) per slot and collider with
+// the same touch-mask reset as the primitives.
+__device__ __forceinline__ void sdf_solve(const SceneDev& S, uint32_t s, float3 xstar, float3 x, float w,
+                                          bool first_iteration, int* ticks) {
+  if (!(w > 0.0f) || S.nsdf == 0) return;
+  uint32_t touch = first_iteration ? 0u : S.sdf_touch[s];
+  const uint32_t touch_in = touch;
+  const float3 disp = pf_sub3(xstar, x);
+  for (uint32_t k = 0; k < S.nsdf; ++k) {
+    const SdfDev& D = S.sdf[k];
+    float phi;
+    float3 g;
+    sdf_sample(D, xstar, &phi, &g);
+    const float gl = pf_len3(g);
+    if (!(gl > 1e-6f)) continue;
+    const float3 n = make_float3(g.x / gl, g.y / gl, g.z / gl);
+    const float reach = S.radius + D.thickness;
+    if (!(phi < reach)) continue;
+    const float depth = reach - phi;
+    float* lam = S.sdf_lambda + ((size_t)s * PF_MAX_SDF + k) * 2u;
+    const bool had = (touch >> k) & 1u;
+    const float ln = (had ? lam[0] : 0.0f) + depth / w;
+    const float lt_old = had ? lam[1] : 0.0f;
+    const float3 tang = pf_sub3(disp, pf_mul3(n, pf_dot3(disp, n)));
+    const float tl = pf_len3(tang);
+    const bool sticks = tl > 1e-10f && (tl / w) <= (D.mus * ln - lt_old);
+    const float3 push = pf_mul3(n, depth);
+    const float3 corr = sticks ? pf_sub3(push, tang) : push;
+    ticks[0] = pf_wrap_add(ticks[0], pf_ticks(corr.x, S.pos_scale));
+    ticks[1] = pf_wrap_add(ticks[1], pf_ticks(corr.y, S.pos_scale));
+    ticks[2] = pf_wrap_add(ticks[2], pf_ticks(corr.z, S.pos_scale));
+    lam[0] = ln;
+    lam[1] = lt_old + (sticks ? tl / w : 0.0f);
+    touch |= 1u << k;
+  }
+  if (first_iteration || touch != touch_in) S.sdf_touch[s] = touch;
+}
+
+// Kinetic friction against every SDF collider the particle pressed on this
+// substep (SDFCollider.ts:271-291, collider.ts:96-118). Velocity ticks.
+__device__ __forceinline__ void sdf_friction(const SceneDev& S, uint32_t s, float3 xstar, float3 vel, float w,
+                                             int* ticks) {
+  if (!(w > 0.0f) || S.nsdf == 0) return;
+  const uint32_t touch = S.sdf_touch[s];
+  for (uint32_t k = 0; k < S.nsdf; ++k) {
+    if (!((touch >> k) & 1u)) continue;
+    const float ln = S.sdf_lambda[((size_t)s * PF_MAX_SDF + k) * 2u];
+    if (!(ln > 0.0f)) continue;
+    const SdfDev& D = S.sdf[k];
+    float phi;
+    float3 g;
+    sdf_sample(D, xstar, &phi, &g);
+    const float gl = pf_len3(g);
+    if (!(gl > 1e-6f)) continue;
+    const float3 n = make_float3(g.x / gl, g.y / gl, g.z / gl);
+    const float3 vt = pf_sub3(vel, pf_mul3(n, pf_dot3(n, vel)));
+    const float vl = pf_len3(vt);
+    if (vl > 1e-6f) {
+      const float change = fminf(((D.muk * ln) * w) / S.dt, vl);
+      const float3 d = make_float3(-((vt.x / vl) * change), -((vt.y / vl) * change), -((vt.z / vl) * change));
+      ticks[0] = pf_wrap_add(ticks[0], pf_ticks(d.x, S.vel_scale));
+      ticks[1] = pf_wrap_add(ticks[1], pf_ticks(d.y, S.vel_scale));
+      ticks[2] = pf_wrap_add(ticks[2], pf_ticks(d.z, S.vel_scale));
+    }
+  }
+}
+
 // Accumulator apply (accumulator.ts:101-131): ticks -> f32, saturation test,
 // scaled change. Returns the change; sets *saturated.
 __device__ __forceinline__ float3 ticks_to_change(const int* ticks, float inv, bool* saturated) {

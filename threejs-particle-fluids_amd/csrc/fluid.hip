@@ -874,6 +874,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     if (WALLS) {
       int tk[3] = {0, 0, 0};
       walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
+      sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
       bool sat;
       const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
       if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
@@ -975,6 +976,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_walls(const SceneDev* __restrict__
   const float4 x = S.x[p][s];
   int tk[3] = {0, 0, 0};
   walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
+  sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
   bool sat;
   const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
   if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
@@ -1219,6 +1221,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
   if (FRICTION && touch_valid) {
     int tk[3] = {0, 0, 0};
     walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
+    sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
     bool sat;
     const float3 ch = ticks_to_change(tk, S.vel_inv, &sat);
     if (sat) flag_set(S, 0, PF_FLAG_VELOCITIES);
@@ -1240,6 +1243,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restric
   float4 v = S.v[p][s];
   int tk[3] = {0, 0, 0};
   walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
+  sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
   bool sat;
   const float3 ch = ticks_to_change(tk, S.vel_inv, &sat);
   if (sat) flag_set(S, 0, PF_FLAG_VELOCITIES);

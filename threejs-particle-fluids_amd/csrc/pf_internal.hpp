@@ -101,6 +101,22 @@ struct SoftDev {
   float4* rotations;    // [3 * nbodies] rows of R
 };
 
+// A signed distance field collider (SDFCollider.ts:68-311): half-float
+// voxels and the placement, all as the f32 uniforms the reference uploads.
+struct SdfDev {
+  const uint16_t* grid;           // [nx * ny * nz], x fastest
+  int nx, ny, nz;
+  float res[3], inv_res[3];       // resolution as float and 1 / resolution
+  float origin[3];                // corner of the first voxel (local)
+  float voxel[3], inv_voxel[3];
+  float pos[3];                   // world position of the local origin
+  float rot[9], inv_rot[9];       // row-major rotation and its transpose
+  float scale, inv_scale;
+  float thickness;
+  float mus, muk;
+};
+#define PF_MAX_SDF 4
+
 struct SceneDev {
   uint32_t n;          // particle capacity
   uint32_t H;          // hash table size (power of two)
@@ -146,10 +162,15 @@ struct SceneDev {
   float* prim_lambda;          // [2 * prim_cap * n] (lambda_n, lambda_t) per slot/primitive
   uint32_t* prim_touch;        // [n] bit p set when primitive p touched the slot this substep
   float pos_scale, pos_inv, vel_scale, vel_inv; // Accumulator(10), Accumulator(50)
+  uint32_t nsdf;
+  SdfDev sdf[PF_MAX_SDF];
+  float* sdf_lambda;           // [2 * PF_MAX_SDF * n] (lambda_n, lambda_t) per slot/collider
+  uint32_t* sdf_touch;         // [n] bit k set when collider k touched the slot this substep
   float clock[PF_MAX_SUBSTEPS];                 // sweep clock per substep (PrimitiveSet.ts:393-398)
   FluidDev fluid[PF_MAX_FLUIDS];
   SoftDev soft;
 };
+
 
 // One workgroup's share of a per-particle kernel: scene index and first slot.
 struct Tile {

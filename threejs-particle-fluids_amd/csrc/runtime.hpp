@@ -177,6 +177,19 @@ struct pf_primitives_s {
   void upload();
 };
 
+struct pf_sdf_collider_s {
+  pf_particles_s* ps = nullptr;
+  pf_loop_s* loop = nullptr;
+  int slot = -1;                 // index in the scene's SdfDev table
+  uint32_t res[3] = {0, 0, 0};
+  double origin[3] = {0, 0, 0}, voxel[3] = {1, 1, 1};
+  double pos[3] = {0, 0, 0}, quat[4] = {0, 0, 0, 1}, scale = 1, thickness = 0;
+  double mus = 0.5, muk = 0.4;
+  uint16_t* grid = nullptr;      // device half floats
+  bool disposed = false;
+  void write(SdfDev& D) const;   // the f32 uniforms
+};
+
 struct pf_softbody_s {
   pf_particles_s* ps = nullptr;
   struct Body { uint32_t start, count, surface_count; double compliance; };
@@ -215,6 +228,7 @@ struct pf_loop_s {
   std::vector<pf_fluid_s*> fluids;
   pf_softbody_s* soft = nullptr;
   pf_primitives_s* prims = nullptr;
+  std::vector<pf_sdf_collider_s*> sdfs;
   pf_grid_s grid;  // the loop's grid view (owned_by_loop)
   Tile* tiles = nullptr;
   uint32_t ntiles = 0;

@@ -12,6 +12,37 @@ void particles_scatter(pf_particles* ps, uint32_t start, uint32_t count, const s
                        const std::vector<uint32_t>* grp, uint32_t mask);
 void particles_gather(pf_particles* ps, int what, void* host, size_t elem);
 
+// three's DataUtils.toHalfFloat (three@0.184.0): clamp to +-65504, then the
+// base/shift tables of van der Zijp's fast half-float conversion, which
+// truncate the mantissa.
+static uint16_t half_from_float(float v) {
+  static uint16_t base[512];
+  static uint8_t shift[512];
+  static bool ready = false;
+  if (!ready) {
+    for (int i = 0; i < 256; ++i) {
+      const int e = i - 127;
+      uint16_t b;
+      uint8_t sh;
+      if (e < -27) { b = 0x0000; sh = 24; }
+      else if (e < -14) { b = (uint16_t)(0x0400 >> (-e - 14)); sh = (uint8_t)(-e - 1); }
+      else if (e <= 15) { b = (uint16_t)((e + 15) << 10); sh = 13; }
+      else if (e < 128) { b = 0x7c00; sh = 24; }
+      else { b = 0x7c00; sh = 13; }
+      base[i] = b;
+      base[i | 0x100] = (uint16_t)(b | 0x8000);
+      shift[i] = shift[i | 0x100] = sh;
+    }
+    ready = true;
+  }
+  if (v > 65504.0f) v = 65504.0f;
+  if (v < -65504.0f) v = -65504.0f;
+  uint32_t f;
+  std::memcpy(&f, &v, 4);
+  const uint32_t e = (f >> 23) & 0x1ffu;
+  return (uint16_t)(base[e] + ((f & 0x007fffffu) >> shift[e]));
+}
+
 // Predict without a grid (integrate.ts:25-34), for scenes with no neighbour
 // queries: the state keeps its order and the prediction goes to xs[c ^ 1].
 __global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __restrict__ tiles,
@@ -564,7 +595,162 @@ int pf_primitives_destroy(pf_primitives* s) {
   PF_API_END
 }
 
+// ---------------------------------------------------------------------------
+// SDFCollider (SDFCollider.ts:68-311)
+// ---------------------------------------------------------------------------
+static void sdf_friction_ok(double mus, double muk) {
+  if (!(mus >= 0) || !(muk >= 0))
+    fail(PF_EINVAL, "SDFCollider: friction coefficients must be non-negative (got muS=%g, muK=%g)", mus, muk);
+}
+
+static void sdf_alive(pf_sdf_collider* c) {
+  if (!c) fail(PF_EINVAL, "SDFCollider: null handle");
+  if (c->disposed) fail(PF_ESTATE, "SDFCollider: disposed");
+}
+
+// Placement changes reach the scene descriptor of the loop that holds it.
+static void sdf_changed(pf_sdf_collider* c) {
+  if (c->loop && c->slot >= 0) {
+    c->write(c->ps->host.sdf[c->slot]);
+    c->ps->dirty = true;
+  }
+}
+
+static void sdf_set_quat(pf_sdf_collider* c, const double q[4], const char* owner) {
+  const double l = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (!(l > 0) || !std::isfinite(l)) fail(PF_EINVAL, "%s: rotation must be a finite, non-zero quaternion", owner);
+  const double inv = 1.0 / l;  // Quaternion.normalize()
+  for (int a = 0; a < 4; ++a) c->quat[a] = q[a] * inv;
+}
+
+int pf_sdf_collider_create(pf_particles* ps, const float* data, const uint32_t res[3], const double origin[3],
+                           const double voxel[3], const pf_sdf_options* opt, pf_sdf_collider** out) {
+  PF_API_BEGIN
+  alive(ps, "SDFCollider");
+  if (!(data && res && origin && voxel && out)) fail(PF_EINVAL, "SDFCollider: null argument");
+  for (int a = 0; a < 3; ++a)
+    if (res[a] < 4) fail(PF_EINVAL, "SDFCollider: resolution must be integers ≥ 4, got [%u, %u, %u]", res[0], res[1], res[2]);
+  for (int a = 0; a < 3; ++a)
+    if (!(voxel[a] > 0) || !std::isfinite(voxel[a])) fail(PF_EINVAL, "SDFCollider: voxelSize must be positive");
+  auto c = std::make_unique<pf_sdf_collider_s>();
+  c->ps = ps;
+  for (int a = 0; a < 3; ++a) {
+    c->res[a] = res[a];
+    c->origin[a] = origin[a];
+    c->voxel[a] = voxel[a];
+  }
+  if (opt) {
+    const double mus = opt->mu_s < 0 ? 0.5 : opt->mu_s, muk = opt->mu_k < 0 ? 0.4 : opt->mu_k;
+    sdf_friction_ok(mus, muk);
+    c->mus = mus;
+    c->muk = muk;
+    if (!(opt->thickness >= 0) || !std::isfinite(opt->thickness))
+      fail(PF_EINVAL, "SDFCollider: thickness must be non-negative, got %g", opt->thickness);
+    c->thickness = opt->thickness;
+    for (int a = 0; a < 3; ++a) c->pos[a] = opt->position[a];
+    if (opt->rotation) sdf_set_quat(c.get(), opt->rotation, "SDFCollider.setRotation");
+    if (opt->scale > 0) {
+      if (!std::isfinite(opt->scale)) fail(PF_EINVAL, "SDFCollider.setScale: scale must be positive, got %g", opt->scale);
+      c->scale = opt->scale;
+    }
+  }
+  const size_t nv = (size_t)res[0] * res[1] * res[2];
+  std::vector<uint16_t> half(nv);
+  for (size_t k = 0; k < nv; ++k) half[k] = half_from_float(data[k]);
+  c->grid = dalloc<uint16_t>(nv);
+  PF_HIP(hipMemcpy(c->grid, half.data(), nv * sizeof(uint16_t), hipMemcpyHostToDevice));
+  *out = c.release();
+  PF_API_END
+}
+
+int pf_sdf_collider_set_position(pf_sdf_collider* c, const double p[3]) {
+  PF_API_BEGIN
+  sdf_alive(c);
+  for (int a = 0; a < 3; ++a) c->pos[a] = p[a];
+  sdf_changed(c);
+  PF_API_END
+}
+
+int pf_sdf_collider_set_rotation(pf_sdf_collider* c, const double q[4]) {
+  PF_API_BEGIN
+  sdf_alive(c);
+  sdf_set_quat(c, q, "SDFCollider.setRotation");
+  sdf_changed(c);
+  PF_API_END
+}
+
+int pf_sdf_collider_set_scale(pf_sdf_collider* c, double scale) {
+  PF_API_BEGIN
+  sdf_alive(c);
+  if (!(scale > 0) || !std::isfinite(scale)) fail(PF_EINVAL, "SDFCollider.setScale: scale must be positive, got %g", scale);
+  c->scale = scale;
+  sdf_changed(c);
+  PF_API_END
+}
+
+int pf_sdf_collider_set_friction(pf_sdf_collider* c, double mus, double muk) {
+  PF_API_BEGIN
+  sdf_alive(c);
+  sdf_friction_ok(mus, muk);
+  c->mus = mus;
+  c->muk = muk;
+  sdf_changed(c);
+  PF_API_END
+}
+
+int pf_sdf_collider_set_thickness(pf_sdf_collider* c, double t) {
+  PF_API_BEGIN
+  sdf_alive(c);
+  if (!(t >= 0) || !std::isfinite(t)) fail(PF_EINVAL, "SDFCollider: thickness must be non-negative, got %g", t);
+  c->thickness = t;
+  sdf_changed(c);
+  PF_API_END
+}
+
+int pf_sdf_collider_destroy(pf_sdf_collider* c) {
+  PF_API_BEGIN
+  if (!c || c->disposed) return PF_OK;
+  if (c->loop) fail(PF_ESTATE, "SDFCollider: dispose the SimLoop that uses it first");
+  c->disposed = true;
+  dfree(c->grid);
+  c->grid = nullptr;
+  PF_API_END
+}
+
 }  // extern "C"
+
+// The uniforms SDFCollider keeps (SDFCollider.ts:115-139): rotation from the
+// quaternion as Matrix4.makeRotationFromQuaternion, its transpose, 1/scale,
+// 1/voxel and 1/resolution, all formed in double and stored as f32.
+void pf_sdf_collider_s::write(SdfDev& D) const {
+  D.grid = grid;
+  D.nx = (int)res[0];
+  D.ny = (int)res[1];
+  D.nz = (int)res[2];
+  for (int a = 0; a < 3; ++a) {
+    D.res[a] = (float)res[a];
+    D.inv_res[a] = (float)(1.0 / (double)res[a]);
+    D.origin[a] = (float)origin[a];
+    D.voxel[a] = (float)voxel[a];
+    D.inv_voxel[a] = (float)(1.0 / voxel[a]);
+    D.pos[a] = (float)pos[a];
+  }
+  const double x = quat[0], y = quat[1], z = quat[2], w = quat[3];
+  const double x2 = x + x, y2 = y + y, z2 = z + z;
+  const double xx = x * x2, xy = x * y2, xz = x * z2, yy = y * y2, yz = y * z2, zz = z * z2;
+  const double wx = w * x2, wy = w * y2, wz = w * z2;
+  const double R[9] = {1 - (yy + zz), xy - wz, xz + wy, xy + wz, 1 - (xx + zz), yz - wx, xz - wy, yz + wx, 1 - (xx + yy)};
+  for (int r = 0; r < 3; ++r)
+    for (int cc = 0; cc < 3; ++cc) {
+      D.rot[3 * r + cc] = (float)R[3 * r + cc];
+      D.inv_rot[3 * r + cc] = (float)R[3 * cc + r];
+    }
+  D.scale = (float)scale;
+  D.inv_scale = (float)(1.0 / scale);
+  D.thickness = (float)thickness;
+  D.mus = (float)mus;
+  D.muk = (float)muk;
+}
 
 void pf_primitives_s::upload() {
   if (!dirty || !packed) return;
@@ -834,7 +1020,7 @@ static PlanShape loop_shape(const pf_loop* L) {
   sh.iterations = L->iterations;
   sh.nfluids = (uint32_t)L->fluids.size();
   for (uint32_t f = 0; f < sh.nfluids; ++f) sh.fx[f] = L->ps->host.fluid[f].effects;
-  sh.walls = L->prims && !L->prims->prims.empty();
+  sh.walls = (L->prims && !L->prims->prims.empty()) || !L->sdfs.empty();
   sh.soft = L->soft != nullptr;
   sh.any_boundary = L->ps->host.has_boundary != 0;
   for (uint32_t f = 0; f < sh.nfluids; ++f) {
@@ -863,7 +1049,7 @@ static void loop_set_dt(pf_loop* L, double dt) {
 }
 
 int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_material* mats, uint32_t nmats,
-                   pf_primitives* const* colliders, uint32_t ncol, pf_loop** out) {
+                   const pf_collider* colliders, uint32_t ncol, pf_loop** out) {
   PF_API_BEGIN
   alive(ps, "SimLoop");
   if (ps->loop) fail(PF_EINVAL, "SimLoop: the ParticleSystem already has a SimLoop");
@@ -897,13 +1083,24 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
       fail(PF_EINVAL, "SimLoop: unknown material kind %u", mats[m].kind);
     }
   }
-  if (ncol > 1) fail(PF_EINVAL, "SimLoop: one PrimitiveSet per loop in this build");
   for (uint32_t k = 0; k < ncol; ++k) {
-    pf_primitives* s = colliders[k];
-    if (!s || s->disposed) fail(PF_EINVAL, "SimLoop: a collider has been disposed");
-    if (s->ps != ps) fail(PF_EINVAL, "SimLoop: every collider must be built for the same ParticleSystem");
-    if (s->loop) fail(PF_EINVAL, "SimLoop: the collider belongs to another SimLoop");
-    L->prims = s;
+    if (colliders[k].kind == PF_COLLIDER_PRIMITIVES) {
+      auto* s = static_cast<pf_primitives*>(colliders[k].handle);
+      if (!s || s->disposed) fail(PF_EINVAL, "SimLoop: a collider has been disposed");
+      if (s->ps != ps) fail(PF_EINVAL, "SimLoop: every collider must be built for the same ParticleSystem");
+      if (s->loop) fail(PF_EINVAL, "SimLoop: the collider belongs to another SimLoop");
+      if (L->prims) fail(PF_EINVAL, "SimLoop: one PrimitiveSet per loop in this build");
+      L->prims = s;
+    } else if (colliders[k].kind == PF_COLLIDER_SDF) {
+      auto* c = static_cast<pf_sdf_collider*>(colliders[k].handle);
+      if (!c || c->disposed) fail(PF_EINVAL, "SimLoop: a collider has been disposed");
+      if (c->ps != ps) fail(PF_EINVAL, "SimLoop: every collider must be built for the same ParticleSystem");
+      if (c->loop) fail(PF_EINVAL, "SimLoop: the collider belongs to another SimLoop");
+      if (L->sdfs.size() >= PF_MAX_SDF) fail(PF_EINVAL, "SimLoop: at most %d SDFColliders per loop", PF_MAX_SDF);
+      L->sdfs.push_back(c);
+    } else {
+      fail(PF_EINVAL, "SimLoop: unknown collider kind %u", colliders[k].kind);
+    }
   }
   SceneDev& S = ps->host;
   const uint32_t n = ps->n;
@@ -981,6 +1178,17 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     s->dirty = true;
     s->upload();
   }
+  S.nsdf = (uint32_t)L->sdfs.size();
+  for (uint32_t k = 0; k < S.nsdf; ++k) {
+    pf_sdf_collider* c = L->sdfs[k];
+    c->loop = L.get();
+    c->slot = (int)k;
+    c->write(S.sdf[k]);
+  }
+  if (S.nsdf) {
+    S.sdf_lambda = dalloc<float>(2 * (size_t)PF_MAX_SDF * n);
+    S.sdf_touch = dalloc<uint32_t>(n);
+  }
   if (L->soft) {
     pf_softbody* sb = L->soft;
     sb->loop = L.get();
@@ -1023,6 +1231,17 @@ static void loop_alive(pf_loop* L, const char* what) {
   if (L->ps->disposed) fail(PF_ESTATE, "SimLoop: the ParticleSystem has been disposed");
 }
 
+// A loop steps only while every material and collider it was built with is
+// alive: their device buffers go with them.
+static void loop_members_alive(pf_loop* L) {
+  for (auto* f : L->fluids)
+    if (f->disposed) fail(PF_ESTATE, "SimLoop: a material has been disposed");
+  if (L->soft && L->soft->disposed) fail(PF_ESTATE, "SimLoop: a material has been disposed");
+  if (L->prims && L->prims->disposed) fail(PF_ESTATE, "SimLoop: a collider has been disposed");
+  for (auto* c : L->sdfs)
+    if (c->disposed) fail(PF_ESTATE, "SimLoop: a collider has been disposed");
+}
+
 static void loop_invalidate_graphs(pf_loop* L) {
   for (int v = 0; v < 4; ++v) {
     if (L->graph[v]) (void)hipGraphExecDestroy(L->graph[v]);
@@ -1061,6 +1280,7 @@ static void loop_static_boundaries(pf_loop* L) {
 int pf_loop_step(pf_loop* L, double dt) {
   PF_API_BEGIN
   loop_alive(L, "SimLoop.step");
+  loop_members_alive(L);
   if (!(dt > 0) || !std::isfinite(dt)) fail(PF_EINVAL, "SimLoop: step dt must be a positive finite number, got %g", dt);
   pf_particles* ps = L->ps;
   Context* ctx = L->ctx;
@@ -1206,6 +1426,13 @@ int pf_loop_destroy(pf_loop* L) {
     f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr;
   }
   if (L->prims) L->prims->loop = nullptr;
+  for (auto* c : L->sdfs) {
+    c->loop = nullptr;
+    c->slot = -1;
+  }
+  dfree(S.sdf_lambda); dfree(S.sdf_touch);
+  S.sdf_lambda = nullptr; S.sdf_touch = nullptr; S.nsdf = 0;
+  for (auto* f : L->fluids) { dfree(f->reaction); f->reaction = nullptr; }
   if (L->soft) L->soft->loop = nullptr;
   L->ps->loop = nullptr;
   L->ps->dirty = true;
@@ -1279,6 +1506,7 @@ int pf_batch_step(pf_batch* B, double dt) {
   bool changed = false;
   for (auto* L : B->loops) {
     loop_alive(L, "Batch");
+    loop_members_alive(L);
     if (L->prims) L->prims->upload();
     loop_set_dt(L, dt);
     if (L->static_boundaries && L->steps == 0) loop_static_boundaries(L);

@@ -443,6 +443,76 @@ napi_value PrimitivesDestroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------- sdf collider
+// sdfCreate(particles, data: Float32Array, resolution[3], origin[3], voxelSize[3],
+//           {muS, muK, position, rotation, scale, thickness})
+napi_value SdfCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  size_t nd = 0;
+  const float* data = typed<float>(env, a.argv[1], &nd);
+  auto res = doubles(env, a.argv[2]), origin = doubles(env, a.argv[3]), voxel = doubles(env, a.argv[4]);
+  if (res.size() != 3 || origin.size() != 3 || voxel.size() != 3) {
+    napi_throw_error(env, nullptr, "SDFCollider: resolution, origin and voxelSize take three numbers");
+    return nullptr;
+  }
+  const uint32_t r3[3] = {(uint32_t)res[0], (uint32_t)res[1], (uint32_t)res[2]};
+  if ((size_t)r3[0] * r3[1] * r3[2] != nd) {
+    std::string msg = "SDFCollider: " + std::to_string(nd) + " values do not match resolution " +
+                      std::to_string(r3[0]) + "×" + std::to_string(r3[1]) + "×" + std::to_string(r3[2]);
+    napi_throw_error(env, nullptr, msg.c_str());
+    return nullptr;
+  }
+  pf_sdf_options o{};
+  napi_value opts = a.argv[5];
+  o.mu_s = -1;
+  o.mu_k = -1;
+  std::vector<double> q;
+  if (!is_nullish(env, opts)) {
+    napi_value v = prop(env, opts, "muS");
+    if (!is_nullish(env, v)) o.mu_s = num(env, v);
+    v = prop(env, opts, "muK");
+    if (!is_nullish(env, v)) o.mu_k = num(env, v);
+    auto pos = doubles(env, prop(env, opts, "position"));
+    if (pos.size() == 3) for (int k = 0; k < 3; ++k) o.position[k] = pos[k];
+    q = doubles(env, prop(env, opts, "rotation"));
+    if (q.size() == 4) o.rotation = q.data();
+    o.scale = num(env, prop(env, opts, "scale"), 0);
+    o.thickness = num(env, prop(env, opts, "thickness"), 0);
+  }
+  pf_sdf_collider* c = nullptr;
+  PF_CALL(pf_sdf_collider_create((pf_particles*)handle(env, a.argv[0]), data, r3, origin.data(), voxel.data(), &o, &c));
+  return make_handle(env, c);
+}
+napi_value SdfSet(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  auto* c = (pf_sdf_collider*)handle(env, a.argv[0]);
+  char buf[16] = {0};
+  size_t len = 0;
+  napi_get_value_string_utf8(env, a.argv[1], buf, sizeof buf, &len);
+  const std::string what(buf);
+  if (what == "position") {
+    auto p = doubles(env, a.argv[2]);
+    if (p.size() != 3) { napi_throw_error(env, nullptr, "SDFCollider.setPosition: three numbers"); return nullptr; }
+    PF_CALL(pf_sdf_collider_set_position(c, p.data()));
+  } else if (what == "rotation") {
+    auto q = doubles(env, a.argv[2]);
+    if (q.size() != 4) { napi_throw_error(env, nullptr, "SDFCollider.setRotation: four numbers"); return nullptr; }
+    PF_CALL(pf_sdf_collider_set_rotation(c, q.data()));
+  } else if (what == "scale") {
+    PF_CALL(pf_sdf_collider_set_scale(c, num(env, a.argv[2])));
+  } else if (what == "friction") {
+    PF_CALL(pf_sdf_collider_set_friction(c, num(env, a.argv[2]), num(env, a.argv[3])));
+  } else if (what == "thickness") {
+    PF_CALL(pf_sdf_collider_set_thickness(c, num(env, a.argv[2])));
+  }
+  return nullptr;
+}
+napi_value SdfDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_sdf_collider_destroy((pf_sdf_collider*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
 // ---------------------------------------------------------------- softbody
 napi_value SoftbodyCreate(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
@@ -515,11 +585,12 @@ napi_value LoopCreate(napi_env env, napi_callback_info info) {
     mats[i].kind = u32(env, prop(env, m, "kind"));
     mats[i].handle = handle(env, prop(env, m, "handle"));
   }
-  std::vector<pf_primitives*> cols(nc);
+  std::vector<pf_collider> cols(nc);
   for (uint32_t i = 0; i < nc; ++i) {
     napi_value c;
     napi_get_element(env, a.argv[3], i, &c);
-    cols[i] = (pf_primitives*)handle(env, c);
+    cols[i].kind = u32(env, prop(env, c, "kind"));
+    cols[i].handle = handle(env, prop(env, c, "handle"));
   }
   pf_loop* L = nullptr;
   PF_CALL(pf_loop_create((pf_particles*)handle(env, a.argv[0]), &o, mats.data(), nm, cols.data(), nc, &L));
@@ -684,6 +755,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"primitivesCreate", PrimitivesCreate}, {"primitivesAdd", PrimitivesAdd},
       {"primitivesSetSphere", PrimitivesSetSphere}, {"primitivesSetPose", PrimitivesSetPose},
       {"primitivesCount", PrimitivesCount}, {"primitivesDestroy", PrimitivesDestroy},
+      {"sdfCreate", SdfCreate}, {"sdfSet", SdfSet}, {"sdfDestroy", SdfDestroy},
       {"softbodyCreate", SoftbodyCreate}, {"softbodySetCompliance", SoftbodySetCompliance},
       {"softbodyReadbackFrames", SoftbodyReadbackFrames}, {"softbodyDestroy", SoftbodyDestroy},
       {"loopCreate", LoopCreate}, {"loopStep", LoopStep}, {"loopStepAsync", LoopStepAsync},

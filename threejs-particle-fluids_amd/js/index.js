@@ -197,6 +197,9 @@ class PrimitiveSet {
   get count() {
     return native.primitivesCount(this.handle);
   }
+  get collider() {
+    return { kind: 1, handle: this.handle };
+  }
   dispose() {
     if (this.handle) native.primitivesDestroy(this.handle);
     this.handle = null;
@@ -236,6 +239,37 @@ class SoftbodySystem {
   }
 }
 
+/**
+ * SDFCollider (SDFCollider.ts:68-311): sdf = {data: Float32Array (x fastest),
+ * resolution: [nx, ny, nz], origin: [x, y, z], voxelSize: [x, y, z]}.
+ */
+class SDFCollider {
+  constructor(particles, sdf, options = {}) {
+    this.particles = particles;
+    this.handle = native.sdfCreate(particles.handle, Float32Array.from(sdf.data), Array.from(sdf.resolution),
+      Array.from(sdf.origin), Array.from(sdf.voxelSize), {
+        muS: options.muS,
+        muK: options.muK,
+        position: options.position ? vec(options.position) : undefined,
+        rotation: options.rotation ? quat(options.rotation) : undefined,
+        scale: options.scale,
+        thickness: options.thickness,
+      });
+  }
+  get collider() {
+    return { kind: 2, handle: this.handle };
+  }
+  setPosition(p) { native.sdfSet(this.handle, 'position', vec(p)); }
+  setRotation(q) { native.sdfSet(this.handle, 'rotation', quat(q)); }
+  setScale(s) { native.sdfSet(this.handle, 'scale', s); }
+  setFriction(muS, muK) { native.sdfSet(this.handle, 'friction', muS, muK); }
+  set thickness(t) { native.sdfSet(this.handle, 'thickness', t); }
+  dispose() {
+    if (this.handle) native.sdfDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
 class SimLoop {
   constructor(particles, options = {}) {
     if (options.contact) throw new Error('SimLoop: particle contacts are not available in this build');
@@ -249,7 +283,7 @@ class SimLoop {
         hashOrigin: options.hashOrigin ? vec(options.hashOrigin) : undefined,
       },
       (def(options.materials, [])).map((m) => m.material),
-      (def(options.colliders, [])).map((c) => c.handle),
+      (def(options.colliders, [])).map((c) => c.collider),
     );
     this.stepping = null;
   }
@@ -447,6 +481,7 @@ module.exports = {
   HashGrid,
   FluidSystem,
   PrimitiveSet,
+  SDFCollider,
   SoftbodySystem,
   SimLoop,
   Simulation,

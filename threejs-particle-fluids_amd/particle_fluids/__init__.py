@@ -17,7 +17,7 @@ from ._lib import PFError, call, load
 
 __all__ = [
     "PFError", "Context", "ParticleSystem", "HashGrid", "FluidSystem", "PrimitiveSet",
-    "SoftbodySystem", "SimLoop", "Batch", "Simulation", "fill_box", "fit_budget",
+    "SoftbodySystem", "SDFCollider", "SimLoop", "Batch", "Simulation", "fill_box", "fit_budget",
     "radius_for_budget",
 ]
 
@@ -348,6 +348,9 @@ class PrimitiveSet:
         call("pf_primitives_create", particles.handle, int(capacity or 0), C.byref(h))
         self.handle = h
 
+    def collider(self):
+        return _lib.Collider(1, self.handle)
+
     def addPlane(self, normal, point, muS=None, muK=None):
         slot = C.c_uint32()
         call("pf_primitives_add_plane", self.handle, _d3(normal), _d3(point), -1.0 if muS is None else float(muS), -1.0 if muK is None else float(muK), C.byref(slot))
@@ -383,6 +386,57 @@ class PrimitiveSet:
     def dispose(self):
         if self.handle:
             call("pf_primitives_destroy", self.handle)
+            self.handle = None
+
+
+class SDFCollider:
+    """src/core/collision/SDFCollider.ts:68-311.
+
+    ``sdf`` is a dict with ``data`` (distances, x fastest, then y, then z),
+    ``resolution``, ``origin`` (corner of the first voxel) and ``voxelSize``,
+    as SDFData (SDFCollider.ts:32-39)."""
+
+    def __init__(self, particles, sdf, muS=None, muK=None, position=None, rotation=None, scale=None, thickness=None):
+        self.particles = particles
+        data = _f32(np.asarray(sdf["data"]).reshape(-1))
+        res = np.ascontiguousarray(sdf["resolution"], np.uint32)
+        if res.shape != (3,) or data.size != int(res.prod()):
+            raise PFError(1, f"SDFCollider: {data.size} values do not match resolution {'×'.join(str(int(r)) for r in res)}")
+        o = _lib.SdfOptions()
+        o.mu_s = float(muS) if muS is not None else -1.0
+        o.mu_k = float(muK) if muK is not None else -1.0
+        for a, v in enumerate(position if position is not None else (0.0, 0.0, 0.0)):
+            o.position[a] = float(v)
+        self._rot = _d4(rotation) if rotation is not None else None
+        o.rotation = C.cast(self._rot, f64p) if self._rot is not None else None
+        o.scale = float(scale) if scale is not None else 0.0
+        o.thickness = float(thickness) if thickness is not None else 0.0
+        h = C.c_void_p()
+        call("pf_sdf_collider_create", particles.handle, _ptr(data, f32p), _ptr(res, u32p), _d3(sdf["origin"]),
+             _d3(sdf["voxelSize"]), C.byref(o), C.byref(h))
+        self.handle = h
+
+    def collider(self):
+        return _lib.Collider(2, self.handle)
+
+    def setPosition(self, p):
+        call("pf_sdf_collider_set_position", self.handle, _d3(p))
+
+    def setRotation(self, q):
+        call("pf_sdf_collider_set_rotation", self.handle, _d4(q))
+
+    def setScale(self, s):
+        call("pf_sdf_collider_set_scale", self.handle, float(s))
+
+    def setFriction(self, muS, muK):
+        call("pf_sdf_collider_set_friction", self.handle, float(muS), float(muK))
+
+    def setThickness(self, t):
+        call("pf_sdf_collider_set_thickness", self.handle, float(t))
+
+    def dispose(self):
+        if self.handle:
+            call("pf_sdf_collider_destroy", self.handle)
             self.handle = None
 
 
@@ -454,7 +508,7 @@ class SimLoop:
             for a in range(3):
                 o.hash_origin[a] = float(hashOrigin[a])
         mats = (_lib.Material * max(1, len(materials)))(*[m.material() for m in materials])
-        cols = (C.c_void_p * max(1, len(colliders)))(*[c.handle for c in colliders])
+        cols = (_lib.Collider * max(1, len(colliders)))(*[c.collider() for c in colliders])
         h = C.c_void_p()
         call("pf_loop_create", particles.handle, C.byref(o), mats, len(materials), cols, len(colliders), C.byref(h))
         self.handle = h

@@ -64,6 +64,15 @@ class SoftbodyDef(C.Structure):
     ]
 
 
+class Collider(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("handle", C.c_void_p)]
+
+
+class SdfOptions(C.Structure):
+    _fields_ = [("mu_s", C.c_double), ("mu_k", C.c_double), ("position", C.c_double * 3),
+                ("rotation", C.POINTER(C.c_double)), ("scale", C.c_double), ("thickness", C.c_double)]
+
+
 class Material(C.Structure):
     _fields_ = [("kind", u32), ("handle", vp)]
 
@@ -146,7 +155,14 @@ SIGNATURES = {
     "pf_softbody_set_compliance": [vp, u32, f64],
     "pf_softbody_readback_frames": [vp, P(f32), P(f32)],
     "pf_softbody_destroy": [vp],
-    "pf_loop_create": [vp, P(LoopOptions), P(Material), u32, P(vp), u32, P(vp)],
+    "pf_loop_create": [vp, P(LoopOptions), P(Material), u32, P(Collider), u32, P(vp)],
+    "pf_sdf_collider_create": [vp, P(f32), P(u32), P(f64), P(f64), P(SdfOptions), P(vp)],
+    "pf_sdf_collider_set_position": [vp, P(f64)],
+    "pf_sdf_collider_set_rotation": [vp, P(f64)],
+    "pf_sdf_collider_set_scale": [vp, f64],
+    "pf_sdf_collider_set_friction": [vp, f64, f64],
+    "pf_sdf_collider_set_thickness": [vp, f64],
+    "pf_sdf_collider_destroy": [vp],
     "pf_loop_step": [vp, f64],
     "pf_loop_synchronize": [vp],
     "pf_loop_readback_overflow": [vp, P(Overflow)],
