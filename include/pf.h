@@ -188,15 +188,21 @@ int pf_primitives_set_pose(pf_primitives* set, uint32_t slot, const double posit
 int pf_primitives_count(pf_primitives* set, uint32_t* count);
 int pf_primitives_destroy(pf_primitives* set);
 
-/* ---- SoftbodySystem, global shape matching (src/softbody/SoftbodySystem.ts) */
+/* ---- SoftbodySystem (src/softbody/SoftbodySystem.ts:105-359) -------------
+ * SoftbodyDef (SoftbodySystem.ts:28-63) and the shapeMatching option:
+ * global (shapeMatch.ts) or local (shapeMatchImplicit.ts, rotation.ts). */
 typedef struct pf_softbody_def {
   uint32_t start, count;   /* particle range of the body */
   uint32_t surface_count;  /* 0xffffffff = all */
   double compliance;       /* s^2/kg, >= 0 */
   const float* rest_positions; /* xyz triples, nullable = uploaded positions */
+  const uint32_t* edges;   /* pairs [i0, j0, i1, j1, ...] local to the body, nullable */
+  uint32_t edge_count;     /* number of pairs */
 } pf_softbody_def;
+#define PF_SHAPE_MATCHING_GLOBAL 0u
+#define PF_SHAPE_MATCHING_LOCAL 1u
 int pf_softbody_create(pf_particles* ps, const pf_softbody_def* bodies, uint32_t nbodies,
-                       pf_softbody** out);
+                       uint32_t shape_matching, pf_softbody** out);
 /* setCompliance(index, compliance) (SoftbodySystem.ts:220-224) */
 int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double compliance);
 /* bodyCenters (vec4 per body) and bodyRotations (3 vec4 rows per body) */

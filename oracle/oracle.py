@@ -63,6 +63,10 @@ def lib():
         g("scene_add_sdf").restype = i32
         g("scene_sdf_sample").argtypes = [vp, u32, P(f64), P(f64), P(f64)]
         g("scene_sdf_sample").restype = None
+        g("scene_add_body_edges").argtypes = [vp, P(u32), u32]
+        g("scene_add_body_edges").restype = i32
+        g("scene_get_orientation").argtypes = [vp, P(f64), P(f64), P(f64)]
+        g("scene_get_orientation").restype = None
         g("scene_add_boundary").argtypes = [vp, u32, u32, u32, i32]
         g("scene_add_boundary").restype = i32
         g("scene_get_bvol").argtypes = [vp, P(f64)]
@@ -210,6 +214,17 @@ class Scene:
         g = np.zeros(3)
         self._f("scene_sdf_sample")(self.h, int(k), _p(xx, f64), _p(phi, f64), _p(g, f64))
         return float(phi[0]), g
+
+    def add_body_edges(self, edges):
+        """Edges (local pairs) of the last added body; makes shape matching local."""
+        e = np.ascontiguousarray(edges, np.uint32).reshape(-1)
+        rc = self._f("scene_add_body_edges")(self.h, _p(e, u32), len(e) // 2)
+        assert rc == 0
+
+    def orientation(self):
+        q, qp, om = (np.zeros((self.n, 4)) for _ in range(3))
+        self._f("scene_get_orientation")(self.h, _p(q, f64), _p(qp, f64), _p(om, f64))
+        return q, qp, om
 
     def add_boundary(self, fluid, start, count, dynamic=True):
         rc = self._f("scene_add_boundary")(self.h, int(fluid), int(start), int(count), 1 if dynamic else 0)

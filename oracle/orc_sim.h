@@ -73,6 +73,16 @@ typedef struct {
   REAL *rest, *weights, *slam; /* rest/slam: [3n] */
   REAL centers[4096][3];
   REAL rot[4096][9];
+  /* local shape matching (shapeMatchImplicit.ts, rotation.ts), id order */
+  int local;
+  double radius_d;
+  uint32_t nledges, capledges;
+  uint32_t* ledges;            /* global-id pairs, bodies in order */
+  uint32_t span_lo, span_hi;
+  uint32_t *csr_off, *csr_idx; /* built on the first step */
+  REAL *rcen, *gcen, *grot, *lcomp, *plam4; /* [3n], [3n], [9n], [n], [3 * entries] */
+  int32_t* lacc;               /* [3n] */
+  REAL *q, *qp, *om;           /* [4n] orientation, predicted orientation, angular velocity */
   /* loop */
   uint32_t substeps, iterations;
   REAL g[3], dt;
@@ -818,7 +828,7 @@ static void SFX(tree)(REAL (*v)[13]) {
       for (int k = 0; k < 13; ++k) v[t][k] = v[t][k] + v[t + s][k];
 }
 
-static void SFX(shape_match)(SFX(scene)* s, int first) {
+static void SFX(shape_match)(SFX(scene)* s, int first, int pull) {
   static REAL part[256][13];
   for (uint32_t b = 0; b < s->nbodies; ++b) {
     uint32_t st = s->bstart[b], cnt = s->bcount[b];
@@ -845,6 +855,7 @@ static void SFX(shape_match)(SFX(scene)* s, int first) {
     SFX(polar)(A, Rm);
     for (int a = 0; a < 3; ++a) s->centers[b][a] = c[a];
     for (int k = 0; k < 9; ++k) s->rot[b][k] = Rm[k];
+    if (!pull) continue;
     REAL at = s->bcomp[b] / (s->dt * s->dt);
     for (uint32_t i = 0; i < cnt; ++i) {
       uint32_t id = st + i;
@@ -866,6 +877,213 @@ static void SFX(shape_match)(SFX(scene)* s, int first) {
 }
 
 /* --------------------------------------------------------------------- */
+/* local shape matching with oriented particles                          */
+/* --------------------------------------------------------------------- */
+/* buildNeighborGraph (SoftbodySystem.ts:470-495) and the rest centres of the
+ * groups, accumulated into f32 storage (SoftbodySystem.ts:281-292). */
+static void SFX(local_build)(SFX(scene)* s) {
+  uint32_t n = s->n;
+  uint32_t* deg = (uint32_t*)calloc(n, sizeof(uint32_t));
+  s->span_lo = 0xffffffffu;
+  s->span_hi = 0;
+  for (uint32_t b = 0; b < s->nbodies; ++b) {
+    for (uint32_t i = s->bstart[b]; i < s->bstart[b] + s->bcount[b]; ++i) deg[i] += 1;
+    if (s->bstart[b] < s->span_lo) s->span_lo = s->bstart[b];
+    if (s->bstart[b] + s->bcount[b] > s->span_hi) s->span_hi = s->bstart[b] + s->bcount[b];
+  }
+  for (uint32_t e = 0; e < s->nledges; ++e) {
+    deg[s->ledges[2 * e]] += 1;
+    deg[s->ledges[2 * e + 1]] += 1;
+  }
+  s->csr_off = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; ++i) s->csr_off[i + 1] = s->csr_off[i] + deg[i];
+  uint32_t total = s->csr_off[n];
+  s->csr_idx = (uint32_t*)calloc(total ? total : 1, sizeof(uint32_t));
+  uint32_t* cur = deg;
+  for (uint32_t i = 0; i < n; ++i) cur[i] = s->csr_off[i];
+  /* per body: its particles, then its edges (edges are stored body by body) */
+  uint32_t e = 0;
+  for (uint32_t b = 0; b < s->nbodies; ++b) {
+    uint32_t lo = s->bstart[b], hi = s->bstart[b] + s->bcount[b];
+    for (uint32_t i = lo; i < hi; ++i) s->csr_idx[cur[i]++] = i;
+    for (; e < s->nledges && s->ledges[2 * e] >= lo && s->ledges[2 * e] < hi; ++e) {
+      uint32_t i = s->ledges[2 * e], j = s->ledges[2 * e + 1];
+      s->csr_idx[cur[i]++] = j;
+      s->csr_idx[cur[j]++] = i;
+    }
+  }
+  free(deg);
+  s->rcen = (REAL*)calloc(3 * (size_t)n, sizeof(REAL));
+  s->gcen = (REAL*)calloc(3 * (size_t)n, sizeof(REAL));
+  s->grot = (REAL*)calloc(9 * (size_t)n, sizeof(REAL));
+  s->plam4 = (REAL*)calloc(3 * (size_t)(total ? total : 1), sizeof(REAL));
+  s->lacc = (int32_t*)calloc(3 * (size_t)n, sizeof(int32_t));
+  for (uint32_t b = 0; b < s->nbodies; ++b)
+    for (uint32_t i = s->bstart[b]; i < s->bstart[b] + s->bcount[b]; ++i) {
+      uint32_t from = s->csr_off[i], to = s->csr_off[i + 1];
+      float acc[3] = {0.f, 0.f, 0.f};
+      for (uint32_t k = from; k < to; ++k) {
+        uint32_t j = s->csr_idx[k];
+        for (int a = 0; a < 3; ++a) acc[a] = (float)((double)acc[a] + (double)(float)s->rest[3 * (size_t)j + a] / (double)(to - from));
+      }
+      for (int a = 0; a < 3; ++a) s->rcen[3 * (size_t)i + a] = (REAL)acc[a];
+    }
+}
+
+/* rotation.ts:38-55 */
+static void SFX(orient_predict)(SFX(scene)* s) {
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    REAL* q = s->q + 4 * (size_t)i;
+    REAL* qp = s->qp + 4 * (size_t)i;
+    const REAL* om = s->om + 4 * (size_t)i;
+    REAL mag = RSQRT(SFX(dot)(om, om));
+    if (mag < (REAL)1e-8) {
+      for (int a = 0; a < 4; ++a) qp[a] = q[a];
+      continue;
+    }
+    REAL half = (mag * s->dt) * R(0.5);
+    REAL sr = RSIN(half) / mag;
+    REAL sp[3] = {om[0] * sr, om[1] * sr, om[2] * sr};
+    REAL sw = RCOS(half), cr[3];
+    SFX(cross)(sp, q, cr);
+    for (int a = 0; a < 3; ++a) qp[a] = (q[a] * sw + sp[a] * q[3]) + cr[a];
+    qp[3] = sw * q[3] - SFX(dot)(sp, q);
+  }
+}
+
+/* shapeMatchImplicit.ts:47-402: centres, rotations, pulls, apply, orientations */
+static void SFX(local_iteration)(SFX(scene)* s, int first) {
+  const REAL ai = (REAL)(float)((s->radius_d * s->radius_d) / 5.0);
+  const REAL maxc2 = (REAL)(float)((5.0 * s->radius_d) * (5.0 * s->radius_d));
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    uint32_t o0 = s->csr_off[i], o1 = s->csr_off[i + 1], cnt = o1 - o0;
+    REAL sum[3] = {0, 0, 0};
+    for (uint32_t k = o0; k < o1; ++k)
+      for (int a = 0; a < 3; ++a) sum[a] = sum[a] + s->xs[3 * (size_t)s->csr_idx[k] + a];
+    REAL inv = R(1.0) / (REAL)(cnt > 1 ? cnt : 1);
+    for (int a = 0; a < 3; ++a) s->gcen[3 * (size_t)i + a] = cnt ? sum[a] * inv : R(0.0);
+  }
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    uint32_t o0 = s->csr_off[i], o1 = s->csr_off[i + 1];
+    REAL A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Rm[9];
+    const REAL* ci = s->gcen + 3 * (size_t)i;
+    const REAL* cb = s->rcen + 3 * (size_t)i;
+    for (uint32_t k = o0; k < o1; ++k) {
+      uint32_t j = s->csr_idx[k];
+      const REAL* qj = s->qp + 4 * (size_t)j;
+      REAL xx = qj[0] * qj[0], yy = qj[1] * qj[1], zz = qj[2] * qj[2];
+      REAL xy = qj[0] * qj[1], xz = qj[0] * qj[2], yz = qj[1] * qj[2];
+      REAL wx = qj[3] * qj[0], wy = qj[3] * qj[1], wz = qj[3] * qj[2];
+      REAL rj[9] = {R(1.0) - (yy + zz) * R(2.0), (xy - wz) * R(2.0), (xz + wy) * R(2.0),
+                    (xy + wz) * R(2.0), R(1.0) - (xx + zz) * R(2.0), (yz - wx) * R(2.0),
+                    (xz - wy) * R(2.0), (yz + wx) * R(2.0), R(1.0) - (xx + yy) * R(2.0)};
+      for (int t = 0; t < 9; ++t) A[t] = A[t] + ai * rj[t];
+      REAL xv[3], tv[3];
+      for (int a = 0; a < 3; ++a) {
+        xv[a] = s->xs[3 * (size_t)j + a] - ci[a];
+        tv[a] = s->rest[3 * (size_t)j + a] - cb[a];
+      }
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) A[3 * r + c] = A[3 * r + c] + xv[r] * tv[c];
+    }
+    SFX(polar)(A, Rm);
+    if (o1 == o0)
+      for (int t = 0; t < 9; ++t) Rm[t] = (t % 4 == 0) ? R(1.0) : R(0.0);
+    for (int t = 0; t < 9; ++t) s->grot[9 * (size_t)i + t] = Rm[t];
+  }
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    uint32_t o0 = s->csr_off[i], o1 = s->csr_off[i + 1];
+    const REAL* Rm = s->grot + 9 * (size_t)i;
+    const REAL* ci = s->gcen + 3 * (size_t)i;
+    const REAL* cb = s->rcen + 3 * (size_t)i;
+    REAL at = s->lcomp[i] / (s->dt * s->dt);
+    for (uint32_t k = o0; k < o1; ++k) {
+      uint32_t j = s->csr_idx[k];
+      REAL rr[3], C[3], dl[3], dx[3];
+      for (int a = 0; a < 3; ++a) rr[a] = s->rest[3 * (size_t)j + a] - cb[a];
+      for (int a = 0; a < 3; ++a) {
+        REAL goal = ((Rm[3 * a] * rr[0] + Rm[3 * a + 1] * rr[1]) + Rm[3 * a + 2] * rr[2]) + ci[a];
+        C[a] = s->xs[3 * (size_t)j + a] - goal;
+      }
+      REAL wj = s->w[j];
+      REAL den = wj + at;
+      REAL* lam = s->plam4 + 3 * (size_t)k;
+      for (int a = 0; a < 3; ++a) {
+        REAL lo = first ? R(0.0) : lam[a];
+        dl[a] = ((-C[a]) - lo * at) / den;
+        lam[a] = lo + dl[a];
+      }
+      uint32_t mc = s->csr_off[j + 1] - s->csr_off[j];
+      REAL mj = RFMAX((REAL)mc, R(1.0));
+      for (int a = 0; a < 3; ++a) dx[a] = (dl[a] * wj) / mj;
+      REAL sc = RSQRT(maxc2 / RFMAX(SFX(dot)(dx, dx), maxc2));
+      for (int a = 0; a < 3; ++a)
+        s->lacc[3 * (size_t)j + a] = SFX(wadd)(s->lacc[3 * (size_t)j + a], SFX(ticks)(dx[a] * sc, s->pos_scale));
+    }
+  }
+  for (uint32_t j = s->span_lo; j < s->span_hi; ++j)
+    for (int a = 0; a < 3; ++a) {
+      s->xs[3 * (size_t)j + a] = s->xs[3 * (size_t)j + a] + (REAL)s->lacc[3 * (size_t)j + a] * s->pos_inv;
+      s->lacc[3 * (size_t)j + a] = 0;
+    }
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    if (!(s->csr_off[i + 1] > s->csr_off[i])) continue;
+    const REAL* M = s->grot + 9 * (size_t)i;
+    REAL m00 = M[0], m01 = M[1], m02 = M[2], m10 = M[3], m11 = M[4], m12 = M[5], m20 = M[6], m21 = M[7], m22 = M[8];
+    REAL trace = (m00 + m11) + m22, qx, qy, qz, qw;
+    if (trace > R(0.0)) {
+      REAL sq = R(0.5) / RSQRT(trace + R(1.0));
+      qw = R(0.25) / sq; qx = (m21 - m12) * sq; qy = (m02 - m20) * sq; qz = (m10 - m01) * sq;
+    } else if (m00 > m11 && m00 > m22) {
+      REAL sq = R(2.0) * RSQRT(((R(1.0) + m00) - m11) - m22);
+      qw = (m21 - m12) / sq; qx = R(0.25) * sq; qy = (m01 + m10) / sq; qz = (m02 + m20) / sq;
+    } else if (m11 > m22) {
+      REAL sq = R(2.0) * RSQRT(((R(1.0) + m11) - m00) - m22);
+      qw = (m02 - m20) / sq; qx = (m01 + m10) / sq; qy = R(0.25) * sq; qz = (m12 + m21) / sq;
+    } else {
+      REAL sq = R(2.0) * RSQRT(((R(1.0) + m22) - m00) - m11);
+      qw = (m10 - m01) / sq; qx = (m02 + m20) / sq; qy = (m12 + m21) / sq; qz = R(0.25) * sq;
+    }
+    const REAL* qs = s->q + 4 * (size_t)i;
+    REAL d = ((qs[0] * qx + qs[1] * qy) + qs[2] * qz) + qs[3] * qw;
+    REAL f = d < R(0.0) ? R(-1.0) : R(1.0);
+    REAL* qp = s->qp + 4 * (size_t)i;
+    qp[0] = qx * f; qp[1] = qy * f; qp[2] = qz * f; qp[3] = qw * f;
+  }
+}
+
+/* rotation.ts:57-82 */
+static void SFX(orient_advect)(SFX(scene)* s) {
+  for (uint32_t i = s->span_lo; i < s->span_hi; ++i) {
+    REAL* q = s->q + 4 * (size_t)i;
+    const REAL* raw = s->qp + 4 * (size_t)i;
+    REAL l = RSQRT(RFMAX(((raw[0] * raw[0] + raw[1] * raw[1]) + raw[2] * raw[2]) + raw[3] * raw[3], (REAL)1e-12));
+    REAL qp[4] = {raw[0] / l, raw[1] / l, raw[2] / l, raw[3] / l};
+    REAL qi[3] = {-q[0], -q[1], -q[2]}, cr[3], rv[3];
+    SFX(cross)(qp, qi, cr);
+    REAL rpre[3];
+    for (int a = 0; a < 3; ++a) rpre[a] = (qp[a] * q[3] + qi[a] * qp[3]) + cr[a];
+    REAL wpre = qp[3] * q[3] - SFX(dot)(qp, qi);
+    REAL f = wpre < R(0.0) ? R(-1.0) : R(1.0);
+    for (int a = 0; a < 3; ++a) rv[a] = rpre[a] * f;
+    REAL rw = wpre * f;
+    REAL hs = RSQRT(SFX(dot)(rv, rv));
+    REAL* om = s->om + 4 * (size_t)i;
+    if (hs < (REAL)1e-8) {
+      om[0] = om[1] = om[2] = om[3] = R(0.0);
+    } else {
+      REAL angle = RATAN2(hs, rw) * R(2.0);
+      REAL k = (angle / hs) / s->dt, o[3];
+      for (int a = 0; a < 3; ++a) o[a] = rv[a] * k;
+      REAL sc = RSQRT(R(10000.0) / RFMAX(SFX(dot)(o, o), R(10000.0)));
+      for (int a = 0; a < 3; ++a) om[a] = o[a] * sc;
+      om[3] = R(0.0);
+    }
+    for (int a = 0; a < 4; ++a) q[a] = qp[a];
+  }
+}
+
+/* --------------------------------------------------------------------- */
 /* one step (loop.ts:175-279)                                            */
 /* --------------------------------------------------------------------- */
 static void SFX(step)(SFX(scene)* s, double dt_step) {
@@ -880,6 +1098,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
   }
   s->flag_pos = 0;
   s->flag_vel = 0;
+  if (s->local && !s->csr_off) SFX(local_build)(s);
   if (!s->initialized) {
     /* first step: static boundary volumes from committed positions, over a
      * grid of the current predictions (loop.ts:236, 276-277) */
@@ -907,6 +1126,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
         s->xs[3 * (size_t)i + a] = s->x[3 * (size_t)i + a] + vel * s->dt;
       }
     }
+    if (s->local) SFX(orient_predict)(s);
     s->flag_nl = 0;
     if (s->nfluids) SFX(grid_build)(s);
     for (uint32_t f = 0; f < s->nfluids; ++f)
@@ -925,7 +1145,8 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
         SFX(lambda_pass)(s, &s->fluid[f]);
         SFX(delta_pass)(s, &s->fluid[f]);
       }
-      if (s->nbodies) SFX(shape_match)(s, it == 0);
+      if (s->local) SFX(local_iteration)(s, it == 0);
+      else if (s->nbodies) SFX(shape_match)(s, it == 0, 1);
       if (s->nprim || s->nsdf) SFX(walls_solve)(s, s->clock[k]);
     }
     /* advect (integrate.ts:36-44) */
@@ -937,6 +1158,10 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       }
     }
     for (uint32_t f = 0; f < s->nfluids; ++f) SFX(velocity_passes)(s, &s->fluid[f]);
+    if (s->local) {
+      SFX(orient_advect)(s);
+      SFX(shape_match)(s, 0, 0);
+    }
     if ((s->nprim || s->nsdf) && s->iterations > 0) SFX(walls_friction)(s, s->clock[k]);
   }
 }

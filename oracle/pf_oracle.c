@@ -199,11 +199,13 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
 #define RFABS fabsf
 #define RSIN sinf
 #define RCOS cosf
+#define RATAN2 atan2f
 #define RRINT rintf
 #include "orc_sim.h"
 #undef REAL
 #undef SFX
 #undef R
+#undef RATAN2
 #undef RSQRT
 #undef RFLOOR
 #undef RFMAX
@@ -223,11 +225,13 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
 #define RFABS fabs
 #define RSIN sin
 #define RCOS cos
+#define RATAN2 atan2
 #define RRINT rint
 #include "orc_sim.h"
 #undef REAL
 #undef SFX
 #undef R
+#undef RATAN2
 
 /* ---------------------------------------------------------------------- */
 /* exported scene API (same entry points for both precisions)              */
@@ -253,6 +257,12 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     s->rest = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
     s->weights = (T*)calloc(n, sizeof(T));                                                         \
     s->slam = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
+    s->radius_d = radius;                                                                          \
+    s->q = (T*)calloc(4 * (size_t)n, sizeof(T));                                                   \
+    s->qp = (T*)calloc(4 * (size_t)n, sizeof(T));                                                  \
+    s->om = (T*)calloc(4 * (size_t)n, sizeof(T));                                                  \
+    s->lcomp = (T*)calloc(n, sizeof(T));                                                           \
+    for (size_t k = 0; k < n; ++k) { s->q[4 * k + 3] = (T)1; s->qp[4 * k + 3] = (T)1; }            \
     s->substeps = 4;                                                                               \
     s->iterations = 2;                                                                             \
     s->g[1] = (T)-9.81;                                                                            \
@@ -269,6 +279,8 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     free(s->key); free(s->perm); free(s->start); free(s->plam); free(s->pacc); free(s->vacc);      \
     free(s->iacc); free(s->racc); free(s->rest); free(s->weights); free(s->slam); free(s->sdflam); \
     for (uint32_t k = 0; k < s->nsdf; ++k) free(s->sdf[k].grid);                                   \
+    free(s->q); free(s->qp); free(s->om); free(s->lcomp); free(s->ledges); free(s->csr_off);      \
+    free(s->csr_idx); free(s->rcen); free(s->gcen); free(s->grot); free(s->plam4); free(s->lacc); \
     for (uint32_t f = 0; f < s->nfluids; ++f) {                                                    \
       fluid_##SUF* F = &s->fluid[f];                                                               \
       free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
@@ -416,11 +428,39 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     if (s->nbodies >= 4096) return -1;                                                             \
     uint32_t b = s->nbodies++;                                                                     \
     s->bstart[b] = start; s->bcount[b] = count; s->bcomp[b] = (T)(float)compliance;                \
+    for (uint32_t i = 0; i < count; ++i) s->lcomp[start + i] = (T)(float)compliance;               \
     for (uint32_t i = 0; i < count; ++i) {                                                         \
       for (int a = 0; a < 3; ++a) s->rest[3 * (size_t)(start + i) + a] = (T)rest3[3 * (size_t)i + a]; \
       s->weights[start + i] = (T)weights[i];                                                       \
     }                                                                                              \
     return 0;                                                                                      \
+  }                                                                                                \
+  /* edges of the last added body, local pairs (SoftbodyDef.edges); switches the scene to local   \
+   * shape matching (SoftbodySystem shapeMatching: 'local') */                                     \
+  int orc_scene_add_body_edges_##SUF(void* p, const uint32_t* edges, uint32_t nedges) {            \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (!s->nbodies) return -1;                                                                    \
+    uint32_t b = s->nbodies - 1, st = s->bstart[b];                                                \
+    if (s->nledges + nedges > s->capledges) {                                                      \
+      s->capledges = 2 * (s->nledges + nedges);                                                    \
+      s->ledges = (uint32_t*)realloc(s->ledges, 2 * (size_t)s->capledges * sizeof(uint32_t));      \
+    }                                                                                              \
+    for (uint32_t e = 0; e < nedges; ++e) {                                                        \
+      s->ledges[2 * (size_t)(s->nledges + e)] = st + edges[2 * e];                                 \
+      s->ledges[2 * (size_t)(s->nledges + e) + 1] = st + edges[2 * e + 1];                         \
+    }                                                                                              \
+    s->nledges += nedges;                                                                          \
+    s->local = 1;                                                                                  \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* orientation, predicted orientation, angular velocity (xyzw per particle) */                  \
+  void orc_scene_get_orientation_##SUF(void* p, double* q4, double* qp4, double* om4) {           \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    for (size_t k = 0; k < 4 * (size_t)s->n; ++k) {                                                \
+      if (q4) q4[k] = (double)s->q[k];                                                             \
+      if (qp4) qp4[k] = (double)s->qp[k];                                                          \
+      if (om4) om4[k] = (double)s->om[k];                                                          \
+    }                                                                                              \
   }                                                                                                \
   void orc_scene_step_##SUF(void* p, double dt) { step_##SUF((scene_##SUF*)p, dt); }               \
   /* state out as double: x, x*, v as xyz triples, density */                                     \

@@ -74,3 +74,56 @@ def oracle_boundary_scene(pos, w, nf, nb, radius, dynamic, substeps=2, iteration
         for nrm, which in scenes.PLANES:
             s.add_plane(nrm, mn if which == "min" else mx)
     return s
+
+
+def voxel_grid(nx, ny, nz, radius):
+    """An nx*ny*nz particle block at spacing 2r centred on the origin, with its
+    6-face edge list (i < j), as the reference's local shape-matching tests
+    build it (tests/analytical/softbody/softbody-implicit.gpu.test.ts:24-62)."""
+    sp = 2.0 * radius
+    ox, oy, oz = -((nx - 1) * sp) / 2, -((ny - 1) * sp) / 2, -((nz - 1) * sp) / 2
+    idx = lambda ix, iy, iz: iz * nx * ny + iy * nx + ix
+    rest = np.zeros((nx * ny * nz, 3), np.float32)
+    edges = []
+    for iz in range(nz):
+        for iy in range(ny):
+            for ix in range(nx):
+                i = idx(ix, iy, iz)
+                rest[i] = (ox + ix * sp, oy + iy * sp, oz + iz * sp)
+                if ix + 1 < nx:
+                    edges += [i, idx(ix + 1, iy, iz)]
+                if iy + 1 < ny:
+                    edges += [i, idx(ix, iy + 1, iz)]
+                if iz + 1 < nz:
+                    edges += [i, idx(ix, iy, iz + 1)]
+    return rest, np.array(edges, np.uint32)
+
+
+def rest_offsets(rest, weights=None):
+    """Rest offsets from the mass-weighted rest centre, as SoftbodySystem
+    stores them (f32 of the double difference, SoftbodySystem.ts:178-185)."""
+    r = np.asarray(rest, np.float64)
+    w = np.ones(len(r)) if weights is None else np.asarray(weights, np.float64)
+    c = (r * w[:, None]).sum(0) / w.sum()
+    return (r - c).astype(np.float32), np.asarray(w, np.float32)
+
+
+def oracle_local_body(positions, velocities, inv_mass, rest, edges, compliance, radius, substeps, iterations,
+                      gravity, precision="f32"):
+    """One locally shape-matched body owning every particle, in the oracle."""
+    n = len(positions)
+    s = orc.Scene(n, radius, precision)
+    s.set_particles(positions, velocities, inv_mass)
+    s.set_loop(substeps, iterations, gravity, 4 * radius, 64)
+    off, wts = rest_offsets(rest)
+    s.add_body(0, n, compliance, off, wts)
+    s.add_body_edges(edges)
+    return s
+
+
+def quat_to_mat(q):
+    q = np.asarray(q, np.float64)
+    x, y, z, w = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])

@@ -99,6 +99,22 @@ struct SoftDev {
   float4* lambda;       // [n] XPBD multiplier per particle
   float4* centers;      // [nbodies]
   float4* rotations;    // [3 * nbodies] rows of R
+  // Local shape matching (shapeMatchImplicit.ts, rotation.ts), id order over
+  // the span of every body.
+  uint32_t local;       // 1: local shape matching
+  uint32_t span_lo, span_hi;
+  const uint32_t* csr_off;   // [n + 1] row offsets; row i = i, then its edge neighbours
+  const uint32_t* csr_idx;   // [csr_off[n]]
+  const float4* rest_center; // [n] rest centre of each particle's group
+  float4* group_center;      // [n] current centre c_i
+  float4* group_rot;         // [3n] rows of R_i
+  float4* pair_lambda;       // [csr_off[n]] multiplier per (group, member)
+  int* acc;                  // [3n] Accumulator(10) of the pulls
+  float4* q;                 // [n] orientation (rotation)
+  float4* qp;                // [n] predicted orientation
+  float4* omega;             // [n] angular velocity
+  float ai;                  // r^2 / 5 (orientation term weight)
+  float maxc2;               // (5 r)^2 (pull clamp)
 };
 
 // A signed distance field collider (SDFCollider.ts:68-311): half-float

@@ -50,4 +50,8 @@ void launch_scatter_upload(hipStream_t st, const SceneDev* sc, int p, int c, uin
                            uint32_t count, const float4* pos, const float4* vel, const float* inv_mass,
                            const uint32_t* group, uint32_t mask);
 
+void launch_local_predict(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles);
+void launch_local_solve(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles, int p, int c,
+                        int first_iteration);
+void launch_local_advect(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles);
 }  // namespace pf

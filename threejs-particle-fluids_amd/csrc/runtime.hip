@@ -505,12 +505,22 @@ int pf_particles_readback(pf_particles* ps, const pf_snapshot* out) {
   if (out->predicted_positions) particles_gather(ps, G_PRED, out->predicted_positions, sizeof(float4));
   if (out->velocities) particles_gather(ps, G_VEL, out->velocities, sizeof(float4));
   if (out->inv_mass) particles_gather(ps, G_INVMASS, out->inv_mass, sizeof(float));
-  // Orientations only change under local shape matching; every particle stays at identity here.
-  for (float* q : {out->rotation, out->predicted_rotation}) {
-    if (!q) continue;
-    for (uint32_t i = 0; i < ps->n; ++i) { q[4 * i] = 0; q[4 * i + 1] = 0; q[4 * i + 2] = 0; q[4 * i + 3] = 1; }
+  // Orientations only change under local shape matching, whose loop keeps
+  // them in particle order; otherwise every particle stays at the identity.
+  const pf_softbody_s* sb = ps->loop && ps->loop->soft && ps->loop->soft->mode == PF_SHAPE_MATCHING_LOCAL
+                                ? ps->loop->soft : nullptr;
+  const size_t bytes = sizeof(float) * 4 * (size_t)ps->n;
+  if (sb) {
+    if (out->rotation) PF_HIP(hipMemcpy(out->rotation, sb->d_q, bytes, hipMemcpyDeviceToHost));
+    if (out->predicted_rotation) PF_HIP(hipMemcpy(out->predicted_rotation, sb->d_qp, bytes, hipMemcpyDeviceToHost));
+    if (out->angular_velocity) PF_HIP(hipMemcpy(out->angular_velocity, sb->d_omega, bytes, hipMemcpyDeviceToHost));
+  } else {
+    for (float* q : {out->rotation, out->predicted_rotation}) {
+      if (!q) continue;
+      for (uint32_t i = 0; i < ps->n; ++i) { q[4 * i] = 0; q[4 * i + 1] = 0; q[4 * i + 2] = 0; q[4 * i + 3] = 1; }
+    }
+    if (out->angular_velocity) std::memset(out->angular_velocity, 0, bytes);
   }
-  if (out->angular_velocity) std::memset(out->angular_velocity, 0, sizeof(float) * 4 * (size_t)ps->n);
   PF_API_END
 }
 

@@ -204,6 +204,19 @@ struct pf_softbody_s {
   float4* d_lambda = nullptr;
   float4* d_centers = nullptr;
   float4* d_rotations = nullptr;
+  // local shape matching
+  uint32_t mode = 0;             // PF_SHAPE_MATCHING_*
+  uint32_t span_lo = 0, span_hi = 0;
+  uint32_t* d_csr_off = nullptr;
+  uint32_t* d_csr_idx = nullptr;
+  float4* d_rest_center = nullptr;
+  float4* d_group_center = nullptr;
+  float4* d_group_rot = nullptr;
+  float4* d_pair_lambda = nullptr;
+  int* d_acc = nullptr;
+  float4* d_q = nullptr;
+  float4* d_qp = nullptr;
+  float4* d_omega = nullptr;
   pf_loop_s* loop = nullptr;
   bool disposed = false;
 };
@@ -216,6 +229,7 @@ struct PlanShape {
   bool any_boundary = false;
   uint32_t nb[PF_MAX_FLUIDS] = {0, 0, 0, 0};    // boundary ranges per fluid slot (max over scenes)
   uint32_t bdyn[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of dynamic-boundary bits per fluid slot
+  bool local_soft = false;  // any scene's soft bodies use local shape matching
   bool walls = false;       // any scene has primitives
   bool soft = false;        // any scene has soft bodies
 };
@@ -236,6 +250,8 @@ struct pf_loop_s {
   uint32_t nstiles = 0;
   uint2* btiles = nullptr;
   uint32_t nbtiles = 0;
+  uint2* ltiles = nullptr;   // (scene, first id) tiles over the local soft-body span
+  uint32_t nltiles = 0;
   bool static_boundaries = false;  // a fluid has a static boundary (volumes computed on the first step)
   double last_dt = -1;
   uint32_t last_flag_half = 0;
@@ -258,6 +274,8 @@ struct pf_batch_s {
   uint32_t nstiles = 0;
   uint2* btiles = nullptr;
   uint32_t nbtiles = 0;
+  uint2* ltiles = nullptr;
+  uint32_t nltiles = 0;
   void* scan_ctl = nullptr;
   PlanShape shape;
   bool use_graph = false;
@@ -270,7 +288,8 @@ namespace pf {
 // The substep schedule, shared by loops and batches (sim.hip).
 uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& shape,
                       const Tile* tiles, uint32_t ntiles, const uint2* stiles, uint32_t nstiles,
-                      const uint2* btiles, uint32_t nbtiles, void* scan_ctl);
+                      const uint2* btiles, uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles,
+                      void* scan_ctl);
 // Swap descriptor halves after a step with an odd number of state flips.
 void finish_step_host(Particles* ps, const PlanShape& shape);
 }  // namespace pf

@@ -73,7 +73,7 @@ __global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __res
 // Returns the number of kernel launches.
 uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh, const Tile* tiles,
                       uint32_t ntiles, const uint2* stiles, uint32_t nstiles, const uint2* btiles,
-                      uint32_t nbtiles, void* scan_ctl) {
+                      uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles, void* scan_ctl) {
   hipStream_t st = ctx->stream;
   uint32_t nk = 0;
   int c = 0;
@@ -104,6 +104,11 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       hipLaunchKernelGGL(k_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, 0, c, k == 0);
       ++nk;
       c ^= 1;
+    }
+    if (sh.local_soft) {
+      PF_LAUNCH(ctx, "orient_predict");
+      launch_local_predict(st, scenes, ltiles, nltiles);
+      ++nk;
     }
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       if (sh.bdyn[f]) {
@@ -151,6 +156,11 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
         launch_shape_match(st, scenes, btiles, nbtiles, p, c, it == 0, 1);
         ++nk;
       }
+      if (sh.local_soft) {
+        PF_LAUNCH(ctx, "local_shape_match");
+        launch_local_solve(st, scenes, ltiles, nltiles, p, c, it == 0);
+        nk += 5;
+      }
       if (sh.walls && last == WALLS) {
         PF_LAUNCH(ctx, "walls");
         launch_walls(st, scenes, tiles, ntiles, p, c, lastit, it == 0, (int)k);
@@ -161,6 +171,12 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       PF_LAUNCH(ctx, "advect");
       launch_advect(st, scenes, tiles, ntiles, p, c);
       ++nk;
+    }
+    if (sh.local_soft) {
+      PF_LAUNCH(ctx, "orient_advect");
+      launch_local_advect(st, scenes, ltiles, nltiles);
+      launch_shape_match(st, scenes, btiles, nbtiles, p, c, 0, 0);
+      nk += 2;
     }
     const int touch_valid = sh.iterations > 0 ? 1 : 0;
     bool friction_done = !sh.walls;
@@ -815,12 +831,16 @@ static double smallest_eigenvalue(double a00, double a01, double a02, double a11
   return std::min(a00, std::min(a11, a22));
 }
 
-int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t nb, pf_softbody** out) {
+int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t nb, uint32_t mode,
+                       pf_softbody** out) {
   PF_API_BEGIN
   alive(ps, "SoftbodySystem");
   if (nb == 0 || !defs) fail(PF_EINVAL, "SoftbodySystem: at least one body is required");
+  if (mode != PF_SHAPE_MATCHING_GLOBAL && mode != PF_SHAPE_MATCHING_LOCAL)
+    fail(PF_EINVAL, "SoftbodySystem: shapeMatching must be 'global' or 'local'");
   auto sb = std::make_unique<pf_softbody_s>();
   sb->ps = ps;
+  sb->mode = mode;
   const uint32_t n = ps->n;
   sb->rest.assign(4 * (size_t)n, 0.0f);
   sb->weights.assign(n, 0.0f);
@@ -840,6 +860,16 @@ int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t n
       restp[k] = d.rest_positions ? d.rest_positions[k] : ps->cpu_positions[3 * (size_t)d.start + k];
     const uint32_t surface = d.surface_count == 0xffffffffu ? d.count : d.surface_count;
     if (surface > d.count) fail(PF_EINVAL, "SoftbodySystem: body %u surfaceCount %u is out of range", b, surface);
+    if (d.edges) {
+      // assertEdges (SoftbodySystem.ts:453-463)
+      for (uint32_t e = 0; e < d.edge_count; ++e) {
+        const uint32_t i = d.edges[2 * e], j = d.edges[2 * e + 1];
+        if (!(i < d.count) || !(j < d.count) || i == j)
+          fail(PF_EINVAL, "SoftbodySystem: body %u has an invalid edge (%u, %u)", b, i, j);
+      }
+    } else if (mode == PF_SHAPE_MATCHING_LOCAL) {
+      fail(PF_EINVAL, "SoftbodySystem: body %u needs edges for local shape matching", b);
+    }
     if (!(d.compliance >= 0) || !std::isfinite(d.compliance))
       fail(PF_EINVAL, "SoftbodySystem: compliance must be ≥ 0, got %g", d.compliance);
     // massWeights (SoftbodySystem.ts:366-372)
@@ -910,6 +940,69 @@ int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t n
     rot[3 * b] = make_float4(1, 0, 0, 0); rot[3 * b + 1] = make_float4(0, 1, 0, 0); rot[3 * b + 2] = make_float4(0, 0, 1, 0);
   }
   PF_HIP(hipMemcpy(sb->d_rotations, rot.data(), rot.size() * 16, hipMemcpyHostToDevice));
+  if (mode == PF_SHAPE_MATCHING_LOCAL) {
+    // buildNeighborGraph (SoftbodySystem.ts:470-495): row i = i, then its edge
+    // neighbours in edge order, over global ids.
+    std::vector<uint32_t> deg(n, 0);
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+      const pf_softbody_def& d = defs[b];
+      for (uint32_t i = 0; i < d.count; ++i) deg[d.start + i] += 1;
+      for (uint32_t e = 0; e < d.edge_count; ++e) {
+        deg[d.start + d.edges[2 * e]] += 1;
+        deg[d.start + d.edges[2 * e + 1]] += 1;
+      }
+      lo = std::min(lo, d.start);
+      hi = std::max(hi, d.start + d.count);
+    }
+    std::vector<uint32_t> off(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + deg[i];
+    std::vector<uint32_t> idx(std::max<uint32_t>(1, off[n]), 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t b = 0; b < nb; ++b) {
+      const pf_softbody_def& d = defs[b];
+      for (uint32_t i = d.start; i < d.start + d.count; ++i) idx[cur[i]++] = i;
+      for (uint32_t e = 0; e < d.edge_count; ++e) {
+        const uint32_t i = d.start + d.edges[2 * e], j = d.start + d.edges[2 * e + 1];
+        idx[cur[i]++] = j;
+        idx[cur[j]++] = i;
+      }
+    }
+    // Rest centre of each group, accumulated into f32 storage one member at
+    // a time like the reference's Float32Array (SoftbodySystem.ts:281-292).
+    std::vector<float4> rc(n, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (uint32_t b = 0; b < nb; ++b) {
+      const pf_softbody_def& d = defs[b];
+      for (uint32_t i = d.start; i < d.start + d.count; ++i) {
+        const uint32_t from = off[i], to = off[i + 1];
+        float* acc = &rc[i].x;
+        for (uint32_t k = from; k < to; ++k) {
+          const uint32_t j = idx[k];
+          for (int a = 0; a < 3; ++a)
+            acc[a] = (float)((double)acc[a] + (double)sb->rest[4 * (size_t)j + a] / (double)(to - from));
+        }
+      }
+    }
+    sb->span_lo = lo;
+    sb->span_hi = hi;
+    sb->d_csr_off = dalloc<uint32_t>(n + 1);
+    sb->d_csr_idx = dalloc<uint32_t>(idx.size());
+    sb->d_rest_center = dalloc<float4>(n);
+    sb->d_group_center = dalloc<float4>(n);
+    sb->d_group_rot = dalloc<float4>(3 * (size_t)n);
+    sb->d_pair_lambda = dalloc<float4>(idx.size());
+    sb->d_acc = dalloc<int>(3 * (size_t)n);
+    sb->d_q = dalloc<float4>(n);
+    sb->d_qp = dalloc<float4>(n);
+    sb->d_omega = dalloc<float4>(n);
+    PF_HIP(hipMemcpy(sb->d_csr_off, off.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(sb->d_csr_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(sb->d_rest_center, rc.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    // Orientations start at the identity (particles.ts:104-112).
+    std::vector<float4> ident(n, make_float4(0.f, 0.f, 0.f, 1.f));
+    PF_HIP(hipMemcpy(sb->d_q, ident.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(sb->d_qp, ident.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+  }
   *out = sb.release();
   PF_API_END
 }
@@ -945,6 +1038,9 @@ int pf_softbody_destroy(pf_softbody* sb) {
   sb->disposed = true;
   dfree(sb->d_start); dfree(sb->d_count); dfree(sb->d_compliance); dfree(sb->d_rest); dfree(sb->d_weights);
   dfree(sb->d_lambda); dfree(sb->d_centers); dfree(sb->d_rotations);
+  dfree(sb->d_csr_off); dfree(sb->d_csr_idx); dfree(sb->d_rest_center); dfree(sb->d_group_center);
+  dfree(sb->d_group_rot); dfree(sb->d_pair_lambda); dfree(sb->d_acc); dfree(sb->d_q); dfree(sb->d_qp);
+  dfree(sb->d_omega);
   PF_API_END
 }
 
@@ -1022,6 +1118,7 @@ static PlanShape loop_shape(const pf_loop* L) {
   for (uint32_t f = 0; f < sh.nfluids; ++f) sh.fx[f] = L->ps->host.fluid[f].effects;
   sh.walls = (L->prims && !L->prims->prims.empty()) || !L->sdfs.empty();
   sh.soft = L->soft != nullptr;
+  sh.local_soft = L->soft != nullptr && L->soft->mode == PF_SHAPE_MATCHING_LOCAL;
   sh.any_boundary = L->ps->host.has_boundary != 0;
   for (uint32_t f = 0; f < sh.nfluids; ++f) {
     sh.nb[f] = L->ps->host.fluid[f].nboundary;
@@ -1197,6 +1294,13 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     B.body_start = sb->d_start; B.body_count = sb->d_count; B.body_compliance = sb->d_compliance;
     B.rest = sb->d_rest; B.weights = sb->d_weights; B.lambda = sb->d_lambda;
     B.centers = sb->d_centers; B.rotations = sb->d_rotations;
+    B.local = sb->mode == PF_SHAPE_MATCHING_LOCAL ? 1u : 0u;
+    B.span_lo = sb->span_lo; B.span_hi = sb->span_hi;
+    B.csr_off = sb->d_csr_off; B.csr_idx = sb->d_csr_idx; B.rest_center = sb->d_rest_center;
+    B.group_center = sb->d_group_center; B.group_rot = sb->d_group_rot; B.pair_lambda = sb->d_pair_lambda;
+    B.acc = sb->d_acc; B.q = sb->d_q; B.qp = sb->d_qp; B.omega = sb->d_omega;
+    B.ai = (float)((ps->radius * ps->radius) / 5.0);
+    B.maxc2 = (float)((5.0 * ps->radius) * (5.0 * ps->radius));
   }
   // Work tiles.
   L->ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
@@ -1210,6 +1314,14 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     for (uint32_t i = 0; i < L->nstiles; ++i) stl[i] = make_uint2(0u, i);
     L->stiles = dalloc<uint2>(L->nstiles);
     PF_HIP(hipMemcpy(L->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
+  if (L->soft && S.soft.local) {
+    const uint32_t span = S.soft.span_hi - S.soft.span_lo;
+    L->nltiles = (span + PF_BLOCK - 1) / PF_BLOCK;
+    std::vector<uint2> ltl(L->nltiles);
+    for (uint32_t i = 0; i < L->nltiles; ++i) ltl[i] = make_uint2(0u, S.soft.span_lo + i * PF_BLOCK);
+    L->ltiles = dalloc<uint2>(L->nltiles);
+    PF_HIP(hipMemcpy(L->ltiles, ltl.data(), ltl.size() * sizeof(uint2), hipMemcpyHostToDevice));
   }
   L->nbtiles = S.soft.nbodies;
   if (L->nbtiles) {
@@ -1301,7 +1413,7 @@ int pf_loop_step(pf_loop* L, double dt) {
       ctx->capturing = true;
       try {
         L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles,
-                                           L->btiles, L->nbtiles, ctx->scan_ctl);
+                                           L->btiles, L->nbtiles, L->ltiles, L->nltiles, ctx->scan_ctl);
       } catch (...) {
         ctx->capturing = false;
         hipGraph_t dummy;
@@ -1317,7 +1429,7 @@ int pf_loop_step(pf_loop* L, double dt) {
     PF_HIP(hipGraphLaunch(L->graph[v], ctx->stream));
   } else {
     L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles, L->btiles,
-                                       L->nbtiles, ctx->scan_ctl);
+                                       L->nbtiles, L->ltiles, L->nltiles, ctx->scan_ctl);
   }
   PF_HIP(hipGetLastError());
   ps->variant ^= step_variant_flips(sh);
@@ -1417,7 +1529,7 @@ int pf_loop_destroy(pf_loop* L) {
   dfree(S.tmp_src); dfree(S.tmp_id); dfree(S.tmp_key); dfree(S.prim_lambda); dfree(S.prim_touch);
   S.key = S.rank = S.counts = S.start = S.tmp_src = S.tmp_id = S.tmp_key = nullptr;
   S.scan_status = nullptr; S.scan_ctl = nullptr; S.prim_lambda = nullptr; S.prim_touch = nullptr;
-  dfree(L->tiles); dfree(L->stiles); dfree(L->btiles);
+  dfree(L->tiles); dfree(L->stiles); dfree(L->btiles); dfree(L->ltiles);
   for (auto* f : L->fluids) {
     f->loop = nullptr;
     dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv);
@@ -1451,7 +1563,7 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
   sh.substeps = loops[0]->substeps;
   sh.iterations = loops[0]->iterations;
   std::vector<Tile> tl;
-  std::vector<uint2> stl, btl;
+  std::vector<uint2> stl, btl, ltl;
   for (uint32_t s = 0; s < n; ++s) {
     pf_loop* L = loops[s];
     loop_alive(L, "Batch");
@@ -1465,6 +1577,7 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
     for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) sh.fx[f] |= ls.fx[f];
     sh.walls = sh.walls || ls.walls;
     sh.soft = sh.soft || ls.soft;
+    sh.local_soft = sh.local_soft || ls.local_soft;
     sh.any_boundary = sh.any_boundary || ls.any_boundary;
     for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
       sh.nb[f] = std::max(sh.nb[f], ls.nb[f]);
@@ -1473,6 +1586,11 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
     for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{s, i * PF_BLOCK});
     for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(s, i));
     for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(s, i));
+    if (ls.local_soft) {
+      const SoftDev& SB = L->ps->host.soft;
+      const uint32_t nl = (SB.span_hi - SB.span_lo + PF_BLOCK - 1) / PF_BLOCK;
+      for (uint32_t i = 0; i < nl; ++i) ltl.push_back(make_uint2(s, SB.span_lo + i * PF_BLOCK));
+    }
     B->loops.push_back(L);
   }
   if (sh.nfluids > 0)
@@ -1482,6 +1600,11 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
   B->ntiles = (uint32_t)tl.size();
   B->nstiles = (uint32_t)stl.size();
   B->nbtiles = (uint32_t)btl.size();
+  B->nltiles = (uint32_t)ltl.size();
+  if (B->nltiles) {
+    B->ltiles = dalloc<uint2>(B->nltiles);
+    PF_HIP(hipMemcpy(B->ltiles, ltl.data(), ltl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  }
   B->tiles = dalloc<Tile>(B->ntiles);
   PF_HIP(hipMemcpy(B->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
   if (B->nstiles) {
@@ -1528,7 +1651,7 @@ int pf_batch_step(pf_batch* B, double dt) {
       PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       ctx->capturing = true;
       try {
-        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->scan_ctl);
+        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles, B->nltiles, B->scan_ctl);
       } catch (...) {
         ctx->capturing = false;
         hipGraph_t dummy;
@@ -1543,7 +1666,7 @@ int pf_batch_step(pf_batch* B, double dt) {
     }
     PF_HIP(hipGraphLaunch(B->graph, ctx->stream));
   } else {
-    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->scan_ctl);
+    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles, B->nltiles, B->scan_ctl);
   }
   PF_HIP(hipGetLastError());
   const int flips = step_variant_flips(B->shape);
@@ -1573,7 +1696,7 @@ int pf_batch_destroy(pf_batch* B) {
   if (!B) return PF_OK;
   (void)hipStreamSynchronize(B->ctx->stream);
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
-  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->scan_ctl);
+  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->ltiles); dfree(B->scan_ctl);
   delete B;
   PF_API_END
 }

@@ -145,6 +145,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_shape_match(const SceneDev* __rest
   const uint2 bt = bodies[blockIdx.x];
   const SceneDev& S = scenes[bt.x];
   const SoftDev& B = S.soft;
+  // pull 0: refresh the body frames of local-matching scenes after the
+  // substep (their solve does not fit body frames).
+  if (!pull && !B.local) return;
   const uint32_t b = bt.y;
   const uint32_t start = B.body_start[b];
   const uint32_t count = B.body_count[b];
@@ -184,6 +187,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_shape_match(const SceneDev* __rest
   }
   __syncthreads();
   if (!pull) return;
+  if (B.local) return;  // local shape matching pulls through its own kernels
   const float at = B.body_compliance[b] / (S.dt * S.dt);
   const float R00 = sR[0], R01 = sR[1], R02 = sR[2], R10 = sR[3], R11 = sR[4], R12 = sR[5],
               R20 = sR[6], R21 = sR[7], R22 = sR[8];
@@ -206,6 +210,268 @@ __global__ __launch_bounds__(PF_BLOCK) void k_shape_match(const SceneDev* __rest
     B.lambda[id] = make_float4(lo.x + dlx, lo.y + dly, lo.z + dlz, 0.0f);
     xs[slot] = make_float4(x.x + dlx * w, x.y + dly * w, x.z + dlz * w, 0.0f);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Local shape matching with oriented particles (shapeMatchImplicit.ts,
+// rotation.ts): every particle's group is itself plus its edge neighbours.
+// Kernels run over particle ids of the soft-body span; positions are read
+// and written through slot_of.
+// ---------------------------------------------------------------------------
+
+// One (scene, first id) tile of the span; false past its end or when the
+// scene does not use local matching.
+static __device__ __forceinline__ bool local_id(const SceneDev* scenes, const uint2* tiles,
+                                                const SceneDev** Sp, uint32_t* id) {
+  const uint2 t = tiles[blockIdx.x];
+  const SceneDev& S = scenes[t.x];
+  *Sp = &S;
+  *id = t.y + threadIdx.x;
+  return S.soft.local && *id < S.soft.span_hi;
+}
+
+// Orientation predict (rotation.ts:38-55): qp = rotate(omega, dt) * q.
+__global__ __launch_bounds__(PF_BLOCK) void k_orient_predict(const SceneDev* __restrict__ scenes,
+                                                             const uint2* __restrict__ tiles) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  const float4 q = B.q[i];
+  const float3 om = pf_xyz(B.omega[i]);
+  const float mag = pf_len3(om);
+  if (mag < 1e-8f) {
+    B.qp[i] = q;
+    return;
+  }
+  const float half = (mag * S.dt) * 0.5f;
+  const float sr = sinf(half) / mag;
+  const float3 sp = make_float3(om.x * sr, om.y * sr, om.z * sr);
+  const float sw = cosf(half);
+  const float3 qv = pf_xyz(q);
+  const float3 nv = pf_add3(pf_add3(pf_mul3(qv, sw), pf_mul3(sp, q.w)), pf_cross3(sp, qv));
+  B.qp[i] = make_float4(nv.x, nv.y, nv.z, sw * q.w - pf_dot3(sp, qv));
+}
+
+// Step 1 (shapeMatchImplicit.ts:47-71): group centre c_i, the mean of the
+// members' predicted positions.
+__global__ __launch_bounds__(PF_BLOCK) void k_local_center(const SceneDev* __restrict__ scenes,
+                                                           const uint2* __restrict__ tiles, int c) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  const uint32_t o0 = B.csr_off[i], o1 = B.csr_off[i + 1];
+  float3 sum = make_float3(0.f, 0.f, 0.f);
+  for (uint32_t k = o0; k < o1; ++k) sum = pf_add3(sum, pf_xyz(S.xs[c][S.slot_of[B.csr_idx[k]]]));
+  const uint32_t cnt = o1 - o0;
+  const float inv = 1.0f / (float)(cnt > 1u ? cnt : 1u);
+  B.group_center[i] = cnt == 0u ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                : make_float4(sum.x * inv, sum.y * inv, sum.z * inv, 0.0f);
+}
+
+// Step 2 (shapeMatchImplicit.ts:100-217): A_i = (r^2/5) sum_j R(qp_j)
+// + sum_j (x*_j - c_i)(rest_j - rest centre_i)^T, R_i = polar(A_i).
+__global__ __launch_bounds__(PF_BLOCK) void k_local_rotation(const SceneDev* __restrict__ scenes,
+                                                             const uint2* __restrict__ tiles, int c) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  const uint32_t o0 = B.csr_off[i], o1 = B.csr_off[i + 1];
+  float a[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  const float3 ci = pf_xyz(B.group_center[i]);
+  const float3 cb = pf_xyz(B.rest_center[i]);
+  const float ai = B.ai;
+  for (uint32_t k = o0; k < o1; ++k) {
+    const uint32_t j = B.csr_idx[k];
+    const float4 qj = B.qp[j];
+    const float xx = qj.x * qj.x, yy = qj.y * qj.y, zz = qj.z * qj.z;
+    const float xy = qj.x * qj.y, xz = qj.x * qj.z, yz = qj.y * qj.z;
+    const float wx = qj.w * qj.x, wy = qj.w * qj.y, wz = qj.w * qj.z;
+    const float rj[3][3] = {{1.0f - (yy + zz) * 2.0f, (xy - wz) * 2.0f, (xz + wy) * 2.0f},
+                            {(xy + wz) * 2.0f, 1.0f - (xx + zz) * 2.0f, (yz - wx) * 2.0f},
+                            {(xz - wy) * 2.0f, (yz + wx) * 2.0f, 1.0f - (xx + yy) * 2.0f}};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) a[r][cc] = a[r][cc] + ai * rj[r][cc];
+    const float3 xj = pf_sub3(pf_xyz(S.xs[c][S.slot_of[j]]), ci);
+    const float3 tj = pf_sub3(pf_xyz(B.rest[j]), cb);
+    const float xv[3] = {xj.x, xj.y, xj.z}, tv[3] = {tj.x, tj.y, tj.z};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) a[r][cc] = a[r][cc] + xv[r] * tv[cc];
+  }
+  Mat3 A;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) A.m[r][cc] = a[r][cc];
+  Mat3 R = polar_decomposition(A);
+  if (o1 == o0) {
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc) R.m[r][cc] = r == cc ? 1.0f : 0.0f;
+  }
+  for (int r = 0; r < 3; ++r) B.group_rot[3 * (size_t)i + r] = make_float4(R.m[r][0], R.m[r][1], R.m[r][2], 0.0f);
+}
+
+// Step 3 (shapeMatchImplicit.ts:248-308): pull every member toward
+// R_i (rest_j - rest centre_i) + c_i with its own multiplier, the correction
+// shared among the member's groups and clamped to 5 r, into the accumulator.
+__global__ __launch_bounds__(PF_BLOCK) void k_local_scatter(const SceneDev* __restrict__ scenes,
+                                                            const uint2* __restrict__ tiles, int p, int c,
+                                                            int first_iteration) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  const uint32_t o0 = B.csr_off[i], o1 = B.csr_off[i + 1];
+  const float3 R0 = pf_xyz(B.group_rot[3 * (size_t)i]);
+  const float3 R1 = pf_xyz(B.group_rot[3 * (size_t)i + 1]);
+  const float3 R2 = pf_xyz(B.group_rot[3 * (size_t)i + 2]);
+  const float3 ci = pf_xyz(B.group_center[i]);
+  const float3 cb = pf_xyz(B.rest_center[i]);
+  const float at = B.rest[i].w / (S.dt * S.dt);
+  for (uint32_t k = o0; k < o1; ++k) {
+    const uint32_t j = B.csr_idx[k];
+    const uint32_t slot = S.slot_of[j];
+    const float3 rr = pf_sub3(pf_xyz(B.rest[j]), cb);
+    const float3 goal = pf_add3(make_float3(pf_dot3(R0, rr), pf_dot3(R1, rr), pf_dot3(R2, rr)), ci);
+    const float3 C = pf_sub3(pf_xyz(S.xs[c][slot]), goal);
+    const float wj = S.w[p][slot];
+    const float den = wj + at;
+    const float4 lo = first_iteration ? make_float4(0.f, 0.f, 0.f, 0.f) : B.pair_lambda[k];
+    const float3 dl = make_float3(((-C.x) - lo.x * at) / den, ((-C.y) - lo.y * at) / den, ((-C.z) - lo.z * at) / den);
+    B.pair_lambda[k] = make_float4(lo.x + dl.x, lo.y + dl.y, lo.z + dl.z, 0.0f);
+    const uint32_t mcnt = B.csr_off[j + 1] - B.csr_off[j];
+    const float mj = fmaxf((float)mcnt, 1.0f);
+    const float3 dx = make_float3((dl.x * wj) / mj, (dl.y * wj) / mj, (dl.z * wj) / mj);
+    const float sc = sqrtf(B.maxc2 / fmaxf(pf_dot3(dx, dx), B.maxc2));
+    atomicAdd(B.acc + 3 * (size_t)j + 0, pf_ticks(dx.x * sc, S.pos_scale));
+    atomicAdd(B.acc + 3 * (size_t)j + 1, pf_ticks(dx.y * sc, S.pos_scale));
+    atomicAdd(B.acc + 3 * (size_t)j + 2, pf_ticks(dx.z * sc, S.pos_scale));
+  }
+}
+
+// The pulls' accumulator apply over the span (accumulator.ts:101-131), then
+// step 4 (shapeMatchImplicit.ts:325-402): R_i as the predicted orientation,
+// on the shorter arc from the substep-start orientation.
+__global__ __launch_bounds__(PF_BLOCK) void k_local_apply(const SceneDev* __restrict__ scenes,
+                                                          const uint2* __restrict__ tiles, int c) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  int* acc = B.acc + 3 * (size_t)i;
+  const int tk[3] = {acc[0], acc[1], acc[2]};
+  const float tx = (float)tk[0], ty = (float)tk[1], tz = (float)tk[2];
+  const uint32_t slot = S.slot_of[i];
+  const float4 x = S.xs[c][slot];
+  S.xs[c][slot] = make_float4(x.x + tx * S.pos_inv, x.y + ty * S.pos_inv, x.z + tz * S.pos_inv, x.w);
+  acc[0] = 0;
+  acc[1] = 0;
+  acc[2] = 0;
+}
+
+__global__ __launch_bounds__(PF_BLOCK) void k_local_orient(const SceneDev* __restrict__ scenes,
+                                                           const uint2* __restrict__ tiles) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SoftDev& B = Sp->soft;
+  if (!(B.csr_off[i + 1] > B.csr_off[i])) return;
+  const float4 r0 = B.group_rot[3 * (size_t)i], r1 = B.group_rot[3 * (size_t)i + 1], r2 = B.group_rot[3 * (size_t)i + 2];
+  const float m00 = r0.x, m01 = r0.y, m02 = r0.z, m10 = r1.x, m11 = r1.y, m12 = r1.z, m20 = r2.x, m21 = r2.y, m22 = r2.z;
+  const float trace = (m00 + m11) + m22;
+  float qx, qy, qz, qw;
+  if (trace > 0.0f) {
+    const float s = 0.5f / sqrtf(trace + 1.0f);
+    qw = 0.25f / s;
+    qx = (m21 - m12) * s;
+    qy = (m02 - m20) * s;
+    qz = (m10 - m01) * s;
+  } else if (m00 > m11 && m00 > m22) {
+    const float s = 2.0f * sqrtf(((1.0f + m00) - m11) - m22);
+    qw = (m21 - m12) / s;
+    qx = 0.25f * s;
+    qy = (m01 + m10) / s;
+    qz = (m02 + m20) / s;
+  } else if (m11 > m22) {
+    const float s = 2.0f * sqrtf(((1.0f + m11) - m00) - m22);
+    qw = (m02 - m20) / s;
+    qx = (m01 + m10) / s;
+    qy = 0.25f * s;
+    qz = (m12 + m21) / s;
+  } else {
+    const float s = 2.0f * sqrtf(((1.0f + m22) - m00) - m11);
+    qw = (m10 - m01) / s;
+    qx = (m02 + m20) / s;
+    qy = (m12 + m21) / s;
+    qz = 0.25f * s;
+  }
+  const float4 qs = B.q[i];
+  const float d = ((qs.x * qx + qs.y * qy) + qs.z * qz) + qs.w * qw;
+  const float f = d < 0.0f ? -1.0f : 1.0f;
+  B.qp[i] = make_float4(qx * f, qy * f, qz * f, qw * f);
+}
+
+// Orientation advect (rotation.ts:57-82): omega from qp * q^-1 on the
+// shorter arc, capped at 100 rad/s, then q = normalised qp.
+__global__ __launch_bounds__(PF_BLOCK) void k_orient_advect(const SceneDev* __restrict__ scenes,
+                                                            const uint2* __restrict__ tiles) {
+  const SceneDev* Sp;
+  uint32_t i;
+  if (!local_id(scenes, tiles, &Sp, &i)) return;
+  const SceneDev& S = *Sp;
+  const SoftDev& B = S.soft;
+  const float4 q = B.q[i];
+  const float4 raw = B.qp[i];
+  const float l = sqrtf(fmaxf(((raw.x * raw.x + raw.y * raw.y) + raw.z * raw.z) + raw.w * raw.w, 1e-12f));
+  const float4 qp = make_float4(raw.x / l, raw.y / l, raw.z / l, raw.w / l);
+  const float3 qi = make_float3(-q.x, -q.y, -q.z);
+  const float3 pv = pf_xyz(qp);
+  const float3 rpre = pf_add3(pf_add3(pf_mul3(pv, q.w), pf_mul3(qi, qp.w)), pf_cross3(pv, qi));
+  const float wpre = qp.w * q.w - pf_dot3(pv, qi);
+  const float f = wpre < 0.0f ? -1.0f : 1.0f;
+  const float3 rv = pf_mul3(rpre, f);
+  const float rw = wpre * f;
+  const float hs = pf_len3(rv);
+  if (hs < 1e-8f) {
+    B.omega[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    const float angle = atan2f(hs, rw) * 2.0f;
+    const float3 om = pf_mul3(rv, (angle / hs) / S.dt);
+    const float sc = sqrtf(10000.0f / fmaxf(pf_dot3(om, om), 10000.0f));
+    B.omega[i] = make_float4(om.x * sc, om.y * sc, om.z * sc, 0.0f);
+  }
+  B.q[i] = qp;
+}
+
+void launch_local_predict(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_orient_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles);
+}
+
+void launch_local_solve(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles, int p, int c,
+                        int first_iteration) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_local_center, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles, c);
+  hipLaunchKernelGGL(k_local_rotation, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles, c);
+  hipLaunchKernelGGL(k_local_scatter, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles, p, c, first_iteration);
+  hipLaunchKernelGGL(k_local_apply, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles, c);
+  hipLaunchKernelGGL(k_local_orient, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles);
+}
+
+void launch_local_advect(hipStream_t st, const SceneDev* sc, const uint2* tiles, uint32_t ntiles) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_orient_advect, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tiles);
 }
 
 void launch_shape_match(hipStream_t st, const SceneDev* sc, const uint2* bodies, uint32_t nbodies,

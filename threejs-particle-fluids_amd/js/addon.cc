@@ -528,9 +528,13 @@ napi_value SoftbodyCreate(napi_env env, napi_callback_info info) {
     defs[b].surface_count = is_nullish(env, sc) ? 0xffffffffu : u32(env, sc);
     defs[b].compliance = num(env, prop(env, o, "compliance"), 0);
     defs[b].rest_positions = typed<float>(env, prop(env, o, "restPositions"));
+    size_t ne = 0;
+    defs[b].edges = typed<uint32_t>(env, prop(env, o, "edges"), &ne);
+    defs[b].edge_count = (uint32_t)(ne / 2);
   }
+  const uint32_t mode = u32(env, a.argv[2], PF_SHAPE_MATCHING_GLOBAL);
   pf_softbody* sb = nullptr;
-  PF_CALL(pf_softbody_create((pf_particles*)handle(env, a.argv[0]), defs.data(), nb, &sb));
+  PF_CALL(pf_softbody_create((pf_particles*)handle(env, a.argv[0]), defs.data(), nb, mode, &sb));
   return make_handle(env, sb);
 }
 napi_value SoftbodySetCompliance(napi_env env, napi_callback_info info) {

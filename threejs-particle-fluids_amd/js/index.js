@@ -208,10 +208,12 @@ class PrimitiveSet {
 
 class SoftbodySystem {
   constructor(particles, options) {
-    if ((def(options.shapeMatching, 'global')) !== 'global') {
-      throw new Error('SoftbodySystem: local shape matching is not available in this build');
+    const mode = def(options.shapeMatching, 'global');
+    if (mode !== 'global' && mode !== 'local') {
+      throw new Error("SoftbodySystem: shapeMatching must be 'global' or 'local'");
     }
     this.particles = particles;
+    this.shapeMatching = mode;
     this.bodies = options.bodies;
     this.handle = native.softbodyCreate(
       particles.handle,
@@ -221,7 +223,9 @@ class SoftbodySystem {
         surfaceCount: b.surfaceCount,
         compliance: def(b.compliance, 0),
         restPositions: b.restPositions,
+        edges: b.edges ? Uint32Array.from(b.edges) : undefined,
       })),
+      mode === 'local' ? 1 : 0,
     );
   }
   setCompliance(index, compliance) {

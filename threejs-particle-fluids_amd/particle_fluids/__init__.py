@@ -441,16 +441,18 @@ class SDFCollider:
 
 
 class SoftbodySystem:
-    """src/softbody/SoftbodySystem.ts:105-359, global shape matching.
+    """src/softbody/SoftbodySystem.ts:105-359.
 
     bodies: list of dicts with ``range`` (start, count), optional
-    ``compliance``, ``surfaceCount`` and ``restPositions`` ((count, 3)).
+    ``compliance``, ``surfaceCount``, ``restPositions`` ((count, 3)) and
+    ``edges`` (local index pairs, required for ``shapeMatching="local"``).
     """
 
     def __init__(self, particles, bodies, shapeMatching="global"):
-        if shapeMatching != "global":
-            raise PFError(1, "SoftbodySystem: local shape matching is not available in this build")
+        if shapeMatching not in ("global", "local"):
+            raise PFError(1, "SoftbodySystem: shapeMatching must be 'global' or 'local'")
         self.particles = particles
+        self.shapeMatching = shapeMatching
         defs = (_lib.SoftbodyDef * len(bodies))()
         self._keep = []
         for k, b in enumerate(bodies):
@@ -464,8 +466,16 @@ class SoftbodySystem:
                 arr = _f32(rp).reshape(-1)
                 self._keep.append(arr)
                 defs[k].rest_positions = _ptr(arr, f32p)
+            ed = b.get("edges")
+            if ed is not None:
+                e = np.ascontiguousarray(ed, np.uint32).reshape(-1)
+                if e.size % 2:
+                    raise PFError(1, f"SoftbodySystem: body {k} has an odd number of edge indices")
+                self._keep.append(e)
+                defs[k].edges = _ptr(e, u32p)
+                defs[k].edge_count = e.size // 2
         h = C.c_void_p()
-        call("pf_softbody_create", particles.handle, defs, len(bodies), C.byref(h))
+        call("pf_softbody_create", particles.handle, defs, len(bodies), 1 if shapeMatching == "local" else 0, C.byref(h))
         self.handle = h
         self.nbodies = len(bodies)
 

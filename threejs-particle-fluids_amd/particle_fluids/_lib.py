@@ -61,6 +61,8 @@ class SoftbodyDef(C.Structure):
         ("surface_count", u32),
         ("compliance", f64),
         ("rest_positions", P(f32)),
+        ("edges", P(u32)),
+        ("edge_count", u32),
     ]
 
 
@@ -151,7 +153,7 @@ SIGNATURES = {
     "pf_primitives_set_pose": [vp, u32, P(f64), P(f64), f64],
     "pf_primitives_count": [vp, P(u32)],
     "pf_primitives_destroy": [vp],
-    "pf_softbody_create": [vp, P(SoftbodyDef), u32, P(vp)],
+    "pf_softbody_create": [vp, P(SoftbodyDef), u32, u32, P(vp)],
     "pf_softbody_set_compliance": [vp, u32, f64],
     "pf_softbody_readback_frames": [vp, P(f32), P(f32)],
     "pf_softbody_destroy": [vp],
