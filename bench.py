@@ -46,6 +46,74 @@ SUBSTEP_BYTES_PER_PARTICLE = lambda iters: 336 + 48 * iters  # noqa: E731
 SCAN_BYTES_PER_BUCKET = 12
 
 
+def run_batched(args, ctx):
+    """256 independent scenes, particle budgets log-uniform in [5,000, 200,000];
+    every other scene adds 20 soft cubes of 2,197 particles (global shape
+    matching, stiffness uniform in [0.1, 1.0]). Times the same scenes stepped
+    one loop per launch (each loop's own HIP graph) and as one batch (one HIP
+    graph over every scene), both with inputs resident in HBM."""
+    import particle_fluids as pf
+    from particle_fluids import scenes
+
+    dt = 1.0 / 60.0
+    counts = scenes.log_uniform_counts(args.batch_scenes, 5000, 200000, args.batch_seed)
+    t0 = time.perf_counter()
+    built = []
+    for i, c in enumerate(counts):
+        built.append(scenes.build_mixed_scene(int(c), 20 if i % 2 else 0, seed=args.batch_seed + i,
+                                              substeps=args.substeps, iterations=args.iterations, use_graph=True))
+    loops = [b["loop"] for b in built]
+    n_total = sum(b["ps"].capacity for b in built)
+    n_fluid = sum(b["layout"]["fluid_count"] for b in built)
+    print(f"[bench] batched: {len(built)} scenes, {n_total} particles ({n_fluid} fluid), built in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    batch = pf.Batch(loops)
+    batch.useGraph(True)
+    for _ in range(args.batch_warmup):
+        for L in loops:
+            L.step(dt)
+        batch.step(dt)
+    ctx.synchronize()
+
+    def timed(fn):
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.batch_steps):
+            fn()
+        ctx.synchronize()
+        return time.perf_counter() - t
+
+    def plain_step():
+        for L in loops:
+            L.step(dt)
+
+    t_plain = timed(plain_step)
+    t_batch = timed(lambda: batch.step(dt))
+    t_plain2 = timed(plain_step)
+    t_plain = 0.5 * (t_plain + t_plain2)
+    work = n_total * args.substeps * args.batch_steps
+    out = {
+        "workload": f"{len(built)} scenes, budgets log-uniform in [5000, 200000] (seed {args.batch_seed}), "
+                    f"every other scene + 20 soft cubes of 2197 particles, {args.substeps} substeps x "
+                    f"{args.iterations} iterations, dt 1/60",
+        "scenes": len(built), "particles": n_total, "fluid_particles": n_fluid,
+        "steps": args.batch_steps,
+        "plain": {"particle_substeps_per_s": round(work / t_plain, 1),
+                  "scene_steps_per_s": round(len(built) * args.batch_steps / t_plain, 2),
+                  "ms_per_round": round(t_plain / args.batch_steps * 1e3, 3)},
+        "batched": {"particle_substeps_per_s": round(work / t_batch, 1),
+                    "scene_steps_per_s": round(len(built) * args.batch_steps / t_batch, 2),
+                    "ms_per_step": round(t_batch / args.batch_steps * 1e3, 3)},
+        "speedup": round(t_plain / t_batch, 3),
+    }
+    print(f"[bench] batched: plain {out['plain']}, batched {out['batched']}, x{out['speedup']}",
+          file=sys.stderr, flush=True)
+    batch.dispose()
+    for b in built:
+        scenes.dispose_scene(b)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +126,10 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-baseline-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--batch-seed", type=int, default=0xBA7C4)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,6 +217,10 @@ def main():
                "sample": f"{args.cpu_baseline_steps} step(s) = {args.substeps * args.cpu_baseline_steps} substeps of the "
                          f"same {n}-particle scene, FP32 C oracle, 1 thread, {csec:.1f} s on {platform.processor() or 'host'}"}
 
+    batched = None
+    if args.batch_scenes > 0:
+        batched = run_batched(args, ctx)
+
     line = {
         "metric": "particle-substeps/s at 1M PBF",
         "value": round(value, 1),
@@ -167,6 +243,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "gpu_ms_timed": round(gpu_ms, 3),
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -74,6 +74,16 @@ typedef struct pf_snapshot {
 /* new ParticleSystem(renderer, capacity, particleRadius) (particles.ts:81-113) */
 int pf_particles_create(pf_context* ctx, uint32_t capacity, double particle_radius,
                         pf_particles** out);
+/* Like pf_particles_create, but every device buffer of the scene built on
+ * these particles (grid, materials, colliders, loop) lives in one arena of
+ * reserved device addresses, so a pf_server can move the scene to host memory
+ * and back without changing any device address. No counterpart in the
+ * reference (its buffers are WebGPU storage buffers). */
+int pf_particles_create_evictable(pf_context* ctx, uint32_t capacity, double particle_radius,
+                                  pf_particles** out);
+/* Device bytes the scene's arena holds while resident (0 for particles not
+ * created evictable). */
+int pf_particles_device_bytes(pf_particles* ps, uint64_t* bytes);
 /* dispose() (particles.ts:234-257); idempotent. */
 int pf_particles_destroy(pf_particles* ps);
 /* uploadParticles(data, start) (particles.ts:121-160): xyz triples; velocity
@@ -286,6 +296,63 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t nloops, pf_
 int pf_batch_step(pf_batch* batch, double dt);
 int pf_batch_set_graph(pf_batch* batch, int enabled);
 int pf_batch_destroy(pf_batch* batch);
+
+/* ---- Served form: a request loop over live scenes. No counterpart in the
+ * reference. Step requests for scenes added to the server are queued as they
+ * arrive; a worker thread launches every queued request whose scene is not
+ * already on the GPU as one multi-scene launch (continuous batching) on one of
+ * `slots` streams, so launches overlap on the device. Scenes stay resident in
+ * HBM; admission is capped by `hbm_budget_bytes` of scene arenas: before a
+ * launch, idle scenes are evicted to host memory, least recently stepped
+ * first, to make room for the evicted scenes it needs back. A scene's requests
+ * run in submission order, one step each. While a scene is served, the
+ * ordinary calls on it (step, readback, upload, new materials) fail with
+ * PF_ESTATE; pf_server_remove_scene hands it back, resident. */
+typedef struct pf_server_s pf_server;
+typedef struct pf_server_options {
+  uint64_t hbm_budget_bytes;     /* resident scene bytes; 0 = no cap */
+  uint32_t max_batch_scenes;     /* scenes per launch; 0 = 256 */
+  uint64_t max_batch_particles;  /* particles per launch; 0 = no cap */
+  uint32_t slots;                /* launches in flight at once; 0 = 2 */
+} pf_server_options;
+typedef struct pf_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t launches;             /* multi-scene launches */
+  uint64_t scene_steps;          /* completed requests' steps */
+  uint64_t particle_substeps;    /* sum over completed requests of particles x substeps */
+  uint64_t evictions, restores;
+  uint64_t resident_bytes, peak_resident_bytes, hbm_budget_bytes;
+  uint32_t scenes, resident_scenes;
+  uint32_t max_launch_scenes;    /* largest launch so far */
+} pf_server_stats;
+/* Times of one request in seconds on the server's clock: submitted,
+ * launched, finished (0 while not reached). status: 0 pending, 1 launched,
+ * 2 done, 3 failed. */
+typedef struct pf_request_info {
+  uint32_t scene, status, launch_scenes, pad;
+  double t_submit, t_launch, t_done;
+} pf_request_info;
+int pf_server_create(pf_context* ctx, const pf_server_options* opts, pf_server** out);
+/* Takes a loop built on evictable particles; every served loop shares the
+ * first one's substeps and iterations. Admission may evict idle scenes. */
+int pf_server_add_scene(pf_server* srv, pf_loop* loop, uint32_t* scene_id);
+/* Waits for the scene's queued requests, restores it if evicted and hands
+ * it back to its owner. */
+int pf_server_remove_scene(pf_server* srv, uint32_t scene_id);
+/* Queues one step(dt) of a scene and returns at once; thread-safe. */
+int pf_server_submit(pf_server* srv, uint32_t scene_id, double dt, uint64_t* ticket);
+/* Blocks until the request finished (or failed). */
+int pf_server_wait(pf_server* srv, uint64_t ticket, pf_request_info* info);
+/* Blocks until every submitted request finished. */
+int pf_server_drain(pf_server* srv);
+int pf_server_request_info(pf_server* srv, uint64_t first_ticket, uint32_t count, pf_request_info* out);
+int pf_server_stats_get(pf_server* srv, pf_server_stats* out);
+/* Current server clock, seconds (the clock of pf_request_info). */
+double pf_server_now(pf_server* srv);
+/* Moves an idle scene to host memory now (also done by admission). */
+int pf_server_evict_scene(pf_server* srv, uint32_t scene_id);
+/* Drains, stops the worker and hands every scene back, resident. */
+int pf_server_destroy(pf_server* srv);
 
 /* ---- Scene construction (src/simulation/layout.ts:7-42, Simulation.ts:957-1104) */
 /* fillBox(box, spacing): lattice points, y -> z -> x order. out may be null to count. */

@@ -127,3 +127,40 @@ def quat_to_mat(q):
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def oracle_mixed_scene(layout, substeps, iterations, H, precision="f32"):
+    """A scene of particle_fluids.scenes.mixed_layout (dam break plus globally
+    shape-matched cubes in the extended tank) in the CPU oracle."""
+    pos, nf, r = layout["positions"], layout["fluid_count"], layout["radius"]
+    n = len(pos)
+    spacing = 2 * r
+    h = 2 * spacing
+    fx = scenes.FLUID_DEFAULTS
+    s = orc.Scene(n, r, precision)
+    w = layout["inv_mass"]
+    s.set_particles(pos, None, w)
+    s.set_loop(substeps, iterations, (0.0, -9.81, 0.0), h, H)
+    s.add_fluid(0, nf, 1000.0, spacing, h, 1e-4, viscosity=fx["viscosity"], vorticity=fx["vorticity"],
+                surface_tension=fx["surfaceTension"], adhesion=fx["adhesion"])
+    for b in layout["bodies"]:
+        st, c = b["range"]
+        inv = np.asarray(w[st:st + c], np.float64)
+        rest = pos[st:st + c].astype(np.float64)
+        # massWeights (SoftbodySystem.ts:366-372): 1 for equal masses, else m / mean(m)
+        m = np.where(inv > 0, 1.0 / np.where(inv > 0, inv, 1.0), 0.0)
+        dyn = m[m > 0]
+        if np.all(dyn == dyn[0]):
+            wts = np.ones(c, np.float32)
+        else:
+            wts = np.where(m > 0, m / dyn.mean(), 1.0).astype(np.float32)
+        # weighted rest centre summed in order, in double (SoftbodySystem.ts:374-386)
+        ctr = np.zeros(3)
+        for p, wk in zip(rest, wts.astype(np.float64)):
+            ctr += p * wk
+        ctr /= float(wts.astype(np.float64).sum())
+        s.add_body(st, c, b["compliance"], (rest - ctr).astype(np.float32), wts)
+    mn, mx = layout["container"][:3], layout["container"][3:]
+    for nrm, which in scenes.PLANES:
+        s.add_plane(nrm, mn if which == "min" else mx)
+    return s

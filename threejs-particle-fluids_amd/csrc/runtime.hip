@@ -3,7 +3,9 @@
 // (the ABI) and cell-sorted order (the device).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
+#include <unordered_set>
 
 #include "runtime.hpp"
 
@@ -31,8 +33,159 @@ void fail(int code, const char* fmt, ...) {
   throw PfError{code};
 }
 
+// ---------------------------------------------------------------------------
+// Device allocation: hipMalloc, or a scene arena inside an ArenaScope.
+// ---------------------------------------------------------------------------
+static std::mutex g_alloc_mu;
+static std::unordered_set<void*> g_malloced;  // buffers dfree may hand to hipFree
+static thread_local Arena* g_arena = nullptr;
+
+static constexpr size_t kArenaMinChunk = 4u << 20;
+static constexpr size_t kArenaAlign = 256;
+// Mapped tail no buffer uses: reads a little past a buffer's end stay inside
+// mapped memory, as they do with hipMalloc's rounded allocations.
+static constexpr size_t kArenaTail = 64u << 10;
+
+static hipMemAllocationProp arena_prop(int device) {
+  hipMemAllocationProp prop{};
+  prop.type = hipMemAllocationTypePinned;
+  prop.location.type = hipMemLocationTypeDevice;
+  prop.location.id = device;
+  return prop;
+}
+
+static void arena_map(Arena::Chunk& c, int device) {
+  const hipMemAllocationProp prop = arena_prop(device);
+  hipError_t e = hipMemCreate(&c.handle, c.size, &prop, 0);
+  if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", c.size, hipGetErrorString(e));
+  e = hipMemMap(c.va, c.size, 0, c.handle, 0);
+  if (e != hipSuccess) {
+    (void)hipMemRelease(c.handle);
+    PF_HIP(e);
+  }
+  hipMemAccessDesc acc{};
+  acc.location = prop.location;
+  acc.flags = hipMemAccessFlagsProtReadWrite;
+  e = hipMemSetAccess(c.va, c.size, &acc, 1);
+  if (e != hipSuccess) {
+    (void)hipMemUnmap(c.va, c.size);
+    (void)hipMemRelease(c.handle);
+    PF_HIP(e);
+  }
+}
+
+Arena::Arena(int dev) : device(dev) {
+  int vmm = 0;
+  PF_HIP(hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, dev));
+  if (!vmm) fail(PF_EDEVICE, "ParticleSystem: device %d has no virtual memory management; evictable scenes need it", dev);
+  const hipMemAllocationProp prop = arena_prop(dev);
+  PF_HIP(hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum));
+  if (gran == 0) gran = 4096;
+}
+
+Arena::~Arena() {
+  for (auto& c : chunks) {
+    if (resident) {
+      (void)hipMemUnmap(c.va, c.size);
+      (void)hipMemRelease(c.handle);
+    }
+    if (c.host) (void)hipHostFree(c.host);
+    (void)hipMemAddressFree(c.va, c.size);
+  }
+}
+
+void* Arena::alloc(size_t bytes) {
+  if (!resident) fail(PF_ESTATE, "ParticleSystem: the scene is evicted to host memory");
+  bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+  if (!chunks.empty() && chunks.back().size - chunks.back().used >= bytes + kArenaTail) {
+    Chunk& c = chunks.back();
+    void* p = c.va + c.used;
+    c.used += bytes;
+    return p;
+  }
+  Chunk c;
+  c.size = (std::max(bytes + kArenaTail, kArenaMinChunk) + gran - 1) / gran * gran;
+  void* va = nullptr;
+  hipError_t e = hipMemAddressReserve(&va, c.size, 0, nullptr, 0);
+  if (e != hipSuccess) fail(PF_ENOMEM, "device address reservation of %zu bytes failed: %s", c.size, hipGetErrorString(e));
+  c.va = static_cast<char*>(va);
+  try {
+    arena_map(c, device);
+  } catch (...) {
+    (void)hipMemAddressFree(c.va, c.size);
+    throw;
+  }
+  c.used = bytes;
+  chunks.push_back(c);
+  return c.va;
+}
+
+size_t Arena::bytes() const {
+  size_t b = 0;
+  for (const auto& c : chunks) b += c.size;
+  return b;
+}
+
+void Arena::evict() {
+  if (!resident) return;
+  for (auto& c : chunks) {
+    if (!c.host) PF_HIP(hipHostMalloc(&c.host, std::max<size_t>(c.used, 1), hipHostMallocDefault));
+    PF_HIP(hipMemcpy(c.host, c.va, c.used, hipMemcpyDeviceToHost));
+  }
+  for (auto& c : chunks) {
+    PF_HIP(hipMemUnmap(c.va, c.size));
+    PF_HIP(hipMemRelease(c.handle));
+  }
+  resident = false;
+}
+
+void Arena::restore() {
+  if (resident) return;
+  size_t mapped = 0;
+  try {
+    for (; mapped < chunks.size(); ++mapped) arena_map(chunks[mapped], device);
+  } catch (...) {
+    for (size_t k = 0; k < mapped; ++k) {
+      (void)hipMemUnmap(chunks[k].va, chunks[k].size);
+      (void)hipMemRelease(chunks[k].handle);
+    }
+    throw;
+  }
+  resident = true;
+  for (auto& c : chunks) {
+    PF_HIP(hipMemcpy(c.va, c.host, c.used, hipMemcpyHostToDevice));
+    PF_HIP(hipHostFree(c.host));
+    c.host = nullptr;
+  }
+}
+
+ArenaScope::ArenaScope(Arena* arena) : prev(g_arena) { g_arena = arena; }
+ArenaScope::~ArenaScope() { g_arena = prev; }
+Arena* current_arena() { return g_arena; }
+
+void* dalloc_bytes(size_t bytes, bool temporary) {
+  void* p = nullptr;
+  if (g_arena && !temporary) {
+    p = g_arena->alloc(bytes);
+  } else {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_malloced.insert(p);
+  }
+  PF_HIP(hipMemset(p, 0, bytes));
+  // Library streams are non-blocking: finish the clear before any of them runs.
+  PF_HIP(hipDeviceSynchronize());
+  return p;
+}
+
 void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (g_malloced.erase(p) == 0) return;  // arena buffer
+  }
+  (void)hipFree(p);
 }
 
 hipEvent_t Context::get_event() {
@@ -338,14 +491,15 @@ int pf_context_profile_reset(pf_context* ctx) {
 // ---------------------------------------------------------------------------
 // ParticleSystem
 // ---------------------------------------------------------------------------
-int pf_particles_create(pf_context* ctx, uint32_t capacity, double radius, pf_particles** out) {
-  PF_API_BEGIN
+static pf_particles* particles_new(pf_context* ctx, uint32_t capacity, double radius, bool evictable) {
   PF_REQUIRE(ctx, "ParticleSystem");
-  PF_REQUIRE(out, "ParticleSystem");
   if (capacity == 0) fail(PF_EINVAL, "ParticleSystem: capacity must be a positive integer, got %u", capacity);
   if (!(radius > 0) || !std::isfinite(radius))
     fail(PF_EINVAL, "ParticleSystem: particleRadius must be a positive finite number, got %g", radius);
-  auto* ps = new pf_particles_s();
+  std::unique_ptr<pf_particles_s> owner(new pf_particles_s());
+  pf_particles_s* ps = owner.get();
+  if (evictable) ps->arena.reset(new Arena(ctx->device));
+  ArenaScope scope(ps->arena.get());
   ps->ctx = ctx;
   ps->n = capacity;
   ps->radius = radius;
@@ -372,7 +526,28 @@ int pf_particles_create(pf_context* ctx, uint32_t capacity, double radius, pf_pa
   ps->cpu_groups.assign(capacity, 0u);
   ps->dirty = true;
   ps->upload_desc();
-  *out = ps;
+  return owner.release();
+}
+
+int pf_particles_create(pf_context* ctx, uint32_t capacity, double radius, pf_particles** out) {
+  PF_API_BEGIN
+  PF_REQUIRE(out, "ParticleSystem");
+  *out = particles_new(ctx, capacity, radius, false);
+  PF_API_END
+}
+
+int pf_particles_create_evictable(pf_context* ctx, uint32_t capacity, double radius, pf_particles** out) {
+  PF_API_BEGIN
+  PF_REQUIRE(out, "ParticleSystem");
+  *out = particles_new(ctx, capacity, radius, true);
+  PF_API_END
+}
+
+int pf_particles_device_bytes(pf_particles* ps, uint64_t* bytes) {
+  PF_API_BEGIN
+  PF_REQUIRE(ps, "ParticleSystem");
+  PF_REQUIRE(bytes, "ParticleSystem");
+  *bytes = ps->arena ? (uint64_t)ps->arena->bytes() : 0;
   PF_API_END
 }
 
@@ -384,11 +559,13 @@ static void particles_free(pf_particles* ps) {
   }
   dfree(S.mass); dfree(S.slot_of); dfree(S.flags); dfree(ps->dev);
   ps->dev = nullptr;
+  ps->arena.reset();  // every other buffer of an evictable scene goes with it
 }
 
 int pf_particles_destroy(pf_particles* ps) {
   PF_API_BEGIN
   if (!ps || ps->disposed) return PF_OK;
+  if (ps->server) fail(PF_ESTATE, "ParticleSystem: remove the scene from its server before disposing it");
   (void)hipStreamSynchronize(ps->ctx->stream);
   ps->disposed = true;
   particles_free(ps);
@@ -398,6 +575,7 @@ int pf_particles_destroy(pf_particles* ps) {
 static void check_alive(pf_particles* ps, const char* owner) {
   if (!ps) fail(PF_EINVAL, "%s: null handle", owner);
   if (ps->disposed) fail(PF_ESTATE, "ParticleSystem has been disposed");
+  if (ps->server) fail(PF_ESTATE, "%s: the scene is being served; remove it from the server first", owner);
 }
 
 static void check_range(pf_particles* ps, uint32_t start, uint32_t count, const char* owner) {
@@ -416,10 +594,10 @@ void particles_scatter(pf_particles* ps, uint32_t start, uint32_t count, const s
   ps->upload_desc();
   Context* ctx = ps->ctx;
   float4* dpos = nullptr; float4* dvel = nullptr; float* df = nullptr; uint32_t* dg = nullptr;
-  if (pos) { dpos = dalloc<float4>(count); PF_HIP(hipMemcpy(dpos, pos->data(), count * sizeof(float4), hipMemcpyHostToDevice)); }
-  if (vel) { dvel = dalloc<float4>(count); PF_HIP(hipMemcpy(dvel, vel->data(), count * sizeof(float4), hipMemcpyHostToDevice)); }
-  if (f) { df = dalloc<float>(count); PF_HIP(hipMemcpy(df, f->data(), count * sizeof(float), hipMemcpyHostToDevice)); }
-  if (grp) { dg = dalloc<uint32_t>(count); PF_HIP(hipMemcpy(dg, grp->data(), count * sizeof(uint32_t), hipMemcpyHostToDevice)); }
+  if (pos) { dpos = dalloc_temp<float4>(count); PF_HIP(hipMemcpy(dpos, pos->data(), count * sizeof(float4), hipMemcpyHostToDevice)); }
+  if (vel) { dvel = dalloc_temp<float4>(count); PF_HIP(hipMemcpy(dvel, vel->data(), count * sizeof(float4), hipMemcpyHostToDevice)); }
+  if (f) { df = dalloc_temp<float>(count); PF_HIP(hipMemcpy(df, f->data(), count * sizeof(float), hipMemcpyHostToDevice)); }
+  if (grp) { dg = dalloc_temp<uint32_t>(count); PF_HIP(hipMemcpy(dg, grp->data(), count * sizeof(uint32_t), hipMemcpyHostToDevice)); }
   launch_scatter_upload(ctx->stream, ps->cur(), 0, 0, start, count, dpos, dvel, df, dg, mask);
   PF_HIP(hipGetLastError());
   PF_HIP(hipStreamSynchronize(ctx->stream));
@@ -431,7 +609,7 @@ void particles_gather(pf_particles* ps, int what, void* host, size_t elem) {
   ps->upload_desc();
   Context* ctx = ps->ctx;
   void* d = nullptr;
-  PF_HIP(hipMalloc(&d, (size_t)ps->n * elem));
+  d = dalloc_bytes((size_t)ps->n * elem, true);
   launch_gather_by_id(ctx->stream, ps->cur(), 0, 0, ps->n, what, d);
   PF_HIP(hipGetLastError());
   PF_HIP(hipMemcpyAsync(host, d, (size_t)ps->n * elem, hipMemcpyDeviceToHost, ctx->stream));
@@ -615,6 +793,7 @@ int pf_grid_create(pf_particles* ps, double cell_size, uint32_t hash_table_size,
   PF_API_BEGIN
   check_alive(ps, "HashGrid");
   PF_REQUIRE(out, "HashGrid");
+  ArenaScope arena_scope(ps->arena.get());
   auto* g = new pf_grid_s();
   try {
     grid_init(g, ps->ctx, ps->n, cell_size, hash_table_size, origin, "HashGrid");
@@ -727,7 +906,7 @@ int pf_grid_readback(pf_grid* g, uint32_t* cell_index, uint32_t* counts, uint32_
     if (cell_index) particles_gather(ps, G_KEY, cell_index, sizeof(uint32_t));
     if (sorted_indices) {
       void* d = nullptr;
-      PF_HIP(hipMalloc(&d, (size_t)g->n * 4));
+      d = dalloc_bytes((size_t)g->n * 4, true);
       launch_gather_by_id(st, ps->cur(), 0, 0, g->n, G_IDS, d);
       PF_HIP(hipMemcpyAsync(sorted_indices, d, (size_t)g->n * 4, hipMemcpyDeviceToHost, st));
       PF_HIP(hipStreamSynchronize(st));
@@ -812,9 +991,9 @@ int pf_grid_query_counts(pf_grid* g, const float* query4, uint32_t nq, double ra
   grid_alive(g, "HashGrid.query");
   if (g->owned_by_loop || !g->sorted_positions) fail(PF_EINVAL, "HashGrid.query: needs a standalone grid with sorted positions");
   hipStream_t st = g->ctx->stream;
-  float4* dq = dalloc<float4>(nq);
-  uint32_t* dc = dalloc<uint32_t>(nq);
-  uint32_t* dw = dalloc<uint32_t>(nq);
+  float4* dq = dalloc_temp<float4>(nq);
+  uint32_t* dc = dalloc_temp<uint32_t>(nq);
+  uint32_t* dw = dalloc_temp<uint32_t>(nq);
   PF_HIP(hipMemcpy(dq, query4, (size_t)nq * sizeof(float4), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_grid_query, dim3((nq + 255) / 256), dim3(256), 0, st, g->dev, g->perm, g->spos, dq, nq,
                      (float)(radius * radius), dc, dw);

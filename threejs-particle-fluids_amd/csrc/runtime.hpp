@@ -38,17 +38,55 @@ struct PfError {
 // Throw an argument error with the reference's "<Owner>: <reason>" wording.
 [[noreturn]] void fail(int code, const char* fmt, ...);
 
+// Device memory of one evictable scene. Each chunk is a range of reserved
+// device addresses backed by one physical allocation; eviction copies the
+// chunk's used bytes to pinned host memory and releases the physical memory,
+// restore maps fresh physical memory at the same addresses and copies back.
+// Pointers into the arena (descriptors, tile tables, captured graphs) stay
+// valid across an evict/restore round trip. Buffers are bump-allocated and
+// released together with the arena.
+struct Arena {
+  struct Chunk {
+    char* va = nullptr;
+    size_t size = 0, used = 0;
+    hipMemGenericAllocationHandle_t handle{};
+    void* host = nullptr;  // pinned copy while evicted
+  };
+  int device = 0;
+  size_t gran = 0;
+  std::vector<Chunk> chunks;
+  bool resident = true;
+  explicit Arena(int device);
+  ~Arena();
+  void* alloc(size_t bytes);
+  size_t bytes() const;  // physical bytes held while resident
+  void evict();
+  void restore();
+};
+
+// While alive, dalloc places buffers in `arena` (creation calls of an
+// evictable scene); temporary buffers outside a scope use hipMalloc.
+struct ArenaScope {
+  Arena* prev;
+  explicit ArenaScope(Arena* arena);
+  ~ArenaScope();
+};
+Arena* current_arena();
+
+void* dalloc_bytes(size_t bytes, bool temporary);
+
 template <typename T>
 T* dalloc(size_t n) {
   if (n == 0) n = 1;
-  void* p = nullptr;
-  hipError_t e = hipMalloc(&p, n * sizeof(T));
-  if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
-  PF_HIP(hipMemset(p, 0, n * sizeof(T)));
-  // Library streams are non-blocking: finish the clear before any of them runs.
-  PF_HIP(hipDeviceSynchronize());
-  return static_cast<T*>(p);
+  return static_cast<T*>(dalloc_bytes(n * sizeof(T), false));
 }
+// A short-lived buffer (uploads, readbacks): always hipMalloc, never an arena.
+template <typename T>
+T* dalloc_temp(size_t n) {
+  if (n == 0) n = 1;
+  return static_cast<T*>(dalloc_bytes(n * sizeof(T), true));
+}
+// Frees a hipMalloc'd buffer; arena buffers are released with their arena.
 void dfree(void* p);
 
 struct KernelStat {
@@ -99,6 +137,8 @@ struct Particles {
   std::vector<float> cpu_positions;  // host mirror of uploaded positions (xyz)
   std::vector<uint32_t> cpu_groups;
   struct pf_loop_s* loop = nullptr;  // attached loop, if any
+  std::unique_ptr<Arena> arena;      // every device buffer of an evictable scene
+  void* server = nullptr;            // the pf_server stepping this scene, if any
   void upload_desc();
 };
 
@@ -290,6 +330,12 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sha
                       const Tile* tiles, uint32_t ntiles, const uint2* stiles, uint32_t nstiles,
                       const uint2* btiles, uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles,
                       void* scan_ctl);
-// Swap descriptor halves after a step with an odd number of state flips.
-void finish_step_host(Particles* ps, const PlanShape& shape);
+int step_variant_flips(const PlanShape& sh);
+// Multi-scene launches (pf_batch, pf_server; sim.hip).
+PlanShape loop_plan_shape(const pf_loop_s* L);
+void plan_merge(PlanShape& into, const PlanShape& scene);
+void loop_append_tiles(const pf_loop_s* L, uint32_t scene, std::vector<Tile>& tiles, std::vector<uint2>& stiles,
+                       std::vector<uint2>& btiles, std::vector<uint2>& ltiles);
+bool loop_prepare_step(pf_loop_s* L, double dt);
+void loop_finish_step(pf_loop_s* L, int flips);
 }  // namespace pf

@@ -243,6 +243,7 @@ using namespace pf;
 static void alive(pf_particles* ps, const char* owner) {
   if (!ps) fail(PF_EINVAL, "%s: null handle", owner);
   if (ps->disposed) fail(PF_ESTATE, "%s: the ParticleSystem has been disposed", owner);
+  if (ps->server) fail(PF_ESTATE, "%s: the scene is being served; remove it from the server first", owner);
 }
 
 static void range_ok(pf_particles* ps, uint32_t start, uint32_t count, const char* owner) {
@@ -643,6 +644,7 @@ int pf_sdf_collider_create(pf_particles* ps, const float* data, const uint32_t r
                            const double voxel[3], const pf_sdf_options* opt, pf_sdf_collider** out) {
   PF_API_BEGIN
   alive(ps, "SDFCollider");
+  ArenaScope arena_scope(ps->arena.get());
   if (!(data && res && origin && voxel && out)) fail(PF_EINVAL, "SDFCollider: null argument");
   for (int a = 0; a < 3; ++a)
     if (res[a] < 4) fail(PF_EINVAL, "SDFCollider: resolution must be integers ≥ 4, got [%u, %u, %u]", res[0], res[1], res[2]);
@@ -835,6 +837,7 @@ int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t n
                        pf_softbody** out) {
   PF_API_BEGIN
   alive(ps, "SoftbodySystem");
+  ArenaScope arena_scope(ps->arena.get());
   if (nb == 0 || !defs) fail(PF_EINVAL, "SoftbodySystem: at least one body is required");
   if (mode != PF_SHAPE_MATCHING_GLOBAL && mode != PF_SHAPE_MATCHING_LOCAL)
     fail(PF_EINVAL, "SoftbodySystem: shapeMatching must be 'global' or 'local'");
@@ -1149,6 +1152,7 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
                    const pf_collider* colliders, uint32_t ncol, pf_loop** out) {
   PF_API_BEGIN
   alive(ps, "SimLoop");
+  ArenaScope arena_scope(ps->arena.get());
   if (ps->loop) fail(PF_EINVAL, "SimLoop: the ParticleSystem already has a SimLoop");
   const uint32_t substeps = opt && opt->substeps ? opt->substeps : 4;
   const uint32_t iterations = opt && opt->iterations != 0xffffffffu ? opt->iterations : 2;
@@ -1341,6 +1345,7 @@ static void loop_alive(pf_loop* L, const char* what) {
   if (!L) fail(PF_EINVAL, "%s: null handle", what);
   if (L->disposed) fail(PF_ESTATE, "SimLoop: the loop has been disposed");
   if (L->ps->disposed) fail(PF_ESTATE, "SimLoop: the ParticleSystem has been disposed");
+  if (L->ps->server) fail(PF_ESTATE, "%s: the scene is being served; remove it from the server first", what);
 }
 
 // A loop steps only while every material and collider it was built with is
@@ -1388,6 +1393,57 @@ static void loop_static_boundaries(pf_loop* L) {
   S.dt = dt_keep;
   ps->dirty = true;
 }
+
+}  // extern "C"
+
+namespace pf {
+PlanShape loop_plan_shape(const pf_loop_s* L) { return loop_shape(L); }
+
+void plan_merge(PlanShape& sh, const PlanShape& ls) {
+  sh.nfluids = std::max(sh.nfluids, ls.nfluids);
+  for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
+    sh.fx[f] |= ls.fx[f];
+    sh.nb[f] = std::max(sh.nb[f], ls.nb[f]);
+    sh.bdyn[f] |= ls.bdyn[f];
+  }
+  sh.walls = sh.walls || ls.walls;
+  sh.soft = sh.soft || ls.soft;
+  sh.local_soft = sh.local_soft || ls.local_soft;
+  sh.any_boundary = sh.any_boundary || ls.any_boundary;
+}
+
+void loop_append_tiles(const pf_loop_s* L, uint32_t scene, std::vector<Tile>& tl, std::vector<uint2>& stl,
+                       std::vector<uint2>& btl, std::vector<uint2>& ltl) {
+  for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{scene, i * PF_BLOCK});
+  for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(scene, i));
+  for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(scene, i));
+  if (L->soft && L->soft->mode == PF_SHAPE_MATCHING_LOCAL) {
+    const SoftDev& SB = L->ps->host.soft;
+    const uint32_t nl = (SB.span_hi - SB.span_lo + PF_BLOCK - 1) / PF_BLOCK;
+    for (uint32_t i = 0; i < nl; ++i) ltl.push_back(make_uint2(scene, SB.span_lo + i * PF_BLOCK));
+  }
+}
+
+// Host work before a loop's step runs inside a batch or a server launch.
+// Returns true when the scene's descriptor changed.
+bool loop_prepare_step(pf_loop_s* L, double dt) {
+  loop_members_alive(L);
+  if (L->prims) L->prims->upload();
+  loop_set_dt(L, dt);
+  if (L->static_boundaries && L->steps == 0) loop_static_boundaries(L);
+  return L->ps->dirty;
+}
+
+// Host bookkeeping after a loop's step was enqueued in a batch or a server launch.
+void loop_finish_step(pf_loop_s* L, int flips) {
+  L->ps->variant ^= flips;
+  L->ps->stepped = true;
+  L->last_flag_half = (L->substeps - 1) & 1u;
+  L->steps += 1;
+}
+}  // namespace pf
+
+extern "C" {
 
 int pf_loop_step(pf_loop* L, double dt) {
   PF_API_BEGIN
@@ -1572,25 +1628,8 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
       fail(PF_EINVAL, "Batch: every loop must share substeps and iterations");
     for (uint32_t t = 0; t < s; ++t)
       if (loops[t] == L) fail(PF_EINVAL, "Batch: a loop appears twice");
-    const PlanShape ls = loop_shape(L);
-    sh.nfluids = std::max(sh.nfluids, ls.nfluids);
-    for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) sh.fx[f] |= ls.fx[f];
-    sh.walls = sh.walls || ls.walls;
-    sh.soft = sh.soft || ls.soft;
-    sh.local_soft = sh.local_soft || ls.local_soft;
-    sh.any_boundary = sh.any_boundary || ls.any_boundary;
-    for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
-      sh.nb[f] = std::max(sh.nb[f], ls.nb[f]);
-      sh.bdyn[f] |= ls.bdyn[f];
-    }
-    for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{s, i * PF_BLOCK});
-    for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(s, i));
-    for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(s, i));
-    if (ls.local_soft) {
-      const SoftDev& SB = L->ps->host.soft;
-      const uint32_t nl = (SB.span_hi - SB.span_lo + PF_BLOCK - 1) / PF_BLOCK;
-      for (uint32_t i = 0; i < nl; ++i) ltl.push_back(make_uint2(s, SB.span_lo + i * PF_BLOCK));
-    }
+    plan_merge(sh, loop_shape(L));
+    loop_append_tiles(L, s, tl, stl, btl, ltl);
     B->loops.push_back(L);
   }
   if (sh.nfluids > 0)
@@ -1629,11 +1668,7 @@ int pf_batch_step(pf_batch* B, double dt) {
   bool changed = false;
   for (auto* L : B->loops) {
     loop_alive(L, "Batch");
-    loop_members_alive(L);
-    if (L->prims) L->prims->upload();
-    loop_set_dt(L, dt);
-    if (L->static_boundaries && L->steps == 0) loop_static_boundaries(L);
-    changed = changed || L->ps->dirty;
+    changed = loop_prepare_step(L, dt) || changed;
   }
   if (changed || B->desc_dirty) {
     PF_HIP(hipStreamSynchronize(ctx->stream));
@@ -1670,12 +1705,7 @@ int pf_batch_step(pf_batch* B, double dt) {
   }
   PF_HIP(hipGetLastError());
   const int flips = step_variant_flips(B->shape);
-  for (auto* L : B->loops) {
-    L->ps->variant ^= flips;
-    L->ps->stepped = true;
-    L->last_flag_half = (L->substeps - 1) & 1u;
-    L->steps += 1;
-  }
+  for (auto* L : B->loops) loop_finish_step(L, flips);
   // The graph reads the descriptor array at run time; swapped buffer pairs
   // only need the array re-uploaded before the next step.
   if (flips) B->desc_dirty = true;

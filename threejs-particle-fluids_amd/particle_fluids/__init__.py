@@ -118,11 +118,13 @@ HashGridSnapshot = namedtuple(
 class ParticleSystem:
     """src/core/particles.ts:52-258. Positions are (N, 3) arrays."""
 
-    def __init__(self, capacity, particleRadius, context=None):
+    def __init__(self, capacity, particleRadius, context=None, evictable=False):
         self.context = context or default_context()
         h = C.c_void_p()
-        call("pf_particles_create", self.context.handle, int(capacity), float(particleRadius), C.byref(h))
+        fn = "pf_particles_create_evictable" if evictable else "pf_particles_create"
+        call(fn, self.context.handle, int(capacity), float(particleRadius), C.byref(h))
         self.handle = h
+        self.evictable = bool(evictable)
         self.capacity = int(capacity)
         self.particleRadius = float(particleRadius)
 
@@ -148,6 +150,12 @@ class ParticleSystem:
         snap = _lib.Snapshot(*[_ptr(a, f32p) for a in arrs])
         call("pf_particles_readback", self.handle, C.byref(snap))
         return ParticleSnapshot(n, *arrs)
+
+    def deviceBytes(self):
+        """Device bytes of an evictable scene's arena (0 otherwise)."""
+        b = C.c_uint64()
+        call("pf_particles_device_bytes", self.handle, C.byref(b))
+        return b.value
 
     def readbackBoundaryVolume(self):
         """The boundaryVolume buffer (Akinci volumes of boundary particles, else 0)."""
@@ -594,6 +602,80 @@ class Batch:
     def dispose(self):
         if self.handle:
             call("pf_batch_destroy", self.handle)
+            self.handle = None
+
+
+class Server:
+    """The served form (no reference counterpart): step requests for live
+    scenes queue as they arrive and run in multi-scene launches as slots and
+    memory free up; scenes stay resident in HBM under a byte budget and idle
+    scenes are evicted to host memory. Scenes need evictable particles.
+    Request times are seconds on the server's clock (``now()``)."""
+
+    STATUS = ("pending", "launched", "done", "failed")
+
+    def __init__(self, context=None, hbmBudget=0, maxBatchScenes=0, maxBatchParticles=0, slots=0):
+        ctx = context or default_context()
+        o = _lib.ServerOptions(int(hbmBudget), int(maxBatchScenes), int(maxBatchParticles), int(slots))
+        h = C.c_void_p()
+        call("pf_server_create", ctx.handle, C.byref(o), C.byref(h))
+        self.handle = h
+        self.context = ctx
+        self.loops = {}
+
+    def addScene(self, loop):
+        sid = C.c_uint32()
+        call("pf_server_add_scene", self.handle, loop.handle, C.byref(sid))
+        self.loops[sid.value] = loop
+        return sid.value
+
+    def removeScene(self, sceneId):
+        call("pf_server_remove_scene", self.handle, int(sceneId))
+        self.loops.pop(int(sceneId), None)
+
+    def submit(self, sceneId, dt):
+        t = C.c_uint64()
+        call("pf_server_submit", self.handle, int(sceneId), float(dt), C.byref(t))
+        return t.value
+
+    def wait(self, ticket):
+        info = _lib.RequestInfo()
+        call("pf_server_wait", self.handle, int(ticket), C.byref(info))
+        return self._info(info)
+
+    def drain(self):
+        call("pf_server_drain", self.handle)
+
+    def evictScene(self, sceneId):
+        call("pf_server_evict_scene", self.handle, int(sceneId))
+
+    def now(self):
+        return _lib.load().pf_server_now(self.handle)
+
+    def requests(self, first, count):
+        """Times of requests [first, first + count) as a structured array."""
+        arr = (_lib.RequestInfo * max(1, int(count)))()
+        call("pf_server_request_info", self.handle, int(first), int(count), arr)
+        out = np.zeros(int(count), dtype=[("scene", np.uint32), ("status", np.uint32), ("launch_scenes", np.uint32),
+                                          ("t_submit", np.float64), ("t_launch", np.float64), ("t_done", np.float64)])
+        for k in range(int(count)):
+            r = arr[k]
+            out[k] = (r.scene, r.status, r.launch_scenes, r.t_submit, r.t_launch, r.t_done)
+        return out
+
+    def stats(self):
+        st = _lib.ServerStats()
+        call("pf_server_stats_get", self.handle, C.byref(st))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    @staticmethod
+    def _info(r):
+        return {"scene": r.scene, "status": Server.STATUS[r.status], "launchScenes": r.launch_scenes,
+                "tSubmit": r.t_submit, "tLaunch": r.t_launch, "tDone": r.t_done}
+
+    def dispose(self):
+        if self.handle:
+            call("pf_server_destroy", self.handle)
             self.handle = None
 
 

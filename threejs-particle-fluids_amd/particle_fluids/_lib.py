@@ -101,6 +101,27 @@ class Overflow(C.Structure):
     ]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_uint64),
+        ("max_batch_scenes", u32),
+        ("max_batch_particles", C.c_uint64),
+        ("slots", u32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in (
+        "submitted", "completed", "failed", "launches", "scene_steps", "particle_substeps", "evictions",
+        "restores", "resident_bytes", "peak_resident_bytes", "hbm_budget_bytes")] + [
+        ("scenes", u32), ("resident_scenes", u32), ("max_launch_scenes", u32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("scene", u32), ("status", u32), ("launch_scenes", u32), ("pad", u32),
+                ("t_submit", f64), ("t_launch", f64), ("t_done", f64)]
+
+
 # name -> argtypes (all return int status unless listed in RESTYPES)
 SIGNATURES = {
     "pf_last_error": [],
@@ -115,6 +136,8 @@ SIGNATURES = {
     "pf_context_profile_entry": [vp, u32, C.c_char_p, u32, P(u32), P(f64)],
     "pf_context_profile_reset": [vp],
     "pf_particles_create": [vp, u32, f64, P(vp)],
+    "pf_particles_create_evictable": [vp, u32, f64, P(vp)],
+    "pf_particles_device_bytes": [vp, P(C.c_uint64)],
     "pf_particles_destroy": [vp],
     "pf_particles_upload": [vp, u32, u32, P(f32), P(f32), P(f32), P(u32)],
     "pf_particles_set_inv_mass": [vp, u32, u32, f64],
@@ -178,6 +201,17 @@ SIGNATURES = {
     "pf_batch_step": [vp, f64],
     "pf_batch_set_graph": [vp, i32],
     "pf_batch_destroy": [vp],
+    "pf_server_create": [vp, P(ServerOptions), P(vp)],
+    "pf_server_add_scene": [vp, vp, P(u32)],
+    "pf_server_remove_scene": [vp, u32],
+    "pf_server_submit": [vp, u32, f64, P(C.c_uint64)],
+    "pf_server_wait": [vp, C.c_uint64, P(RequestInfo)],
+    "pf_server_drain": [vp],
+    "pf_server_request_info": [vp, C.c_uint64, u32, P(RequestInfo)],
+    "pf_server_stats_get": [vp, P(ServerStats)],
+    "pf_server_now": [vp],
+    "pf_server_evict_scene": [vp, u32],
+    "pf_server_destroy": [vp],
     "pf_layout_fill_box": [P(f64), P(f64), f64, P(f32), u32, P(u32)],
     "pf_layout_radius_for_budget": [f64, f64, f64, u32],
     "pf_layout_fit_budget": [u32, P(f64), P(f64), u32, P(f64), P(u32)],
@@ -186,6 +220,7 @@ RESTYPES = {
     "pf_last_error": C.c_char_p,
     "pf_abi_version": u32,
     "pf_layout_radius_for_budget": f64,
+    "pf_server_now": f64,
 }
 
 _lib = None

@@ -9,7 +9,7 @@ five container planes (Simulation.ts:777-787). The lattice gets a seeded
 """
 import numpy as np
 
-from . import FluidSystem, ParticleSystem, PrimitiveSet, SimLoop, fill_box, fit_budget
+from . import FluidSystem, ParticleSystem, PrimitiveSet, SimLoop, SoftbodySystem, fill_box, fit_budget
 
 CONTAINER = (-0.5, 0.0, -0.3, 0.5, 0.8, 0.3)
 FLUID_BOX = (-0.5, 0.0, -0.3, -0.1, 0.5, 0.3)
@@ -75,6 +75,116 @@ def build_dam_break(budget, substeps=2, iterations=4, seed=0xC0FFEE, context=Non
     if use_graph:
         loop.useGraph(True)
     return dict(ps=ps, fluid=fluid, walls=walls, loop=loop, positions=pos, radius=r)
+
+
+# Soft-body cubes for the batched and served scene mixes.
+CUBE_SIDE = 13          # 13^3 = 2,197 particles, the lattice cube nearest 2,000
+CUBE_DENSITY = 500.0    # Simulation's soft-body density default (Simulation.ts:409, 1052)
+CUBE_GAP = 0.1          # free space between neighbouring cubes, in cube widths
+CUBE_DROP = 0.05        # initial height above the floor, in cube widths
+
+
+def particle_cube(center, side, spacing):
+    """A side^3 lattice cube centred on `center`, surface particles first
+    (tests/perf/_scenes/_helpers.ts:88-111). Returns (positions, surfaceCount)."""
+    i = np.arange(side)
+    z, y, x = np.meshgrid(i, i, i, indexing="ij")  # x fastest, then y, then z
+    idx = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    surf = np.any((idx == 0) | (idx == side - 1), axis=1)
+    off = (idx - (side - 1) / 2.0) * spacing + np.asarray(center, np.float64)
+    pos = np.concatenate([off[surf], off[~surf]])
+    return pos, int(surf.sum())
+
+
+def softbody_compliance(softness, count):
+    """Shape-matching compliance from softness (src/simulation/handles.ts:246-248)."""
+    return 10.0 ** (-6.0 + softness * 3.0) * (count / 200.0)
+
+
+def mixed_layout(fluid_budget, bodies, seed):
+    """A dam break fitted to `fluid_budget` plus `bodies` soft cubes of
+    CUBE_SIDE^3 particles at the fluid's particle spacing.
+
+    The cubes rest in one layer on the floor of an extension of the tank past
+    x = 0.5 (rows of cubes across z, as many rows as needed), so every cube
+    fits at any particle radius; the dam-break water reaches them as it
+    spreads. Stiffness k is uniform in [0.1, 1.0] per cube from `seed`;
+    softness = 1 - k and compliance follows softbodyCompliance.
+    Returns dict(positions, inv_mass, fluid_count, bodies, radius, container).
+    """
+    pos, r = dam_break_layout(fluid_budget)
+    pos = jitter(pos, 2 * r, seed)
+    nf = len(pos)
+    container = list(CONTAINER)
+    out = dict(fluid_count=nf, radius=r, bodies=[], container=tuple(container))
+    if bodies == 0:
+        out.update(positions=pos, inv_mass=None)
+        return out
+    spacing = 2 * r
+    width = CUBE_SIDE * spacing
+    pitch = (1.0 + CUBE_GAP) * width
+    depth = CONTAINER[5] - CONTAINER[2]
+    per_row = max(1, int(depth // pitch))
+    rows = (bodies + per_row - 1) // per_row
+    container[3] = CONTAINER[3] + rows * pitch
+    rng = np.random.default_rng(seed)
+    stiffness = rng.uniform(0.1, 1.0, bodies)
+    cubes = []
+    for b in range(bodies):
+        row, col = divmod(b, per_row)
+        c = (CONTAINER[3] + (row + 0.5) * pitch, width / 2 + CUBE_DROP * width,
+             CONTAINER[2] + (col + 0.5) * depth / per_row)
+        cp, surf = particle_cube(c, CUBE_SIDE, spacing)
+        start = nf + sum(len(q) for q in cubes)
+        out["bodies"].append(dict(range=(start, len(cp)), surfaceCount=surf,
+                                  compliance=softbody_compliance(1.0 - stiffness[b], len(cp))))
+        cubes.append(cp)
+    allpos = np.concatenate([pos.astype(np.float64)] + cubes).astype(np.float32)
+    w = np.empty(len(allpos), np.float32)
+    w[:nf] = 0.0  # FluidSystem sets the fluid range's inverse mass
+    w[nf:] = 1.0 / (CUBE_DENSITY * spacing ** 3)
+    out.update(positions=allpos, inv_mass=w, container=tuple(container))
+    return out
+
+
+def build_mixed_scene(fluid_budget, bodies, seed, substeps=2, iterations=4, context=None, use_graph=False,
+                      evictable=False):
+    """One scene of the batched / served mixes on the GPU: the dam break with
+    Simulation's default fluid effects, `bodies` globally shape-matched cubes
+    and the five planes of the (extended) tank. `evictable` places every
+    device buffer in one arena so a Server can move the scene to host memory."""
+    lay = mixed_layout(fluid_budget, bodies, seed)
+    pos, nf, r = lay["positions"], lay["fluid_count"], lay["radius"]
+    ps = ParticleSystem(len(pos), r, context=context, evictable=evictable)
+    ps.uploadParticles(pos, invMass=lay["inv_mass"])
+    fluid = FluidSystem(ps, range=(0, nf), **FLUID_DEFAULTS)
+    mats = [fluid]
+    soft = None
+    if bodies:
+        soft = SoftbodySystem(ps, lay["bodies"])
+        mats.append(soft)
+    walls = PrimitiveSet(ps)
+    mn, mx = lay["container"][:3], lay["container"][3:]
+    for n, which in PLANES:
+        walls.addPlane(n, mn if which == "min" else mx)
+    loop = SimLoop(ps, substeps=substeps, iterations=iterations, materials=mats, colliders=[walls])
+    if use_graph:
+        loop.useGraph(True)
+    return dict(ps=ps, fluid=fluid, soft=soft, walls=walls, loop=loop, layout=lay)
+
+
+def dispose_scene(sc):
+    """Release a scene built by build_mixed_scene or build_dam_break, loop first."""
+    for key in ("loop", "soft", "fluid", "walls", "ps"):
+        obj = sc.get(key)
+        if obj is not None:
+            obj.dispose()
+
+
+def log_uniform_counts(count, lo, hi, seed):
+    """`count` particle budgets log-uniform in [lo, hi] from `seed`."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)
 
 
 def sort_points(n, distribution, seed=0x5EED):
