@@ -70,9 +70,14 @@ def run_batched(args, ctx):
     batch = pf.Batch(loops)
     batch.useGraph(True)
     for _ in range(args.batch_warmup):
-        for L in loops:
-            L.step(dt)
         batch.step(dt)
+    tuned = None
+    if not args.no_autotune:
+        best, times = batch.autotune(dt, steps=1)
+        tuned = {"tile_union_capacity": best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
+        print(f"[bench] batched autotune: {tuned}", file=sys.stderr, flush=True)
+    for L in loops:  # the plain form's graphs, captured before timing
+        L.step(dt)
     ctx.synchronize()
 
     def timed(fn):
@@ -91,6 +96,16 @@ def run_batched(args, ctx):
     t_batch = timed(lambda: batch.step(dt))
     t_plain2 = timed(plain_step)
     t_plain = 0.5 * (t_plain + t_plain2)
+    # per-kernel breakdown of one batched step, launched kernel by kernel
+    batch.useGraph(False)
+    ctx.set_profiling(True)
+    ctx.profile_reset()
+    batch.step(dt)
+    stats = ctx.profile()
+    ctx.set_profiling(False)
+    total_ms = sum(t for _, t in stats.values())
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "share": round(v[1] / total_ms, 4)}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
     work = n_total * args.substeps * args.batch_steps
     out = {
         "workload": f"{len(built)} scenes, budgets log-uniform in [5000, 200000] (seed {args.batch_seed}), "
@@ -105,6 +120,8 @@ def run_batched(args, ctx):
                     "scene_steps_per_s": round(len(built) * args.batch_steps / t_batch, 2),
                     "ms_per_step": round(t_batch / args.batch_steps * 1e3, 3)},
         "speedup": round(t_plain / t_batch, 3),
+        "autotune": tuned,
+        "kernels_one_step": kernels,
     }
     print(f"[bench] batched: plain {out['plain']}, batched {out['batched']}, x{out['speedup']}",
           file=sys.stderr, flush=True)
@@ -112,6 +129,116 @@ def run_batched(args, ctx):
     for b in built:
         scenes.dispose_scene(b)
     return out
+
+
+def run_served(args, ctx, tile_union=None):
+    """1,000 live scenes (fluid budgets log-uniform in [5,000, 500,000], 0-30
+    soft cubes each, all from a seed), resident in HBM under a 200 GB budget,
+    served by the request loop with continuous batching. A Poisson stream of
+    step(1/60) requests at --serve-rate req/s picks scenes uniformly; reports
+    throughput and p50/p99 latency (submission to completion), then the
+    capacity with every request queued at once, then the plain form (each
+    request as its own loop step, HIP graph per loop) on the same kind of
+    stream."""
+    import threading
+
+    import numpy as np
+
+    import particle_fluids as pf
+    from particle_fluids import scenes
+
+    dt = 1.0 / 60.0
+    rng = np.random.default_rng(args.serve_seed)
+    counts = scenes.log_uniform_counts(args.serve_scenes, 5000, 500000, args.serve_seed)
+    bodies = rng.integers(0, 31, args.serve_scenes)
+    t0 = time.perf_counter()
+    srv = pf.Server(context=ctx, hbmBudget=int(args.serve_budget_gb * (1 << 30)), slots=args.serve_slots)
+    built, sizes = [], []
+    for i, (c, b) in enumerate(zip(counts, bodies)):
+        sc = scenes.build_mixed_scene(int(c), int(b), seed=args.serve_seed + i, substeps=args.substeps,
+                                      iterations=args.iterations, context=ctx, evictable=True)
+        if tile_union:
+            sc["fluid"].setTileUnionLimit(tile_union)
+        sc["id"] = srv.addScene(sc["loop"])
+        built.append(sc)
+        sizes.append(sc["ps"].capacity)
+        if (i + 1) % 100 == 0:
+            print(f"[bench] served: {i + 1} scenes built ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+                  flush=True)
+    sizes = np.asarray(sizes, np.int64)
+    st0 = srv.stats()
+    print(f"[bench] served: {len(built)} scenes, {int(sizes.sum())} particles, {st0['resident_bytes'] / 2**30:.1f} "
+          f"GiB resident", file=sys.stderr, flush=True)
+
+    def run_stream(n, rate, seed):
+        r = np.random.default_rng(seed)
+        targets = r.integers(0, len(built), n)
+        arrivals = np.cumsum(r.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+        first = srv.stats()["submitted"]
+
+        def feeder():
+            base = time.perf_counter()
+            for k in range(n):
+                while True:
+                    late = arrivals[k] - (time.perf_counter() - base)
+                    if late <= 0:
+                        break
+                    if late > 3e-4:
+                        time.sleep(late - 2e-4)
+                srv.submit(built[targets[k]]["id"], dt)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        srv.drain()
+        info = srv.requests(first, n)
+        lat = (info["t_done"] - info["t_submit"]) * 1e3
+        span = info["t_done"].max() - info["t_submit"].min()
+        work = float(sizes[targets].sum()) * args.substeps
+        return {"requests": n, "offered_req_per_s": rate if rate > 0 else None,
+                "particle_substeps_per_s": round(work / span, 1), "scene_steps_per_s": round(n / span, 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_launch_scenes": round(float(info["launch_scenes"].mean()), 2), "seconds": round(span, 3),
+                "failed": int((info["status"] != 2).sum())}
+
+    run_stream(min(200, 2 * len(built)), 0, args.serve_seed + 1)  # warm-up
+    poisson = run_stream(args.serve_requests, args.serve_rate, args.serve_seed + 2)
+    print(f"[bench] served: Poisson {poisson}", file=sys.stderr, flush=True)
+    saturated = run_stream(args.serve_requests, 0, args.serve_seed + 3)
+    print(f"[bench] served: saturated {saturated}", file=sys.stderr, flush=True)
+    st = srv.stats()
+    for sc in built:
+        srv.removeScene(sc["id"])
+    srv.dispose()
+
+    # plain form: the same kind of stream, one loop step per request
+    r = np.random.default_rng(args.serve_seed + 4)
+    targets = r.integers(0, len(built), args.serve_plain_requests)
+    for sc in built:  # capture every loop's graph before the timed stream
+        sc["loop"].useGraph(True)
+        sc["loop"].step(dt)
+    ctx.synchronize()
+    tp = time.perf_counter()
+    for k in targets:
+        built[k]["loop"].step(dt)
+    ctx.synchronize()
+    tp = time.perf_counter() - tp
+    plain = {"requests": int(len(targets)),
+             "particle_substeps_per_s": round(float(sizes[targets].sum()) * args.substeps / tp, 1),
+             "scene_steps_per_s": round(len(targets) / tp, 2)}
+    print(f"[bench] served: plain {plain}", file=sys.stderr, flush=True)
+    for sc in built:
+        scenes.dispose_scene(sc)
+    return {
+        "workload": f"{len(built)} scenes, fluid budgets log-uniform in [5000, 500000], 0-30 soft cubes of 2197 "
+                    f"particles (seed {args.serve_seed}), {args.substeps} substeps x {args.iterations} iterations, "
+                    f"step(1/60) requests on uniformly chosen scenes",
+        "scenes": len(built), "particles": int(sizes.sum()),
+        "hbm_budget_gib": args.serve_budget_gb, "resident_gib": round(st0["resident_bytes"] / 2**30, 2),
+        "slots": args.serve_slots, "tile_union_capacity": tile_union, "poisson": poisson, "saturated": saturated, "plain": plain,
+        "speedup_saturated_vs_plain": round(saturated["particle_substeps_per_s"] / plain["particle_substeps_per_s"], 3),
+        "launches": st["launches"], "evictions": st["evictions"], "restores": st["restores"],
+    }
 
 
 def main():
@@ -123,6 +250,7 @@ def main():
     ap.add_argument("--substeps", type=int, default=2)
     ap.add_argument("--iterations", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-baseline-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -130,6 +258,15 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--batch-seed", type=int, default=0xBA7C4)
+    ap.add_argument("--serve-scenes", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=4000)
+    ap.add_argument("--serve-plain-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-seed", type=int, default=0x5E7E)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="end with a device copy of this size (a known-traffic kernel for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -148,6 +285,18 @@ def main():
     loop = sc["loop"]
     n = sc["ps"].capacity
     H = loop.info()["hashTableSize"]
+    tuned = None
+    if not args.no_autotune:
+        # tile-union LDS capacity, tuned on a twin of the scene at the same
+        # point of its run, so the measured scene keeps its configured window
+        twin = scenes.build_dam_break(args.budget, substeps=args.substeps, iterations=args.iterations, use_graph=True)
+        for _ in range(args.warmup):
+            twin["loop"].step(dt)
+        best, times = twin["loop"].autotune(dt, steps=2)
+        scenes.dispose_scene(twin)
+        sc["fluid"].setTileUnionLimit(best)
+        tuned = {"tile_union_capacity": best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         loop.step(dt)
     ctx.synchronize()
@@ -220,6 +369,10 @@ def main():
     batched = None
     if args.batch_scenes > 0:
         batched = run_batched(args, ctx)
+    served = None
+    if args.serve_scenes > 0:
+        tu = batched["autotune"]["tile_union_capacity"] if batched and batched.get("autotune") else None
+        served = run_served(args, ctx, tu)
 
     line = {
         "metric": "particle-substeps/s at 1M PBF",
@@ -237,15 +390,28 @@ def main():
         "config": {"workload": f"dam break, plain method, {n} particles (budget {args.budget}), "
                                f"{args.substeps} substeps x {args.iterations} iterations, dt 1/60, one scene",
                    "particles": n, "hash_table_size": H, "substeps": args.substeps,
-                   "iterations": args.iterations, "graph": not args.no_graph},
+                   "iterations": args.iterations, "graph": not args.no_graph, "autotune": tuned},
         "roofline": roofline,
         "roofline_substep": roofline_substep,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "gpu_ms_timed": round(gpu_ms, 3),
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration_mib:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # calibrate the counters against the bytes it must move.
+        import torch
+
+        nbytes = args.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -297,6 +297,17 @@ int pf_batch_step(pf_batch* batch, double dt);
 int pf_batch_set_graph(pf_batch* batch, int enabled);
 int pf_batch_destroy(pf_batch* batch);
 
+/* ---- Autotuner (no counterpart in the reference): times `steps` steps of
+ * the loop's or batch's real workload for each candidate tile-union LDS
+ * capacity (256..2048 particles staged per 256-slot tile; larger unions walk
+ * global memory) and sets the fastest on every fluid. The tuning steps
+ * advance the scenes. ms_per_step (n, nullable) receives each candidate's
+ * mean over a forward and a backward sweep. */
+int pf_loop_autotune(pf_loop* loop, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                     uint32_t* best, float* ms_per_step);
+int pf_batch_autotune(pf_batch* batch, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                      uint32_t* best, float* ms_per_step);
+
 /* ---- Served form: a request loop over live scenes. No counterpart in the
  * reference. Step requests for scenes added to the server are queued as they
  * arrive; a worker thread launches every queued request whose scene is not

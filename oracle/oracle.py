@@ -57,6 +57,8 @@ def lib():
         g("scene_add_fluid").restype = i32
         g("scene_add_prim").argtypes = [vp, u32, u32, P(f64), P(f64), P(f64), P(f64), P(f64)]
         g("scene_add_prim").restype = i32
+        g("scene_set_prim").argtypes = [vp, u32, u32, u32, P(f64), P(f64), P(f64), P(f64), P(f64)]
+        g("scene_set_prim").restype = i32
         g("scene_add_body").argtypes = [vp, u32, u32, f64, P(f32), P(f32)]
         g("scene_add_body").restype = i32
         g("scene_add_sdf").argtypes = [vp, P(f32), P(u32), P(f64), P(f64), P(f64), P(f64), f64, f64, f64, f64]
@@ -185,6 +187,27 @@ class Scene:
         arr = [np.asarray(a, np.float64) for a in (d0, d1, vel, rot, spin)]
         rc = self._f("scene_add_prim")(self.h, int(kind), int(flags), *[_p(a, f64) for a in arr])
         assert rc == 0
+
+    def set_prim(self, k, kind, flags, d0, d1, vel, rot=(0, 0, 0, 1), spin=(0, 0, 0)):
+        arr = [np.asarray(a, np.float64) for a in (d0, d1, vel, rot, spin)]
+        rc = self._f("scene_set_prim")(self.h, int(k), int(kind), int(flags), *[_p(a, f64) for a in arr])
+        assert rc == 0
+
+    # PrimitiveSet add* (PrimitiveSet.ts:160-209): d0/d1 hold the shape, d1.w
+    # static friction, vel.w kinetic friction; flags bit 0 inverts the shape.
+    def add_sphere(self, center, radius, invert=False, mus=0.5, muk=0.4):
+        c = np.asarray(center, np.float64)
+        self.add_prim(1, 1 if invert else 0, [c[0], c[1], c[2], radius], [0, 0, 0, mus], [0, 0, 0, muk])
+
+    def add_box(self, center, half_extents, rotation=None, invert=False, mus=0.5, muk=0.4):
+        c, he = np.asarray(center, np.float64), np.asarray(half_extents, np.float64)
+        q = np.array([0, 0, 0, 1.0]) if rotation is None else np.asarray(rotation, np.float64)
+        q = q / np.sqrt((q * q).sum())
+        self.add_prim(2, 1 if invert else 0, [c[0], c[1], c[2], 0], [he[0], he[1], he[2], mus], [0, 0, 0, muk], q)
+
+    def add_capsule(self, a, b, radius, mus=0.5, muk=0.4):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        self.add_prim(3, 0, [a[0], a[1], a[2], radius], [b[0], b[1], b[2], mus], [0, 0, 0, muk])
 
     def add_plane(self, normal, point, mus=0.5, muk=0.4):
         n = np.asarray(normal, np.float64)

@@ -421,6 +421,21 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     for (int a = 0; a < 3; ++a) P->spin[a] = (T)(float)spin[a];                                    \
     return 0;                                                                                      \
   }                                                                                                \
+  /* replace primitive k (a collider moved between steps) */                                      \
+  int orc_scene_set_prim_##SUF(void* p, uint32_t k, uint32_t kind, uint32_t flags, const double d0[4], \
+                               const double d1[4], const double vel[4], const double rot[4],       \
+                               const double spin[3]) {                                             \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (k >= s->nprim) return -1;                                                                  \
+    prim_##SUF* P = &s->prim[k];                                                                   \
+    P->kind = kind; P->flags = flags;                                                              \
+    for (int a = 0; a < 4; ++a) {                                                                  \
+      P->d0[a] = (T)(float)d0[a]; P->d1[a] = (T)(float)d1[a];                                      \
+      P->vel[a] = (T)(float)vel[a]; P->rot[a] = (T)(float)rot[a];                                  \
+    }                                                                                              \
+    for (int a = 0; a < 3; ++a) P->spin[a] = (T)(float)spin[a];                                    \
+    return 0;                                                                                      \
+  }                                                                                                \
   /* one global shape-matching body; rest offsets and weights as SoftbodySystem computes them */  \
   int orc_scene_add_body_##SUF(void* p, uint32_t start, uint32_t count, double compliance,        \
                                const float* rest3, const float* weights) {                         \

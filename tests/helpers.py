@@ -164,3 +164,75 @@ def oracle_mixed_scene(layout, substeps, iterations, H, precision="f32"):
     for nrm, which in scenes.PLANES:
         s.add_plane(nrm, mn if which == "min" else mx)
     return s
+
+
+# ---------------------------------------------------------------------------
+# Posed primitives: PrimitiveSet.attach / update / place (PrimitiveSet.ts:
+# 211-234, 259-285, 469-486) restated in double precision, giving the values
+# the oracle's set_prim takes. Quaternions are (x, y, z, w) as in three.js.
+# ---------------------------------------------------------------------------
+def qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([ax * bw + aw * bx + ay * bz - az * by, ay * bw + aw * by + az * bx - ax * bz,
+                     az * bw + aw * bz + ax * by - ay * bx, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def qconj(q):
+    return np.array([-q[0], -q[1], -q[2], q[3]])
+
+
+def qapply(q, v):
+    # Vector3.applyQuaternion: t = 2 q.xyz x v; v + w t + q.xyz x t
+    u = np.asarray(q[:3], np.float64)
+    t = 2.0 * np.cross(u, v)
+    return np.asarray(v, np.float64) + q[3] * t + np.cross(u, t)
+
+
+class PosedPrimitive:
+    """One attached box or capsule: data0/data1/velocity/rotation/spin as
+    PrimitiveSet keeps them, driven by world poses per step."""
+    KINDS = {"box": 2, "capsule": 3}
+
+    def __init__(self, kind, data0, data1, mus=0.5, muk=0.4):
+        self.kind = self.KINDS[kind]
+        self.d0 = np.array(data0, np.float64)
+        self.d1 = np.array(data1, np.float64)
+        self.d1[3] = mus
+        self.vel = np.array([0.0, 0.0, 0.0, muk])
+        self.rot = np.array([0.0, 0.0, 0.0, 1.0])
+        self.spin = np.zeros(3)
+        self.flags = 0
+        self.attached = False
+
+    def pose(self, position, rotation, dt=0.0):
+        p = np.asarray(position, np.float64)
+        q = np.asarray(rotation, np.float64)
+        if not self.attached:
+            if self.kind == 3:
+                self.direction = (self.d1[:3] - self.d0[:3]) * 0.5
+            else:
+                self.direction = np.zeros(3)
+            self.offset = np.array([0.0, 0.0, 0.0, 1.0]) if self.kind == 2 else qconj(q)
+            self.flags |= 2
+            self.attached = True
+        elif dt > 0:
+            self.vel[:3] = (p - self.prev) / dt
+            turn = qmul(q, qconj(self.prev_rot))
+            if turn[3] < 0:
+                turn = -turn
+            s = np.hypot(turn[0], np.hypot(turn[1], turn[2]))
+            rate = (2 * np.arctan2(s, turn[3])) / (s * dt) if s > 1e-12 else 0.0
+            self.spin = turn[:3] * rate
+        self.rot = qmul(q, self.offset)
+        d = qapply(self.rot, self.direction)
+        if self.kind == 3:
+            self.d0[:3] = p - d
+            self.d1[:3] = p + d
+        else:
+            self.d0[:3] = p
+        self.prev = p.copy()
+        self.prev_rot = q.copy()
+
+    def oracle_args(self):
+        return self.kind, self.flags, self.d0, self.d1, self.vel, self.rot, self.spin

@@ -52,3 +52,12 @@ def test_js_simulation_matches_python_simulation(tmp_path):
     assert js_pos.shape == py_pos.shape
     assert np.array_equal(js_pos, py_pos)
     sim.dispose()
+
+
+@pytest.mark.gpu
+def test_js_server_steps_scenes_like_plain_loops():
+    r = subprocess.run([NODE, os.path.join(JS, "test", "served.js")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print(f"[node] served {out}", flush=True)
+    assert out["same"] and out["rejected"] and out["done"] and out["completed"] == 9

@@ -150,13 +150,15 @@ static __device__ __forceinline__ void tu_load_slots(const uint32_t* __restrict_
 template <bool NORMAL>
 __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
-                                                        uint32_t ntiles, int fi, int p, int c) {
+                                                        uint32_t ntiles, int fi, int p, int c,
+                                                        uint32_t cap) {
   __shared__ uint32_t h_key[PF_TU_HASH];    // bucket + 1, 0 = empty
   __shared__ uint32_t h_off[PF_TU_HASH];    // (union offset << 16) | bucket count
   __shared__ uint32_t h_start[PF_TU_HASH];  // first sorted slot of the bucket
-  // The staged union; before staging the same words hold each thread's query
-  // cell and activity for the leader test.
-  __shared__ float4 u_x[PF_TU_MAX];
+  // The staged union (cap >= PF_BLOCK entries of dynamic LDS); before staging
+  // the same words hold each thread's query cell and activity for the leader test.
+  extern __shared__ float4 pf_lds[];
+  float4* u_x = pf_lds;
   int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(&u_x[0]);
   uint32_t* a_sh = reinterpret_cast<uint32_t*>(&u_x[0]) + 3 * PF_BLOCK;
   __shared__ uint32_t ctl[4 + PF_BLOCK / 64];  // distinct buckets, give-up flag, -, -, scan
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       h_start[tid * PER + i] = st[i];
       off += cn[i];
     }
-    global_walk = U > F.tu_limit;
+    global_walk = U > min(F.tu_limit, cap);
     __syncthreads();
   }
 
@@ -661,9 +663,10 @@ static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, f
 
 __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, uint32_t ntiles,
-                                                     int fi, int p, int c) {
-  __shared__ float4 u_x[PF_TU_MAX];  // position, neighbour mass
-  __shared__ float u_w[PF_TU_MAX];   // inverse mass
+                                                     int fi, int p, int c, uint32_t cap) {
+  extern __shared__ float4 pf_lds[];
+  float4* u_x = pf_lds;                                  // position, neighbour mass
+  float* u_w = reinterpret_cast<float*>(pf_lds + cap);   // inverse mass
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -783,9 +786,10 @@ template <bool WALLS, bool ADVECT>
 __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, uint32_t ntiles,
                                                     int fi, int p, int c, int first_iteration,
-                                                    int substep) {
-  __shared__ float4 u_x[PF_TU_MAX];  // position, lambda
-  __shared__ float u_m[PF_TU_MAX];   // neighbour mass
+                                                    int substep, uint32_t cap) {
+  extern __shared__ float4 pf_lds[];
+  float4* u_x = pf_lds;                                  // position, lambda
+  float* u_m = reinterpret_cast<float*>(pf_lds + cap);   // neighbour mass
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -1022,9 +1026,10 @@ static __device__ __forceinline__ void velocity_term(const FluidDev& F, bool ok,
 template <bool VORT, bool VISC>
 __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
-                                                       uint32_t ntiles, int fi, int p) {
-  __shared__ float4 u_x[PF_TU_MAX];  // position, neighbour volume
-  __shared__ float4 u_v[PF_TU_MAX];  // velocity
+                                                       uint32_t ntiles, int fi, int p, uint32_t cap) {
+  extern __shared__ float4 pf_lds[];
+  float4* u_x = pf_lds;        // position, neighbour volume
+  float4* u_v = pf_lds + cap;  // velocity
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -1124,9 +1129,10 @@ template <bool VORT, bool VISC, bool FRICTION>
 __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
-                                                        int substep, int touch_valid) {
-  __shared__ float4 u_x[PF_TU_MAX];  // position, |omega|
-  __shared__ uint8_t u_skip[PF_TU_MAX]; // boundary particle (left out of the gradient)
+                                                        int substep, int touch_valid, uint32_t cap) {
+  extern __shared__ float4 pf_lds[];
+  float4* u_x = pf_lds;                                       // position, |omega|
+  uint8_t* u_skip = reinterpret_cast<uint8_t*>(pf_lds + cap);  // boundary particle (left out of the gradient)
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -1257,12 +1263,15 @@ __global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restric
 // Host launchers
 // ---------------------------------------------------------------------------
 #define PF_GRID dim3(ntiles), dim3(PF_BLOCK), 0, st
+// Launch with `lds` bytes of dynamic LDS (the tile-union arrays).
+#define PF_GRID_LDS(lds) dim3(ntiles), dim3(PF_BLOCK), (lds), st
 
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                      int p, int c, bool normal) {
+                      int p, int c, bool normal, uint32_t cap) {
   if (!ntiles) return;
-  if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID, sc, tl, ntiles, fi, p, c);
-  else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID, sc, tl, ntiles, fi, p, c);
+  const size_t lds = (size_t)cap * sizeof(float4);
+  if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
+  else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
 }
 
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
@@ -1275,18 +1284,22 @@ void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_
 }
 
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                   int p, int c) {
+                   int p, int c, uint32_t cap) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_lambda, PF_GRID, sc, tl, ntiles, fi, p, c);
+  const size_t lds = (size_t)cap * (sizeof(float4) + sizeof(float));
+  hipLaunchKernelGGL(k_lambda, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
 }
 
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                  int p, int c, bool walls, bool advect, int first_iteration, int substep) {
+                  int p, int c, bool walls, bool advect, int first_iteration, int substep, uint32_t cap) {
   if (!ntiles) return;
-  if (walls && advect) hipLaunchKernelGGL((k_delta<true, true>), PF_GRID, sc, tl, ntiles, fi, p, c, first_iteration, substep);
-  else if (walls) hipLaunchKernelGGL((k_delta<true, false>), PF_GRID, sc, tl, ntiles, fi, p, c, first_iteration, substep);
-  else if (advect) hipLaunchKernelGGL((k_delta<false, true>), PF_GRID, sc, tl, ntiles, fi, p, c, first_iteration, substep);
-  else hipLaunchKernelGGL((k_delta<false, false>), PF_GRID, sc, tl, ntiles, fi, p, c, first_iteration, substep);
+  const size_t lds = (size_t)cap * (sizeof(float4) + sizeof(float));
+#define PF_DK(w, a) hipLaunchKernelGGL((k_delta<w, a>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, first_iteration, substep, cap)
+  if (walls && advect) PF_DK(true, true);
+  else if (walls) PF_DK(true, false);
+  else if (advect) PF_DK(false, true);
+  else PF_DK(false, false);
+#undef PF_DK
 }
 
 void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c,
@@ -1314,18 +1327,20 @@ void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t 
 }
 
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, bool vort, bool visc) {
+                     int p, bool vort, bool visc, uint32_t cap) {
   if (!ntiles || !(vort || visc)) return;
-  if (vort && visc) hipLaunchKernelGGL((k_velocity<true, true>), PF_GRID, sc, tl, ntiles, fi, p);
-  else if (vort) hipLaunchKernelGGL((k_velocity<true, false>), PF_GRID, sc, tl, ntiles, fi, p);
-  else hipLaunchKernelGGL((k_velocity<false, true>), PF_GRID, sc, tl, ntiles, fi, p);
+  const size_t lds = (size_t)cap * 2 * sizeof(float4);
+  if (vort && visc) hipLaunchKernelGGL((k_velocity<true, true>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
+  else if (vort) hipLaunchKernelGGL((k_velocity<true, false>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
+  else hipLaunchKernelGGL((k_velocity<false, true>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
 }
 
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool vort, bool visc, bool friction, int substep,
-                      int touch_valid) {
+                      int touch_valid, uint32_t cap) {
   if (!ntiles) return;
-#define PF_VK(a, b, f) hipLaunchKernelGGL((k_vorticity<a, b, f>), PF_GRID, sc, tl, ntiles, fi, p, c, substep, touch_valid)
+  const size_t lds = (size_t)cap * sizeof(float4) + ((cap + 15u) & ~15u);
+#define PF_VK(a, b, f) hipLaunchKernelGGL((k_vorticity<a, b, f>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, substep, touch_valid, cap)
   if (friction) {
     if (vort && visc) PF_VK(true, true, true);
     else if (vort) PF_VK(true, false, true);

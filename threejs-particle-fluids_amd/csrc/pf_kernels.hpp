@@ -15,15 +15,16 @@ void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint
                  int mode, int parity, int xs_prev, int xs_out, uint32_t* out_perm,
                  uint32_t* out_skey, float4* out_spos);
 
-// fluid.hip
+// fluid.hip. `cap`: tile-union entries of dynamic LDS per workgroup
+// (PF_BLOCK..PF_TU_MAX); the neighbour build keeps unions within it.
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                      int p, int c, bool normal);
+                      int p, int c, bool normal, uint32_t cap);
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, int c, bool surface_tension, bool adhesion);
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                   int p, int c);
+                   int p, int c, uint32_t cap);
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                  int p, int c, bool walls, bool advect, int first_iteration, int substep);
+                  int p, int c, bool walls, bool advect, int first_iteration, int substep, uint32_t cap);
 void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c,
                   bool advect, int first_iteration, int substep);
 void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c);
@@ -32,10 +33,10 @@ void launch_reaction_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, u
 void launch_boundary_volume(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
                             int c, int dynamic);
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, bool vort, bool visc);
+                     int p, bool vort, bool visc, uint32_t cap);
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool vort, bool visc, bool friction, int substep,
-                      int touch_valid);
+                      int touch_valid, uint32_t cap);
 void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,
                      int c, int substep);
 

@@ -269,6 +269,7 @@ struct PlanShape {
   bool any_boundary = false;
   uint32_t nb[PF_MAX_FLUIDS] = {0, 0, 0, 0};    // boundary ranges per fluid slot (max over scenes)
   uint32_t bdyn[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of dynamic-boundary bits per fluid slot
+  uint32_t tu_cap[PF_MAX_FLUIDS] = {PF_TU_MAX, PF_TU_MAX, PF_TU_MAX, PF_TU_MAX};  // LDS union capacity (max over scenes)
   bool local_soft = false;  // any scene's soft bodies use local shape matching
   bool walls = false;       // any scene has primitives
   bool soft = false;        // any scene has soft bodies

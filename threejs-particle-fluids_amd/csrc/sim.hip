@@ -119,7 +119,8 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
     }
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       PF_LAUNCH(ctx, "neighbors");
-      launch_neighbors(st, scenes, tiles, ntiles, (int)f, p, c, (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0);
+      launch_neighbors(st, scenes, tiles, ntiles, (int)f, p, c, (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0,
+                       sh.tu_cap[f]);
       ++nk;
     }
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
@@ -136,12 +137,13 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       const bool lastit = it + 1 == sh.iterations;
       for (uint32_t f = 0; f < sh.nfluids; ++f) {
         const bool lastf = f + 1 == sh.nfluids;
-        { PF_LAUNCH(ctx, "lambda"); launch_lambda(st, scenes, tiles, ntiles, (int)f, p, c); ++nk; }
+        { PF_LAUNCH(ctx, "lambda"); launch_lambda(st, scenes, tiles, ntiles, (int)f, p, c, sh.tu_cap[f]); ++nk; }
         const bool fuse_walls = lastf && last == DELTA && sh.walls;
         const bool fuse_adv = lastf && lastit && last == DELTA;
         {
           PF_LAUNCH(ctx, fuse_walls ? "delta_walls" : "delta");
-          launch_delta(st, scenes, tiles, ntiles, (int)f, p, c, fuse_walls, fuse_adv, it == 0, (int)k);
+          launch_delta(st, scenes, tiles, ntiles, (int)f, p, c, fuse_walls, fuse_adv, it == 0, (int)k,
+                       sh.tu_cap[f]);
           ++nk;
         }
         c ^= 1;
@@ -186,13 +188,14 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       const bool lastf = f + 1 == sh.nfluids;
       if (vort || visc) {
         PF_LAUNCH(ctx, "velocity");
-        launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc);
+        launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc, sh.tu_cap[f]);
         ++nk;
       }
       const bool fric = lastf && sh.walls;
       if (vort || visc || fric) {
         PF_LAUNCH(ctx, fric ? "vorticity_friction" : "vorticity");
-        launch_vorticity(st, scenes, tiles, ntiles, (int)f, p, c, vort, visc, fric, (int)k, touch_valid);
+        launch_vorticity(st, scenes, tiles, ntiles, (int)f, p, c, vort, visc, fric, (int)k, touch_valid,
+                         sh.tu_cap[f]);
         ++nk;
         if (fric) friction_done = true;
       }
@@ -416,12 +419,19 @@ int pf_fluid_info(pf_fluid* f, double* mass, double* h, double* spacing, double*
   PF_API_END
 }
 
+static void loop_invalidate_graphs(pf_loop* L);
+
 int pf_fluid_set_tile_union_limit(pf_fluid* f, uint32_t max_particles) {
   PF_API_BEGIN
   if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (f->ps && f->ps->server) fail(PF_ESTATE, "FluidSystem: the scene is being served; remove it from the server first");
   if (max_particles > PF_TU_MAX) fail(PF_EINVAL, "FluidSystem: the tile union limit is at most %u, got %u", PF_TU_MAX, max_particles);
   f->tu_limit = max_particles;
-  if (f->loop) fluid_write_params(f);
+  if (f->loop) {
+    fluid_write_params(f);
+    // captured launches carry the LDS capacity the limit implies
+    loop_invalidate_graphs(f->loop);
+  }
   PF_API_END
 }
 
@@ -1126,6 +1136,7 @@ static PlanShape loop_shape(const pf_loop* L) {
   for (uint32_t f = 0; f < sh.nfluids; ++f) {
     sh.nb[f] = L->ps->host.fluid[f].nboundary;
     sh.bdyn[f] = L->ps->host.fluid[f].bdynamic;
+    sh.tu_cap[f] = std::min<uint32_t>(std::max<uint32_t>(L->ps->host.fluid[f].tu_limit, PF_BLOCK), PF_TU_MAX);
   }
   return sh;
 }
@@ -1400,6 +1411,8 @@ namespace pf {
 PlanShape loop_plan_shape(const pf_loop_s* L) { return loop_shape(L); }
 
 void plan_merge(PlanShape& sh, const PlanShape& ls) {
+  for (uint32_t f = 0; f < ls.nfluids; ++f)
+    sh.tu_cap[f] = f < sh.nfluids ? std::max(sh.tu_cap[f], ls.tu_cap[f]) : ls.tu_cap[f];
   sh.nfluids = std::max(sh.nfluids, ls.nfluids);
   for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
     sh.fx[f] |= ls.fx[f];
@@ -1666,9 +1679,19 @@ int pf_batch_step(pf_batch* B, double dt) {
   if (!(dt > 0) || !std::isfinite(dt)) fail(PF_EINVAL, "Batch: step dt must be a positive finite number, got %g", dt);
   Context* ctx = B->ctx;
   bool changed = false;
+  PlanShape now;
+  now.substeps = B->shape.substeps;
+  now.iterations = B->shape.iterations;
   for (auto* L : B->loops) {
     loop_alive(L, "Batch");
     changed = loop_prepare_step(L, dt) || changed;
+    plan_merge(now, loop_shape(L));
+  }
+  // A changed tile-union limit changes the launches' LDS capacity.
+  if (std::memcmp(now.tu_cap, B->shape.tu_cap, sizeof now.tu_cap) != 0) {
+    std::memcpy(B->shape.tu_cap, now.tu_cap, sizeof now.tu_cap);
+    if (B->graph) { (void)hipGraphExecDestroy(B->graph); B->graph = nullptr; }
+    B->graph_valid = false;
   }
   if (changed || B->desc_dirty) {
     PF_HIP(hipStreamSynchronize(ctx->stream));
@@ -1709,6 +1732,92 @@ int pf_batch_step(pf_batch* B, double dt) {
   // The graph reads the descriptor array at run time; swapped buffer pairs
   // only need the array re-uploaded before the next step.
   if (flips) B->desc_dirty = true;
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: the tile-union LDS capacity (PF_BLOCK..PF_TU_MAX entries per
+// 256-slot tile) trades occupancy of the neighbour kernels against tiles that
+// fall back to global gathers. Each candidate is timed over `steps` steps of
+// the real workload (after one untimed step that captures its graph), in a
+// forward then a backward sweep so the scenes' drift during tuning averages
+// out; the fastest is kept. The tuning steps advance the scenes.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+static void tune_set_limit(const std::vector<pf_loop*>& loops, uint32_t cap) {
+  for (auto* L : loops)
+    for (auto* f : L->fluids)
+      if (pf_fluid_set_tile_union_limit(f, cap) != PF_OK) throw PfError{PF_EINVAL};
+}
+
+template <typename StepFn>
+static uint32_t tune_run(Context* ctx, const std::vector<pf_loop*>& loops, const uint32_t* cand, uint32_t n,
+                         uint32_t steps, float* ms_out, StepFn step) {
+  if (!cand || n == 0) fail(PF_EINVAL, "Autotune: at least one candidate is required");
+  for (uint32_t i = 0; i < n; ++i)
+    if (cand[i] < PF_BLOCK || cand[i] > PF_TU_MAX)
+      fail(PF_EINVAL, "Autotune: tile-union capacity %u outside [%u, %u]", cand[i], PF_BLOCK, PF_TU_MAX);
+  if (steps == 0) steps = 1;
+  std::vector<double> total(n, 0.0);
+  hipEvent_t e0, e1;
+  PF_HIP(hipEventCreate(&e0));
+  PF_HIP(hipEventCreate(&e1));
+  try {
+    for (int pass = 0; pass < 2; ++pass) {
+      for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = pass == 0 ? k : n - 1 - k;
+        tune_set_limit(loops, cand[i]);
+        step();  // captures the candidate's graph
+        PF_HIP(hipEventRecord(e0, ctx->stream));
+        for (uint32_t t = 0; t < steps; ++t) step();
+        PF_HIP(hipEventRecord(e1, ctx->stream));
+        PF_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        total[i] += ms / steps;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  uint32_t best = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (ms_out) ms_out[i] = (float)(total[i] / 2.0);
+    if (total[i] < total[best]) best = i;
+  }
+  tune_set_limit(loops, cand[best]);
+  return cand[best];
+}
+
+extern "C" {
+
+int pf_loop_autotune(pf_loop* L, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                     uint32_t* best, float* ms_per_step) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.autotune");
+  if (L->fluids.empty()) fail(PF_EINVAL, "SimLoop.autotune: the loop has no fluid to tune");
+  std::vector<pf_loop*> loops{L};
+  const uint32_t b = tune_run(L->ctx, loops, candidates, n, steps, ms_per_step, [&] {
+    if (pf_loop_step(L, dt) != PF_OK) throw PfError{PF_EINVAL};
+  });
+  if (best) *best = b;
+  PF_API_END
+}
+
+int pf_batch_autotune(pf_batch* B, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                      uint32_t* best, float* ms_per_step) {
+  PF_API_BEGIN
+  if (!B) fail(PF_EINVAL, "Batch: null handle");
+  if (B->shape.nfluids == 0) fail(PF_EINVAL, "Batch.autotune: the batch has no fluid to tune");
+  const uint32_t b = tune_run(B->ctx, B->loops, candidates, n, steps, ms_per_step, [&] {
+    if (pf_batch_step(B, dt) != PF_OK) throw PfError{PF_EINVAL};
+  });
+  if (best) *best = b;
   PF_API_END
 }
 

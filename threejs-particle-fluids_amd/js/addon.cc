@@ -179,11 +179,22 @@ napi_value ContextSynchronize(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------- particles
+// (ctx, capacity, radius, evictable)
 napi_value ParticlesCreate(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   pf_particles* p = nullptr;
-  PF_CALL(pf_particles_create((pf_context*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2]), &p));
+  bool evictable = false;
+  if (!is_nullish(env, a.argv[3])) napi_get_value_bool(env, a.argv[3], &evictable);
+  auto* ctx = (pf_context*)handle(env, a.argv[0]);
+  if (evictable) PF_CALL(pf_particles_create_evictable(ctx, u32(env, a.argv[1]), num(env, a.argv[2]), &p));
+  else PF_CALL(pf_particles_create(ctx, u32(env, a.argv[1]), num(env, a.argv[2]), &p));
   return make_handle(env, p);
+}
+napi_value ParticlesDeviceBytes(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint64_t b = 0;
+  PF_CALL(pf_particles_device_bytes((pf_particles*)handle(env, a.argv[0]), &b));
+  return number(env, (double)b);
 }
 napi_value ParticlesDestroy(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
@@ -714,6 +725,238 @@ napi_value LoopDestroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------- autotuner
+// (handle, dt, [candidates], steps) -> {best, msPerStep: {capacity: ms}}
+template <typename Fn>
+napi_value autotune(napi_env env, napi_callback_info info, Fn fn) {
+  Args a = get_args(env, info);
+  auto c = doubles(env, a.argv[2]);
+  std::vector<uint32_t> cand(c.begin(), c.end());
+  std::vector<float> ms(cand.size(), 0.f);
+  uint32_t best = 0;
+  PF_CALL(fn(handle(env, a.argv[0]), num(env, a.argv[1]), cand.data(), (uint32_t)cand.size(),
+             u32(env, a.argv[3], 2), &best, ms.data()));
+  napi_value obj, per;
+  napi_create_object(env, &obj);
+  napi_create_object(env, &per);
+  for (size_t i = 0; i < cand.size(); ++i) set(env, per, std::to_string(cand[i]).c_str(), number(env, ms[i]));
+  set(env, obj, "best", number(env, best));
+  set(env, obj, "msPerStep", per);
+  return obj;
+}
+napi_value LoopAutotune(napi_env env, napi_callback_info info) {
+  return autotune(env, info, [](void* h, double dt, const uint32_t* c, uint32_t n, uint32_t st, uint32_t* b,
+                                float* ms) { return pf_loop_autotune((pf_loop*)h, dt, c, n, st, b, ms); });
+}
+
+// ---------------------------------------------------------------- batched form
+// Resolves a Promise from a worker thread once a context's stream is idle.
+struct SyncWork {
+  pf_context* ctx;
+  napi_deferred deferred;
+  napi_async_work work;
+  int rc;
+  std::string error;
+};
+
+napi_value promise_after_sync(napi_env env, pf_context* ctx) {
+  auto* w = new SyncWork{ctx, nullptr, nullptr, PF_OK, {}};
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
+  napi_create_string_utf8(env, "pf_context_synchronize", NAPI_AUTO_LENGTH, &name);
+  CHECK_NAPI(napi_create_async_work(
+      env, nullptr, name,
+      [](napi_env, void* d) {
+        auto* w = static_cast<SyncWork*>(d);
+        w->rc = pf_context_synchronize(w->ctx);
+        if (w->rc != PF_OK) w->error = pf_last_error();
+      },
+      [](napi_env env, napi_status, void* d) {
+        auto* w = static_cast<SyncWork*>(d);
+        if (w->rc == PF_OK) {
+          napi_value u;
+          napi_get_undefined(env, &u);
+          napi_resolve_deferred(env, w->deferred, u);
+        } else {
+          napi_value msg, err;
+          napi_create_string_utf8(env, w->error.c_str(), NAPI_AUTO_LENGTH, &msg);
+          napi_create_error(env, nullptr, msg, &err);
+          napi_reject_deferred(env, w->deferred, err);
+        }
+        napi_delete_async_work(env, w->work);
+        delete w;
+      },
+      w, &w->work));
+  CHECK_NAPI(napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// (ctx, [loops])
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t n = 0;
+  napi_get_array_length(env, a.argv[1], &n);
+  std::vector<pf_loop*> loops(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value e;
+    napi_get_element(env, a.argv[1], i, &e);
+    loops[i] = (pf_loop*)handle(env, e);
+  }
+  pf_batch* b = nullptr;
+  PF_CALL(pf_batch_create((pf_context*)handle(env, a.argv[0]), loops.data(), n, &b));
+  return make_handle(env, b);
+}
+// (ctx, batch, dt) -> Promise resolved when the step has run
+napi_value BatchStepAsync(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_batch_step((pf_batch*)handle(env, a.argv[1]), num(env, a.argv[2])));
+  return promise_after_sync(env, (pf_context*)handle(env, a.argv[0]));
+}
+napi_value BatchSetGraph(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  bool on = false;
+  napi_get_value_bool(env, a.argv[1], &on);
+  PF_CALL(pf_batch_set_graph((pf_batch*)handle(env, a.argv[0]), on ? 1 : 0));
+  return nullptr;
+}
+napi_value BatchAutotune(napi_env env, napi_callback_info info) {
+  return autotune(env, info, [](void* h, double dt, const uint32_t* c, uint32_t n, uint32_t st, uint32_t* b,
+                                float* ms) { return pf_batch_autotune((pf_batch*)h, dt, c, n, st, b, ms); });
+}
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_batch_destroy((pf_batch*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
+// ---------------------------------------------------------------- served form
+napi_value request_info(napi_env env, const pf_request_info& r) {
+  static const char* status[] = {"pending", "launched", "done", "failed"};
+  napi_value obj, st;
+  napi_create_object(env, &obj);
+  set(env, obj, "scene", number(env, r.scene));
+  napi_create_string_utf8(env, status[r.status & 3u], NAPI_AUTO_LENGTH, &st);
+  set(env, obj, "status", st);
+  set(env, obj, "launchScenes", number(env, r.launch_scenes));
+  set(env, obj, "tSubmit", number(env, r.t_submit));
+  set(env, obj, "tLaunch", number(env, r.t_launch));
+  set(env, obj, "tDone", number(env, r.t_done));
+  return obj;
+}
+
+// (ctx, {hbmBudget, maxBatchScenes, maxBatchParticles, slots})
+napi_value ServerCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_server_options o{};
+  napi_value opts = a.argv[1];
+  if (!is_nullish(env, opts)) {
+    o.hbm_budget_bytes = (uint64_t)num(env, prop(env, opts, "hbmBudget"), 0);
+    o.max_batch_scenes = u32(env, prop(env, opts, "maxBatchScenes"), 0);
+    o.max_batch_particles = (uint64_t)num(env, prop(env, opts, "maxBatchParticles"), 0);
+    o.slots = u32(env, prop(env, opts, "slots"), 0);
+  }
+  pf_server* s = nullptr;
+  PF_CALL(pf_server_create((pf_context*)handle(env, a.argv[0]), &o, &s));
+  return make_handle(env, s);
+}
+napi_value ServerAddScene(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint32_t id = 0;
+  PF_CALL(pf_server_add_scene((pf_server*)handle(env, a.argv[0]), (pf_loop*)handle(env, a.argv[1]), &id));
+  return number(env, id);
+}
+napi_value ServerRemoveScene(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_server_remove_scene((pf_server*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value ServerSubmit(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  uint64_t t = 0;
+  PF_CALL(pf_server_submit((pf_server*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2]), &t));
+  return number(env, (double)t);
+}
+
+// Blocks in a worker thread on pf_server_wait (ticket) or pf_server_drain
+// (ticket < 0) and settles the Promise on the JS thread.
+struct WaitWork {
+  pf_server* srv;
+  double ticket;
+  napi_deferred deferred;
+  napi_async_work work;
+  int rc;
+  pf_request_info info;
+  std::string error;
+};
+
+napi_value server_wait_promise(napi_env env, pf_server* srv, double ticket) {
+  auto* w = new WaitWork{srv, ticket, nullptr, nullptr, PF_OK, {}, {}};
+  napi_value promise, name;
+  CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
+  napi_create_string_utf8(env, "pf_server_wait", NAPI_AUTO_LENGTH, &name);
+  CHECK_NAPI(napi_create_async_work(
+      env, nullptr, name,
+      [](napi_env, void* d) {
+        auto* w = static_cast<WaitWork*>(d);
+        w->rc = w->ticket < 0 ? pf_server_drain(w->srv) : pf_server_wait(w->srv, (uint64_t)w->ticket, &w->info);
+        if (w->rc != PF_OK) w->error = pf_last_error();
+      },
+      [](napi_env env, napi_status, void* d) {
+        auto* w = static_cast<WaitWork*>(d);
+        if (w->rc == PF_OK) {
+          napi_value v;
+          if (w->ticket < 0) napi_get_undefined(env, &v);
+          else v = request_info(env, w->info);
+          napi_resolve_deferred(env, w->deferred, v);
+        } else {
+          napi_value msg, err;
+          napi_create_string_utf8(env, w->error.c_str(), NAPI_AUTO_LENGTH, &msg);
+          napi_create_error(env, nullptr, msg, &err);
+          napi_reject_deferred(env, w->deferred, err);
+        }
+        napi_delete_async_work(env, w->work);
+        delete w;
+      },
+      w, &w->work));
+  CHECK_NAPI(napi_queue_async_work(env, w->work));
+  return promise;
+}
+napi_value ServerWait(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  return server_wait_promise(env, (pf_server*)handle(env, a.argv[0]), num(env, a.argv[1]));
+}
+napi_value ServerDrain(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  return server_wait_promise(env, (pf_server*)handle(env, a.argv[0]), -1.0);
+}
+napi_value ServerStats(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_server_stats st{};
+  PF_CALL(pf_server_stats_get((pf_server*)handle(env, a.argv[0]), &st));
+  napi_value obj;
+  napi_create_object(env, &obj);
+#define PF_STAT(k, v) set(env, obj, k, number(env, (double)(v)))
+  PF_STAT("submitted", st.submitted); PF_STAT("completed", st.completed); PF_STAT("failed", st.failed);
+  PF_STAT("launches", st.launches); PF_STAT("sceneSteps", st.scene_steps);
+  PF_STAT("particleSubsteps", st.particle_substeps); PF_STAT("evictions", st.evictions);
+  PF_STAT("restores", st.restores); PF_STAT("residentBytes", st.resident_bytes);
+  PF_STAT("peakResidentBytes", st.peak_resident_bytes); PF_STAT("hbmBudget", st.hbm_budget_bytes);
+  PF_STAT("scenes", st.scenes); PF_STAT("residentScenes", st.resident_scenes);
+  PF_STAT("maxLaunchScenes", st.max_launch_scenes);
+#undef PF_STAT
+  return obj;
+}
+napi_value ServerEvictScene(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_server_evict_scene((pf_server*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value ServerDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_server_destroy((pf_server*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
 // ---------------------------------------------------------------- layout
 napi_value LayoutFillBox(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
@@ -766,6 +1009,12 @@ napi_value Init(napi_env env, napi_value exports) {
       {"loopSynchronize", LoopSynchronize}, {"loopReadbackOverflow", LoopReadbackOverflow},
       {"loopSetSubsteps", LoopSetSubsteps}, {"loopSetGravity", LoopSetGravity}, {"loopSetGraph", LoopSetGraph},
       {"loopInfo", LoopInfo}, {"loopGrid", LoopGrid}, {"loopDestroy", LoopDestroy},
+      {"loopAutotune", LoopAutotune}, {"particlesDeviceBytes", ParticlesDeviceBytes},
+      {"batchCreate", BatchCreate}, {"batchStepAsync", BatchStepAsync}, {"batchSetGraph", BatchSetGraph},
+      {"batchAutotune", BatchAutotune}, {"batchDestroy", BatchDestroy},
+      {"serverCreate", ServerCreate}, {"serverAddScene", ServerAddScene}, {"serverRemoveScene", ServerRemoveScene},
+      {"serverSubmit", ServerSubmit}, {"serverWait", ServerWait}, {"serverDrain", ServerDrain},
+      {"serverStats", ServerStats}, {"serverEvictScene", ServerEvictScene}, {"serverDestroy", ServerDestroy},
       {"layoutFillBox", LayoutFillBox}, {"layoutFitBudget", LayoutFitBudget}, {"abiVersion", AbiVersion},
   };
   for (const Fn& f : fns) {

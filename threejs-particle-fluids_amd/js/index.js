@@ -36,9 +36,10 @@ function getDefaultContext() {
 }
 
 class ParticleSystem {
-  constructor(capacity, particleRadius, context = getDefaultContext()) {
+  /** options.evictable: keep the whole scene in one device arena so a Server can evict it. */
+  constructor(capacity, particleRadius, context = getDefaultContext(), options = {}) {
     this.context = context;
-    this.handle = native.particlesCreate(context.handle, capacity, particleRadius);
+    this.handle = native.particlesCreate(context.handle, capacity, particleRadius, !!options.evictable);
     this.capacity = capacity;
     this.particleRadius = particleRadius;
     this.isDisposed = false;
@@ -70,6 +71,10 @@ class ParticleSystem {
   }
   async readback() {
     return native.particlesReadback(this.handle);
+  }
+  /** Device bytes of an evictable scene's arena (0 otherwise). */
+  deviceBytes() {
+    return native.particlesDeviceBytes(this.handle);
   }
   get disposed() {
     return this.isDisposed;
@@ -304,6 +309,10 @@ class SimLoop {
   useGraph(enabled = true) {
     native.loopSetGraph(this.handle, !!enabled);
   }
+  /** Times each tile-union LDS capacity on this loop's workload and keeps the fastest. */
+  autotune(dt, candidates = TILE_UNION_CANDIDATES, steps = 2) {
+    return native.loopAutotune(this.handle, dt, candidates, steps);
+  }
   get hashGrid() {
     return new HashGrid(this.particles, null, native.loopGrid(this.handle));
   }
@@ -312,6 +321,74 @@ class SimLoop {
   }
   dispose() {
     if (this.handle) native.loopDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+const TILE_UNION_CANDIDATES = [768, 1024, 1280, 1536, 2048];
+
+/** Many independent SimLoops advanced by one launch sequence (no reference counterpart). */
+class Batch {
+  constructor(loops, context = getDefaultContext()) {
+    this.context = context;
+    this.loops = loops;
+    this.handle = native.batchCreate(context.handle, loops.map((l) => l.handle));
+  }
+  /** Resolves when every scene's step has run. */
+  step(dt) {
+    return native.batchStepAsync(this.context.handle, this.handle, dt);
+  }
+  useGraph(enabled = true) {
+    native.batchSetGraph(this.handle, !!enabled);
+  }
+  autotune(dt, candidates = TILE_UNION_CANDIDATES, steps = 2) {
+    return native.batchAutotune(this.handle, dt, candidates, steps);
+  }
+  dispose() {
+    if (this.handle) native.batchDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/**
+ * The served form (no reference counterpart): step requests for live scenes
+ * queue as they arrive and run in multi-scene launches as slots and memory
+ * free up; scenes stay resident under an HBM byte budget and idle scenes are
+ * evicted to host memory. Scenes need particles created with {evictable: true}.
+ */
+class Server {
+  constructor(options = {}, context = getDefaultContext()) {
+    this.context = context;
+    this.handle = native.serverCreate(context.handle, options);
+  }
+  addScene(loop) {
+    return native.serverAddScene(this.handle, loop.handle);
+  }
+  removeScene(id) {
+    native.serverRemoveScene(this.handle, id);
+  }
+  /** Queues one step(dt); returns a Promise of the request's times and status. */
+  step(id, dt) {
+    const ticket = native.serverSubmit(this.handle, id, dt);
+    return native.serverWait(this.handle, ticket);
+  }
+  submit(id, dt) {
+    return native.serverSubmit(this.handle, id, dt);
+  }
+  wait(ticket) {
+    return native.serverWait(this.handle, ticket);
+  }
+  drain() {
+    return native.serverDrain(this.handle);
+  }
+  evictScene(id) {
+    native.serverEvictScene(this.handle, id);
+  }
+  get stats() {
+    return native.serverStats(this.handle);
+  }
+  dispose() {
+    if (this.handle) native.serverDestroy(this.handle);
     this.handle = null;
   }
 }
@@ -489,5 +566,7 @@ module.exports = {
   SoftbodySystem,
   SimLoop,
   Simulation,
+  Batch,
+  Server,
   getDefaultContext,
 };

@@ -508,6 +508,17 @@ class SoftbodySystem:
 SimLoopOverflow = namedtuple("SimLoopOverflow", "positions velocities grid contacts neighbors impulses long_bucket")
 
 
+TILE_UNION_CANDIDATES = (768, 1024, 1280, 1536, 2048)
+
+
+def _autotune(fn, handle, dt, candidates, steps):
+    cand = np.ascontiguousarray(candidates, np.uint32)
+    ms = np.zeros(len(cand), np.float32)
+    best = C.c_uint32()
+    call(fn, handle, float(dt), _ptr(cand, u32p), len(cand), int(steps), C.byref(best), _ptr(ms, f32p))
+    return best.value, {int(c): float(t) for c, t in zip(cand, ms)}
+
+
 class SimLoop:
     """src/core/loop.ts:92-324."""
 
@@ -568,6 +579,11 @@ class SimLoop:
     def useGraph(self, enabled=True):
         call("pf_loop_set_graph", self.handle, 1 if enabled else 0)
 
+    def autotune(self, dt, candidates=TILE_UNION_CANDIDATES, steps=2):
+        """Time each tile-union LDS capacity on this loop's workload and keep
+        the fastest; returns (best, {capacity: ms per step}). Advances the scene."""
+        return _autotune("pf_loop_autotune", self.handle, dt, candidates, steps)
+
     @property
     def hashGrid(self):
         if self._grid is None:
@@ -598,6 +614,11 @@ class Batch:
 
     def useGraph(self, enabled=True):
         call("pf_batch_set_graph", self.handle, 1 if enabled else 0)
+
+    def autotune(self, dt, candidates=TILE_UNION_CANDIDATES, steps=2):
+        """Time each tile-union LDS capacity on the batch's workload and keep
+        the fastest; returns (best, {capacity: ms per step}). Advances the scenes."""
+        return _autotune("pf_batch_autotune", self.handle, dt, candidates, steps)
 
     def dispose(self):
         if self.handle:
