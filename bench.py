@@ -26,9 +26,10 @@ sys.path.insert(0, os.path.join(ROOT, "threejs-particle-fluids_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per particle per launch of each kernel, by the byte model
-# in DESIGN.md (FP32 vec3 = 12 B, scalar = 4 B; neighbour gathers, neighbour
-# lists and collider multipliers count 0; per-bucket arrays 4 B a touch).
+# Algorithmic bytes per particle per launch of each kernel: compulsory state
+# traffic between neighbour barriers (FP32 vec3 = 12 B, scalar = 4 B;
+# neighbour gathers, neighbour lists and collider multipliers count 0;
+# per-bucket arrays 4 B a touch).
 KERNEL_BYTES = {
     "hash_predict": 32,        # r x, v; w key, rank
     "place": 24,               # r key, rank, id; w src, id, key
@@ -250,7 +251,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=2)
     ap.add_argument("--iterations", type=int, default=4)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batched form: keep the default tile-union capacity")
+    ap.add_argument("--autotune-plain", action="store_true",
+                    help="tune the single scene's tile-union capacity on a twin (the default 2048 won its sweep)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-baseline-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -271,7 +274,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (scenes shard "
-                                   "across GPUs without exchange, see DESIGN.md)"}))
+                                   "across GPUs without exchange; one scene per GPU)"}))
         sys.exit(2)
 
     import particle_fluids as pf
@@ -286,7 +289,7 @@ def main():
     n = sc["ps"].capacity
     H = loop.info()["hashTableSize"]
     tuned = None
-    if not args.no_autotune:
+    if args.autotune_plain:
         # tile-union LDS capacity, tuned on a twin of the scene at the same
         # point of its run, so the measured scene keeps its configured window
         twin = scenes.build_dam_break(args.budget, substeps=args.substeps, iterations=args.iterations, use_graph=True)
@@ -371,8 +374,7 @@ def main():
         batched = run_batched(args, ctx)
     served = None
     if args.serve_scenes > 0:
-        tu = batched["autotune"]["tile_union_capacity"] if batched and batched.get("autotune") else None
-        served = run_served(args, ctx, tu)
+        served = run_served(args, ctx)
 
     line = {
         "metric": "particle-substeps/s at 1M PBF",
@@ -401,17 +403,31 @@ def main():
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration_mib:
-        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings
-        # calibrate the counters against the bytes it must move.
-        import torch
+        calibration_copy(args.copy_calibration_mib << 20)
 
-        nbytes = args.copy_calibration_mib << 20
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"[bench] calibration copy: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr, flush=True)
+
+def calibration_copy(nbytes):
+    """A fill and a device-to-device copy of known size through the HIP
+    runtime (its fill and copy kernels): their FETCH_SIZE / WRITE_SIZE
+    readings calibrate the HBM counters against the bytes they must move."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [vp]
+    src, dst = vp(), vp()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise RuntimeError("calibration: hipMalloc failed")
+    rc = hip.hipMemset(src, 1, nbytes) or hip.hipMemcpy(dst, src, nbytes, 3) or hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if rc:
+        raise RuntimeError(f"calibration: HIP error {rc}")
+    print(f"[bench] calibration: fill of {nbytes} bytes, then copy of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

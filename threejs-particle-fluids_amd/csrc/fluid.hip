@@ -430,8 +430,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       for (uint32_t k0 = 0; k0 < count; k0 += 4) {
         const uint32_t w0 = nl16[(size_t)(k0 >> 1) * n + s];
         const uint32_t w1 = nl16[(size_t)min((k0 >> 1) + 1u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 1)) * n + s];
-        const uint32_t idx[4] = {w0 & (PF_TU_MAX - 1), (w0 >> 16) & (PF_TU_MAX - 1), w1 & (PF_TU_MAX - 1),
-                                 (w1 >> 16) & (PF_TU_MAX - 1)};
+        // entries past the count may hold stale indices: keep them inside the union
+        const uint32_t ul = U ? U - 1u : 0u;
+        const uint32_t idx[4] = {min(w0 & (PF_TU_MAX - 1), ul), min((w0 >> 16) & (PF_TU_MAX - 1), ul),
+                                 min(w1 & (PF_TU_MAX - 1), ul), min((w1 >> 16) & (PF_TU_MAX - 1), ul)};
         float4 xj[4];
         float vol[4];
 #pragma unroll
@@ -451,6 +453,55 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   }
   if (tid == 0) F.tu_size[tile] = U;
 }
+
+// ---------------------------------------------------------------------------
+// Neighbour-list cursors. Both walk the list four entries at a time with the
+// next four entries' list words already loaded; rows are clamped so those
+// loads stay in bounds past the end of the list, and entries past the count
+// resolve to a harmless index whose terms the callers discard.
+// ---------------------------------------------------------------------------
+struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
+  const uint32_t* __restrict__ col;
+  size_t n;
+  uint32_t w0, w1;
+  uint32_t lim;  // last union index: entries past the count keep LDS reads inside the union
+  __device__ __forceinline__ Nl16Cursor(const uint32_t* nl16, uint32_t nn, uint32_t s, uint32_t U)
+      : col(nl16 + s), n(nn), lim(U ? U - 1u : 0u) {
+    w0 = col[0];
+    w1 = col[n];
+  }
+  // Union indices of entries k0..k0+3; prefetches those of k0+4..k0+7.
+  __device__ __forceinline__ void next(uint32_t k0, uint32_t idx[4]) {
+    const uint32_t c0 = w0, c1 = w1;
+    const uint32_t r = min((k0 >> 1) + 2u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 2));
+    w0 = col[(size_t)r * n];
+    w1 = col[(size_t)(r + 1) * n];
+    idx[0] = min(c0 & (PF_TU_MAX - 1), lim);
+    idx[1] = min((c0 >> 16) & (PF_TU_MAX - 1), lim);
+    idx[2] = min(c1 & (PF_TU_MAX - 1), lim);
+    idx[3] = min((c1 >> 16) & (PF_TU_MAX - 1), lim);
+  }
+};
+
+struct NlCursor {  // global-gather tile: sorted slots
+  const uint32_t* __restrict__ col;
+  size_t n;
+  uint32_t self, cnt;
+  uint32_t w[4];
+  __device__ __forceinline__ NlCursor(const uint32_t* nl, uint32_t nn, uint32_t s, uint32_t count)
+      : col(nl + s), n(nn), self(s), cnt(count) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)u * n];
+  }
+  // Slots of entries k0..k0+3 (the own slot past the count).
+  __device__ __forceinline__ void next(uint32_t k0, uint32_t jj[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? w[u] : self;
+    const uint32_t r = min(k0 + 4u, (uint32_t)(PF_MAX_NEIGHBORS - 4));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)(r + u) * n];
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Surface tension (pairwise, gathered) + adhesion + impulse apply
@@ -595,54 +646,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour-list cursors. Both walk the list four entries at a time with the
-// next four entries' list words already loaded; rows are clamped so those
-// loads stay in bounds past the end of the list, and entries past the count
-// resolve to a harmless index whose terms the callers discard.
-// ---------------------------------------------------------------------------
-struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
-  const uint32_t* __restrict__ col;
-  size_t n;
-  uint32_t w0, w1;
-  __device__ __forceinline__ Nl16Cursor(const uint32_t* nl16, uint32_t nn, uint32_t s)
-      : col(nl16 + s), n(nn) {
-    w0 = col[0];
-    w1 = col[n];
-  }
-  // Union indices of entries k0..k0+3; prefetches those of k0+4..k0+7.
-  __device__ __forceinline__ void next(uint32_t k0, uint32_t idx[4]) {
-    const uint32_t c0 = w0, c1 = w1;
-    const uint32_t r = min((k0 >> 1) + 2u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 2));
-    w0 = col[(size_t)r * n];
-    w1 = col[(size_t)(r + 1) * n];
-    idx[0] = c0 & (PF_TU_MAX - 1);
-    idx[1] = (c0 >> 16) & (PF_TU_MAX - 1);
-    idx[2] = c1 & (PF_TU_MAX - 1);
-    idx[3] = (c1 >> 16) & (PF_TU_MAX - 1);
-  }
-};
-
-struct NlCursor {  // global-gather tile: sorted slots
-  const uint32_t* __restrict__ col;
-  size_t n;
-  uint32_t self, cnt;
-  uint32_t w[4];
-  __device__ __forceinline__ NlCursor(const uint32_t* nl, uint32_t nn, uint32_t s, uint32_t count)
-      : col(nl + s), n(nn), self(s), cnt(count) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)u * n];
-  }
-  // Slots of entries k0..k0+3 (the own slot past the count).
-  __device__ __forceinline__ void next(uint32_t k0, uint32_t jj[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) jj[u] = (k0 + u < cnt) ? w[u] : self;
-    const uint32_t r = min(k0 + 4u, (uint32_t)(PF_MAX_NEIGHBORS - 4));
-#pragma unroll
-    for (int u = 0; u < 4; ++u) w[u] = col[(size_t)(r + u) * n];
-  }
-};
-
-// ---------------------------------------------------------------------------
 // Density and lambda; lambda is written to xs[c].w for the correction pass
 // ---------------------------------------------------------------------------
 
@@ -708,7 +711,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
   float3 gs = make_float3(0.f, 0.f, 0.f);
   // Four neighbours' loads in flight, then the sums in list order.
   if (staged) {
-    Nl16Cursor cur(F.nl16, n, s);
+    Nl16Cursor cur(F.nl16, n, s, U);
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
       uint32_t idx[4];
       cur.next(k0, idx);
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
     float3 sum = make_float3(0.f, 0.f, 0.f);
     if (staged) {
-      Nl16Cursor cur(F.nl16, n, s);
+      Nl16Cursor cur(F.nl16, n, s, U);
       for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
         uint32_t idx[4];
         cur.next(k0, idx);
@@ -1082,7 +1085,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
   float3 curl = make_float3(0.f, 0.f, 0.f);
   float3 sm = make_float3(0.f, 0.f, 0.f);
   if (staged) {
-    Nl16Cursor cur(F.nl16, n, s);
+    Nl16Cursor cur(F.nl16, n, s, U);
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
       uint32_t idx[4];
       cur.next(k0, idx);
@@ -1181,7 +1184,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
       const uint32_t cnt = F.nl_count[s];
       float3 eta = make_float3(0.f, 0.f, 0.f);
       if (staged) {
-        Nl16Cursor cur(F.nl16, n, s);
+        Nl16Cursor cur(F.nl16, n, s, U);
         for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
           uint32_t idx[4];
           cur.next(k0, idx);
@@ -1277,10 +1280,12 @@ void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, int c, bool surface_tension, bool adhesion) {
   if (!ntiles) return;
-  if (surface_tension && adhesion) hipLaunchKernelGGL((k_impulses<true, true>), PF_GRID, sc, tl, ntiles, fi, p, c);
-  else if (surface_tension) hipLaunchKernelGGL((k_impulses<true, false>), PF_GRID, sc, tl, ntiles, fi, p, c);
-  else if (adhesion) hipLaunchKernelGGL((k_impulses<false, true>), PF_GRID, sc, tl, ntiles, fi, p, c);
-  else hipLaunchKernelGGL((k_impulses<false, false>), PF_GRID, sc, tl, ntiles, fi, p, c);
+#define PF_IK(a, b) hipLaunchKernelGGL((k_impulses<a, b>), PF_GRID, sc, tl, ntiles, fi, p, c)
+  if (surface_tension && adhesion) PF_IK(true, true);
+  else if (surface_tension) PF_IK(true, false);
+  else if (adhesion) PF_IK(false, true);
+  else PF_IK(false, false);
+#undef PF_IK
 }
 
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
