@@ -540,6 +540,56 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
   return (r * 2.0f > F.h && r <= F.h) ? root * F.adh_coef : 0.0f;
 }
 
+// Surface tension and adhesion of one particle in a table below 64 buckets
+// (global slots, one neighbour at a time; a partner's truncated list is
+// searched directly).
+static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
+                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
+                                     int tk[3]) {
+  const float4* __restrict__ xs = S.xs[c];
+  const uint32_t n = S.n;
+  const uint32_t* __restrict__ nl = F.nl;
+  const uint32_t fend = F.start + F.count;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const uint32_t j = nl[(size_t)k * n + s];
+    if (st_on) {
+      const uint32_t idj = S.id[p][j];
+      if (idj > id && idj < fend) {
+        float3 dv;
+        if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) {
+          tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
+          tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
+          tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
+        }
+      } else if (idj < id && idj >= F.start) {
+        bool listed = F.nl_last[j] == 0xffffffffu;
+        for (uint32_t m = 0; m < PF_MAX_NEIGHBORS && !listed; ++m) listed |= (nl[(size_t)m * n + j] == s);
+        float3 dv;
+        if (listed && pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
+          tk[0] = pf_wrap_add(tk[0], pf_ticks(-dv.x, F.imp_scale));
+          tk[1] = pf_wrap_add(tk[1], pf_ticks(-dv.y, F.imp_scale));
+          tk[2] = pf_wrap_add(tk[2], pf_ticks(-dv.z, F.imp_scale));
+        }
+      }
+    }
+    if (adh_on) {
+      const float vol = S.bvol[p][j];
+      if (vol > 0.0f) {
+        const float3 off = pf_sub3(xi, pf_xyz(xs[j]));
+        const float r = pf_len3(off);
+        if (!(r < 1e-20f)) {
+          const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
+          const float ir = 1.0f / r;
+          const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
+          tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
+          tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
+          tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
+        }
+      }
+    }
+  }
+}
+
 template <bool ST, bool ADH>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
@@ -577,59 +627,86 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   const uint32_t fend = F.start + F.count;
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const uint32_t j = (U == PF_TU_GLOBAL)
-                           ? nl[(size_t)k * n + s]
-                           : uslots[(F.nl16[(size_t)(k >> 1) * n + s] >> ((k & 1u) * 16)) & (PF_TU_MAX - 1)];
-    if (st_on) {
-      const uint32_t idj = S.id[p][j];
-      if (idj > id && idj < fend) {
-        float3 dv;
-        if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) {
-          tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
+  if (S.H < 64u) {
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, tk);
+  } else {
+    // Four neighbours at a time, every gather of the four issued before the
+    // arithmetic (the pass is latency-bound). Each pair is evaluated from its
+    // lower id's side: operands swapped and the change negated on the upper
+    // side. The fixed-point sums do not depend on the order.
+    const bool staged = U != PF_TU_GLOBAL;
+    const int qx = pf_query_coord(xi.x, S.ox, S.cell), qy = pf_query_coord(xi.y, S.oy, S.cell),
+              qz = pf_query_coord(xi.z, S.oz, S.cell);
+    Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
+    NlCursor cg(nl, n, s, cnt);
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t jj[4];
+      if (staged) {
+        uint32_t idx[4];
+        c16.next(k0, idx);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
+      } else {
+        cg.next(k0, jj);
+      }
+      uint32_t idj[4], lastj[4];
+      float4 xj[4], nj[4];
+      float rj[4], vol[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        idj[u] = S.id[p][jj[u]];
+        xj[u] = xs[jj[u]];
+        if (st_on) {
+          nj[u] = F.normal[jj[u]];
+          rj[u] = S.rho[p][jj[u]];
+          lastj[u] = F.nl_last[jj[u]];
         }
-      } else if (idj < id && idj >= F.start) {
-        // The pair belongs to j's thread in the reference; it exists there iff
-        // this slot is in j's (possibly truncated) list. Lists keep their first
-        // 64 neighbours in walk order, so compare this slot's walk key in j's
-        // walk with the key of j's last kept neighbour.
-        const uint32_t last = F.nl_last[j];
-        bool listed = last == 0xffffffffu;
-        if (!listed) {
-          if (S.H < 64u) {
-            for (uint32_t m = 0; m < PF_MAX_NEIGHBORS; ++m) listed |= (nl[(size_t)m * n + j] == s);
-          } else {
-            const float4 xj4 = xs[j];
-            const int ddx = pf_query_coord(xi.x, S.ox, S.cell) - pf_query_coord(xj4.x, S.ox, S.cell);
-            const int ddy = pf_query_coord(xi.y, S.oy, S.cell) - pf_query_coord(xj4.y, S.oy, S.cell);
-            const int ddz = pf_query_coord(xi.z, S.oz, S.cell) - pf_query_coord(xj4.z, S.oz, S.cell);
+        if (adh_on) vol[u] = S.bvol[p][jj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool in = k0 + u < cnt;
+        const float3 xjv = pf_xyz(xj[u]);
+        if (st_on) {
+          const bool lower = in && idj[u] > id && idj[u] < fend;
+          bool upper = in && idj[u] < id && idj[u] >= F.start;
+          if (upper && lastj[u] != 0xffffffffu) {
+            // The pair exists in the reference iff this slot is in j's
+            // truncated list: compare this slot's walk key in j's walk with
+            // the key of j's last kept neighbour.
+            const int ddx = qx - pf_query_coord(xjv.x, S.ox, S.cell);
+            const int ddy = qy - pf_query_coord(xjv.y, S.oy, S.cell);
+            const int ddz = qz - pf_query_coord(xjv.z, S.oz, S.cell);
             const uint32_t key = ((uint32_t)((ddz + 1) * 9 + (ddy + 1) * 3 + (ddx + 1)) << 27) | s;
-            listed = key <= last;
+            upper = key <= lastj[u];
+          }
+          if (lower || upper) {
+            const float3 njv = pf_xyz(nj[u]);
+            float3 dv;
+            const bool ok = pair_dv(F, S.dt, sel3(lower, xi, xjv), sel3(lower, xjv, xi), sel3(lower, ni, njv),
+                                    sel3(lower, njv, ni), lower ? rhoi : rj[u], lower ? rj[u] : rhoi, &dv);
+            if (ok) {
+              const float sg = lower ? 1.0f : -1.0f;
+              tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x * sg, F.imp_scale));
+              tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y * sg, F.imp_scale));
+              tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z * sg, F.imp_scale));
+            }
           }
         }
-        float3 dv;
-        if (listed &&
-            pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
-          tk[0] = pf_wrap_add(tk[0], pf_ticks(-dv.x, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks(-dv.y, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks(-dv.z, F.imp_scale));
-        }
-      }
-    }
-    if (adh_on) {
-      const float vol = S.bvol[p][j];
-      if (vol > 0.0f) {
-        const float3 off = pf_sub3(xi, pf_xyz(xs[j]));
-        const float r = pf_len3(off);
-        if (!(r < 1e-20f)) {
-          const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
-          const float ir = 1.0f / r;
-          const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
-          tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
+        if (adh_on && in) {
+          const float vj = vol[u];
+          if (vj > 0.0f) {
+            const float3 off = pf_sub3(xi, xjv);
+            const float r = pf_len3(off);
+            if (!(r < 1e-20f)) {
+              const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vj)) * adhesion_spline(F, r));
+              const float ir = 1.0f / r;
+              const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
+              tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
+              tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
+              tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
+            }
+          }
         }
       }
     }
