@@ -83,6 +83,28 @@ static __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint
   return before + inc - v;
 }
 
+// Block-wide exclusive max of v (0 for thread 0); sh holds PF_BLOCK/64 words.
+static __device__ __forceinline__ uint32_t block_exclusive_max(uint32_t v, uint32_t* sh) {
+  const uint32_t lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d);
+    if (lane >= (uint32_t)d) inc = max(inc, y);
+  }
+  uint32_t exc = __shfl_up(inc, 1);
+  if (lane == 0) exc = 0u;
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (int w = 0; w < PF_BLOCK / 64; ++w)
+    if (w < wave) before = max(before, sh[w]);
+  __syncthreads();
+  return max(before, exc);
+}
+
 // Hash-set slot of a bucket (Fibonacci hashing into PF_TU_HASH slots).
 static __device__ __forceinline__ uint32_t tu_hash(uint32_t b) {
   return (b * 0x9E3779B1u) >> (32 - 10);
@@ -241,12 +263,23 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   __syncthreads();
   const bool leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] ||
                                  q_sh[1][tid - 1] != q[1] || q_sh[2][tid - 1] != q[2]);
+  // The leaders' cells, in lane order (h_start is free until the layout);
+  // then every thread inserts (cell, stencil) pairs, 27 per leader.
+  uint32_t nlead;
+  const uint32_t li = block_exclusive_scan(leader ? 1u : 0u, ctl + 4, &nlead);
   if (leader) {
+    h_start[3 * li + 0] = (uint32_t)q[0];
+    h_start[3 * li + 1] = (uint32_t)q[1];
+    h_start[3 * li + 2] = (uint32_t)q[2];
+  }
+  __syncthreads();
+  {
     volatile uint32_t* give_up = &ctl[1];
-    for (int d = 0; d < 27 && !*give_up; ++d) {
-      const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
-      const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
-                                   (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+    for (uint32_t task = tid; task < nlead * 27u && !*give_up; task += PF_BLOCK) {
+      const uint32_t ci = task / 27u, d = task - ci * 27u;
+      const uint32_t dx = d % 3u - 1u, dy = (d / 3u) % 3u - 1u, dz = d / 9u - 1u;
+      const uint32_t b = pf_morton((int)(h_start[3 * ci + 0] + dx), (int)(h_start[3 * ci + 1] + dy),
+                                   (int)(h_start[3 * ci + 2] + dz)) & S.hash_mask;
       uint32_t h = tu_hash(b);
       bool placed = false;
       for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
@@ -343,21 +376,55 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     return;
   }
 
-  // 3. Stage the union: entry u belongs to the last hash slot whose offset is
-  // not above u.
+  // 3. Stage the union: entry u belongs to the last bucket whose union offset
+  // is not above u. Each bucket marks its offset with its hash slot + 1 and a
+  // chunked prefix max spreads the marks; the marks and the resolved slots
+  // use the union's LDS (4 entries per float4, so 8 per thread fit).
   {
     uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+    uint32_t* mark = reinterpret_cast<uint32_t*>(u_x);
+    const uint32_t u8 = (U + 7u) & ~7u;
+    for (uint32_t u = tid; u < u8; u += PF_BLOCK) mark[u] = 0u;
+    __syncthreads();
+    constexpr int PER = PF_TU_HASH / PF_BLOCK;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const uint32_t h = tid * PER + i;
+      const uint32_t ho = h_off[h];
+      if (ho & 0xffffu) mark[ho >> 16] = h + 1u;
+    }
+    __syncthreads();
+    const uint32_t u0 = tid * 8u;
+    uint32_t mk[8], run = 0;
+    if (u0 < U) {
+      const uint4 a = *reinterpret_cast<const uint4*>(mark + u0);
+      const uint4 b = *reinterpret_cast<const uint4*>(mark + u0 + 4u);
+      const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        run = max(run, m[k]);
+        mk[k] = run;
+      }
+    }
+    const uint32_t carry = block_exclusive_max(run, ctl + 4);
+    if (u0 < U) {
+      uint32_t r[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t h = max(mk[k], carry) - 1u;  // entry 0 opens the first bucket
+        r[k] = h_start[h] + ((u0 + (uint32_t)k) - (h_off[h] >> 16));
+      }
+      *reinterpret_cast<uint4*>(mark + u0) = make_uint4(r[0], r[1], r[2], r[3]);
+      *reinterpret_cast<uint4*>(mark + u0 + 4u) = make_uint4(r[4], r[5], r[6], r[7]);
+    }
+    __syncthreads();
     uint32_t js[PF_TU_PER];
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
-      // fixed-depth search over the PF_TU_HASH offsets
-      uint32_t a = 0;
-#pragma unroll
-      for (uint32_t step = PF_TU_HASH / 2; step; step >>= 1)
-        if ((h_off[a + step] >> 16) <= u) a += step;
-      js[i] = u < U ? h_start[a] + (u - (h_off[a] >> 16)) : 0u;
+      js[i] = u < U ? mark[u] : 0u;
     }
+    __syncthreads();
     float4 xv[PF_TU_PER];
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xs[js[i]];
