@@ -44,6 +44,18 @@ KERNEL_BYTES = {
     "vorticity": 64,
 }
 SUBSTEP_BYTES_PER_PARTICLE = lambda iters: 336 + 48 * iters  # noqa: E731
+
+# Measured HBM bytes per particle per launch (FETCH_SIZE + WRITE_SIZE, each
+# calibrated against a copy of known size) of the 1M dam break, from the
+# rocprofv3 counter summary profiles/dam_break_1m_hbm_traffic.md.
+HBM_TRAFFIC_B_PER_PARTICLE = {
+    "neighbors": 628.2,
+    "impulses": 390.5,
+    "lambda": 191.5,
+    "delta_walls": 371.9,
+    "velocity": 383.2,
+    "vorticity_friction": 267.0,
+}
 SCAN_BYTES_PER_BUCKET = 12
 
 
@@ -336,8 +348,10 @@ def main():
     roofline = None
     if bpp is not None:
         achieved = bpp * n / avg_s / 1e9
+        tb = HBM_TRAFFIC_B_PER_PARTICLE.get(dominant)
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": round(tb * n) if tb is not None and args.budget == 1 << 20 else None,
                     "bytes_per_launch": bpp * n, "avg_launch_us": round(avg_s * 1e6, 3)}
     sub_bytes = SUBSTEP_BYTES_PER_PARTICLE(args.iterations) * n + SCAN_BYTES_PER_BUCKET * H
     sub_s = seconds / substeps_done

@@ -856,7 +856,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
   // Four neighbours' loads in flight, then the sums in list order.
   if (staged) {
     Nl16Cursor cur(F.nl16, n, s, U);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    // Whole groups of four need no per-entry test; the last partial group does.
+    auto group = [&](uint32_t k0, bool tail) {
       uint32_t idx[4];
       cur.next(k0, idx);
       float4 xj[4];
@@ -867,8 +868,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
         wj[u] = u_w[idx[u]];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) lambda_term(F, k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
-    }
+      for (int u = 0; u < 4; ++u) lambda_term(F, !tail || k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
+    };
+    const uint32_t full = cnt & ~3u;
+    for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+    if (full < cnt) group(full, true);
   } else {
     NlCursor cur(F.nl, n, s, cnt);
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
@@ -982,7 +986,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     float3 sum = make_float3(0.f, 0.f, 0.f);
     if (staged) {
       Nl16Cursor cur(F.nl16, n, s, U);
-      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      // Whole groups of four need no per-entry test; the last partial group does.
+      auto group = [&](uint32_t k0, bool tail) {
         uint32_t idx[4];
         cur.next(k0, idx);
         float4 xj[4];
@@ -994,10 +999,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const float3 g = delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
-          if (react && k0 + u < cnt) reaction_add(S, F, p, uslots[idx[u]], li, g);
+          const bool ok = !tail || k0 + u < cnt;
+          const float3 g = delta_term(F, ok, xn, li, xj[u], mj[u], sum);
+          if (react && ok) reaction_add(S, F, p, uslots[idx[u]], li, g);
         }
-      }
+      };
+      const uint32_t full = cnt & ~3u;
+      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+      if (full < cnt) group(full, true);
     } else {
       NlCursor cur(F.nl, n, s, cnt);
       for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
