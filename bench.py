@@ -86,7 +86,7 @@ def run_batched(args, ctx):
         batch.step(dt)
     tuned = None
     if not args.no_autotune:
-        best, times = batch.autotune(dt, steps=1)
+        best, times = batch.autotune(dt, steps=2)
         tuned = {"tile_union_capacity": best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
         print(f"[bench] batched autotune: {tuned}", file=sys.stderr, flush=True)
     for L in loops:  # the plain form's graphs, captured before timing
@@ -96,8 +96,7 @@ def run_batched(args, ctx):
     def timed(fn):
         ctx.synchronize()
         t = time.perf_counter()
-        for _ in range(args.batch_steps):
-            fn()
+        fn()
         ctx.synchronize()
         return time.perf_counter() - t
 
@@ -105,10 +104,13 @@ def run_batched(args, ctx):
         for L in loops:
             L.step(dt)
 
-    t_plain = timed(plain_step)
-    t_batch = timed(lambda: batch.step(dt))
-    t_plain2 = timed(plain_step)
-    t_plain = 0.5 * (t_plain + t_plain2)
+    # Alternate one plain round (every scene stepped by its own launches) and
+    # one batched step, so both forms see nearly the same scene states: the
+    # cost of a step changes quickly while the dam breaks collapse.
+    t_plain = t_batch = 0.0
+    for _ in range(args.batch_steps):
+        t_plain += timed(plain_step)
+        t_batch += timed(lambda: batch.step(dt))
     # per-kernel breakdown of one batched step, launched kernel by kernel
     batch.useGraph(False)
     ctx.set_profiling(True)
@@ -125,7 +127,7 @@ def run_batched(args, ctx):
                     f"every other scene + 20 soft cubes of 2197 particles, {args.substeps} substeps x "
                     f"{args.iterations} iterations, dt 1/60",
         "scenes": len(built), "particles": n_total, "fluid_particles": n_fluid,
-        "steps": args.batch_steps,
+        "steps": args.batch_steps, "timing": "alternating plain rounds and batched steps",
         "plain": {"particle_substeps_per_s": round(work / t_plain, 1),
                   "scene_steps_per_s": round(len(built) * args.batch_steps / t_plain, 2),
                   "ms_per_round": round(t_plain / args.batch_steps * 1e3, 3)},

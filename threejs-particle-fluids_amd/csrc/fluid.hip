@@ -494,13 +494,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       // Colour-field normal over the finished list, in list order.
       const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
       float3 nsum = make_float3(0.f, 0.f, 0.f);
-      for (uint32_t k0 = 0; k0 < count; k0 += 4) {
+      // Whole groups of four need no per-entry test; the last partial group does.
+      auto group = [&](uint32_t k0, bool tail) {
         const uint32_t w0 = nl16[(size_t)(k0 >> 1) * n + s];
         const uint32_t w1 = nl16[(size_t)min((k0 >> 1) + 1u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 1)) * n + s];
         // entries past the count may hold stale indices: keep them inside the union
         const uint32_t ul = U ? U - 1u : 0u;
-        const uint32_t idx[4] = {min(w0 & (PF_TU_MAX - 1), ul), min((w0 >> 16) & (PF_TU_MAX - 1), ul),
-                                 min(w1 & (PF_TU_MAX - 1), ul), min((w1 >> 16) & (PF_TU_MAX - 1), ul)};
+        const uint32_t idx[4] = {(uint32_t)min(w0 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w0 >> 16) & (PF_TU_MAX - 1u), ul),
+                                 (uint32_t)min(w1 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w1 >> 16) & (PF_TU_MAX - 1u), ul)};
         float4 xj[4];
         float vol[4];
 #pragma unroll
@@ -511,9 +512,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-          nsum = sel3(k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
+          nsum = sel3(!tail || k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
         }
-      }
+      };
+      const uint32_t full = count & ~3u;
+      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+      if (full < count) group(full, true);
       F.normal[s] = make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
     }
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
@@ -1239,7 +1243,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
   float3 sm = make_float3(0.f, 0.f, 0.f);
   if (staged) {
     Nl16Cursor cur(F.nl16, n, s, U);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    // Whole groups of four need no per-entry test; the last partial group does.
+    auto group = [&](uint32_t k0, bool tail) {
       uint32_t idx[4];
       cur.next(k0, idx);
       float4 xj[4], vj4[4];
@@ -1250,8 +1255,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        velocity_term<VORT, VISC>(F, k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), xj[u].w, curl, sm);
-    }
+        velocity_term<VORT, VISC>(F, !tail || k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), xj[u].w,
+                                  curl, sm);
+    };
+    const uint32_t full = cnt & ~3u;
+    for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+    if (full < cnt) group(full, true);
   } else {
     NlCursor cur(F.nl, n, s, cnt);
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
@@ -1338,7 +1347,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
       float3 eta = make_float3(0.f, 0.f, 0.f);
       if (staged) {
         Nl16Cursor cur(F.nl16, n, s, U);
-        for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        // Whole groups of four in a scene without boundary particles need no
+        // per-entry test; the last partial group, or skipped boundary
+        // neighbours, do.
+        auto group = [&](uint32_t k0, bool tail, bool skips) {
           uint32_t idx[4];
           cur.next(k0, idx);
           float4 xj[4];
@@ -1346,11 +1358,18 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             xj[u] = u_x[idx[u]];
-            ok[u] = (k0 + u < cnt) && !u_skip[idx[u]];
+            ok[u] = (!tail || k0 + u < cnt) && !(skips && u_skip[idx[u]]);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], xj[u].w, eta);
+        };
+        const uint32_t full = cnt & ~3u;
+        if (S.has_boundary) {
+          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
+        } else {
+          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
         }
+        if (full < cnt) group(full, true, S.has_boundary != 0);
       } else {
         NlCursor cur(F.nl, n, s, cnt);
         for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
