@@ -38,6 +38,7 @@ typedef struct pf_softbody_s pf_softbody;
 typedef struct pf_sdf_collider_s pf_sdf_collider;
 typedef struct pf_loop_s pf_loop;
 typedef struct pf_batch_s pf_batch;
+typedef struct pf_viscosity_s pf_viscosity;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* pf_last_error(void);
@@ -173,6 +174,17 @@ int pf_fluid_info(pf_fluid* fluid, double* mass, double* smoothing_radius,
 int pf_fluid_set_tile_union_limit(pf_fluid* fluid, uint32_t max_particles);
 int pf_fluid_destroy(pf_fluid* fluid);
 
+/* ---- ViscositySolver (src/fluids/ViscositySolver.ts:38-131) -----------
+ * Implicit viscosity for thick liquids: each substep solves
+ * (I - dt nu L) v = v0 for the fluid's velocities with `iterations` Jacobi
+ * sweeps over the fluid's neighbour lists. List it directly after its fluid
+ * in pf_loop_create's materials (kind PF_MATERIAL_VISCOSITY). viscosity >= 0
+ * (m^2/s, about 20 for honey); iterations 1..64 (default 12). */
+int pf_viscosity_create(pf_fluid* fluid, double viscosity, uint32_t iterations, pf_viscosity** out);
+/* the `viscosity` setter (ViscositySolver.ts:65-70) */
+int pf_viscosity_set(pf_viscosity* solver, double viscosity);
+int pf_viscosity_destroy(pf_viscosity* solver);
+
 /* ---- PrimitiveSet (src/core/collision/PrimitiveSet.ts:121-518) -----------
  * mu_s / mu_k < 0 take the defaults 0.5 / 0.4 (PrimitiveSet.ts:529-544). */
 int pf_primitives_create(pf_particles* ps, uint32_t capacity, pf_primitives** out);
@@ -247,9 +259,10 @@ int pf_sdf_collider_destroy(pf_sdf_collider* c);
 /* ---- SimLoop (src/core/loop.ts:92-324) ----------------------------------- */
 #define PF_MATERIAL_FLUID 1u
 #define PF_MATERIAL_SOFTBODY 2u
+#define PF_MATERIAL_VISCOSITY 3u
 typedef struct pf_material {
   uint32_t kind;   /* PF_MATERIAL_* */
-  void* handle;    /* pf_fluid* or pf_softbody* */
+  void* handle;    /* pf_fluid*, pf_softbody* or pf_viscosity* */
 } pf_material;
 #define PF_COLLIDER_PRIMITIVES 1u
 #define PF_COLLIDER_SDF 2u

@@ -57,6 +57,8 @@ def lib():
         g("scene_add_fluid").restype = i32
         g("scene_add_prim").argtypes = [vp, u32, u32, P(f64), P(f64), P(f64), P(f64), P(f64)]
         g("scene_add_prim").restype = i32
+        g("scene_add_viscosity").argtypes = [vp, u32, f64, u32]
+        g("scene_add_viscosity").restype = i32
         g("scene_set_prim").argtypes = [vp, u32, u32, u32, P(f64), P(f64), P(f64), P(f64), P(f64)]
         g("scene_set_prim").restype = i32
         g("scene_add_body").argtypes = [vp, u32, u32, f64, P(f32), P(f32)]
@@ -181,6 +183,11 @@ class Scene:
         rc = self._f("scene_add_fluid")(self.h, int(start), int(count), float(rho0), float(spacing), float(h),
                                         float(compliance), eff, float(viscosity or 0), float(vorticity or 0),
                                         float(surface_tension or 0), float(adhesion or 0))
+        assert rc == 0
+
+    def add_viscosity(self, fluid, viscosity, iterations=12):
+        """ViscositySolver after fluid `fluid` (ViscositySolver.ts:49-123)."""
+        rc = self._f("scene_add_viscosity")(self.h, int(fluid), float(viscosity), int(iterations))
         assert rc == 0
 
     def add_prim(self, kind, flags, d0, d1, vel, rot=(0, 0, 0, 1), spin=(0, 0, 0)):

@@ -31,6 +31,10 @@ typedef struct {
   REAL* omlen;      /* [n] */
   REAL* eta;        /* [3n] */
   REAL* dv;         /* [3n] */
+  double h_d;       /* smoothing radius in double (host-side constants) */
+  uint32_t isv_iters;             /* ViscositySolver sweeps, 0: none */
+  REAL isv_nu, isv_c10;           /* viscosity, 10 / h^2 */
+  REAL *isv_rhs, *isv_a, *isv_b;  /* [3n] by particle id, zero outside the fluid */
 } SFX(fluid);
 
 typedef struct {
@@ -478,6 +482,43 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
     if (visc) for (int a = 0; a < 3; ++a) ch[a] = ch[a] + F->dv[3 * (size_t)i + a];
     for (int a = 0; a < 3; ++a) s->v[3 * (size_t)i + a] = s->v[3 * (size_t)i + a] + ch[a];
   }
+}
+
+/* ViscositySolver (ViscositySolver.ts:78-123): seed rhs and the first
+ * iterate with the fluid's velocities, Jacobi sweeps of
+ * (I - dt nu L) v = v0 over the neighbour list, then copy the last iterate
+ * into the velocities. Both iterates are zero outside the fluid, so other
+ * neighbours add weight but no velocity. */
+static void SFX(viscosity_solve)(SFX(scene)* s, SFX(fluid)* F) {
+  uint32_t end = F->start + F->count;
+  REAL* rhs = F->isv_rhs;
+  for (uint32_t i = F->start; i < end; ++i)
+    for (int a = 0; a < 3; ++a) {
+      rhs[3 * (size_t)i + a] = s->v[3 * (size_t)i + a];
+      F->isv_a[3 * (size_t)i + a] = s->v[3 * (size_t)i + a];
+    }
+  REAL rate = (F->isv_nu * s->dt) * F->isv_c10;
+  for (uint32_t k = 0; k < F->isv_iters; ++k) {
+    const REAL* src = (k & 1u) ? F->isv_b : F->isv_a;
+    REAL* dst = (k & 1u) ? F->isv_a : F->isv_b;
+    for (uint32_t i = F->start; i < end; ++i) {
+      const REAL* xi = s->x + 3 * (size_t)i;
+      REAL sum[3] = {0, 0, 0}, weight = 0;
+      for (uint32_t k2 = 0; k2 < F->nlc[i]; ++k2) {
+        uint32_t j = F->nl[(size_t)i * 64 + k2];
+        const REAL* xj = s->x + 3 * (size_t)j;
+        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        REAL w = SFX(poly6)(SFX(dot)(off, off), F->h2, F->poly6) * F->pv;
+        for (int a = 0; a < 3; ++a) sum[a] = sum[a] + src[3 * (size_t)j + a] * w;
+        weight = weight + w;
+      }
+      for (int a = 0; a < 3; ++a)
+        dst[3 * (size_t)i + a] = (rhs[3 * (size_t)i + a] + sum[a] * rate) / (weight * rate + R(1.0));
+    }
+  }
+  const REAL* res = (F->isv_iters & 1u) ? F->isv_b : F->isv_a;
+  for (uint32_t i = F->start; i < end; ++i)
+    for (int a = 0; a < 3; ++a) s->v[3 * (size_t)i + a] = res[3 * (size_t)i + a];
 }
 
 /* --------------------------------------------------------------------- */
@@ -1157,7 +1198,10 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
         s->x[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a];
       }
     }
-    for (uint32_t f = 0; f < s->nfluids; ++f) SFX(velocity_passes)(s, &s->fluid[f]);
+    for (uint32_t f = 0; f < s->nfluids; ++f) {
+      SFX(velocity_passes)(s, &s->fluid[f]);
+      if (s->fluid[f].isv_iters) SFX(viscosity_solve)(s, &s->fluid[f]);
+    }
     if (s->local) {
       SFX(orient_advect)(s);
       SFX(shape_match)(s, 0, 0);

@@ -285,6 +285,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
       fluid_##SUF* F = &s->fluid[f];                                                               \
       free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
       free(F->omega); free(F->omlen); free(F->eta); free(F->dv);                                   \
+      free(F->isv_rhs); free(F->isv_a); free(F->isv_b);                                            \
     }                                                                                              \
     free(s);                                                                                       \
   }                                                                                                \
@@ -322,7 +323,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     double mass = rho0 * spacing * spacing * spacing;                                              \
     F->start = start; F->count = count; F->effects = effects;                                      \
     F->rho0 = (T)rho0; F->pv = (T)(spacing * spacing * spacing); F->mass = (T)mass; F->inv_mass = (T)1 / F->mass;                \
-    F->h = (T)h; F->h2 = (T)(h * h);                                                               \
+    F->h = (T)h; F->h2 = (T)(h * h); F->h_d = h;                                                   \
     F->poly6 = (T)(315.0 / (64.0 * M_PI * pow(h, 9)));                                              \
     F->spiky = (T)(45.0 / (M_PI * pow(h, 6)));                                                      \
     F->compliance = (T)compliance;                                                                 \
@@ -419,6 +420,19 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
       P->vel[a] = (T)(float)vel[a]; P->rot[a] = (T)(float)rot[a];                                  \
     }                                                                                              \
     for (int a = 0; a < 3; ++a) P->spin[a] = (T)(float)spin[a];                                    \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* ViscositySolver listed after fluid f (ViscositySolver.ts:49-58, 112) */                      \
+  int orc_scene_add_viscosity_##SUF(void* p, uint32_t f, double nu, uint32_t iterations) {         \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (f >= s->nfluids || iterations < 1 || iterations > 64) return -1;                          \
+    fluid_##SUF* F = &s->fluid[f];                                                                 \
+    F->isv_iters = iterations;                                                                     \
+    F->isv_nu = (T)nu;                                                                             \
+    F->isv_c10 = (T)(10.0 / (F->h_d * F->h_d));                                                    \
+    F->isv_rhs = (T*)calloc(3 * (size_t)s->n, sizeof(T));                                          \
+    F->isv_a = (T*)calloc(3 * (size_t)s->n, sizeof(T));                                            \
+    F->isv_b = (T*)calloc(3 * (size_t)s->n, sizeof(T));                                            \
     return 0;                                                                                      \
   }                                                                                                \
   /* replace primitive k (a collider moved between steps) */                                      \

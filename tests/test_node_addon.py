@@ -61,3 +61,19 @@ def test_js_server_steps_scenes_like_plain_loops():
     out = json.loads(r.stdout.strip().splitlines()[-1])
     print(f"[node] served {out}", flush=True)
     assert out["same"] and out["rejected"] and out["done"] and out["completed"] == 9
+
+
+@pytest.mark.gpu
+def test_js_thick_fluid_matches_python(tmp_path):
+    out = tmp_path / "pos.bin"
+    r = subprocess.run([NODE, os.path.join(JS, "test", "dam_break.js"), str(out), "2000", "4", "20"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    js_pos = np.fromfile(out, np.float32).reshape(-1, 4)
+    sim = pf.Simulation(particles=2000, container=(-0.5, 0, -0.3, 0.5, 0.8, 0.3), substeps=2, iterations=4)
+    sim.addFluid((-0.5, 0, -0.3, -0.1, 0.5, 0.3), thickness=20.0)
+    for _ in range(4):
+        sim.step(1.0 / 60.0)
+    py_pos = sim.particleSystem.readback().positions
+    assert np.array_equal(js_pos, py_pos)
+    sim.dispose()

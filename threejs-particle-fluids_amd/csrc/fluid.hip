@@ -794,6 +794,103 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
 }
 
 // ---------------------------------------------------------------------------
+// ViscositySolver (ViscositySolver.ts:78-123): Jacobi sweeps of
+// (I - dt nu L) v = v0 over the fluid's neighbour lists, after its velocity
+// passes. The iterates live in slot order for one substep; slots outside the
+// fluid hold zero velocity in both, as the reference's id-indexed buffers do.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(PF_BLOCK) void k_isv_seed(const SceneDev* __restrict__ scenes,
+                                                       const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                       int fi, int p) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || F.isv_iters == 0u) return;
+  if (in_range(S.id[p][s], F.start, F.count)) {
+    const float4 v = S.v[p][s];
+    F.isv_rhs[s] = v;
+    F.isv_a[s] = v;
+  } else {
+    F.isv_a[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    F.isv_b[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Sweep k reads iterate k & 1 and writes the other. A scene of a batch with
+// fewer sweeps copies its iterate through the extra ones.
+__global__ __launch_bounds__(PF_BLOCK) void k_isv_sweep(const SceneDev* __restrict__ scenes,
+                                                        const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                        int fi, int p, uint32_t k) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  const uint32_t n = S.n;
+  if (s >= n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
+  const float4* __restrict__ src = (k & 1u) ? F.isv_b : F.isv_a;
+  float4* __restrict__ dst = (k & 1u) ? F.isv_a : F.isv_b;
+  if (k >= F.isv_iters) {
+    dst[s] = src[s];
+    return;
+  }
+  const float4* __restrict__ X = S.x[p];
+  const float3 xi = pf_xyz(X[s]);
+  const uint32_t cnt = F.nl_count[s];
+  const uint32_t tile = t.first / PF_BLOCK;
+  const uint32_t U = F.tu_size[tile];
+  const bool staged = U != PF_TU_GLOBAL;
+  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+  Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
+  NlCursor cg(F.nl, n, s, cnt);
+  float3 sum = make_float3(0.f, 0.f, 0.f);
+  float weight = 0.0f;
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    uint32_t jj[4];
+    if (staged) {
+      uint32_t idx[4];
+      c16.next(k0, idx);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
+    } else {
+      cg.next(k0, jj);
+    }
+    float4 xj[4], vj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xj[u] = X[jj[u]];
+      vj[u] = src[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+      const float w = pf_poly6(pf_dot3(d, d), F.h2, F.poly6) * F.pv;
+      const bool ok = k0 + u < cnt;
+      sum = sel3(ok, make_float3(sum.x + vj[u].x * w, sum.y + vj[u].y * w, sum.z + vj[u].z * w), sum);
+      weight = ok ? weight + w : weight;
+    }
+  }
+  const float rate = (F.isv_nu * S.dt) * F.isv_c10;
+  const float4 r = F.isv_rhs[s];
+  const float den = weight * rate + 1.0f;
+  dst[s] = make_float4((r.x + sum.x * rate) / den, (r.y + sum.y * rate) / den, (r.z + sum.z * rate) / den, 0.0f);
+}
+
+// The last iterate replaces the fluid's velocities (w kept).
+__global__ __launch_bounds__(PF_BLOCK) void k_isv_apply(const SceneDev* __restrict__ scenes,
+                                                        const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                        int fi, int p, uint32_t sweeps) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
+  const float4 r = ((sweeps & 1u) ? F.isv_b : F.isv_a)[s];
+  const float4 v = S.v[p][s];
+  S.v[p][s] = make_float4(r.x, r.y, r.z, v.w);
+}
+
+// ---------------------------------------------------------------------------
 // Density and lambda; lambda is written to xs[c].w for the correction pass
 // ---------------------------------------------------------------------------
 
@@ -1529,6 +1626,14 @@ void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
     else if (visc) PF_VK(false, true, false);
   }
 #undef PF_VK
+}
+
+void launch_viscosity_solve(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                            int p, uint32_t sweeps) {
+  if (!ntiles || !sweeps) return;
+  hipLaunchKernelGGL(k_isv_seed, PF_GRID, sc, tl, ntiles, fi, p);
+  for (uint32_t k = 0; k < sweeps; ++k) hipLaunchKernelGGL(k_isv_sweep, PF_GRID, sc, tl, ntiles, fi, p, k);
+  hipLaunchKernelGGL(k_isv_apply, PF_GRID, sc, tl, ntiles, fi, p, sweeps);
 }
 
 void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,

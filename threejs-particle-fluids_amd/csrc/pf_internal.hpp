@@ -85,6 +85,13 @@ struct FluidDev {
   float4* omega;       // [n] vorticity (xyz) and |omega| (w)
   float4* dv;          // [n] XSPH velocity change
   int* reaction;       // [3n] fixed-point boundary reaction (only with boundaries)
+  // ViscositySolver listed after this fluid (ViscositySolver.ts:38-123)
+  uint32_t isv_iters;  // Jacobi sweeps per substep, 0: no solver
+  float isv_nu;        // kinematic viscosity
+  float isv_c10;       // 10 / h^2, computed in double
+  float4* isv_rhs;     // [n] this substep's velocities, slot order
+  float4* isv_a;       // [n] iterates (ping, pong); zero outside the fluid
+  float4* isv_b;
 };
 
 // Global shape matching of one scene's soft bodies, in original-id order

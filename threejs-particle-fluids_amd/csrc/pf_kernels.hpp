@@ -37,6 +37,9 @@ void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool vort, bool visc, bool friction, int substep,
                       int touch_valid, uint32_t cap);
+// the ViscositySolver of fluid fi: seed, `sweeps` Jacobi sweeps, apply
+void launch_viscosity_solve(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                            int p, uint32_t sweeps);
 void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,
                      int c, int substep);
 

@@ -195,6 +195,18 @@ struct pf_fluid_s {
   float4* omega = nullptr;
   float4* dv = nullptr;
   int* reaction = nullptr;
+  struct pf_viscosity_s* isv = nullptr;  // attached ViscositySolver (set by the loop)
+  bool disposed = false;
+};
+
+struct pf_viscosity_s {
+  pf_fluid_s* fluid = nullptr;
+  double viscosity = 0;
+  uint32_t iterations = 12;
+  pf_loop_s* loop = nullptr;
+  float4* rhs = nullptr;
+  float4* a = nullptr;
+  float4* b = nullptr;
   bool disposed = false;
 };
 
@@ -270,6 +282,7 @@ struct PlanShape {
   uint32_t nb[PF_MAX_FLUIDS] = {0, 0, 0, 0};    // boundary ranges per fluid slot (max over scenes)
   uint32_t bdyn[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // union of dynamic-boundary bits per fluid slot
   uint32_t tu_cap[PF_MAX_FLUIDS] = {PF_TU_MAX, PF_TU_MAX, PF_TU_MAX, PF_TU_MAX};  // LDS union capacity (max over scenes)
+  uint32_t isv[PF_MAX_FLUIDS] = {0, 0, 0, 0};  // viscosity-solver sweeps per fluid slot (max over scenes)
   bool local_soft = false;  // any scene's soft bodies use local shape matching
   bool walls = false;       // any scene has primitives
   bool soft = false;        // any scene has soft bodies
@@ -281,6 +294,7 @@ struct pf_loop_s {
   uint32_t substeps = 4, iterations = 2;
   double gravity[3] = {0, -9.81, 0};
   std::vector<pf_fluid_s*> fluids;
+  std::vector<pf_viscosity_s*> viscosity;  // solvers, each directly after its fluid
   pf_softbody_s* soft = nullptr;
   pf_primitives_s* prims = nullptr;
   std::vector<pf_sdf_collider_s*> sdfs;

@@ -191,13 +191,20 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
         launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc, sh.tu_cap[f]);
         ++nk;
       }
-      const bool fric = lastf && sh.walls;
+      // Friction follows every material's post-solve (loop.ts:241): it fuses
+      // into the last fluid's vorticity pass unless a viscosity solver runs after it.
+      const bool fric = lastf && sh.walls && sh.isv[f] == 0u;
       if (vort || visc || fric) {
         PF_LAUNCH(ctx, fric ? "vorticity_friction" : "vorticity");
         launch_vorticity(st, scenes, tiles, ntiles, (int)f, p, c, vort, visc, fric, (int)k, touch_valid,
                          sh.tu_cap[f]);
         ++nk;
         if (fric) friction_done = true;
+      }
+      if (sh.isv[f]) {
+        PF_LAUNCH(ctx, "viscosity_solve");
+        launch_viscosity_solve(st, scenes, tiles, ntiles, (int)f, p, sh.isv[f]);
+        nk += sh.isv[f] + 2u;
       }
     }
     if (!friction_done) {
@@ -416,6 +423,49 @@ int pf_fluid_info(pf_fluid* f, double* mass, double* h, double* spacing, double*
   if (h) *h = f->h;
   if (spacing) *spacing = f->spacing;
   if (rho0) *rho0 = f->rest_density;
+  PF_API_END
+}
+
+// ---------------------------------------------------------------------------
+// ViscositySolver (ViscositySolver.ts:38-131)
+// ---------------------------------------------------------------------------
+static void viscosity_ok(double v) {
+  if (!(v >= 0) || !std::isfinite(v)) fail(PF_EINVAL, "ViscositySolver: viscosity must be ≥ 0, got %g", v);
+}
+
+int pf_viscosity_create(pf_fluid* f, double viscosity, uint32_t iterations, pf_viscosity** out) {
+  PF_API_BEGIN
+  if (!f || !out) fail(PF_EINVAL, "ViscositySolver: null argument");
+  if (f->disposed) fail(PF_ESTATE, "ViscositySolver: the FluidSystem has been disposed");
+  if (iterations < 1 || iterations > 64)
+    fail(PF_EINVAL, "ViscositySolver: iterations must be an integer from 1 to 64, got %u", iterations);
+  viscosity_ok(viscosity);
+  auto* v = new pf_viscosity_s();
+  v->fluid = f;
+  v->viscosity = viscosity;
+  v->iterations = iterations;
+  *out = v;
+  PF_API_END
+}
+
+static void fluid_write_params(pf_fluid* f);
+
+int pf_viscosity_set(pf_viscosity* v, double viscosity) {
+  PF_API_BEGIN
+  if (!v || v->disposed) fail(PF_ESTATE, "ViscositySolver: disposed");
+  if (v->fluid && v->fluid->ps && v->fluid->ps->server)
+    fail(PF_ESTATE, "ViscositySolver: the scene is being served; remove it from the server first");
+  viscosity_ok(viscosity);
+  v->viscosity = viscosity;
+  if (v->loop && !v->fluid->disposed) fluid_write_params(v->fluid);
+  PF_API_END
+}
+
+int pf_viscosity_destroy(pf_viscosity* v) {
+  PF_API_BEGIN
+  if (!v || v->disposed) return PF_OK;
+  v->disposed = true;
+  if (v->loop) (void)hipStreamSynchronize(v->loop->ctx->stream);
   PF_API_END
 }
 
@@ -1116,6 +1166,18 @@ static void fluid_write_params(pf_fluid* f) {
   F.tu_limit = f->tu_limit;
   F.tu_size = f->tu_size;
   F.tu_slots = f->tu_slots;
+  if (f->isv) {
+    // rate = (nu * dt) * (10 / h^2), the constant computed in double (ViscositySolver.ts:103)
+    F.isv_iters = f->isv->iterations;
+    F.isv_nu = (float)f->isv->viscosity;
+    F.isv_c10 = (float)(10.0 / (h * h));
+    F.isv_rhs = f->isv->rhs;
+    F.isv_a = f->isv->a;
+    F.isv_b = f->isv->b;
+  } else {
+    F.isv_iters = 0;
+    F.isv_rhs = F.isv_a = F.isv_b = nullptr;
+  }
   F.normal = f->normal;
   F.omega = f->omega;
   F.dv = f->dv;
@@ -1137,6 +1199,7 @@ static PlanShape loop_shape(const pf_loop* L) {
     sh.nb[f] = L->ps->host.fluid[f].nboundary;
     sh.bdyn[f] = L->ps->host.fluid[f].bdynamic;
     sh.tu_cap[f] = std::min<uint32_t>(std::max<uint32_t>(L->ps->host.fluid[f].tu_limit, PF_BLOCK), PF_TU_MAX);
+    sh.isv[f] = L->ps->host.fluid[f].isv_iters;
   }
   return sh;
 }
@@ -1185,6 +1248,13 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
       if (L->fluids.size() >= PF_MAX_FLUIDS) fail(PF_EINVAL, "SimLoop: at most %d fluids per loop", PF_MAX_FLUIDS);
       L->fluids.push_back(f);
       neighbor_radius = std::max(neighbor_radius, f->h);
+    } else if (mats[m].kind == PF_MATERIAL_VISCOSITY) {
+      auto* v = static_cast<pf_viscosity*>(mats[m].handle);
+      if (!v || v->disposed || v->fluid->disposed) fail(PF_EINVAL, "SimLoop: a material has been disposed");
+      if (v->loop) fail(PF_EINVAL, "SimLoop: a ViscositySolver belongs to another SimLoop");
+      if (m == 0 || mats[m - 1].kind != PF_MATERIAL_FLUID || mats[m - 1].handle != v->fluid)
+        fail(PF_EINVAL, "ViscositySolver: list it directly after its FluidSystem in `materials`");
+      L->viscosity.push_back(v);
     } else if (mats[m].kind == PF_MATERIAL_SOFTBODY) {
       auto* sb = static_cast<pf_softbody*>(mats[m].handle);
       if (!sb || sb->disposed) fail(PF_EINVAL, "SimLoop: a material has been disposed");
@@ -1265,6 +1335,15 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
       S.has_boundary = 1u;
       for (auto& b : f->boundaries) L->static_boundaries = L->static_boundaries || !b.dynamic;
     }
+    f->isv = nullptr;
+    for (auto* v : L->viscosity)
+      if (v->fluid == f) {
+        v->loop = L.get();
+        v->rhs = dalloc<float4>(n);
+        v->a = dalloc<float4>(n);
+        v->b = dalloc<float4>(n);
+        f->isv = v;
+      }
     fluid_write_params(f);
   }
   const double s10 = std::floor(1073741824.0 / 10.0), s50 = std::floor(1073741824.0 / 50.0);
@@ -1365,6 +1444,8 @@ static void loop_members_alive(pf_loop* L) {
   for (auto* f : L->fluids)
     if (f->disposed) fail(PF_ESTATE, "SimLoop: a material has been disposed");
   if (L->soft && L->soft->disposed) fail(PF_ESTATE, "SimLoop: a material has been disposed");
+  for (auto* v : L->viscosity)
+    if (v->disposed) fail(PF_ESTATE, "SimLoop: a material has been disposed");
   if (L->prims && L->prims->disposed) fail(PF_ESTATE, "SimLoop: a collider has been disposed");
   for (auto* c : L->sdfs)
     if (c->disposed) fail(PF_ESTATE, "SimLoop: a collider has been disposed");
@@ -1411,8 +1492,10 @@ namespace pf {
 PlanShape loop_plan_shape(const pf_loop_s* L) { return loop_shape(L); }
 
 void plan_merge(PlanShape& sh, const PlanShape& ls) {
-  for (uint32_t f = 0; f < ls.nfluids; ++f)
+  for (uint32_t f = 0; f < ls.nfluids; ++f) {
     sh.tu_cap[f] = f < sh.nfluids ? std::max(sh.tu_cap[f], ls.tu_cap[f]) : ls.tu_cap[f];
+    sh.isv[f] = std::max(sh.isv[f], ls.isv[f]);
+  }
   sh.nfluids = std::max(sh.nfluids, ls.nfluids);
   for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f) {
     sh.fx[f] |= ls.fx[f];
@@ -1613,7 +1696,12 @@ int pf_loop_destroy(pf_loop* L) {
   }
   dfree(S.sdf_lambda); dfree(S.sdf_touch);
   S.sdf_lambda = nullptr; S.sdf_touch = nullptr; S.nsdf = 0;
-  for (auto* f : L->fluids) { dfree(f->reaction); f->reaction = nullptr; }
+  for (auto* f : L->fluids) { dfree(f->reaction); f->reaction = nullptr; f->isv = nullptr; }
+  for (auto* v : L->viscosity) {
+    dfree(v->rhs); dfree(v->a); dfree(v->b);
+    v->rhs = v->a = v->b = nullptr;
+    v->loop = nullptr;
+  }
   if (L->soft) L->soft->loop = nullptr;
   L->ps->loop = nullptr;
   L->ps->dirty = true;

@@ -725,6 +725,25 @@ napi_value LoopDestroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------- viscosity solver
+// (fluid, viscosity, iterations)
+napi_value ViscosityCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  pf_viscosity* v = nullptr;
+  PF_CALL(pf_viscosity_create((pf_fluid*)handle(env, a.argv[0]), num(env, a.argv[1]), u32(env, a.argv[2], 12), &v));
+  return make_handle(env, v);
+}
+napi_value ViscositySet(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_viscosity_set((pf_viscosity*)handle(env, a.argv[0]), num(env, a.argv[1])));
+  return nullptr;
+}
+napi_value ViscosityDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_viscosity_destroy((pf_viscosity*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
 // ---------------------------------------------------------------- autotuner
 // (handle, dt, [candidates], steps) -> {best, msPerStep: {capacity: ms}}
 template <typename Fn>
@@ -1010,6 +1029,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"loopSetSubsteps", LoopSetSubsteps}, {"loopSetGravity", LoopSetGravity}, {"loopSetGraph", LoopSetGraph},
       {"loopInfo", LoopInfo}, {"loopGrid", LoopGrid}, {"loopDestroy", LoopDestroy},
       {"loopAutotune", LoopAutotune}, {"particlesDeviceBytes", ParticlesDeviceBytes},
+      {"viscosityCreate", ViscosityCreate}, {"viscositySet", ViscositySet}, {"viscosityDestroy", ViscosityDestroy},
       {"batchCreate", BatchCreate}, {"batchStepAsync", BatchStepAsync}, {"batchSetGraph", BatchSetGraph},
       {"batchAutotune", BatchAutotune}, {"batchDestroy", BatchDestroy},
       {"serverCreate", ServerCreate}, {"serverAddScene", ServerAddScene}, {"serverRemoveScene", ServerRemoveScene},

@@ -174,6 +174,31 @@ class FluidSystem {
   }
 }
 
+/**
+ * new ViscositySolver(fluid, {viscosity, iterations = 12}) (src/fluids/ViscositySolver.ts):
+ * implicit viscosity for thick liquids. List it directly after its fluid in `materials`.
+ */
+class ViscositySolver {
+  constructor(fluid, options) {
+    this.fluid = fluid;
+    this.iterations = def(options.iterations, 12);
+    this.handle = native.viscosityCreate(fluid.handle, options.viscosity, this.iterations);
+    this.value = options.viscosity;
+  }
+  get viscosity() { return this.value; }
+  set viscosity(v) {
+    native.viscositySet(this.handle, v);
+    this.value = v;
+  }
+  get material() {
+    return { kind: 3, handle: this.handle };
+  }
+  dispose() {
+    if (this.handle) native.viscosityDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
 class PrimitiveSet {
   constructor(particles, options = {}) {
     this.particles = particles;
@@ -420,6 +445,7 @@ class Simulation {
       surfaceTension: def(options.surfaceTension, 0.1),
       vorticity: def(options.vorticity, 0.02),
       adhesion: def(options.adhesion, 0.1),
+      thickness: options.thickness,
     };
     this.fluids.push(entry);
     return entry;
@@ -519,6 +545,19 @@ class Simulation {
       range: ranges[i], viscosity: f.viscosity, surfaceTension: f.surfaceTension, vorticity: f.vorticity,
       adhesion: f.adhesion,
     }));
+    // thickness adds a ViscositySolver right after its fluid (Simulation.ts:740-745)
+    this.materials = [];
+    this.viscositySolvers = [];
+    this.fluids.forEach((f, i) => {
+      this.materials.push(this.fluidSystems[i]);
+      if (f.thickness !== undefined && f.thickness !== null) {
+        const solver = new ViscositySolver(this.fluidSystems[i], {
+          viscosity: f.thickness, iterations: Math.round(16 * Math.cbrt(ranges[i].count / 10000)),
+        });
+        this.viscositySolvers.push(solver);
+        this.materials.push(solver);
+      }
+    });
     this.walls = new PrimitiveSet(this.system);
     if (container) {
       const mn = container.slice(0, 3);
@@ -543,12 +582,13 @@ class Simulation {
       substeps,
       iterations: def(this.options.iterations, 2),
       gravity: this.gravityValue,
-      materials: this.fluidSystems,
+      materials: this.materials,
       colliders: this.walls.count > 0 ? [this.walls] : [],
     });
   }
   dispose() {
     if (this.simLoop) this.simLoop.dispose();
+    for (const v of this.viscositySolvers || []) v.dispose();
     for (const f of this.fluidSystems || []) f.dispose();
     if (this.walls) this.walls.dispose();
     if (this.system) this.system.dispose();
@@ -566,6 +606,7 @@ module.exports = {
   SoftbodySystem,
   SimLoop,
   Simulation,
+  ViscositySolver,
   Batch,
   Server,
   getDefaultContext,

@@ -347,6 +347,37 @@ class FluidSystem:
             self.handle = None
 
 
+class ViscositySolver:
+    """src/fluids/ViscositySolver.ts:38-131: implicit viscosity for thick
+    liquids, Jacobi sweeps of (I - dt nu L) v = v0 each substep. List it
+    directly after its FluidSystem in the SimLoop's materials."""
+
+    def __init__(self, fluid, viscosity, iterations=12):
+        h = C.c_void_p()
+        call("pf_viscosity_create", fluid.handle, float(viscosity), int(iterations), C.byref(h))
+        self.handle = h
+        self.fluid = fluid
+        self.iterations = int(iterations)
+        self._viscosity = float(viscosity)
+
+    @property
+    def viscosity(self):
+        return self._viscosity
+
+    @viscosity.setter
+    def viscosity(self, value):
+        call("pf_viscosity_set", self.handle, float(value))
+        self._viscosity = float(value)
+
+    def material(self):
+        return _lib.Material(3, self.handle)
+
+    def dispose(self):
+        if self.handle:
+            call("pf_viscosity_destroy", self.handle)
+            self.handle = None
+
+
 class PrimitiveSet:
     """src/core/collision/PrimitiveSet.ts:121-518."""
 

@@ -167,6 +167,9 @@ SIGNATURES = {
     "pf_fluid_info": [vp, P(f64), P(f64), P(f64), P(f64)],
     "pf_fluid_set_tile_union_limit": [vp, u32],
     "pf_fluid_destroy": [vp],
+    "pf_viscosity_create": [vp, f64, u32, P(vp)],
+    "pf_viscosity_set": [vp, f64],
+    "pf_viscosity_destroy": [vp],
     "pf_primitives_create": [vp, u32, P(vp)],
     "pf_primitives_add_plane": [vp, P(f64), P(f64), f64, f64, P(u32)],
     "pf_primitives_add_sphere": [vp, P(f64), f64, i32, f64, f64, P(u32)],

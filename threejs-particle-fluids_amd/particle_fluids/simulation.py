@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-from . import (FluidSystem, ParticleSystem, PFError, PrimitiveSet, SimLoop, default_context, fill_box,
+from . import (FluidSystem, ParticleSystem, PFError, PrimitiveSet, SimLoop, ViscositySolver, default_context, fill_box,
                radius_for_budget)
 
 
@@ -81,13 +81,15 @@ class Simulation:
         if self.started:
             raise PFError(1, f"Simulation: {what} must happen before the first step() or start()")
 
-    def addFluid(self, box, viscosity=0.01, surfaceTension=0.1, vorticity=0.02, adhesion=0.1):
-        """addFluid({box, viscosity, surfaceTension, vorticity, adhesion}) (Simulation.ts:342-368)."""
+    def addFluid(self, box, viscosity=0.01, surfaceTension=0.1, vorticity=0.02, adhesion=0.1, thickness=None):
+        """addFluid({box, viscosity, surfaceTension, vorticity, adhesion, thickness}) (Simulation.ts:342-368).
+        `thickness` adds a ViscositySolver (honey is about 20) with round(16 cbrt(N / 1e4))
+        sweeps per substep (Simulation.ts:740-745)."""
         self._not_started("addFluid")
         if box is None:
             raise PFError(1, "addFluid: give either `box` or `mesh`")
         entry = dict(box=[float(v) for v in box], viscosity=viscosity, surfaceTension=surfaceTension,
-                     vorticity=vorticity, adhesion=adhesion)
+                     vorticity=vorticity, adhesion=adhesion, thickness=thickness)
         self.fluids.append(entry)
         return entry
 
@@ -208,10 +210,16 @@ class Simulation:
         ps = ParticleSystem(len(pos), r, context=self.context)
         ps.uploadParticles(pos)
         materials = []
+        fluid_systems = []
         for f, rng in zip(self.fluids, ranges):
             fl = FluidSystem(ps, range=rng, viscosity=f["viscosity"], surfaceTension=f["surfaceTension"],
                              vorticity=f["vorticity"], adhesion=f["adhesion"])
             materials.append(fl)
+            fluid_systems.append(fl)
+            if f.get("thickness") is not None:
+                # Math.round(16 * Math.cbrt(count / 10000)): halves round up, as in JS
+                sweeps = int(math.floor(16.0 * float(np.cbrt(rng[1] / 10000.0)) + 0.5))
+                materials.append(ViscositySolver(fl, f["thickness"], iterations=sweeps))
         walls = PrimitiveSet(ps)
         container = self.options["container"]
         if container is not None:
@@ -239,7 +247,8 @@ class Simulation:
         colliders = [walls] if walls.count > 0 else []
         self.simLoop = SimLoop(ps, substeps=substeps, iterations=self.options["iterations"], gravity=self.gravity,
                                materials=materials, colliders=colliders)
-        self.system, self.walls, self.fluidSystems, self.fluidRanges = ps, walls, materials, ranges
+        self.system, self.walls, self.fluidSystems, self.fluidRanges = ps, walls, fluid_systems, ranges
+        self.materials = materials
         self.started = True
 
     def step(self, dt=1.0 / 60.0):
@@ -270,6 +279,9 @@ class Simulation:
     def dispose(self):
         if self.simLoop:
             self.simLoop.dispose()
+        for m in getattr(self, "materials", []):
+            if isinstance(m, ViscositySolver):
+                m.dispose()
         for f in self.fluidSystems:
             f.dispose()
         if self.walls:
