@@ -75,6 +75,12 @@ def lib():
         g("scene_add_boundary").restype = i32
         g("scene_get_bvol").argtypes = [vp, P(f64)]
         g("scene_get_bvol").restype = None
+        g("scene_set_groups").argtypes = [vp, P(u32)]
+        g("scene_set_groups").restype = None
+        g("scene_set_contact").argtypes = [vp, f64, f64, u32, P(u32), u32]
+        g("scene_set_contact").restype = i32
+        g("scene_contacts").argtypes = [vp, P(u32), u32]
+        g("scene_contacts").restype = u32
         g("scene_step").argtypes = [vp, f64]
         g("scene_step").restype = None
         g("scene_get").argtypes = [vp, P(f64), P(f64), P(f64), P(f64)]
@@ -264,6 +270,27 @@ class Scene:
         out = np.zeros(self.n)
         self._f("scene_get_bvol")(self.h, _p(out, f64))
         return out
+
+    def set_groups(self, groups):
+        g = np.ascontiguousarray(groups, np.uint32)
+        assert g.size == self.n
+        self._f("scene_set_groups")(self.h, _p(g, u32))
+
+    def set_contact(self, mus=0.5, muk=0.4, max_contacts=None, quiet=()):
+        """SimLoop `contact` option (loop.ts:23-31); quiet: extra (start, count)
+        ranges that never contact each other (fluids are always quiet)."""
+        q = np.ascontiguousarray(np.asarray(quiet, np.uint32).reshape(-1), np.uint32)
+        mc = 8 * self.n if max_contacts is None else int(max_contacts)
+        rc = self._f("scene_set_contact")(self.h, float(mus), float(muk), mc, _p(q, u32), len(q) // 2)
+        assert rc == 0
+        self.ct_max = mc
+
+    def contacts(self):
+        """(emitted count, stored (i, j) pairs) of the last substep."""
+        cap = self.ct_max
+        pairs = np.zeros((cap, 2), np.uint32)
+        count = int(self._f("scene_contacts")(self.h, _p(pairs, u32), cap))
+        return count, pairs[:min(count, cap)].copy()
 
     def step(self, dt):
         self._f("scene_step")(self.h, float(dt))

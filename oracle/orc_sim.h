@@ -87,6 +87,16 @@ typedef struct {
   REAL *rcen, *gcen, *grot, *lcomp, *plam4; /* [3n], [3n], [9n], [n], [3 * entries] */
   int32_t* lacc;               /* [3n] */
   REAL *q, *qp, *om;           /* [4n] orientation, predicted orientation, angular velocity */
+  /* particle contacts (src/core/contact), records in generation order */
+  uint32_t* grp;                     /* [n] collision group */
+  REAL* xs_grid;                     /* [3n] predictions at the last grid build (sortedPredictedPositions) */
+  int contact;                       /* SimLoop contact option on */
+  REAL ct_dist, ct_cr2, ct_mus, ct_muk; /* 2r, (2r * 1.1)^2, muS, muK */
+  uint32_t ct_max, ct_count;         /* maxContacts; pairs emitted last substep (dropped included) */
+  uint32_t nquiet, qstart[8], qcount[8]; /* extra noSelfContacts ranges (fluids are always quiet) */
+  uint32_t *ct_i, *ct_j;             /* [ct_max] */
+  int32_t *ct_ln, *ct_lt;            /* [ct_max] multipliers, 2^20 ticks per kg m */
+  REAL* ct_n;                        /* [3 ct_max] last normal */
   /* loop */
   uint32_t substeps, iterations;
   REAL g[3], dt;
@@ -180,9 +190,14 @@ static void SFX(grid_build)(SFX(scene)* s) {
   for (uint32_t i = 0; i < n; ++i) s->perm[counts[s->key[i]]++] = i;
   free(counts);
   s->flag_grid = over;
+  /* the grid's sorted copy of the predictions (sortedPositions.ts:31-39) */
+  memcpy(s->xs_grid, s->xs, sizeof(REAL) * 3 * (size_t)n);
 }
 
-/* neighborList.ts:45-72 over query.ts:24-70 */
+/* neighborList.ts:45-72 over query.ts:24-70: the query point is the current
+ * prediction, the candidates' positions come from the grid's sorted copy
+ * (neighborList.ts:59); the two differ only where a contact stabilisation
+ * moved particles after the grid build. */
 static void SFX(neighbors)(SFX(scene)* s, SFX(fluid)* F) {
   uint32_t H = s->H;
   int over = 0;
@@ -202,7 +217,7 @@ static void SFX(neighbors)(SFX(scene)* s, SFX(fluid)* F) {
           if (seen) continue;
           for (uint32_t t = s->start[b]; t < s->start[b + 1]; ++t) {
             uint32_t j = s->perm[t];
-            const REAL* xj = s->xs + 3 * (size_t)j;
+            const REAL* xj = s->xs_grid + 3 * (size_t)j;
             REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
             if (SFX(dot)(d, d) < F->h2) {
               if (cnt < 64) F->nl[(size_t)i * 64 + cnt++] = j;
@@ -1125,6 +1140,171 @@ static void SFX(orient_advect)(SFX(scene)* s) {
 }
 
 /* --------------------------------------------------------------------- */
+/* particle contacts (src/core/contact, loop.ts:346-418)                 */
+/* --------------------------------------------------------------------- */
+/* LAMBDA_SCALE (ContactBuffer.ts:16): ticks per kg m of the multipliers */
+#define ORC_CT_LSCALE 1048576.0
+
+/* A particle searches for contacts unless a material keeps it quiet
+ * (loop.ts:372-376): every fluid's range, plus the extra quiet ranges. */
+static int SFX(ct_emitter)(const SFX(scene)* s, uint32_t i) {
+  for (uint32_t f = 0; f < s->nfluids; ++f)
+    if (i >= s->fluid[f].start && i - s->fluid[f].start < s->fluid[f].count) return 0;
+  for (uint32_t q = 0; q < s->nquiet; ++q)
+    if (i >= s->qstart[q] && i - s->qstart[q] < s->qcount[q]) return 0;
+  return 1;
+}
+
+static int32_t SFX(ct_lticks)(REAL v) { return orc_f2i_sat((double)(v * (REAL)ORC_CT_LSCALE)); }
+
+/* buildContactGenerateKernel (generate.ts:39-88): each emitter walks the 27
+ * buckets around its prediction; a pair is kept once (by the emitter, the
+ * lower index when both emit), never inside one non-zero collision group,
+ * never between two pinned particles, and only within the candidate radius. */
+static void SFX(contact_generate)(SFX(scene)* s) {
+  uint32_t H = s->H;
+  s->ct_count = 0;
+  for (uint32_t i = 0; i < s->n; ++i) {
+    if (!SFX(ct_emitter)(s, i)) continue;
+    const REAL* xi = s->xs + 3 * (size_t)i;
+    uint32_t gi = s->grp[i];
+    REAL wi = s->w[i];
+    int32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = orc_f2i_sat((double)RFLOOR((xi[a] - s->origin[a]) / s->cell));
+    uint32_t visited[27], nv = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          uint32_t b = orc_morton((int32_t)((uint32_t)q[0] + (uint32_t)dx), (int32_t)((uint32_t)q[1] + (uint32_t)dy),
+                                  (int32_t)((uint32_t)q[2] + (uint32_t)dz)) & (H - 1);
+          int seen = 0;
+          for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
+          visited[nv++] = b;
+          if (seen) continue;
+          for (uint32_t t = s->start[b]; t < s->start[b + 1]; ++t) {
+            uint32_t j = s->perm[t];
+            if (j == i) continue;
+            if (j < i && SFX(ct_emitter)(s, j)) continue;
+            if (gi != 0u && gi == s->grp[j]) continue;
+            if (wi <= R(0.0) && s->w[j] <= R(0.0)) continue;
+            const REAL* xj = s->xs + 3 * (size_t)j;
+            REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            if (SFX(dot)(d, d) >= s->ct_cr2) continue;
+            uint32_t slot = s->ct_count++;
+            if (slot < s->ct_max) {
+              s->ct_i[slot] = i;
+              s->ct_j[slot] = j;
+            }
+          }
+        }
+  }
+  /* resetLambdaKernel (ContactBuffer.ts:67-71) */
+  for (uint32_t c = 0; c < s->ct_max; ++c) {
+    s->ct_ln[c] = 0;
+    s->ct_lt[c] = 0;
+  }
+}
+
+static uint32_t SFX(ct_used)(const SFX(scene)* s) { return s->ct_count < s->ct_max ? s->ct_count : s->ct_max; }
+
+/* buildContactStabilizeKernel (solve.ts:134-181), then the positions
+ * accumulator applied to the committed and the predicted positions
+ * (loop.ts:389-393). */
+static void SFX(contact_stabilize)(SFX(scene)* s) {
+  uint32_t used = SFX(ct_used)(s);
+  for (uint32_t c = 0; c < used; ++c) {
+    uint32_t i = s->ct_i[c], j = s->ct_j[c];
+    const REAL* xi = s->x + 3 * (size_t)i;
+    const REAL* xj = s->x + 3 * (size_t)j;
+    REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+    REAL dist = RSQRT(SFX(dot)(off, off));
+    REAL wi = s->w[i], wj = s->w[j], ws = wi + wj;
+    if (dist <= (REAL)1e-8 || dist >= s->ct_dist || ws <= R(0.0)) continue;
+    REAL nrm[3] = {off[0] / dist, off[1] / dist, off[2] / dist};
+    REAL dl = (s->ct_dist - dist) / ws;
+    for (int a = 0; a < 3; ++a) {
+      s->pacc[3 * (size_t)i + a] = SFX(wadd)(s->pacc[3 * (size_t)i + a], SFX(ticks)(nrm[a] * (wi * dl), s->pos_scale));
+      s->pacc[3 * (size_t)j + a] = SFX(wadd)(s->pacc[3 * (size_t)j + a], SFX(ticks)(-(nrm[a] * (wj * dl)), s->pos_scale));
+    }
+  }
+  for (uint32_t p = 0; p < s->n; ++p) {
+    REAL c[3];
+    if (SFX(acc_change)(s->pacc + 3 * (size_t)p, s->pos_inv, c)) s->flag_pos = 1;
+    for (int a = 0; a < 3; ++a) {
+      s->x[3 * (size_t)p + a] = s->x[3 * (size_t)p + a] + c[a];
+      s->xs[3 * (size_t)p + a] = s->xs[3 * (size_t)p + a] + c[a];
+      s->pacc[3 * (size_t)p + a] = 0;
+    }
+  }
+}
+
+/* buildContactSolveKernel (solve.ts:33-96): one pass over the records; the
+ * corrections go to the positions accumulator the colliders share. */
+static void SFX(contact_solve)(SFX(scene)* s) {
+  uint32_t used = SFX(ct_used)(s);
+  for (uint32_t c = 0; c < used; ++c) {
+    uint32_t i = s->ct_i[c], j = s->ct_j[c];
+    const REAL* pi = s->xs + 3 * (size_t)i;
+    const REAL* pj = s->xs + 3 * (size_t)j;
+    REAL off[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
+    REAL dist = RSQRT(SFX(dot)(off, off));
+    REAL wi = s->w[i], wj = s->w[j], ws = wi + wj;
+    if (dist <= (REAL)1e-8 || dist >= s->ct_dist || ws <= R(0.0)) continue;
+    REAL nrm[3] = {off[0] / dist, off[1] / dist, off[2] / dist};
+    REAL depth = s->ct_dist - dist;
+    REAL dln = depth / ws;
+    s->ct_ln[c] = SFX(wadd)(s->ct_ln[c], SFX(ct_lticks)(dln));
+    for (int a = 0; a < 3; ++a) s->ct_n[3 * (size_t)c + a] = nrm[a];
+    const REAL* ci = s->x + 3 * (size_t)i;
+    const REAL* cj = s->x + 3 * (size_t)j;
+    REAL slip[3], tang[3];
+    for (int a = 0; a < 3; ++a) slip[a] = (pi[a] - ci[a]) - (pj[a] - cj[a]);
+    REAL sd = SFX(dot)(slip, nrm);
+    for (int a = 0; a < 3; ++a) tang[a] = slip[a] - nrm[a] * sd;
+    REAL tl = RSQRT(SFX(dot)(tang, tang));
+    REAL ln = (REAL)s->ct_ln[c] / (REAL)ORC_CT_LSCALE;
+    REAL lt = (REAL)s->ct_lt[c] / (REAL)ORC_CT_LSCALE;
+    REAL dlt = tl / ws;
+    int sticks = tl > (REAL)1e-10 && dlt <= s->ct_mus * ln - lt;
+    for (int a = 0; a < 3; ++a) {
+      REAL ni = nrm[a] * (wi * dln);
+      REAL nj = -(nrm[a] * (wj * dln));
+      REAL fi = -(tang[a] * (wi / ws));
+      REAL fj = tang[a] * (wj / ws);
+      s->pacc[3 * (size_t)i + a] = SFX(wadd)(s->pacc[3 * (size_t)i + a], SFX(ticks)(sticks ? ni + fi : ni, s->pos_scale));
+      s->pacc[3 * (size_t)j + a] = SFX(wadd)(s->pacc[3 * (size_t)j + a], SFX(ticks)(sticks ? nj + fj : nj, s->pos_scale));
+    }
+    s->ct_lt[c] = SFX(wadd)(s->ct_lt[c], SFX(ct_lticks)(sticks ? dlt : R(0.0)));
+  }
+}
+
+/* buildContactFrictionKernel (solve.ts:183-189 and 147-182): kinetic
+ * friction into the velocities accumulator. */
+static void SFX(contact_friction)(SFX(scene)* s) {
+  uint32_t used = SFX(ct_used)(s);
+  for (uint32_t c = 0; c < used; ++c) {
+    uint32_t i = s->ct_i[c], j = s->ct_j[c];
+    REAL wi = s->w[i], wj = s->w[j], ws = wi + wj;
+    REAL ln = (REAL)s->ct_ln[c] / (REAL)ORC_CT_LSCALE;
+    if (ws <= R(0.0) || ln <= R(0.0)) continue;
+    const REAL* nrm = s->ct_n + 3 * (size_t)c;
+    const REAL* vi = s->v + 3 * (size_t)i;
+    const REAL* vj = s->v + 3 * (size_t)j;
+    REAL rel[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]}, vt[3];
+    REAL nd = SFX(dot)(nrm, rel);
+    for (int a = 0; a < 3; ++a) vt[a] = rel[a] - nrm[a] * nd;
+    REAL vl = RSQRT(SFX(dot)(vt, vt));
+    if (vl < (REAL)1e-6) continue;
+    REAL change = RFMIN(((s->ct_muk * ln) * ws) / s->dt, vl);
+    for (int a = 0; a < 3; ++a) {
+      REAL imp = (-((vt[a] / vl) * change)) / ws;
+      s->vacc[3 * (size_t)i + a] = SFX(wadd)(s->vacc[3 * (size_t)i + a], SFX(ticks)(imp * wi, s->vel_scale));
+      s->vacc[3 * (size_t)j + a] = SFX(wadd)(s->vacc[3 * (size_t)j + a], SFX(ticks)(-(imp * wj), s->vel_scale));
+    }
+  }
+}
+
+/* --------------------------------------------------------------------- */
 /* one step (loop.ts:175-279)                                            */
 /* --------------------------------------------------------------------- */
 static void SFX(step)(SFX(scene)* s, double dt_step) {
@@ -1169,7 +1349,11 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
     }
     if (s->local) SFX(orient_predict)(s);
     s->flag_nl = 0;
-    if (s->nfluids) SFX(grid_build)(s);
+    if (s->nfluids || s->contact) SFX(grid_build)(s);
+    if (s->contact) {
+      SFX(contact_generate)(s);
+      SFX(contact_stabilize)(s);
+    }
     for (uint32_t f = 0; f < s->nfluids; ++f)
       for (uint32_t b = 0; b < s->fluid[f].nb; ++b)
         if (s->fluid[f].bdyn[b]) SFX(boundary_volume)(s, &s->fluid[f], b, s->xs);
@@ -1188,7 +1372,8 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       }
       if (s->local) SFX(local_iteration)(s, it == 0);
       else if (s->nbodies) SFX(shape_match)(s, it == 0, 1);
-      if (s->nprim || s->nsdf) SFX(walls_solve)(s, s->clock[k]);
+      if (s->contact) SFX(contact_solve)(s);
+      if (s->nprim || s->nsdf || s->contact) SFX(walls_solve)(s, s->clock[k]);
     }
     /* advect (integrate.ts:36-44) */
     for (uint32_t i = 0; i < n; ++i) {
@@ -1206,6 +1391,9 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       SFX(orient_advect)(s);
       SFX(shape_match)(s, 0, 0);
     }
-    if ((s->nprim || s->nsdf) && s->iterations > 0) SFX(walls_friction)(s, s->clock[k]);
+    if ((s->nprim || s->nsdf || s->contact) && s->iterations > 0) {
+      if (s->contact) SFX(contact_friction)(s);
+      SFX(walls_friction)(s, s->clock[k]);
+    }
   }
 }

@@ -258,6 +258,8 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     s->weights = (T*)calloc(n, sizeof(T));                                                         \
     s->slam = (T*)calloc(3 * (size_t)n, sizeof(T));                                                \
     s->radius_d = radius;                                                                          \
+    s->grp = (uint32_t*)calloc(n, sizeof(uint32_t));                                               \
+    s->xs_grid = (T*)calloc(3 * (size_t)n, sizeof(T));                                             \
     s->q = (T*)calloc(4 * (size_t)n, sizeof(T));                                                   \
     s->qp = (T*)calloc(4 * (size_t)n, sizeof(T));                                                  \
     s->om = (T*)calloc(4 * (size_t)n, sizeof(T));                                                  \
@@ -281,6 +283,8 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     for (uint32_t k = 0; k < s->nsdf; ++k) free(s->sdf[k].grid);                                   \
     free(s->q); free(s->qp); free(s->om); free(s->lcomp); free(s->ledges); free(s->csr_off);      \
     free(s->csr_idx); free(s->rcen); free(s->gcen); free(s->grot); free(s->plam4); free(s->lacc); \
+    free(s->grp); free(s->xs_grid); free(s->ct_i); free(s->ct_j); free(s->ct_ln); free(s->ct_lt);  \
+    free(s->ct_n);                                                                                 \
     for (uint32_t f = 0; f < s->nfluids; ++f) {                                                    \
       fluid_##SUF* F = &s->fluid[f];                                                               \
       free(F->nl); free(F->nlc); free(F->normal); free(F->lambda); free(F->dx);                    \
@@ -490,6 +494,40 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
       if (qp4) qp4[k] = (double)s->qp[k];                                                          \
       if (om4) om4[k] = (double)s->om[k];                                                          \
     }                                                                                              \
+  }                                                                                                \
+  void orc_scene_set_groups_##SUF(void* p, const uint32_t* g) {                                    \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    memcpy(s->grp, g, sizeof(uint32_t) * s->n);                                                    \
+  }                                                                                                \
+  /* SimLoop contact option (loop.ts:23-31, 346-418): contact distance 2r,                       \
+   * candidate radius 2r * 1.1 (generate.ts:19), quiet: [start, count] pairs */                   \
+  int orc_scene_set_contact_##SUF(void* p, double mus, double muk, uint32_t max_contacts,         \
+                                  const uint32_t* quiet, uint32_t nquiet) {                        \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    if (nquiet > 8 || max_contacts == 0) return -1;                                                \
+    double cr = 2.0 * s->radius_d * 1.1;                                                           \
+    s->contact = 1;                                                                                \
+    s->ct_dist = (T)(2.0 * s->radius_d);                                                           \
+    s->ct_cr2 = (T)(cr * cr);                                                                      \
+    s->ct_mus = (T)mus;                                                                            \
+    s->ct_muk = (T)muk;                                                                            \
+    s->ct_max = max_contacts;                                                                      \
+    s->nquiet = nquiet;                                                                            \
+    for (uint32_t q = 0; q < nquiet; ++q) { s->qstart[q] = quiet[2 * q]; s->qcount[q] = quiet[2 * q + 1]; } \
+    s->ct_i = (uint32_t*)calloc(max_contacts, sizeof(uint32_t));                                   \
+    s->ct_j = (uint32_t*)calloc(max_contacts, sizeof(uint32_t));                                   \
+    s->ct_ln = (int32_t*)calloc(max_contacts, sizeof(int32_t));                                    \
+    s->ct_lt = (int32_t*)calloc(max_contacts, sizeof(int32_t));                                    \
+    s->ct_n = (T*)calloc(3 * (size_t)max_contacts, sizeof(T));                                     \
+    return 0;                                                                                      \
+  }                                                                                                \
+  /* readbackCount / readbackPairs (ContactBuffer.ts:74-97): returns the emitted                 \
+   * count; pairs receives min(count, maxContacts, cap) (i, j) pairs */                           \
+  uint32_t orc_scene_contacts_##SUF(void* p, uint32_t* pairs, uint32_t cap) {                      \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    uint32_t used = s->ct_count < s->ct_max ? s->ct_count : s->ct_max;                             \
+    for (uint32_t c = 0; c < used && c < cap; ++c) { pairs[2 * c] = s->ct_i[c]; pairs[2 * c + 1] = s->ct_j[c]; } \
+    return s->ct_count;                                                                            \
   }                                                                                                \
   void orc_scene_step_##SUF(void* p, double dt) { step_##SUF((scene_##SUF*)p, dt); }               \
   /* state out as double: x, x*, v as xyz triples, density */                                     \

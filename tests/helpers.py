@@ -236,3 +236,54 @@ class PosedPrimitive:
 
     def oracle_args(self):
         return self.kind, self.flags, self.d0, self.d1, self.vel, self.rot, self.spin
+
+
+CONTACT_RADIUS_EXPANSION = 1.1  # candidate radius 2r * 1.1 (generate.ts:19)
+
+
+def hash_table_size(n):
+    """H = min(nextPow2(2n), 2^20) (HashGrid.ts:122-124)."""
+    H = 1
+    while H < 2 * n:
+        H <<= 1
+    return min(H, 1 << 20)
+
+
+def oracle_contact_scene(pos, radius, substeps, iterations, gravity=(0.0, -9.81, 0.0), vel=None, w=None,
+                         groups=None, mus=0.5, muk=0.4, max_contacts=None, quiet=(), precision="f32", cell=None):
+    """Particles with SimLoop contacts on (loop.ts:346-418) and no materials
+    in the oracle; the grid's cell is the contact candidate radius."""
+    n = len(pos)
+    s = orc.Scene(n, radius, precision)
+    s.set_particles(pos, vel, w)
+    if cell is None:
+        cell = 2 * radius * CONTACT_RADIUS_EXPANSION
+    s.set_loop(substeps, iterations, gravity, cell, hash_table_size(n))
+    if groups is not None:
+        s.set_groups(groups)
+    s.set_contact(mus, muk, max_contacts, quiet)
+    return s
+
+
+def brute_force_pairs(pos, w, groups, radius, emitter):
+    """Every pair the contact search must find (generate.ts:39-88), from all
+    pairs rather than the grid walk: at least one end emits, not one non-zero
+    group, not both pinned, closer than 2r * 1.1 (distance in f32)."""
+    p = np.asarray(pos, np.float32)
+    cr = 2.0 * radius * CONTACT_RADIUS_EXPANSION
+    cr2 = np.float32(cr * cr)
+    out = set()
+    n = len(p)
+    for i in range(n):
+        for j in range(i + 1, n):
+            if not (emitter[i] or emitter[j]):
+                continue
+            if groups[i] != 0 and groups[i] == groups[j]:
+                continue
+            if w[i] <= 0 and w[j] <= 0:
+                continue
+            d = p[i] - p[j]
+            d2 = np.float32(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])) + np.float32(d[2] * d[2])
+            if d2 < cr2:
+                out.add((i, j))
+    return out
