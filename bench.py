@@ -62,7 +62,8 @@ SCAN_BYTES_PER_BUCKET = 12
 def run_batched(args, ctx):
     """256 independent scenes, particle budgets log-uniform in [5,000, 200,000];
     every other scene adds 20 soft cubes of 2,197 particles (global shape
-    matching, stiffness uniform in [0.1, 1.0]). Times the same scenes stepped
+    matching, stiffness uniform in [0.1, 1.0]) with particle contacts on, as
+    Simulation sets them for scenes with soft bodies. Times the same scenes stepped
     one loop per launch (each loop's own HIP graph) and as one batch (one HIP
     graph over every scene), both with inputs resident in HBM."""
     import particle_fluids as pf
@@ -124,7 +125,7 @@ def run_batched(args, ctx):
     work = n_total * args.substeps * args.batch_steps
     out = {
         "workload": f"{len(built)} scenes, budgets log-uniform in [5000, 200000] (seed {args.batch_seed}), "
-                    f"every other scene + 20 soft cubes of 2197 particles, {args.substeps} substeps x "
+                    f"every other scene + 20 soft cubes of 2197 particles with particle contacts, {args.substeps} substeps x "
                     f"{args.iterations} iterations, dt 1/60",
         "scenes": len(built), "particles": n_total, "fluid_particles": n_fluid,
         "steps": args.batch_steps, "timing": "alternating plain rounds and batched steps",
@@ -146,9 +147,38 @@ def run_batched(args, ctx):
     return out
 
 
+def process_mappings():
+    """Memory mappings this process holds (each device chunk of an evictable
+    scene is one or more), or -1 where /proc is unavailable."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sum(1 for _ in f)
+    except OSError:
+        return -1
+
+
+def device_free_gib():
+    """Free device memory (hipMemGetInfo), GiB."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        return -1.0
+    return free.value / 2 ** 30
+
+
+def max_mappings():
+    try:
+        with open("/proc/sys/vm/max_map_count") as f:
+            return int(f.read().strip())
+    except (OSError, ValueError):
+        return -1
+
+
 def run_served(args, ctx, tile_union=None):
     """1,000 live scenes (fluid budgets log-uniform in [5,000, 500,000], 0-30
-    soft cubes each, all from a seed), resident in HBM under a 200 GB budget,
+    soft cubes each, particle contacts on in scenes with cubes, all from a
+    seed), resident in HBM under a 200 GB budget,
     served by the request loop with continuous batching. A Poisson stream of
     step(1/60) requests at --serve-rate req/s picks scenes uniformly; reports
     throughput and p50/p99 latency (submission to completion), then the
@@ -178,8 +208,9 @@ def run_served(args, ctx, tile_union=None):
         built.append(sc)
         sizes.append(sc["ps"].capacity)
         if (i + 1) % 100 == 0:
-            print(f"[bench] served: {i + 1} scenes built ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
-                  flush=True)
+            print(f"[bench] served: {i + 1} scenes built ({time.perf_counter() - t0:.1f} s, "
+                  f"{srv.stats()['resident_bytes'] / 2**30:.1f} GiB resident, {device_free_gib():.1f} GiB free, "
+                  f"{process_mappings()} memory mappings)", file=sys.stderr, flush=True)
     sizes = np.asarray(sizes, np.int64)
     st0 = srv.stats()
     print(f"[bench] served: {len(built)} scenes, {int(sizes.sum())} particles, {st0['resident_bytes'] / 2**30:.1f} "
@@ -246,7 +277,8 @@ def run_served(args, ctx, tile_union=None):
         scenes.dispose_scene(sc)
     return {
         "workload": f"{len(built)} scenes, fluid budgets log-uniform in [5000, 500000], 0-30 soft cubes of 2197 "
-                    f"particles (seed {args.serve_seed}), {args.substeps} substeps x {args.iterations} iterations, "
+                    f"particles with particle contacts (seed {args.serve_seed}), {args.substeps} substeps x "
+                    f"{args.iterations} iterations, "
                     f"step(1/60) requests on uniformly chosen scenes",
         "scenes": len(built), "particles": int(sizes.sum()),
         "hbm_budget_gib": args.serve_budget_gb, "resident_gib": round(st0["resident_bytes"] / 2**30, 2),

@@ -225,6 +225,10 @@ typedef struct pf_softbody_def {
 #define PF_SHAPE_MATCHING_LOCAL 1u
 int pf_softbody_create(pf_particles* ps, const pf_softbody_def* bodies, uint32_t nbodies,
                        uint32_t shape_matching, pf_softbody** out);
+/* selfCollision option (SoftbodySystem.ts:71-72, 218-233); before the loop
+ * is built. Without it the SimLoop gives each body its own collision group
+ * (unless the whole body already shares a non-zero one). */
+int pf_softbody_set_self_collision(pf_softbody* sb, int enabled);
 /* setCompliance(index, compliance) (SoftbodySystem.ts:220-224) */
 int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double compliance);
 /* bodyCenters (vec4 per body) and bodyRotations (3 vec4 rows per body) */
@@ -276,6 +280,16 @@ typedef struct pf_loop_options {
   double gravity[3];
   int has_hash_origin;
   double hash_origin[3];
+  /* contact: true | ContactOptions (loop.ts:23-31, 43-49): particle-particle
+   * contacts. mu < 0 takes the defaults 0.5 / 0.4; max_contacts 0 takes
+   * 8 x capacity. Fluids never contact their own particles; quiet adds up to
+   * 8 more [start, count] ranges that never contact each other (a material's
+   * noSelfContacts, materials.ts:45). */
+  int contact;
+  double contact_mu_s, contact_mu_k;
+  uint32_t max_contacts;
+  uint32_t nquiet;
+  const uint32_t* quiet;
 } pf_loop_options;
 typedef struct pf_overflow {
   int positions, velocities, grid, contacts, neighbors, impulses, long_bucket;
@@ -295,6 +309,12 @@ int pf_loop_set_substeps(pf_loop* loop, uint32_t substeps);
 int pf_loop_set_gravity(pf_loop* loop, const double gravity[3]);
 int pf_loop_info(pf_loop* loop, uint32_t* substeps, uint32_t* iterations, double* cell_size,
                  uint32_t* hash_table_size, uint32_t* kernels_per_step);
+/* SimLoop.contacts.readbackCount / readbackPairs (ContactBuffer.ts:74-97):
+ * pairs emitted in the last substep (dropped ones included) and up to
+ * `cap` stored (i, j) particle-index pairs, i the particle that found the
+ * pair. Fails when the loop has no contacts. */
+int pf_loop_readback_contacts(pf_loop* loop, uint32_t* count, uint32_t* pairs, uint32_t cap,
+                              uint32_t* stored);
 /* The loop's grid (SimLoop.hashGrid, loop.ts:104-106); owned by the loop. */
 int pf_loop_grid(pf_loop* loop, pf_grid** out);
 /* Capture each step's launches in a HIP graph and replay it (1) or launch

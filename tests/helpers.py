@@ -129,9 +129,12 @@ def quat_to_mat(q):
                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
 
 
-def oracle_mixed_scene(layout, substeps, iterations, H, precision="f32"):
+def oracle_mixed_scene(layout, substeps, iterations, H, precision="f32", contact=None):
     """A scene of particle_fluids.scenes.mixed_layout (dam break plus globally
-    shape-matched cubes in the extended tank) in the CPU oracle."""
+    shape-matched cubes in the extended tank) in the CPU oracle. `contact` as
+    scenes.build_mixed_scene: None turns contacts on when there are cubes, and
+    each cube then has its own collision group (1, 2, ... in body order, as
+    the SimLoop hands them out)."""
     pos, nf, r = layout["positions"], layout["fluid_count"], layout["radius"]
     n = len(pos)
     spacing = 2 * r
@@ -163,6 +166,20 @@ def oracle_mixed_scene(layout, substeps, iterations, H, precision="f32"):
     mn, mx = layout["container"][:3], layout["container"][3:]
     for nrm, which in scenes.PLANES:
         s.add_plane(nrm, mn if which == "min" else mx)
+    if layout["bodies"]:
+        groups = np.zeros(n, np.uint32)
+        for b, body in enumerate(layout["bodies"]):
+            st, c = body["range"]
+            groups[st:st + c] = b + 1
+        s.set_groups(groups)
+    if contact is None and layout["bodies"]:
+        soft_count = sum(b["range"][1] for b in layout["bodies"])
+        contact = dict(scenes.SIMULATION_CONTACT, maxContacts=8 * soft_count)
+    elif contact is None:
+        contact = False
+    if contact:
+        c = {} if contact is True else contact
+        s.set_contact(c.get("muS", 0.5), c.get("muK", 0.4), c.get("maxContacts"))
     return s
 
 
@@ -287,3 +304,30 @@ def brute_force_pairs(pos, w, groups, radius, emitter):
             if d2 < cr2:
                 out.add((i, j))
     return out
+
+
+def floating_cubes_layout(budget=3000, side=5, seed=11):
+    """A dam-break water block with two soft cubes resting on its surface,
+    each overlapping the top water layer and the other cube by a tenth of a
+    spacing: contacts (and their stabilisation) start on the first substep."""
+    pos, r = scenes.dam_break_layout(budget)
+    pos = scenes.jitter(pos, 2 * r, seed)
+    nf = len(pos)
+    sp = 2 * r
+    lo, hi = pos.min(0).astype(np.float64), pos.max(0).astype(np.float64)
+    half = (side - 1) / 2 * sp
+    xc = 0.5 * (lo[0] + hi[0])
+    yc = hi[1] + 0.9 * sp + half
+    za = lo[2] + half + 2 * sp
+    cubes, bodies = [], []
+    for k, zc in enumerate((za, za + (side - 1) * sp + 0.9 * sp)):
+        cp, surf = scenes.particle_cube((xc, yc, zc), side, sp)
+        bodies.append(dict(range=(nf + k * side ** 3, side ** 3), surfaceCount=surf,
+                           compliance=scenes.softbody_compliance(0.5, side ** 3)))
+        cubes.append(cp)
+    allpos = np.concatenate([pos.astype(np.float64)] + cubes).astype(np.float32)
+    w = np.empty(len(allpos), np.float32)
+    w[:nf] = 0.0
+    w[nf:] = 1.0 / (scenes.CUBE_DENSITY * sp ** 3)
+    return dict(positions=allpos, inv_mass=w, fluid_count=nf, bodies=bodies, radius=r,
+                container=tuple(scenes.CONTAINER))

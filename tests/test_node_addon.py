@@ -77,3 +77,33 @@ def test_js_thick_fluid_matches_python(tmp_path):
     py_pos = sim.particleSystem.readback().positions
     assert np.array_equal(js_pos, py_pos)
     sim.dispose()
+
+
+@pytest.mark.gpu
+def test_js_contacts_match_python(tmp_path):
+    """Particle contacts through the JS API (loop option, ContactBuffer,
+    a noSelfContacts material): the pile equals the Python run bit for bit."""
+    out = tmp_path / "pile.bin"
+    frames = 30
+    r = subprocess.run([NODE, os.path.join(JS, "test", "contact_pile.js"), str(out), str(frames)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["same"] and info["fromSlab"] and info["pairs"] > 0
+    rr = 0.05
+    sp = 2 * rr * 1.05
+    pos = np.array([[(ix - 1) * sp, 0.5 + (iy + 1) * sp, (iz - 1) * sp]
+                    for ix in range(3) for iy in range(3) for iz in range(3)])
+    ps = pf.ParticleSystem(len(pos), rr)
+    ps.uploadParticles(pos)
+    floor = pf.PrimitiveSet(ps)
+    floor.addPlane((0, 1, 0), (0, 0, 0))
+    loop = pf.SimLoop(ps, substeps=4, iterations=4, colliders=[floor],
+                      contact={"maxContacts": 256, "muS": 0.4, "muK": 0.3})
+    for _ in range(frames):
+        loop.step(1 / 60)
+    assert np.array_equal(np.fromfile(out, np.float32).reshape(-1, 4), ps.readback().positions)
+    assert loop.contacts.readbackCount() == info["count"]
+    loop.dispose()
+    floor.dispose()
+    ps.dispose()

@@ -168,12 +168,17 @@ static __device__ __forceinline__ void tu_load_slots(const uint32_t* __restrict_
 // particles or PF_TU_HASH_LIMIT buckets), and every tile of a table smaller
 // than 64 buckets (where stencil buckets can coincide), walks global memory
 // and stores global slots instead. Both walks give the same list.
+// With `snap` set, a contact stabilisation has moved some predictions since
+// the sort: the query point is the current prediction (xs[c]) and candidates
+// are tested at the positions the grid was built from (xs[c ^ 1], the
+// reference's sortedPredictedPositions, neighborList.ts:59), while the
+// colour-field normal reads current positions.
 // ---------------------------------------------------------------------------
 template <bool NORMAL>
 __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
-                                                        uint32_t cap) {
+                                                        uint32_t cap, int snap) {
   __shared__ uint32_t h_key[PF_TU_HASH];    // bucket + 1, 0 = empty
   __shared__ uint32_t h_off[PF_TU_HASH];    // (union offset << 16) | bucket count
   __shared__ uint32_t h_start[PF_TU_HASH];  // first sorted slot of the bucket
@@ -193,6 +198,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   const uint32_t n = S.n;
   const bool active = s < n && in_range(S.id[p][s], F.start, F.count);
   const float4* __restrict__ xs = S.xs[c];
+  const float4* __restrict__ xw = snap ? S.xs[c ^ 1] : xs;  // candidate positions
   float3 xi = make_float3(0.f, 0.f, 0.f);
   int q[3] = {0, 0, 0};
   // Distance from the particle to the faces of its own cell, to skip stencil
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           visited[nvisited++] = b;
           if (seen) continue;
           for (uint32_t j = S.start[b]; j < S.start[b + 1] && !over; ++j) {
-            const float4 xj = xs[j];
+            const float4 xj = xw[j];
             const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
             if (pf_dot3(d, d) < F.h2) {
               if (count < PF_MAX_NEIGHBORS) {
@@ -346,7 +352,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
             float4 xj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = xs[min(j0 + u, en - 1u)];
+            for (int u = 0; u < 4; ++u) xj[u] = xw[min(j0 + u, en - 1u)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const uint32_t j = j0 + u;
@@ -427,7 +433,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     __syncthreads();
     float4 xv[PF_TU_PER];
 #pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xs[js[i]];
+    for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
@@ -506,7 +512,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
         float vol[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          xj[u] = u_x[idx[u]];
+          xj[u] = snap ? xs[uslots[idx[u]]] : u_x[idx[u]];
           vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
         }
 #pragma unroll
@@ -661,7 +667,58 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
   }
 }
 
-template <bool ST, bool ADH>
+// Surface tension the way the reference scatters it (cohesion.ts:60-100):
+// each pair from the list of its lower id, the fixed-point change added to
+// both ends with integer atomics. Scenes with contacts use it: their
+// stabilisation can make two lists disagree about a pair (the query point is
+// the stabilised prediction, the candidates the sorted copy), which the
+// gathered form, evaluating upper-side pairs from its own list, assumes away.
+// The sums go to the contact accumulator, idle between the stabilisation and
+// the first contact solve; k_impulses adds and clears them.
+__global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* __restrict__ scenes,
+                                                               const Tile* __restrict__ tiles,
+                                                               uint32_t ntiles, int fi, int p, int c) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || !S.ct.on || !(F.effects & PF_FX_SURFACE_TENSION)) return;
+  const uint32_t id = S.id[p][s];
+  if (!in_range(id, F.start, F.count)) return;
+  const float4* __restrict__ xs = S.xs[c];
+  const float3 xi = pf_xyz(xs[s]);
+  const float3 ni = pf_xyz(F.normal[s]);
+  const float rhoi = S.rho[p][s];
+  const uint32_t n = S.n;
+  const uint32_t cnt = F.nl_count[s];
+  const uint32_t fend = F.start + F.count;
+  const uint32_t U = F.tu_size[t.first / PF_BLOCK];
+  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
+  const bool staged = U != PF_TU_GLOBAL;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    uint32_t j;
+    if (staged) {
+      const uint32_t w = F.nl16[(size_t)(k >> 1) * n + s];
+      j = uslots[min(((k & 1u) ? (w >> 16) : w) & (PF_TU_MAX - 1u), U - 1u)];
+    } else {
+      j = F.nl[(size_t)k * n + s];
+    }
+    const uint32_t idj = S.id[p][j];
+    if (!(idj > id && idj < fend)) continue;
+    float3 dv;
+    if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
+    int* ai = reinterpret_cast<int*>(S.ct.acc + s);
+    int* aj = reinterpret_cast<int*>(S.ct.acc + j);
+    atomicAdd(ai + 0, pf_ticks(dv.x, F.imp_scale));
+    atomicAdd(ai + 1, pf_ticks(dv.y, F.imp_scale));
+    atomicAdd(ai + 2, pf_ticks(dv.z, F.imp_scale));
+    atomicAdd(aj + 0, pf_ticks(-dv.x, F.imp_scale));
+    atomicAdd(aj + 1, pf_ticks(-dv.y, F.imp_scale));
+    atomicAdd(aj + 2, pf_ticks(-dv.z, F.imp_scale));
+  }
+}
+
+template <bool ST, bool ADH, bool CT>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, int c) {
@@ -695,11 +752,20 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
+  // With contacts the pairs were scattered already (k_cohesion_scatter); CT:
+  // some scene of the launch has contacts.
+  const bool scattered = CT && st_on && S.ct.on;
+  const bool st_pairs = st_on && !scattered;
+  if (scattered) {
+    const int4 a = S.ct.acc[s];
+    S.ct.acc[s] = make_int4(0, 0, 0, 0);
+    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+  }
   const uint32_t fend = F.start + F.count;
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, tk);
   } else {
     // Four neighbours at a time, every gather of the four issued before the
     // arithmetic (the pass is latency-bound). Each pair is evaluated from its
@@ -727,7 +793,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
       for (int u = 0; u < 4; ++u) {
         idj[u] = S.id[p][jj[u]];
         xj[u] = xs[jj[u]];
-        if (st_on) {
+        if (st_pairs) {
           nj[u] = F.normal[jj[u]];
           rj[u] = S.rho[p][jj[u]];
           lastj[u] = F.nl_last[jj[u]];
@@ -738,7 +804,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
       for (int u = 0; u < 4; ++u) {
         const bool in = k0 + u < cnt;
         const float3 xjv = pf_xyz(xj[u]);
-        if (st_on) {
+        if (st_pairs) {
           const bool lower = in && idj[u] > id && idj[u] < fend;
           bool upper = in && idj[u] < id && idj[u] >= F.start;
           if (upper && lastj[u] != 0xffffffffu) {
@@ -1236,12 +1302,21 @@ __global__ __launch_bounds__(PF_BLOCK) void k_walls(const SceneDev* __restrict__
   const float w = S.w[p][s];
   const float4 x = S.x[p][s];
   int tk[3] = {0, 0, 0};
+  if (S.ct.on) {  // the contact solve's sums share the accumulator (loop.ts:183-236)
+    const int4 a = S.ct.acc[s];
+    S.ct.acc[s] = make_int4(0, 0, 0, 0);
+    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+  }
   walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
   sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
-  bool sat;
-  const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
-  if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
-  xn = pf_add3(xn, ch);
+  // A scene in the same launch with neither colliders nor contacts has no
+  // accumulator apply in its own schedule.
+  if (S.nprim || S.nsdf || S.ct.on) {
+    bool sat;
+    const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
+    if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
+    xn = pf_add3(xn, ch);
+  }
   if (ADVECT && w > 0.0f) {
     S.v[p][s] = make_float4((xn.x - x.x) / S.dt, (xn.y - x.y) / S.dt, (xn.z - x.z) / S.dt, 0.0f);
     S.x[p][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
@@ -1518,8 +1593,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restric
   const SceneDev& S = scenes[t.scene];
   const uint32_t s = t.first + threadIdx.x;
   if (s >= S.n) return;
+  if (!(S.nprim || S.nsdf || S.ct.on)) return;  // nothing to apply in this scene
   float4 v = S.v[p][s];
   int tk[3] = {0, 0, 0};
+  if (S.ct.on) {  // the contacts' kinetic friction sums (loop.ts:226-230)
+    const int4 a = S.ct.acc[s];
+    S.ct.acc[s] = make_int4(0, 0, 0, 0);
+    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+  }
   walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
   sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
   bool sat;
@@ -1539,21 +1620,30 @@ __global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restric
 #define PF_GRID_LDS(lds) dim3(ntiles), dim3(PF_BLOCK), (lds), st
 
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                      int p, int c, bool normal, uint32_t cap) {
+                      int p, int c, bool normal, uint32_t cap, int snap) {
   if (!ntiles) return;
   const size_t lds = (size_t)cap * sizeof(float4);
-  if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
-  else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
+  if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
+  else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
+}
+
+void launch_cohesion_scatter(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                             int p, int c) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_cohesion_scatter, PF_GRID, sc, tl, ntiles, fi, p, c);
 }
 
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, int c, bool surface_tension, bool adhesion) {
+                     int p, int c, bool surface_tension, bool adhesion, bool contact) {
   if (!ntiles) return;
-#define PF_IK(a, b) hipLaunchKernelGGL((k_impulses<a, b>), PF_GRID, sc, tl, ntiles, fi, p, c)
-  if (surface_tension && adhesion) PF_IK(true, true);
-  else if (surface_tension) PF_IK(true, false);
-  else if (adhesion) PF_IK(false, true);
-  else PF_IK(false, false);
+#define PF_IK(a, b, t) hipLaunchKernelGGL((k_impulses<a, b, t>), PF_GRID, sc, tl, ntiles, fi, p, c)
+  if (surface_tension && contact) {
+    if (adhesion) PF_IK(true, true, true);
+    else PF_IK(true, false, true);
+  } else if (surface_tension && adhesion) PF_IK(true, true, false);
+  else if (surface_tension) PF_IK(true, false, false);
+  else if (adhesion) PF_IK(false, true, false);
+  else PF_IK(false, false, false);
 #undef PF_IK
 }
 

@@ -68,6 +68,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_hash(const SceneDev* __restrict__ 
     __hip_atomic_store(flag_word(S, 0, PF_FLAG_POSITIONS), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(flag_word(S, 0, PF_FLAG_VELOCITIES), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  if (MODE == 1 && S.ct.on && t.first == 0 && threadIdx.x == 0) {
+    // Substep start: the contact counter (resetCounterKernel, ContactBuffer.ts:62-65).
+    __hip_atomic_store(S.ct.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if (valid) {
     float4 p;
     if (MODE == 0) {

@@ -140,6 +140,26 @@ struct SdfDev {
 };
 #define PF_MAX_SDF 4
 
+// Particle-particle contacts (src/core/contact, loop.ts:346-418). Records hold
+// sorted slots of the substep that generated them; the multipliers are
+// fixed point, 2^20 ticks per kg m (ContactBuffer.ts:16).
+#define PF_MAX_QUIET 8
+#define PF_CT_LAMBDA_SCALE 1048576.0f
+struct ContactDev {
+  uint32_t on;               // the scene's loop has contacts
+  uint32_t max;              // maxContacts
+  uint32_t blocks;           // workgroups of the per-contact passes (grid stride)
+  uint32_t nquiet;           // extra noSelfContacts ranges (fluids are always quiet)
+  uint32_t qstart[PF_MAX_QUIET], qcount[PF_MAX_QUIET];
+  float dist, cr2;           // 2r and (2r * 1.1)^2 (generate.ts:19)
+  float mus, muk;
+  uint32_t* count;           // [1] pairs emitted this substep, dropped ones included
+  uint2* pair;               // [max] (emitter slot, other slot)
+  int2* lam;                 // [max] (lambda_n, lambda_t) ticks
+  float4* nrm;               // [max] last normal the solve used
+  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes
+};
+
 struct SceneDev {
   uint32_t n;          // particle capacity
   uint32_t H;          // hash table size (power of two)
@@ -192,6 +212,7 @@ struct SceneDev {
   float clock[PF_MAX_SUBSTEPS];                 // sweep clock per substep (PrimitiveSet.ts:393-398)
   FluidDev fluid[PF_MAX_FLUIDS];
   SoftDev soft;
+  ContactDev ct;
 };
 
 

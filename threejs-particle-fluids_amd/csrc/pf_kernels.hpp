@@ -18,9 +18,12 @@ void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint
 // fluid.hip. `cap`: tile-union entries of dynamic LDS per workgroup
 // (PF_BLOCK..PF_TU_MAX); the neighbour build keeps unions within it.
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                      int p, int c, bool normal, uint32_t cap);
+                      int p, int c, bool normal, uint32_t cap, int snap);
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, int c, bool surface_tension, bool adhesion);
+                     int p, int c, bool surface_tension, bool adhesion, bool contact);
+// surface-tension pairs scattered into the contact accumulator (scenes with contacts)
+void launch_cohesion_scatter(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
+                             int p, int c);
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                    int p, int c, uint32_t cap);
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
@@ -42,6 +45,14 @@ void launch_viscosity_solve(hipStream_t st, const SceneDev* sc, const Tile* tl, 
                             int p, uint32_t sweeps);
 void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,
                      int c, int substep);
+
+// contact.hip: pair search (slot tiles), then the per-contact passes over
+// contact tiles (scene, first record) in a grid-stride loop
+void launch_contact_generate(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c);
+void launch_contact_stabilize(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles,
+                              const Tile* ctl, uint32_t nctiles, int p, int c);
+void launch_contact_solve(hipStream_t st, const SceneDev* sc, const Tile* ctl, uint32_t nctiles, int p, int c);
+void launch_contact_friction(hipStream_t st, const SceneDev* sc, const Tile* ctl, uint32_t nctiles, int p);
 
 // softbody.hip
 void launch_shape_match(hipStream_t st, const SceneDev* sc, const uint2* bodies, uint32_t nbodies,

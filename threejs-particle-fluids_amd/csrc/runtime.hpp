@@ -269,6 +269,7 @@ struct pf_softbody_s {
   float4* d_q = nullptr;
   float4* d_qp = nullptr;
   float4* d_omega = nullptr;
+  bool self_collision = false;   // selfCollision (SoftbodySystem.ts:71-72)
   pf_loop_s* loop = nullptr;
   bool disposed = false;
 };
@@ -286,6 +287,8 @@ struct PlanShape {
   bool local_soft = false;  // any scene's soft bodies use local shape matching
   bool walls = false;       // any scene has primitives
   bool soft = false;        // any scene has soft bodies
+  bool grid = false;        // the scenes sort by hash bucket every substep (fluids or contacts)
+  bool contact = false;     // any scene has particle contacts
 };
 
 struct pf_loop_s {
@@ -307,6 +310,9 @@ struct pf_loop_s {
   uint32_t nbtiles = 0;
   uint2* ltiles = nullptr;   // (scene, first id) tiles over the local soft-body span
   uint32_t nltiles = 0;
+  Tile* ctiles = nullptr;    // (scene, first contact) tiles of the per-contact passes
+  uint32_t nctiles = 0;
+  bool contact = false;      // SimLoop contact option (loop.ts:23-49)
   bool static_boundaries = false;  // a fluid has a static boundary (volumes computed on the first step)
   double last_dt = -1;
   uint32_t last_flag_half = 0;
@@ -331,6 +337,8 @@ struct pf_batch_s {
   uint32_t nbtiles = 0;
   uint2* ltiles = nullptr;
   uint32_t nltiles = 0;
+  Tile* ctiles = nullptr;
+  uint32_t nctiles = 0;
   void* scan_ctl = nullptr;
   PlanShape shape;
   bool use_graph = false;
@@ -344,13 +352,15 @@ namespace pf {
 uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& shape,
                       const Tile* tiles, uint32_t ntiles, const uint2* stiles, uint32_t nstiles,
                       const uint2* btiles, uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles,
-                      void* scan_ctl);
+                      const Tile* ctiles, uint32_t nctiles, void* scan_ctl);
 int step_variant_flips(const PlanShape& sh);
 // Multi-scene launches (pf_batch, pf_server; sim.hip).
 PlanShape loop_plan_shape(const pf_loop_s* L);
 void plan_merge(PlanShape& into, const PlanShape& scene);
 void loop_append_tiles(const pf_loop_s* L, uint32_t scene, std::vector<Tile>& tiles, std::vector<uint2>& stiles,
-                       std::vector<uint2>& btiles, std::vector<uint2>& ltiles);
+                       std::vector<uint2>& btiles, std::vector<uint2>& ltiles, std::vector<Tile>& ctiles);
+// The loop sorts its particles into a hash grid every substep.
+bool loop_has_grid(const pf_loop_s* L);
 bool loop_prepare_step(pf_loop_s* L, double dt);
 void loop_finish_step(pf_loop_s* L, int flips);
 }  // namespace pf

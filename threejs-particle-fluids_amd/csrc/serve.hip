@@ -200,7 +200,7 @@ struct pf_server_s {
     sh.substeps = substeps;
     sh.iterations = iterations;
     std::vector<SceneDev> descs;
-    std::vector<Tile> tl;
+    std::vector<Tile> tl, ctl;
     std::vector<uint2> stl, btl, ltl;
     descs.reserve(run.size());
     for (uint32_t i = 0; i < run.size(); ++i) {
@@ -208,14 +208,15 @@ struct pf_server_s {
       pf_loop_s* L = scenes[r.scene].loop;
       loop_prepare_step(L, r.dt);
       plan_merge(sh, loop_plan_shape(L));
-      loop_append_tiles(L, i, tl, stl, btl, ltl);
+      loop_append_tiles(L, i, tl, stl, btl, ltl, ctl);
       descs.push_back(L->ps->desc(L->ps->variant));
     }
     const size_t o_tl = align256(descs.size() * sizeof(SceneDev));
     const size_t o_st = o_tl + align256(tl.size() * sizeof(Tile));
     const size_t o_bt = o_st + align256(stl.size() * sizeof(uint2));
     const size_t o_lt = o_bt + align256(btl.size() * sizeof(uint2));
-    const size_t total = o_lt + align256(std::max<size_t>(ltl.size(), 1) * sizeof(uint2));
+    const size_t o_ct = o_lt + align256(std::max<size_t>(ltl.size(), 1) * sizeof(uint2));
+    const size_t total = o_ct + align256(std::max<size_t>(ctl.size(), 1) * sizeof(Tile));
     if (total > s.cap) {
       if (s.d_blob) PF_HIP(hipFree(s.d_blob));
       if (s.h_blob) PF_HIP(hipHostFree(s.h_blob));
@@ -232,12 +233,14 @@ struct pf_server_s {
     std::memcpy(s.h_blob + o_st, stl.data(), stl.size() * sizeof(uint2));
     std::memcpy(s.h_blob + o_bt, btl.data(), btl.size() * sizeof(uint2));
     if (!ltl.empty()) std::memcpy(s.h_blob + o_lt, ltl.data(), ltl.size() * sizeof(uint2));
+    if (!ctl.empty()) std::memcpy(s.h_blob + o_ct, ctl.data(), ctl.size() * sizeof(Tile));
     PF_HIP(hipMemcpyAsync(s.d_blob, s.h_blob, total, hipMemcpyHostToDevice, s.ctx.stream));
     enqueue_step(&s.ctx, reinterpret_cast<const SceneDev*>(s.d_blob), sh,
                  reinterpret_cast<const Tile*>(s.d_blob + o_tl), (uint32_t)tl.size(),
                  reinterpret_cast<const uint2*>(s.d_blob + o_st), (uint32_t)stl.size(),
                  reinterpret_cast<const uint2*>(s.d_blob + o_bt), (uint32_t)btl.size(),
-                 reinterpret_cast<const uint2*>(s.d_blob + o_lt), (uint32_t)ltl.size(), s.ctx.scan_ctl);
+                 reinterpret_cast<const uint2*>(s.d_blob + o_lt), (uint32_t)ltl.size(),
+                 reinterpret_cast<const Tile*>(s.d_blob + o_ct), (uint32_t)ctl.size(), s.ctx.scan_ctl);
     PF_HIP(hipGetLastError());
     PF_HIP(hipEventRecord(s.done, s.ctx.stream));
 
@@ -378,7 +381,7 @@ int pf_server_add_scene(pf_server* S, pf_loop* L, uint32_t* scene_id) {
   if (L->ctx != S->ctx) fail(PF_EINVAL, "Server: the loop must use the server's context");
   if (!L->ps->arena) fail(PF_EINVAL, "Server: the scene's particles must be created evictable");
   if (L->ps->server) fail(PF_EINVAL, "Server: the scene is already served");
-  const int has_fluid = L->fluids.empty() ? 0 : 1;
+  const int has_fluid = loop_has_grid(L) ? 1 : 0;
   PF_HIP(hipStreamSynchronize(L->ctx->stream));  // the owner's last plain step
   std::unique_lock<std::mutex> lk(S->mu);
   S->check_error();
@@ -390,7 +393,8 @@ int pf_server_add_scene(pf_server* S, pf_loop* L, uint32_t* scene_id) {
     if (L->substeps != S->substeps || L->iterations != S->iterations)
       fail(PF_EINVAL, "Server: every scene must share substeps and iterations (%u x %u)", S->substeps,
            S->iterations);
-    if (has_fluid != S->has_fluid) fail(PF_EINVAL, "Server: mixing scenes with and without fluid is not supported");
+    if (has_fluid != S->has_fluid)
+      fail(PF_EINVAL, "Server: mixing scenes with and without a neighbour grid (fluids or contacts) is not supported");
   }
   ServedScene sc;
   sc.loop = L;

@@ -73,20 +73,24 @@ __global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __res
 // Returns the number of kernel launches.
 uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh, const Tile* tiles,
                       uint32_t ntiles, const uint2* stiles, uint32_t nstiles, const uint2* btiles,
-                      uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles, void* scan_ctl) {
+                      uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles, const Tile* ctiles,
+                      uint32_t nctiles, void* scan_ctl) {
   hipStream_t st = ctx->stream;
   uint32_t nk = 0;
   int c = 0;
-  const bool grid = sh.nfluids > 0;
+  const bool grid = sh.grid;
   enum LastOp { NONE, DELTA, WALLS, SOFT };
   // The last fluid's correction pass can carry the container contact and the
   // advect only when nothing runs after it inside an iteration (soft bodies,
   // a boundary reaction apply).
   bool reaction = false;
   for (uint32_t f = 0; f < sh.nfluids; ++f) reaction = reaction || sh.nb[f] > 0;
-  const bool delta_can_end = !sh.soft && !reaction && sh.nfluids > 0;
+  // Contacts solve after every material and before the colliders, so the
+  // accumulator apply cannot ride on the fluid's correction pass.
+  const bool delta_can_end = !sh.soft && !reaction && sh.nfluids > 0 && !sh.contact;
+  const bool apply = sh.walls || sh.contact;  // a positions/velocities accumulator exists
   LastOp last = NONE;
-  if (sh.walls) last = delta_can_end ? DELTA : WALLS;
+  if (apply) last = delta_can_end ? DELTA : WALLS;
   else if (sh.soft) last = SOFT;
   else if (delta_can_end) last = DELTA;
   for (uint32_t k = 0; k < sh.substeps; ++k) {
@@ -99,6 +103,18 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       { PF_LAUNCH(ctx, "rank_permute"); launch_rank(st, scenes, tiles, ntiles, 1, p, c, c ^ 1, nullptr, nullptr, nullptr); ++nk; }
       c ^= 1;
       p ^= 1;
+      if (sh.contact) {
+        // Contacts (loop.ts:370-393): the pair search on this substep's
+        // predictions, then the stabilisation, whose predictions go to the
+        // other buffer so the grid's sorted copy stays for the neighbour build.
+        { PF_LAUNCH(ctx, "contact_generate"); launch_contact_generate(st, scenes, tiles, ntiles, p, c); ++nk; }
+        {
+          PF_LAUNCH(ctx, "contact_stabilize");
+          launch_contact_stabilize(st, scenes, tiles, ntiles, ctiles, nctiles, p, c);
+          nk += 2;
+        }
+        c ^= 1;
+      }
     } else {
       PF_LAUNCH(ctx, "predict");
       hipLaunchKernelGGL(k_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, 0, c, k == 0);
@@ -120,15 +136,20 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       PF_LAUNCH(ctx, "neighbors");
       launch_neighbors(st, scenes, tiles, ntiles, (int)f, p, c, (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0,
-                       sh.tu_cap[f]);
+                       sh.tu_cap[f], sh.contact ? 1 : 0);
       ++nk;
     }
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       const bool stx = (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0;
       const bool adh = (sh.fx[f] & PF_FX_ADHESION) != 0;
+      if (stx && sh.contact) {
+        PF_LAUNCH(ctx, "cohesion_scatter");
+        launch_cohesion_scatter(st, scenes, tiles, ntiles, (int)f, p, c);
+        ++nk;
+      }
       if (stx || adh) {
         PF_LAUNCH(ctx, "impulses");
-        launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary);
+        launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary, sh.contact);
         ++nk;
         c ^= 1;
       }
@@ -163,7 +184,12 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
         launch_local_solve(st, scenes, ltiles, nltiles, p, c, it == 0);
         nk += 5;
       }
-      if (sh.walls && last == WALLS) {
+      if (sh.contact) {
+        PF_LAUNCH(ctx, "contact_solve");
+        launch_contact_solve(st, scenes, ctiles, nctiles, p, c);
+        ++nk;
+      }
+      if (apply && last == WALLS) {
         PF_LAUNCH(ctx, "walls");
         launch_walls(st, scenes, tiles, ntiles, p, c, lastit, it == 0, (int)k);
         ++nk;
@@ -181,7 +207,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       nk += 2;
     }
     const int touch_valid = sh.iterations > 0 ? 1 : 0;
-    bool friction_done = !sh.walls;
+    bool friction_done = !apply;
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       const bool vort = (sh.fx[f] & PF_FX_VORTICITY) != 0;
       const bool visc = (sh.fx[f] & PF_FX_VISCOSITY) != 0;
@@ -193,7 +219,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       }
       // Friction follows every material's post-solve (loop.ts:241): it fuses
       // into the last fluid's vorticity pass unless a viscosity solver runs after it.
-      const bool fric = lastf && sh.walls && sh.isv[f] == 0u;
+      const bool fric = lastf && sh.walls && sh.isv[f] == 0u && !sh.contact;
       if (vort || visc || fric) {
         PF_LAUNCH(ctx, fric ? "vorticity_friction" : "vorticity");
         launch_vorticity(st, scenes, tiles, ntiles, (int)f, p, c, vort, visc, fric, (int)k, touch_valid,
@@ -208,6 +234,11 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       }
     }
     if (!friction_done) {
+      if (sh.contact && touch_valid) {
+        PF_LAUNCH(ctx, "contact_friction");
+        launch_contact_friction(st, scenes, ctiles, nctiles, p);
+        ++nk;
+      }
       PF_LAUNCH(ctx, "friction");
       if (touch_valid) launch_friction(st, scenes, tiles, ntiles, p, c, (int)k);
       ++nk;
@@ -219,10 +250,11 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
 // Parity bookkeeping of one step: returns the descriptor variant flips.
 int step_variant_flips(const PlanShape& sh) {
   int state = 0, xs = 0;
-  const bool grid = sh.nfluids > 0;
+  const bool grid = sh.grid;
   for (uint32_t k = 0; k < sh.substeps; ++k) {
     if (grid) state ^= 1;
     xs ^= 1;  // hash/predict
+    if (grid && sh.contact) xs ^= 1;  // contact stabilisation
     for (uint32_t f = 0; f < sh.nfluids; ++f)
       if (sh.fx[f] & (PF_FX_SURFACE_TENSION | PF_FX_ADHESION)) xs ^= 1;
     for (uint32_t it = 0; it < sh.iterations; ++it)
@@ -1070,6 +1102,14 @@ int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t n
   PF_API_END
 }
 
+int pf_softbody_set_self_collision(pf_softbody* sb, int enabled) {
+  PF_API_BEGIN
+  if (!sb || sb->disposed) fail(PF_ESTATE, "SoftbodySystem: the system has been disposed");
+  if (sb->loop) fail(PF_ESTATE, "SoftbodySystem: selfCollision is fixed once a SimLoop is built");
+  sb->self_collision = enabled != 0;
+  PF_API_END
+}
+
 int pf_softbody_set_compliance(pf_softbody* sb, uint32_t index, double compliance) {
   PF_API_BEGIN
   if (!sb || sb->disposed) fail(PF_ESTATE, "SoftbodySystem: disposed");
@@ -1192,6 +1232,8 @@ static PlanShape loop_shape(const pf_loop* L) {
   sh.nfluids = (uint32_t)L->fluids.size();
   for (uint32_t f = 0; f < sh.nfluids; ++f) sh.fx[f] = L->ps->host.fluid[f].effects;
   sh.walls = (L->prims && !L->prims->prims.empty()) || !L->sdfs.empty();
+  sh.contact = L->contact;
+  sh.grid = loop_has_grid(L);
   sh.soft = L->soft != nullptr;
   sh.local_soft = L->soft != nullptr && L->soft->mode == PF_SHAPE_MATCHING_LOCAL;
   sh.any_boundary = L->ps->host.has_boundary != 0;
@@ -1282,6 +1324,44 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
       L->sdfs.push_back(c);
     } else {
       fail(PF_EINVAL, "SimLoop: unknown collider kind %u", colliders[k].kind);
+    }
+  }
+  // Contacts (loop.ts:148-153, 346-418).
+  double ct_mus = 0.5, ct_muk = 0.4;
+  uint32_t ct_max = 0;
+  if (opt && opt->contact) {
+    ct_mus = opt->contact_mu_s;
+    ct_muk = opt->contact_mu_k;
+    friction_defaults(ct_mus, ct_muk, "SimLoop");
+    ct_max = opt->max_contacts ? opt->max_contacts : 8u * ps->n;
+    if (opt->nquiet > PF_MAX_QUIET) fail(PF_EINVAL, "SimLoop: at most %d quiet ranges", PF_MAX_QUIET);
+    if (opt->nquiet && !opt->quiet) fail(PF_EINVAL, "SimLoop: quiet ranges are missing");
+    for (uint32_t q = 0; q < opt->nquiet; ++q) range_ok(ps, opt->quiet[2 * q], opt->quiet[2 * q + 1], "SimLoop");
+    L->contact = true;
+    neighbor_radius = std::max(neighbor_radius, 2.0 * ps->radius * 1.1);  // CONTACT_RADIUS_EXPANSION
+  }
+  // Collision groups handed out above the uploaded ones (loop.ts:164-168);
+  // without selfCollision each soft body gets its own, unless the whole body
+  // already shares a non-zero group (SoftbodySystem.ts:218-233).
+  if (L->soft && !L->soft->self_collision) {
+    uint32_t next = 0;
+    for (uint32_t g : ps->cpu_groups) next = std::max(next, g);
+    next += 1;
+    const auto& bodies = L->soft->bodies;
+    for (size_t b = 0; b < bodies.size(); ++b) {
+      const uint32_t g0 = ps->cpu_groups[bodies[b].start];
+      bool shared = true;
+      for (uint32_t k = 1; k < bodies[b].count; ++k) shared = shared && ps->cpu_groups[bodies[b].start + k] == g0;
+      if (!shared)
+        fail(PF_EINVAL, "SoftbodySystem: body %zu has mixed collision groups; give all its particles one group, or none", b);
+    }
+    for (size_t b = 0; b < bodies.size(); ++b) {
+      const uint32_t g0 = ps->cpu_groups[bodies[b].start];
+      if (g0 != 0u) continue;
+      const uint32_t group = next++;
+      std::vector<uint32_t> g(bodies[b].count, group);
+      for (uint32_t k = 0; k < bodies[b].count; ++k) ps->cpu_groups[bodies[b].start + k] = group;
+      particles_scatter(ps, bodies[b].start, bodies[b].count, nullptr, nullptr, nullptr, &g, 8u);
     }
   }
   SceneDev& S = ps->host;
@@ -1396,6 +1476,46 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     B.ai = (float)((ps->radius * ps->radius) / 5.0);
     B.maxc2 = (float)((5.0 * ps->radius) * (5.0 * ps->radius));
   }
+  S.ct = ContactDev{};
+  if (L->contact) {
+    ContactDev& C = S.ct;
+    C.on = 1u;
+    C.max = ct_max;
+    // Workgroups of the per-contact passes (grid-stride over the emitted
+    // count): one record per emitting particle in flight, at most 1024.
+    std::vector<uint8_t> quiet_mask(n, 0);
+    for (auto* f : L->fluids) std::fill(quiet_mask.begin() + f->start, quiet_mask.begin() + f->start + f->count, 1);
+    for (uint32_t q = 0; q < opt->nquiet; ++q)
+      std::fill(quiet_mask.begin() + opt->quiet[2 * q], quiet_mask.begin() + opt->quiet[2 * q] + opt->quiet[2 * q + 1], 1);
+    uint32_t emitters = 0;
+    for (uint8_t m : quiet_mask) emitters += m ? 0u : 1u;
+    const uint32_t want = std::min(std::max(emitters, 1u), ct_max);
+    C.blocks = std::min<uint32_t>((want + PF_BLOCK - 1) / PF_BLOCK, 1024u);
+    C.nquiet = opt->nquiet;
+    for (uint32_t q = 0; q < opt->nquiet; ++q) {
+      C.qstart[q] = opt->quiet[2 * q];
+      C.qcount[q] = opt->quiet[2 * q + 1];
+    }
+    const double cr = 2.0 * ps->radius * 1.1;
+    C.dist = (float)(2.0 * ps->radius);
+    C.cr2 = (float)(cr * cr);
+    C.mus = (float)ct_mus;
+    C.muk = (float)ct_muk;
+    C.count = dalloc<uint32_t>(1);
+    C.pair = dalloc<uint2>(ct_max);
+    C.lam = dalloc<int2>(ct_max);
+    C.nrm = dalloc<float4>(ct_max);
+    C.acc = dalloc<int4>(n);
+    PF_HIP(hipMemset(C.count, 0, sizeof(uint32_t)));
+    PF_HIP(hipMemset(C.acc, 0, (size_t)n * sizeof(int4)));
+    PF_HIP(hipMemset(C.nrm, 0, (size_t)ct_max * sizeof(float4)));
+    PF_HIP(hipMemset(C.lam, 0, (size_t)ct_max * sizeof(int2)));
+    L->nctiles = C.blocks;
+    std::vector<Tile> ctl(L->nctiles);
+    for (uint32_t i = 0; i < L->nctiles; ++i) ctl[i] = Tile{0u, i * PF_BLOCK};
+    L->ctiles = dalloc<Tile>(L->nctiles);
+    PF_HIP(hipMemcpy(L->ctiles, ctl.data(), ctl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  }
   // Work tiles.
   L->ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
   std::vector<Tile> tl(L->ntiles);
@@ -1503,14 +1623,20 @@ void plan_merge(PlanShape& sh, const PlanShape& ls) {
     sh.bdyn[f] |= ls.bdyn[f];
   }
   sh.walls = sh.walls || ls.walls;
+  sh.contact = sh.contact || ls.contact;
+  sh.grid = sh.grid || ls.grid;
   sh.soft = sh.soft || ls.soft;
   sh.local_soft = sh.local_soft || ls.local_soft;
   sh.any_boundary = sh.any_boundary || ls.any_boundary;
 }
 
+bool loop_has_grid(const pf_loop_s* L) { return !L->fluids.empty() || L->contact; }
+
 void loop_append_tiles(const pf_loop_s* L, uint32_t scene, std::vector<Tile>& tl, std::vector<uint2>& stl,
-                       std::vector<uint2>& btl, std::vector<uint2>& ltl) {
+                       std::vector<uint2>& btl, std::vector<uint2>& ltl, std::vector<Tile>& ctl) {
   for (uint32_t i = 0; i < L->ntiles; ++i) tl.push_back(Tile{scene, i * PF_BLOCK});
+  if (L->contact)
+    for (uint32_t i = 0; i < L->ps->host.ct.blocks; ++i) ctl.push_back(Tile{scene, i * PF_BLOCK});
   for (uint32_t i = 0; i < L->nstiles; ++i) stl.push_back(make_uint2(scene, i));
   for (uint32_t i = 0; i < L->nbtiles; ++i) btl.push_back(make_uint2(scene, i));
   if (L->soft && L->soft->mode == PF_SHAPE_MATCHING_LOCAL) {
@@ -1565,7 +1691,8 @@ int pf_loop_step(pf_loop* L, double dt) {
       ctx->capturing = true;
       try {
         L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles,
-                                           L->btiles, L->nbtiles, L->ltiles, L->nltiles, ctx->scan_ctl);
+                                           L->btiles, L->nbtiles, L->ltiles, L->nltiles, L->ctiles, L->nctiles,
+                                           ctx->scan_ctl);
       } catch (...) {
         ctx->capturing = false;
         hipGraph_t dummy;
@@ -1581,7 +1708,7 @@ int pf_loop_step(pf_loop* L, double dt) {
     PF_HIP(hipGraphLaunch(L->graph[v], ctx->stream));
   } else {
     L->kernels_per_step = enqueue_step(ctx, ps->cur(), sh, L->tiles, L->ntiles, L->stiles, L->nstiles, L->btiles,
-                                       L->nbtiles, L->ltiles, L->nltiles, ctx->scan_ctl);
+                                       L->nbtiles, L->ltiles, L->nltiles, L->ctiles, L->nctiles, ctx->scan_ctl);
   }
   PF_HIP(hipGetLastError());
   ps->variant ^= step_variant_flips(sh);
@@ -1610,9 +1737,41 @@ int pf_loop_readback_overflow(pf_loop* L, pf_overflow* o) {
   o->velocities = fl[PF_FLAG_VELOCITIES] != 0;
   o->grid = last[PF_FLAG_GRID] != 0;
   o->contacts = 0;
+  if (L->contact) {  // the last substep found more pairs than it could keep (loop.ts:292)
+    uint32_t cnt = 0;
+    PF_HIP(hipMemcpy(&cnt, L->ps->host.ct.count, sizeof cnt, hipMemcpyDeviceToHost));
+    o->contacts = cnt > L->ps->host.ct.max;
+  }
   o->neighbors = last[PF_FLAG_NEIGHBORS] != 0;
   o->impulses = fl[PF_FLAG_IMPULSES] != 0;
   o->long_bucket = last[PF_FLAG_LONG_BUCKET] != 0;
+  PF_API_END
+}
+
+int pf_loop_readback_contacts(pf_loop* L, uint32_t* count, uint32_t* pairs, uint32_t cap, uint32_t* stored) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.contacts");
+  if (!L->contact) fail(PF_EINVAL, "SimLoop: the loop has no contacts");
+  PF_HIP(hipStreamSynchronize(L->ctx->stream));
+  const ContactDev& C = L->ps->host.ct;
+  uint32_t cnt = 0;
+  PF_HIP(hipMemcpy(&cnt, C.count, sizeof cnt, hipMemcpyDeviceToHost));
+  const uint32_t m = pairs ? std::min(std::min(cnt, C.max), cap) : 0u;
+  if (m) {
+    std::vector<uint2> pr(m);
+    PF_HIP(hipMemcpy(pr.data(), C.pair, m * sizeof(uint2), hipMemcpyDeviceToHost));
+    // Records hold slots of the last substep's order, which the state keeps
+    // until the next step sorts again.
+    std::vector<uint32_t> ids(L->ps->n);
+    const SceneDev d = L->ps->desc(L->ps->variant);
+    PF_HIP(hipMemcpy(ids.data(), d.id[0], ids.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < m; ++k) {
+      pairs[2 * k] = ids[pr[k].x];
+      pairs[2 * k + 1] = ids[pr[k].y];
+    }
+  }
+  if (count) *count = cnt;
+  if (stored) *stored = m;
   PF_API_END
 }
 
@@ -1681,7 +1840,9 @@ int pf_loop_destroy(pf_loop* L) {
   dfree(S.tmp_src); dfree(S.tmp_id); dfree(S.tmp_key); dfree(S.prim_lambda); dfree(S.prim_touch);
   S.key = S.rank = S.counts = S.start = S.tmp_src = S.tmp_id = S.tmp_key = nullptr;
   S.scan_status = nullptr; S.scan_ctl = nullptr; S.prim_lambda = nullptr; S.prim_touch = nullptr;
-  dfree(L->tiles); dfree(L->stiles); dfree(L->btiles); dfree(L->ltiles);
+  dfree(L->tiles); dfree(L->stiles); dfree(L->btiles); dfree(L->ltiles); dfree(L->ctiles);
+  dfree(S.ct.count); dfree(S.ct.pair); dfree(S.ct.lam); dfree(S.ct.nrm); dfree(S.ct.acc);
+  S.ct = ContactDev{};
   for (auto* f : L->fluids) {
     f->loop = nullptr;
     dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv);
@@ -1719,7 +1880,7 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
   PlanShape sh;
   sh.substeps = loops[0]->substeps;
   sh.iterations = loops[0]->iterations;
-  std::vector<Tile> tl;
+  std::vector<Tile> tl, ctl;
   std::vector<uint2> stl, btl, ltl;
   for (uint32_t s = 0; s < n; ++s) {
     pf_loop* L = loops[s];
@@ -1730,12 +1891,12 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
     for (uint32_t t = 0; t < s; ++t)
       if (loops[t] == L) fail(PF_EINVAL, "Batch: a loop appears twice");
     plan_merge(sh, loop_shape(L));
-    loop_append_tiles(L, s, tl, stl, btl, ltl);
+    loop_append_tiles(L, s, tl, stl, btl, ltl, ctl);
     B->loops.push_back(L);
   }
-  if (sh.nfluids > 0)
-    for (auto* L : B->loops)
-      if (L->fluids.empty()) fail(PF_EINVAL, "Batch: mixing scenes with and without fluid is not supported");
+  for (auto* L : B->loops)
+    if (loop_has_grid(L) != sh.grid)
+      fail(PF_EINVAL, "Batch: mixing scenes with and without a neighbour grid (fluids or contacts) is not supported");
   B->shape = sh;
   B->ntiles = (uint32_t)tl.size();
   B->nstiles = (uint32_t)stl.size();
@@ -1747,6 +1908,11 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
   }
   B->tiles = dalloc<Tile>(B->ntiles);
   PF_HIP(hipMemcpy(B->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  B->nctiles = (uint32_t)ctl.size();
+  if (B->nctiles) {
+    B->ctiles = dalloc<Tile>(B->nctiles);
+    PF_HIP(hipMemcpy(B->ctiles, ctl.data(), ctl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  }
   if (B->nstiles) {
     B->stiles = dalloc<uint2>(B->nstiles);
     PF_HIP(hipMemcpy(B->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
@@ -1797,7 +1963,8 @@ int pf_batch_step(pf_batch* B, double dt) {
       PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       ctx->capturing = true;
       try {
-        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles, B->nltiles, B->scan_ctl);
+        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles,
+                     B->nltiles, B->ctiles, B->nctiles, B->scan_ctl);
       } catch (...) {
         ctx->capturing = false;
         hipGraph_t dummy;
@@ -1812,7 +1979,8 @@ int pf_batch_step(pf_batch* B, double dt) {
     }
     PF_HIP(hipGraphLaunch(B->graph, ctx->stream));
   } else {
-    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles, B->nltiles, B->scan_ctl);
+    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles,
+                     B->nltiles, B->ctiles, B->nctiles, B->scan_ctl);
   }
   PF_HIP(hipGetLastError());
   const int flips = step_variant_flips(B->shape);
@@ -1923,7 +2091,8 @@ int pf_batch_destroy(pf_batch* B) {
   if (!B) return PF_OK;
   (void)hipStreamSynchronize(B->ctx->stream);
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
-  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->ltiles); dfree(B->scan_ctl);
+  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->ltiles); dfree(B->ctiles);
+  dfree(B->scan_ctl);
   delete B;
   PF_API_END
 }

@@ -9,6 +9,7 @@
 // src/core/loop.ts:267-279).
 #include <node_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -548,6 +549,13 @@ napi_value SoftbodyCreate(napi_env env, napi_callback_info info) {
   PF_CALL(pf_softbody_create((pf_particles*)handle(env, a.argv[0]), defs.data(), nb, mode, &sb));
   return make_handle(env, sb);
 }
+napi_value SoftbodySetSelfCollision(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  bool b = false;
+  napi_get_value_bool(env, a.argv[1], &b);
+  PF_CALL(pf_softbody_set_self_collision((pf_softbody*)handle(env, a.argv[0]), b ? 1 : 0));
+  return nullptr;
+}
 napi_value SoftbodySetCompliance(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   PF_CALL(pf_softbody_set_compliance((pf_softbody*)handle(env, a.argv[0]), u32(env, a.argv[1]), num(env, a.argv[2])));
@@ -589,6 +597,19 @@ napi_value LoopCreate(napi_env env, napi_callback_info info) {
   if (ho.size() == 3) {
     o.has_hash_origin = 1;
     o.hash_origin[0] = ho[0]; o.hash_origin[1] = ho[1]; o.hash_origin[2] = ho[2];
+  }
+  // contact: {muS, muK, maxContacts} (defaults applied natively); quiet:
+  // flattened [start, count] ranges of materials' noSelfContacts
+  napi_value ct = prop(env, opts, "contact");
+  size_t nq = 0;
+  const uint32_t* quiet = typed<uint32_t>(env, prop(env, opts, "quiet"), &nq);
+  if (!is_nullish(env, ct)) {
+    o.contact = 1;
+    o.contact_mu_s = num(env, prop(env, ct, "muS"), -1.0);
+    o.contact_mu_k = num(env, prop(env, ct, "muK"), -1.0);
+    o.max_contacts = u32(env, prop(env, ct, "maxContacts"), 0);
+    o.nquiet = (uint32_t)(nq / 2);
+    o.quiet = quiet;
   }
   uint32_t nm = 0, nc = 0;
   if (!is_nullish(env, a.argv[2])) napi_get_array_length(env, a.argv[2], &nm);
@@ -675,6 +696,23 @@ napi_value LoopReadbackOverflow(napi_env env, napi_callback_info info) {
   set(env, obj, "velocities", boolean(env, o.velocities));
   set(env, obj, "grid", boolean(env, o.grid));
   set(env, obj, "contacts", boolean(env, o.contacts));
+  return obj;
+}
+// (loop, cap) -> {count, pairs: Uint32Array of the stored (i, j) pairs}
+napi_value LoopReadbackContacts(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  const uint32_t cap = u32(env, a.argv[1], 0);
+  std::vector<uint32_t> pairs(2 * (size_t)cap);
+  uint32_t count = 0, stored = 0;
+  PF_CALL(pf_loop_readback_contacts((pf_loop*)handle(env, a.argv[0]), &count, cap ? pairs.data() : nullptr, cap,
+                                    &stored));
+  uint32_t* out;
+  napi_value arr = u32_array(env, 2 * (size_t)stored, &out);
+  std::copy(pairs.begin(), pairs.begin() + 2 * (size_t)stored, out);
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "count", number(env, count));
+  set(env, obj, "pairs", arr);
   return obj;
 }
 napi_value LoopSetSubsteps(napi_env env, napi_callback_info info) {
@@ -1023,6 +1061,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"primitivesCount", PrimitivesCount}, {"primitivesDestroy", PrimitivesDestroy},
       {"sdfCreate", SdfCreate}, {"sdfSet", SdfSet}, {"sdfDestroy", SdfDestroy},
       {"softbodyCreate", SoftbodyCreate}, {"softbodySetCompliance", SoftbodySetCompliance},
+      {"softbodySetSelfCollision", SoftbodySetSelfCollision}, {"loopReadbackContacts", LoopReadbackContacts},
       {"softbodyReadbackFrames", SoftbodyReadbackFrames}, {"softbodyDestroy", SoftbodyDestroy},
       {"loopCreate", LoopCreate}, {"loopStep", LoopStep}, {"loopStepAsync", LoopStepAsync},
       {"loopSynchronize", LoopSynchronize}, {"loopReadbackOverflow", LoopReadbackOverflow},

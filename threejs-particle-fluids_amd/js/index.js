@@ -257,6 +257,7 @@ class SoftbodySystem {
       })),
       mode === 'local' ? 1 : 0,
     );
+    if (options.selfCollision) native.softbodySetSelfCollision(this.handle, true);
   }
   setCompliance(index, compliance) {
     native.softbodySetCompliance(this.handle, index, compliance);
@@ -304,10 +305,48 @@ class SDFCollider {
   }
 }
 
+/** SimLoop.contacts (src/core/contact/ContactBuffer.ts:38-98). */
+class ContactBuffer {
+  constructor(loop, maxContacts) {
+    this.loop = loop;
+    this.maxContacts = maxContacts;
+  }
+  /** Pairs emitted in the last substep, dropped ones included. */
+  async readbackCount() {
+    return native.loopReadbackContacts(this.loop.handle, 0).count;
+  }
+  /** The stored (i, j) pairs, flattened. */
+  async readbackPairs() {
+    const r = native.loopReadbackContacts(this.loop.handle, this.maxContacts);
+    return { nContacts: r.pairs.length / 2, pairs: r.pairs };
+  }
+}
+
+/** A material's noSelfContacts range (materials.ts:45): either a native
+ * material, or a plain one whose build() returns only noSelfContacts. */
+function quietRange(m) {
+  if (m.material) return null;
+  const built = typeof m.build === 'function' ? m.build({}) : m;
+  const keys = Object.keys(built || {}).filter((k) => built[k] !== undefined);
+  if (!built || !built.noSelfContacts || keys.some((k) => k !== 'noSelfContacts')) {
+    throw new Error('SimLoop: materials other than the built-in ones may only declare noSelfContacts here');
+  }
+  return built.noSelfContacts;
+}
+
 class SimLoop {
   constructor(particles, options = {}) {
-    if (options.contact) throw new Error('SimLoop: particle contacts are not available in this build');
     this.particles = particles;
+    const materials = def(options.materials, []);
+    const quiet = [];
+    for (const m of materials) {
+      const r = quietRange(m);
+      if (r) quiet.push(r.start, r.count);
+    }
+    const c = options.contact === true ? {} : options.contact || undefined;
+    if (c && c.maxContacts !== undefined && !(Number.isInteger(c.maxContacts) && c.maxContacts > 0)) {
+      throw new Error(`ContactBuffer: maxContacts must be a positive integer, got ${c.maxContacts}`);
+    }
     this.handle = native.loopCreate(
       particles.handle,
       {
@@ -315,10 +354,13 @@ class SimLoop {
         iterations: options.iterations,
         gravity: options.gravity ? vec(options.gravity) : undefined,
         hashOrigin: options.hashOrigin ? vec(options.hashOrigin) : undefined,
+        contact: c ? { muS: c.muS, muK: c.muK, maxContacts: c.maxContacts } : undefined,
+        quiet: Uint32Array.from(quiet),
       },
-      (def(options.materials, [])).map((m) => m.material),
-      (def(options.colliders, [])).map((c) => c.collider),
+      materials.filter((m) => m.material).map((m) => m.material),
+      (def(options.colliders, [])).map((col) => col.collider),
     );
+    this.contacts = c ? new ContactBuffer(this, def(c.maxContacts, 8 * particles.capacity)) : undefined;
     this.stepping = null;
   }
   /** step(dt) (loop.ts:267-279): resolves when the GPU has finished the step. */
@@ -605,6 +647,7 @@ module.exports = {
   SDFCollider,
   SoftbodySystem,
   SimLoop,
+  ContactBuffer,
   Simulation,
   ViscositySolver,
   Batch,

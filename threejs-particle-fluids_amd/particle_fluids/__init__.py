@@ -18,7 +18,7 @@ from ._lib import PFError, call, load
 __all__ = [
     "PFError", "Context", "ParticleSystem", "HashGrid", "FluidSystem", "PrimitiveSet",
     "SoftbodySystem", "SDFCollider", "SimLoop", "Batch", "Simulation", "fill_box", "fit_budget",
-    "radius_for_budget",
+    "radius_for_budget", "NoSelfContacts", "ContactBuffer",
 ]
 
 P = C.POINTER
@@ -487,7 +487,7 @@ class SoftbodySystem:
     ``edges`` (local index pairs, required for ``shapeMatching="local"``).
     """
 
-    def __init__(self, particles, bodies, shapeMatching="global"):
+    def __init__(self, particles, bodies, shapeMatching="global", selfCollision=False):
         if shapeMatching not in ("global", "local"):
             raise PFError(1, "SoftbodySystem: shapeMatching must be 'global' or 'local'")
         self.particles = particles
@@ -517,6 +517,8 @@ class SoftbodySystem:
         call("pf_softbody_create", particles.handle, defs, len(bodies), 1 if shapeMatching == "local" else 0, C.byref(h))
         self.handle = h
         self.nbodies = len(bodies)
+        if selfCollision:
+            call("pf_softbody_set_self_collision", h, 1)
 
     def setCompliance(self, index, compliance):
         call("pf_softbody_set_compliance", self.handle, int(index), float(compliance))
@@ -550,15 +552,62 @@ def _autotune(fn, handle, dt, candidates, steps):
     return best.value, {int(c): float(t) for c, t in zip(cand, ms)}
 
 
+class NoSelfContacts:
+    """A material that only keeps its range from contacting itself: what a
+    Material whose build() returns just ``noSelfContacts`` does
+    (materials.ts:45, loop.ts:372-376)."""
+
+    def __init__(self, range_):
+        self.noSelfContacts = (int(range_[0]), int(range_[1]))
+
+
+class ContactBuffer:
+    """SimLoop.contacts (src/core/contact/ContactBuffer.ts:38-98)."""
+
+    def __init__(self, loop, maxContacts):
+        self._loop = loop
+        self.maxContacts = maxContacts
+
+    def readbackCount(self):
+        """Pairs emitted in the last substep, dropped ones included."""
+        n = C.c_uint32()
+        call("pf_loop_readback_contacts", self._loop.handle, C.byref(n), None, 0, None)
+        return n.value
+
+    def readbackPairs(self):
+        """(nContacts, pairs): the stored (i, j) pairs, flattened."""
+        pairs = np.zeros(2 * self.maxContacts, np.uint32)
+        n, stored = C.c_uint32(), C.c_uint32()
+        call("pf_loop_readback_contacts", self._loop.handle, C.byref(n), _ptr(pairs, u32p), self.maxContacts,
+             C.byref(stored))
+        return stored.value, pairs[:2 * stored.value].copy()
+
+
 class SimLoop:
     """src/core/loop.ts:92-324."""
 
     def __init__(self, particles, substeps=4, iterations=2, gravity=(0.0, -9.81, 0.0), materials=(),
                  colliders=(), hashOrigin=None, contact=None):
-        if contact:
-            raise PFError(1, "SimLoop: particle contacts are not available in this build")
         self.particles = particles
         o = _lib.LoopOptions()
+        self.contacts = None
+        quiet = [m.noSelfContacts for m in materials if isinstance(m, NoSelfContacts)]
+        materials = [m for m in materials if not isinstance(m, NoSelfContacts)]
+        self._quiet = np.ascontiguousarray(np.asarray(quiet, np.uint32).reshape(-1), np.uint32)
+        if contact is not None and contact is not False:
+            c = {} if contact is True else dict(contact)
+            o.contact = 1
+            o.contact_mu_s = float(c.get("muS", -1.0))
+            o.contact_mu_k = float(c.get("muK", -1.0))
+            if c.get("muS") is not None and not float(c["muS"]) >= 0 or c.get("muK") is not None and not float(c["muK"]) >= 0:
+                raise PFError(1, f"SimLoop: friction coefficients must be non-negative (got muS={c.get('muS')}, muK={c.get('muK')})")
+            mc = c.get("maxContacts")
+            if mc is not None and (int(mc) != mc or mc <= 0):
+                raise PFError(1, f"ContactBuffer: maxContacts must be a positive integer, got {mc}")
+            o.max_contacts = 0 if mc is None else int(mc)
+            o.nquiet = len(quiet)
+            o.quiet = _ptr(self._quiet, u32p) if len(quiet) else None
+            self.contacts = ContactBuffer(self, int(mc) if mc is not None else 8 * particles.capacity)
         o.substeps = int(substeps)
         o.iterations = int(iterations)
         for a in range(3):

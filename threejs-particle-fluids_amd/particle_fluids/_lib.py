@@ -86,6 +86,12 @@ class LoopOptions(C.Structure):
         ("gravity", f64 * 3),
         ("has_hash_origin", i32),
         ("hash_origin", f64 * 3),
+        ("contact", i32),
+        ("contact_mu_s", f64),
+        ("contact_mu_k", f64),
+        ("max_contacts", u32),
+        ("nquiet", u32),
+        ("quiet", P(u32)),
     ]
 
 
@@ -181,6 +187,7 @@ SIGNATURES = {
     "pf_primitives_destroy": [vp],
     "pf_softbody_create": [vp, P(SoftbodyDef), u32, u32, P(vp)],
     "pf_softbody_set_compliance": [vp, u32, f64],
+    "pf_softbody_set_self_collision": [vp, i32],
     "pf_softbody_readback_frames": [vp, P(f32), P(f32)],
     "pf_softbody_destroy": [vp],
     "pf_loop_create": [vp, P(LoopOptions), P(Material), u32, P(Collider), u32, P(vp)],
@@ -194,6 +201,7 @@ SIGNATURES = {
     "pf_loop_step": [vp, f64],
     "pf_loop_synchronize": [vp],
     "pf_loop_readback_overflow": [vp, P(Overflow)],
+    "pf_loop_readback_contacts": [vp, P(u32), P(u32), u32, P(u32)],
     "pf_loop_set_substeps": [vp, u32],
     "pf_loop_set_gravity": [vp, P(f64)],
     "pf_loop_info": [vp, P(u32), P(u32), P(f64), P(u32), P(u32)],

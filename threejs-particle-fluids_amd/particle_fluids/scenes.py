@@ -147,14 +147,28 @@ def mixed_layout(fluid_budget, bodies, seed):
     return out
 
 
+# Particle contacts Simulation turns on when a scene has soft bodies
+# (Simulation.ts:821-836).
+SIMULATION_CONTACT = {"muS": 0.3, "muK": 0.2}
+
+
 def build_mixed_scene(fluid_budget, bodies, seed, substeps=2, iterations=4, context=None, use_graph=False,
-                      evictable=False):
+                      evictable=False, contact=None):
     """One scene of the batched / served mixes on the GPU: the dam break with
     Simulation's default fluid effects, `bodies` globally shape-matched cubes
     and the five planes of the (extended) tank. `evictable` places every
-    device buffer in one arena so a Server can move the scene to host memory."""
+    device buffer in one arena so a Server can move the scene to host memory.
+    `contact` is SimLoop's option; None turns contacts on as Simulation does:
+    with SIMULATION_CONTACT whenever the scene has cubes."""
     lay = mixed_layout(fluid_budget, bodies, seed)
+    return build_layout_scene(lay, substeps, iterations, context, use_graph, evictable, contact)
+
+
+def build_layout_scene(lay, substeps=2, iterations=4, context=None, use_graph=False, evictable=False, contact=None):
+    """The GPU scene of a mixed_layout-shaped dict (positions, inv_mass,
+    fluid_count, bodies, radius, container); see build_mixed_scene."""
     pos, nf, r = lay["positions"], lay["fluid_count"], lay["radius"]
+    bodies = len(lay["bodies"])
     ps = ParticleSystem(len(pos), r, context=context, evictable=evictable)
     ps.uploadParticles(pos, invMass=lay["inv_mass"])
     fluid = FluidSystem(ps, range=(0, nf), **FLUID_DEFAULTS)
@@ -167,7 +181,16 @@ def build_mixed_scene(fluid_budget, bodies, seed, substeps=2, iterations=4, cont
     mn, mx = lay["container"][:3], lay["container"][3:]
     for n, which in PLANES:
         walls.addPlane(n, mn if which == "min" else mx)
-    loop = SimLoop(ps, substeps=substeps, iterations=iterations, materials=mats, colliders=[walls])
+    if contact is None and bodies:
+        # Only the cubes' particles search for contacts (the water keeps its own
+        # particles apart), so 8 records per cube particle bound the pairs;
+        # readbackOverflow().contacts reports it if they ever do not.
+        soft_count = sum(b["range"][1] for b in lay["bodies"])
+        contact = dict(SIMULATION_CONTACT, maxContacts=8 * soft_count)
+    elif contact is None:
+        contact = False
+    loop = SimLoop(ps, substeps=substeps, iterations=iterations, materials=mats, colliders=[walls],
+                   contact=contact)
     if use_graph:
         loop.useGraph(True)
     return dict(ps=ps, fluid=fluid, soft=soft, walls=walls, loop=loop, layout=lay)
