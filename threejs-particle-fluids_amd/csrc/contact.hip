@@ -41,12 +41,16 @@ static __device__ __forceinline__ void ct_scatter(int4* acc, uint32_t s, float3 
 }
 
 // ---------------------------------------------------------------------------
-// Pair search (generate.ts:39-88). One thread per slot; emitters walk the 27
-// buckets around their prediction with the reference's bucket dedup. A pair
-// is stored once: by the emitter, by the lower original index when both emit.
-// The block appends its pairs to LDS and reserves their records with one
-// global atomic; a thread whose pair does not fit in LDS appends it directly.
-// The counter counts every pair, stored or not (ContactBuffer.ts:44-46).
+// Pair search (generate.ts:39-88). One thread per slot; emitters walk the
+// buckets around their prediction with the reference's bucket dedup, skipping
+// stencil cells that no particle within the candidate radius can occupy (the
+// grid's cell is the widest query, often the fluid's smoothing radius; a wide
+// margin covers the rounding of the cell assignment). A pair is stored once:
+// by the emitter, by the lower original index when both emit. Candidates are
+// tested by distance before their other attributes are read. The block
+// appends its pairs to LDS and reserves their records with one global atomic;
+// a thread whose pair does not fit in LDS appends it directly. The counter
+// counts every pair, stored or not (ContactBuffer.ts:44-46).
 // ---------------------------------------------------------------------------
 #define PF_CT_LDS 1024
 
@@ -69,30 +73,43 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_generate(const SceneDev* _
       const float3 xi = pf_xyz(xs[s]);
       const uint32_t gi = S.grp[p][s];
       const float wi = S.w[p][s];
-      const int q0 = pf_query_coord(xi.x, S.ox, S.cell);
-      const int q1 = pf_query_coord(xi.y, S.oy, S.cell);
-      const int q2 = pf_query_coord(xi.z, S.oz, S.cell);
+      const float r3[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
+      int q[3];
+      float lo[3], hi[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float fl = floorf(r3[a]);
+        q[a] = pf_f2i_sat(fl);
+        lo[a] = (r3[a] - fl) * S.cell;
+        hi[a] = (1.0f - (r3[a] - fl)) * S.cell;
+      }
+      const float lim = S.ct.cr + 1e-4f * S.cell;
+      const float lim2 = lim * lim;
       uint32_t visited[27];
       uint32_t nv = 0;
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
+        for (int dy = -1; dy <= 1; ++dy) {
+          const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
           for (int dx = -1; dx <= 1; ++dx) {
-            const uint32_t b = pf_morton((int)((uint32_t)q0 + (uint32_t)dx), (int)((uint32_t)q1 + (uint32_t)dy),
-                                         (int)((uint32_t)q2 + (uint32_t)dz)) & S.hash_mask;
+            const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
+            if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
+            const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                         (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
             bool seen = false;
             for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
             visited[nv++] = b;
             if (seen) continue;
             const uint32_t en = S.start[b + 1];
             for (uint32_t j = S.start[b]; j < en; ++j) {
+              const float3 xj = pf_xyz(xs[j]);
+              const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+              if (pf_dot3(d, d) >= S.ct.cr2) continue;
               const uint32_t J = S.id[p][j];
               if (J == I) continue;
               if (J < I && ct_emitter(S, J)) continue;
               if (gi != 0u && gi == S.grp[p][j]) continue;
               if (wi <= 0.0f && S.w[p][j] <= 0.0f) continue;
-              const float3 xj = pf_xyz(xs[j]);
-              const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-              if (pf_dot3(d, d) >= S.ct.cr2) continue;
               const uint32_t k = atomicAdd(&nbuf, 1u);
               if (k < PF_CT_LDS) {
                 buf[k] = make_uint2(s, j);
@@ -105,6 +122,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_generate(const SceneDev* _
               }
             }
           }
+        }
+      }
     }
   }
   __syncthreads();
@@ -168,7 +187,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_stabilize_apply(const Scen
   float4 xs = S.xs[c][s];
   if (S.ct.on) {
     const int4 a = S.ct.acc[s];
-    S.ct.acc[s] = make_int4(0, 0, 0, 0);
+    // .w marks a particle whose prediction now differs from the grid's sorted
+    // copy; the surface-tension passes read it until the first contact solve.
+    S.ct.acc[s] = make_int4(0, 0, 0, (a.x | a.y | a.z) != 0 ? 1 : 0);
     const int tk[3] = {a.x, a.y, a.z};
     bool sat;
     const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);

@@ -226,6 +226,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   // Tables below 64 buckets: global walk with the reference's bucket dedup.
   if (S.H < 64u) {
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
+    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
     if (!active) return;
     uint32_t visited[27];
     uint32_t nvisited = 0;
@@ -335,6 +336,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
 
   if (global_walk) {
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
+    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
     if (!active) return;
     uint32_t* row = F.nl + s;  // the row of the next entry
     for (int dz = -1; dz <= 1 && !over; ++dz) {
@@ -434,13 +436,21 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     float4 xv[PF_TU_PER];
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
+    int moved = 0;
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
       if (u < U) {
         u_x[u] = xv[i];
         slots[u] = js[i];
+        if (snap && S.ct.on) moved |= S.ct.acc[js[i]].w;
       }
+    }
+    // Whether any union member was moved by the contact stabilisation: the
+    // surface-tension passes handle this tile's pairs with them apart.
+    if (snap) {
+      const int any = __syncthreads_or(moved);
+      if (tid == 0 && S.ct.on) F.tu_moved[tile] = any ? 1u : 0u;
     }
   }
   __syncthreads();
@@ -622,14 +632,15 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
 // searched directly).
 static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
                                      float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
-                                     int tk[3]) {
+                                     bool ct_scene, bool moved_i, int tk[3]) {
   const float4* __restrict__ xs = S.xs[c];
   const uint32_t n = S.n;
   const uint32_t* __restrict__ nl = F.nl;
   const uint32_t fend = F.start + F.count;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = nl[(size_t)k * n + s];
-    if (st_on) {
+    // pairs with a moved end were scattered (k_cohesion_scatter)
+    if (st_on && !(ct_scene && (moved_i || S.ct.acc[j].w != 0))) {
       const uint32_t idj = S.id[p][j];
       if (idj > id && idj < fend) {
         float3 dv;
@@ -669,12 +680,15 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
 
 // Surface tension the way the reference scatters it (cohesion.ts:60-100):
 // each pair from the list of its lower id, the fixed-point change added to
-// both ends with integer atomics. Scenes with contacts use it: their
-// stabilisation can make two lists disagree about a pair (the query point is
-// the stabilised prediction, the candidates the sorted copy), which the
-// gathered form, evaluating upper-side pairs from its own list, assumes away.
-// The sums go to the contact accumulator, idle between the stabilisation and
-// the first contact solve; k_impulses adds and clears them.
+// both ends with integer atomics. Scenes with contacts need it for the pairs
+// with a particle the stabilisation moved: for those, the two lists can
+// disagree about the pair (the query point is the stabilised prediction, the
+// candidates the sorted copy), which the gathered form, evaluating
+// upper-side pairs from its own list, assumes away. Only tiles whose union
+// holds a moved particle run it, and only pairs with a moved end scatter;
+// k_impulses gathers the others. The sums go to the contact accumulator,
+// idle between the stabilisation and the first contact solve; k_impulses
+// adds and clears them.
 __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* __restrict__ scenes,
                                                                const Tile* __restrict__ tiles,
                                                                uint32_t ntiles, int fi, int p, int c) {
@@ -683,12 +697,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
   const FluidDev& F = S.fluid[fi];
   const uint32_t s = t.first + threadIdx.x;
   if (s >= S.n || !S.ct.on || !(F.effects & PF_FX_SURFACE_TENSION)) return;
+  if (!F.tu_moved[t.first / PF_BLOCK]) return;  // no pair of this tile needs it
   const uint32_t id = S.id[p][s];
   if (!in_range(id, F.start, F.count)) return;
   const float4* __restrict__ xs = S.xs[c];
   const float3 xi = pf_xyz(xs[s]);
   const float3 ni = pf_xyz(F.normal[s]);
   const float rhoi = S.rho[p][s];
+  const bool moved_i = S.ct.acc[s].w != 0;
   const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
   const uint32_t fend = F.start + F.count;
@@ -705,6 +721,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
     }
     const uint32_t idj = S.id[p][j];
     if (!(idj > id && idj < fend)) continue;
+    // Pairs of two unmoved particles: both lists agree, the gather has them.
+    if (!moved_i && S.ct.acc[j].w == 0) continue;
     float3 dv;
     if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
     int* ai = reinterpret_cast<int*>(S.ct.acc + s);
@@ -752,20 +770,26 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
-  // With contacts the pairs were scattered already (k_cohesion_scatter); CT:
-  // some scene of the launch has contacts.
-  const bool scattered = CT && st_on && S.ct.on;
-  const bool st_pairs = st_on && !scattered;
-  if (scattered) {
-    const int4 a = S.ct.acc[s];
-    S.ct.acc[s] = make_int4(0, 0, 0, 0);
-    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+  // CT: some scene of the launch has contacts. In such a scene, pairs with a
+  // particle the contact stabilisation moved were scattered already
+  // (k_cohesion_scatter) into the accumulator; the gather skips them. The
+  // moved marks (.w) stay for the other threads of this pass.
+  const bool ct_scene = CT && st_on && S.ct.on;
+  const bool st_pairs = st_on;
+  bool moved_tile = false, moved_i = false;
+  if (ct_scene) {
+    int* a = reinterpret_cast<int*>(S.ct.acc + s);
+    const int4 v = S.ct.acc[s];
+    a[0] = 0; a[1] = 0; a[2] = 0;
+    tk[0] = v.x; tk[1] = v.y; tk[2] = v.z;
+    moved_tile = F.tu_moved[t.first / PF_BLOCK] != 0u;
+    moved_i = v.w != 0;
   }
   const uint32_t fend = F.start + F.count;
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, ct_scene, moved_i, tk);
   } else {
     // Four neighbours at a time, every gather of the four issued before the
     // arithmetic (the pass is latency-bound). Each pair is evaluated from its
@@ -789,6 +813,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
       uint32_t idj[4], lastj[4];
       float4 xj[4], nj[4];
       float rj[4], vol[4];
+      bool mvj[4] = {false, false, false, false};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         idj[u] = S.id[p][jj[u]];
@@ -797,6 +822,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
           nj[u] = F.normal[jj[u]];
           rj[u] = S.rho[p][jj[u]];
           lastj[u] = F.nl_last[jj[u]];
+          if (moved_tile) mvj[u] = moved_i || S.ct.acc[jj[u]].w != 0;
         }
         if (adh_on) vol[u] = S.bvol[p][jj[u]];
       }
@@ -805,8 +831,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
         const bool in = k0 + u < cnt;
         const float3 xjv = pf_xyz(xj[u]);
         if (st_pairs) {
-          const bool lower = in && idj[u] > id && idj[u] < fend;
-          bool upper = in && idj[u] < id && idj[u] >= F.start;
+          const bool lower = in && !mvj[u] && idj[u] > id && idj[u] < fend;
+          bool upper = in && !mvj[u] && idj[u] < id && idj[u] >= F.start;
           if (upper && lastj[u] != 0xffffffffu) {
             // The pair exists in the reference iff this slot is in j's
             // truncated list: compare this slot's walk key in j's walk with

@@ -79,6 +79,8 @@ struct FluidDev {
   uint32_t tu_limit;   // largest union a tile stages (<= PF_TU_MAX)
   uint32_t* tu_size;   // [tiles] union size of each 256-slot tile, or PF_TU_GLOBAL
   uint32_t* tu_slots;  // [tiles * PF_TU_MAX] sorted slot of each union entry
+  uint32_t* tu_moved;  // [tiles] scenes with contacts: the tile's union holds a particle the
+                       // contact stabilisation moved this substep (1 for global-walk tiles)
   uint32_t* nl_count;  // [n]
   uint32_t* nl_last;   // [n] walk key of the last kept neighbour when the list overflowed, else ~0
   float4* normal;      // [n] colour-field normal
@@ -151,13 +153,14 @@ struct ContactDev {
   uint32_t blocks;           // workgroups of the per-contact passes (grid stride)
   uint32_t nquiet;           // extra noSelfContacts ranges (fluids are always quiet)
   uint32_t qstart[PF_MAX_QUIET], qcount[PF_MAX_QUIET];
-  float dist, cr2;           // 2r and (2r * 1.1)^2 (generate.ts:19)
+  float dist, cr, cr2;       // 2r, 2r * 1.1 and (2r * 1.1)^2 (generate.ts:19)
   float mus, muk;
   uint32_t* count;           // [1] pairs emitted this substep, dropped ones included
   uint2* pair;               // [max] (emitter slot, other slot)
   int2* lam;                 // [max] (lambda_n, lambda_t) ticks
   float4* nrm;               // [max] last normal the solve used
-  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes
+  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes; from the
+                             // stabilisation to the first solve, .w = 1 marks a particle it moved
 };
 
 struct SceneDev {

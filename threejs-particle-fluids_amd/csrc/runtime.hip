@@ -45,7 +45,6 @@ static constexpr size_t kArenaAlign = 256;
 // Mapped tail no buffer uses: reads a little past a buffer's end stay inside
 // mapped memory, as they do with hipMalloc's rounded allocations.
 static constexpr size_t kArenaTail = 64u << 10;
-static constexpr size_t kArenaMaxGrow = 64u << 20;
 
 static hipMemAllocationProp arena_prop(int device) {
   hipMemAllocationProp prop{};
@@ -108,13 +107,7 @@ void* Arena::alloc(size_t bytes) {
     return p;
   }
   Chunk c;
-  // Chunks grow with the arena (half of what it holds, up to 64 MiB), so a
-  // large scene maps tens of chunks rather than one per big buffer: every
-  // chunk is a separate device mapping, and a process's mappings are finite.
-  size_t held = 0;
-  for (const auto& ch : chunks) held += ch.size;
-  const size_t grow = std::min<size_t>(held / 2, kArenaMaxGrow);
-  c.size = (std::max(std::max(bytes + kArenaTail, kArenaMinChunk), grow) + gran - 1) / gran * gran;
+  c.size = (std::max(bytes + kArenaTail, kArenaMinChunk) + gran - 1) / gran * gran;
   void* va = nullptr;
   hipError_t e = hipMemAddressReserve(&va, c.size, gran, nullptr, 0);
   if (e != hipSuccess) fail(PF_ENOMEM, "device address reservation of %zu bytes failed: %s", c.size, hipGetErrorString(e));

@@ -522,8 +522,8 @@ int pf_fluid_destroy(pf_fluid* f) {
   if (!f || f->disposed) return PF_OK;
   f->disposed = true;
   dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv); dfree(f->reaction);
-  dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
-  f->nl16 = f->tu_size = f->tu_slots = nullptr;
+  dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots); dfree(f->tu_moved);
+  f->nl16 = f->tu_size = f->tu_slots = f->tu_moved = nullptr;
   f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr; f->reaction = nullptr;
   PF_API_END
 }
@@ -1206,6 +1206,7 @@ static void fluid_write_params(pf_fluid* f) {
   F.tu_limit = f->tu_limit;
   F.tu_size = f->tu_size;
   F.tu_slots = f->tu_slots;
+  F.tu_moved = f->tu_moved;
   if (f->isv) {
     // rate = (nu * dt) * (10 / h^2), the constant computed in double (ViscositySolver.ts:103)
     F.isv_iters = f->isv->iterations;
@@ -1407,6 +1408,7 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     f->nl16 = dalloc<uint32_t>((size_t)(PF_MAX_NEIGHBORS / 2) * n);
     f->tu_size = dalloc<uint32_t>(ntiles);
     f->tu_slots = dalloc<uint32_t>(ntiles * PF_TU_MAX);
+    f->tu_moved = L->contact ? dalloc<uint32_t>(ntiles) : nullptr;
     if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
@@ -1498,6 +1500,7 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     }
     const double cr = 2.0 * ps->radius * 1.1;
     C.dist = (float)(2.0 * ps->radius);
+    C.cr = (float)cr;
     C.cr2 = (float)(cr * cr);
     C.mus = (float)ct_mus;
     C.muk = (float)ct_muk;
@@ -1846,8 +1849,8 @@ int pf_loop_destroy(pf_loop* L) {
   for (auto* f : L->fluids) {
     f->loop = nullptr;
     dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->normal); dfree(f->omega); dfree(f->dv);
-    dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
-    f->nl16 = f->tu_size = f->tu_slots = nullptr;
+    dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots); dfree(f->tu_moved);
+    f->nl16 = f->tu_size = f->tu_slots = f->tu_moved = nullptr;
     f->nl = nullptr; f->nl_count = nullptr; f->nl_last = nullptr; f->normal = f->omega = f->dv = nullptr;
   }
   if (L->prims) L->prims->loop = nullptr;
