@@ -260,6 +260,17 @@ int pf_sdf_collider_set_friction(pf_sdf_collider* c, double mu_s, double mu_k);
 int pf_sdf_collider_set_thickness(pf_sdf_collider* c, double thickness);
 int pf_sdf_collider_destroy(pf_sdf_collider* c);
 
+/* ---- SDF binary format (src/sdf/writeBinary.ts:21-106) ------------------
+ * decodeSdfBinary: validates the 48-byte PSDF header as the reference does
+ * (same messages) and returns resolution, origin and voxel size (f32 in the
+ * file, widened); the voxels (x fastest) are copied only when `data` holds
+ * all `voxel_count` of them, so a first call with data = NULL reads the
+ * header. encodeSdfBinary: writes the format; out = NULL returns the size. */
+int pf_sdf_decode(const void* bytes, size_t size, uint32_t resolution[3], double origin[3], double voxel_size[3],
+                  float* data, size_t data_capacity, size_t* voxel_count);
+int pf_sdf_encode(const float* data, size_t data_count, const uint32_t resolution[3], const double origin[3],
+                  const double voxel_size[3], void* out, size_t capacity, size_t* size);
+
 /* ---- SimLoop (src/core/loop.ts:92-324) ----------------------------------- */
 #define PF_MATERIAL_FLUID 1u
 #define PF_MATERIAL_SOFTBODY 2u

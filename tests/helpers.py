@@ -331,3 +331,46 @@ def floating_cubes_layout(budget=3000, side=5, seed=11):
     w[nf:] = 1.0 / (scenes.CUBE_DENSITY * sp ** 3)
     return dict(positions=allpos, inv_mass=w, fluid_count=nf, bodies=bodies, radius=r,
                 container=tuple(scenes.CONTAINER))
+
+
+def oracle_elastic_scene(sc, precision="f32"):
+    """The GPU elastic studies scene `sc` (scenes.build_elastic_scene) in the
+    oracle: the same particles, locally shape-matched bodies with their
+    edges, contacts with every collision group 0 (selfCollision), the tank's
+    planes and the two plates as posed boxes. Returns (scene, plates)."""
+    lay = sc["layout"]
+    pos, w = lay["positions"], lay["inv_mass"]
+    n = len(pos)
+    s = orc.Scene(n, lay["radius"], precision)
+    s.set_particles(pos, None, w)
+    info = sc["loop"].info()
+    s.set_loop(info["substeps"], info["iterations"], (0.0, -6.0, 0.0), info["cellSize"], info["hashTableSize"])
+    comp = scenes.softbody_compliance(0.9, lay["per_body"])
+    for b in lay["bodies"]:
+        st, c = b["range"]
+        off, wts = rest_offsets(pos[st:st + c])
+        s.add_body(st, c, comp, off, wts)
+        s.add_body_edges(np.asarray(b["edges"], np.uint32))
+    s.set_contact(0.12, 0.06, 20 * n)
+    tank = dict(mus=0.08, muk=0.04)
+    s.add_plane((0, 1, 0), (0, 0, 0), **tank)
+    s.add_plane((1, 0, 0), (-1.05, 0, 0), **tank)
+    s.add_plane((-1, 0, 0), (1.05, 0, 0), **tank)
+    s.add_plane((0, 0, 1), (0, 0, -0.87), **tank)
+    s.add_plane((0, 0, -1), (0, 0, 0.87), **tank)
+    plates = [PosedPrimitive("box", [0, 0, 0, 0], [0.02, 0.21, 0.87, 0], mus=0.1, muk=0.05) for _ in range(2)]
+    for k, pl in enumerate(plates):
+        pl.pose((-1.0 if k == 0 else 1.0) * scenes.elastic_plate_distance(0.0) * np.array([1, 0, 0])
+                + np.array([0, 0.2, 0]), (0, 0, 0, 1))
+        s.add_prim(*pl.oracle_args())
+    return s, plates
+
+
+def step_oracle_elastic(s, plates, time, dt):
+    """One frame of the elastic studies in the oracle (plates first, as
+    scenes.step_elastic_scene does); `time` is the frame's end time."""
+    d = scenes.elastic_plate_distance(time)
+    for k, pl in enumerate(plates):
+        pl.pose(((-1.0 if k == 0 else 1.0) * d, 0.2, 0.0), (0, 0, 0, 1), dt)
+        s.set_prim(5 + k, *pl.oracle_args())
+    s.step(dt)

@@ -165,3 +165,48 @@ def test_softbody_global_matches_oracle():
     c_gpu, r_gpu = sb.readbackFrames()
     c_o, r_o = o.body_frame(3)
     assert np.abs(c_gpu[3] - c_o).max() < 1e-5
+
+
+def compressed_block(budget=5000, squeeze=0.7):
+    """The dam-break lattice squeezed toward the container corner: about
+    1/squeeze^3 the rest density, so the 64-neighbour lists truncate from the
+    first substep (neighborList.ts:63-66)."""
+    pos, r = scenes.dam_break_layout(budget)
+    lo = pos.min(0)
+    pos = (lo + (pos - lo) * squeeze).astype(np.float32)
+    return scenes.jitter(pos, 2 * r * squeeze), r
+
+
+@pytest.mark.parametrize("effects", ["defaults", "no_surface_tension"])
+def test_truncated_neighbour_lists_match_fp32_oracle(effects):
+    fx = dict(scenes.FLUID_DEFAULTS)
+    if effects == "no_surface_tension":
+        fx.pop("surfaceTension")
+    pos, r = compressed_block()
+    print(f"[pbf] truncated lists, {len(pos)} particles squeezed 0.7, effects {sorted(fx)}", flush=True)
+    ps = pf.ParticleSystem(len(pos), r)
+    ps.uploadParticles(pos)
+    fluid = pf.FluidSystem(ps, **fx)
+    walls = pf.PrimitiveSet(ps)
+    mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
+    for n, which in scenes.PLANES:
+        walls.addPlane(n, mn if which == "min" else mx)
+    loop = pf.SimLoop(ps, substeps=2, iterations=4, materials=[fluid], colliders=[walls])
+    o = oracle_dam_break(pos, r, 2, 4, "f32", effects=fx)
+    sc = dict(ps=ps, fluid=fluid)
+    errs, flags = [], []
+    for _ in range(3):
+        loop.step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+        errs.append(compare(gpu_state(sc), o.get()))
+        flags.append((loop.readbackOverflow().neighbors, o.flags()["neighbors"]))
+    print(f"[pbf] truncated lists rel err per step {errs}, neighbour overflow (gpu, oracle) {flags}", flush=True)
+    assert flags[0] == (True, True)
+    assert all(a == b for a, b in flags)
+    for e in errs:
+        for k, v in e.items():
+            assert v <= TOL, (k, v, errs)
+    loop.dispose()
+    walls.dispose()
+    fluid.dispose()
+    ps.dispose()

@@ -628,28 +628,30 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
 }
 
 // Surface tension and adhesion of one particle in a table below 64 buckets
-// (global slots, one neighbour at a time; a partner's truncated list is
-// searched directly).
+// (global slots, one neighbour at a time).
 static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
                                      float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
-                                     bool ct_scene, bool moved_i, int tk[3]) {
+                                     bool trunc_scene, bool ct_scene, bool moved_i, bool trunc_i, int tk[3]) {
   const float4* __restrict__ xs = S.xs[c];
   const uint32_t n = S.n;
   const uint32_t* __restrict__ nl = F.nl;
   const uint32_t fend = F.start + F.count;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = nl[(size_t)k * n + s];
-    // pairs with a moved end were scattered (k_cohesion_scatter)
-    if (st_on && !(ct_scene && (moved_i || S.ct.acc[j].w != 0))) {
+    // pairs with a moved end, or whose upper end's list is truncated, were
+    // scattered (k_cohesion_scatter)
+    const bool moved = moved_i || (ct_scene && S.ct.acc[j].w != 0);
+    if (st_on && !moved) {
       const uint32_t idj = S.id[p][j];
-      if (idj > id && idj < fend) {
+      if (idj > id && idj < fend && !(trunc_scene && F.nl_last[j] != 0xffffffffu)) {
         float3 dv;
         if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) {
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
         }
-      } else if (idj < id && idj >= F.start) {
+      } else if (idj < id && idj >= F.start && !trunc_i) {
+        // this list is whole: the pair exists iff j's (maybe truncated) list holds this slot
         bool listed = F.nl_last[j] == 0xffffffffu;
         for (uint32_t m = 0; m < PF_MAX_NEIGHBORS && !listed; ++m) listed |= (nl[(size_t)m * n + j] == s);
         float3 dv;
@@ -680,37 +682,56 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
 
 // Surface tension the way the reference scatters it (cohesion.ts:60-100):
 // each pair from the list of its lower id, the fixed-point change added to
-// both ends with integer atomics. Scenes with contacts need it for the pairs
-// with a particle the stabilisation moved: for those, the two lists can
-// disagree about the pair (the query point is the stabilised prediction, the
-// candidates the sorted copy), which the gathered form, evaluating
-// upper-side pairs from its own list, assumes away. Only tiles whose union
-// holds a moved particle run it, and only pairs with a moved end scatter;
-// k_impulses gathers the others. The sums go to the contact accumulator,
-// idle between the stabilisation and the first contact solve; k_impulses
-// adds and clears them.
+// both ends with integer atomics. The gathered form in k_impulses evaluates
+// a pair from either end: the lower end from its own list, the upper end
+// from its own list too, checking a truncated partner's list by walk key.
+// That misses pairs in two cases, and this pass takes exactly those:
+//  - the upper end's list was truncated at 64 (neighborList.ts:63-66): a pair
+//    cut from it still exists when the lower end's list kept it;
+//  - in scenes with contacts, an end was moved by the stabilisation, whose
+//    query point is the stabilised prediction and candidates the sorted copy,
+//    so the two lists can disagree about the pair.
+// A tile runs it only when its union holds such a particle. The sums go to
+// the scatter accumulator (idle between the stabilisation and the first
+// contact solve); k_impulses adds and clears them.
+static __device__ __forceinline__ bool nl_truncated(const FluidDev& F, uint32_t s) {
+  return F.nl_last[s] != 0xffffffffu;
+}
+
 __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* __restrict__ scenes,
                                                                const Tile* __restrict__ tiles,
                                                                uint32_t ntiles, int fi, int p, int c) {
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
+  // Uniform over the block from here to the union test.
+  if (!S.ct.acc || !(F.effects & PF_FX_SURFACE_TENSION)) return;
+  const uint32_t tile = t.first / PF_BLOCK;
+  const bool trunc_scene = S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
+  const bool moved_tile = S.ct.on && F.tu_moved[tile];
+  if (!trunc_scene && !moved_tile) return;
+  const uint32_t U = F.tu_size[tile];
+  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+  const bool staged = U != PF_TU_GLOBAL;
+  if (!moved_tile && staged) {
+    // Any truncated list among the union's particles (a global-walk tile is
+    // taken as having one).
+    int any = 0;
+    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) any |= nl_truncated(F, uslots[u]) ? 1 : 0;
+    if (!__syncthreads_or(any)) return;
+  }
   const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || !S.ct.on || !(F.effects & PF_FX_SURFACE_TENSION)) return;
-  if (!F.tu_moved[t.first / PF_BLOCK]) return;  // no pair of this tile needs it
+  if (s >= S.n) return;
   const uint32_t id = S.id[p][s];
   if (!in_range(id, F.start, F.count)) return;
   const float4* __restrict__ xs = S.xs[c];
   const float3 xi = pf_xyz(xs[s]);
   const float3 ni = pf_xyz(F.normal[s]);
   const float rhoi = S.rho[p][s];
-  const bool moved_i = S.ct.acc[s].w != 0;
+  const bool moved_i = S.ct.on && S.ct.acc[s].w != 0;
   const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
   const uint32_t fend = F.start + F.count;
-  const uint32_t U = F.tu_size[t.first / PF_BLOCK];
-  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
-  const bool staged = U != PF_TU_GLOBAL;
   for (uint32_t k = 0; k < cnt; ++k) {
     uint32_t j;
     if (staged) {
@@ -721,8 +742,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
     }
     const uint32_t idj = S.id[p][j];
     if (!(idj > id && idj < fend)) continue;
-    // Pairs of two unmoved particles: both lists agree, the gather has them.
-    if (!moved_i && S.ct.acc[j].w == 0) continue;
+    // Other pairs are the gather's.
+    if (!moved_i && !(trunc_scene && nl_truncated(F, j)) && !(S.ct.on && S.ct.acc[j].w != 0)) continue;
     float3 dv;
     if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
     int* ai = reinterpret_cast<int*>(S.ct.acc + s);
@@ -770,26 +791,29 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
-  // CT: some scene of the launch has contacts. In such a scene, pairs with a
-  // particle the contact stabilisation moved were scattered already
-  // (k_cohesion_scatter) into the accumulator; the gather skips them. The
+  // Pairs with a truncated list at either end, and in scenes with contacts
+  // (CT: some scene of the launch has them) pairs with a particle the
+  // stabilisation moved, were scattered already (k_cohesion_scatter) into
+  // the accumulator; the gather adds those sums and skips such pairs. The
   // moved marks (.w) stay for the other threads of this pass.
+  const bool trunc_scene = st_on && S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
   const bool ct_scene = CT && st_on && S.ct.on;
   const bool st_pairs = st_on;
-  bool moved_tile = false, moved_i = false;
-  if (ct_scene) {
+  bool moved_tile = false, moved_i = false, trunc_i = false;
+  if (trunc_scene || ct_scene) {
     int* a = reinterpret_cast<int*>(S.ct.acc + s);
     const int4 v = S.ct.acc[s];
     a[0] = 0; a[1] = 0; a[2] = 0;
     tk[0] = v.x; tk[1] = v.y; tk[2] = v.z;
-    moved_tile = F.tu_moved[t.first / PF_BLOCK] != 0u;
-    moved_i = v.w != 0;
+    moved_tile = ct_scene && F.tu_moved[t.first / PF_BLOCK] != 0u;
+    moved_i = ct_scene && v.w != 0;
+    trunc_i = trunc_scene && F.nl_last[s] != 0xffffffffu;
   }
   const uint32_t fend = F.start + F.count;
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, ct_scene, moved_i, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, trunc_scene, ct_scene, moved_i, trunc_i, tk);
   } else {
     // Four neighbours at a time, every gather of the four issued before the
     // arithmetic (the pass is latency-bound). Each pair is evaluated from its
@@ -813,7 +837,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
       uint32_t idj[4], lastj[4];
       float4 xj[4], nj[4];
       float rj[4], vol[4];
-      bool mvj[4] = {false, false, false, false};
+      bool moved[4] = {false, false, false, false};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         idj[u] = S.id[p][jj[u]];
@@ -822,7 +846,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
           nj[u] = F.normal[jj[u]];
           rj[u] = S.rho[p][jj[u]];
           lastj[u] = F.nl_last[jj[u]];
-          if (moved_tile) mvj[u] = moved_i || S.ct.acc[jj[u]].w != 0;
+          moved[u] = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
         }
         if (adh_on) vol[u] = S.bvol[p][jj[u]];
       }
@@ -831,8 +855,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
         const bool in = k0 + u < cnt;
         const float3 xjv = pf_xyz(xj[u]);
         if (st_pairs) {
-          const bool lower = in && !mvj[u] && idj[u] > id && idj[u] < fend;
-          bool upper = in && !mvj[u] && idj[u] < id && idj[u] >= F.start;
+          // The scatter took pairs with a moved end, and pairs whose upper
+          // end's list is truncated.
+          const bool lower = in && !moved[u] && idj[u] > id && idj[u] < fend && lastj[u] == 0xffffffffu;
+          bool upper = in && !moved[u] && !trunc_i && idj[u] < id && idj[u] >= F.start;
           if (upper && lastj[u] != 0xffffffffu) {
             // The pair exists in the reference iff this slot is in j's
             // truncated list: compare this slot's walk key in j's walk with

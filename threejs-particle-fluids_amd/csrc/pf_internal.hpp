@@ -159,7 +159,8 @@ struct ContactDev {
   uint2* pair;               // [max] (emitter slot, other slot)
   int2* lam;                 // [max] (lambda_n, lambda_t) ticks
   float4* nrm;               // [max] last normal the solve used
-  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes; from the
+  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes and by surface
+                             // tension's scattered pairs (also without contacts); from the
                              // stabilisation to the first solve, .w = 1 marks a particle it moved
 };
 

@@ -142,7 +142,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       const bool stx = (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0;
       const bool adh = (sh.fx[f] & PF_FX_ADHESION) != 0;
-      if (stx && sh.contact) {
+      if (stx) {
         PF_LAUNCH(ctx, "cohesion_scatter");
         launch_cohesion_scatter(st, scenes, tiles, ntiles, (int)f, p, c);
         ++nk;
@@ -1518,6 +1518,13 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     for (uint32_t i = 0; i < L->nctiles; ++i) ctl[i] = Tile{0u, i * PF_BLOCK};
     L->ctiles = dalloc<Tile>(L->nctiles);
     PF_HIP(hipMemcpy(L->ctiles, ctl.data(), ctl.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  }
+  // The scatter accumulator also carries surface tension's scattered pairs.
+  bool any_st = false;
+  for (auto* f : L->fluids) any_st = any_st || (f->present & PF_FLUID_SURFACE_TENSION);
+  if (any_st && !S.ct.acc) {
+    S.ct.acc = dalloc<int4>(n);
+    PF_HIP(hipMemset(S.ct.acc, 0, (size_t)n * sizeof(int4)));
   }
   // Work tiles.
   L->ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;

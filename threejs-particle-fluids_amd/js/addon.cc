@@ -698,6 +698,55 @@ napi_value LoopReadbackOverflow(napi_env env, napi_callback_info info) {
   set(env, obj, "contacts", boolean(env, o.contacts));
   return obj;
 }
+// ---------------------------------------------------------------- formats
+napi_value num_array(napi_env env, const double* v, uint32_t n) {
+  napi_value arr;
+  napi_create_array_with_length(env, n, &arr);
+  for (uint32_t i = 0; i < n; ++i) napi_set_element(env, arr, i, number(env, v[i]));
+  return arr;
+}
+// (bytes: Uint8Array) -> SDFData {data, resolution, origin, voxelSize}
+napi_value SdfDecode(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  size_t n = 0;
+  const uint8_t* b = typed<uint8_t>(env, a.argv[0], &n);
+  uint32_t res[3] = {0, 0, 0};
+  double o[3], v[3];
+  size_t count = 0;
+  PF_CALL(pf_sdf_decode(b, n, res, o, v, nullptr, 0, &count));
+  float* d = nullptr;
+  napi_value data = f32_array(env, count, &d);
+  PF_CALL(pf_sdf_decode(b, n, res, o, v, d, count, &count));
+  const double rd[3] = {(double)res[0], (double)res[1], (double)res[2]};
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "data", data);
+  set(env, obj, "resolution", num_array(env, rd, 3));
+  set(env, obj, "origin", num_array(env, o, 3));
+  set(env, obj, "voxelSize", num_array(env, v, 3));
+  return obj;
+}
+// (data: Float32Array, resolution, origin, voxelSize) -> Uint8Array
+napi_value SdfEncode(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  size_t count = 0;
+  const float* d = typed<float>(env, a.argv[0], &count);
+  auto r = doubles(env, a.argv[1]), o = doubles(env, a.argv[2]), v = doubles(env, a.argv[3]);
+  if (r.size() != 3 || o.size() != 3 || v.size() != 3) {
+    napi_throw_type_error(env, nullptr, "encodeSdfBinary: resolution, origin and voxelSize need three numbers");
+    return nullptr;
+  }
+  const uint32_t res[3] = {(uint32_t)r[0], (uint32_t)r[1], (uint32_t)r[2]};
+  size_t size = 0;
+  PF_CALL(pf_sdf_encode(d, count, res, o.data(), v.data(), nullptr, 0, &size));
+  napi_value ab, arr;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, size, &p, &ab);
+  napi_create_typedarray(env, napi_uint8_array, size, ab, 0, &arr);
+  PF_CALL(pf_sdf_encode(d, count, res, o.data(), v.data(), p, size, &size));
+  return arr;
+}
+
 // (loop, cap) -> {count, pairs: Uint32Array of the stored (i, j) pairs}
 napi_value LoopReadbackContacts(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
@@ -1062,6 +1111,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"sdfCreate", SdfCreate}, {"sdfSet", SdfSet}, {"sdfDestroy", SdfDestroy},
       {"softbodyCreate", SoftbodyCreate}, {"softbodySetCompliance", SoftbodySetCompliance},
       {"softbodySetSelfCollision", SoftbodySetSelfCollision}, {"loopReadbackContacts", LoopReadbackContacts},
+      {"sdfDecode", SdfDecode}, {"sdfEncode", SdfEncode},
       {"softbodyReadbackFrames", SoftbodyReadbackFrames}, {"softbodyDestroy", SoftbodyDestroy},
       {"loopCreate", LoopCreate}, {"loopStep", LoopStep}, {"loopStepAsync", LoopStepAsync},
       {"loopSynchronize", LoopSynchronize}, {"loopReadbackOverflow", LoopReadbackOverflow},

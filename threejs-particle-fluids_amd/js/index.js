@@ -637,6 +637,53 @@ class Simulation {
   }
 }
 
+// ---------------------------------------------------------------- formats
+/** decodeSdfBinary(buffer) (src/sdf/writeBinary.ts:54-106): SDFData from PSDF bytes. */
+function decodeSdfBinary(buffer) {
+  const bytes = buffer instanceof Uint8Array ? buffer : new Uint8Array(buffer.buffer ? buffer.buffer : buffer);
+  return native.sdfDecode(bytes);
+}
+/** encodeSdfBinary(sdf) (src/sdf/writeBinary.ts:25-52): PSDF bytes (an ArrayBuffer). */
+function encodeSdfBinary(sdf) {
+  const data = sdf.data instanceof Float32Array ? sdf.data : Float32Array.from(sdf.data);
+  const out = native.sdfEncode(data, Array.from(sdf.resolution), Array.from(sdf.origin), Array.from(sdf.voxelSize));
+  return out.buffer;
+}
+/**
+ * A baked soft-body template (SampledBody, demo/presets/elastic.ts:26-34):
+ * positions, per-particle surface flags (surface first), local edges. Returns
+ * the pieces a SoftbodySystem body takes.
+ */
+function loadSampledBody(json) {
+  const body = typeof json === 'string' ? JSON.parse(json) : json;
+  if (body.positions.length % 3) throw new Error('SampledBody: positions must hold xyz triples');
+  const count = body.positions.length / 3;
+  const surface = body.surface || new Array(count).fill(1);
+  if (surface.length !== count) throw new Error(`SampledBody: ${surface.length} surface flags for ${count} particles`);
+  const surfaceCount = surface.filter(Boolean).length;
+  for (let i = 0; i < count; i++) {
+    if ((i < surfaceCount) !== !!surface[i]) throw new Error('SampledBody: surface particles must come first');
+  }
+  const edges = Uint32Array.from(body.edges || []);
+  if (edges.length % 2) throw new Error('SampledBody: edges must be index pairs');
+  for (let e = 0; e < edges.length; e += 2) {
+    if (!(edges[e] < edges[e + 1] && edges[e + 1] < count)) {
+      throw new Error('SampledBody: edges must be pairs a < b of particle indices');
+    }
+  }
+  return { positions: Float32Array.from(body.positions), surfaceCount, edges, scale: def(body.scale, 1) };
+}
+/** The demo's deterministic [0, 1) generator (demo/presets/shared.ts:66-74): mulberry32. */
+function seededRandom(seed) {
+  let state = seed | 0;
+  return () => {
+    state = (state + 0x6d2b79f5) | 0;
+    let t = Math.imul(state ^ (state >>> 15), 1 | state);
+    t = (t + Math.imul(t ^ (t >>> 7), 61 | t)) ^ t;
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  };
+}
+
 module.exports = {
   native,
   Context,
@@ -648,6 +695,10 @@ module.exports = {
   SoftbodySystem,
   SimLoop,
   ContactBuffer,
+  decodeSdfBinary,
+  encodeSdfBinary,
+  loadSampledBody,
+  seededRandom,
   Simulation,
   ViscositySolver,
   Batch,

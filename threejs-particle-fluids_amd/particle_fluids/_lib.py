@@ -202,6 +202,8 @@ SIGNATURES = {
     "pf_loop_synchronize": [vp],
     "pf_loop_readback_overflow": [vp, P(Overflow)],
     "pf_loop_readback_contacts": [vp, P(u32), P(u32), u32, P(u32)],
+    "pf_sdf_decode": [vp, C.c_size_t, P(u32), P(f64), P(f64), P(f32), C.c_size_t, P(C.c_size_t)],
+    "pf_sdf_encode": [P(f32), C.c_size_t, P(u32), P(f64), P(f64), vp, C.c_size_t, P(C.c_size_t)],
     "pf_loop_set_substeps": [vp, u32],
     "pf_loop_set_gravity": [vp, P(f64)],
     "pf_loop_info": [vp, P(u32), P(u32), P(f64), P(u32), P(u32)],

@@ -7,9 +7,12 @@ five container planes (Simulation.ts:777-787). The lattice gets a seeded
 +-1 % spacing jitter from the reference tests' LCG
 (tests/analytical/hashgrid/hashgrid-random-stress.gpu.test.ts:21-27).
 """
+import math
+
 import numpy as np
 
 from . import FluidSystem, ParticleSystem, PrimitiveSet, SimLoop, SoftbodySystem, fill_box, fit_budget
+from .simulation import _euler_xyz_quat, _quat_apply
 
 CONTAINER = (-0.5, 0.0, -0.3, 0.5, 0.8, 0.3)
 FLUID_BOX = (-0.5, 0.0, -0.3, -0.1, 0.5, 0.3)
@@ -194,6 +197,98 @@ def build_layout_scene(lay, substeps=2, iterations=4, context=None, use_graph=Fa
     if use_graph:
         loop.useGraph(True)
     return dict(ps=ps, fluid=fluid, soft=soft, walls=walls, loop=loop, layout=lay)
+
+
+# Elastic studies (demo/presets/elastic.ts:78-230): twenty baked soft forms
+# dropped into a tank between two plates that squeeze them.
+ELASTIC_NAMES = ("donut", "croissant", "banana", "ginger-bread")
+ELASTIC_BODY_BUDGETS = (250, 500, 750, 1250, 2500)  # PARTICLE_LEVELS / 20 (demo/types.ts:35-49)
+ELASTIC_BODIES = 20
+ELASTIC_RELEASE_HEIGHT = 0.55
+
+
+def scaled_substeps(base, particles, dimensions=3):
+    """scaledSubsteps (demo/presets/shared.ts:61-64)."""
+    scale = (particles / 10000.0) ** (1.0 / dimensions)
+    return max(base, math.ceil(base * scale - 0.1))
+
+
+def elastic_layout(templates, particles=5000, seed=0x9E3779B9):
+    """Particle state of the elastic studies scene from baked templates (a
+    dict name -> SampledBody from formats.load_sampled_body at the chosen
+    per-body level): twenty forms, staggered, jittered and tumbled by the
+    demo's seeded generator, each weighing the same at every level.
+    Returns dict(positions, inv_mass, bodies, radius, per_body)."""
+    from .formats import seeded_random
+    per_body = min(ELASTIC_BODY_BUDGETS, key=lambda b: (abs(b - particles / ELASTIC_BODIES), b))
+    radius = 0.015 * (200.0 / per_body) ** (1.0 / 3.0)
+    random = seeded_random(seed)
+    pos, inv, bodies = [], [], []
+    start = 0
+    for n in range(ELASTIC_BODIES):
+        shape = templates[ELASTIC_NAMES[n % len(ELASTIC_NAMES)]]
+        count = len(shape["positions"])
+        layer = n // 12
+        cx = ((n % 4) - 1.5) * 0.44 + (0.22 if layer else 0.0) + (random() - 0.5) * 0.12
+        cy = ELASTIC_RELEASE_HEIGHT + layer * 0.55 + random() * 0.14
+        cz = ((n // 4) % 3 - 1) * 0.47 + (random() - 0.5) * 0.12
+        e = (random() * math.pi * 2, random() * math.pi * 2, random() * math.pi * 2)
+        q = _euler_xyz_quat(e)
+        p = _quat_apply(q, shape["positions"].astype(np.float64)) + np.array([cx, cy, cz])
+        pos.append(p)
+        inv.append(np.full(count, count / 20.0))
+        bodies.append(dict(range=(start, count), surfaceCount=shape["surfaceCount"], edges=shape["edges"]))
+        start += count
+    return dict(positions=np.concatenate(pos).astype(np.float32), inv_mass=np.concatenate(inv).astype(np.float32),
+                bodies=bodies, radius=radius, per_body=per_body)
+
+
+def elastic_plate_distance(time, compression=0.8):
+    """Where the squeezing plates stand at `time` (elastic.ts:205-211)."""
+    cycle = (1.0 - math.cos(max(0.0, time - 1.0) * 1.05)) / 2.0
+    return max(0.1, 1.06 - cycle * compression * 1.0875)
+
+
+def build_elastic_scene(templates, particles=5000, softness=0.9, gravity=6.0, context=None, substeps=None,
+                        iterations=2):
+    """The elastic studies scene on the GPU: locally shape-matched forms with
+    selfCollision, the tank (tank(1.05, 0.87), shared.ts:26-39), two plates
+    and the demo's contacts (muS 0.12, muK 0.06, 20 records per particle).
+    `step(sc, dt)` moves the plates and steps the loop."""
+    lay = elastic_layout(templates, particles)
+    ps = ParticleSystem(len(lay["positions"]), lay["radius"], context=context)
+    ps.uploadParticles(lay["positions"], invMass=lay["inv_mass"])
+    comp = softbody_compliance(softness, lay["per_body"])
+    soft = SoftbodySystem(ps, [dict(b, compliance=comp) for b in lay["bodies"]], shapeMatching="local",
+                          selfCollision=True)
+    walls = PrimitiveSet(ps)
+    tank_friction = dict(muS=0.08, muK=0.04)
+    walls.addPlane((0, 1, 0), (0, 0, 0), **tank_friction)
+    walls.addPlane((1, 0, 0), (-1.05, 0, 0), **tank_friction)
+    walls.addPlane((-1, 0, 0), (1.05, 0, 0), **tank_friction)
+    walls.addPlane((0, 0, 1), (0, 0, -0.87), **tank_friction)
+    walls.addPlane((0, 0, -1), (0, 0, 0.87), **tank_friction)
+    plates = [walls.addBox((0, 0, 0), (0.02, 0.21, 0.87), muS=0.1, muK=0.05) for _ in range(2)]
+    n = len(lay["positions"])
+    loop = SimLoop(ps, substeps=substeps or scaled_substeps(6, n), iterations=iterations, gravity=(0.0, -gravity, 0.0),
+                   materials=[soft], colliders=[walls], contact={"muS": 0.12, "muK": 0.06, "maxContacts": 20 * n})
+    sc = dict(ps=ps, soft=soft, walls=walls, loop=loop, layout=lay, plates=plates, time=0.0)
+    _place_plates(sc, 0.0)
+    return sc
+
+
+def _place_plates(sc, dt):
+    d = elastic_plate_distance(sc["time"])
+    for side, slot in zip((-1.0, 1.0), sc["plates"]):
+        # the panel's box sits 0.2 m above its group origin (elastic.ts:183-196)
+        sc["walls"].setPose(slot, (side * d, 0.2, 0.0), (0.0, 0.0, 0.0, 1.0), dt)
+
+
+def step_elastic_scene(sc, dt):
+    """One frame of the elastic studies: plates follow the squeeze cycle, then the loop steps."""
+    sc["time"] += dt
+    _place_plates(sc, dt)
+    sc["loop"].step(dt)
 
 
 def dispose_scene(sc):
