@@ -185,6 +185,37 @@ int pf_viscosity_create(pf_fluid* fluid, double viscosity, uint32_t iterations, 
 int pf_viscosity_set(pf_viscosity* solver, double viscosity);
 int pf_viscosity_destroy(pf_viscosity* solver);
 
+/* ---- Surface field (src/fluids/render/SurfaceField.ts:101-369) -----------
+ * The liquid's surface as a voxel grid, the field FluidSurfaceRenderer
+ * sphere-traces (FluidSurfaceRenderer.ts:181-230, update at 632-658),
+ * computed headless from the committed particles: a cloud-in-cell splat,
+ * three Gaussian passes and phi = |p - mean| - radius. Each voxel holds four
+ * floats (phi in voxels, volume fraction, wall proximity, 1), x fastest.
+ * Options < 0 (or 0 for voxel_budget) take the reference's defaults:
+ * defaultVoxelBudget (FluidSurfaceRenderer.ts:590-592), radius 1.25 r,
+ * meniscus height 1.6 r and width 2.5 r for particle radius r.
+ * wet = 1 makes every primitive and SDF collider of the fluid's loop a
+ * wetted wall (the `colliders` option): the liquid draws a meniscus there
+ * and is carved out of them. The `solids` range's particles wet the liquid
+ * too. Rendering options (appearance, carve, cavities) have no counterpart. */
+typedef struct pf_surface_field_options {
+  double min[3], max[3];  /* bounds (a Box3) */
+  double voxel_budget;
+  double motion_stretch;  /* seconds, >= 0 */
+  uint32_t solids_start, solids_count;
+  int wet;
+  double radius, meniscus_height, meniscus_width;
+} pf_surface_field_options;
+typedef struct pf_surface_field_s pf_surface_field;
+int pf_surface_field_create(pf_fluid* fluid, const pf_surface_field_options* options, pf_surface_field** out);
+/* grid resolution, origin (bounds.min) and voxel edge in metres */
+int pf_surface_field_info(pf_surface_field* field, uint32_t resolution[3], double origin[3], double* voxel);
+/* update(): enqueues the field's passes on the fluid's stream */
+int pf_surface_field_update(pf_surface_field* field);
+/* copies resolution[0]*resolution[1]*resolution[2] float4 voxels to rgba */
+int pf_surface_field_readback(pf_surface_field* field, float* rgba);
+int pf_surface_field_destroy(pf_surface_field* field);
+
 /* ---- PrimitiveSet (src/core/collision/PrimitiveSet.ts:121-518) -----------
  * mu_s / mu_k < 0 take the defaults 0.5 / 0.4 (PrimitiveSet.ts:529-544). */
 int pf_primitives_create(pf_particles* ps, uint32_t capacity, pf_primitives** out);

@@ -81,6 +81,8 @@ def lib():
         g("scene_set_contact").restype = i32
         g("scene_contacts").argtypes = [vp, P(u32), u32]
         g("scene_contacts").restype = u32
+        g("scene_surface_field").argtypes = [vp, P(f64), P(f64), f64, f64, f64, P(u32), i32, P(f64), vp, P(u32), P(f64)]
+        g("scene_surface_field").restype = i32
         g("scene_step").argtypes = [vp, f64]
         g("scene_step").restype = None
         g("scene_get").argtypes = [vp, P(f64), P(f64), P(f64), P(f64)]
@@ -161,6 +163,7 @@ class Scene:
 
     def __init__(self, n, radius, precision="f32"):
         self.suf = precision
+        self.radius = radius
         self.L = lib()
         self.n = int(n)
         self.h = self._f("scene_create")(self.n, float(radius))
@@ -291,6 +294,27 @@ class Scene:
         pairs = np.zeros((cap, 2), np.uint32)
         count = int(self._f("scene_contacts")(self.h, _p(pairs, u32), cap))
         return count, pairs[:min(count, cap)].copy()
+
+    def surface_field(self, mn, mx, voxel_budget, spacing, fluid_range, solid_range=(0, 0), stretch=0.0, wet=False,
+                      radius=None, meniscus_height=None, meniscus_width=None):
+        """One frame of SurfaceField (SurfaceField.ts:101-369) over the
+        committed positions: (field (nz, ny, nx, 4) of phi in voxels, volume
+        fraction, wall proximity, 1; voxel size)."""
+        r = float(self.radius)
+        uni = np.array([r * 1.25 if radius is None else radius, r * 1.6 if meniscus_height is None else meniscus_height,
+                        r * 2.5 if meniscus_width is None else meniscus_width], np.float64)
+        lo, hi = np.asarray(mn, np.float64), np.asarray(mx, np.float64)
+        rng = np.array([fluid_range[0], fluid_range[1], solid_range[0], solid_range[1]], np.uint32)
+        res = np.zeros(3, np.uint32)
+        vox = np.zeros(1)
+        f = self._f("scene_surface_field")
+        assert f(self.h, _p(lo, f64), _p(hi, f64), float(voxel_budget), float(stretch), float(spacing), _p(rng, u32),
+                 1 if wet else 0, _p(uni, f64), None, _p(res, u32), _p(vox, f64)) == 0
+        dt = np.float32 if self.suf == "f32" else np.float64
+        out = np.zeros((int(res[2]), int(res[1]), int(res[0]), 4), dt)
+        assert f(self.h, _p(lo, f64), _p(hi, f64), float(voxel_budget), float(stretch), float(spacing), _p(rng, u32),
+                 1 if wet else 0, _p(uni, f64), out.ctypes.data_as(C.c_void_p), _p(res, u32), _p(vox, f64)) == 0
+        return out, float(vox[0])
 
     def step(self, dt):
         self._f("scene_step")(self.h, float(dt))

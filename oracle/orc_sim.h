@@ -555,6 +555,10 @@ static void SFX(pivot)(const SFX(prim)* P, REAL o[3]) {
   }
 }
 
+/* The primitive's signed distance and gradient at x as it stands now
+ * (primitives.ts:119-217), inversion included. */
+static void SFX(prim_shape)(const SFX(prim)* P, const REAL x[3], REAL* phi_out, REAL g_out[3]);
+
 static void SFX(sdf)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL* phi_out, REAL grad[3]) {
   REAL t = (P->flags & 2u) ? rewind : R(0.0);
   REAL shifted[3], pv[3], x[3], turn[4], rel[3], rr[3];
@@ -571,6 +575,14 @@ static void SFX(sdf)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL* ph
   for (int a = 0; a < 3; ++a) rel[a] = shifted[a] - pv[a];
   SFX(qrot)(turn, rel, rr);
   for (int a = 0; a < 3; ++a) x[a] = pv[a] + rr[a];
+  REAL phi, g[3];
+  SFX(prim_shape)(P, x, &phi, g);
+  *phi_out = phi;
+  REAL un[4] = {-turn[0], -turn[1], -turn[2], turn[3]};
+  SFX(qrot)(un, g, grad);
+}
+
+static void SFX(prim_shape)(const SFX(prim)* P, const REAL x[3], REAL* phi_out, REAL g_out[3]) {
   REAL phi = 0, g[3] = {0, 0, 0};
   if (P->kind == 0) {
     REAL d[3] = {x[0] - P->d1[0], x[1] - P->d1[1], x[2] - P->d1[2]};
@@ -621,8 +633,7 @@ static void SFX(sdf)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL* ph
     g[0] = -g[0]; g[1] = -g[1]; g[2] = -g[2];
   }
   *phi_out = phi;
-  REAL un[4] = {-turn[0], -turn[1], -turn[2], turn[3]};
-  SFX(qrot)(un, g, grad);
+  g_out[0] = g[0]; g_out[1] = g[1]; g_out[2] = g[2];
 }
 
 static void SFX(surface_velocity)(const SFX(prim)* P, const REAL xw[3], REAL rewind, REAL o[3]) {
@@ -1302,6 +1313,182 @@ static void SFX(contact_friction)(SFX(scene)* s) {
       s->vacc[3 * (size_t)j + a] = SFX(wadd)(s->vacc[3 * (size_t)j + a], SFX(ticks)(-(imp * wj), s->vel_scale));
     }
   }
+}
+
+/* --------------------------------------------------------------------- */
+/* surface field (src/fluids/render/SurfaceField.ts:101-369)             */
+/* --------------------------------------------------------------------- */
+/* FIXED (SurfaceField.ts:32) and FIELD_BAND (34) */
+#define ORC_FIELD_FIXED 65536.0
+#define ORC_FIELD_BAND 6.0
+
+/* The field's constants, as the reference turns its JS doubles into the
+ * f32 literals and uniforms the kernels read. */
+typedef struct {
+  uint32_t n[3], count, taps, samples;
+  uint32_t fluid_start, fluid_count, solid_start, solid_count, wet;
+  REAL origin[3], dims[3], voxel, inv_voxel;
+  REAL weights[13];
+  REAL stretch, reach3, inv_samples, vol_scale, solid_scale;
+  REAL radius, men_h, men_w;
+} SFX(fieldp);
+
+static inline uint32_t SFX(fidx)(const SFX(fieldp)* F, REAL cx, REAL cy, REAL cz) {
+  /* indexOf over float cell coordinates (SurfaceField.ts:147-152) */
+  REAL v = (cz * (REAL)(F->n[0] * F->n[1]) + cy * (REAL)F->n[0]) + cx;
+  return (uint32_t)v;
+}
+
+/* splat (SurfaceField.ts:174-214): cloud-in-cell weights of each sample in
+ * fixed point; `fluid` adds mass and mass-weighted offsets, else solid mass. */
+static void SFX(field_splat)(const SFX(scene)* s, const SFX(fieldp)* F, int32_t* mass, int32_t* off, int32_t* solid,
+                             uint32_t start, uint32_t count, int fluid) {
+  for (uint32_t i = start; i < start + count; ++i) {
+    const REAL* pos = s->x + 3 * (size_t)i;
+    uint32_t samples = fluid ? F->samples : 1u;
+    REAL trail[3] = {0, 0, 0};
+    if (samples > 1) {
+      REAL vv[3];
+      for (int a = 0; a < 3; ++a) vv[a] = s->v[3 * (size_t)i + a] * F->stretch;
+      REAL len = RSQRT(SFX(dot)(vv, vv));
+      REAL k = F->reach3 / RFMAX(len, F->reach3);
+      for (int a = 0; a < 3; ++a) trail[a] = vv[a] * k;
+    }
+    for (uint32_t sm = 0; sm < samples; ++sm) {
+      REAL at[3], g[3];
+      REAL frac = samples > 1 ? (REAL)((double)sm / (double)(samples - 1)) : R(0.0);
+      for (int a = 0; a < 3; ++a) at[a] = sm == 0 ? pos[a] : pos[a] - trail[a] * frac;
+      for (int a = 0; a < 3; ++a) g[a] = (at[a] - F->origin[a]) * F->inv_voxel - R(0.5);
+      int inside = 1;
+      for (int a = 0; a < 3; ++a) inside &= g[a] >= R(0.0) && g[a] < F->dims[a] - R(1.0);
+      if (!inside) continue;
+      REAL b[3], f[3];
+      for (int a = 0; a < 3; ++a) { b[a] = RFLOOR(g[a]); f[a] = g[a] - b[a]; }
+      for (int z = 0; z < 2; ++z)
+        for (int y = 0; y < 2; ++y)
+          for (int x = 0; x < 2; ++x) {
+            REAL w = (((x ? f[0] : R(1.0) - f[0]) * (y ? f[1] : R(1.0) - f[1])) * (z ? f[2] : R(1.0) - f[2])) *
+                     (fluid ? F->inv_samples : R(1.0));
+            uint32_t c = SFX(fidx)(F, b[0] + (REAL)x, b[1] + (REAL)y, b[2] + (REAL)z);
+            if (fluid) {
+              mass[c] = SFX(wadd)(mass[c], orc_f2i_sat((double)RRINT(w * (REAL)ORC_FIELD_FIXED)));
+              off[3 * (size_t)c + 0] = SFX(wadd)(off[3 * (size_t)c + 0], orc_f2i_sat((double)RRINT((w * (f[0] - (REAL)x)) * (REAL)ORC_FIELD_FIXED)));
+              off[3 * (size_t)c + 1] = SFX(wadd)(off[3 * (size_t)c + 1], orc_f2i_sat((double)RRINT((w * (f[1] - (REAL)y)) * (REAL)ORC_FIELD_FIXED)));
+              off[3 * (size_t)c + 2] = SFX(wadd)(off[3 * (size_t)c + 2], orc_f2i_sat((double)RRINT((w * (f[2] - (REAL)z)) * (REAL)ORC_FIELD_FIXED)));
+            } else {
+              solid[c] = SFX(wadd)(solid[c], orc_f2i_sat((double)RRINT(w * (REAL)ORC_FIELD_FIXED)));
+            }
+          }
+    }
+  }
+}
+
+/* wallKernel (SurfaceField.ts:251-285): wetting proximity and the nearest
+ * collider distance per voxel, from the scene's primitives and SDF colliders. */
+static void SFX(field_walls)(const SFX(scene)* s, const SFX(fieldp)* F, REAL* walls) {
+  for (uint32_t i = 0; i < F->count; ++i) {
+    REAL prox = R(0.0), nearest = R(1e3);
+    if (F->wet && (s->nprim || s->nsdf)) {
+      uint32_t c[3] = {i % F->n[0], (i / F->n[0]) % F->n[1], i / (F->n[0] * F->n[1])};
+      REAL p[3];
+      for (int a = 0; a < 3; ++a) p[a] = F->origin[a] + ((REAL)c[a] + R(0.5)) * F->voxel;
+      for (uint32_t k = 0; k < s->nprim; ++k) {
+        REAL phi, g[3];
+        SFX(prim_shape)(&s->prim[k], p, &phi, g);
+        nearest = RFMIN(nearest, phi);
+      }
+      for (uint32_t k = 0; k < s->nsdf; ++k) {
+        const SFX(sdfc)* D = &s->sdf[k];
+        REAL phi, g[3];
+        SFX(sdf_sample)(D, p, &phi, g);
+        REAL half[3], bo[3], ctr[3], d[3];
+        for (int a = 0; a < 3; ++a) { half[a] = (D->res[a] * D->voxel[a]) * R(0.5); bo[a] = D->origin[a] + half[a]; }
+        for (int r = 0; r < 3; ++r)
+          ctr[r] = D->pos[r] + ((D->rot[3 * r] * bo[0] + D->rot[3 * r + 1] * bo[1]) + D->rot[3 * r + 2] * bo[2]) * D->scale;
+        for (int a = 0; a < 3; ++a) d[a] = p[a] - ctr[a];
+        REAL bound = RSQRT(SFX(dot)(d, d)) - RSQRT(SFX(dot)(half, half)) * D->scale;
+        nearest = RFMIN(nearest, RFMAX(phi, bound));
+      }
+      REAL width = RFMAX(F->men_w, (REAL)1e-5);
+      prox = REXP(-(RFMAX(nearest, R(0.0)) / width));
+    }
+    walls[2 * (size_t)i] = prox;
+    walls[2 * (size_t)i + 1] = nearest;
+  }
+}
+
+/* The three separable Gaussian passes and the final field
+ * (SurfaceField.ts:286-353); out: (phi in voxels, volume fraction,
+ * wall proximity, 1) per voxel. */
+static void SFX(field_blur)(const SFX(fieldp)* F, const int32_t* mass, const int32_t* off, const int32_t* solid,
+                            const REAL* walls, REAL* out) {
+  uint32_t count = F->count, nx = F->n[0], ny = F->n[1];
+  REAL* a = (REAL*)calloc(4 * (size_t)count, sizeof(REAL));
+  REAL* b = (REAL*)calloc(4 * (size_t)count, sizeof(REAL));
+  REAL* sa = (REAL*)calloc(count, sizeof(REAL));
+  REAL* sb = (REAL*)calloc(count, sizeof(REAL));
+  for (int axis = 0; axis < 3; ++axis) {
+    for (uint32_t i = 0; i < count; ++i) {
+      int32_t c[3] = {(int32_t)(i % nx), (int32_t)((i / nx) % ny), (int32_t)(i / (nx * ny))};
+      REAL sum[4] = {0, 0, 0, 0}, ssum = R(0.0);
+      for (int k = -(int)F->taps; k <= (int)F->taps; ++k) {
+        int32_t nb[3] = {c[0], c[1], c[2]};
+        nb[axis] += k;
+        if (nb[axis] < 0 || nb[axis] >= (int32_t)F->n[axis]) continue;
+        uint32_t j = (uint32_t)nb[2] * (nx * ny) + (uint32_t)nb[1] * nx + (uint32_t)nb[0];
+        REAL sm[4], ss;
+        if (axis == 0) {
+          sm[0] = (REAL)mass[j] / (REAL)ORC_FIELD_FIXED;
+          for (int q = 0; q < 3; ++q) sm[1 + q] = (REAL)off[3 * (size_t)j + q] / (REAL)ORC_FIELD_FIXED;
+          ss = (REAL)solid[j] / (REAL)ORC_FIELD_FIXED;
+        } else {
+          const REAL* src = axis == 1 ? a : b;
+          for (int q = 0; q < 4; ++q) sm[q] = src[4 * (size_t)j + q];
+          ss = (axis == 1 ? sa : sb)[j];
+        }
+        REAL reb[4] = {sm[0], sm[1], sm[2], sm[3]};
+        for (int q = 0; q < 3; ++q) reb[1 + q] = sm[1 + q] + (REAL)(q == axis ? k : 0) * sm[0];
+        REAL wk = F->weights[k + (int)F->taps];
+        for (int q = 0; q < 4; ++q) sum[q] = sum[q] + reb[q] * wk;
+        ssum = ssum + ss * wk;
+      }
+      if (axis < 2) {
+        REAL* dst = axis == 0 ? a : b;
+        for (int q = 0; q < 4; ++q) dst[4 * (size_t)i + q] = sum[q];
+        (axis == 0 ? sa : sb)[i] = ssum;
+        continue;
+      }
+      REAL w = sum[0];
+      REAL den = RFMAX(w, (REAL)1e-8);
+      REAL mean[3] = {sum[1] / den, sum[2] / den, sum[3] / den};
+      REAL radius = F->radius * F->inv_voxel;
+      REAL phi = w > (REAL)1e-5 ? RSQRT(SFX(dot)(mean, mean)) - radius : (REAL)ORC_FIELD_BAND;
+      phi = RFMIN(phi, (REAL)ORC_FIELD_BAND);
+      REAL sp = RFMIN(RFMAX(ssum * F->solid_scale, R(0.0)), R(1.0));
+      REAL prox = RFMAX(sp, walls[2 * (size_t)i]);
+      REAL lift = F->men_h * F->inv_voxel;
+      REAL reach = RFMIN(RFMAX(R(1.0) - phi / RFMAX(lift * R(1.5), (REAL)1e-5), R(0.0)), R(1.0));
+      phi = phi - (lift * prox) * (reach * reach);
+      phi = RFMAX(phi, -walls[2 * (size_t)i + 1] * F->inv_voxel);
+      out[4 * (size_t)i + 0] = phi;
+      out[4 * (size_t)i + 1] = RFMIN(w * F->vol_scale, R(4.0));
+      out[4 * (size_t)i + 2] = prox;
+      out[4 * (size_t)i + 3] = R(1.0);
+    }
+  }
+  free(a); free(b); free(sa); free(sb);
+}
+
+static void SFX(surface_field)(const SFX(scene)* s, const SFX(fieldp)* F, REAL* out) {
+  int32_t* mass = (int32_t*)calloc(F->count, sizeof(int32_t));
+  int32_t* off = (int32_t*)calloc(3 * (size_t)F->count, sizeof(int32_t));
+  int32_t* solid = (int32_t*)calloc(F->count, sizeof(int32_t));
+  REAL* walls = (REAL*)calloc(2 * (size_t)F->count, sizeof(REAL));
+  SFX(field_splat)(s, F, mass, off, solid, F->fluid_start, F->fluid_count, 1);
+  if (F->solid_count) SFX(field_splat)(s, F, mass, off, solid, F->solid_start, F->solid_count, 0);
+  SFX(field_walls)(s, F, walls);
+  SFX(field_blur)(F, mass, off, solid, walls, out);
+  free(mass); free(off); free(solid); free(walls);
 }
 
 /* --------------------------------------------------------------------- */

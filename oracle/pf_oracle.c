@@ -201,6 +201,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
 #define RCOS cosf
 #define RATAN2 atan2f
 #define RRINT rintf
+#define REXP expf
 #include "orc_sim.h"
 #undef REAL
 #undef SFX
@@ -214,6 +215,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
 #undef RSIN
 #undef RCOS
 #undef RRINT
+#undef REXP
 
 #define REAL double
 #define SFX(name) name##_f64
@@ -227,6 +229,7 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
 #define RCOS cos
 #define RATAN2 atan2
 #define RRINT rint
+#define REXP exp
 #include "orc_sim.h"
 #undef REAL
 #undef SFX
@@ -528,6 +531,54 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     uint32_t used = s->ct_count < s->ct_max ? s->ct_count : s->ct_max;                             \
     for (uint32_t c = 0; c < used && c < cap; ++c) { pairs[2 * c] = s->ct_i[c]; pairs[2 * c + 1] = s->ct_j[c]; } \
     return s->ct_count;                                                                            \
+  }                                                                                                \
+  /* new SurfaceField({min, max, voxelBudget, motionStretch, solids, colliders})                 \
+   * (SurfaceField.ts:116-244): the constructor's double arithmetic, then one                     \
+   * frame of its kernels; out: 4 values per voxel, x fastest; res, voxel: the                     \
+   * grid. radius / men_h / men_w: the uniforms (r 1.25, r 1.6, r 2.5 by default). */            \
+  int orc_scene_surface_field_##SUF(void* p, const double mn[3], const double mx[3], double budget,   \
+                                    double stretch, double spacing, const uint32_t ranges[4],       \
+                                    int wet, const double uni[3], T* out, uint32_t res[3],          \
+                                    double* voxel_out) {                                            \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    double sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};                                 \
+    if (!(sz[0] > 0 && sz[1] > 0 && sz[2] > 0)) return -1;                                         \
+    double voxel = fmax(s->radius_d, cbrt((sz[0] * sz[1] * sz[2]) / budget));                       \
+    fieldp_##SUF F;                                                                                \
+    memset(&F, 0, sizeof F);                                                                       \
+    for (int a = 0; a < 3; ++a) {                                                                  \
+      F.n[a] = (uint32_t)ceil(sz[a] / voxel);                                                      \
+      F.origin[a] = (T)mn[a];                                                                      \
+      F.dims[a] = (T)F.n[a];                                                                       \
+      res[a] = F.n[a];                                                                             \
+    }                                                                                              \
+    *voxel_out = voxel;                                                                            \
+    F.count = F.n[0] * F.n[1] * F.n[2];                                                            \
+    if (!out) return 0;                                                                            \
+    F.voxel = (T)voxel;                                                                            \
+    F.inv_voxel = (T)(1.0 / voxel);                                                                \
+    double sigma = (0.75 * spacing) / voxel;                                                       \
+    int taps = (int)fmin(6.0, fmax(2.0, ceil(2.5 * sigma)));                                       \
+    double wsum = 0, wt[13];                                                                       \
+    for (int k = 0; k <= 2 * taps; ++k) {                                                          \
+      wt[k] = exp(-((double)(k - taps) * (double)(k - taps)) / (2 * sigma * sigma));              \
+      wsum += wt[k];                                                                               \
+    }                                                                                              \
+    for (int k = 0; k <= 2 * taps; ++k) F.weights[k] = (T)(wt[k] / wsum);                         \
+    F.taps = (uint32_t)taps;                                                                       \
+    F.samples = stretch > 0 ? 3u : 1u;                                                             \
+    F.stretch = (T)stretch;                                                                        \
+    F.reach3 = (T)(3 * spacing);                                                                   \
+    F.inv_samples = (T)(1.0 / F.samples);                                                          \
+    double vs = (spacing / voxel) * (spacing / voxel) * (spacing / voxel);                         \
+    F.vol_scale = (T)vs;                                                                           \
+    F.solid_scale = (T)(vs * 1.5);                                                                 \
+    F.fluid_start = ranges[0]; F.fluid_count = ranges[1];                                          \
+    F.solid_start = ranges[2]; F.solid_count = ranges[3];                                          \
+    F.wet = wet ? 1u : 0u;                                                                         \
+    F.radius = (T)uni[0]; F.men_h = (T)uni[1]; F.men_w = (T)uni[2];                                \
+    surface_field_##SUF(s, &F, out);                                                               \
+    return 0;                                                                                      \
   }                                                                                                \
   void orc_scene_step_##SUF(void* p, double dt) { step_##SUF((scene_##SUF*)p, dt); }               \
   /* state out as double: x, x*, v as xyz triples, density */                                     \

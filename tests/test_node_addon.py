@@ -128,3 +128,34 @@ def test_js_formats_match_python():
     assert out["bytes"] == raw.hex() and out["res"] == [3, 4, 5]
     rnd = formats.seeded_random(0x9E3779B9)
     assert out["draws"] == [rnd() for _ in range(8)]
+
+
+@pytest.mark.gpu
+def test_js_surface_field_matches_python(tmp_path):
+    """SurfaceField through the JS API (bounds, voxelBudget, motionStretch,
+    wetted colliders) equals the Python API's field bit for bit."""
+    out = tmp_path / "field.bin"
+    r = subprocess.run([NODE, os.path.join(JS, "test", "surface_field.js"), str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    rr = 0.01
+    pos = np.array([(-0.05 + (ix + 0.5) * 2 * rr, 0.01 + iy * 2 * rr, -0.05 + (iz + 0.5) * 2 * rr)
+                    for ix in range(5) for iy in range(4) for iz in range(5)], np.float32)
+    ps = pf.ParticleSystem(len(pos), rr)
+    ps.uploadParticles(pos)
+    fluid = pf.FluidSystem(ps)
+    walls = pf.PrimitiveSet(ps)
+    walls.addPlane((0, 1, 0), (0, 0, 0))
+    loop = pf.SimLoop(ps, substeps=2, iterations=3, materials=[fluid], colliders=[walls])
+    for _ in range(3):
+        loop.step(1 / 60)
+    field = pf.SurfaceField(fluid, ((-0.2, -0.02, -0.2), (0.2, 0.2, 0.2)), voxelBudget=100000, motionStretch=0.01,
+                            wet=True)
+    field.update()
+    want = field.readback()
+    assert info["resolution"] == list(field.resolution) and info["voxel"] == field.voxel
+    got = np.fromfile(out, np.float32).reshape(want.shape)
+    assert (got[..., 0] < 0).sum() > 50
+    assert np.array_equal(got, want)
+    field.dispose()

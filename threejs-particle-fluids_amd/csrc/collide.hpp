@@ -34,6 +34,11 @@ __device__ __forceinline__ float3 prim_pivot(uint32_t kind, float4 d0, float4 d1
 
 // Signed distance and outward gradient of primitive `slot` at xw, evaluated
 // as the primitive was `rewind` seconds ago when it sweeps (primitives.ts:85-217).
+// Signed distance and gradient of primitive `slot` at x as it stands now
+// (primitives.ts:119-217), inversion included.
+__device__ __forceinline__ void prim_shape(const SceneDev& S, uint32_t slot, uint32_t kind, uint32_t flags,
+                                           float4 d0, float4 d1, float3 x, float* phi_out, float3* grad_out);
+
 __device__ __forceinline__ void prim_sdf(const SceneDev& S, uint32_t slot, float3 xw, float rewind,
                                          float* phi_out, float3* grad_out) {
   const uint32_t packed = S.prim_packed[slot];
@@ -46,6 +51,25 @@ __device__ __forceinline__ void prim_sdf(const SceneDev& S, uint32_t slot, float
   const float3 pivot = prim_pivot(kind, d0, d1);
   const float4 turn = quat_from_spin(pf_xyz(S.prim_spin[slot]), t);
   const float3 x = pf_add3(pivot, quat_rotate(turn, pf_sub3(shifted, pivot)));
+  float phi;
+  float3 g;
+  prim_shape(S, slot, kind, flags, d0, d1, x, &phi, &g);
+  *phi_out = phi;
+  *grad_out = quat_rotate(quat_conj(turn), g);
+}
+
+// The primitive's distance at xw with no sweep rewind (primitives.ts:85-92
+// called without `rewind`).
+__device__ __forceinline__ float prim_phi_now(const SceneDev& S, uint32_t slot, float3 xw) {
+  const uint32_t packed = S.prim_packed[slot];
+  float phi;
+  float3 g;
+  prim_shape(S, slot, (packed >> 16) & 0xffffu, packed & 0xffffu, S.prim_d0[slot], S.prim_d1[slot], xw, &phi, &g);
+  return phi;
+}
+
+__device__ __forceinline__ void prim_shape(const SceneDev& S, uint32_t slot, uint32_t kind, uint32_t flags,
+                                           float4 d0, float4 d1, float3 x, float* phi_out, float3* grad_out) {
   float phi = 0.0f;
   float3 g = make_float3(0.f, 0.f, 0.f);
   if (kind == PF_KIND_PLANE) {
@@ -99,7 +123,7 @@ __device__ __forceinline__ void prim_sdf(const SceneDev& S, uint32_t slot, float
     g = make_float3(-g.x, -g.y, -g.z);
   }
   *phi_out = phi;
-  *grad_out = quat_rotate(quat_conj(turn), g);
+  *grad_out = g;
 }
 
 // v + omega x (x - c + v t) (primitives.ts:224-242).

@@ -831,6 +831,69 @@ napi_value ViscosityDestroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------- surface field
+// (fluid, {min, max, voxelBudget, motionStretch, solids: {start, count}, wet,
+//  radius, meniscusHeight, meniscusWidth}) -> {handle, resolution, origin, voxel}
+napi_value SurfaceFieldCreate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  napi_value o = a.argv[1];
+  pf_surface_field_options opt{};
+  auto mn = doubles(env, prop(env, o, "min")), mx = doubles(env, prop(env, o, "max"));
+  if (mn.size() != 3 || mx.size() != 3) {
+    napi_throw_type_error(env, nullptr, "FluidSurfaceRenderer: bounds need min and max of three numbers");
+    return nullptr;
+  }
+  for (int k = 0; k < 3; ++k) {
+    opt.min[k] = mn[k];
+    opt.max[k] = mx[k];
+  }
+  opt.voxel_budget = num(env, prop(env, o, "voxelBudget"), 0);
+  opt.motion_stretch = num(env, prop(env, o, "motionStretch"), 0);
+  napi_value sol = prop(env, o, "solids");
+  if (!is_nullish(env, sol)) {
+    opt.solids_start = u32(env, prop(env, sol, "start"));
+    opt.solids_count = u32(env, prop(env, sol, "count"));
+  }
+  bool wet = false;
+  napi_value w = prop(env, o, "wet");
+  if (!is_nullish(env, w)) napi_get_value_bool(env, w, &wet);
+  opt.wet = wet ? 1 : 0;
+  opt.radius = num(env, prop(env, o, "radius"), -1);
+  opt.meniscus_height = num(env, prop(env, o, "meniscusHeight"), -1);
+  opt.meniscus_width = num(env, prop(env, o, "meniscusWidth"), -1);
+  pf_surface_field* f = nullptr;
+  PF_CALL(pf_surface_field_create((pf_fluid*)handle(env, a.argv[0]), &opt, &f));
+  uint32_t res[3];
+  double org[3], voxel = 0;
+  PF_CALL(pf_surface_field_info(f, res, org, &voxel));
+  const double rd[3] = {(double)res[0], (double)res[1], (double)res[2]};
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "handle", make_handle(env, f));
+  set(env, obj, "resolution", num_array(env, rd, 3));
+  set(env, obj, "origin", num_array(env, org, 3));
+  set(env, obj, "voxel", number(env, voxel));
+  return obj;
+}
+napi_value SurfaceFieldUpdate(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_surface_field_update((pf_surface_field*)handle(env, a.argv[0])));
+  return nullptr;
+}
+// (handle, voxelCount) -> Float32Array of 4 floats per voxel
+napi_value SurfaceFieldReadback(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  float* d = nullptr;
+  napi_value arr = f32_array(env, 4 * (size_t)num(env, a.argv[1]), &d);
+  PF_CALL(pf_surface_field_readback((pf_surface_field*)handle(env, a.argv[0]), d));
+  return arr;
+}
+napi_value SurfaceFieldDestroy(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_surface_field_destroy((pf_surface_field*)handle(env, a.argv[0])));
+  return nullptr;
+}
+
 // ---------------------------------------------------------------- autotuner
 // (handle, dt, [candidates], steps) -> {best, msPerStep: {capacity: ms}}
 template <typename Fn>
@@ -1119,6 +1182,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"loopInfo", LoopInfo}, {"loopGrid", LoopGrid}, {"loopDestroy", LoopDestroy},
       {"loopAutotune", LoopAutotune}, {"particlesDeviceBytes", ParticlesDeviceBytes},
       {"viscosityCreate", ViscosityCreate}, {"viscositySet", ViscositySet}, {"viscosityDestroy", ViscosityDestroy},
+      {"surfaceFieldCreate", SurfaceFieldCreate}, {"surfaceFieldUpdate", SurfaceFieldUpdate},
+      {"surfaceFieldReadback", SurfaceFieldReadback}, {"surfaceFieldDestroy", SurfaceFieldDestroy},
       {"batchCreate", BatchCreate}, {"batchStepAsync", BatchStepAsync}, {"batchSetGraph", BatchSetGraph},
       {"batchAutotune", BatchAutotune}, {"batchDestroy", BatchDestroy},
       {"serverCreate", ServerCreate}, {"serverAddScene", ServerAddScene}, {"serverRemoveScene", ServerRemoveScene},

@@ -199,6 +199,51 @@ class ViscositySolver {
   }
 }
 
+/**
+ * The liquid's surface as a voxel grid (src/fluids/render/SurfaceField.ts:
+ * 101-369), the field FluidSurfaceRenderer sphere-traces, computed headless.
+ * Options follow FluidSurfaceRendererOptions: bounds ({min, max}),
+ * voxelBudget, motionStretch, solids (a particle range); `wet: true` makes
+ * the loop's colliders wetted walls. update() enqueues the passes;
+ * readback() resolves to a Float32Array of (phi in voxels, volume fraction,
+ * wall proximity, 1) per voxel, x fastest.
+ */
+class SurfaceField {
+  constructor(fluid, options) {
+    const b = options.bounds;
+    const info = native.surfaceFieldCreate(fluid.handle, {
+      min: vec(b.min),
+      max: vec(b.max),
+      voxelBudget: options.voxelBudget,
+      motionStretch: options.motionStretch,
+      solids: options.solids,
+      wet: !!options.wet,
+      radius: options.radius,
+      meniscusHeight: options.meniscusHeight,
+      meniscusWidth: options.meniscusWidth,
+    });
+    this.fluid = fluid;
+    this.handle = info.handle;
+    this.resolution = info.resolution;
+    this.origin = info.origin;
+    this.voxel = info.voxel;
+  }
+  static defaultVoxelBudget(particleCount) {
+    return particleCount >= 50000 ? 1200000 : particleCount >= 20000 ? 900000 : 600000;
+  }
+  update() {
+    native.surfaceFieldUpdate(this.handle);
+  }
+  readback() {
+    const [nx, ny, nz] = this.resolution;
+    return native.surfaceFieldReadback(this.handle, nx * ny * nz);
+  }
+  dispose() {
+    if (this.handle) native.surfaceFieldDestroy(this.handle);
+    this.handle = null;
+  }
+}
+
 class PrimitiveSet {
   constructor(particles, options = {}) {
     this.particles = particles;
@@ -695,6 +740,7 @@ module.exports = {
   SoftbodySystem,
   SimLoop,
   ContactBuffer,
+  SurfaceField,
   decodeSdfBinary,
   encodeSdfBinary,
   loadSampledBody,

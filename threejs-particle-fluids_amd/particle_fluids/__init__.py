@@ -18,7 +18,7 @@ from ._lib import PFError, call, load
 __all__ = [
     "PFError", "Context", "ParticleSystem", "HashGrid", "FluidSystem", "PrimitiveSet",
     "SoftbodySystem", "SDFCollider", "SimLoop", "Batch", "Simulation", "fill_box", "fit_budget",
-    "radius_for_budget", "NoSelfContacts", "ContactBuffer",
+    "radius_for_budget", "NoSelfContacts", "ContactBuffer", "SurfaceField", "ViscositySolver",
 ]
 
 P = C.POINTER
@@ -375,6 +375,57 @@ class ViscositySolver:
     def dispose(self):
         if self.handle:
             call("pf_viscosity_destroy", self.handle)
+            self.handle = None
+
+
+class SurfaceField:
+    """The liquid's surface as a voxel grid (src/fluids/render/SurfaceField.ts:
+    101-369), the field FluidSurfaceRenderer sphere-traces, computed headless:
+    `update()` splats the committed particles and blurs them; `readback()`
+    returns (nz, ny, nx, 4) float32 voxels of (phi in voxels, volume
+    fraction, wall proximity, 1). `bounds` is ((min x, y, z), (max x, y, z));
+    `wet` makes the loop's colliders wetted walls (the `colliders` option);
+    `solids` is a particle range the liquid wets. `radius`,
+    `meniscusHeight` and `meniscusWidth` default to 1.25, 1.6 and 2.5
+    particle radii (SurfaceField.ts:101-115)."""
+
+    def __init__(self, fluid, bounds, voxelBudget=None, motionStretch=0.0, solids=None, wet=False,
+                 radius=None, meniscusHeight=None, meniscusWidth=None):
+        o = _lib.SurfaceFieldOptions()
+        (o.min[0], o.min[1], o.min[2]), (o.max[0], o.max[1], o.max[2]) = (
+            tuple(float(v) for v in bounds[0]), tuple(float(v) for v in bounds[1]))
+        o.voxel_budget = float(voxelBudget) if voxelBudget is not None else 0.0
+        o.motion_stretch = float(motionStretch)
+        if solids is not None:
+            o.solids_start, o.solids_count = _range(solids, 0)
+        o.wet = 1 if wet else 0
+        o.radius = float(radius) if radius is not None else -1.0
+        o.meniscus_height = float(meniscusHeight) if meniscusHeight is not None else -1.0
+        o.meniscus_width = float(meniscusWidth) if meniscusWidth is not None else -1.0
+        h = C.c_void_p()
+        call("pf_surface_field_create", fluid.handle, C.byref(o), C.byref(h))
+        self.handle = h
+        self.fluid = fluid
+        res = (C.c_uint32 * 3)()
+        org = (C.c_double * 3)()
+        vox = C.c_double()
+        call("pf_surface_field_info", h, res, org, C.byref(vox))
+        self.resolution = (res[0], res[1], res[2])
+        self.origin = (org[0], org[1], org[2])
+        self.voxel = vox.value
+
+    def update(self):
+        call("pf_surface_field_update", self.handle)
+
+    def readback(self):
+        nx, ny, nz = self.resolution
+        out = np.zeros((nz, ny, nx, 4), np.float32)
+        call("pf_surface_field_readback", self.handle, _ptr(out, f32p))
+        return out
+
+    def dispose(self):
+        if self.handle:
+            call("pf_surface_field_destroy", self.handle)
             self.handle = None
 
 

@@ -123,6 +123,21 @@ class ServerStats(C.Structure):
         ("scenes", u32), ("resident_scenes", u32), ("max_launch_scenes", u32)]
 
 
+class SurfaceFieldOptions(C.Structure):
+    _fields_ = [
+        ("min", f64 * 3),
+        ("max", f64 * 3),
+        ("voxel_budget", f64),
+        ("motion_stretch", f64),
+        ("solids_start", u32),
+        ("solids_count", u32),
+        ("wet", i32),
+        ("radius", f64),
+        ("meniscus_height", f64),
+        ("meniscus_width", f64),
+    ]
+
+
 class RequestInfo(C.Structure):
     _fields_ = [("scene", u32), ("status", u32), ("launch_scenes", u32), ("pad", u32),
                 ("t_submit", f64), ("t_launch", f64), ("t_done", f64)]
@@ -204,6 +219,11 @@ SIGNATURES = {
     "pf_loop_readback_contacts": [vp, P(u32), P(u32), u32, P(u32)],
     "pf_sdf_decode": [vp, C.c_size_t, P(u32), P(f64), P(f64), P(f32), C.c_size_t, P(C.c_size_t)],
     "pf_sdf_encode": [P(f32), C.c_size_t, P(u32), P(f64), P(f64), vp, C.c_size_t, P(C.c_size_t)],
+    "pf_surface_field_create": [vp, P(SurfaceFieldOptions), P(vp)],
+    "pf_surface_field_info": [vp, P(u32), P(f64), P(f64)],
+    "pf_surface_field_update": [vp],
+    "pf_surface_field_readback": [vp, P(f32)],
+    "pf_surface_field_destroy": [vp],
     "pf_loop_set_substeps": [vp, u32],
     "pf_loop_set_gravity": [vp, P(f64)],
     "pf_loop_info": [vp, P(u32), P(u32), P(f64), P(u32), P(u32)],
