@@ -732,6 +732,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
   const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
   const uint32_t fend = F.start + F.count;
+  // This particle's half of its pairs is summed here and added once; wrap
+  // sums do not depend on the order.
+  int own[3] = {0, 0, 0};
+  bool any = false;
   for (uint32_t k = 0; k < cnt; ++k) {
     uint32_t j;
     if (staged) {
@@ -746,14 +750,145 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
     if (!moved_i && !(trunc_scene && nl_truncated(F, j)) && !(S.ct.on && S.ct.acc[j].w != 0)) continue;
     float3 dv;
     if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
-    int* ai = reinterpret_cast<int*>(S.ct.acc + s);
+    const int t[3] = {pf_ticks(dv.x, F.imp_scale), pf_ticks(dv.y, F.imp_scale), pf_ticks(dv.z, F.imp_scale)};
+    own[0] = pf_wrap_add(own[0], t[0]);
+    own[1] = pf_wrap_add(own[1], t[1]);
+    own[2] = pf_wrap_add(own[2], t[2]);
+    any = true;
+    // The partner's change is the exact negation (round is symmetric).
     int* aj = reinterpret_cast<int*>(S.ct.acc + j);
-    atomicAdd(ai + 0, pf_ticks(dv.x, F.imp_scale));
-    atomicAdd(ai + 1, pf_ticks(dv.y, F.imp_scale));
-    atomicAdd(ai + 2, pf_ticks(dv.z, F.imp_scale));
     atomicAdd(aj + 0, pf_ticks(-dv.x, F.imp_scale));
     atomicAdd(aj + 1, pf_ticks(-dv.y, F.imp_scale));
     atomicAdd(aj + 2, pf_ticks(-dv.z, F.imp_scale));
+  }
+  if (any) {
+    int* ai = reinterpret_cast<int*>(S.ct.acc + s);
+    atomicAdd(ai + 0, own[0]);
+    atomicAdd(ai + 1, own[1]);
+    atomicAdd(ai + 2, own[2]);
+  }
+}
+
+// The gathered surface tension and adhesion of one particle over its list
+// (cohesion.ts:63-103, adhesion.ts:34-60). Four neighbours at a time, every
+// gather of the four issued before the arithmetic (the pass is
+// latency-bound). Each pair is evaluated from its lower id's side: operands
+// swapped and the change negated on the upper side. The fixed-point sums do
+// not depend on the order. SPECIAL: the scene cut a list at 64 or has
+// contacts, so the walk keys and moved marks decide which pairs are the
+// scatter's; the partner's id, key and mark are read first and its position,
+// normal and density only for the pairs evaluated here. Otherwise every
+// fluid partner is evaluated here and keys and marks are not read.
+template <bool SPECIAL>
+static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const FluidDev& F, int p,
+                                                    const float4* __restrict__ xs, uint32_t s, uint32_t id,
+                                                    float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
+                                                    bool adh_on, bool moved_i, bool moved_tile, bool trunc_i,
+                                                    uint32_t U, const uint32_t* __restrict__ uslots, int tk[3]) {
+  const uint32_t n = S.n;
+  const uint32_t fend = F.start + F.count;
+  const bool staged = U != PF_TU_GLOBAL;
+  const int qx = pf_query_coord(xi.x, S.ox, S.cell), qy = pf_query_coord(xi.y, S.oy, S.cell),
+            qz = pf_query_coord(xi.z, S.oz, S.cell);
+  Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
+  NlCursor cg(F.nl, n, s, cnt);
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    uint32_t jj[4];
+    if (staged) {
+      uint32_t idx[4];
+      c16.next(k0, idx);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
+    } else {
+      cg.next(k0, jj);
+    }
+    uint32_t idj[4], lastj[4];
+    float4 xj[4], nj[4];
+    float rj[4], vol[4];
+    bool lower[4], upper[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      idj[u] = S.id[p][jj[u]];
+      lastj[u] = 0xffffffffu;
+      bool moved = false;
+      if (SPECIAL && st_pairs) {
+        lastj[u] = F.nl_last[jj[u]];
+        moved = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
+      }
+      vol[u] = adh_on ? S.bvol[p][jj[u]] : 0.0f;
+      if (!SPECIAL) {
+        xj[u] = xs[jj[u]];
+        if (st_pairs) {
+          nj[u] = F.normal[jj[u]];
+          rj[u] = S.rho[p][jj[u]];
+        }
+      }
+      const bool in = k0 + u < cnt;
+      // The scatter took pairs with a moved end, and pairs whose upper end's
+      // list is truncated.
+      lower[u] = st_pairs && in && !moved && idj[u] > id && idj[u] < fend && lastj[u] == 0xffffffffu;
+      upper[u] = st_pairs && in && !moved && !trunc_i && idj[u] < id && idj[u] >= F.start;
+    }
+    if (SPECIAL) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool pair = lower[u] || upper[u];
+        xj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rj[u] = 0.0f;
+        if (pair || vol[u] > 0.0f) xj[u] = xs[jj[u]];
+        if (pair) {
+          nj[u] = F.normal[jj[u]];
+          rj[u] = S.rho[p][jj[u]];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = k0 + u < cnt;
+      const float3 xjv = pf_xyz(xj[u]);
+      if (st_pairs) {
+        bool up = upper[u];
+        if (SPECIAL && up && lastj[u] != 0xffffffffu) {
+          // The pair exists in the reference iff this slot is in j's
+          // truncated list: compare this slot's walk key in j's walk with
+          // the key of j's last kept neighbour.
+          const int ddx = qx - pf_query_coord(xjv.x, S.ox, S.cell);
+          const int ddy = qy - pf_query_coord(xjv.y, S.oy, S.cell);
+          const int ddz = qz - pf_query_coord(xjv.z, S.oz, S.cell);
+          const uint32_t key = ((uint32_t)((ddz + 1) * 9 + (ddy + 1) * 3 + (ddx + 1)) << 27) | s;
+          up = key <= lastj[u];
+        }
+        const bool lo = lower[u];
+        if (lo || up) {
+          const float3 njv = pf_xyz(nj[u]);
+          float3 dv;
+          const bool ok = pair_dv(F, S.dt, sel3(lo, xi, xjv), sel3(lo, xjv, xi), sel3(lo, ni, njv), sel3(lo, njv, ni),
+                                  lo ? rhoi : rj[u], lo ? rj[u] : rhoi, &dv);
+          if (ok) {
+            const float sg = lo ? 1.0f : -1.0f;
+            tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x * sg, F.imp_scale));
+            tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y * sg, F.imp_scale));
+            tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z * sg, F.imp_scale));
+          }
+        }
+      }
+      if (adh_on && in) {
+        const float vj = vol[u];
+        if (vj > 0.0f) {
+          const float3 off = pf_sub3(xi, xjv);
+          const float r = pf_len3(off);
+          if (!(r < 1e-20f)) {
+            const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vj)) * adhesion_spline(F, r));
+            const float ir = 1.0f / r;
+            const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
+            tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
+            tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
+            tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
+          }
+        }
+      }
+    }
   }
 }
 
@@ -774,15 +909,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     return;
   }
   const float3 xi = pf_xyz(xi4);
-  const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
-  const uint32_t* __restrict__ nl = F.nl;
   int tk[3] = {0, 0, 0};
   float3 ni = make_float3(0.f, 0.f, 0.f);
   float rhoi = 0.0f;
   // A scene of a batch may lack an effect another scene has.
   const bool st_on = ST && (F.effects & PF_FX_SURFACE_TENSION);
-  const bool adh_on = ADH && (F.effects & PF_FX_ADHESION);
+  // Adhesion pulls only toward boundary particles (boundaryVolume > 0).
+  const bool adh_on = ADH && (F.effects & PF_FX_ADHESION) && S.has_boundary;
   if (!(st_on || adh_on)) {
     S.xs[c ^ 1][s] = xi4;
     return;
@@ -809,96 +943,15 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     moved_i = ct_scene && v.w != 0;
     trunc_i = trunc_scene && F.nl_last[s] != 0xffffffffu;
   }
-  const uint32_t fend = F.start + F.count;
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   if (S.H < 64u) {
     impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, trunc_scene, ct_scene, moved_i, trunc_i, tk);
+  } else if (trunc_scene || ct_scene) {
+    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, moved_i, moved_tile, trunc_i, U,
+                       uslots, tk);
   } else {
-    // Four neighbours at a time, every gather of the four issued before the
-    // arithmetic (the pass is latency-bound). Each pair is evaluated from its
-    // lower id's side: operands swapped and the change negated on the upper
-    // side. The fixed-point sums do not depend on the order.
-    const bool staged = U != PF_TU_GLOBAL;
-    const int qx = pf_query_coord(xi.x, S.ox, S.cell), qy = pf_query_coord(xi.y, S.oy, S.cell),
-              qz = pf_query_coord(xi.z, S.oz, S.cell);
-    Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
-    NlCursor cg(nl, n, s, cnt);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-      uint32_t jj[4];
-      if (staged) {
-        uint32_t idx[4];
-        c16.next(k0, idx);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
-      } else {
-        cg.next(k0, jj);
-      }
-      uint32_t idj[4], lastj[4];
-      float4 xj[4], nj[4];
-      float rj[4], vol[4];
-      bool moved[4] = {false, false, false, false};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        idj[u] = S.id[p][jj[u]];
-        xj[u] = xs[jj[u]];
-        if (st_pairs) {
-          nj[u] = F.normal[jj[u]];
-          rj[u] = S.rho[p][jj[u]];
-          lastj[u] = F.nl_last[jj[u]];
-          moved[u] = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
-        }
-        if (adh_on) vol[u] = S.bvol[p][jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool in = k0 + u < cnt;
-        const float3 xjv = pf_xyz(xj[u]);
-        if (st_pairs) {
-          // The scatter took pairs with a moved end, and pairs whose upper
-          // end's list is truncated.
-          const bool lower = in && !moved[u] && idj[u] > id && idj[u] < fend && lastj[u] == 0xffffffffu;
-          bool upper = in && !moved[u] && !trunc_i && idj[u] < id && idj[u] >= F.start;
-          if (upper && lastj[u] != 0xffffffffu) {
-            // The pair exists in the reference iff this slot is in j's
-            // truncated list: compare this slot's walk key in j's walk with
-            // the key of j's last kept neighbour.
-            const int ddx = qx - pf_query_coord(xjv.x, S.ox, S.cell);
-            const int ddy = qy - pf_query_coord(xjv.y, S.oy, S.cell);
-            const int ddz = qz - pf_query_coord(xjv.z, S.oz, S.cell);
-            const uint32_t key = ((uint32_t)((ddz + 1) * 9 + (ddy + 1) * 3 + (ddx + 1)) << 27) | s;
-            upper = key <= lastj[u];
-          }
-          if (lower || upper) {
-            const float3 njv = pf_xyz(nj[u]);
-            float3 dv;
-            const bool ok = pair_dv(F, S.dt, sel3(lower, xi, xjv), sel3(lower, xjv, xi), sel3(lower, ni, njv),
-                                    sel3(lower, njv, ni), lower ? rhoi : rj[u], lower ? rj[u] : rhoi, &dv);
-            if (ok) {
-              const float sg = lower ? 1.0f : -1.0f;
-              tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x * sg, F.imp_scale));
-              tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y * sg, F.imp_scale));
-              tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z * sg, F.imp_scale));
-            }
-          }
-        }
-        if (adh_on && in) {
-          const float vj = vol[u];
-          if (vj > 0.0f) {
-            const float3 off = pf_sub3(xi, xjv);
-            const float r = pf_len3(off);
-            if (!(r < 1e-20f)) {
-              const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vj)) * adhesion_spline(F, r));
-              const float ir = 1.0f / r;
-              const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
-              tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
-              tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
-              tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
-            }
-          }
-        }
-      }
-    }
+    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, false, false, false, U, uslots, tk);
   }
   bool sat;
   const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);
