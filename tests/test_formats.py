@@ -131,3 +131,32 @@ def test_seeded_random_is_mulberry32():
     assert abs(np.mean(draws) - 0.5) < 0.03
     a, b = formats.seeded_random(1), formats.seeded_random(1)
     assert [a() for _ in range(5)] == [b() for _ in range(5)]
+
+
+HONEY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "honey", "bunny.sdf.bin")
+
+
+def test_reference_bunny_psdf_decodes_to_a_distance_field():
+    """The reference's own bunny.sdf.bin: the codec reads it and writes the
+    same bytes back, and what it reads is a signed distance field (|grad phi|
+    = 1 away from the surface, positive on the grid's boundary, one connected
+    negative interior), which a wrong header size, offset or byte order would
+    break."""
+    raw = open(HONEY, "rb").read()
+    sdf = formats.decodeSdfBinary(raw)
+    nx, ny, nz = sdf["resolution"]
+    assert len(raw) == 48 + 4 * nx * ny * nz
+    assert formats.encodeSdfBinary(sdf) == raw
+    phi = sdf["data"].reshape(nz, ny, nx).astype(np.float64)
+    vx, vy, vz = sdf["voxelSize"]
+    gz, gy, gx = np.gradient(phi, vz, vy, vx)
+    g = np.sqrt(gx * gx + gy * gy + gz * gz)
+    band = (np.abs(phi) > 2 * max(vx, vy, vz)) & (np.abs(phi) < 8 * max(vx, vy, vz))
+    assert band.sum() > 1000
+    assert np.median(np.abs(g[band] - 1.0)) < 0.05
+    inside = phi < 0
+    assert inside.sum() > 100 and phi[0].min() > 0 and phi[-1].min() > 0
+    assert phi[:, 0].min() > 0 and phi[:, -1].min() > 0 and phi[:, :, 0].min() > 0 and phi[:, :, -1].min() > 0
+    from scipy import ndimage
+    _, parts = ndimage.label(inside)
+    assert parts == 1
