@@ -147,6 +147,120 @@ def run_batched(args, ctx):
     return out
 
 
+# Algorithmic bytes of one surface-field update: per voxel the clear (20),
+# the walls (8) and the three blur passes (40, 40, 44: int or float4
+# accumulators plus the solid channel in and out; the last reads the walls and
+# writes the float4 field); per particle the splat's reads (x, v, id: 36).
+FIELD_BYTES_PER_VOXEL = 152
+FIELD_BYTES_PER_PARTICLE = 36
+# Viscosity solve per fluid particle: seed and apply (48 each) and 48 per
+# Jacobi sweep (read v0 and the previous iterate's own value, write the next;
+# neighbour gathers count 0).
+VISCOSITY_BYTES = lambda sweeps: 48 * (sweeps + 2)  # noqa: E731
+
+
+def kernel_table(stats):
+    total = sum(t for _, t in stats.values()) or 1.0
+    return {k: {"launches": v[0], "avg_us": round(v[1] / v[0] * 1000.0, 3), "share": round(v[1] / total, 4)}
+            for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+
+
+def run_surface_field(args, ctx, sc):
+    """SurfaceField over the stepped 1M dam break: the renderer's per-frame
+    update at its largest default grid (defaultVoxelBudget 1.2 M voxels for
+    50,000 particles or more), wetted container walls, motionStretch 0.02."""
+    import particle_fluids as pf
+    from particle_fluids import scenes
+
+    mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
+    field = pf.SurfaceField(sc["fluid"], (mn, mx), motionStretch=0.02, wet=True)
+    nx, ny, nz = field.resolution
+    voxels = nx * ny * nz
+    n = sc["ps"].capacity
+    for _ in range(3):
+        field.update()
+    ctx.synchronize()
+    ctx.timer_begin()
+    for _ in range(args.field_updates):
+        field.update()
+    ms = ctx.timer_end() / args.field_updates
+    ctx.set_profiling(True)
+    ctx.profile_reset()
+    for _ in range(5):
+        field.update()
+    stats = ctx.profile()
+    ctx.set_profiling(False)
+    ctx.profile_reset()
+    field.dispose()
+    bytes_ = FIELD_BYTES_PER_VOXEL * voxels + FIELD_BYTES_PER_PARTICLE * n
+    achieved = bytes_ / (ms / 1000.0) / 1e9
+    out = {"workload": f"SurfaceField update over the {n}-particle dam break: {nx}x{ny}x{nz} voxels "
+                       f"(defaultVoxelBudget), motionStretch 0.02, container walls wetted",
+           "voxels": voxels, "particles": n, "updates": args.field_updates, "ms_per_update": round(ms, 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_update": bytes_},
+           "kernels": kernel_table(stats)}
+    print(f"[bench] surface field: {out['ms_per_update']} ms per update, {voxels} voxels", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def run_viscosity(args, ctx, dt):
+    """The 1M dam break with a ViscositySolver after the fluid (a thick
+    liquid, Simulation.addFluid({thickness}) at honey's 20 m^2/s, 12 Jacobi
+    sweeps per substep), as a HIP graph."""
+    import particle_fluids as pf
+    from particle_fluids import scenes
+
+    pos, r = scenes.dam_break_layout(args.budget)
+    pos = scenes.jitter(pos, 2 * r)
+    n = len(pos)
+    ps = pf.ParticleSystem(n, r)
+    ps.uploadParticles(pos)
+    fluid = pf.FluidSystem(ps, **scenes.FLUID_DEFAULTS)
+    solver = pf.ViscositySolver(fluid, 20.0, iterations=args.viscosity_sweeps)
+    walls = pf.PrimitiveSet(ps)
+    mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
+    for nrm, which in scenes.PLANES:
+        walls.addPlane(nrm, mn if which == "min" else mx)
+    loop = pf.SimLoop(ps, substeps=args.substeps, iterations=args.iterations, materials=[fluid, solver],
+                      colliders=[walls])
+    loop.useGraph(True)
+    for _ in range(args.warmup):
+        loop.step(dt)
+    ctx.synchronize()
+    ctx.timer_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.viscosity_steps):
+        loop.step(dt)
+    gpu_ms = ctx.timer_end()
+    seconds = max(time.perf_counter() - t0, gpu_ms / 1000.0)
+    loop.useGraph(False)
+    ctx.set_profiling(True)
+    ctx.profile_reset()
+    for _ in range(3):
+        loop.step(dt)
+    stats = ctx.profile()
+    ctx.set_profiling(False)
+    ctx.profile_reset()
+    for obj in (loop, solver, fluid, walls, ps):
+        obj.dispose()
+    launches, total = stats.get("viscosity_solve", (0, 0.0))
+    out = {"workload": f"dam break, {n} particles, ViscositySolver 20 m^2/s with {args.viscosity_sweeps} sweeps, "
+                       f"{args.substeps} substeps x {args.iterations} iterations, dt 1/60",
+           "particles": n, "steps": args.viscosity_steps,
+           "particle_substeps_per_s": round(n * args.substeps * args.viscosity_steps / seconds, 1),
+           "ms_per_step": round(seconds / args.viscosity_steps * 1000.0, 4), "kernels": kernel_table(stats)}
+    if launches:
+        avg_s = total / launches / 1000.0
+        b = VISCOSITY_BYTES(args.viscosity_sweeps) * n
+        out["roofline"] = {"bound": "hbm", "kernel": "viscosity_solve", "achieved": round(b / avg_s / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                           "bytes_per_launch": b, "avg_launch_us": round(avg_s * 1e6, 3)}
+    print(f"[bench] viscosity: {out['particle_substeps_per_s']:.4e} particle-substeps/s", file=sys.stderr, flush=True)
+    return out
+
+
 def process_mappings():
     """Memory mappings this process holds (each device chunk of an evictable
     scene is one or more), or -1 where /proc is unavailable."""
@@ -314,6 +428,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-seed", type=int, default=0x5E7E)
+    ap.add_argument("--field-updates", type=int, default=20, help="surface-field updates timed (0: skip)")
+    ap.add_argument("--viscosity-steps", type=int, default=20, help="thick-liquid steps timed (0: skip)")
+    ap.add_argument("--viscosity-sweeps", type=int, default=12)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="end with a device copy of this size (a known-traffic kernel for HBM counters)")
     args = ap.parse_args()
@@ -417,6 +534,9 @@ def main():
                "sample": f"{args.cpu_baseline_steps} step(s) = {args.substeps * args.cpu_baseline_steps} substeps of the "
                          f"same {n}-particle scene, FP32 C oracle, 1 thread, {csec:.1f} s on {platform.processor() or 'host'}"}
 
+    field = run_surface_field(args, ctx, sc) if args.field_updates > 0 else None
+    viscosity = run_viscosity(args, ctx, dt) if args.viscosity_steps > 0 else None
+
     batched = None
     if args.batch_scenes > 0:
         batched = run_batched(args, ctx)
@@ -446,6 +566,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "gpu_ms_timed": round(gpu_ms, 3),
+        "surface_field": field,
+        "viscosity": viscosity,
         "batched": batched,
         "served": served,
     }

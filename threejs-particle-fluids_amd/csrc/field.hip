@@ -59,57 +59,119 @@ __global__ __launch_bounds__(PF_BLOCK) void k_field_clear(FieldDev F) {
   F.solid[i] = 0;
 }
 
+// The workgroup's voxel sums: an LDS hash of voxel index -> integer channels
+// (mass, three offsets, solid mass). Neighbouring slots are neighbouring
+// particles, so a workgroup's 256 particles touch a few hundred voxels; summing
+// them in LDS leaves one global atomic per voxel and channel instead of one per
+// particle corner. A voxel that finds no free slot within a few probes adds
+// straight to global memory. Wrap-around integer sums do not depend on order,
+// so the result is the same either way.
+constexpr uint32_t kSplatHash = 1024;
+constexpr uint32_t kSplatProbes = 16;
+
+struct SplatLds {
+  uint32_t key[kSplatHash];  // voxel + 1, 0 = empty
+  int ch[5][kSplatHash];
+};
+
+__device__ __forceinline__ void splat_add(SplatLds& L, const FieldDev& F, uint32_t c, bool fluid, const int v[4]) {
+  uint32_t h = (c * 0x9E3779B1u) >> 22;
+  for (uint32_t probe = 0; probe < kSplatProbes; ++probe) {
+    const uint32_t prev = atomicCAS(&L.key[h], 0u, c + 1u);
+    if (prev == 0u || prev == c + 1u) {
+      if (fluid) {
+        atomicAdd(&L.ch[0][h], v[0]);
+        atomicAdd(&L.ch[1][h], v[1]);
+        atomicAdd(&L.ch[2][h], v[2]);
+        atomicAdd(&L.ch[3][h], v[3]);
+      } else {
+        atomicAdd(&L.ch[4][h], v[0]);
+      }
+      return;
+    }
+    h = (h + 1u) & (kSplatHash - 1u);
+  }
+  if (fluid) {
+    atomicAdd(F.mass + c, v[0]);
+    atomicAdd(F.off + 3 * (size_t)c + 0, v[1]);
+    atomicAdd(F.off + 3 * (size_t)c + 1, v[2]);
+    atomicAdd(F.off + 3 * (size_t)c + 2, v[3]);
+  } else {
+    atomicAdd(F.solid + c, v[0]);
+  }
+}
+
 // splat (SurfaceField.ts:174-214), one thread per particle slot.
 __global__ __launch_bounds__(PF_BLOCK) void k_field_splat(const SceneDev* __restrict__ sc, FieldDev F) {
+  __shared__ SplatLds L;
+  for (uint32_t i = threadIdx.x; i < kSplatHash; i += PF_BLOCK) {
+    L.key[i] = 0u;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) L.ch[q][i] = 0;
+  }
+  __syncthreads();
   const SceneDev& S = *sc;
   const uint32_t s = blockIdx.x * PF_BLOCK + threadIdx.x;
-  if (s >= S.n) return;
-  const uint32_t id = S.id[0][s];
-  const bool fluid = fd_in(id, F.fluid_start, F.fluid_count);
-  if (!fluid && !fd_in(id, F.solid_start, F.solid_count)) return;
-  const float3 pos = pf_xyz(S.x[0][s]);
-  const uint32_t samples = fluid ? F.samples : 1u;
-  float3 trail = make_float3(0.f, 0.f, 0.f);
-  if (samples > 1) {
-    const float3 v = pf_xyz(S.v[0][s]);
-    const float3 vv = make_float3(v.x * F.stretch, v.y * F.stretch, v.z * F.stretch);
-    const float k = F.reach3 / fmaxf(pf_len3(vv), F.reach3);
-    trail = make_float3(vv.x * k, vv.y * k, vv.z * k);
-  }
-  for (uint32_t sm = 0; sm < samples; ++sm) {
-    const float frac = sm == 0 ? 0.0f : (float)sm / (float)(samples - 1);
-    const float3 at = sm == 0 ? pos : make_float3(pos.x - trail.x * frac, pos.y - trail.y * frac, pos.z - trail.z * frac);
-    const float g[3] = {(at.x - F.origin[0]) * F.inv_voxel - 0.5f, (at.y - F.origin[1]) * F.inv_voxel - 0.5f,
-                        (at.z - F.origin[2]) * F.inv_voxel - 0.5f};
-    bool inside = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) inside = inside && g[a] >= 0.0f && g[a] < F.dims[a] - 1.0f;
-    if (!inside) continue;
-    float b[3], f[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      b[a] = floorf(g[a]);
-      f[a] = g[a] - b[a];
+  const uint32_t id = s < S.n ? S.id[0][s] : 0xffffffffu;
+  const bool fluid = s < S.n && fd_in(id, F.fluid_start, F.fluid_count);
+  const bool solid = s < S.n && !fluid && fd_in(id, F.solid_start, F.solid_count);
+  if (fluid || solid) {
+    const float3 pos = pf_xyz(S.x[0][s]);
+    const uint32_t samples = fluid ? F.samples : 1u;
+    float3 trail = make_float3(0.f, 0.f, 0.f);
+    if (samples > 1) {
+      const float3 v = pf_xyz(S.v[0][s]);
+      const float3 vv = make_float3(v.x * F.stretch, v.y * F.stretch, v.z * F.stretch);
+      const float k = F.reach3 / fmaxf(pf_len3(vv), F.reach3);
+      trail = make_float3(vv.x * k, vv.y * k, vv.z * k);
     }
+    for (uint32_t sm = 0; sm < samples; ++sm) {
+      const float frac = sm == 0 ? 0.0f : (float)sm / (float)(samples - 1);
+      const float3 at =
+          sm == 0 ? pos : make_float3(pos.x - trail.x * frac, pos.y - trail.y * frac, pos.z - trail.z * frac);
+      const float g[3] = {(at.x - F.origin[0]) * F.inv_voxel - 0.5f, (at.y - F.origin[1]) * F.inv_voxel - 0.5f,
+                          (at.z - F.origin[2]) * F.inv_voxel - 0.5f};
+      bool inside = true;
 #pragma unroll
-    for (int z = 0; z < 2; ++z)
+      for (int a = 0; a < 3; ++a) inside = inside && g[a] >= 0.0f && g[a] < F.dims[a] - 1.0f;
+      if (!inside) continue;
+      float b[3], f[3];
 #pragma unroll
-      for (int y = 0; y < 2; ++y)
+      for (int a = 0; a < 3; ++a) {
+        b[a] = floorf(g[a]);
+        f[a] = g[a] - b[a];
+      }
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-          const float w = (((x ? f[0] : 1.0f - f[0]) * (y ? f[1] : 1.0f - f[1])) * (z ? f[2] : 1.0f - f[2])) *
-                          (fluid ? F.inv_samples : 1.0f);
-          // indexOf over float cell coordinates (SurfaceField.ts:147-152)
-          const uint32_t c = (uint32_t)(((b[2] + (float)z) * F.nxny + (b[1] + (float)y) * F.nx) + (b[0] + (float)x));
-          if (fluid) {
-            atomicAdd(F.mass + c, pf_f2i_sat(rintf(w * kFixed)));
-            atomicAdd(F.off + 3 * (size_t)c + 0, pf_f2i_sat(rintf((w * (f[0] - (float)x)) * kFixed)));
-            atomicAdd(F.off + 3 * (size_t)c + 1, pf_f2i_sat(rintf((w * (f[1] - (float)y)) * kFixed)));
-            atomicAdd(F.off + 3 * (size_t)c + 2, pf_f2i_sat(rintf((w * (f[2] - (float)z)) * kFixed)));
-          } else {
-            atomicAdd(F.solid + c, pf_f2i_sat(rintf(w * kFixed)));
+      for (int z = 0; z < 2; ++z)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int x = 0; x < 2; ++x) {
+            const float w = (((x ? f[0] : 1.0f - f[0]) * (y ? f[1] : 1.0f - f[1])) * (z ? f[2] : 1.0f - f[2])) *
+                            (fluid ? F.inv_samples : 1.0f);
+            // indexOf over float cell coordinates (SurfaceField.ts:147-152)
+            const uint32_t c =
+                (uint32_t)(((b[2] + (float)z) * F.nxny + (b[1] + (float)y) * F.nx) + (b[0] + (float)x));
+            int v[4] = {pf_f2i_sat(rintf(w * kFixed)), 0, 0, 0};
+            if (fluid) {
+              v[1] = pf_f2i_sat(rintf((w * (f[0] - (float)x)) * kFixed));
+              v[2] = pf_f2i_sat(rintf((w * (f[1] - (float)y)) * kFixed));
+              v[3] = pf_f2i_sat(rintf((w * (f[2] - (float)z)) * kFixed));
+            }
+            splat_add(L, F, c, fluid, v);
           }
-        }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kSplatHash; i += PF_BLOCK) {
+    const uint32_t key = L.key[i];
+    if (!key) continue;
+    const uint32_t c = key - 1u;
+    if (L.ch[0][i]) atomicAdd(F.mass + c, L.ch[0][i]);
+    if (L.ch[1][i]) atomicAdd(F.off + 3 * (size_t)c + 0, L.ch[1][i]);
+    if (L.ch[2][i]) atomicAdd(F.off + 3 * (size_t)c + 1, L.ch[2][i]);
+    if (L.ch[3][i]) atomicAdd(F.off + 3 * (size_t)c + 2, L.ch[3][i]);
+    if (L.ch[4][i]) atomicAdd(F.solid + c, L.ch[4][i]);
   }
 }
 
@@ -337,12 +399,25 @@ int pf_surface_field_update(pf_surface_field* f) {
   hipStream_t st = ps->ctx->stream;
   const FieldDev& D = f->dev;
   const dim3 vox((D.count + PF_BLOCK - 1) / PF_BLOCK), parts((ps->n + PF_BLOCK - 1) / PF_BLOCK), blk(PF_BLOCK);
-  hipLaunchKernelGGL(k_field_clear, vox, blk, 0, st, D);
-  hipLaunchKernelGGL(k_field_splat, parts, blk, 0, st, ps->cur(), D);
-  hipLaunchKernelGGL(k_field_walls, vox, blk, 0, st, ps->cur(), D);
-  hipLaunchKernelGGL(k_field_blur<0>, vox, blk, 0, st, D);
-  hipLaunchKernelGGL(k_field_blur<1>, vox, blk, 0, st, D);
-  hipLaunchKernelGGL(k_field_blur<2>, vox, blk, 0, st, D);
+  Context* ctx = ps->ctx;
+  {
+    PF_LAUNCH(ctx, "field_clear");
+    hipLaunchKernelGGL(k_field_clear, vox, blk, 0, st, D);
+  }
+  {
+    PF_LAUNCH(ctx, "field_splat");
+    hipLaunchKernelGGL(k_field_splat, parts, blk, 0, st, ps->cur(), D);
+  }
+  {
+    PF_LAUNCH(ctx, "field_walls");
+    hipLaunchKernelGGL(k_field_walls, vox, blk, 0, st, ps->cur(), D);
+  }
+  {
+    PF_LAUNCH(ctx, "field_blur");
+    hipLaunchKernelGGL(k_field_blur<0>, vox, blk, 0, st, D);
+    hipLaunchKernelGGL(k_field_blur<1>, vox, blk, 0, st, D);
+    hipLaunchKernelGGL(k_field_blur<2>, vox, blk, 0, st, D);
+  }
   PF_HIP(hipGetLastError());
   PF_API_END
 }
