@@ -297,13 +297,18 @@ struct pf_surface_field_s {
 static void field_alive(pf_surface_field* f) {
   if (!f || f->disposed) fail(PF_ESTATE, "FluidSurfaceRenderer: the renderer has been disposed");
   if (f->fluid->disposed || f->ps->disposed) fail(PF_ESTATE, "FluidSurfaceRenderer: the fluid has been disposed");
+  // A served scene's buffers belong to the server (and may be evicted).
+  if (f->ps->server) fail(PF_ESTATE, "FluidSurfaceRenderer: the scene is being served; remove it from the server first");
 }
 
 extern "C" {
 
 int pf_surface_field_create(pf_fluid* fluid, const pf_surface_field_options* o, pf_surface_field** out) {
   PF_API_BEGIN
-  if (!fluid || fluid->disposed) fail(PF_EINVAL, "FluidSurfaceRenderer: the fluid has been disposed");
+  if (!fluid || fluid->disposed || fluid->ps->disposed)
+    fail(PF_EINVAL, "FluidSurfaceRenderer: the fluid has been disposed");
+  if (fluid->ps->server)
+    fail(PF_ESTATE, "FluidSurfaceRenderer: the scene is being served; remove it from the server first");
   if (!o || !out) fail(PF_EINVAL, "FluidSurfaceRenderer: null argument");
   const double stretch = o->motion_stretch;
   if (!(stretch >= 0) || !std::isfinite(stretch))
