@@ -115,6 +115,24 @@ static __device__ __forceinline__ float3 sel3(bool ok, float3 a, float3 b) {
   return make_float3(ok ? a.x : b.x, ok ? a.y : b.y, ok ? a.z : b.z);
 }
 
+// A walk's query cell, the low 10 bits of each axis.
+static __device__ __forceinline__ uint32_t pack_cell(const int q[3]) {
+  return ((uint32_t)q[0] & 1023u) | (((uint32_t)q[1] & 1023u) << 10) | (((uint32_t)q[2] & 1023u) << 20);
+}
+
+// Walk key (walk order: stencil dz -> dy -> dx, then slot) of the particle in
+// slot `found`, whose list-build cell is `cf`, in the walk from cell `cw`.
+// Tables of 64 buckets or more give the 27 stencil cells distinct buckets, so
+// a particle within h is found at exactly one stencil position, its cell
+// minus the walk's; that difference is -1, 0 or 1 per axis and survives the
+// 10-bit wrap.
+static __device__ __forceinline__ uint32_t walk_key(uint32_t cf, uint32_t cw, uint32_t found) {
+  const uint32_t dx = (cf - cw + 1u) & 1023u;
+  const uint32_t dy = ((cf >> 10) - (cw >> 10) + 1u) & 1023u;
+  const uint32_t dz = ((cf >> 20) - (cw >> 20) + 1u) & 1023u;
+  return ((dz * 9u + dy * 3u + dx) << 27) | found;
+}
+
 // Colour-field normal n_i = h * sum_j V_j grad W (cohesion.ts:25-39) over a
 // finished global-slot list, in list order.
 static __device__ __forceinline__ float4 color_normal_global(const SceneDev& S, const FluidDev& F, int p,
@@ -224,6 +242,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   bool over = false;
 
   // Tables below 64 buckets: global walk with the reference's bucket dedup.
+  // Such tables belong to scenes of at most 16 particles (H = nextPow2(2N)),
+  // whose lists cannot reach the 64-entry cap, so no walk key is kept.
   if (S.H < 64u) {
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
     if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
@@ -376,9 +396,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       }
     }
     F.nl_count[s] = count;
-    // Walk position of the last kept neighbour of a truncated list; the
-    // surface-tension pass uses it to know which partners this list dropped.
+    // Walk position of the last kept neighbour of a truncated list, and the
+    // walk's cell: the surface-tension passes use them to know which
+    // partners a list dropped.
     F.nl_last[s] = over ? last_key : 0xffffffffu;
+    F.nl_cell[s] = pack_cell(q);
     if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     return;
@@ -506,6 +528,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     if (count & 1u) *row = pend;
     F.nl_count[s] = count;
     F.nl_last[s] = over ? last_key : 0xffffffffu;
+    F.nl_cell[s] = pack_cell(q);
     if (want_normal) {
       // Colour-field normal over the finished list, in list order.
       const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
@@ -628,34 +651,31 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
 }
 
 // Surface tension and adhesion of one particle in a table below 64 buckets
-// (global slots, one neighbour at a time).
+// (global slots, one neighbour at a time). Such scenes hold at most 16
+// particles, so no list is truncated and both ends list every pair; pairs
+// with a moved end were scattered (k_cohesion_scatter).
 static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
                                      float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
-                                     bool trunc_scene, bool ct_scene, bool moved_i, bool trunc_i, int tk[3]) {
+                                     bool ct_scene, bool moved_i, int tk[3]) {
   const float4* __restrict__ xs = S.xs[c];
   const uint32_t n = S.n;
   const uint32_t* __restrict__ nl = F.nl;
   const uint32_t fend = F.start + F.count;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = nl[(size_t)k * n + s];
-    // pairs with a moved end, or whose upper end's list is truncated, were
-    // scattered (k_cohesion_scatter)
     const bool moved = moved_i || (ct_scene && S.ct.acc[j].w != 0);
     if (st_on && !moved) {
       const uint32_t idj = S.id[p][j];
-      if (idj > id && idj < fend && !(trunc_scene && F.nl_last[j] != 0xffffffffu)) {
+      if (idj > id && idj < fend) {
         float3 dv;
         if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) {
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
         }
-      } else if (idj < id && idj >= F.start && !trunc_i) {
-        // this list is whole: the pair exists iff j's (maybe truncated) list holds this slot
-        bool listed = F.nl_last[j] == 0xffffffffu;
-        for (uint32_t m = 0; m < PF_MAX_NEIGHBORS && !listed; ++m) listed |= (nl[(size_t)m * n + j] == s);
+      } else if (idj < id && idj >= F.start) {
         float3 dv;
-        if (listed && pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
+        if (pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
           tk[0] = pf_wrap_add(tk[0], pf_ticks(-dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(-dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(-dv.z, F.imp_scale));
@@ -682,15 +702,15 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
 
 // Surface tension the way the reference scatters it (cohesion.ts:60-100):
 // each pair from the list of its lower id, the fixed-point change added to
-// both ends with integer atomics. The gathered form in k_impulses evaluates
-// a pair from either end: the lower end from its own list, the upper end
-// from its own list too, checking a truncated partner's list by walk key.
-// That misses pairs in two cases, and this pass takes exactly those:
-//  - the upper end's list was truncated at 64 (neighborList.ts:63-66): a pair
-//    cut from it still exists when the lower end's list kept it;
+// both ends. The gathered form in k_impulses evaluates a pair from either
+// end: the lower end always (the pair is in its list), the upper end from its
+// own list, checking a truncated lower partner's list by walk key. That misses
+// two kinds of pairs, and this pass takes exactly those:
+//  - the upper end's list was truncated at 64 (neighborList.ts:63-66) and
+//    dropped the lower end, which kept it: the upper end's half is added here;
 //  - in scenes with contacts, an end was moved by the stabilisation, whose
 //    query point is the stabilised prediction and candidates the sorted copy,
-//    so the two lists can disagree about the pair.
+//    so the two lists can disagree about the pair: both halves are added here.
 // A tile runs it only when its union holds such a particle. The sums go to
 // the scatter accumulator (idle between the stabilisation and the first
 // contact solve); k_impulses adds and clears them.
@@ -729,11 +749,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
   const float3 ni = pf_xyz(F.normal[s]);
   const float rhoi = S.rho[p][s];
   const bool moved_i = S.ct.on && S.ct.acc[s].w != 0;
+  const uint32_t ci = F.nl_cell[s];
   const uint32_t n = S.n;
   const uint32_t cnt = F.nl_count[s];
   const uint32_t fend = F.start + F.count;
-  // This particle's half of its pairs is summed here and added once; wrap
-  // sums do not depend on the order.
+  // This particle's half of its moved pairs is summed here and added once;
+  // wrap sums do not depend on the order.
   int own[3] = {0, 0, 0};
   bool any = false;
   for (uint32_t k = 0; k < cnt; ++k) {
@@ -746,15 +767,21 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
     }
     const uint32_t idj = S.id[p][j];
     if (!(idj > id && idj < fend)) continue;
-    // Other pairs are the gather's.
-    if (!moved_i && !(trunc_scene && nl_truncated(F, j)) && !(S.ct.on && S.ct.acc[j].w != 0)) continue;
+    const bool moved = moved_i || (S.ct.on && S.ct.acc[j].w != 0);
+    if (!moved) {
+      // Unmoved pairs are the gather's unless j's truncated list dropped this slot.
+      if (!trunc_scene) continue;
+      const uint32_t lastj = F.nl_last[j];
+      if (lastj == 0xffffffffu || walk_key(ci, F.nl_cell[j], s) <= lastj) continue;
+    }
     float3 dv;
     if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
-    const int t[3] = {pf_ticks(dv.x, F.imp_scale), pf_ticks(dv.y, F.imp_scale), pf_ticks(dv.z, F.imp_scale)};
-    own[0] = pf_wrap_add(own[0], t[0]);
-    own[1] = pf_wrap_add(own[1], t[1]);
-    own[2] = pf_wrap_add(own[2], t[2]);
-    any = true;
+    if (moved) {
+      own[0] = pf_wrap_add(own[0], pf_ticks(dv.x, F.imp_scale));
+      own[1] = pf_wrap_add(own[1], pf_ticks(dv.y, F.imp_scale));
+      own[2] = pf_wrap_add(own[2], pf_ticks(dv.z, F.imp_scale));
+      any = true;
+    }
     // The partner's change is the exact negation (round is symmetric).
     int* aj = reinterpret_cast<int*>(S.ct.acc + j);
     atomicAdd(aj + 0, pf_ticks(-dv.x, F.imp_scale));
@@ -775,21 +802,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* _
 // latency-bound). Each pair is evaluated from its lower id's side: operands
 // swapped and the change negated on the upper side. The fixed-point sums do
 // not depend on the order. SPECIAL: the scene cut a list at 64 or has
-// contacts, so the walk keys and moved marks decide which pairs are the
-// scatter's; the partner's id, key and mark are read first and its position,
-// normal and density only for the pairs evaluated here. Otherwise every
-// fluid partner is evaluated here and keys and marks are not read.
+// contacts. Pairs with a moved end are then the scatter's, and an upper-end
+// pair exists only if the lower partner's list holds this slot, which a
+// truncated partner's walk key decides; the partner's id, key and mark are
+// read first and its position, normal and density only for the pairs
+// evaluated here. Otherwise every fluid partner is evaluated here and keys
+// and marks are not read.
 template <bool SPECIAL>
 static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const FluidDev& F, int p,
                                                     const float4* __restrict__ xs, uint32_t s, uint32_t id,
                                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
-                                                    bool adh_on, bool moved_i, bool moved_tile, bool trunc_i,
-                                                    uint32_t U, const uint32_t* __restrict__ uslots, int tk[3]) {
+                                                    bool adh_on, bool moved_i, bool moved_tile, uint32_t U,
+                                                    const uint32_t* __restrict__ uslots, int tk[3]) {
   const uint32_t n = S.n;
   const uint32_t fend = F.start + F.count;
   const bool staged = U != PF_TU_GLOBAL;
-  const int qx = pf_query_coord(xi.x, S.ox, S.cell), qy = pf_query_coord(xi.y, S.oy, S.cell),
-            qz = pf_query_coord(xi.z, S.oz, S.cell);
+  const uint32_t ci = SPECIAL ? F.nl_cell[s] : 0u;
   Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
   NlCursor cg(F.nl, n, s, cnt);
   for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
@@ -802,7 +830,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
     } else {
       cg.next(k0, jj);
     }
-    uint32_t idj[4], lastj[4];
+    uint32_t idj[4], lastj[4], cj[4];
     float4 xj[4], nj[4];
     float rj[4], vol[4];
     bool lower[4], upper[4];
@@ -824,10 +852,9 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         }
       }
       const bool in = k0 + u < cnt;
-      // The scatter took pairs with a moved end, and pairs whose upper end's
-      // list is truncated.
-      lower[u] = st_pairs && in && !moved && idj[u] > id && idj[u] < fend && lastj[u] == 0xffffffffu;
-      upper[u] = st_pairs && in && !moved && !trunc_i && idj[u] < id && idj[u] >= F.start;
+      // The scatter took pairs with a moved end.
+      lower[u] = st_pairs && in && !moved && idj[u] > id && idj[u] < fend;
+      upper[u] = st_pairs && in && !moved && idj[u] < id && idj[u] >= F.start;
     }
     if (SPECIAL) {
 #pragma unroll
@@ -836,6 +863,8 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         xj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         rj[u] = 0.0f;
+        cj[u] = 0u;
+        if (upper[u] && lastj[u] != 0xffffffffu) cj[u] = F.nl_cell[jj[u]];
         if (pair || vol[u] > 0.0f) xj[u] = xs[jj[u]];
         if (pair) {
           nj[u] = F.normal[jj[u]];
@@ -851,13 +880,9 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         bool up = upper[u];
         if (SPECIAL && up && lastj[u] != 0xffffffffu) {
           // The pair exists in the reference iff this slot is in j's
-          // truncated list: compare this slot's walk key in j's walk with
-          // the key of j's last kept neighbour.
-          const int ddx = qx - pf_query_coord(xjv.x, S.ox, S.cell);
-          const int ddy = qy - pf_query_coord(xjv.y, S.oy, S.cell);
-          const int ddz = qz - pf_query_coord(xjv.z, S.oz, S.cell);
-          const uint32_t key = ((uint32_t)((ddz + 1) * 9 + (ddy + 1) * 3 + (ddx + 1)) << 27) | s;
-          up = key <= lastj[u];
+          // truncated list: compare this slot's walk key in j's walk (from
+          // both list-build cells) with the key of j's last kept neighbour.
+          up = walk_key(ci, cj[u], s) <= lastj[u];
         }
         const bool lo = lower[u];
         if (lo || up) {
@@ -925,15 +950,16 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
-  // Pairs with a truncated list at either end, and in scenes with contacts
-  // (CT: some scene of the launch has them) pairs with a particle the
-  // stabilisation moved, were scattered already (k_cohesion_scatter) into
-  // the accumulator; the gather adds those sums and skips such pairs. The
-  // moved marks (.w) stay for the other threads of this pass.
+  // The halves of pairs an upper end's truncated list dropped, and in scenes
+  // with contacts (CT: some scene of the launch has them) pairs with a
+  // particle the stabilisation moved, were scattered already
+  // (k_cohesion_scatter) into the accumulator; the gather adds those sums
+  // and skips such pairs. The moved marks (.w) stay for the other threads of
+  // this pass.
   const bool trunc_scene = st_on && S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
   const bool ct_scene = CT && st_on && S.ct.on;
   const bool st_pairs = st_on;
-  bool moved_tile = false, moved_i = false, trunc_i = false;
+  bool moved_tile = false, moved_i = false;
   if (trunc_scene || ct_scene) {
     int* a = reinterpret_cast<int*>(S.ct.acc + s);
     const int4 v = S.ct.acc[s];
@@ -941,17 +967,15 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     tk[0] = v.x; tk[1] = v.y; tk[2] = v.z;
     moved_tile = ct_scene && F.tu_moved[t.first / PF_BLOCK] != 0u;
     moved_i = ct_scene && v.w != 0;
-    trunc_i = trunc_scene && F.nl_last[s] != 0xffffffffu;
   }
   const uint32_t U = F.tu_size[t.first / PF_BLOCK];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, trunc_scene, ct_scene, moved_i, trunc_i, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, ct_scene, moved_i, tk);
   } else if (trunc_scene || ct_scene) {
-    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, moved_i, moved_tile, trunc_i, U,
-                       uslots, tk);
+    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, moved_i, moved_tile, U, uslots, tk);
   } else {
-    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, false, false, false, U, uslots, tk);
+    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, false, false, U, uslots, tk);
   }
   bool sat;
   const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);

@@ -83,6 +83,7 @@ struct FluidDev {
                        // contact stabilisation moved this substep (1 for global-walk tiles)
   uint32_t* nl_count;  // [n]
   uint32_t* nl_last;   // [n] walk key of the last kept neighbour when the list overflowed, else ~0
+  uint32_t* nl_cell;   // [n] the walk's query cell, low 10 bits per axis (x | y << 10 | z << 20)
   float4* normal;      // [n] colour-field normal
   float4* omega;       // [n] vorticity (xyz) and |omega| (w)
   float4* dv;          // [n] XSPH velocity change
