@@ -179,13 +179,19 @@ def compressed_block(budget=5000, squeeze=0.7):
 
 @pytest.mark.parametrize("effects", ["defaults", "no_surface_tension"])
 def test_truncated_neighbour_lists_match_fp32_oracle(effects):
+    """Truncated lists against the oracle; "fast" starts every particle at up
+    to 3 m/s (about half a cell per substep), so particles cross cells between
+    the list build and surface tension while lists are cut."""
     fx = dict(scenes.FLUID_DEFAULTS)
     if effects == "no_surface_tension":
         fx.pop("surfaceTension")
     pos, r = compressed_block()
-    print(f"[pbf] truncated lists, {len(pos)} particles squeezed 0.7, effects {sorted(fx)}", flush=True)
+    vel = None
+    if effects == "fast":
+        vel = np.random.default_rng(29).uniform(-3.0, 3.0, pos.shape).astype(np.float32)
+    print(f"[pbf] truncated lists, {len(pos)} particles squeezed 0.7, effects {effects}", flush=True)
     ps = pf.ParticleSystem(len(pos), r)
-    ps.uploadParticles(pos)
+    ps.uploadParticles(pos, velocities=vel)
     fluid = pf.FluidSystem(ps, **fx)
     walls = pf.PrimitiveSet(ps)
     mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
@@ -193,6 +199,8 @@ def test_truncated_neighbour_lists_match_fp32_oracle(effects):
         walls.addPlane(n, mn if which == "min" else mx)
     loop = pf.SimLoop(ps, substeps=2, iterations=4, materials=[fluid], colliders=[walls])
     o = oracle_dam_break(pos, r, 2, 4, "f32", effects=fx)
+    if vel is not None:
+        o.set_particles(pos, vel)
     sc = dict(ps=ps, fluid=fluid)
     errs, flags = [], []
     for _ in range(3):
@@ -210,3 +218,25 @@ def test_truncated_neighbour_lists_match_fp32_oracle(effects):
     walls.dispose()
     fluid.dispose()
     ps.dispose()
+
+
+def test_fast_truncated_dam_break_matches_fp32_oracle():
+    """The dam break at a 60,000 budget under the bench's step (2 substeps x
+    4 iterations): lists are cut at 64 from step 10 while the flow reaches
+    10 m/s, so constraint corrections carry particles across cells between
+    the list build and surface tension. Every step is bit-identical with the
+    oracle."""
+    sc = scenes.build_dam_break(60000)
+    n = sc["ps"].capacity
+    print(f"[pbf] fast truncated dam break, {n} particles", flush=True)
+    o = oracle_dam_break(sc["positions"], sc["radius"], 2, 4, "f32")
+    cut = []
+    for step in range(16):
+        sc["loop"].step(1.0 / 60.0)
+        o.step(1.0 / 60.0)
+        err = compare(gpu_state(sc), o.get())
+        cut.append((sc["loop"].readbackOverflow().neighbors, o.flags()["neighbors"]))
+        assert max(err.values()) == 0.0, (step + 1, err)
+    print(f"[pbf] neighbour overflow per step (gpu, oracle) {cut}", flush=True)
+    assert all(a == b for a, b in cut) and sum(a for a, _ in cut) >= 3
+    scenes.dispose_scene(sc)
