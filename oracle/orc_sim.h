@@ -9,6 +9,21 @@
  * bucket), which is one of the orders the reference's atomic scatter may
  * produce (sort.ts:187-192).
  *
+ * How each part is pinned (tests/test_oracle_*.py, tests/test_formats.py):
+ *  - grid: the reference tests' collision-rate curve and Morton cells, the
+ *    overflow rules, lattice counts;
+ *  - fluid passes: closed forms (free fall, an isolated particle's density
+ *    and zero lambda, momentum conservation of surface tension) and the
+ *    FP64 instantiation; the exact FP32 values of lambda, position delta,
+ *    surface tension, adhesion, XSPH and vorticity as the reference's WGSL
+ *    computes them are parity unpinned (that code cannot run here);
+ *  - colliders: contact at exactly one radius per shape, moving shapes,
+ *    dyadic fields sampled exactly, the half-float table;
+ *  - boundaries, viscosity, contacts, shape matching: closed forms, a
+ *    brute-force pair search, fixed points and the reference's scenarios;
+ *  - surface field: a single particle's distance, the separable Gaussian,
+ *    volume conservation, the motion-stretch centroid, plane proximity.
+ *
  * Expected macros: REAL, SFX(name) (symbol suffixing), R(x) (literal),
  * RSQRT, RFLOOR, RFMAX, RFMIN, RFABS, RSIN, RCOS, RRINT.
  */
