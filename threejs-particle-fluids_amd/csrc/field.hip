@@ -277,6 +277,18 @@ struct pf_surface_field_s {
   FieldDev dev{};
   double voxel = 0, origin[3] = {0, 0, 0};
   bool disposed = false;
+  // Frees whatever was allocated (also when creation fails part way).
+  void release() {
+    FieldDev& D = dev;
+    for (void* b : {(void*)D.mass, (void*)D.off, (void*)D.solid, (void*)D.a, (void*)D.b, (void*)D.sa, (void*)D.sb,
+                    (void*)D.walls, (void*)D.out})
+      dfree(b);
+    D.mass = D.off = D.solid = nullptr;
+    D.a = D.b = D.out = nullptr;
+    D.sa = D.sb = nullptr;
+    D.walls = nullptr;
+  }
+  ~pf_surface_field_s() { release(); }
 };
 
 #define PF_API_BEGIN try {
@@ -440,11 +452,10 @@ int pf_surface_field_destroy(pf_surface_field* f) {
   PF_API_BEGIN
   if (!f || f->disposed) return PF_OK;
   (void)hipStreamSynchronize(f->ps->ctx->stream);
-  FieldDev& D = f->dev;
-  dfree(D.mass); dfree(D.off); dfree(D.solid); dfree(D.a); dfree(D.b); dfree(D.sa); dfree(D.sb); dfree(D.walls);
-  dfree(D.out);
+  // The handle stays valid (disposed) so a second destroy is harmless, as
+  // for the library's other objects.
+  f->release();
   f->disposed = true;
-  delete f;
   PF_API_END
 }
 
