@@ -515,9 +515,12 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
 
-        pos, r = sc["positions"], sc["radius"]
+        # The same scene at the end of the timed window (positions and
+        # velocities read back), so the CPU steps cost what the GPU's did.
+        snap = sc["ps"].readback()
+        r = sc["radius"]
         o = orc.Scene(n, r, "f32")
-        o.set_particles(pos)
+        o.set_particles(snap.positions[:, :3], snap.velocities[:, :3])
         o.set_loop(args.substeps, args.iterations, (0.0, -9.81, 0.0), 4 * r, H)
         o.add_fluid(0, n, 1000.0, 2 * r, 4 * r, 1e-4, viscosity=0.01, vorticity=0.02, surface_tension=0.1,
                     adhesion=0.1)
@@ -532,7 +535,9 @@ def main():
         cpu = {"value": round(n * args.substeps * args.cpu_baseline_steps / csec, 1), "unit": "particle-substeps/s",
                "cores": 1, "kind": "port",
                "sample": f"{args.cpu_baseline_steps} step(s) = {args.substeps * args.cpu_baseline_steps} substeps of the "
-                         f"same {n}-particle scene, FP32 C oracle, 1 thread, {csec:.1f} s on {platform.processor() or 'host'}"}
+                         f"same {n}-particle scene from its state after the timed window (step "
+                         f"{args.warmup + args.steps + args.profile_steps}), FP32 C oracle, 1 thread, {csec:.1f} s on "
+                         f"{platform.processor() or 'host'}"}
 
     field = run_surface_field(args, ctx, sc) if args.field_updates > 0 else None
     viscosity = run_viscosity(args, ctx, dt) if args.viscosity_steps > 0 else None
