@@ -47,7 +47,7 @@ static uint16_t half_from_float(float v) {
 // queries: the state keeps its order and the prediction goes to xs[c ^ 1].
 __global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __restrict__ tiles,
                           uint32_t ntiles, int p, int c, int reset_step_flags) {
-  const Tile t = tiles[blockIdx.x];
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const uint32_t s = t.first + threadIdx.x;
   if (reset_step_flags && t.first == 0 && threadIdx.x == 0) {
