@@ -480,7 +480,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   // 4. Walk the union. Completed entry pairs go straight to their list row.
   uint32_t* __restrict__ nl16 = F.nl16;
   if (active) {
-    uint32_t pend = 0;
+    uint32_t pend = 0, last_local = 0;
     uint32_t* row = nl16 + s;  // the row of the next entry pair
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
@@ -496,18 +496,17 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           if (h_key[h] != b + 1u) continue;  // unreachable: the cell's leader inserted b
           const uint32_t ho = h_off[h];
           const uint32_t off = ho >> 16, cn = ho & 0xffffu;
-          const uint32_t st = h_start[h];
-          const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
-          // Branch-free append: only a completed entry pair stores.
-          for (uint32_t i0 = 0; i0 < cn && !over; i0 += 4) {
+          // Branch-free append: only a completed entry pair stores. Whole
+          // groups of four need no bounds test; the last partial group does.
+          auto group = [&](uint32_t i0, bool tail) {
             float4 xj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = u_x[off + min(i0 + u, cn - 1u)];
+            for (int u = 0; u < 4; ++u) xj[u] = u_x[off + (tail ? min(i0 + u, cn - 1u) : i0 + u)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const uint32_t i = i0 + u;
               const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              const bool acc = i < cn && pf_dot3(d, d) < F.h2;
+              const bool acc = (!tail || i < cn) && pf_dot3(d, d) < F.h2;
               const bool keep = acc && count < PF_MAX_NEIGHBORS;
               // The list is full and another neighbour exists: it overflows,
               // nothing later in the walk can enter it.
@@ -518,15 +517,26 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
                 row += n;
               }
               pend = (keep && !(count & 1u)) ? local : pend;
-              last_key = keep ? ((stencil << 27) | (st + i)) : last_key;
+              last_local = keep ? local : last_local;
               count += keep ? 1u : 0u;
             }
-          }
+          };
+          const uint32_t full = cn & ~3u;
+          for (uint32_t i0 = 0; i0 < full && !over; i0 += 4) group(i0, false);
+          if (full < cn && !over) group(full, true);
         }
       }
     }
     if (count & 1u) *row = pend;
     F.nl_count[s] = count;
+    // Walk key of the last kept neighbour of a truncated list, from its
+    // list-build cell and sorted slot (the staged union holds both).
+    if (over) {
+      const float4 xl = u_x[last_local];
+      const int cf[3] = {pf_query_coord(xl.x, S.ox, S.cell), pf_query_coord(xl.y, S.oy, S.cell),
+                         pf_query_coord(xl.z, S.oz, S.cell)};
+      last_key = walk_key(pack_cell(cf), pack_cell(q), F.tu_slots[(size_t)tile * PF_TU_MAX + last_local]);
+    }
     F.nl_last[s] = over ? last_key : 0xffffffffu;
     F.nl_cell[s] = pack_cell(q);
     if (want_normal) {
