@@ -17,20 +17,23 @@ LIB = os.path.join(HERE, "liboracle.so")
 P = C.POINTER
 u32, i32, f32, f64, vp = C.c_uint32, C.c_int, C.c_float, C.c_double, C.c_void_p
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB):
+def lib(path=None):
+    """The oracle library (oracle/liboracle.so, built on first use), or the
+    build of the oracle's sources at `path` (the planted-mistake tests load
+    mutated copies that way)."""
+    key = path or LIB
+    if key in _libs:
+        return _libs[key]
+    if path is None and not os.path.exists(LIB):
         build()
-    L = C.CDLL(LIB)
+    L = C.CDLL(key)
     L.orc_part1by2.argtypes = [u32]
     L.orc_part1by2.restype = u32
     L.orc_morton.argtypes = [C.c_int32, C.c_int32, C.c_int32]
@@ -95,11 +98,13 @@ def lib():
         g("polar").restype = None
         g("scene_neighbors").argtypes = [vp, u32, P(u32), P(u32)]
         g("scene_neighbors").restype = None
+        g("scene_fluid_fields").argtypes = [vp, u32] + [P(f64)] * 7
+        g("scene_fluid_fields").restype = None
     L.orc_poly6_f32.argtypes = [f64, f64]
     L.orc_poly6_f32.restype = f64
     L.orc_spiky_f32.argtypes = [P(f64), f64, P(f64)]
     L.orc_spiky_f32.restype = None
-    _lib = L
+    _libs[key] = L
     return L
 
 
@@ -161,10 +166,10 @@ def polar(a, precision="f32"):
 class Scene:
     """One scene stepped on the CPU, in FP32 ("f32") or FP64 ("f64")."""
 
-    def __init__(self, n, radius, precision="f32"):
+    def __init__(self, n, radius, precision="f32", library=None):
         self.suf = precision
         self.radius = radius
-        self.L = lib()
+        self.L = lib() if library is None else library
         self.n = int(n)
         self.h = self._f("scene_create")(self.n, float(radius))
 
@@ -343,6 +348,17 @@ class Scene:
         cnt = np.zeros(self.n, np.uint32)
         self._f("scene_neighbors")(self.h, int(f), _p(nl, u32), _p(cnt, u32))
         return nl, cnt
+
+    def fluid_fields(self, f=0):
+        """Fluid f's per-pass fields of the last substep, by particle id:
+        lambda and delta x of the last iteration, colour normal, omega,
+        |omega|, eta and the XSPH velocity change."""
+        n = self.n
+        out = dict(lam=np.zeros(n), dx=np.zeros((n, 3)), normal=np.zeros((n, 3)), omega=np.zeros((n, 3)),
+                   omlen=np.zeros(n), eta=np.zeros((n, 3)), dv=np.zeros((n, 3)))
+        self._f("scene_fluid_fields")(self.h, int(f), *[_p(out[k], f64) for k in
+                                                        ("lam", "dx", "normal", "omega", "omlen", "eta", "dv")])
+        return out
 
     def __del__(self):
         try:

@@ -12,11 +12,18 @@
  * How each part is pinned (tests/test_oracle_*.py, tests/test_formats.py):
  *  - grid: the reference tests' collision-rate curve and Morton cells, the
  *    overflow rules, lattice counts;
- *  - fluid passes: closed forms (free fall, an isolated particle's density
- *    and zero lambda, momentum conservation of surface tension) and the
- *    FP64 instantiation; the exact FP32 values of lambda, position delta,
- *    surface tension, adhesion, XSPH and vorticity as the reference's WGSL
- *    computes them are parity unpinned (that code cannot run here);
+ *  - neighbour list and fluid passes (tests/fluid_pins.py): the reference
+ *    tests' brute-force neighbour sets, 64-entry cap and density sum; the
+ *    walk order against an all-pairs search; lambda and position delta of a
+ *    compressed pair in closed form and the point symmetry of a lattice's
+ *    corrections; XSPH on a pair and a uniform drift; omega of a rigid
+ *    rotation, eta and the confinement change of a linear |omega| field;
+ *    the cohesion and adhesion splines' magnitudes on pairs; the viscosity
+ *    solver's shear decay. Each plausible slip (a dropped factor, a sign, an
+ *    operand order, a constant, <= for <) fails at least one of them
+ *    (tests/test_oracle_planted_mistakes.py). What stays unpinned is the
+ *    last-bit rounding of the reference's WGSL (its 2.5-ulp division and
+ *    square root, its summation order), which cannot run here;
  *  - colliders: contact at exactly one radius per shape, moving shapes,
  *    dyadic fields sampled exactly, the half-float table;
  *  - boundaries, viscosity, contacts, shape matching: closed forms, a

@@ -613,6 +613,26 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
     fluid_##SUF* F = &s->fluid[f];                                                                 \
     memcpy(nl, F->nl, sizeof(uint32_t) * 64 * (size_t)s->n);                                       \
     memcpy(cnt, F->nlc, sizeof(uint32_t) * (size_t)s->n);                                          \
+  }                                                                                                \
+  /* fluid f's per-pass fields as the last substep left them, by particle id:                     \
+   * lambda and position delta of the last iteration (lambda.ts, positionDelta.ts),               \
+   * colour normal (cohesion.ts:25-39), omega, |omega|, eta and the XSPH change                   \
+   * (velocity.ts:36-92); any output may be null */                                              \
+  void orc_scene_fluid_fields_##SUF(void* p, uint32_t f, double* lam, double* dx3, double* nrm3,  \
+                                     double* om3, double* oml, double* eta3, double* dv3) {        \
+    scene_##SUF* s = (scene_##SUF*)p;                                                              \
+    fluid_##SUF* F = &s->fluid[f];                                                                 \
+    for (size_t i = 0; i < s->n; ++i) {                                                            \
+      if (lam) lam[i] = (double)F->lambda[i];                                                      \
+      if (oml) oml[i] = (double)F->omlen[i];                                                       \
+      for (int a = 0; a < 3; ++a) {                                                                \
+        if (dx3) dx3[3 * i + a] = (double)F->dx[3 * i + a];                                        \
+        if (nrm3) nrm3[3 * i + a] = (double)F->normal[3 * i + a];                                  \
+        if (om3) om3[3 * i + a] = (double)F->omega[3 * i + a];                                     \
+        if (eta3) eta3[3 * i + a] = (double)F->eta[3 * i + a];                                     \
+        if (dv3) dv3[3 * i + a] = (double)F->dv[3 * i + a];                                        \
+      }                                                                                            \
+    }                                                                                              \
   }
 
 ORC_SCENE_API(f32, float)
