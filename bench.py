@@ -10,9 +10,14 @@ walls), advect, XSPH + vorticity, friction).
 
 Prints one JSON line. `roofline` is the dominant kernel's algorithmic bytes
 per launch over its average duration (HIP events on the library stream);
-`roofline_substep` applies the same byte model to the whole substep.
+`roofline_substep` applies the same byte model to the whole substep. The
+per-kernel durations come from a twin of the scene stepped launch by launch
+over the same steps (the solver is deterministic, so the twin passes through
+the same states as the timed scene).
 `cpu_baseline` times the CPU oracle (the FP32 restatement of the reference
-schedule, one thread) on one step of the same scene.
+schedule, the host's cores) on steps of the same scene; `cpu_c1` times the
+5,000-budget dam break (4,998 particles, 200 steps) on one thread and on all
+cores. `sort` times the isolated 4,194,304-point hash-grid sort.
 """
 import argparse
 import json
@@ -57,6 +62,130 @@ HBM_TRAFFIC_B_PER_PARTICLE = {
     "vorticity_friction": 267.0,
 }
 SCAN_BYTES_PER_BUCKET = 12
+
+# VALU instructions (wave64 SQ_INSTS_VALU) per particle per launch of the
+# 1M dam break's kernels over the bench's window, from the rocprofv3 counter
+# summary profiles/dam_break_1m_sq_counters.md. The VALU peak is the MI355X's
+# issue rate with packed FP32 off: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per SIMD every 4 cycles.
+VALU_INSTS_PER_PARTICLE = {}
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+# The isolated sort: 4,194,304 positions, cell 0.01, 2^22 buckets. Algorithmic
+# bytes per sort 32 N + 12 H: per point read x (16), write and read its key
+# (8), write its sorted key and its permutation entry (8); per bucket clear the
+# count, read it for the scan, write the offset (12).
+SORT_N = 4_194_304
+SORT_H = 1 << 22
+SORT_CELL = 0.01
+SORT_BYTES = 32 * SORT_N + 12 * SORT_H
+
+
+def host_threads():
+    """Host threads for the CPU baseline: the cores this process may use,
+    at most 16 (a one-GPU box's share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "host"
+
+
+def run_sort(args, ctx):
+    """The hash-grid counting sort in isolation (HashGrid.rebuild over a
+    standalone grid): positions resident in HBM, outputs the keys, counts,
+    offsets, sorted keys and permutation; both seeded distributions."""
+    import particle_fluids as pf
+    from particle_fluids import scenes
+
+    out = {"workload": f"hash-grid counting sort, {SORT_N} positions, cell {SORT_CELL}, {SORT_H} buckets",
+           "particles": SORT_N, "hash_table_size": SORT_H, "bytes_per_sort": SORT_BYTES, "reps": args.sort_reps}
+    for dist in ("uniform", "blobs"):
+        pos = scenes.sort_points(SORT_N, dist)
+        g = pf.HashGrid.standalone(SORT_N, SORT_CELL, SORT_H)
+        g.setSortedPositions(False)
+        g.uploadPositions(pos)
+        for _ in range(3):
+            g.rebuild()
+        ctx.synchronize()
+        ctx.timer_begin()
+        for _ in range(args.sort_reps):
+            g.rebuild()
+        ms = ctx.timer_end() / args.sort_reps
+        ctx.set_profiling(True)
+        ctx.profile_reset()
+        for _ in range(5):
+            g.rebuild()
+        stats = ctx.profile()
+        ctx.set_profiling(False)
+        ctx.profile_reset()
+        over = g.readbackOverflow()
+        g.dispose()
+        gbs = SORT_BYTES / (ms / 1000.0) / 1e9
+        out[dist] = {"us_per_sort": round(ms * 1000.0, 2),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                     "overflow": bool(over), "kernels": kernel_table(stats)}
+        print(f"[bench] sort {dist}: {out[dist]['us_per_sort']} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return out
+
+
+def oracle_dam_break(orc, pos, vel, r, substeps, iterations, H):
+    from particle_fluids import scenes
+
+    n = len(pos)
+    o = orc.Scene(n, r, "f32")
+    o.set_particles(pos, vel)
+    o.set_loop(substeps, iterations, (0.0, -9.81, 0.0), 4 * r, H)
+    o.add_fluid(0, n, 1000.0, 2 * r, 4 * r, 1e-4, viscosity=0.01, vorticity=0.02, surface_tension=0.1, adhesion=0.1)
+    mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
+    for nrm, which in scenes.PLANES:
+        o.add_plane(nrm, mn if which == "min" else mx)
+    return o
+
+
+def run_cpu_c1(args):
+    """The reference's CPU-runnable dam break: a 5,000 budget (Simulation's
+    fit: 4,998 particles), +-1% jitter (seed 0xC0FFEE), 2 substeps x 4
+    iterations, 200 steps of 1/60, on the FP32 oracle: one thread, then the
+    host's cores. A restatement of the reference (which has no CPU solver);
+    baseline only."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from particle_fluids import scenes
+
+    pos, r = scenes.dam_break_layout(5000)
+    pos = scenes.jitter(pos, 2 * r)
+    n = len(pos)
+    H = 1
+    while H < 2 * n:
+        H <<= 1
+    out = {"workload": f"dam break, {n} particles (budget 5000), {args.substeps} substeps x {args.iterations} "
+                       f"iterations, dt 1/60, {args.cpu_c1_steps} steps, FP32 C oracle",
+           "particles": n, "steps": args.cpu_c1_steps, "cpu": cpu_model(), "kind": "port"}
+    for label, threads in (("one_thread", 1), ("all_cores", host_threads())):
+        orc.set_threads(threads)
+        o = oracle_dam_break(orc, pos, None, r, args.substeps, args.iterations, H)
+        t = time.perf_counter()
+        for _ in range(args.cpu_c1_steps):
+            o.step(1.0 / 60.0)
+        sec = time.perf_counter() - t
+        out[label] = {"threads": threads, "seconds": round(sec, 3),
+                      "particle_substeps_per_s": round(n * args.substeps * args.cpu_c1_steps / sec, 1)}
+        print(f"[bench] C1 on the CPU, {threads} thread(s): {sec:.2f} s", file=sys.stderr, flush=True)
+    orc.set_threads(0)
+    return out
 
 
 def run_batched(args, ctx):
@@ -414,9 +543,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="batched form: keep the default tile-union capacity")
     ap.add_argument("--autotune-plain", action="store_true",
                     help="tune the single scene's tile-union capacity on a twin (the default 2048 won its sweep)")
-    ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-baseline-steps", type=int, default=1)
+    ap.add_argument("--cpu-baseline-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-c1-steps", type=int, default=200, help="C1 dam break steps on the CPU oracle (0: skip)")
+    ap.add_argument("--sort-reps", type=int, default=20, help="isolated 4M sort repetitions timed (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=3)
@@ -480,15 +610,24 @@ def main():
     ov = loop.readbackOverflow()
     print(f"[bench] {seconds:.3f} s, {value:.4e} particle-substeps/s, overflow {ov}", file=sys.stderr, flush=True)
 
-    # Per-kernel durations: the same workload launched kernel by kernel with
-    # HIP events around every launch on the library's stream.
-    loop.useGraph(False)
+    # Per-kernel durations of the timed window: a twin of the scene, stepped
+    # through the same warm-up with its graph, then launched kernel by kernel
+    # with HIP events around every launch over the same steps.
+    twin = scenes.build_dam_break(args.budget, substeps=args.substeps, iterations=args.iterations,
+                                  use_graph=not args.no_graph)
+    for _ in range(args.warmup):
+        twin["loop"].step(dt)
+    twin["loop"].useGraph(False)
+    ctx.synchronize()
     ctx.set_profiling(True)
     ctx.profile_reset()
-    for _ in range(args.profile_steps):
-        loop.step(dt)
+    for _ in range(args.steps):
+        twin["loop"].step(dt)
     stats = ctx.profile()
     ctx.set_profiling(False)
+    ctx.profile_reset()
+    twin_ov = twin["loop"].readbackOverflow()
+    scenes.dispose_scene(twin)
     profiled_ms = sum(t for _, t in stats.values())
     dominant = max(stats, key=lambda k: stats[k][1])
     launches, total = stats[dominant]
@@ -504,6 +643,14 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                     "traffic": round(tb * n) if tb is not None and args.budget == 1 << 20 else None,
                     "bytes_per_launch": bpp * n, "avg_launch_us": round(avg_s * 1e6, 3)}
+    roofline_valu = None
+    vpp = VALU_INSTS_PER_PARTICLE.get(dominant)
+    if vpp is not None and args.budget == 1 << 20:
+        rate = vpp * n / avg_s
+        roofline_valu = {"bound": "valu", "kernel": dominant, "achieved": round(rate / 1e9, 2),
+                         "peak": round(VALU_PEAK_WAVE_INSTS / 1e9, 1), "unit": "G wave-instructions/s",
+                         "frac": round(rate / VALU_PEAK_WAVE_INSTS, 4),
+                         "instructions_per_launch": round(vpp * n)}
     sub_bytes = SUBSTEP_BYTES_PER_PARTICLE(args.iterations) * n + SCAN_BYTES_PER_BUCKET * H
     sub_s = seconds / substeps_done
     roofline_substep = {"bound": "hbm", "achieved": round(sub_bytes / sub_s / 1e9, 2), "peak": HBM_PEAK_GBS,
@@ -518,26 +665,26 @@ def main():
         # The same scene at the end of the timed window (positions and
         # velocities read back), so the CPU steps cost what the GPU's did.
         snap = sc["ps"].readback()
-        r = sc["radius"]
-        o = orc.Scene(n, r, "f32")
-        o.set_particles(snap.positions[:, :3], snap.velocities[:, :3])
-        o.set_loop(args.substeps, args.iterations, (0.0, -9.81, 0.0), 4 * r, H)
-        o.add_fluid(0, n, 1000.0, 2 * r, 4 * r, 1e-4, viscosity=0.01, vorticity=0.02, surface_tension=0.1,
-                    adhesion=0.1)
-        mn, mx = scenes.CONTAINER[:3], scenes.CONTAINER[3:]
-        for nrm, which in scenes.PLANES:
-            o.add_plane(nrm, mn if which == "min" else mx)
-        print(f"[bench] CPU baseline: {args.cpu_baseline_steps} step(s) of the oracle", file=sys.stderr, flush=True)
+        threads = host_threads()
+        orc.set_threads(threads)
+        o = oracle_dam_break(orc, snap.positions[:, :3], snap.velocities[:, :3], sc["radius"], args.substeps,
+                             args.iterations, H)
+        print(f"[bench] CPU baseline: {args.cpu_baseline_steps} step(s) of the oracle on {threads} threads",
+              file=sys.stderr, flush=True)
         c0 = time.perf_counter()
         for _ in range(args.cpu_baseline_steps):
             o.step(dt)
         csec = time.perf_counter() - c0
+        orc.set_threads(0)
+        del o
         cpu = {"value": round(n * args.substeps * args.cpu_baseline_steps / csec, 1), "unit": "particle-substeps/s",
-               "cores": 1, "kind": "port",
+               "cores": threads, "kind": "port",
                "sample": f"{args.cpu_baseline_steps} step(s) = {args.substeps * args.cpu_baseline_steps} substeps of the "
                          f"same {n}-particle scene from its state after the timed window (step "
-                         f"{args.warmup + args.steps + args.profile_steps}), FP32 C oracle, 1 thread, {csec:.1f} s on "
-                         f"{platform.processor() or 'host'}"}
+                         f"{args.warmup + args.steps}), FP32 C oracle (OpenMP particle loops), {threads} threads, "
+                         f"{csec:.1f} s on {cpu_model()}"}
+    cpu_c1 = run_cpu_c1(args) if args.cpu_c1_steps > 0 else None
+    sort = run_sort(args, ctx) if args.sort_reps > 0 else None
 
     field = run_surface_field(args, ctx, sc) if args.field_updates > 0 else None
     viscosity = run_viscosity(args, ctx, dt) if args.viscosity_steps > 0 else None
@@ -567,8 +714,12 @@ def main():
                    "particles": n, "hash_table_size": H, "substeps": args.substeps,
                    "iterations": args.iterations, "graph": not args.no_graph, "autotune": tuned},
         "roofline": roofline,
+        "roofline_valu": roofline_valu,
         "roofline_substep": roofline_substep,
+        "overflow": {"timed_window": dict(ov._asdict()), "profiled_twin": dict(twin_ov._asdict())},
         "cpu_baseline": cpu,
+        "cpu_c1": cpu_c1,
+        "sort": sort,
         "kernels": kernels,
         "gpu_ms_timed": round(gpu_ms, 3),
         "surface_field": field,

@@ -39,6 +39,7 @@ typedef struct pf_sdf_collider_s pf_sdf_collider;
 typedef struct pf_loop_s pf_loop;
 typedef struct pf_batch_s pf_batch;
 typedef struct pf_viscosity_s pf_viscosity;
+typedef struct pf_fence_s pf_fence;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* pf_last_error(void);
@@ -137,6 +138,7 @@ int pf_grid_set_origin(pf_grid* grid, const double origin[3]);
 int pf_grid_query_counts(pf_grid* grid, const float* query4, uint32_t nquery, double radius,
                          uint32_t* candidates, uint32_t* within);
 int pf_grid_destroy(pf_grid* grid);
+
 
 /* ---- FluidSystem (src/fluids/FluidSystem.ts:71-322) ---------------------- */
 #define PF_FLUID_VISCOSITY 1u
@@ -251,6 +253,8 @@ typedef struct pf_softbody_def {
   const float* rest_positions; /* xyz triples, nullable = uploaded positions */
   const uint32_t* edges;   /* pairs [i0, j0, i1, j1, ...] local to the body, nullable */
   uint32_t edge_count;     /* number of pairs */
+  uint32_t rest_count;     /* floats in rest_positions; must be 3 * count when it is given
+                              (SoftbodySystem.ts:171-173) */
 } pf_softbody_def;
 #define PF_SHAPE_MATCHING_GLOBAL 0u
 #define PF_SHAPE_MATCHING_LOCAL 1u
@@ -345,6 +349,19 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* options, const pf_ma
 /* step(dt) (loop.ts:267-279): enqueues every substep and returns. */
 int pf_loop_step(pf_loop* loop, double dt);
 int pf_loop_synchronize(pf_loop* loop);
+/* The completion behind step()'s Promise (loop.ts:267-279, which resolves
+ * when computeAsync's work is done): a fence records the point of the
+ * loop's stream after the steps enqueued so far; pf_fence_wait blocks until
+ * the GPU passes it, pf_fence_query polls. Waiting touches neither the loop
+ * nor its stream, so another thread may wait while the owner keeps stepping.
+ * Destroy each fence once. */
+int pf_loop_fence(pf_loop* loop, pf_fence** out);
+/* The same behind everything enqueued on the context's stream (the batched
+ * form's step Promise). */
+int pf_context_fence(pf_context* ctx, pf_fence** out);
+int pf_fence_wait(pf_fence* fence);
+int pf_fence_query(pf_fence* fence, int* done);
+int pf_fence_destroy(pf_fence* fence);
 /* readbackOverflow() (loop.ts:286-295) */
 int pf_loop_readback_overflow(pf_loop* loop, pf_overflow* out);
 int pf_loop_set_substeps(pf_loop* loop, uint32_t substeps);

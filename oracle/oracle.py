@@ -100,12 +100,26 @@ def lib(path=None):
         g("scene_neighbors").restype = None
         g("scene_fluid_fields").argtypes = [vp, u32] + [P(f64)] * 7
         g("scene_fluid_fields").restype = None
+    L.orc_set_threads.argtypes = [i32]
+    L.orc_set_threads.restype = None
+    L.orc_get_threads.argtypes = []
+    L.orc_get_threads.restype = i32
     L.orc_poly6_f32.argtypes = [f64, f64]
     L.orc_poly6_f32.restype = f64
     L.orc_spiky_f32.argtypes = [P(f64), f64, P(f64)]
     L.orc_spiky_f32.restype = None
     _libs[key] = L
     return L
+
+
+def set_threads(n):
+    """Threads of the oracle's particle loops (0: all the host's cores); the
+    results do not depend on it."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def _p(a, t):

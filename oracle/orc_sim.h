@@ -31,6 +31,11 @@
  *  - surface field: a single particle's distance, the separable Gaussian,
  *    volume conservation, the motion-stretch centroid, plane proximity.
  *
+ * Per-particle loops run in parallel (OpenMP) where each particle writes
+ * only its own outputs; scatters into another particle's fixed-point
+ * accumulator use atomic integer adds. Integer sums do not depend on order,
+ * so every thread count gives the same bits.
+ *
  * Expected macros: REAL, SFX(name) (symbol suffixing), R(x) (literal),
  * RSQRT, RFLOOR, RFMAX, RFMIN, RFABS, RSIN, RCOS, RRINT.
  */
@@ -223,6 +228,7 @@ static void SFX(grid_build)(SFX(scene)* s) {
 static void SFX(neighbors)(SFX(scene)* s, SFX(fluid)* F) {
   uint32_t H = s->H;
   int over = 0;
+#pragma omp parallel for schedule(dynamic, 128) reduction(|| : over)
   for (uint32_t i = F->start; i < F->start + F->count; ++i) {
     const REAL* xi = s->xs + 3 * (size_t)i;
     int32_t q[3];
@@ -266,6 +272,7 @@ static inline REAL SFX(nmass)(const SFX(scene)* s, const SFX(fluid)* F, uint32_t
  * one-time pass, predicted for a dynamic one). */
 static void SFX(boundary_volume)(SFX(scene)* s, const SFX(fluid)* F, uint32_t b, const REAL* pos) {
   uint32_t H = s->H, lo = F->bstart[b], hi = F->bstart[b] + F->bcount[b];
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = lo; i < hi; ++i) {
     const REAL* xi = pos + 3 * (size_t)i;
     int32_t q[3];
@@ -303,6 +310,7 @@ static inline int SFX(in_boundary)(const SFX(fluid)* F, uint32_t j) {
 
 /* cohesion.ts:25-39 */
 static void SFX(color_normal)(SFX(scene)* s, SFX(fluid)* F) {
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = F->start; i < F->start + F->count; ++i) {
     const REAL* xi = s->xs + 3 * (size_t)i;
     REAL sum[3] = {0, 0, 0};
@@ -337,6 +345,7 @@ static inline REAL SFX(adh_spline)(const SFX(fluid)* F, REAL r) {
 static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
   int st = (F->effects & 4u) != 0, adh = (F->effects & 8u) != 0;
   uint32_t end = F->start + F->count;
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = F->start; i < end; ++i) {
     const REAL* xi = s->xs + 3 * (size_t)i;
     for (uint32_t k = 0; k < F->nlc[i]; ++k) {
@@ -354,8 +363,8 @@ static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
             REAL coh = (off[a] * ir) * cs;
             REAL curv = -((F->normal[3 * (size_t)i + a] - F->normal[3 * (size_t)j + a]) * gm);
             REAL dv = (((coh + curv) * corr) * F->inv_mass) * s->dt;
-            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)(dv, F->imp_scale));
-            s->iacc[3 * (size_t)j + a] = SFX(wadd)(s->iacc[3 * (size_t)j + a], SFX(ticks)(-dv, F->imp_scale));
+            orc_acc_add(&s->iacc[3 * (size_t)i + a], SFX(ticks)(dv, F->imp_scale));
+            orc_acc_add(&s->iacc[3 * (size_t)j + a], SFX(ticks)(-dv, F->imp_scale));
           }
         }
       }
@@ -367,25 +376,29 @@ static void SFX(impulses)(SFX(scene)* s, SFX(fluid)* F) {
           REAL ir = R(1.0) / r;
           for (int a = 0; a < 3; ++a) {
             REAL f = (off[a] * ir) * fs;
-            s->iacc[3 * (size_t)i + a] = SFX(wadd)(s->iacc[3 * (size_t)i + a], SFX(ticks)((f * F->inv_mass) * s->dt, F->imp_scale));
+            orc_acc_add(&s->iacc[3 * (size_t)i + a], SFX(ticks)((f * F->inv_mass) * s->dt, F->imp_scale));
           }
         }
       }
     }
   }
+  int sat = 0;
+#pragma omp parallel for schedule(static) reduction(|| : sat)
   for (uint32_t i = F->start; i < end; ++i) {
     REAL c[3];
-    if (SFX(acc_change)(s->iacc + 3 * (size_t)i, F->imp_inv, c)) s->flag_imp = 1;
+    if (SFX(acc_change)(s->iacc + 3 * (size_t)i, F->imp_inv, c)) sat = 1;
     for (int a = 0; a < 3; ++a) {
       s->v[3 * (size_t)i + a] = s->v[3 * (size_t)i + a] + c[a];
       s->xs[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a] + c[a] * s->dt;
       s->iacc[3 * (size_t)i + a] = 0;
     }
   }
+  if (sat) s->flag_imp = 1;
 }
 
 /* lambda.ts:30-71 */
 static void SFX(lambda_pass)(SFX(scene)* s, SFX(fluid)* F) {
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = F->start; i < F->start + F->count; ++i) {
     const REAL* xi = s->xs + 3 * (size_t)i;
     REAL rho = 0, gss = 0, gs[3] = {0, 0, 0};
@@ -416,6 +429,7 @@ static void SFX(lambda_pass)(SFX(scene)* s, SFX(fluid)* F) {
 /* positionDelta.ts:26-85 (deltaX pass, then the separate apply) */
 static void SFX(delta_pass)(SFX(scene)* s, SFX(fluid)* F) {
   uint32_t end = F->start + F->count;
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = F->start; i < end; ++i) {
     const REAL* xi = s->xs + 3 * (size_t)i;
     REAL li = F->lambda[i], sum[3] = {0, 0, 0};
@@ -431,11 +445,12 @@ static void SFX(delta_pass)(SFX(scene)* s, SFX(fluid)* F) {
       if (F->nb && s->bvol[j] > R(0.0) && s->w[j] > R(0.0) && li != R(0.0) && SFX(in_boundary)(F, j)) {
         REAL rs = ((-(li / F->rho0)) * (F->rho0 * s->bvol[j])) * s->w[j];
         for (int a = 0; a < 3; ++a)
-          s->racc[3 * (size_t)j + a] = SFX(wadd)(s->racc[3 * (size_t)j + a], SFX(ticks)(g[a] * rs, F->rea_scale));
+          orc_acc_add(&s->racc[3 * (size_t)j + a], SFX(ticks)(g[a] * rs, F->rea_scale));
       }
     }
     for (int a = 0; a < 3; ++a) F->dx[3 * (size_t)i + a] = (sum[a] / F->rho0) * s->w[i];
   }
+#pragma omp parallel for schedule(static)
   for (uint32_t i = F->start; i < end; ++i)
     for (int a = 0; a < 3; ++a) s->xs[3 * (size_t)i + a] = s->xs[3 * (size_t)i + a] + F->dx[3 * (size_t)i + a];
   if (F->nb) {
@@ -461,6 +476,7 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
   int vort = (F->effects & 2u) != 0, visc = (F->effects & 1u) != 0;
   if (!vort && !visc) return;
   uint32_t end = F->start + F->count;
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t i = F->start; i < end; ++i) {
     const REAL* xi = s->x + 3 * (size_t)i;
     const REAL* vi = s->v + 3 * (size_t)i;
@@ -489,6 +505,7 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
     if (visc) for (int a = 0; a < 3; ++a) F->dv[3 * (size_t)i + a] = sm[a] * F->visc;
   }
   if (vort) {
+#pragma omp parallel for schedule(dynamic, 128)
     for (uint32_t i = F->start; i < end; ++i) {
       const REAL* xi = s->x + 3 * (size_t)i;
       REAL oi = F->omlen[i], eta[3] = {0, 0, 0};
@@ -505,6 +522,7 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
       for (int a = 0; a < 3; ++a) F->eta[3 * (size_t)i + a] = eta[a];
     }
   }
+#pragma omp parallel for schedule(static)
   for (uint32_t i = F->start; i < end; ++i) {
     REAL ch[3] = {0, 0, 0};
     if (vort) {
@@ -538,6 +556,7 @@ static void SFX(viscosity_solve)(SFX(scene)* s, SFX(fluid)* F) {
   for (uint32_t k = 0; k < F->isv_iters; ++k) {
     const REAL* src = (k & 1u) ? F->isv_b : F->isv_a;
     REAL* dst = (k & 1u) ? F->isv_a : F->isv_b;
+#pragma omp parallel for schedule(dynamic, 128)
     for (uint32_t i = F->start; i < end; ++i) {
       const REAL* xi = s->x + 3 * (size_t)i;
       REAL sum[3] = {0, 0, 0}, weight = 0;
@@ -768,6 +787,7 @@ static void SFX(sdf_friction)(SFX(scene)* s, uint32_t p, REAL w, const REAL* xs,
 
 static void SFX(walls_solve)(SFX(scene)* s, REAL clock) {
   uint32_t n = s->n;
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t p = 0; p < n; ++p) {
     REAL w = s->w[p];
     if (!(w > R(0.0))) continue;
@@ -800,19 +820,23 @@ static void SFX(walls_solve)(SFX(scene)* s, REAL clock) {
     }
     if (s->nsdf) SFX(sdf_solve)(s, p, w, xs, disp);
   }
+  int sat = 0;
+#pragma omp parallel for schedule(static) reduction(|| : sat)
   for (uint32_t p = 0; p < n; ++p) {
     REAL c[3];
-    if (SFX(acc_change)(s->pacc + 3 * (size_t)p, s->pos_inv, c)) s->flag_pos = 1;
+    if (SFX(acc_change)(s->pacc + 3 * (size_t)p, s->pos_inv, c)) sat = 1;
     for (int a = 0; a < 3; ++a) {
       s->xs[3 * (size_t)p + a] = s->xs[3 * (size_t)p + a] + c[a];
       s->pacc[3 * (size_t)p + a] = 0;
     }
   }
+  if (sat) s->flag_pos = 1;
 }
 
 /* PrimitiveSet.ts:366-391 then the velocities accumulator apply (loop.ts:226-230) */
 static void SFX(walls_friction)(SFX(scene)* s, REAL clock) {
   uint32_t n = s->n;
+#pragma omp parallel for schedule(dynamic, 128)
   for (uint32_t p = 0; p < n; ++p) {
     REAL w = s->w[p];
     if (!(w > R(0.0))) continue;
@@ -841,14 +865,17 @@ static void SFX(walls_friction)(SFX(scene)* s, REAL clock) {
     }
     if (s->nsdf) SFX(sdf_friction)(s, p, w, xs, v);
   }
+  int sat = 0;
+#pragma omp parallel for schedule(static) reduction(|| : sat)
   for (uint32_t p = 0; p < n; ++p) {
     REAL c[3];
-    if (SFX(acc_change)(s->vacc + 3 * (size_t)p, s->vel_inv, c)) s->flag_vel = 1;
+    if (SFX(acc_change)(s->vacc + 3 * (size_t)p, s->vel_inv, c)) sat = 1;
     for (int a = 0; a < 3; ++a) {
       s->v[3 * (size_t)p + a] = s->v[3 * (size_t)p + a] + c[a];
       s->vacc[3 * (size_t)p + a] = 0;
     }
   }
+  if (sat) s->flag_vel = 1;
 }
 
 /* --------------------------------------------------------------------- */
@@ -1548,6 +1575,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
   }
   for (uint32_t k = 0; k < s->substeps; ++k) {
     /* predict (integrate.ts:25-34) */
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n; ++i) {
       if (!(s->w[i] > R(0.0))) continue;
       for (int a = 0; a < 3; ++a) {
@@ -1585,6 +1613,7 @@ static void SFX(step)(SFX(scene)* s, double dt_step) {
       if (s->nprim || s->nsdf || s->contact) SFX(walls_solve)(s, s->clock[k]);
     }
     /* advect (integrate.ts:36-44) */
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n; ++i) {
       if (!(s->w[i] > R(0.0))) continue;
       for (int a = 0; a < 3; ++a) {

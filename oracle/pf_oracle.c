@@ -17,6 +17,7 @@
  * three.js or WebGPU), so nothing under /root/reference is compiled.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -185,6 +186,18 @@ void orc_grid_query_counts(const float* pos4, uint32_t n, float cell, const floa
   free(ce);
   free(perm);
 }
+
+/* Atomic wrap-around add into a fixed-point accumulator word: a scatter
+ * from parallel particle loops (integer sums do not depend on order). */
+static inline void orc_acc_add(int32_t* a, int32_t v) {
+  uint32_t* u = (uint32_t*)a;
+#pragma omp atomic
+  *u += (uint32_t)v;
+}
+
+/* Threads the oracle's particle loops use (0: OpenMP's default). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 /* ---------------------------------------------------------------------- */
 /* FP32 mirror and FP64 restatement of the substep                        */

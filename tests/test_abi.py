@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import particle_fluids as pf
@@ -55,3 +56,18 @@ def test_fit_budget_reproduces_simulation_sizes():
     r, n = pf.fit_budget(1 << 20, c, box)
     assert n == 82 * 102 * 123 == 1028772
     assert r == pytest.approx(0.00242753, abs=5e-9)
+
+
+def test_upload_arrays_must_cover_every_particle():
+    # The C ABI reads n entries of each array (particles.ts:121-160 takes one record per particle).
+    pos = np.zeros((5, 3), np.float32)
+    p, v, m, g = pf.upload_arrays(pos, np.ones((5, 3)), 2.0, 3)
+    assert v.shape == (5, 3) and (m == 2.0).all() and (g == 3).all() and g.dtype == np.uint32
+    with pytest.raises(ValueError, match="4 velocities for 5 particles"):
+        pf.upload_arrays(pos, np.zeros((4, 3)))
+    with pytest.raises(ValueError):
+        pf.upload_arrays(pos, np.zeros(14))
+    with pytest.raises(ValueError, match="need one value or 5 values"):
+        pf.upload_arrays(pos, None, np.ones(3))
+    with pytest.raises(ValueError, match="need one value or 5 values"):
+        pf.upload_arrays(pos, None, None, np.zeros(6))

@@ -159,3 +159,33 @@ def test_js_surface_field_matches_python(tmp_path):
     assert (got[..., 0] < 0).sum() > 50
     assert np.array_equal(got, want)
     field.dispose()
+
+
+def test_addon_rejects_short_and_mistyped_arrays():
+    # The addon checks each TypedArray's element type and length before the
+    # C ABI reads count entries of it (particles.ts:121-160, SoftbodySystem.ts:171-173).
+    script = """
+const pf = require('./index.js');
+const out = [];
+function attempt(f) { try { f(); out.push('ok'); } catch (e) { out.push(e.constructor.name + ': ' + e.message); } }
+attempt(() => pf.native.particlesUpload(null, 0, new Float64Array(6), null, null, null));
+attempt(() => pf.native.particlesUpload(null, 0, new Float32Array(6), new Float32Array(3), null, null));
+attempt(() => pf.native.particlesUpload(null, 0, new Float32Array(6), null, new Uint8Array(2), null));
+attempt(() => pf.native.particlesUpload(null, 0, new Float32Array(6), null, new Float32Array(1), null));
+attempt(() => pf.native.particlesUpload(null, 0, new Float32Array(6), null, null, new Uint32Array(3)));
+attempt(() => pf.native.particlesUpload(null, 0, new Float32Array(7), null, null, null));
+attempt(() => pf.native.softbodyCreate(null, [{start: 0, count: 2, restPositions: new Float64Array(6)}], 0));
+attempt(() => pf.native.softbodyCreate(null, [{start: 0, count: 2, edges: new Uint32Array(3)}], 0));
+console.log(JSON.stringify(out));
+"""
+    r = node(script)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out[0] == "TypeError: pf addon: expected a Float32Array"
+    assert out[1] == "RangeError: ParticleSystem.uploadParticles: 3 velocity values for 2 particles"
+    assert out[2] == "TypeError: pf addon: expected a Float32Array"
+    assert out[3] == "RangeError: ParticleSystem.uploadParticles: 1 inverse masses for 2 particles"
+    assert out[4] == "RangeError: ParticleSystem.uploadParticles: 3 collision groups for 2 particles"
+    assert out[5].startswith("RangeError: ParticleSystem.uploadParticles: positions must hold xyz triples")
+    assert out[6] == "TypeError: pf addon: expected a Float32Array"
+    assert out[7] == "RangeError: SoftbodySystem: body 0 has an odd number of edge indices"

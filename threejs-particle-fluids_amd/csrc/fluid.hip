@@ -6,7 +6,9 @@
 //   k_neighbors     27-bucket walk -> neighbour list, fused colour normal
 //                   (neighborList.ts:45-72, cohesion.ts:25-39)
 //   k_impulses      surface tension + adhesion as a gather with exact
-//                   fixed-point sums, fused impulse apply
+//                   fixed-point sums, fused impulse apply; in scenes that cut
+//                   lists or have contacts the halves only the lower end's
+//                   list holds are scattered there and k_impulse_apply applies
 //                   (cohesion.ts:63-103, adhesion.ts:34-60, accumulator.ts:101-131)
 //   k_lambda        density and lambda (lambda.ts:30-71)
 //   k_delta         position correction + apply, optionally fused container
@@ -660,10 +662,21 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
   return (r * 2.0f > F.h && r <= F.h) ? root * F.adh_coef : 0.0f;
 }
 
+// The partner's half of a pair, added to its accumulator: the exact
+// negation of the lower end's change (round is symmetric). Integer sums do
+// not depend on the order the halves arrive in.
+static __device__ __forceinline__ void scatter_half(int4* acc, uint32_t j, float3 dv, float scale) {
+  int* aj = reinterpret_cast<int*>(acc + j);
+  atomicAdd(aj + 0, pf_ticks(-dv.x, scale));
+  atomicAdd(aj + 1, pf_ticks(-dv.y, scale));
+  atomicAdd(aj + 2, pf_ticks(-dv.z, scale));
+}
+
 // Surface tension and adhesion of one particle in a table below 64 buckets
 // (global slots, one neighbour at a time). Such scenes hold at most 16
-// particles, so no list is truncated and both ends list every pair; pairs
-// with a moved end were scattered (k_cohesion_scatter).
+// particles, so no list is truncated and both ends list every pair. A pair
+// with an end the contact stabilisation moved is the lower end's alone: it
+// adds its half and scatters the partner's into the accumulator.
 static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
                                      float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
                                      bool ct_scene, bool moved_i, int tk[3]) {
@@ -674,7 +687,7 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = nl[(size_t)k * n + s];
     const bool moved = moved_i || (ct_scene && S.ct.acc[j].w != 0);
-    if (st_on && !moved) {
+    if (st_on) {
       const uint32_t idj = S.id[p][j];
       if (idj > id && idj < fend) {
         float3 dv;
@@ -682,8 +695,9 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
+          if (moved) scatter_half(S.ct.acc, j, dv, F.imp_scale);
         }
-      } else if (idj < id && idj >= F.start) {
+      } else if (!moved && idj < id && idj >= F.start) {
         float3 dv;
         if (pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
           tk[0] = pf_wrap_add(tk[0], pf_ticks(-dv.x, F.imp_scale));
@@ -710,120 +724,31 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
   }
 }
 
-// Surface tension the way the reference scatters it (cohesion.ts:60-100):
-// each pair from the list of its lower id, the fixed-point change added to
-// both ends. The gathered form in k_impulses evaluates a pair from either
-// end: the lower end always (the pair is in its list), the upper end from its
-// own list, checking a truncated lower partner's list by walk key. That misses
-// two kinds of pairs, and this pass takes exactly those:
-//  - the upper end's list was truncated at 64 (neighborList.ts:63-66) and
-//    dropped the lower end, which kept it: the upper end's half is added here;
-//  - in scenes with contacts, an end was moved by the stabilisation, whose
-//    query point is the stabilised prediction and candidates the sorted copy,
-//    so the two lists can disagree about the pair: both halves are added here.
-// A tile runs it only when its union holds such a particle. The sums go to
-// the scatter accumulator (idle between the stabilisation and the first
-// contact solve); k_impulses adds and clears them.
-static __device__ __forceinline__ bool nl_truncated(const FluidDev& F, uint32_t s) {
-  return F.nl_last[s] != 0xffffffffu;
-}
-
-__global__ __launch_bounds__(PF_BLOCK) void k_cohesion_scatter(const SceneDev* __restrict__ scenes,
-                                                               const Tile* __restrict__ tiles,
-                                                               uint32_t ntiles, int fi, int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  // Uniform over the block from here to the union test.
-  if (!S.ct.acc || !(F.effects & PF_FX_SURFACE_TENSION)) return;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const bool trunc_scene = S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
-  const bool moved_tile = S.ct.on && F.tu_moved[tile];
-  if (!trunc_scene && !moved_tile) return;
-  const uint32_t U = F.tu_size[tile];
-  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-  const bool staged = U != PF_TU_GLOBAL;
-  if (!moved_tile && staged) {
-    // Any truncated list among the union's particles (a global-walk tile is
-    // taken as having one).
-    int any = 0;
-    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) any |= nl_truncated(F, uslots[u]) ? 1 : 0;
-    if (!__syncthreads_or(any)) return;
-  }
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  const uint32_t id = S.id[p][s];
-  if (!in_range(id, F.start, F.count)) return;
-  const float4* __restrict__ xs = S.xs[c];
-  const float3 xi = pf_xyz(xs[s]);
-  const float3 ni = pf_xyz(F.normal[s]);
-  const float rhoi = S.rho[p][s];
-  const bool moved_i = S.ct.on && S.ct.acc[s].w != 0;
-  const uint32_t ci = F.nl_cell[s];
-  const uint32_t n = S.n;
-  const uint32_t cnt = F.nl_count[s];
-  const uint32_t fend = F.start + F.count;
-  // This particle's half of its moved pairs is summed here and added once;
-  // wrap sums do not depend on the order.
-  int own[3] = {0, 0, 0};
-  bool any = false;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    uint32_t j;
-    if (staged) {
-      const uint32_t w = F.nl16[(size_t)(k >> 1) * n + s];
-      j = uslots[min(((k & 1u) ? (w >> 16) : w) & (PF_TU_MAX - 1u), U - 1u)];
-    } else {
-      j = F.nl[(size_t)k * n + s];
-    }
-    const uint32_t idj = S.id[p][j];
-    if (!(idj > id && idj < fend)) continue;
-    const bool moved = moved_i || (S.ct.on && S.ct.acc[j].w != 0);
-    if (!moved) {
-      // Unmoved pairs are the gather's unless j's truncated list dropped this slot.
-      if (!trunc_scene) continue;
-      const uint32_t lastj = F.nl_last[j];
-      if (lastj == 0xffffffffu || walk_key(ci, F.nl_cell[j], s) <= lastj) continue;
-    }
-    float3 dv;
-    if (!pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) continue;
-    if (moved) {
-      own[0] = pf_wrap_add(own[0], pf_ticks(dv.x, F.imp_scale));
-      own[1] = pf_wrap_add(own[1], pf_ticks(dv.y, F.imp_scale));
-      own[2] = pf_wrap_add(own[2], pf_ticks(dv.z, F.imp_scale));
-      any = true;
-    }
-    // The partner's change is the exact negation (round is symmetric).
-    int* aj = reinterpret_cast<int*>(S.ct.acc + j);
-    atomicAdd(aj + 0, pf_ticks(-dv.x, F.imp_scale));
-    atomicAdd(aj + 1, pf_ticks(-dv.y, F.imp_scale));
-    atomicAdd(aj + 2, pf_ticks(-dv.z, F.imp_scale));
-  }
-  if (any) {
-    int* ai = reinterpret_cast<int*>(S.ct.acc + s);
-    atomicAdd(ai + 0, own[0]);
-    atomicAdd(ai + 1, own[1]);
-    atomicAdd(ai + 2, own[2]);
-  }
-}
-
 // The gathered surface tension and adhesion of one particle over its list
 // (cohesion.ts:63-103, adhesion.ts:34-60). Four neighbours at a time, every
 // gather of the four issued before the arithmetic (the pass is
 // latency-bound). Each pair is evaluated from its lower id's side: operands
 // swapped and the change negated on the upper side. The fixed-point sums do
-// not depend on the order. SPECIAL: the scene cut a list at 64 or has
-// contacts. Pairs with a moved end are then the scatter's, and an upper-end
-// pair exists only if the lower partner's list holds this slot, which a
-// truncated partner's walk key decides; the partner's id, key and mark are
-// read first and its position, normal and density only for the pairs
-// evaluated here. Otherwise every fluid partner is evaluated here and keys
-// and marks are not read.
+// not depend on the order.
+// The reference scatters each pair from the list of its lower id, the change
+// added to both ends, so a pair exists iff the lower end's list holds it.
+// SPECIAL: the scene cut a list at 64 or has contacts. The upper end then
+// takes its half only if the lower partner's list holds this slot, which a
+// truncated partner's walk key decides (the partner's id, key and mark are
+// read first, its position, normal and density only for the pairs evaluated
+// here). The halves the upper end cannot see are scattered by the lower end
+// into the upper end's accumulator: those of pairs its truncated list
+// dropped (neighborList.ts:63-66), and both halves' partner side for pairs
+// with an end the contact stabilisation moved (whose query point is the
+// stabilised prediction and candidates the sorted copy, so the two lists may
+// disagree). Otherwise every fluid partner is evaluated here and keys and
+// marks are not read.
 template <bool SPECIAL>
 static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const FluidDev& F, int p,
                                                     const float4* __restrict__ xs, uint32_t s, uint32_t id,
                                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
                                                     bool adh_on, bool moved_i, bool moved_tile, uint32_t U,
-                                                    const uint32_t* __restrict__ uslots, int tk[3]) {
+                                                    const uint32_t* __restrict__ uslots, int* acc_l, int tk[3]) {
   const uint32_t n = S.n;
   const uint32_t fend = F.start + F.count;
   const bool staged = U != PF_TU_GLOBAL;
@@ -832,8 +757,8 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
   NlCursor cg(F.nl, n, s, cnt);
   for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
     uint32_t jj[4];
+    uint32_t idx[4] = {0u, 0u, 0u, 0u};
     if (staged) {
-      uint32_t idx[4];
       c16.next(k0, idx);
 #pragma unroll
       for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
@@ -843,15 +768,15 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
     uint32_t idj[4], lastj[4], cj[4];
     float4 xj[4], nj[4];
     float rj[4], vol[4];
-    bool lower[4], upper[4];
+    bool lower[4], upper[4], moved[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       idj[u] = S.id[p][jj[u]];
       lastj[u] = 0xffffffffu;
-      bool moved = false;
+      moved[u] = false;
       if (SPECIAL && st_pairs) {
         lastj[u] = F.nl_last[jj[u]];
-        moved = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
+        moved[u] = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
       }
       vol[u] = adh_on ? S.bvol[p][jj[u]] : 0.0f;
       if (!SPECIAL) {
@@ -862,9 +787,9 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         }
       }
       const bool in = k0 + u < cnt;
-      // The scatter took pairs with a moved end.
-      lower[u] = st_pairs && in && !moved && idj[u] > id && idj[u] < fend;
-      upper[u] = st_pairs && in && !moved && idj[u] < id && idj[u] >= F.start;
+      // A pair with a moved end is its lower end's alone.
+      lower[u] = st_pairs && in && idj[u] > id && idj[u] < fend;
+      upper[u] = st_pairs && in && !moved[u] && idj[u] < id && idj[u] >= F.start;
     }
     if (SPECIAL) {
 #pragma unroll
@@ -874,7 +799,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         rj[u] = 0.0f;
         cj[u] = 0u;
-        if (upper[u] && lastj[u] != 0xffffffffu) cj[u] = F.nl_cell[jj[u]];
+        if ((upper[u] || lower[u]) && lastj[u] != 0xffffffffu) cj[u] = F.nl_cell[jj[u]];
         if (pair || vol[u] > 0.0f) xj[u] = xs[jj[u]];
         if (pair) {
           nj[u] = F.normal[jj[u]];
@@ -905,6 +830,19 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
             tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x * sg, F.imp_scale));
             tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y * sg, F.imp_scale));
             tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z * sg, F.imp_scale));
+            // The partner's half when its own walk cannot take it: a moved
+            // end, or its truncated list dropped this slot.
+            if (SPECIAL && lo &&
+                (moved[u] || (lastj[u] != 0xffffffffu && walk_key(ci, cj[u], s) > lastj[u]))) {
+              if (acc_l) {
+                // the partner is union entry idx: summed in LDS, flushed once per tile
+                atomicAdd(acc_l + 3 * idx[u] + 0, pf_ticks(-dv.x, F.imp_scale));
+                atomicAdd(acc_l + 3 * idx[u] + 1, pf_ticks(-dv.y, F.imp_scale));
+                atomicAdd(acc_l + 3 * idx[u] + 2, pf_ticks(-dv.z, F.imp_scale));
+              } else {
+                scatter_half(S.ct.acc, jj[u], dv, F.imp_scale);
+              }
+            }
           }
         }
       }
@@ -927,15 +865,13 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
   }
 }
 
+// One particle's surface tension and adhesion. In a special scene (cut
+// lists or contacts) its own sum is parked, as bits, in its slot of the
+// output prediction buffer for k_impulse_apply; otherwise it is applied here.
 template <bool ST, bool ADH, bool CT>
-__global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ tiles,
-                                                       uint32_t ntiles, int fi, int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
+static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const FluidDev& F, int p, int c,
+                                                        uint32_t s, uint32_t tile, bool special, bool ct_scene,
+                                                        int* acc_l) {
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const uint32_t id = S.id[p][s];
@@ -960,32 +896,24 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
-  // The halves of pairs an upper end's truncated list dropped, and in scenes
-  // with contacts (CT: some scene of the launch has them) pairs with a
-  // particle the stabilisation moved, were scattered already
-  // (k_cohesion_scatter) into the accumulator; the gather adds those sums
-  // and skips such pairs. The moved marks (.w) stay for the other threads of
-  // this pass.
-  const bool trunc_scene = st_on && S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
-  const bool ct_scene = CT && st_on && S.ct.on;
-  const bool st_pairs = st_on;
   bool moved_tile = false, moved_i = false;
-  if (trunc_scene || ct_scene) {
-    int* a = reinterpret_cast<int*>(S.ct.acc + s);
-    const int4 v = S.ct.acc[s];
-    a[0] = 0; a[1] = 0; a[2] = 0;
-    tk[0] = v.x; tk[1] = v.y; tk[2] = v.z;
-    moved_tile = ct_scene && F.tu_moved[t.first / PF_BLOCK] != 0u;
-    moved_i = ct_scene && v.w != 0;
+  if (ct_scene) {
+    moved_tile = F.tu_moved[tile] != 0u;
+    moved_i = S.ct.acc[s].w != 0;
   }
-  const uint32_t U = F.tu_size[t.first / PF_BLOCK];
-  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)(t.first / PF_BLOCK) * PF_TU_MAX;
+  const uint32_t U = F.tu_size[tile];
+  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, ct_scene, moved_i, tk);
-  } else if (trunc_scene || ct_scene) {
-    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, moved_i, moved_tile, U, uslots, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, ct_scene, moved_i, tk);
+  } else if (special) {
+    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, moved_i, moved_tile, U, uslots, acc_l,
+                       tk);
   } else {
-    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_pairs, adh_on, false, false, U, uslots, tk);
+    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, false, false, U, uslots, nullptr, tk);
+  }
+  if (special) {
+    S.xs[c ^ 1][s] = make_float4(__int_as_float(tk[0]), __int_as_float(tk[1]), __int_as_float(tk[2]), xi4.w);
+    return;
   }
   bool sat;
   const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);
@@ -996,6 +924,84 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   v.z = v.z + ch.z;
   S.v[p][s] = v;
   S.xs[c ^ 1][s] = make_float4(xi.x + ch.x * S.dt, xi.y + ch.y * S.dt, xi.z + ch.z * S.dt, xi4.w);
+}
+
+// Scenes that cut a list or have contacts (CT: some scene of the launch has
+// them) scatter pair halves to partners during this pass; a staged tile sums
+// the halves for its union's entries in LDS and adds each touched entry's
+// sums to the accumulator once at the end. k_impulse_apply applies once
+// every half has arrived. The moved marks (.w) stay.
+template <bool ST, bool ADH, bool CT>
+__global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
+                                                       const Tile* __restrict__ tiles,
+                                                       uint32_t ntiles, int fi, int p, int c) {
+  __shared__ int acc_l[ST ? 3 * PF_TU_MAX : 1];
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  const uint32_t tile = t.first / PF_BLOCK;
+  // Uniform over the block.
+  const bool st_on = ST && (F.effects & PF_FX_SURFACE_TENSION);
+  const bool trunc_scene = st_on && S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
+  const bool ct_scene = CT && st_on && S.ct.on;
+  const bool special = trunc_scene || ct_scene;
+  const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
+  const bool lds = ST && special && S.H >= 64u && U != PF_TU_GLOBAL;
+  if (lds) {
+    for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
+    __syncthreads();
+  }
+  const uint32_t s = t.first + threadIdx.x;
+  if (s < S.n) impulse_particle<ST, ADH, CT>(S, F, p, c, s, tile, special, ct_scene, lds ? acc_l : nullptr);
+  if (lds) {
+    __syncthreads();
+    const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) {
+      const int a0 = acc_l[3 * u], a1 = acc_l[3 * u + 1], a2 = acc_l[3 * u + 2];
+      if ((a0 | a1 | a2) != 0) {
+        int* a = reinterpret_cast<int*>(S.ct.acc + uslots[u]);
+        atomicAdd(a + 0, a0);
+        atomicAdd(a + 1, a1);
+        atomicAdd(a + 2, a2);
+      }
+    }
+  }
+}
+
+// The impulse apply (FluidSystem.ts:253-258, accumulator.ts:101-131) of the
+// fluid particles of scenes whose surface tension went through the
+// accumulator in k_impulses: the particle's own sum (parked in its output
+// prediction slot) plus the halves scattered to it; v += change,
+// x* += change dt, sums cleared (moved marks kept). Other slots and scenes
+// were applied by k_impulses.
+__global__ __launch_bounds__(PF_BLOCK) void k_impulse_apply(const SceneDev* __restrict__ scenes,
+                                                            const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                            int fi, int p, int c, int contact) {
+  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
+  const SceneDev& S = scenes[t.scene];
+  const FluidDev& F = S.fluid[fi];
+  if (!(F.effects & PF_FX_SURFACE_TENSION)) return;
+  const bool trunc_scene = S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
+  if (!trunc_scene && !(contact && S.ct.on)) return;
+  const uint32_t s = t.first + threadIdx.x;
+  if (s >= S.n || !in_range(S.id[p][s], F.start, F.count)) return;
+  int* a = reinterpret_cast<int*>(S.ct.acc + s);
+  const float4 own = S.xs[c ^ 1][s];
+  int tk[3] = {pf_wrap_add(a[0], __float_as_int(own.x)), pf_wrap_add(a[1], __float_as_int(own.y)),
+               pf_wrap_add(a[2], __float_as_int(own.z))};
+  a[0] = 0;
+  a[1] = 0;
+  a[2] = 0;
+  bool sat;
+  const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);
+  if (sat) flag_set(S, 0, PF_FLAG_IMPULSES);
+  const float4 xi4 = S.xs[c][s];
+  float4 v = S.v[p][s];
+  v.x = v.x + ch.x;
+  v.y = v.y + ch.y;
+  v.z = v.z + ch.z;
+  S.v[p][s] = v;
+  S.xs[c ^ 1][s] = make_float4(xi4.x + ch.x * S.dt, xi4.y + ch.y * S.dt, xi4.z + ch.z * S.dt, xi4.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1766,10 +1772,10 @@ void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
   else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
 }
 
-void launch_cohesion_scatter(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                             int p, int c) {
+void launch_impulse_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                          int c, bool contact) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_cohesion_scatter, PF_GRID, sc, tl, ntiles, fi, p, c);
+  hipLaunchKernelGGL(k_impulse_apply, PF_GRID, sc, tl, ntiles, fi, p, c, contact ? 1 : 0);
 }
 
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,

@@ -22,8 +22,8 @@ void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, int c, bool surface_tension, bool adhesion, bool contact);
 // surface-tension pairs scattered into the contact accumulator (scenes with contacts)
-void launch_cohesion_scatter(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                             int p, int c);
+void launch_impulse_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
+                          int c, bool contact);
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                    int p, int c, uint32_t cap);
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,

@@ -151,6 +151,9 @@ struct Loop;
 }  // namespace pf
 
 struct pf_context_s : pf::Context {};
+struct pf_fence_s {
+  hipEvent_t ev = nullptr;
+};
 struct pf_particles_s : pf::Particles {};
 
 struct pf_grid_s {

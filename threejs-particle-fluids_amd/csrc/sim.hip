@@ -142,15 +142,17 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
     for (uint32_t f = 0; f < sh.nfluids; ++f) {
       const bool stx = (sh.fx[f] & PF_FX_SURFACE_TENSION) != 0;
       const bool adh = (sh.fx[f] & PF_FX_ADHESION) != 0;
-      if (stx) {
-        PF_LAUNCH(ctx, "cohesion_scatter");
-        launch_cohesion_scatter(st, scenes, tiles, ntiles, (int)f, p, c);
-        ++nk;
-      }
       if (stx || adh) {
-        PF_LAUNCH(ctx, "impulses");
-        launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary, sh.contact);
-        ++nk;
+        {
+          PF_LAUNCH(ctx, "impulses");
+          launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary, sh.contact);
+          ++nk;
+        }
+        if (stx) {
+          PF_LAUNCH(ctx, "impulse_apply");
+          launch_impulse_apply(st, scenes, tiles, ntiles, (int)f, p, c, sh.contact);
+          ++nk;
+        }
         c ^= 1;
       }
     }
@@ -952,6 +954,8 @@ int pf_softbody_create(pf_particles* ps, const pf_softbody_def* defs, uint32_t n
       if (used[i]) fail(PF_EINVAL, "SoftbodySystem: particle %u belongs to two bodies", i);
       used[i] = 1;
     }
+    if (d.rest_positions && (uint64_t)d.rest_count != 3ull * d.count)
+      fail(PF_EINVAL, "SoftbodySystem: body %u needs %llu rest coordinates", b, 3ull * d.count);
     std::vector<float> restp(3 * (size_t)d.count);
     for (uint32_t k = 0; k < 3 * d.count; ++k)
       restp[k] = d.rest_positions ? d.rest_positions[k] : ps->cpu_positions[3 * (size_t)d.start + k];
@@ -1585,7 +1589,12 @@ static void loop_members_alive(pf_loop* L) {
     if (c->disposed) fail(PF_ESTATE, "SimLoop: a collider has been disposed");
 }
 
+// Every graph of the loop is dropped; a replay still running on the stream
+// finishes first, since an executable graph must not be destroyed in flight.
 static void loop_invalidate_graphs(pf_loop* L) {
+  bool any = false;
+  for (int v = 0; v < 4; ++v) any = any || L->graph[v];
+  if (any) PF_HIP(hipStreamSynchronize(L->ctx->stream));
   for (int v = 0; v < 4; ++v) {
     if (L->graph[v]) (void)hipGraphExecDestroy(L->graph[v]);
     L->graph[v] = nullptr;
@@ -1736,6 +1745,63 @@ int pf_loop_synchronize(pf_loop* L) {
   PF_API_BEGIN
   loop_alive(L, "SimLoop");
   PF_HIP(hipStreamSynchronize(L->ctx->stream));
+  PF_API_END
+}
+
+// Completion fences: an event recorded on the loop's stream behind the work
+// enqueued so far. Waiting on one touches neither the loop nor its stream,
+// so a host thread may wait while the loop's owner enqueues (or captures)
+// more work.
+static pf_fence* fence_on(Context* ctx) {
+  if (ctx->capturing) fail(PF_ESTATE, "SimLoop: no fence while a step is being captured");
+  hipEvent_t ev = nullptr;
+  PF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const hipError_t e = hipEventRecord(ev, ctx->stream);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(ev);
+    PF_HIP(e);
+  }
+  auto* f = new pf_fence_s();
+  f->ev = ev;
+  return f;
+}
+
+int pf_loop_fence(pf_loop* L, pf_fence** out) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop");
+  if (!out) fail(PF_EINVAL, "SimLoop: null handle");
+  *out = fence_on(L->ctx);
+  PF_API_END
+}
+
+int pf_context_fence(pf_context* ctx, pf_fence** out) {
+  PF_API_BEGIN
+  if (!ctx || !out) fail(PF_EINVAL, "pf_context_fence: null handle");
+  *out = fence_on(ctx);
+  PF_API_END
+}
+
+int pf_fence_wait(pf_fence* f) {
+  PF_API_BEGIN
+  if (!f) fail(PF_EINVAL, "fence: null handle");
+  PF_HIP(hipEventSynchronize(f->ev));
+  PF_API_END
+}
+
+int pf_fence_query(pf_fence* f, int* done) {
+  PF_API_BEGIN
+  if (!f || !done) fail(PF_EINVAL, "fence: null handle");
+  const hipError_t e = hipEventQuery(f->ev);
+  if (e != hipSuccess && e != hipErrorNotReady) PF_HIP(e);
+  *done = e == hipSuccess ? 1 : 0;
+  PF_API_END
+}
+
+int pf_fence_destroy(pf_fence* f) {
+  PF_API_BEGIN
+  if (!f) return PF_OK;
+  (void)hipEventDestroy(f->ev);
+  delete f;
   PF_API_END
 }
 
@@ -1958,7 +2024,11 @@ int pf_batch_step(pf_batch* B, double dt) {
   // A changed tile-union limit changes the launches' LDS capacity.
   if (std::memcmp(now.tu_cap, B->shape.tu_cap, sizeof now.tu_cap) != 0) {
     std::memcpy(B->shape.tu_cap, now.tu_cap, sizeof now.tu_cap);
-    if (B->graph) { (void)hipGraphExecDestroy(B->graph); B->graph = nullptr; }
+    if (B->graph) {
+      PF_HIP(hipStreamSynchronize(ctx->stream));  // not destroyed while a replay runs
+      (void)hipGraphExecDestroy(B->graph);
+      B->graph = nullptr;
+    }
     B->graph_valid = false;
   }
   if (changed || B->desc_dirty) {
@@ -2095,7 +2165,11 @@ int pf_batch_set_graph(pf_batch* B, int enabled) {
   PF_API_BEGIN
   if (!B) fail(PF_EINVAL, "Batch: null handle");
   B->use_graph = enabled != 0;
-  if (B->graph) { (void)hipGraphExecDestroy(B->graph); B->graph = nullptr; }
+  if (B->graph) {
+    PF_HIP(hipStreamSynchronize(B->ctx->stream));  // not destroyed while a replay runs
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+  }
   B->graph_valid = false;
   PF_API_END
 }

@@ -10,6 +10,8 @@
 #include <node_api.h>
 
 #include <algorithm>
+#include <cstdarg>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -83,24 +85,63 @@ uint32_t u32(napi_env env, napi_value v, uint32_t dflt = 0) {
   return (uint32_t)d;
 }
 
+// The TypedArray kind each element type is read from.
+template <typename T>
+struct TypedKind;
+template <>
+struct TypedKind<float> {
+  static bool ok(napi_typedarray_type t) { return t == napi_float32_array; }
+  static constexpr const char* name = "a Float32Array";
+};
+template <>
+struct TypedKind<uint32_t> {
+  static bool ok(napi_typedarray_type t) { return t == napi_uint32_array; }
+  static constexpr const char* name = "a Uint32Array";
+};
+template <>
+struct TypedKind<uint8_t> {
+  static bool ok(napi_typedarray_type t) { return t == napi_uint8_array || t == napi_uint8_clamped_array; }
+  static constexpr const char* name = "a Uint8Array";
+};
+
 // Pointer into a TypedArray (null for null/undefined); *count = element count.
+// Any other value, or a TypedArray of another element type, throws a
+// TypeError and returns null (check pending() before using the result).
 template <typename T>
 T* typed(napi_env env, napi_value v, size_t* count = nullptr) {
-  if (is_nullish(env, v)) {
-    if (count) *count = 0;
-    return nullptr;
-  }
+  if (count) *count = 0;
+  if (is_nullish(env, v)) return nullptr;
   napi_typedarray_type type;
   size_t length = 0;
   void* data = nullptr;
   napi_value buf;
   size_t offset = 0;
-  if (napi_get_typedarray_info(env, v, &type, &length, &data, &buf, &offset) != napi_ok) {
-    napi_throw_type_error(env, nullptr, "pf addon: expected a TypedArray");
+  bool is_typed = false;
+  napi_is_typedarray(env, v, &is_typed);
+  if (!is_typed || napi_get_typedarray_info(env, v, &type, &length, &data, &buf, &offset) != napi_ok ||
+      !TypedKind<T>::ok(type)) {
+    std::string msg = std::string("pf addon: expected ") + TypedKind<T>::name;
+    napi_throw_type_error(env, nullptr, msg.c_str());
     return nullptr;
   }
   if (count) *count = length;
   return static_cast<T*>(data);
+}
+
+bool pending(napi_env env) {
+  bool p = false;
+  napi_is_exception_pending(env, &p);
+  return p;
+}
+
+napi_value throw_range(napi_env env, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  napi_throw_range_error(env, nullptr, buf);
+  return nullptr;
 }
 
 // A JS array of numbers into doubles.
@@ -205,11 +246,24 @@ napi_value ParticlesDestroy(napi_env env, napi_callback_info info) {
 // (ps, start, positions3, velocities3|null, invMass|null, groups|null)
 napi_value ParticlesUpload(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
-  size_t n3 = 0;
+  size_t n3 = 0, nv = 0, nm = 0, ng = 0;
   float* pos = typed<float>(env, a.argv[2], &n3);
-  PF_CALL(pf_particles_upload((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]), (uint32_t)(n3 / 3), pos,
-                              typed<float>(env, a.argv[3]), typed<float>(env, a.argv[4]),
-                              typed<uint32_t>(env, a.argv[5])));
+  float* vel = pending(env) ? nullptr : typed<float>(env, a.argv[3], &nv);
+  float* inv = pending(env) ? nullptr : typed<float>(env, a.argv[4], &nm);
+  uint32_t* grp = pending(env) ? nullptr : typed<uint32_t>(env, a.argv[5], &ng);
+  if (pending(env)) return nullptr;
+  // Every array covers the same particles; the library reads count entries of each.
+  if (!pos || n3 % 3 != 0)
+    return throw_range(env, "ParticleSystem.uploadParticles: positions must hold xyz triples, got %zu values", n3);
+  const size_t n = n3 / 3;
+  if (vel && nv != 3 * n)
+    return throw_range(env, "ParticleSystem.uploadParticles: %zu velocity values for %zu particles", nv, n);
+  if (inv && nm != n)
+    return throw_range(env, "ParticleSystem.uploadParticles: %zu inverse masses for %zu particles", nm, n);
+  if (grp && ng != n)
+    return throw_range(env, "ParticleSystem.uploadParticles: %zu collision groups for %zu particles", ng, n);
+  PF_CALL(pf_particles_upload((pf_particles*)handle(env, a.argv[0]), u32(env, a.argv[1]), (uint32_t)n, pos, vel,
+                              inv, grp));
   return nullptr;
 }
 napi_value ParticlesSetInvMass(napi_env env, napi_callback_info info) {
@@ -462,10 +516,15 @@ napi_value SdfCreate(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   size_t nd = 0;
   const float* data = typed<float>(env, a.argv[1], &nd);
+  if (pending(env)) return nullptr;
   auto res = doubles(env, a.argv[2]), origin = doubles(env, a.argv[3]), voxel = doubles(env, a.argv[4]);
   if (res.size() != 3 || origin.size() != 3 || voxel.size() != 3) {
     napi_throw_error(env, nullptr, "SDFCollider: resolution, origin and voxelSize take three numbers");
     return nullptr;
+  }
+  if (!(res[0] >= 0 && res[1] >= 0 && res[2] >= 0) || (double)nd != res[0] * res[1] * res[2]) {
+    return throw_range(env, "SDFCollider: data holds %zu values, resolution %gx%gx%g needs %g", nd, res[0], res[1],
+                       res[2], res[0] * res[1] * res[2]);
   }
   const uint32_t r3[3] = {(uint32_t)res[0], (uint32_t)res[1], (uint32_t)res[2]};
   if ((size_t)r3[0] * r3[1] * r3[2] != nd) {
@@ -539,9 +598,13 @@ napi_value SoftbodyCreate(napi_env env, napi_callback_info info) {
     napi_value sc = prop(env, o, "surfaceCount");
     defs[b].surface_count = is_nullish(env, sc) ? 0xffffffffu : u32(env, sc);
     defs[b].compliance = num(env, prop(env, o, "compliance"), 0);
-    defs[b].rest_positions = typed<float>(env, prop(env, o, "restPositions"));
-    size_t ne = 0;
+    size_t nr = 0, ne = 0;
+    defs[b].rest_positions = typed<float>(env, prop(env, o, "restPositions"), &nr);
+    defs[b].rest_count = (uint32_t)nr;
+    if (pending(env)) return nullptr;
     defs[b].edges = typed<uint32_t>(env, prop(env, o, "edges"), &ne);
+    if (pending(env)) return nullptr;
+    if (ne % 2 != 0) return throw_range(env, "SoftbodySystem: body %u has an odd number of edge indices", b);
     defs[b].edge_count = (uint32_t)(ne / 2);
   }
   const uint32_t mode = u32(env, a.argv[2], PF_SHAPE_MATCHING_GLOBAL);
@@ -603,6 +666,7 @@ napi_value LoopCreate(napi_env env, napi_callback_info info) {
   napi_value ct = prop(env, opts, "contact");
   size_t nq = 0;
   const uint32_t* quiet = typed<uint32_t>(env, prop(env, opts, "quiet"), &nq);
+  if (pending(env)) return nullptr;
   if (!is_nullish(env, ct)) {
     o.contact = 1;
     o.contact_mu_s = num(env, prop(env, ct, "muS"), -1.0);
@@ -639,7 +703,7 @@ napi_value LoopStep(napi_env env, napi_callback_info info) {
 }
 
 struct StepWork {
-  pf_loop* loop;
+  pf_fence* fence;
   napi_deferred deferred;
   napi_async_work work;
   int rc;
@@ -647,8 +711,10 @@ struct StepWork {
 };
 
 void step_execute(napi_env, void* data) {
+  // Waits on the step's fence only: the loop and its stream stay the main
+  // thread's (it may step or dispose meanwhile).
   StepWork* w = static_cast<StepWork*>(data);
-  w->rc = pf_loop_synchronize(w->loop);
+  w->rc = pf_fence_wait(w->fence);
   if (w->rc != PF_OK) w->error = pf_last_error();
 }
 
@@ -665,6 +731,7 @@ void step_complete(napi_env env, napi_status, void* data) {
     napi_reject_deferred(env, w->deferred, err);
   }
   napi_delete_async_work(env, w->work);
+  pf_fence_destroy(w->fence);
   delete w;
 }
 
@@ -673,7 +740,9 @@ napi_value LoopStepAsync(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   pf_loop* L = (pf_loop*)handle(env, a.argv[0]);
   PF_CALL(pf_loop_step(L, num(env, a.argv[1])));
-  auto* w = new StepWork{L, nullptr, nullptr, PF_OK, {}};
+  pf_fence* fence = nullptr;
+  PF_CALL(pf_loop_fence(L, &fence));
+  auto* w = new StepWork{fence, nullptr, nullptr, PF_OK, {}};
   napi_value promise, name;
   CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
   napi_create_string_utf8(env, "pf_loop_step", NAPI_AUTO_LENGTH, &name);
@@ -710,6 +779,7 @@ napi_value SdfDecode(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   size_t n = 0;
   const uint8_t* b = typed<uint8_t>(env, a.argv[0], &n);
+  if (pending(env)) return nullptr;
   uint32_t res[3] = {0, 0, 0};
   double o[3], v[3];
   size_t count = 0;
@@ -731,6 +801,7 @@ napi_value SdfEncode(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   size_t count = 0;
   const float* d = typed<float>(env, a.argv[0], &count);
+  if (pending(env)) return nullptr;
   auto r = doubles(env, a.argv[1]), o = doubles(env, a.argv[2]), v = doubles(env, a.argv[3]);
   if (r.size() != 3 || o.size() != 3 || v.size() != 3) {
     napi_throw_type_error(env, nullptr, "encodeSdfBinary: resolution, origin and voxelSize need three numbers");
@@ -919,9 +990,10 @@ napi_value LoopAutotune(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------- batched form
-// Resolves a Promise from a worker thread once a context's stream is idle.
+// Resolves a Promise from a worker thread once the GPU has passed a fence
+// recorded on the context's stream (the worker never touches the stream).
 struct SyncWork {
-  pf_context* ctx;
+  pf_fence* fence;
   napi_deferred deferred;
   napi_async_work work;
   int rc;
@@ -929,7 +1001,9 @@ struct SyncWork {
 };
 
 napi_value promise_after_sync(napi_env env, pf_context* ctx) {
-  auto* w = new SyncWork{ctx, nullptr, nullptr, PF_OK, {}};
+  pf_fence* fence = nullptr;
+  PF_CALL(pf_context_fence(ctx, &fence));
+  auto* w = new SyncWork{fence, nullptr, nullptr, PF_OK, {}};
   napi_value promise, name;
   CHECK_NAPI(napi_create_promise(env, &w->deferred, &promise));
   napi_create_string_utf8(env, "pf_context_synchronize", NAPI_AUTO_LENGTH, &name);
@@ -937,7 +1011,7 @@ napi_value promise_after_sync(napi_env env, pf_context* ctx) {
       env, nullptr, name,
       [](napi_env, void* d) {
         auto* w = static_cast<SyncWork*>(d);
-        w->rc = pf_context_synchronize(w->ctx);
+        w->rc = pf_fence_wait(w->fence);
         if (w->rc != PF_OK) w->error = pf_last_error();
       },
       [](napi_env env, napi_status, void* d) {
@@ -953,6 +1027,7 @@ napi_value promise_after_sync(napi_env env, pf_context* ctx) {
           napi_reject_deferred(env, w->deferred, err);
         }
         napi_delete_async_work(env, w->work);
+        pf_fence_destroy(w->fence);
         delete w;
       },
       w, &w->work));

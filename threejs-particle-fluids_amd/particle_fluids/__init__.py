@@ -51,6 +51,26 @@ def _range(r, default_count):
     return int(r[0]), int(r[1])
 
 
+def upload_arrays(positions, velocities=None, invMass=None, collisionGroup=None):
+    """uploadParticles' arrays as the C ABI reads them: n xyz positions, n xyz
+    velocities, n inverse masses and n groups (one value broadcasts). Arrays
+    that do not cover the n particles raise ValueError, since the library
+    reads n entries of each."""
+    pos = _f32(positions).reshape(-1, 3)
+    n = pos.shape[0]
+    vel = None if velocities is None else _f32(velocities).reshape(-1, 3)
+    if vel is not None and vel.shape[0] != n:
+        raise ValueError(f"ParticleSystem.uploadParticles: {vel.shape[0]} velocities for {n} particles")
+    try:
+        im = None if invMass is None else _f32(np.broadcast_to(invMass, (n,)))
+        grp = None if collisionGroup is None else np.ascontiguousarray(np.broadcast_to(collisionGroup, (n,)),
+                                                                        dtype=np.uint32)
+    except ValueError:
+        raise ValueError(f"ParticleSystem.uploadParticles: invMass and collisionGroup need one value or {n} values") \
+            from None
+    return pos, vel, im, grp
+
+
 class Context:
     """One GPU and one HIP stream (replaces the WebGPU renderer handle)."""
 
@@ -129,12 +149,9 @@ class ParticleSystem:
         self.particleRadius = float(particleRadius)
 
     def uploadParticles(self, positions, velocities=None, invMass=None, collisionGroup=None, start=0):
-        pos = _f32(positions).reshape(-1, 3)
-        n = pos.shape[0]
-        vel = None if velocities is None else _f32(velocities).reshape(-1, 3)
-        im = None if invMass is None else _f32(np.broadcast_to(invMass, (n,)))
-        grp = None if collisionGroup is None else np.ascontiguousarray(np.broadcast_to(collisionGroup, (n,)), dtype=np.uint32)
-        call("pf_particles_upload", self.handle, int(start), n, _ptr(pos, f32p), _ptr(vel, f32p), _ptr(im, f32p), _ptr(grp, u32p))
+        pos, vel, im, grp = upload_arrays(positions, velocities, invMass, collisionGroup)
+        call("pf_particles_upload", self.handle, int(start), pos.shape[0], _ptr(pos, f32p), _ptr(vel, f32p),
+             _ptr(im, f32p), _ptr(grp, u32p))
 
     def setInvMass(self, range, invMass):
         s, c = _range(range, self.capacity)
@@ -556,6 +573,7 @@ class SoftbodySystem:
                 arr = _f32(rp).reshape(-1)
                 self._keep.append(arr)
                 defs[k].rest_positions = _ptr(arr, f32p)
+                defs[k].rest_count = arr.size
             ed = b.get("edges")
             if ed is not None:
                 e = np.ascontiguousarray(ed, np.uint32).reshape(-1)

@@ -63,6 +63,7 @@ class SoftbodyDef(C.Structure):
         ("rest_positions", P(f32)),
         ("edges", P(u32)),
         ("edge_count", u32),
+        ("rest_count", u32),
     ]
 
 
@@ -179,6 +180,11 @@ SIGNATURES = {
     "pf_grid_set_origin": [vp, P(f64)],
     "pf_grid_query_counts": [vp, P(f32), u32, f64, P(u32), P(u32)],
     "pf_grid_destroy": [vp],
+    "pf_loop_fence": [vp, P(vp)],
+    "pf_context_fence": [vp, P(vp)],
+    "pf_fence_wait": [vp],
+    "pf_fence_query": [vp, P(i32)],
+    "pf_fence_destroy": [vp],
     "pf_fluid_create": [vp, P(FluidOptions), P(vp)],
     "pf_fluid_set": [vp, u32, f64],
     "pf_fluid_get": [vp, u32, P(f64)],
