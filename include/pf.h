@@ -4,7 +4,8 @@
  * The ABI is the drop-in boundary for threejs-particle-fluids' per-substep
  * solver: each entry point replaces one reference interface, cited as
  * path:line in the reference repository. Plain pointers and sizes only: no
- * device pointers, streams or torch types cross it. Every call returns 0 on
+ * streams or torch types cross it, and no device pointers except the
+ * buffers of pf_grid_sort_noalloc, whose callers hold their data on the GPU. Every call returns 0 on
  * success or a PF_E* code; pf_last_error() (thread-local) then holds the
  * message, worded as the reference's "<Owner>: <reason>" errors so a host
  * facade can rethrow it unchanged.
@@ -139,6 +140,27 @@ int pf_grid_query_counts(pf_grid* grid, const float* query4, uint32_t nquery, do
                          uint32_t* candidates, uint32_t* within);
 int pf_grid_destroy(pf_grid* grid);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the standalone
+ * HashGrid rebuild + readback (pf_grid_create_standalone, _upload_positions,
+ * _rebuild, _readback, _readback_sorted_keys, _readback_overflow) as one call
+ * that allocates nothing. Every pointer but origin is DEVICE memory on the
+ * context's GPU: positions4 [n] vec4 (16-byte aligned); outputs cell_index,
+ * sorted_indices, sorted_keys [n] and counts, cell_start, cell_end [Hpad]
+ * (hashTableSizePadded); overflow [1] nullable. The work is enqueued on the
+ * context's stream and the call returns without waiting
+ * (pf_context_synchronize waits). Its scratch is one fixed device array of
+ * PF_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and shared by
+ * every call on that GPU: calls that overlap on one GPU race. Sizes above
+ * PF_NOALLOC_MAX_PARTICLES positions or PF_NOALLOC_MAX_BUCKETS buckets fail
+ * with PF_EINVAL. Results equal the standalone grid's bit for bit. */
+#define PF_NOALLOC_MAX_PARTICLES (1u << 22)
+#define PF_NOALLOC_MAX_BUCKETS (1u << 22)
+#define PF_NOALLOC_SCRATCH_BYTES 50482688u
+int pf_grid_sort_noalloc(pf_context* ctx, const float* positions4, uint32_t n, double cell_size,
+                         uint32_t hash_table_size, const double origin[3], uint32_t* cell_index,
+                         uint32_t* counts, uint32_t* cell_start, uint32_t* cell_end,
+                         uint32_t* sorted_indices, uint32_t* sorted_keys, uint32_t* overflow);
 
 /* ---- FluidSystem (src/fluids/FluidSystem.ts:71-322) ---------------------- */
 #define PF_FLUID_VISCOSITY 1u

@@ -185,6 +185,7 @@ SIGNATURES = {
     "pf_fence_wait": [vp],
     "pf_fence_query": [vp, P(i32)],
     "pf_fence_destroy": [vp],
+    "pf_grid_sort_noalloc": [vp, vp, u32, f64, u32, P(f64), vp, vp, vp, vp, vp, vp, vp],
     "pf_fluid_create": [vp, P(FluidOptions), P(vp)],
     "pf_fluid_set": [vp, u32, f64],
     "pf_fluid_get": [vp, u32, P(f64)],
