@@ -500,6 +500,28 @@ def run_served(args, ctx, tile_union=None):
         srv.removeScene(sc["id"])
     srv.dispose()
 
+    # The same scenes under a budget below their resident size: admission
+    # evicts to host memory, and the stream restores and evicts as it goes.
+    evicting = None
+    if args.serve_evict_frac > 0:
+        budget = int(st0["resident_bytes"] * args.serve_evict_frac)
+        srv = pf.Server(context=ctx, hbmBudget=budget, slots=args.serve_slots)
+        for sc in built:
+            sc["id"] = srv.addScene(sc["loop"])
+        ste = srv.stats()
+        print(f"[bench] served, evicting: budget {budget / 2**30:.1f} GiB, {ste['evictions']} evictions at "
+              f"admission, {ste['host_bytes'] / 2**30:.1f} GiB in host memory", file=sys.stderr, flush=True)
+        poisson_e = run_stream(args.serve_requests, args.serve_rate, args.serve_seed + 5)
+        st1 = srv.stats()
+        evicting = {"hbm_budget_gib": round(budget / 2**30, 2), "poisson": poisson_e,
+                    "evictions": st1["evictions"], "restores": st1["restores"],
+                    "evict_failures": st1["evict_failures"], "host_gib": round(st1["host_bytes"] / 2**30, 2),
+                    "peak_resident_gib": round(st1["peak_resident_bytes"] / 2**30, 2)}
+        print(f"[bench] served, evicting: {evicting}", file=sys.stderr, flush=True)
+        for sc in built:
+            srv.removeScene(sc["id"])
+        srv.dispose()
+
     # plain form: the same kind of stream, one loop step per request
     r = np.random.default_rng(args.serve_seed + 4)
     targets = r.integers(0, len(built), args.serve_plain_requests)
@@ -528,6 +550,7 @@ def run_served(args, ctx, tile_union=None):
         "slots": args.serve_slots, "tile_union_capacity": tile_union, "poisson": poisson, "saturated": saturated, "plain": plain,
         "speedup_saturated_vs_plain": round(saturated["particle_substeps_per_s"] / plain["particle_substeps_per_s"], 3),
         "launches": st["launches"], "evictions": st["evictions"], "restores": st["restores"],
+        "evicting": evicting,
     }
 
 
@@ -558,6 +581,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-seed", type=int, default=0x5E7E)
+    ap.add_argument("--serve-evict-frac", type=float, default=0.6,
+                    help="then serve the same scenes under this fraction of their resident bytes (0: skip)")
     ap.add_argument("--field-updates", type=int, default=20, help="surface-field updates timed (0: skip)")
     ap.add_argument("--viscosity-steps", type=int, default=20, help="thick-liquid steps timed (0: skip)")
     ap.add_argument("--viscosity-sweeps", type=int, default=12)

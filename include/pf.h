@@ -439,6 +439,8 @@ typedef struct pf_server_options {
   uint32_t max_batch_scenes;     /* scenes per launch; 0 = 256 */
   uint64_t max_batch_particles;  /* particles per launch; 0 = no cap */
   uint32_t slots;                /* launches in flight at once; 0 = 2 */
+  uint64_t host_budget_bytes;    /* host bytes evicted scenes may hold; 0 = no cap. An
+                                    eviction beyond it is refused: the scene stays resident */
 } pf_server_options;
 typedef struct pf_server_stats {
   uint64_t submitted, completed, failed;
@@ -449,6 +451,8 @@ typedef struct pf_server_stats {
   uint64_t resident_bytes, peak_resident_bytes, hbm_budget_bytes;
   uint32_t scenes, resident_scenes;
   uint32_t max_launch_scenes;    /* largest launch so far */
+  uint64_t host_bytes;           /* host memory evicted scenes hold now */
+  uint64_t evict_failures;       /* evictions refused (host budget, host memory) */
 } pf_server_stats;
 /* Times of one request in seconds on the server's clock: submitted,
  * launched, finished (0 while not reached). status: 0 pending, 1 launched,

@@ -45,8 +45,11 @@ def test_one_substep_vs_fp64_oracle():
     err = compare(gpu_state(sc), o.get())
     print("[pbf] rel err vs FP64 oracle", err, flush=True)
     assert err["x"] <= TOL and err["xs"] <= TOL and err["rho"] <= TOL
-    # velocity = (x* - x) / dt amplifies FP32 position rounding
-    assert err["v"] <= 1e-3
+    # Measured: x 1.41e-7, v 4.26e-5 (7.9e-6 m/s against |v| up to 0.19 m/s).
+    # advect's v = (x* - x) / dt turns one FP32 ulp of a position near 0.5 m
+    # (6e-8 m) into 7.2e-6 m/s at dt = 1/120 s, so no FP32 evaluation of the
+    # step gets closer than about one such ulp; the bound sits just above it.
+    assert err["v"] <= 5e-5
 
 
 def diagnostics(x, v, rho, m):

@@ -129,3 +129,47 @@ def test_scene_larger_than_budget_fails_its_request():
     plain["loop"].step(DT)
     assert_same(state(sc), state(plain), "after failed request")
     srv.dispose()
+
+
+def test_refused_eviction_keeps_scenes_and_serving():
+    # Host memory for evicted scenes is capped (hostBudget): an eviction past
+    # it is refused, the scene stays resident and intact, the request that
+    # needed the room fails alone, and serving goes on.
+    spec = [(3000, 0), (3000, 0), (3000, 0)]
+    alone, served = twins(spec, seed0=400)
+    sizes = [s["ps"].deviceBytes() for s in served]
+    budget = 2 * max(sizes) + max(sizes) // 2
+    # the host bytes one evicted scene takes, from a server without a host cap
+    probe = pf.Server(hbmBudget=budget, slots=2)
+    pids = [probe.addScene(s["loop"]) for s in served]
+    one = probe.stats()["host_bytes"]
+    for i in pids:
+        probe.removeScene(i)
+    probe.dispose()
+    host = one + one // 2  # room in host memory for one evicted scene
+    print(f"[serve] refused eviction: budget {budget}, host budget {host}, scene bytes {sizes}", flush=True)
+    srv = pf.Server(hbmBudget=budget, hostBudget=host, slots=2)
+    ids = [srv.addScene(s["loop"]) for s in served]
+    st0 = srv.stats()
+    assert st0["evictions"] == 1 and st0["resident_scenes"] == 2 and st0["host_bytes"] == one
+    srv.wait(srv.submit(ids[1], DT))
+    srv.wait(srv.submit(ids[2], DT))
+    t0 = srv.submit(ids[0], DT)  # needs an eviction the host budget refuses
+    with pytest.raises(pf.PFError, match="host memory"):
+        srv.wait(t0)
+    with pytest.raises(pf.PFError, match="host memory"):
+        srv.evictScene(ids[1])
+    srv.wait(srv.submit(ids[1], DT))  # still serving
+    st = srv.stats()
+    print(f"[serve] stats {st}", flush=True)
+    assert st["failed"] == 1 and st["completed"] == 3 and st["evict_failures"] >= 2
+    assert st["peak_resident_bytes"] <= budget
+    for i in ids:
+        srv.removeScene(i)
+    steps = [0, 2, 1]
+    for s, k in zip(alone, steps):
+        for _ in range(k):
+            s["loop"].step(DT)
+    for k, (a, b) in enumerate(zip(alone, served)):
+        assert_same(state(a), state(b), k)
+    srv.dispose()

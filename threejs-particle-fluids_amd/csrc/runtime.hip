@@ -41,6 +41,8 @@ static std::unordered_set<void*> g_malloced;  // buffers dfree may hand to hipFr
 static thread_local Arena* g_arena = nullptr;
 
 static constexpr size_t kArenaMinChunk = 4u << 20;
+static constexpr size_t kArenaBytesPerParticle = 1024;
+static constexpr size_t kArenaPage = 2u << 20;
 static constexpr size_t kArenaAlign = 256;
 // Mapped tail no buffer uses: reads a little past a buffer's end stay inside
 // mapped memory, as they do with hipMalloc's rounded allocations.
@@ -54,32 +56,36 @@ static hipMemAllocationProp arena_prop(int device) {
   return prop;
 }
 
-static void arena_map(Arena::Chunk& c, int device) {
+// Backs a segment's whole address range with one new physical allocation.
+static void arena_map(Arena::Segment& g, int device) {
   const hipMemAllocationProp prop = arena_prop(device);
-  hipError_t e = hipMemCreate(&c.handle, c.size, &prop, 0);
-  if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", c.size, hipGetErrorString(e));
-  e = hipMemMap(c.va, c.size, 0, c.handle, 0);
+  hipError_t e = hipMemCreate(&g.handle, g.size, &prop, 0);
+  if (e != hipSuccess) fail(PF_ENOMEM, "device allocation of %zu bytes failed: %s", g.size, hipGetErrorString(e));
+  e = hipMemMap(g.base, g.size, 0, g.handle, 0);
   if (e != hipSuccess) {
-    (void)hipMemRelease(c.handle);
-    PF_HIP(e);
+    (void)hipMemRelease(g.handle);
+    fail(PF_EDEVICE, "HIP: mapping %zu bytes at %p failed: %s", g.size, (void*)g.base, hipGetErrorString(e));
   }
   hipMemAccessDesc acc{};
   acc.location = prop.location;
   acc.flags = hipMemAccessFlagsProtReadWrite;
-  e = hipMemSetAccess(c.va, c.size, &acc, 1);
+  e = hipMemSetAccess(g.base, g.size, &acc, 1);
   if (e != hipSuccess) {
-    (void)hipMemUnmap(c.va, c.size);
-    (void)hipMemRelease(c.handle);
-    PF_HIP(e);
+    (void)hipMemUnmap(g.base, g.size);
+    (void)hipMemRelease(g.handle);
+    fail(PF_EDEVICE, "HIP: access to %zu bytes at %p failed: %s", g.size, (void*)g.base, hipGetErrorString(e));
   }
 }
 
-Arena::Arena(int dev) : device(dev) {
+static void arena_unmap(Arena::Segment& g) {
+  (void)hipMemUnmap(g.base, g.size);
+  (void)hipMemRelease(g.handle);
+}
+
+Arena::Arena(int dev, size_t hint_bytes) : device(dev), hint(hint_bytes) {
   int vmm = 0;
   PF_HIP(hipDeviceGetAttribute(&vmm, hipDeviceAttributeVirtualMemoryManagementSupported, dev));
   if (!vmm) fail(PF_EDEVICE, "ParticleSystem: device %d has no virtual memory management; evictable scenes need it", dev);
-  // Chunks are sized and their address ranges aligned at the recommended
-  // granularity, which lets the runtime back them with its large pages.
   const hipMemAllocationProp prop = arena_prop(dev);
   PF_HIP(hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended));
   if (gran == 0) PF_HIP(hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum));
@@ -87,79 +93,176 @@ Arena::Arena(int dev) : device(dev) {
 }
 
 Arena::~Arena() {
-  for (auto& c : chunks) {
-    if (resident) {
-      (void)hipMemUnmap(c.va, c.size);
-      (void)hipMemRelease(c.handle);
-    }
-    if (c.host) (void)hipHostFree(c.host);
-    (void)hipMemAddressFree(c.va, c.size);
+  for (auto& g : segs) {
+    if (resident) arena_unmap(g);
+    std::free(g.host);
+    (void)hipMemAddressFree(g.base, g.size);
   }
 }
 
+// Evicted chunks live in pageable host memory; they move through a fixed
+// ring of two pinned staging buffers per device (allocated once), so the
+// pinned memory an eviction needs does not grow with the scenes evicted.
+// One piece's DMA overlaps the host copy of the piece before it.
+namespace {
+struct StagingRing {
+  static constexpr size_t kBytes = 32u << 20;
+  std::mutex mu;
+  char* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipStream_t stream = nullptr;
+
+  void init() {
+    if (stream) return;
+    for (int k = 0; k < 2; ++k) {
+      PF_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf[k]), kBytes, hipHostMallocDefault));
+      PF_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+    }
+    PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
+  // Device -> pageable host.
+  void download(char* dst, const char* src, size_t n) {
+    size_t off = 0, pend_off = 0, pend_n = 0;
+    int slot = 0, pend = -1;
+    while (off < n || pend >= 0) {
+      int issued = -1;
+      size_t m = 0;
+      if (off < n) {
+        m = std::min(kBytes, n - off);
+        PF_HIP(hipMemcpyAsync(buf[slot], src + off, m, hipMemcpyDeviceToHost, stream));
+        PF_HIP(hipEventRecord(ev[slot], stream));
+        issued = slot;
+      }
+      if (pend >= 0) {
+        PF_HIP(hipEventSynchronize(ev[pend]));
+        std::memcpy(dst + pend_off, buf[pend], pend_n);
+      }
+      pend = issued;
+      pend_off = off;
+      pend_n = m;
+      off += m;
+      slot ^= 1;
+    }
+  }
+  // Pageable host -> device; returns once the device holds the data.
+  void upload(char* dst, const char* src, size_t n) {
+    int slot = 0;
+    for (size_t off = 0; off < n; off += kBytes) {
+      const size_t m = std::min(kBytes, n - off);
+      PF_HIP(hipEventSynchronize(ev[slot]));  // the copy that last read this buffer is done
+      std::memcpy(buf[slot], src + off, m);
+      PF_HIP(hipMemcpyAsync(dst + off, buf[slot], m, hipMemcpyHostToDevice, stream));
+      PF_HIP(hipEventRecord(ev[slot], stream));
+      slot ^= 1;
+    }
+    PF_HIP(hipStreamSynchronize(stream));
+  }
+};
+
+std::mutex g_ring_mu;
+std::map<int, std::unique_ptr<StagingRing>> g_rings;
+
+StagingRing& ring_for(int device) {
+  std::lock_guard<std::mutex> lk(g_ring_mu);
+  auto& r = g_rings[device];
+  if (!r) r.reset(new StagingRing());
+  return *r;
+}
+}  // namespace
+
+// Buffers are bump-allocated in segments. The first is sized by the
+// scene's expected bytes, each later one at least a quarter of what the
+// arena holds already, so a scene holds few physical allocations (the device
+// caps their number at about 32k per process, far below its memory: 32k
+// chunks of 4 MiB stop at 128 GiB).
 void* Arena::alloc(size_t bytes) {
   if (!resident) fail(PF_ESTATE, "ParticleSystem: the scene is evicted to host memory");
   bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
-  if (!chunks.empty() && chunks.back().size - chunks.back().used >= bytes + kArenaTail) {
-    Chunk& c = chunks.back();
-    void* p = c.va + c.used;
-    c.used += bytes;
+  if (!segs.empty() && segs.back().size - segs.back().used >= bytes + kArenaTail) {
+    Segment& g = segs.back();
+    void* p = g.base + g.used;
+    g.used += bytes;
     return p;
   }
-  Chunk c;
-  c.size = (std::max(bytes + kArenaTail, kArenaMinChunk) + gran - 1) / gran * gran;
+  Segment g;
+  const size_t want = std::max({bytes + kArenaTail, kArenaMinChunk, segs.empty() ? hint : this->bytes() / 4});
+  // whole 2 MiB pages: the device backs large allocations with them
+  const size_t page = std::max<size_t>(gran, kArenaPage);
+  g.size = (want + page - 1) / page * page;
   void* va = nullptr;
-  hipError_t e = hipMemAddressReserve(&va, c.size, gran, nullptr, 0);
-  if (e != hipSuccess) fail(PF_ENOMEM, "device address reservation of %zu bytes failed: %s", c.size, hipGetErrorString(e));
-  c.va = static_cast<char*>(va);
+  const hipError_t e = hipMemAddressReserve(&va, g.size, gran, nullptr, 0);
+  if (e != hipSuccess) fail(PF_ENOMEM, "device address reservation of %zu bytes failed: %s", g.size, hipGetErrorString(e));
+  g.base = static_cast<char*>(va);
   try {
-    arena_map(c, device);
+    arena_map(g, device);
   } catch (...) {
-    (void)hipMemAddressFree(c.va, c.size);
+    (void)hipMemAddressFree(g.base, g.size);
     throw;
   }
-  c.used = bytes;
-  chunks.push_back(c);
-  return c.va;
+  g.used = bytes;
+  segs.push_back(g);
+  return g.base;
 }
 
 size_t Arena::bytes() const {
   size_t b = 0;
-  for (const auto& c : chunks) b += c.size;
+  for (const auto& g : segs) b += g.size;
   return b;
 }
 
+size_t Arena::used_bytes() const {
+  size_t b = 0;
+  for (const auto& g : segs) b += g.used;
+  return b;
+}
+
+// Copies each segment's used bytes to pageable host memory, then releases
+// the physical memory. Any failure (no host memory, a copy error) frees the
+// partial copies and leaves the scene resident and intact.
 void Arena::evict() {
   if (!resident) return;
-  for (auto& c : chunks) {
-    if (!c.host) PF_HIP(hipHostMalloc(&c.host, std::max<size_t>(c.used, 1), hipHostMallocDefault));
-    PF_HIP(hipMemcpy(c.host, c.va, c.used, hipMemcpyDeviceToHost));
+  std::vector<char*> copies(segs.size(), nullptr);
+  try {
+    for (size_t k = 0; k < segs.size(); ++k) {
+      copies[k] = static_cast<char*>(std::malloc(std::max<size_t>(segs[k].used, 1)));
+      if (!copies[k]) fail(PF_ENOMEM, "Server: no host memory for an evicted scene's %zu bytes", segs[k].used);
+    }
+    StagingRing& R = ring_for(device);
+    std::lock_guard<std::mutex> lk(R.mu);
+    R.init();
+    for (size_t k = 0; k < segs.size(); ++k) R.download(copies[k], segs[k].base, segs[k].used);
+  } catch (...) {
+    for (char* p : copies) std::free(p);
+    throw;
   }
-  for (auto& c : chunks) {
-    PF_HIP(hipMemUnmap(c.va, c.size));
-    PF_HIP(hipMemRelease(c.handle));
+  for (size_t k = 0; k < segs.size(); ++k) {
+    segs[k].host = copies[k];
+    arena_unmap(segs[k]);
   }
   resident = false;
 }
 
+// Backs each segment with a fresh physical allocation at the same addresses
+// and copies the host copies back. On failure every new mapping is released
+// and the scene stays evicted, its host copies intact.
 void Arena::restore() {
   if (resident) return;
   size_t mapped = 0;
   try {
-    for (; mapped < chunks.size(); ++mapped) arena_map(chunks[mapped], device);
+    for (; mapped < segs.size(); ++mapped) arena_map(segs[mapped], device);
+    StagingRing& R = ring_for(device);
+    std::lock_guard<std::mutex> lk(R.mu);
+    R.init();
+    for (auto& g : segs) R.upload(g.base, static_cast<const char*>(g.host), g.used);
   } catch (...) {
-    for (size_t k = 0; k < mapped; ++k) {
-      (void)hipMemUnmap(chunks[k].va, chunks[k].size);
-      (void)hipMemRelease(chunks[k].handle);
-    }
+    for (size_t k = 0; k < mapped; ++k) arena_unmap(segs[k]);
     throw;
   }
-  resident = true;
-  for (auto& c : chunks) {
-    PF_HIP(hipMemcpy(c.va, c.host, c.used, hipMemcpyHostToDevice));
-    PF_HIP(hipHostFree(c.host));
-    c.host = nullptr;
+  for (auto& g : segs) {
+    std::free(g.host);
+    g.host = nullptr;
   }
+  resident = true;
 }
 
 ArenaScope::ArenaScope(Arena* arena) : prev(g_arena) { g_arena = arena; }
@@ -501,7 +604,10 @@ static pf_particles* particles_new(pf_context* ctx, uint32_t capacity, double ra
     fail(PF_EINVAL, "ParticleSystem: particleRadius must be a positive finite number, got %g", radius);
   std::unique_ptr<pf_particles_s> owner(new pf_particles_s());
   pf_particles_s* ps = owner.get();
-  if (evictable) ps->arena.reset(new Arena(ctx->device));
+  // The arena's first segment is sized for the scene a fluid of this many
+  // particles builds (state, sort, neighbour lists, tiles, contact records:
+  // about 1 KB each).
+  if (evictable) ps->arena.reset(new Arena(ctx->device, (size_t)capacity * kArenaBytesPerParticle));
   ArenaScope scope(ps->arena.get());
   ps->ctx = ctx;
   ps->n = capacity;

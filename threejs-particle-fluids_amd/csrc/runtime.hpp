@@ -46,20 +46,24 @@ struct PfError {
 // valid across an evict/restore round trip. Buffers are bump-allocated and
 // released together with the arena.
 struct Arena {
-  struct Chunk {
-    char* va = nullptr;
+  // A range of reserved device addresses backed, while resident, by one
+  // physical allocation of the same size; buffers are bump-allocated in it.
+  struct Segment {
+    char* base = nullptr;
     size_t size = 0, used = 0;
     hipMemGenericAllocationHandle_t handle{};
-    void* host = nullptr;  // pinned copy while evicted
+    void* host = nullptr;  // pageable copy of the used bytes while evicted
   };
   int device = 0;
   size_t gran = 0;
-  std::vector<Chunk> chunks;
+  size_t hint = 0;  // expected bytes of the scene (sizes its first segment)
+  std::vector<Segment> segs;
   bool resident = true;
-  explicit Arena(int device);
+  explicit Arena(int device, size_t hint_bytes = 0);
   ~Arena();
   void* alloc(size_t bytes);
-  size_t bytes() const;  // physical bytes held while resident
+  size_t bytes() const;       // physical bytes held while resident
+  size_t used_bytes() const;  // bytes an eviction copies to host memory
   void evict();
   void restore();
 };

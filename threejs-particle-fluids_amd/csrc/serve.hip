@@ -27,6 +27,7 @@ struct ServedScene {
   bool picked = false;   // taken by the launch being formed
   uint64_t last_use = 0;
   uint64_t bytes = 0;    // arena bytes while resident
+  uint64_t host = 0;     // host bytes while evicted
   uint32_t particles = 0;
   uint32_t queued = 0;   // requests not yet launched
 };
@@ -73,6 +74,7 @@ struct pf_server_s {
   pf_server_stats st{};
   int err_code = 0;
   std::string err;
+  std::string note;  // the last refused eviction or restore (serving goes on)
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
   bool any_busy() const {
@@ -81,40 +83,77 @@ struct pf_server_s {
     return false;
   }
   uint64_t budget() const { return opt.hbm_budget_bytes ? opt.hbm_budget_bytes : ~0ull; }
+  uint64_t host_budget() const { return opt.host_budget_bytes ? opt.host_budget_bytes : ~0ull; }
 
-  void evict(uint32_t id) {
+  // Moves an idle scene to host memory. Refused (false, the scene stays
+  // resident and intact) when its copy would exceed the host budget or the
+  // host memory or a copy fails; such failures never end serving.
+  bool evict(uint32_t id) {
     ServedScene& sc = scenes[id];
     Arena* a = sc.loop->ps->arena.get();
-    if (!a->resident) return;
-    a->evict();
+    if (!a->resident) return true;
+    const uint64_t need = a->used_bytes();
+    if (st.host_bytes + need > host_budget()) {
+      st.evict_failures += 1;
+      note = "Server: host memory for evicted scenes is at its budget";
+      return false;
+    }
+    try {
+      a->evict();
+    } catch (const PfError&) {
+      st.evict_failures += 1;
+      note = pf_last_error();
+      return false;
+    } catch (const HipError&) {
+      st.evict_failures += 1;
+      note = pf_last_error();
+      return false;
+    }
+    sc.host = need;
+    st.host_bytes += need;
     st.resident_bytes -= sc.bytes;
     st.resident_scenes -= 1;
     st.evictions += 1;
+    return true;
   }
+  // Brings a scene back; throws (the scene stays evicted) when no device
+  // memory can be mapped.
   void restore(uint32_t id) {
     ServedScene& sc = scenes[id];
     Arena* a = sc.loop->ps->arena.get();
     if (a->resident) return;
     a->restore();
+    st.host_bytes -= sc.host;
+    sc.host = 0;
     st.resident_bytes += sc.bytes;
     st.resident_scenes += 1;
     st.peak_resident_bytes = std::max(st.peak_resident_bytes, st.resident_bytes);
     st.restores += 1;
   }
   // Evicts idle scenes, least recently stepped first, until `need` more
-  // bytes fit the budget. Scenes in flight or picked stay. Returns success.
-  bool make_room(uint64_t need, int64_t keep) {
+  // bytes fit the budget. Scenes in flight or picked stay, and a scene whose
+  // eviction is refused is passed over. Returns whether the room was made;
+  // *refused tells whether a refused eviction stood in the way (waiting
+  // would not help then).
+  bool make_room(uint64_t need, int64_t keep, bool* refused_any = nullptr) {
+    std::vector<uint32_t> refused;
+    bool ok = true;
     while (st.resident_bytes + need > budget()) {
       int64_t victim = -1;
       for (uint32_t i = 0; i < scenes.size(); ++i) {
         const ServedScene& s = scenes[i];
         if (!s.live || s.busy || s.picked || (int64_t)i == keep || !s.loop->ps->arena->resident) continue;
+        if (std::find(refused.begin(), refused.end(), i) != refused.end()) continue;
         if (victim < 0 || s.last_use < scenes[victim].last_use) victim = i;
       }
-      if (victim < 0) return false;
-      evict((uint32_t)victim);
+      if (victim < 0) {
+        ok = false;
+        break;
+      }
+      if (!evict((uint32_t)victim)) refused.push_back((uint32_t)victim);
     }
-    return true;
+    if (refused_any) *refused_any = !refused.empty();
+    return ok;
   }
 
   bool poll() {
@@ -172,16 +211,36 @@ struct pf_server_s {
     // for a later launch), so no request starves behind younger ones.
     for (uint64_t tk : pick) scenes[requests[tk].scene].picked = false;
     std::vector<uint64_t> run, failed;
+    std::string why;
     for (uint64_t tk : pick) {
       const uint32_t id = requests[tk].scene;
       ServedScene& sc = scenes[id];
       if (!sc.loop->ps->arena->resident) {
         if (sc.bytes > budget()) {
           failed.push_back(tk);
+          why = "Server: a scene needs more device memory than the HBM budget";
           continue;
         }
-        if (!make_room(sc.bytes, id)) continue;  // waits for memory to free up
-        restore(id);
+        bool refused = false;
+        if (!make_room(sc.bytes, id, &refused)) {
+          if (refused) {
+            // idle scenes cannot go to host memory: this request fails alone
+            failed.push_back(tk);
+            why = note;
+          }
+          continue;  // otherwise it waits for a launch in flight to free memory
+        }
+        try {
+          restore(id);
+        } catch (const PfError&) {
+          failed.push_back(tk);  // this request fails alone; the scene stays in host memory
+          why = note = pf_last_error();
+          continue;
+        } catch (const HipError&) {
+          failed.push_back(tk);
+          why = note = pf_last_error();
+          continue;
+        }
       }
       sc.picked = true;
       run.push_back(tk);
@@ -191,7 +250,7 @@ struct pf_server_s {
       pending.erase(std::find(pending.begin(), pending.end(), tk));
     }
     if (!failed.empty()) {
-      err = "Server: a scene needs more device memory than the HBM budget";
+      err = why;
       cv_done.notify_all();
     }
     if (run.empty()) return !failed.empty();
@@ -411,8 +470,17 @@ int pf_server_add_scene(pf_server* S, pf_loop* L, uint32_t* scene_id) {
     S->st.resident_scenes += 1;
   }
   // Admission: the new scene arrives resident (its owner built it); others
-  // make room, else it waits in host memory. The cap holds from here on.
-  if (!S->make_room(0, id)) S->evict(id);
+  // make room, else it waits in host memory. The cap holds from here on: a
+  // scene that can neither fit nor go to host memory is not admitted.
+  if (!S->make_room(0, id) && !S->evict(id)) {
+    const std::string why = S->note;
+    S->scenes.pop_back();
+    L->ps->server = nullptr;
+    S->st.scenes -= 1;
+    S->st.resident_bytes -= sc.bytes;
+    S->st.resident_scenes -= 1;
+    fail(PF_ENOMEM, "Server: no room for the scene within the HBM budget, and it cannot be evicted (%s)", why.c_str());
+  }
   S->st.peak_resident_bytes = std::max(S->st.peak_resident_bytes, S->st.resident_bytes);
   *scene_id = id;
   PF_API_END
@@ -517,7 +585,7 @@ int pf_server_evict_scene(pf_server* S, uint32_t id) {
   S->scene_at(id, "Server.evictScene");
   S->cv_done.wait(lk, [&] { return S->err_code != 0 || !S->scenes[id].busy; });
   S->check_error();
-  S->evict(id);
+  if (!S->evict(id)) fail(PF_ENOMEM, "%s", S->note.c_str());
   PF_API_END
 }
 

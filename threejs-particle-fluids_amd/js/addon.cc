@@ -1088,7 +1088,7 @@ napi_value request_info(napi_env env, const pf_request_info& r) {
   return obj;
 }
 
-// (ctx, {hbmBudget, maxBatchScenes, maxBatchParticles, slots})
+// (ctx, {hbmBudget, maxBatchScenes, maxBatchParticles, slots, hostBudget})
 napi_value ServerCreate(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   pf_server_options o{};
@@ -1098,6 +1098,7 @@ napi_value ServerCreate(napi_env env, napi_callback_info info) {
     o.max_batch_scenes = u32(env, prop(env, opts, "maxBatchScenes"), 0);
     o.max_batch_particles = (uint64_t)num(env, prop(env, opts, "maxBatchParticles"), 0);
     o.slots = u32(env, prop(env, opts, "slots"), 0);
+    o.host_budget_bytes = (uint64_t)num(env, prop(env, opts, "hostBudget"), 0);
   }
   pf_server* s = nullptr;
   PF_CALL(pf_server_create((pf_context*)handle(env, a.argv[0]), &o, &s));
@@ -1187,6 +1188,7 @@ napi_value ServerStats(napi_env env, napi_callback_info info) {
   PF_STAT("peakResidentBytes", st.peak_resident_bytes); PF_STAT("hbmBudget", st.hbm_budget_bytes);
   PF_STAT("scenes", st.scenes); PF_STAT("residentScenes", st.resident_scenes);
   PF_STAT("maxLaunchScenes", st.max_launch_scenes);
+  PF_STAT("hostBytes", st.host_bytes); PF_STAT("evictFailures", st.evict_failures);
 #undef PF_STAT
   return obj;
 }

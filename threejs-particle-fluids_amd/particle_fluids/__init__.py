@@ -784,9 +784,10 @@ class Server:
 
     STATUS = ("pending", "launched", "done", "failed")
 
-    def __init__(self, context=None, hbmBudget=0, maxBatchScenes=0, maxBatchParticles=0, slots=0):
+    def __init__(self, context=None, hbmBudget=0, maxBatchScenes=0, maxBatchParticles=0, slots=0, hostBudget=0):
         ctx = context or default_context()
-        o = _lib.ServerOptions(int(hbmBudget), int(maxBatchScenes), int(maxBatchParticles), int(slots))
+        o = _lib.ServerOptions(int(hbmBudget), int(maxBatchScenes), int(maxBatchParticles), int(slots),
+                               int(hostBudget))
         h = C.c_void_p()
         call("pf_server_create", ctx.handle, C.byref(o), C.byref(h))
         self.handle = h

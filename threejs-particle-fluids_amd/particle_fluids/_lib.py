@@ -114,6 +114,7 @@ class ServerOptions(C.Structure):
         ("max_batch_scenes", u32),
         ("max_batch_particles", C.c_uint64),
         ("slots", u32),
+        ("host_budget_bytes", C.c_uint64),
     ]
 
 
@@ -121,7 +122,8 @@ class ServerStats(C.Structure):
     _fields_ = [(f, C.c_uint64) for f in (
         "submitted", "completed", "failed", "launches", "scene_steps", "particle_substeps", "evictions",
         "restores", "resident_bytes", "peak_resident_bytes", "hbm_budget_bytes")] + [
-        ("scenes", u32), ("resident_scenes", u32), ("max_launch_scenes", u32)]
+        ("scenes", u32), ("resident_scenes", u32), ("max_launch_scenes", u32),
+        ("host_bytes", C.c_uint64), ("evict_failures", C.c_uint64)]
 
 
 class SurfaceFieldOptions(C.Structure):
