@@ -162,6 +162,20 @@ int pf_grid_sort_noalloc(pf_context* ctx, const float* positions4, uint32_t n, d
                          uint32_t* counts, uint32_t* cell_start, uint32_t* cell_end,
                          uint32_t* sorted_indices, uint32_t* sorted_keys, uint32_t* overflow);
 
+/* ---- Device self-check (no counterpart in the reference) -------------------
+ * The passes take reciprocals and square roots through short sequences that
+ * start from the hardware approximations and round them exactly. This call
+ * compares one of them bit for bit with the IEEE operation (1.0f / x, sqrtf)
+ * for every float whose bit pattern lies in [first_bits, end_bits), on the
+ * context's GPU, and returns the number of mismatches and the first one
+ * (0xffffffff if none). It is for tests and for users who move the library
+ * to another GPU. Blocks until done. */
+#define PF_EXACT_RCP 0        /* 1/x, exact for 2^-126 <= |x| < 2^126 */
+#define PF_EXACT_SQRT 1       /* sqrt(q), exact for every q >= 0 and -0 */
+#define PF_EXACT_SQRT_FAST 2  /* its branch-free part, exact for q = 0 and q >= 2^-96 */
+int pf_check_exact_math(pf_context* ctx, int function, uint32_t first_bits, uint32_t end_bits,
+                        uint64_t* mismatches, uint32_t* first_mismatch);
+
 /* ---- FluidSystem (src/fluids/FluidSystem.ts:71-322) ---------------------- */
 #define PF_FLUID_VISCOSITY 1u
 #define PF_FLUID_VORTICITY 2u

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpf.so")
-SOURCES = ["grid.hip", "fluid.hip", "softbody.hip", "contact.hip", "runtime.hip", "sim.hip", "serve.hip", "formats.hip", "field.hip", "noalloc.hip"]
+SOURCES = ["grid.hip", "fluid.hip", "softbody.hip", "contact.hip", "runtime.hip", "sim.hip", "serve.hip", "formats.hip", "field.hip", "noalloc.hip", "selfcheck.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -167,10 +167,11 @@ static __device__ __forceinline__ float4 color_normal_global(const SceneDev& S, 
 #define PF_TU_PER (PF_TU_MAX / PF_BLOCK)
 static __device__ __forceinline__ void tu_load_slots(const uint32_t* __restrict__ slots, uint32_t U,
                                                      uint32_t js[PF_TU_PER]) {
+  const PF_G uint32_t* g = pf_g(slots);
 #pragma unroll
   for (int i = 0; i < PF_TU_PER; ++i) {
     const uint32_t u = threadIdx.x + i * PF_BLOCK;
-    js[i] = u < U ? slots[u] : 0u;
+    js[i] = u < U ? g[u] : 0u;
   }
 }
 
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
     if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
     if (!active) return;
-    uint32_t* row = F.nl + s;  // the row of the next entry
+    PF_G uint32_t* row = pf_g(F.nl) + s;  // the row of the next entry
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
       for (int dy = -1; dy <= 1 && !over; ++dy) {
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
       if (u < U) {
-        u_x[u] = xv[i];
+        u_x[u] = pf_f4(xv[i]);
         slots[u] = js[i];
         if (snap && S.ct.on) moved |= S.ct.acc[js[i]].w;
       }
@@ -480,10 +481,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   __syncthreads();
 
   // 4. Walk the union. Completed entry pairs go straight to their list row.
-  uint32_t* __restrict__ nl16 = F.nl16;
+  PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
   if (active) {
     uint32_t pend = 0, last_local = 0;
-    uint32_t* row = nl16 + s;  // the row of the next entry pair
+    PF_G uint32_t* row = nl16 + s;  // the row of the next entry pair
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
       for (int dy = -1; dy <= 1 && !over; ++dy) {
@@ -543,7 +544,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     F.nl_cell[s] = pack_cell(q);
     if (want_normal) {
       // Colour-field normal over the finished list, in list order.
-      const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+      const PF_G uint32_t* __restrict__ uslots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+      const PF_G float4* __restrict__ gxs = pf_g(xs);
       float3 nsum = make_float3(0.f, 0.f, 0.f);
       // Whole groups of four need no per-entry test; the last partial group does.
       auto group = [&](uint32_t k0, bool tail) {
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
         float vol[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          xj[u] = snap ? xs[uslots[idx[u]]] : u_x[idx[u]];
+          xj[u] = snap ? pf_f4(gxs[uslots[idx[u]]]) : u_x[idx[u]];
           vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
         }
 #pragma unroll
@@ -583,12 +585,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
 // resolve to a harmless index whose terms the callers discard.
 // ---------------------------------------------------------------------------
 struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
-  const uint32_t* __restrict__ col;
+  const PF_G uint32_t* __restrict__ col;
   size_t n;
   uint32_t w0, w1;
   uint32_t lim;  // last union index: entries past the count keep LDS reads inside the union
   __device__ __forceinline__ Nl16Cursor(const uint32_t* nl16, uint32_t nn, uint32_t s, uint32_t U)
-      : col(nl16 + s), n(nn), lim(U ? U - 1u : 0u) {
+      : col(pf_g(nl16) + s), n(nn), lim(U ? U - 1u : 0u) {
     w0 = col[0];
     w1 = col[n];
   }
@@ -606,12 +608,12 @@ struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
 };
 
 struct NlCursor {  // global-gather tile: sorted slots
-  const uint32_t* __restrict__ col;
+  const PF_G uint32_t* __restrict__ col;
   size_t n;
   uint32_t self, cnt;
   uint32_t w[4];
   __device__ __forceinline__ NlCursor(const uint32_t* nl, uint32_t nn, uint32_t s, uint32_t count)
-      : col(nl + s), n(nn), self(s), cnt(count) {
+      : col(pf_g(nl) + s), n(nn), self(s), cnt(count) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) w[u] = col[(size_t)u * n];
   }
@@ -647,7 +649,7 @@ static __device__ __forceinline__ bool pair_dv(const FluidDev& F, float dt, floa
   if (r < 1e-20f) return false;
   const float corr = (F.rho0 * 2.0f) / fmaxf(rhoa + rhob, 1.0f);
   const float cs = -(((F.gamma * F.mass) * F.mass) * cohesion_spline(F, r));
-  const float ir = 1.0f / r;
+  const float ir = pf_rcp_rn(r);  // r >= 1e-20
   const float3 coh = make_float3((off.x * ir) * cs, (off.y * ir) * cs, (off.z * ir) * cs);
   const float gm = F.gamma * F.mass;
   const float3 curv = make_float3(-((na.x - nb.x) * gm), -((na.y - nb.y) * gm), -((na.z - nb.z) * gm));
@@ -658,7 +660,7 @@ static __device__ __forceinline__ bool pair_dv(const FluidDev& F, float dt, floa
 
 // Adhesion spline A(r) (adhesion.ts:16-27).
 static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float r) {
-  const float root = sqrtf(sqrtf(fmaxf(((r * r) * F.adh_m4_over_h + r * 6.0f) - F.adh_2h, 0.0f)));
+  const float root = pf_sqrt_rn(pf_sqrt_rn(fmaxf(((r * r) * F.adh_m4_over_h + r * 6.0f) - F.adh_2h, 0.0f)));
   return (r * 2.0f > F.h && r <= F.h) ? root * F.adh_coef : 0.0f;
 }
 
@@ -666,10 +668,10 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
 // negation of the lower end's change (round is symmetric). Integer sums do
 // not depend on the order the halves arrive in.
 static __device__ __forceinline__ void scatter_half(int4* acc, uint32_t j, float3 dv, float scale) {
-  int* aj = reinterpret_cast<int*>(acc + j);
-  atomicAdd(aj + 0, pf_ticks(-dv.x, scale));
-  atomicAdd(aj + 1, pf_ticks(-dv.y, scale));
-  atomicAdd(aj + 2, pf_ticks(-dv.z, scale));
+  PF_G int* aj = pf_g(reinterpret_cast<int*>(acc + j));
+  pf_atomic_add_g(aj + 0, pf_ticks(-dv.x, scale));
+  pf_atomic_add_g(aj + 1, pf_ticks(-dv.y, scale));
+  pf_atomic_add_g(aj + 2, pf_ticks(-dv.z, scale));
 }
 
 // Surface tension and adhesion of one particle in a table below 64 buckets
@@ -713,7 +715,7 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
         const float r = pf_len3(off);
         if (!(r < 1e-20f)) {
           const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
-          const float ir = 1.0f / r;
+          const float ir = pf_rcp_rn(r);  // r >= 1e-20
           const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
           tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
@@ -748,20 +750,30 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
                                                     const float4* __restrict__ xs, uint32_t s, uint32_t id,
                                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
                                                     bool adh_on, bool moved_i, bool moved_tile, uint32_t U,
-                                                    const uint32_t* __restrict__ uslots, int* acc_l, int tk[3]) {
+                                                    const uint32_t* __restrict__ uslots, int* acc_l, bool lds_acc,
+                                                    int tk[3]) {
   const uint32_t n = S.n;
   const uint32_t fend = F.start + F.count;
   const bool staged = U != PF_TU_GLOBAL;
   const uint32_t ci = SPECIAL ? F.nl_cell[s] : 0u;
   Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
   NlCursor cg(F.nl, n, s, cnt);
+  const PF_G uint32_t* __restrict__ g_slot = pf_g(uslots);
+  const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
+  const PF_G float4* __restrict__ g_x = pf_g(xs);
+  const PF_G float4* __restrict__ g_nrm = pf_g(F.normal);
+  const PF_G float* __restrict__ g_rho = pf_g(S.rho[p]);
+  const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
+  const PF_G uint32_t* __restrict__ g_last = pf_g(F.nl_last);
+  const PF_G uint32_t* __restrict__ g_cell = pf_g(F.nl_cell);
+  const PF_G int4* __restrict__ g_acc = pf_g(S.ct.acc);
   for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
     uint32_t jj[4];
     uint32_t idx[4] = {0u, 0u, 0u, 0u};
     if (staged) {
       c16.next(k0, idx);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
+      for (int u = 0; u < 4; ++u) jj[u] = g_slot[idx[u]];
     } else {
       cg.next(k0, jj);
     }
@@ -771,19 +783,19 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
     bool lower[4], upper[4], moved[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      idj[u] = S.id[p][jj[u]];
+      idj[u] = g_id[jj[u]];
       lastj[u] = 0xffffffffu;
       moved[u] = false;
       if (SPECIAL && st_pairs) {
-        lastj[u] = F.nl_last[jj[u]];
-        moved[u] = moved_i || (moved_tile && S.ct.acc[jj[u]].w != 0);
+        lastj[u] = g_last[jj[u]];
+        moved[u] = moved_i || (moved_tile && g_acc[jj[u]].w != 0);
       }
-      vol[u] = adh_on ? S.bvol[p][jj[u]] : 0.0f;
+      vol[u] = adh_on ? g_vol[jj[u]] : 0.0f;
       if (!SPECIAL) {
-        xj[u] = xs[jj[u]];
+        xj[u] = pf_f4(g_x[jj[u]]);
         if (st_pairs) {
-          nj[u] = F.normal[jj[u]];
-          rj[u] = S.rho[p][jj[u]];
+          nj[u] = pf_f4(g_nrm[jj[u]]);
+          rj[u] = g_rho[jj[u]];
         }
       }
       const bool in = k0 + u < cnt;
@@ -799,11 +811,11 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
         nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         rj[u] = 0.0f;
         cj[u] = 0u;
-        if ((upper[u] || lower[u]) && lastj[u] != 0xffffffffu) cj[u] = F.nl_cell[jj[u]];
-        if (pair || vol[u] > 0.0f) xj[u] = xs[jj[u]];
+        if ((upper[u] || lower[u]) && lastj[u] != 0xffffffffu) cj[u] = g_cell[jj[u]];
+        if (pair || vol[u] > 0.0f) xj[u] = pf_f4(g_x[jj[u]]);
         if (pair) {
-          nj[u] = F.normal[jj[u]];
-          rj[u] = S.rho[p][jj[u]];
+          nj[u] = pf_f4(g_nrm[jj[u]]);
+          rj[u] = g_rho[jj[u]];
         }
       }
     }
@@ -834,7 +846,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
             // end, or its truncated list dropped this slot.
             if (SPECIAL && lo &&
                 (moved[u] || (lastj[u] != 0xffffffffu && walk_key(ci, cj[u], s) > lastj[u]))) {
-              if (acc_l) {
+              if (lds_acc) {
                 // the partner is union entry idx: summed in LDS, flushed once per tile
                 atomicAdd(acc_l + 3 * idx[u] + 0, pf_ticks(-dv.x, F.imp_scale));
                 atomicAdd(acc_l + 3 * idx[u] + 1, pf_ticks(-dv.y, F.imp_scale));
@@ -853,7 +865,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
           const float r = pf_len3(off);
           if (!(r < 1e-20f)) {
             const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vj)) * adhesion_spline(F, r));
-            const float ir = 1.0f / r;
+            const float ir = pf_rcp_rn(r);  // r >= 1e-20
             const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
             tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
             tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
@@ -871,7 +883,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
 template <bool ST, bool ADH, bool CT>
 static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const FluidDev& F, int p, int c,
                                                         uint32_t s, uint32_t tile, bool special, bool ct_scene,
-                                                        int* acc_l) {
+                                                        int* acc_l, bool lds_acc) {
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const uint32_t id = S.id[p][s];
@@ -907,9 +919,10 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
     impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, ct_scene, moved_i, tk);
   } else if (special) {
     impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, moved_i, moved_tile, U, uslots, acc_l,
-                       tk);
+                       lds_acc, tk);
   } else {
-    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, false, false, U, uslots, nullptr, tk);
+    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, false, false, U, uslots, acc_l, false,
+                        tk);
   }
   if (special) {
     S.xs[c ^ 1][s] = make_float4(__int_as_float(tk[0]), __int_as_float(tk[1]), __int_as_float(tk[2]), xi4.w);
@@ -952,17 +965,17 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
     __syncthreads();
   }
   const uint32_t s = t.first + threadIdx.x;
-  if (s < S.n) impulse_particle<ST, ADH, CT>(S, F, p, c, s, tile, special, ct_scene, lds ? acc_l : nullptr);
+  if (s < S.n) impulse_particle<ST, ADH, CT>(S, F, p, c, s, tile, special, ct_scene, acc_l, lds);
   if (lds) {
     __syncthreads();
     const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
     for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) {
       const int a0 = acc_l[3 * u], a1 = acc_l[3 * u + 1], a2 = acc_l[3 * u + 2];
       if ((a0 | a1 | a2) != 0) {
-        int* a = reinterpret_cast<int*>(S.ct.acc + uslots[u]);
-        atomicAdd(a + 0, a0);
-        atomicAdd(a + 1, a1);
-        atomicAdd(a + 2, a2);
+        PF_G int* a = pf_g(reinterpret_cast<int*>(S.ct.acc + uslots[u]));
+        pf_atomic_add_g(a + 0, a0);
+        pf_atomic_add_g(a + 1, a1);
+        pf_atomic_add_g(a + 2, a2);
       }
     }
   }
@@ -1280,7 +1293,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = threadIdx.x + i * PF_BLOCK;
       if (u < U) {
-        u_x[u] = xv[i];
+        u_x[u] = pf_f4(xv[i]);
         u_m[u] = mv[i];
       }
     }
@@ -1539,7 +1552,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
       const uint32_t u = threadIdx.x + i * PF_BLOCK;
       if (u < U) {
         u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
-        u_v[u] = vv[i];
+        u_v[u] = pf_f4(vv[i]);
       }
     }
     __syncthreads();

@@ -279,6 +279,48 @@ __device__ __forceinline__ int pf_query_coord(float p, float o, float cell) {
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded reciprocal and square root in fewer instructions than the
+// compiler's general IEEE sequences, over the ranges the passes use. Both
+// start from the hardware approximation (v_rcp_f32, v_sqrt_f32: within 1 ulp)
+// and round it exactly with fused multiply-adds, so the results equal IEEE
+// 1.0f / x and sqrtf(q) bit for bit; pf_check_exact_math runs the exhaustive
+// comparison on the device.
+// ---------------------------------------------------------------------------
+
+// 1/x for 2^-126 <= |x| < 2^126: one Newton-Raphson step from the hardware
+// reciprocal y, y + y*(1 - x*y) with both operations fused (Markstein's
+// iteration); 3 instructions against 10 for the general divide.
+__device__ __forceinline__ float pf_rcp_rn(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(e, y, y);
+}
+
+// sqrt(q) for q = 0 and q >= 2^-96, and +inf: the hardware root s, moved to a
+// neighbour when the exact residual q - s'*s says the neighbour rounds
+// better (the rounding test of Markstein, IA-64 and Elementary Functions,
+// 2000). 9 instructions.
+__device__ __forceinline__ float pf_sqrt_rn_fast(float q) {
+  const float s = __builtin_amdgcn_sqrtf(q);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  const float rm = __builtin_fmaf(-sm, s, q);
+  const float rp = __builtin_fmaf(-sp, s, q);
+  const float s1 = (rm <= 0.0f) ? sm : s;
+  return (rp > 0.0f) ? sp : s1;
+}
+
+// sqrt(q) for every q >= 0 (and NaN): arguments below 2^-96 are scaled by
+// 2^64 into the fast form's range and its root by 2^-32 back, both exact
+// (powers of two, the scaled root normal), so the rounding is that of
+// sqrt(q). Branch-free, 14 instructions against 16 for the general sequence.
+__device__ __forceinline__ float pf_sqrt_rn(float q) {
+  const bool tiny = q < 0x1p-96f;
+  const float s = pf_sqrt_rn_fast(tiny ? q * 0x1p64f : q);
+  return tiny ? s * 0x1p-32f : s;
+}
+
+// ---------------------------------------------------------------------------
 // SPH kernels (kernels.ts:64-107), FP32, operand order as the reference
 // ---------------------------------------------------------------------------
 
@@ -293,9 +335,9 @@ __device__ __forceinline__ float pf_poly6(float r2, float h2, float poly6) {
 // reference's 2.5-ulp WGSL division).
 __device__ __forceinline__ float3 pf_spiky_grad(float3 r, float h, float spiky) {
   const float rsq = (r.x * r.x + r.y * r.y) + r.z * r.z;
-  const float rl = sqrtf(rsq);
+  const float rl = pf_sqrt_rn(rsq);
   const float t = fmaxf(h - rl, 0.0f);
-  const float inv = 1.0f / fmaxf(rl, 1e-20f);
+  const float inv = pf_rcp_rn(fmaxf(rl, 1e-20f));
   const float mag = -((t * t) * spiky);
   return make_float3((r.x * inv) * mag, (r.y * inv) * mag, (r.z * inv) * mag);
 }
@@ -315,8 +357,31 @@ __device__ __forceinline__ float3 pf_mul3(float3 a, float s) {
 __device__ __forceinline__ float3 pf_cross3(float3 a, float3 b) {
   return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float pf_len3(float3 a) { return sqrtf(pf_dot3(a, a)); }
+__device__ __forceinline__ float pf_len3(float3 a) { return pf_sqrt_rn(pf_dot3(a, a)); }
 __device__ __forceinline__ float3 pf_xyz(float4 a) { return make_float3(a.x, a.y, a.z); }
+// Component-wise copy. A whole-struct float4 assignment out of an unrolled
+// register array lowers to a memcpy that leaves the array in scratch memory.
+__device__ __forceinline__ float4 pf_f4(float4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+
+// Global-address-space view of a device pointer. Loads through it compile to
+// global_load, counted by vmcnt alone. Loads through a generic pointer (any
+// pointer read out of a descriptor) compile to flat_load, which counts in
+// lgkmcnt as well, so every wait for an LDS read also waits for the global
+// loads still in flight, such as a neighbour-list prefetch.
+// (The host pass of a HIP compile parses device code too; it gets plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PF_G __attribute__((address_space(1)))
+#else
+#define PF_G
+#endif
+template <class T>
+__device__ __forceinline__ PF_G T* pf_g(T* p) {
+  return (PF_G T*)p;
+}
+// atomicAdd (relaxed, device scope) through a global-address-space pointer.
+__device__ __forceinline__ void pf_atomic_add_g(PF_G int* a, int v) {
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Fixed-point accumulator add (accumulator.ts:62-68): i32(round(v * scale)),
 // round-half-to-even, saturating conversion. Sums wrap like WGSL atomicAdd.

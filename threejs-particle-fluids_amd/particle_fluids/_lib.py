@@ -188,6 +188,7 @@ SIGNATURES = {
     "pf_fence_query": [vp, P(i32)],
     "pf_fence_destroy": [vp],
     "pf_grid_sort_noalloc": [vp, vp, u32, f64, u32, P(f64), vp, vp, vp, vp, vp, vp, vp],
+    "pf_check_exact_math": [vp, i32, u32, u32, P(C.c_uint64), P(u32)],
     "pf_fluid_create": [vp, P(FluidOptions), P(vp)],
     "pf_fluid_set": [vp, u32, f64],
     "pf_fluid_get": [vp, u32, P(f64)],
