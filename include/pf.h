@@ -210,6 +210,12 @@ int pf_fluid_info(pf_fluid* fluid, double* mass, double* smoothing_radius,
  * more gather from global memory; 0 makes every tile do so. Results do not
  * depend on it. */
 int pf_fluid_set_tile_union_limit(pf_fluid* fluid, uint32_t max_particles);
+/* No counterpart in the reference (a diagnostic of the same knob): after the
+ * last neighbour build, the number of 256-slot tiles, how many gathered from
+ * global memory, the largest staged union and the mean staged union. Each
+ * output nullable. Waits for the loop's work. */
+int pf_fluid_tile_union_stats(pf_fluid* fluid, uint32_t* tiles, uint32_t* global_tiles,
+                              uint32_t* largest_union, double* mean_union);
 int pf_fluid_destroy(pf_fluid* fluid);
 
 /* ---- ViscositySolver (src/fluids/ViscositySolver.ts:38-131) -----------

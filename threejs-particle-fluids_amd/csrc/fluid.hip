@@ -175,6 +175,96 @@ static __device__ __forceinline__ void tu_load_slots(const uint32_t* __restrict_
   }
 }
 
+// Slot of key in an open-addressing table of hs entries (linear probing).
+static __device__ __forceinline__ uint32_t cu_home(uint32_t key, uint32_t hs) {
+  return __umulhi(key * 0x9E3779B1u, hs);
+}
+
+// Union of the particles named by a global-walk tile's finished lists (global
+// slots, F.nl). Every thread of the block calls it; `count` is the calling
+// thread's list length (0 for threads without a list). When at most `limit`
+// distinct slots appear, they become the tile's union (F.tu_slots, in
+// table order), each list is rewritten as 16-bit union indices (F.nl16) and
+// the union size is returned; otherwise PF_TU_GLOBAL. Scenes with contacts
+// also get the tile's moved mark, over the new union. The table lives in the
+// union's LDS: 4 * cap words, keys (slot + 1) and dense indices, 2 * cap each.
+static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* lds,
+                                         uint32_t cap, uint32_t limit, uint32_t* ctl, uint32_t count,
+                                         uint32_t s, bool snap) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t hs = 2u * cap;
+  uint32_t* hk = lds;       // [hs] slot + 1, 0 = empty
+  uint32_t* hv = lds + hs;  // [hs] union index
+  const uint32_t n = S.n;
+  __syncthreads();
+  for (uint32_t i = tid; i < hs; i += PF_BLOCK) hk[i] = 0u;
+  if (tid < 2) ctl[tid] = 0u;  // distinct slots, give-up flag
+  __syncthreads();
+  const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
+  {
+    volatile uint32_t* give_up = &ctl[1];
+    for (uint32_t k = 0; k < count && !*give_up; ++k) {
+      const uint32_t key = col[(size_t)k * n] + 1u;
+      uint32_t h = cu_home(key, hs);
+      for (uint32_t probe = 0; probe < hs; ++probe) {
+        const uint32_t prev = atomicCAS(&hk[h], 0u, key);
+        if (prev == 0u) {
+          if (atomicAdd(&ctl[0], 1u) + 1u > limit) *give_up = 1u;
+          break;
+        }
+        if (prev == key) break;
+        h = (h + 1u == hs) ? 0u : h + 1u;
+      }
+    }
+  }
+  __syncthreads();
+  // Uniform over the block.
+  if (ctl[1] != 0u || ctl[0] > limit || limit == 0u) {
+    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
+    return PF_TU_GLOBAL;
+  }
+  // Dense indices in table order: each thread numbers a contiguous chunk.
+  const uint32_t per = (hs + PF_BLOCK - 1u) / PF_BLOCK;
+  const uint32_t b0 = min(tid * per, hs), b1 = min(b0 + per, hs);
+  uint32_t mine = 0;
+  for (uint32_t i = b0; i < b1; ++i) mine += hk[i] ? 1u : 0u;
+  uint32_t U = 0;
+  uint32_t at = block_exclusive_scan(mine, ctl + 4, &U);
+  PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+  int moved = 0;
+  for (uint32_t i = b0; i < b1; ++i) {
+    const uint32_t key = hk[i];
+    if (key) {
+      hv[i] = at;
+      slots[at] = key - 1u;
+      if (snap && S.ct.on) moved |= S.ct.acc[key - 1u].w;
+      ++at;
+    }
+  }
+  if (snap) {
+    const int any = __syncthreads_or(moved);
+    if (tid == 0 && S.ct.on) F.tu_moved[tile] = any ? 1u : 0u;
+  } else {
+    __syncthreads();
+  }
+  // The lists as pairs of union indices, column-major like the staged walk's.
+  PF_G uint32_t* __restrict__ row = pf_g(F.nl16) + s;
+  for (uint32_t k = 0; k < count; k += 2u) {
+    uint32_t idx[2] = {0u, 0u};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (k + u < count) {
+        const uint32_t key = col[(size_t)(k + u) * n] + 1u;
+        uint32_t h = cu_home(key, hs);
+        for (uint32_t probe = 0; probe < hs && hk[h] != key; ++probe) h = (h + 1u == hs) ? 0u : h + 1u;
+        idx[u] = hv[h];
+      }
+    }
+    row[(size_t)(k >> 1) * n] = idx[0] | (idx[1] << 16);
+  }
+  return U;
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour list (+ colour-field normal when surface tension is on)
 //
@@ -358,9 +448,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   }
 
   if (global_walk) {
-    if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
-    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
-    if (!active) return;
+    if (active) {
     PF_G uint32_t* row = pf_g(F.nl) + s;  // the row of the next entry
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
@@ -406,6 +494,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     F.nl_cell[s] = pack_cell(q);
     if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+    }
+    // The passes after this one only need the particles the lists name: if
+    // those fit, they become the tile's union and the lists are rewritten as
+    // 16-bit indices into it, so the tile is staged in LDS like the others.
+    const uint32_t Uc = compact_union(S, F, tile, reinterpret_cast<uint32_t*>(u_x), cap, min(F.tu_limit, cap), ctl,
+                                      active ? count : 0u, s, snap != 0);
+    if (tid == 0) F.tu_size[tile] = Uc;
     return;
   }
 

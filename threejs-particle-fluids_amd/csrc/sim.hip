@@ -519,6 +519,32 @@ int pf_fluid_set_tile_union_limit(pf_fluid* f, uint32_t max_particles) {
   PF_API_END
 }
 
+int pf_fluid_tile_union_stats(pf_fluid* f, uint32_t* tiles, uint32_t* global_tiles, uint32_t* largest,
+                              double* mean) {
+  PF_API_BEGIN
+  if (!f || f->disposed) fail(PF_ESTATE, "FluidSystem: disposed");
+  if (!f->loop || !f->tu_size) fail(PF_EINVAL, "FluidSystem: add the fluid to a SimLoop and step it first");
+  PF_HIP(hipStreamSynchronize(f->ps->ctx->stream));
+  const size_t ntiles = ((size_t)f->ps->n + PF_BLOCK - 1) / PF_BLOCK;
+  std::vector<uint32_t> u(ntiles);
+  PF_HIP(hipMemcpy(u.data(), f->tu_size, sizeof(uint32_t) * ntiles, hipMemcpyDeviceToHost));
+  uint32_t g = 0, big = 0;
+  double sum = 0;
+  for (uint32_t v : u) {
+    if (v == PF_TU_GLOBAL) {
+      ++g;
+    } else {
+      big = std::max(big, v);
+      sum += v;
+    }
+  }
+  if (tiles) *tiles = (uint32_t)ntiles;
+  if (global_tiles) *global_tiles = g;
+  if (largest) *largest = big;
+  if (mean) *mean = ntiles > g ? sum / (double)(ntiles - g) : 0.0;
+  PF_API_END
+}
+
 int pf_fluid_destroy(pf_fluid* f) {
   PF_API_BEGIN
   if (!f || f->disposed) return PF_OK;

@@ -355,6 +355,15 @@ class FluidSystem:
         results do not depend on it; 0 gathers every tile from global memory)."""
         call("pf_fluid_set_tile_union_limit", self.handle, int(maxParticles))
 
+    def tileUnionStats(self):
+        """After the last neighbour build: tiles, tiles that gathered from
+        global memory, the largest and the mean staged union (no reference
+        counterpart; a diagnostic of setTileUnionLimit)."""
+        t, g, big = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        mean = C.c_double()
+        call("pf_fluid_tile_union_stats", self.handle, C.byref(t), C.byref(g), C.byref(big), C.byref(mean))
+        return {"tiles": t.value, "global_tiles": g.value, "largest_union": big.value, "mean_union": mean.value}
+
     def material(self):
         return _lib.Material(1, self.handle)
 

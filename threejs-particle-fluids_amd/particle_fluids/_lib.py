@@ -197,6 +197,7 @@ SIGNATURES = {
     "pf_fluid_readback_overflow": [vp, P(i32)],
     "pf_fluid_info": [vp, P(f64), P(f64), P(f64), P(f64)],
     "pf_fluid_set_tile_union_limit": [vp, u32],
+    "pf_fluid_tile_union_stats": [vp, P(u32), P(u32), P(u32), P(f64)],
     "pf_fluid_destroy": [vp],
     "pf_viscosity_create": [vp, f64, u32, P(vp)],
     "pf_viscosity_set": [vp, f64],
