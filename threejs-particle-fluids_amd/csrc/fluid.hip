@@ -187,14 +187,15 @@ static __device__ __forceinline__ uint32_t cu_home(uint32_t key, uint32_t hs) {
 // table order), each list is rewritten as 16-bit union indices (F.nl16) and
 // the union size is returned; otherwise PF_TU_GLOBAL. Scenes with contacts
 // also get the tile's moved mark, over the new union. The table lives in the
-// union's LDS: 4 * cap words, keys (slot + 1) and dense indices, 2 * cap each.
+// union's LDS, 3 * cap words: 2 * cap keys (slot + 1) and as many 16-bit
+// dense indices.
 static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* lds,
                                          uint32_t cap, uint32_t limit, uint32_t* ctl, uint32_t count,
                                          uint32_t s, bool snap) {
   const uint32_t tid = threadIdx.x;
   const uint32_t hs = 2u * cap;
-  uint32_t* hk = lds;       // [hs] slot + 1, 0 = empty
-  uint32_t* hv = lds + hs;  // [hs] union index
+  uint32_t* hk = lds;                                        // [hs] slot + 1, 0 = empty
+  uint16_t* hv = reinterpret_cast<uint16_t*>(lds + hs);      // [hs] union index
   const uint32_t n = S.n;
   __syncthreads();
   for (uint32_t i = tid; i < hs; i += PF_BLOCK) hk[i] = 0u;
@@ -235,7 +236,7 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   for (uint32_t i = b0; i < b1; ++i) {
     const uint32_t key = hk[i];
     if (key) {
-      hv[i] = at;
+      hv[i] = (uint16_t)at;
       slots[at] = key - 1u;
       if (snap && S.ct.on) moved |= S.ct.acc[key - 1u].w;
       ++at;
@@ -293,12 +294,16 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   __shared__ uint32_t h_key[PF_TU_HASH];    // bucket + 1, 0 = empty
   __shared__ uint32_t h_off[PF_TU_HASH];    // (union offset << 16) | bucket count
   __shared__ uint32_t h_start[PF_TU_HASH];  // first sorted slot of the bucket
-  // The staged union (cap >= PF_BLOCK entries of dynamic LDS); before staging
-  // the same words hold each thread's query cell and activity for the leader test.
+  // The staged union's positions, x, y and z arrays of cap entries (dynamic
+  // LDS, 12 bytes per entry and at least 4 KB); before staging the same words
+  // hold each thread's query cell and activity for the leader test.
   extern __shared__ float4 pf_lds[];
-  float4* u_x = pf_lds;
-  int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(&u_x[0]);
-  uint32_t* a_sh = reinterpret_cast<uint32_t*>(&u_x[0]) + 3 * PF_BLOCK;
+  uint32_t* u_words = reinterpret_cast<uint32_t*>(pf_lds);
+  float* u_px = reinterpret_cast<float*>(pf_lds);
+  float* u_py = u_px + cap;
+  float* u_pz = u_py + cap;
+  int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(u_words);
+  uint32_t* a_sh = u_words + 3 * PF_BLOCK;
   __shared__ uint32_t ctl[4 + PF_BLOCK / 64];  // distinct buckets, give-up flag, -, -, scan
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
@@ -450,6 +455,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   if (global_walk) {
     if (active) {
     PF_G uint32_t* row = pf_g(F.nl) + s;  // the row of the next entry
+    const PF_G float4* __restrict__ gxw = pf_g(xw);
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
       for (int dy = -1; dy <= 1 && !over; ++dy) {
@@ -462,25 +468,32 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
           const uint32_t st = S.start[b];
           const uint32_t en = S.start[b + 1];
-          for (uint32_t j0 = st; j0 < en && !over; j0 += 4) {
-            float4 xj[4];
+          // Sixteen candidates (four cache lines) per round trip to memory.
+          for (uint32_t j0 = st; j0 < en && !over; j0 += 16) {
+            float4 xj[16];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = xw[min(j0 + u, en - 1u)];
+            for (int u = 0; u < 16; ++u) xj[u] = pf_f4(gxw[min(j0 + u, en - 1u)]);
+            // The distance tests first (most candidates fail), then the
+            // appends of the accepted ones in candidate order.
+            uint32_t m = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t j = j0 + u;
+            for (int u = 0; u < 16; ++u) {
               const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              const bool acc = j < en && pf_dot3(d, d) < F.h2;
-              const bool keep = acc && count < PF_MAX_NEIGHBORS;
+              m |= (j0 + u < en && pf_dot3(d, d) < F.h2) ? (1u << u) : 0u;
+            }
+            while (m != 0u) {
               // The list is full and another neighbour exists: it overflows,
               // nothing later in the walk can enter it.
-              over = over || (acc && count >= PF_MAX_NEIGHBORS);
-              if (keep) {
-                *row = j;
-                row += n;
+              if (count >= PF_MAX_NEIGHBORS) {
+                over = true;
+                break;
               }
-              last_key = keep ? ((stencil << 27) | j) : last_key;
-              count += keep ? 1u : 0u;
+              const uint32_t j = j0 + (uint32_t)__builtin_ctz(m);
+              m &= m - 1u;
+              *row = j;
+              row += n;
+              last_key = (stencil << 27) | j;
+              ++count;
             }
           }
         }
@@ -498,7 +511,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     // The passes after this one only need the particles the lists name: if
     // those fit, they become the tile's union and the lists are rewritten as
     // 16-bit indices into it, so the tile is staged in LDS like the others.
-    const uint32_t Uc = compact_union(S, F, tile, reinterpret_cast<uint32_t*>(u_x), cap, min(F.tu_limit, cap), ctl,
+    const uint32_t Uc = compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl,
                                       active ? count : 0u, s, snap != 0);
     if (tid == 0) F.tu_size[tile] = Uc;
     return;
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   // use the union's LDS (4 entries per float4, so 8 per thread fit).
   {
     uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-    uint32_t* mark = reinterpret_cast<uint32_t*>(u_x);
+    uint32_t* mark = u_words;
     const uint32_t u8 = (U + 7u) & ~7u;
     for (uint32_t u = tid; u < u8; u += PF_BLOCK) mark[u] = 0u;
     __syncthreads();
@@ -561,7 +574,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
       if (u < U) {
-        u_x[u] = pf_f4(xv[i]);
+        u_px[u] = xv[i].x;
+        u_py[u] = xv[i].y;
+        u_pz[u] = xv[i].z;
         slots[u] = js[i];
         if (snap && S.ct.on) moved |= S.ct.acc[js[i]].w;
       }
@@ -594,33 +609,43 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           if (h_key[h] != b + 1u) continue;  // unreachable: the cell's leader inserted b
           const uint32_t ho = h_off[h];
           const uint32_t off = ho >> 16, cn = ho & 0xffffu;
-          // Branch-free append: only a completed entry pair stores. Whole
-          // groups of four need no bounds test; the last partial group does.
+          // The distance tests of a group of eight first (most candidates
+          // fail), then the appends of the accepted ones in candidate order;
+          // only a completed entry pair stores. Whole groups need no bounds
+          // test; the last partial group does.
           auto group = [&](uint32_t i0, bool tail) {
-            float4 xj[4];
+            float3 xj[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xj[u] = u_x[off + (tail ? min(i0 + u, cn - 1u) : i0 + u)];
+            for (int u = 0; u < 8; ++u) {
+              const uint32_t e = off + (tail ? min(i0 + u, cn - 1u) : i0 + u);
+              xj[u] = make_float3(u_px[e], u_py[e], u_pz[e]);
+            }
+            uint32_t m = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const uint32_t i = i0 + u;
+            for (int u = 0; u < 8; ++u) {
               const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              const bool acc = (!tail || i < cn) && pf_dot3(d, d) < F.h2;
-              const bool keep = acc && count < PF_MAX_NEIGHBORS;
+              m |= ((!tail || i0 + u < cn) && pf_dot3(d, d) < F.h2) ? (1u << u) : 0u;
+            }
+            while (m != 0u) {
               // The list is full and another neighbour exists: it overflows,
               // nothing later in the walk can enter it.
-              over = over || (acc && count >= PF_MAX_NEIGHBORS);
-              const uint32_t local = off + i;
-              if (keep && (count & 1u)) {
+              if (count >= PF_MAX_NEIGHBORS) {
+                over = true;
+                break;
+              }
+              const uint32_t local = off + i0 + (uint32_t)__builtin_ctz(m);
+              m &= m - 1u;
+              if (count & 1u) {
                 *row = pend | (local << 16);
                 row += n;
               }
-              pend = (keep && !(count & 1u)) ? local : pend;
-              last_local = keep ? local : last_local;
-              count += keep ? 1u : 0u;
+              pend = local;
+              last_local = local;
+              ++count;
             }
           };
-          const uint32_t full = cn & ~3u;
-          for (uint32_t i0 = 0; i0 < full && !over; i0 += 4) group(i0, false);
+          const uint32_t full = cn & ~7u;
+          for (uint32_t i0 = 0; i0 < full && !over; i0 += 8) group(i0, false);
           if (full < cn && !over) group(full, true);
         }
       }
@@ -630,7 +655,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     // Walk key of the last kept neighbour of a truncated list, from its
     // list-build cell and sorted slot (the staged union holds both).
     if (over) {
-      const float4 xl = u_x[last_local];
+      const float3 xl = make_float3(u_px[last_local], u_py[last_local], u_pz[last_local]);
       const int cf[3] = {pf_query_coord(xl.x, S.ox, S.cell), pf_query_coord(xl.y, S.oy, S.cell),
                          pf_query_coord(xl.z, S.oz, S.cell)};
       last_key = walk_key(pack_cell(cf), pack_cell(q), F.tu_slots[(size_t)tile * PF_TU_MAX + last_local]);
@@ -650,11 +675,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
         const uint32_t ul = U ? U - 1u : 0u;
         const uint32_t idx[4] = {(uint32_t)min(w0 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w0 >> 16) & (PF_TU_MAX - 1u), ul),
                                  (uint32_t)min(w1 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w1 >> 16) & (PF_TU_MAX - 1u), ul)};
-        float4 xj[4];
+        float3 xj[4];
         float vol[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          xj[u] = snap ? pf_f4(gxs[uslots[idx[u]]]) : u_x[idx[u]];
+          xj[u] = snap ? pf_xyz(gxs[uslots[idx[u]]]) : make_float3(u_px[idx[u]], u_py[idx[u]], u_pz[idx[u]]);
           vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
         }
 #pragma unroll
@@ -1875,7 +1900,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restric
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool normal, uint32_t cap, int snap) {
   if (!ntiles) return;
-  const size_t lds = (size_t)cap * sizeof(float4);
+  // x, y, z of cap union entries; at least the leader test's 4 words per thread
+  const size_t lds = std::max<size_t>((size_t)cap * 3 * sizeof(float), 4 * PF_BLOCK * sizeof(uint32_t));
   if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
   else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
 }
