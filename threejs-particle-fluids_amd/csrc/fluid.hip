@@ -203,18 +203,26 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   __syncthreads();
   const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
   {
+    // Eight list entries loaded per round trip, then inserted.
     volatile uint32_t* give_up = &ctl[1];
-    for (uint32_t k = 0; k < count && !*give_up; ++k) {
-      const uint32_t key = col[(size_t)k * n] + 1u;
-      uint32_t h = cu_home(key, hs);
-      for (uint32_t probe = 0; probe < hs; ++probe) {
-        const uint32_t prev = atomicCAS(&hk[h], 0u, key);
-        if (prev == 0u) {
-          if (atomicAdd(&ctl[0], 1u) + 1u > limit) *give_up = 1u;
-          break;
+    for (uint32_t k0 = 0; k0 < count && !*give_up; k0 += 8u) {
+      uint32_t keys[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) keys[u] = col[(size_t)min(k0 + u, count - 1u) * n] + 1u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k0 + u >= count) break;
+        const uint32_t key = keys[u];
+        uint32_t h = cu_home(key, hs);
+        for (uint32_t probe = 0; probe < hs; ++probe) {
+          const uint32_t prev = atomicCAS(&hk[h], 0u, key);
+          if (prev == 0u) {
+            if (atomicAdd(&ctl[0], 1u) + 1u > limit) *give_up = 1u;
+            break;
+          }
+          if (prev == key) break;
+          h = (h + 1u == hs) ? 0u : h + 1u;
         }
-        if (prev == key) break;
-        h = (h + 1u == hs) ? 0u : h + 1u;
       }
     }
   }
@@ -249,19 +257,22 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
     __syncthreads();
   }
   // The lists as pairs of union indices, column-major like the staged walk's.
+  // Eight entries loaded per round trip; entries past the count are zero.
   PF_G uint32_t* __restrict__ row = pf_g(F.nl16) + s;
-  for (uint32_t k = 0; k < count; k += 2u) {
-    uint32_t idx[2] = {0u, 0u};
+  for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
+    uint32_t keys[8], idx[8];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (k + u < count) {
-        const uint32_t key = col[(size_t)(k + u) * n] + 1u;
-        uint32_t h = cu_home(key, hs);
-        for (uint32_t probe = 0; probe < hs && hk[h] != key; ++probe) h = (h + 1u == hs) ? 0u : h + 1u;
-        idx[u] = hv[h];
-      }
+    for (int u = 0; u < 8; ++u) keys[u] = col[(size_t)min(k0 + u, count - 1u) * n] + 1u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t key = keys[u];
+      uint32_t h = cu_home(key, hs);
+      for (uint32_t probe = 0; probe < hs && hk[h] != key; ++probe) h = (h + 1u == hs) ? 0u : h + 1u;
+      idx[u] = k0 + u < count ? (uint32_t)hv[h] : 0u;
     }
-    row[(size_t)(k >> 1) * n] = idx[0] | (idx[1] << 16);
+#pragma unroll
+    for (int u = 0; u < 8; u += 2)
+      if (k0 + u < count) row[(size_t)((k0 + u) >> 1) * n] = idx[u] | (idx[u + 1] << 16);
   }
   return U;
 }
