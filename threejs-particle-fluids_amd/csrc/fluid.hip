@@ -795,9 +795,10 @@ static __device__ __forceinline__ float adhesion_spline(const FluidDev& F, float
   return (r * 2.0f > F.h && r <= F.h) ? root * F.adh_coef : 0.0f;
 }
 
-// The partner's half of a pair, added to its accumulator: the exact
-// negation of the lower end's change (round is symmetric). Integer sums do
-// not depend on the order the halves arrive in.
+// The partner's half of a pair, added to its accumulator: the change of the
+// lower end negated before it is rounded to ticks, as the reference scatters
+// it (cohesion.ts:96-100). Integer sums do not depend on the order the halves
+// arrive in.
 static __device__ __forceinline__ void scatter_half(int4* acc, uint32_t j, float3 dv, float scale) {
   PF_G int* aj = pf_g(reinterpret_cast<int*>(acc + j));
   pf_atomic_add_g(aj + 0, pf_ticks(-dv.x, scale));
@@ -805,21 +806,33 @@ static __device__ __forceinline__ void scatter_half(int4* acc, uint32_t j, float
   pf_atomic_add_g(aj + 2, pf_ticks(-dv.z, scale));
 }
 
+// Adhesion toward one boundary neighbour of volume vol (adhesion.ts:34-60),
+// added to the particle's own ticks.
+static __device__ __forceinline__ void adhesion_add(const SceneDev& S, const FluidDev& F, float3 xi, float3 xj,
+                                                    float vol, int tk[3]) {
+  const float3 off = pf_sub3(xi, xj);
+  const float r = pf_len3(off);
+  if (r < 1e-20f) return;
+  const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
+  const float ir = pf_rcp_rn(r);  // r >= 1e-20
+  const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
+  tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
+  tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
+  tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
+}
+
 // Surface tension and adhesion of one particle in a table below 64 buckets
-// (global slots, one neighbour at a time). Such scenes hold at most 16
-// particles, so no list is truncated and both ends list every pair. A pair
-// with an end the contact stabilisation moved is the lower end's alone: it
-// adds its half and scatters the partner's into the accumulator.
+// (global slots, one neighbour at a time; such scenes hold at most 16
+// particles).
 static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p, int c, uint32_t s, uint32_t id,
                                      float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_on, bool adh_on,
-                                     bool ct_scene, bool moved_i, int tk[3]) {
+                                     int tk[3]) {
   const float4* __restrict__ xs = S.xs[c];
   const uint32_t n = S.n;
   const uint32_t* __restrict__ nl = F.nl;
   const uint32_t fend = F.start + F.count;
   for (uint32_t k = 0; k < cnt; ++k) {
     const uint32_t j = nl[(size_t)k * n + s];
-    const bool moved = moved_i || (ct_scene && S.ct.acc[j].w != 0);
     if (st_on) {
       const uint32_t idj = S.id[p][j];
       if (idj > id && idj < fend) {
@@ -828,65 +841,34 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
-          if (moved) scatter_half(S.ct.acc, j, dv, F.imp_scale);
-        }
-      } else if (!moved && idj < id && idj >= F.start) {
-        float3 dv;
-        if (pair_dv(F, S.dt, pf_xyz(xs[j]), xi, pf_xyz(F.normal[j]), ni, S.rho[p][j], rhoi, &dv)) {
-          tk[0] = pf_wrap_add(tk[0], pf_ticks(-dv.x, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks(-dv.y, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks(-dv.z, F.imp_scale));
+          scatter_half(S.ct.acc, j, dv, F.imp_scale);
         }
       }
     }
     if (adh_on) {
       const float vol = S.bvol[p][j];
-      if (vol > 0.0f) {
-        const float3 off = pf_sub3(xi, pf_xyz(xs[j]));
-        const float r = pf_len3(off);
-        if (!(r < 1e-20f)) {
-          const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vol)) * adhesion_spline(F, r));
-          const float ir = pf_rcp_rn(r);  // r >= 1e-20
-          const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
-          tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
-          tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
-          tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
-        }
-      }
+      if (vol > 0.0f) adhesion_add(S, F, xi, pf_xyz(xs[j]), vol, tk);
     }
   }
 }
 
-// The gathered surface tension and adhesion of one particle over its list
-// (cohesion.ts:63-103, adhesion.ts:34-60). Four neighbours at a time, every
-// gather of the four issued before the arithmetic (the pass is
-// latency-bound). Each pair is evaluated from its lower id's side: operands
-// swapped and the change negated on the upper side. The fixed-point sums do
-// not depend on the order.
-// The reference scatters each pair from the list of its lower id, the change
-// added to both ends, so a pair exists iff the lower end's list holds it.
-// SPECIAL: the scene cut a list at 64 or has contacts. The upper end then
-// takes its half only if the lower partner's list holds this slot, which a
-// truncated partner's walk key decides (the partner's id, key and mark are
-// read first, its position, normal and density only for the pairs evaluated
-// here). The halves the upper end cannot see are scattered by the lower end
-// into the upper end's accumulator: those of pairs its truncated list
-// dropped (neighborList.ts:63-66), and both halves' partner side for pairs
-// with an end the contact stabilisation moved (whose query point is the
-// stabilised prediction and candidates the sorted copy, so the two lists may
-// disagree). Otherwise every fluid partner is evaluated here and keys and
-// marks are not read.
-template <bool SPECIAL>
+// Surface tension and adhesion of one particle over its list (cohesion.ts:63-103,
+// adhesion.ts:34-60), four neighbours at a time with every gather of the four
+// issued before the arithmetic. The reference scatters each pair from the
+// list of its lower id, the change added to both ends, so a pair exists iff
+// the lower end's list holds it: this particle evaluates the pairs its list
+// holds with a higher fluid id, adds the change to its own ticks and the
+// negated change to the partner's (in LDS for a staged tile's union entry,
+// flushed once per tile; else straight to the accumulator). Pairs where it is
+// the upper end arrive from the partner the same way.
 static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const FluidDev& F, int p,
                                                     const float4* __restrict__ xs, uint32_t s, uint32_t id,
                                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
-                                                    bool adh_on, bool moved_i, bool moved_tile, uint32_t U,
-                                                    const uint32_t* __restrict__ uslots, int* acc_l, bool lds_acc,
-                                                    int tk[3]) {
+                                                    bool adh_on, uint32_t U, const uint32_t* __restrict__ uslots,
+                                                    int* acc_l, bool lds_acc, int tk[3]) {
   const uint32_t n = S.n;
   const uint32_t fend = F.start + F.count;
   const bool staged = U != PF_TU_GLOBAL;
-  const uint32_t ci = SPECIAL ? F.nl_cell[s] : 0u;
   Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
   NlCursor cg(F.nl, n, s, cnt);
   const PF_G uint32_t* __restrict__ g_slot = pf_g(uslots);
@@ -895,9 +877,6 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
   const PF_G float4* __restrict__ g_nrm = pf_g(F.normal);
   const PF_G float* __restrict__ g_rho = pf_g(S.rho[p]);
   const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
-  const PF_G uint32_t* __restrict__ g_last = pf_g(F.nl_last);
-  const PF_G uint32_t* __restrict__ g_cell = pf_g(F.nl_cell);
-  const PF_G int4* __restrict__ g_acc = pf_g(S.ct.acc);
   for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
     uint32_t jj[4];
     uint32_t idx[4] = {0u, 0u, 0u, 0u};
@@ -908,113 +887,60 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
     } else {
       cg.next(k0, jj);
     }
-    uint32_t idj[4], lastj[4], cj[4];
-    float4 xj[4], nj[4];
-    float rj[4], vol[4];
-    bool lower[4], upper[4], moved[4];
+    uint32_t idj[4];
+    float vol[4];
+    bool lower[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       idj[u] = g_id[jj[u]];
-      lastj[u] = 0xffffffffu;
-      moved[u] = false;
-      if (SPECIAL && st_pairs) {
-        lastj[u] = g_last[jj[u]];
-        moved[u] = moved_i || (moved_tile && g_acc[jj[u]].w != 0);
-      }
       vol[u] = adh_on ? g_vol[jj[u]] : 0.0f;
-      if (!SPECIAL) {
-        xj[u] = pf_f4(g_x[jj[u]]);
-        if (st_pairs) {
-          nj[u] = pf_f4(g_nrm[jj[u]]);
-          rj[u] = g_rho[jj[u]];
-        }
-      }
-      const bool in = k0 + u < cnt;
-      // A pair with a moved end is its lower end's alone.
-      lower[u] = st_pairs && in && idj[u] > id && idj[u] < fend;
-      upper[u] = st_pairs && in && !moved[u] && idj[u] < id && idj[u] >= F.start;
     }
-    if (SPECIAL) {
+    float4 xj[4], nj[4];
+    float rj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool pair = lower[u] || upper[u];
-        xj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-        rj[u] = 0.0f;
-        cj[u] = 0u;
-        if ((upper[u] || lower[u]) && lastj[u] != 0xffffffffu) cj[u] = g_cell[jj[u]];
-        if (pair || vol[u] > 0.0f) xj[u] = pf_f4(g_x[jj[u]]);
-        if (pair) {
-          nj[u] = pf_f4(g_nrm[jj[u]]);
-          rj[u] = g_rho[jj[u]];
-        }
+    for (int u = 0; u < 4; ++u) {
+      const bool in = k0 + u < cnt;
+      lower[u] = st_pairs && in && idj[u] > id && idj[u] < fend;
+      xj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      rj[u] = 0.0f;
+      if (lower[u] || (in && vol[u] > 0.0f)) xj[u] = pf_f4(g_x[jj[u]]);
+      if (lower[u]) {
+        nj[u] = pf_f4(g_nrm[jj[u]]);
+        rj[u] = g_rho[jj[u]];
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const bool in = k0 + u < cnt;
       const float3 xjv = pf_xyz(xj[u]);
-      if (st_pairs) {
-        bool up = upper[u];
-        if (SPECIAL && up && lastj[u] != 0xffffffffu) {
-          // The pair exists in the reference iff this slot is in j's
-          // truncated list: compare this slot's walk key in j's walk (from
-          // both list-build cells) with the key of j's last kept neighbour.
-          up = walk_key(ci, cj[u], s) <= lastj[u];
-        }
-        const bool lo = lower[u];
-        if (lo || up) {
-          const float3 njv = pf_xyz(nj[u]);
-          float3 dv;
-          const bool ok = pair_dv(F, S.dt, sel3(lo, xi, xjv), sel3(lo, xjv, xi), sel3(lo, ni, njv), sel3(lo, njv, ni),
-                                  lo ? rhoi : rj[u], lo ? rj[u] : rhoi, &dv);
-          if (ok) {
-            const float sg = lo ? 1.0f : -1.0f;
-            tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x * sg, F.imp_scale));
-            tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y * sg, F.imp_scale));
-            tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z * sg, F.imp_scale));
-            // The partner's half when its own walk cannot take it: a moved
-            // end, or its truncated list dropped this slot.
-            if (SPECIAL && lo &&
-                (moved[u] || (lastj[u] != 0xffffffffu && walk_key(ci, cj[u], s) > lastj[u]))) {
-              if (lds_acc) {
-                // the partner is union entry idx: summed in LDS, flushed once per tile
-                atomicAdd(acc_l + 3 * idx[u] + 0, pf_ticks(-dv.x, F.imp_scale));
-                atomicAdd(acc_l + 3 * idx[u] + 1, pf_ticks(-dv.y, F.imp_scale));
-                atomicAdd(acc_l + 3 * idx[u] + 2, pf_ticks(-dv.z, F.imp_scale));
-              } else {
-                scatter_half(S.ct.acc, jj[u], dv, F.imp_scale);
-              }
-            }
+      if (lower[u]) {
+        float3 dv;
+        if (pair_dv(F, S.dt, xi, xjv, ni, pf_xyz(nj[u]), rhoi, rj[u], &dv)) {
+          tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
+          tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
+          tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
+          if (lds_acc) {
+            // the partner is union entry idx: summed in LDS, flushed once per tile
+            atomicAdd(acc_l + 3 * idx[u] + 0, pf_ticks(-dv.x, F.imp_scale));
+            atomicAdd(acc_l + 3 * idx[u] + 1, pf_ticks(-dv.y, F.imp_scale));
+            atomicAdd(acc_l + 3 * idx[u] + 2, pf_ticks(-dv.z, F.imp_scale));
+          } else {
+            scatter_half(S.ct.acc, jj[u], dv, F.imp_scale);
           }
         }
       }
-      if (adh_on && in) {
-        const float vj = vol[u];
-        if (vj > 0.0f) {
-          const float3 off = pf_sub3(xi, xjv);
-          const float r = pf_len3(off);
-          if (!(r < 1e-20f)) {
-            const float fs = -(((F.adhesion * F.mass) * (F.rho0 * vj)) * adhesion_spline(F, r));
-            const float ir = pf_rcp_rn(r);  // r >= 1e-20
-            const float3 f = make_float3((off.x * ir) * fs, (off.y * ir) * fs, (off.z * ir) * fs);
-            tk[0] = pf_wrap_add(tk[0], pf_ticks((f.x * F.inv_mass) * S.dt, F.imp_scale));
-            tk[1] = pf_wrap_add(tk[1], pf_ticks((f.y * F.inv_mass) * S.dt, F.imp_scale));
-            tk[2] = pf_wrap_add(tk[2], pf_ticks((f.z * F.inv_mass) * S.dt, F.imp_scale));
-          }
-        }
-      }
+      if (adh_on && k0 + u < cnt && vol[u] > 0.0f) adhesion_add(S, F, xi, xjv, vol[u], tk);
     }
   }
 }
 
-// One particle's surface tension and adhesion. In a special scene (cut
-// lists or contacts) its own sum is parked, as bits, in its slot of the
-// output prediction buffer for k_impulse_apply; otherwise it is applied here.
-template <bool ST, bool ADH, bool CT>
+// One particle's surface tension and adhesion. With surface tension its own
+// sum is parked, as bits, in its slot of the output prediction buffer for
+// k_impulse_apply, which adds the halves its lower partners scattered; a
+// fluid with adhesion alone applies here.
+template <bool ST, bool ADH>
 static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const FluidDev& F, int p, int c,
-                                                        uint32_t s, uint32_t tile, bool special, bool ct_scene,
-                                                        int* acc_l, bool lds_acc) {
+                                                        uint32_t s, uint32_t tile, int* acc_l, bool lds_acc) {
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const uint32_t id = S.id[p][s];
@@ -1039,23 +965,14 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
     ni = pf_xyz(F.normal[s]);
     rhoi = S.rho[p][s];
   }
-  bool moved_tile = false, moved_i = false;
-  if (ct_scene) {
-    moved_tile = F.tu_moved[tile] != 0u;
-    moved_i = S.ct.acc[s].w != 0;
-  }
   const uint32_t U = F.tu_size[tile];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
   if (S.H < 64u) {
-    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, ct_scene, moved_i, tk);
-  } else if (special) {
-    impulse_walk<true>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, moved_i, moved_tile, U, uslots, acc_l,
-                       lds_acc, tk);
+    impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, tk);
   } else {
-    impulse_walk<false>(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, false, false, U, uslots, acc_l, false,
-                        tk);
+    impulse_walk(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, uslots, acc_l, lds_acc, tk);
   }
-  if (special) {
+  if (st_on) {
     S.xs[c ^ 1][s] = make_float4(__int_as_float(tk[0]), __int_as_float(tk[1]), __int_as_float(tk[2]), xi4.w);
     return;
   }
@@ -1070,12 +987,11 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
   S.xs[c ^ 1][s] = make_float4(xi.x + ch.x * S.dt, xi.y + ch.y * S.dt, xi.z + ch.z * S.dt, xi4.w);
 }
 
-// Scenes that cut a list or have contacts (CT: some scene of the launch has
-// them) scatter pair halves to partners during this pass; a staged tile sums
-// the halves for its union's entries in LDS and adds each touched entry's
-// sums to the accumulator once at the end. k_impulse_apply applies once
-// every half has arrived. The moved marks (.w) stay.
-template <bool ST, bool ADH, bool CT>
+// A staged tile sums the halves its particles scatter to its union's entries
+// in LDS and adds each touched entry's sums to the accumulator once at the
+// end; k_impulse_apply applies once every half has arrived. The moved marks
+// (.w) stay.
+template <bool ST, bool ADH>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, int c) {
@@ -1086,17 +1002,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   const uint32_t tile = t.first / PF_BLOCK;
   // Uniform over the block.
   const bool st_on = ST && (F.effects & PF_FX_SURFACE_TENSION);
-  const bool trunc_scene = st_on && S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
-  const bool ct_scene = CT && st_on && S.ct.on;
-  const bool special = trunc_scene || ct_scene;
   const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
-  const bool lds = ST && special && S.H >= 64u && U != PF_TU_GLOBAL;
+  const bool lds = ST && st_on && S.H >= 64u && U != PF_TU_GLOBAL;
   if (lds) {
     for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
     __syncthreads();
   }
   const uint32_t s = t.first + threadIdx.x;
-  if (s < S.n) impulse_particle<ST, ADH, CT>(S, F, p, c, s, tile, special, ct_scene, acc_l, lds);
+  if (s < S.n) impulse_particle<ST, ADH>(S, F, p, c, s, tile, acc_l, lds);
   if (lds) {
     __syncthreads();
     const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
@@ -1113,20 +1026,16 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
 }
 
 // The impulse apply (FluidSystem.ts:253-258, accumulator.ts:101-131) of the
-// fluid particles of scenes whose surface tension went through the
-// accumulator in k_impulses: the particle's own sum (parked in its output
-// prediction slot) plus the halves scattered to it; v += change,
-// x* += change dt, sums cleared (moved marks kept). Other slots and scenes
-// were applied by k_impulses.
+// fluid particles of scenes with surface tension: the particle's own sum
+// (parked in its output prediction slot) plus the halves scattered to it;
+// v += change, x* += change dt, sums cleared (moved marks kept).
 __global__ __launch_bounds__(PF_BLOCK) void k_impulse_apply(const SceneDev* __restrict__ scenes,
                                                             const Tile* __restrict__ tiles, uint32_t ntiles,
-                                                            int fi, int p, int c, int contact) {
+                                                            int fi, int p, int c) {
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
   if (!(F.effects & PF_FX_SURFACE_TENSION)) return;
-  const bool trunc_scene = S.flags[(p ^ 1) * PF_NUM_FLAGS + PF_FLAG_NEIGHBORS] != 0u;
-  if (!trunc_scene && !(contact && S.ct.on)) return;
   const uint32_t s = t.first + threadIdx.x;
   if (s >= S.n || !in_range(S.id[p][s], F.start, F.count)) return;
   int* a = reinterpret_cast<int*>(S.ct.acc + s);
@@ -1918,22 +1827,19 @@ void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
 }
 
 void launch_impulse_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
-                          int c, bool contact) {
+                          int c) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_impulse_apply, PF_GRID, sc, tl, ntiles, fi, p, c, contact ? 1 : 0);
+  hipLaunchKernelGGL(k_impulse_apply, PF_GRID, sc, tl, ntiles, fi, p, c);
 }
 
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, int c, bool surface_tension, bool adhesion, bool contact) {
+                     int p, int c, bool surface_tension, bool adhesion) {
   if (!ntiles) return;
-#define PF_IK(a, b, t) hipLaunchKernelGGL((k_impulses<a, b, t>), PF_GRID, sc, tl, ntiles, fi, p, c)
-  if (surface_tension && contact) {
-    if (adhesion) PF_IK(true, true, true);
-    else PF_IK(true, false, true);
-  } else if (surface_tension && adhesion) PF_IK(true, true, false);
-  else if (surface_tension) PF_IK(true, false, false);
-  else if (adhesion) PF_IK(false, true, false);
-  else PF_IK(false, false, false);
+#define PF_IK(a, b) hipLaunchKernelGGL((k_impulses<a, b>), PF_GRID, sc, tl, ntiles, fi, p, c)
+  if (surface_tension && adhesion) PF_IK(true, true);
+  else if (surface_tension) PF_IK(true, false);
+  else if (adhesion) PF_IK(false, true);
+  else PF_IK(false, false);
 #undef PF_IK
 }
 

@@ -20,10 +20,10 @@ void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool normal, uint32_t cap, int snap);
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, int c, bool surface_tension, bool adhesion, bool contact);
-// surface-tension pairs scattered into the contact accumulator (scenes with contacts)
+                     int p, int c, bool surface_tension, bool adhesion);
+// applies the surface-tension halves scattered into the accumulator
 void launch_impulse_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
-                          int c, bool contact);
+                          int c);
 void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                    int p, int c, uint32_t cap);
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,

@@ -145,12 +145,12 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       if (stx || adh) {
         {
           PF_LAUNCH(ctx, "impulses");
-          launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary, sh.contact);
+          launch_impulses(st, scenes, tiles, ntiles, (int)f, p, c, stx, adh && sh.any_boundary);
           ++nk;
         }
         if (stx) {
           PF_LAUNCH(ctx, "impulse_apply");
-          launch_impulse_apply(st, scenes, tiles, ntiles, (int)f, p, c, sh.contact);
+          launch_impulse_apply(st, scenes, tiles, ntiles, (int)f, p, c);
           ++nk;
         }
         c ^= 1;
