@@ -187,9 +187,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_stabilize_apply(const Scen
   float4 xs = S.xs[c][s];
   if (S.ct.on) {
     const int4 a = S.ct.acc[s];
-    // .w marks a particle whose prediction now differs from the grid's sorted
-    // copy; the surface-tension passes read it until the first contact solve.
-    S.ct.acc[s] = make_int4(0, 0, 0, (a.x | a.y | a.z) != 0 ? 1 : 0);
+    S.ct.acc[s] = make_int4(0, 0, 0, 0);
     const int tk[3] = {a.x, a.y, a.z};
     bool sat;
     const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);

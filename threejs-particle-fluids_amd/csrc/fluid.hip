@@ -5,10 +5,9 @@
 // sorted slots, so neighbour reads land in nearby cache lines):
 //   k_neighbors     27-bucket walk -> neighbour list, fused colour normal
 //                   (neighborList.ts:45-72, cohesion.ts:25-39)
-//   k_impulses      surface tension + adhesion as a gather with exact
-//                   fixed-point sums, fused impulse apply; in scenes that cut
-//                   lists or have contacts the halves only the lower end's
-//                   list holds are scattered there and k_impulse_apply applies
+//   k_impulses      surface tension, each pair from its lower end with the
+//                   partner's half scattered (exact fixed-point sums), and
+//                   adhesion; k_impulse_apply applies the sums
 //                   (cohesion.ts:63-103, adhesion.ts:34-60, accumulator.ts:101-131)
 //   k_lambda        density and lambda (lambda.ts:30-71)
 //   k_delta         position correction + apply, optionally fused container
@@ -117,24 +116,6 @@ static __device__ __forceinline__ float3 sel3(bool ok, float3 a, float3 b) {
   return make_float3(ok ? a.x : b.x, ok ? a.y : b.y, ok ? a.z : b.z);
 }
 
-// A walk's query cell, the low 10 bits of each axis.
-static __device__ __forceinline__ uint32_t pack_cell(const int q[3]) {
-  return ((uint32_t)q[0] & 1023u) | (((uint32_t)q[1] & 1023u) << 10) | (((uint32_t)q[2] & 1023u) << 20);
-}
-
-// Walk key (walk order: stencil dz -> dy -> dx, then slot) of the particle in
-// slot `found`, whose list-build cell is `cf`, in the walk from cell `cw`.
-// Tables of 64 buckets or more give the 27 stencil cells distinct buckets, so
-// a particle within h is found at exactly one stencil position, its cell
-// minus the walk's; that difference is -1, 0 or 1 per axis and survives the
-// 10-bit wrap.
-static __device__ __forceinline__ uint32_t walk_key(uint32_t cf, uint32_t cw, uint32_t found) {
-  const uint32_t dx = (cf - cw + 1u) & 1023u;
-  const uint32_t dy = ((cf >> 10) - (cw >> 10) + 1u) & 1023u;
-  const uint32_t dz = ((cf >> 20) - (cw >> 20) + 1u) & 1023u;
-  return ((dz * 9u + dy * 3u + dx) << 27) | found;
-}
-
 // Colour-field normal n_i = h * sum_j V_j grad W (cohesion.ts:25-39) over a
 // finished global-slot list, in list order.
 static __device__ __forceinline__ float4 color_normal_global(const SceneDev& S, const FluidDev& F, int p,
@@ -185,13 +166,12 @@ static __device__ __forceinline__ uint32_t cu_home(uint32_t key, uint32_t hs) {
 // thread's list length (0 for threads without a list). When at most `limit`
 // distinct slots appear, they become the tile's union (F.tu_slots, in
 // table order), each list is rewritten as 16-bit union indices (F.nl16) and
-// the union size is returned; otherwise PF_TU_GLOBAL. Scenes with contacts
-// also get the tile's moved mark, over the new union. The table lives in the
+// the union size is returned; otherwise PF_TU_GLOBAL. The table lives in the
 // union's LDS, 3 * cap words: 2 * cap keys (slot + 1) and as many 16-bit
 // dense indices.
 static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* lds,
                                          uint32_t cap, uint32_t limit, uint32_t* ctl, uint32_t count,
-                                         uint32_t s, bool snap) {
+                                         uint32_t s) {
   const uint32_t tid = threadIdx.x;
   const uint32_t hs = 2u * cap;
   uint32_t* hk = lds;                                        // [hs] slot + 1, 0 = empty
@@ -228,10 +208,7 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   }
   __syncthreads();
   // Uniform over the block.
-  if (ctl[1] != 0u || ctl[0] > limit || limit == 0u) {
-    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
-    return PF_TU_GLOBAL;
-  }
+  if (ctl[1] != 0u || ctl[0] > limit || limit == 0u) return PF_TU_GLOBAL;
   // Dense indices in table order: each thread numbers a contiguous chunk.
   const uint32_t per = (hs + PF_BLOCK - 1u) / PF_BLOCK;
   const uint32_t b0 = min(tid * per, hs), b1 = min(b0 + per, hs);
@@ -240,22 +217,15 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   uint32_t U = 0;
   uint32_t at = block_exclusive_scan(mine, ctl + 4, &U);
   PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
-  int moved = 0;
   for (uint32_t i = b0; i < b1; ++i) {
     const uint32_t key = hk[i];
     if (key) {
       hv[i] = (uint16_t)at;
       slots[at] = key - 1u;
-      if (snap && S.ct.on) moved |= S.ct.acc[key - 1u].w;
       ++at;
     }
   }
-  if (snap) {
-    const int any = __syncthreads_or(moved);
-    if (tid == 0 && S.ct.on) F.tu_moved[tile] = any ? 1u : 0u;
-  } else {
-    __syncthreads();
-  }
+  __syncthreads();
   // The lists as pairs of union indices, column-major like the staged walk's.
   // Eight entries loaded per round trip; entries past the count are zero.
   PF_G uint32_t* __restrict__ row = pf_g(F.nl16) + s;
@@ -347,15 +317,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   const float lim2 = lim * lim;
   const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
   uint32_t count = 0;
-  uint32_t last_key = 0xffffffffu;
   bool over = false;
 
   // Tables below 64 buckets: global walk with the reference's bucket dedup.
-  // Such tables belong to scenes of at most 16 particles (H = nextPow2(2N)),
-  // whose lists cannot reach the 64-entry cap, so no walk key is kept.
+  // Such tables belong to scenes of at most 16 particles (H = nextPow2(2N)).
   if (S.H < 64u) {
     if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
-    if (tid == 0 && snap && S.ct.on) F.tu_moved[tile] = 1u;
     if (!active) return;
     uint32_t visited[27];
     uint32_t nvisited = 0;
@@ -382,7 +349,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           }
         }
     F.nl_count[s] = count;
-    F.nl_last[s] = 0xffffffffu;
     if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     return;
@@ -476,7 +442,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
           const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
                                        (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
-          const uint32_t stencil = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
           const uint32_t st = S.start[b];
           const uint32_t en = S.start[b + 1];
           // Sixteen candidates (four cache lines) per round trip to memory.
@@ -503,7 +468,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
               m &= m - 1u;
               *row = j;
               row += n;
-              last_key = (stencil << 27) | j;
               ++count;
             }
           }
@@ -511,11 +475,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       }
     }
     F.nl_count[s] = count;
-    // Walk position of the last kept neighbour of a truncated list, and the
-    // walk's cell: the surface-tension passes use them to know which
-    // partners a list dropped.
-    F.nl_last[s] = over ? last_key : 0xffffffffu;
-    F.nl_cell[s] = pack_cell(q);
     if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     }
@@ -523,7 +482,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     // those fit, they become the tile's union and the lists are rewritten as
     // 16-bit indices into it, so the tile is staged in LDS like the others.
     const uint32_t Uc = compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl,
-                                      active ? count : 0u, s, snap != 0);
+                                      active ? count : 0u, s);
     if (tid == 0) F.tu_size[tile] = Uc;
     return;
   }
@@ -580,7 +539,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     float4 xv[PF_TU_PER];
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
-    int moved = 0;
 #pragma unroll
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = tid + i * PF_BLOCK;
@@ -589,14 +547,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
         u_py[u] = xv[i].y;
         u_pz[u] = xv[i].z;
         slots[u] = js[i];
-        if (snap && S.ct.on) moved |= S.ct.acc[js[i]].w;
       }
-    }
-    // Whether any union member was moved by the contact stabilisation: the
-    // surface-tension passes handle this tile's pairs with them apart.
-    if (snap) {
-      const int any = __syncthreads_or(moved);
-      if (tid == 0 && S.ct.on) F.tu_moved[tile] = any ? 1u : 0u;
     }
   }
   __syncthreads();
@@ -604,7 +555,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   // 4. Walk the union. Completed entry pairs go straight to their list row.
   PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
   if (active) {
-    uint32_t pend = 0, last_local = 0;
+    uint32_t pend = 0;
     PF_G uint32_t* row = nl16 + s;  // the row of the next entry pair
     for (int dz = -1; dz <= 1 && !over; ++dz) {
       const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
@@ -651,7 +602,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
                 row += n;
               }
               pend = local;
-              last_local = local;
               ++count;
             }
           };
@@ -663,16 +613,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     }
     if (count & 1u) *row = pend;
     F.nl_count[s] = count;
-    // Walk key of the last kept neighbour of a truncated list, from its
-    // list-build cell and sorted slot (the staged union holds both).
-    if (over) {
-      const float3 xl = make_float3(u_px[last_local], u_py[last_local], u_pz[last_local]);
-      const int cf[3] = {pf_query_coord(xl.x, S.ox, S.cell), pf_query_coord(xl.y, S.oy, S.cell),
-                         pf_query_coord(xl.z, S.oz, S.cell)};
-      last_key = walk_key(pack_cell(cf), pack_cell(q), F.tu_slots[(size_t)tile * PF_TU_MAX + last_local]);
-    }
-    F.nl_last[s] = over ? last_key : 0xffffffffu;
-    F.nl_cell[s] = pack_cell(q);
     if (want_normal) {
       // Colour-field normal over the finished list, in list order.
       const PF_G uint32_t* __restrict__ uslots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
@@ -989,8 +929,7 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
 
 // A staged tile sums the halves its particles scatter to its union's entries
 // in LDS and adds each touched entry's sums to the accumulator once at the
-// end; k_impulse_apply applies once every half has arrived. The moved marks
-// (.w) stay.
+// end; k_impulse_apply applies once every half has arrived.
 template <bool ST, bool ADH>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
@@ -1028,7 +967,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
 // The impulse apply (FluidSystem.ts:253-258, accumulator.ts:101-131) of the
 // fluid particles of scenes with surface tension: the particle's own sum
 // (parked in its output prediction slot) plus the halves scattered to it;
-// v += change, x* += change dt, sums cleared (moved marks kept).
+// v += change, x* += change dt, sums cleared.
 __global__ __launch_bounds__(PF_BLOCK) void k_impulse_apply(const SceneDev* __restrict__ scenes,
                                                             const Tile* __restrict__ tiles, uint32_t ntiles,
                                                             int fi, int p, int c) {

@@ -79,11 +79,7 @@ struct FluidDev {
   uint32_t tu_limit;   // largest union a tile stages (<= PF_TU_MAX)
   uint32_t* tu_size;   // [tiles] union size of each 256-slot tile, or PF_TU_GLOBAL
   uint32_t* tu_slots;  // [tiles * PF_TU_MAX] sorted slot of each union entry
-  uint32_t* tu_moved;  // [tiles] scenes with contacts: the tile's union holds a particle the
-                       // contact stabilisation moved this substep (1 for global-walk tiles)
   uint32_t* nl_count;  // [n]
-  uint32_t* nl_last;   // [n] walk key of the last kept neighbour when the list overflowed, else ~0
-  uint32_t* nl_cell;   // [n] the walk's query cell, low 10 bits per axis (x | y << 10 | z << 20)
   float4* normal;      // [n] colour-field normal
   float4* omega;       // [n] vorticity (xyz) and |omega| (w)
   float4* dv;          // [n] XSPH velocity change
@@ -160,9 +156,8 @@ struct ContactDev {
   uint2* pair;               // [max] (emitter slot, other slot)
   int2* lam;                 // [max] (lambda_n, lambda_t) ticks
   float4* nrm;               // [max] last normal the solve used
-  int4* acc;                 // [n] fixed-point sums scattered by the per-contact passes and by surface
-                             // tension's scattered pairs (also without contacts); from the
-                             // stabilisation to the first solve, .w = 1 marks a particle it moved
+  int4* acc;                 // [n] fixed-point sums (xyz) scattered by the per-contact passes and by
+                             // surface tension's pairs (also without contacts); .w unused
 };
 
 struct SceneDev {

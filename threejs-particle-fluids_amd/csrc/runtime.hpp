@@ -196,10 +196,7 @@ struct pf_fluid_s {
   uint32_t tu_limit = PF_TU_MAX;
   uint32_t* tu_size = nullptr;
   uint32_t* tu_slots = nullptr;
-  uint32_t* tu_moved = nullptr;
   uint32_t* nl_count = nullptr;
-  uint32_t* nl_last = nullptr;
-  uint32_t* nl_cell = nullptr;
   float4* normal = nullptr;
   float4* omega = nullptr;
   float4* dv = nullptr;

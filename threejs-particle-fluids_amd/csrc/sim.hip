@@ -549,11 +549,11 @@ int pf_fluid_destroy(pf_fluid* f) {
   PF_API_BEGIN
   if (!f || f->disposed) return PF_OK;
   f->disposed = true;
-  dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->nl_cell); dfree(f->normal); dfree(f->omega); dfree(f->dv);
+  dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv);
   dfree(f->reaction);
-  dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots); dfree(f->tu_moved);
-  f->nl16 = f->tu_size = f->tu_slots = f->tu_moved = nullptr;
-  f->nl = nullptr; f->nl_count = nullptr; f->nl_last = f->nl_cell = nullptr; f->normal = f->omega = f->dv = nullptr;
+  dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
+  f->nl16 = f->tu_size = f->tu_slots = nullptr;
+  f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr;
   f->reaction = nullptr;
   PF_API_END
 }
@@ -1233,13 +1233,10 @@ static void fluid_write_params(pf_fluid* f) {
   }
   F.nl = f->nl;
   F.nl_count = f->nl_count;
-  F.nl_last = f->nl_last;
-  F.nl_cell = f->nl_cell;
   F.nl16 = f->nl16;
   F.tu_limit = f->tu_limit;
   F.tu_size = f->tu_size;
   F.tu_slots = f->tu_slots;
-  F.tu_moved = f->tu_moved;
   if (f->isv) {
     // rate = (nu * dt) * (10 / h^2), the constant computed in double (ViscositySolver.ts:103)
     F.isv_iters = f->isv->iterations;
@@ -1436,13 +1433,10 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     f->loop = L.get();
     f->nl = dalloc<uint32_t>((size_t)PF_MAX_NEIGHBORS * n);
     f->nl_count = dalloc<uint32_t>(n);
-    f->nl_last = dalloc<uint32_t>(n);
-    f->nl_cell = dalloc<uint32_t>(n);
     const size_t ntiles = ((size_t)n + PF_BLOCK - 1) / PF_BLOCK;
     f->nl16 = dalloc<uint32_t>((size_t)(PF_MAX_NEIGHBORS / 2) * n);
     f->tu_size = dalloc<uint32_t>(ntiles);
     f->tu_slots = dalloc<uint32_t>(ntiles * PF_TU_MAX);
-    f->tu_moved = L->contact ? dalloc<uint32_t>(ntiles) : nullptr;
     if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
@@ -1951,10 +1945,10 @@ int pf_loop_destroy(pf_loop* L) {
   S.ct = ContactDev{};
   for (auto* f : L->fluids) {
     f->loop = nullptr;
-    dfree(f->nl); dfree(f->nl_count); dfree(f->nl_last); dfree(f->nl_cell); dfree(f->normal); dfree(f->omega); dfree(f->dv);
-    dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots); dfree(f->tu_moved);
-    f->nl16 = f->tu_size = f->tu_slots = f->tu_moved = nullptr;
-    f->nl = nullptr; f->nl_count = nullptr; f->nl_last = f->nl_cell = nullptr; f->normal = f->omega = f->dv = nullptr;
+    dfree(f->nl); dfree(f->nl_count); dfree(f->normal); dfree(f->omega); dfree(f->dv);
+    dfree(f->nl16); dfree(f->tu_size); dfree(f->tu_slots);
+    f->nl16 = f->tu_size = f->tu_slots = nullptr;
+    f->nl = nullptr; f->nl_count = nullptr; f->normal = f->omega = f->dv = nullptr;
   }
   if (L->prims) L->prims->loop = nullptr;
   for (auto* c : L->sdfs) {
