@@ -151,18 +151,24 @@ static inline REAL SFX(poly6)(REAL r2, REAL h2, REAL coef) {
   return ((d * d) * d) * coef;
 }
 
-/* kernels.ts:89-107. The direction r / max(|r|, 1e-20) takes one rounded
- * reciprocal and three products (the reference's WGSL divides per component
- * with 2.5-ulp division; this form stays within that). */
-static inline void SFX(spiky)(const REAL r[3], REAL h, REAL coef, REAL g[3]) {
+/* kernels.ts:89-107: grad W = r * sp, with the scalar
+ * sp = -(t*t*coef) / max(|r|, 1e-20), t = max(h - |r|, 0). The passes fold
+ * their per-neighbour factor into sp before it meets r (one product per
+ * component); the reference's WGSL divides per component with 2.5-ulp
+ * division, and this stays within that. */
+static inline REAL SFX(spiky_s)(const REAL r[3], REAL h, REAL coef) {
   REAL rsq = SFX(dot)(r, r);
   REAL rl = RSQRT(rsq);
   REAL t = RFMAX(h - rl, R(0.0));
   REAL inv = R(1.0) / RFMAX(rl, (REAL)1e-20);
   REAL mag = -((t * t) * coef);
-  g[0] = (r[0] * inv) * mag;
-  g[1] = (r[1] * inv) * mag;
-  g[2] = (r[2] * inv) * mag;
+  return inv * mag;
+}
+static inline void SFX(spiky)(const REAL r[3], REAL h, REAL coef, REAL g[3]) {
+  REAL sp = SFX(spiky_s)(r, h, coef);
+  g[0] = r[0] * sp;
+  g[1] = r[1] * sp;
+  g[2] = r[2] * sp;
 }
 
 /* accumulator.ts:62-68: i32(round(v * scale)), half-to-even, saturating */
@@ -317,10 +323,10 @@ static void SFX(color_normal)(SFX(scene)* s, SFX(fluid)* F) {
     for (uint32_t k = 0; k < F->nlc[i]; ++k) {
       uint32_t j = F->nl[(size_t)i * 64 + k];
       const REAL* xj = s->xs + 3 * (size_t)j;
-      REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
-      SFX(spiky)(d, F->h, F->spiky, g);
+      REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
       REAL V = SFX(nvol)(s, F, j);
-      sum[0] = sum[0] + g[0] * V; sum[1] = sum[1] + g[1] * V; sum[2] = sum[2] + g[2] * V;
+      REAL sc = SFX(spiky_s)(d, F->h, F->spiky) * V;
+      sum[0] = sum[0] + d[0] * sc; sum[1] = sum[1] + d[1] * sc; sum[2] = sum[2] + d[2] * sc;
     }
     for (int a = 0; a < 3; ++a) F->normal[3 * (size_t)i + a] = sum[a] * F->h;
   }
@@ -408,8 +414,8 @@ static void SFX(lambda_pass)(SFX(scene)* s, SFX(fluid)* F) {
       REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
       REAL mj = SFX(nmass)(s, F, j);
       rho = rho + mj * SFX(poly6)(SFX(dot)(d, d), F->h2, F->poly6);
-      SFX(spiky)(d, F->h, F->spiky, g);
-      g[0] = g[0] * mj; g[1] = g[1] * mj; g[2] = g[2] * mj;
+      REAL sc = SFX(spiky_s)(d, F->h, F->spiky) * mj;
+      g[0] = d[0] * sc; g[1] = d[1] * sc; g[2] = d[2] * sc;
       gs[0] = gs[0] + g[0]; gs[1] = gs[1] + g[1]; gs[2] = gs[2] + g[2];
       gss = gss + SFX(dot)(g, g) * s->w[j];
     }
@@ -437,10 +443,12 @@ static void SFX(delta_pass)(SFX(scene)* s, SFX(fluid)* F) {
       uint32_t j = F->nl[(size_t)i * 64 + k];
       const REAL* xj = s->xs + 3 * (size_t)j;
       REAL d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
-      SFX(spiky)(d, F->h, F->spiky, g);
+      REAL sp = SFX(spiky_s)(d, F->h, F->spiky);
       REAL lj = (j >= F->start && j < end) ? F->lambda[j] : R(0.0);
       REAL sc = (li + lj) * SFX(nmass)(s, F, j);
-      sum[0] = sum[0] + g[0] * sc; sum[1] = sum[1] + g[1] * sc; sum[2] = sum[2] + g[2] * sc;
+      REAL gsc = sp * sc;
+      sum[0] = sum[0] + d[0] * gsc; sum[1] = sum[1] + d[1] * gsc; sum[2] = sum[2] + d[2] * gsc;
+      g[0] = d[0] * sp; g[1] = d[1] * sp; g[2] = d[2] * sp;
       /* reaction on dynamic boundary neighbours (positionDelta.ts:54-69) */
       if (F->nb && s->bvol[j] > R(0.0) && s->w[j] > R(0.0) && li != R(0.0) && SFX(in_boundary)(F, j)) {
         REAL rs = ((-(li / F->rho0)) * (F->rho0 * s->bvol[j])) * s->w[j];
@@ -489,9 +497,10 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
       REAL V = SFX(nvol)(s, F, j);
       if (vort) {
         REAL g[3], dv[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]}, c[3];
-        SFX(spiky)(off, F->h, F->spiky, g);
+        REAL sc = SFX(spiky_s)(off, F->h, F->spiky) * V;
+        g[0] = off[0] * sc; g[1] = off[1] * sc; g[2] = off[2] * sc;
         SFX(cross)(dv, g, c);
-        curl[0] = curl[0] + c[0] * V; curl[1] = curl[1] + c[1] * V; curl[2] = curl[2] + c[2] * V;
+        curl[0] = curl[0] + c[0]; curl[1] = curl[1] + c[1]; curl[2] = curl[2] + c[2];
       }
       if (visc) {
         REAL W = SFX(poly6)(SFX(dot)(off, off), F->h2, F->poly6);
@@ -513,11 +522,11 @@ static void SFX(velocity_passes)(SFX(scene)* s, SFX(fluid)* F) {
         uint32_t j = F->nl[(size_t)i * 64 + k];
         if (s->bvol[j] > R(0.0)) continue;
         const REAL* xj = s->x + 3 * (size_t)j;
-        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]}, g[3];
-        SFX(spiky)(off, F->h, F->spiky, g);
+        REAL off[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
         REAL oj = (j >= F->start && j < end) ? F->omlen[j] : R(0.0);
         REAL sc = (oj - oi) * F->pv;
-        eta[0] = eta[0] + g[0] * sc; eta[1] = eta[1] + g[1] * sc; eta[2] = eta[2] + g[2] * sc;
+        REAL gsc = SFX(spiky_s)(off, F->h, F->spiky) * sc;
+        eta[0] = eta[0] + off[0] * gsc; eta[1] = eta[1] + off[1] * gsc; eta[2] = eta[2] + off[2] * gsc;
       }
       for (int a = 0; a < 3; ++a) F->eta[3 * (size_t)i + a] = eta[a];
     }

@@ -137,7 +137,7 @@ static __device__ __forceinline__ float4 color_normal_global(const SceneDev& S, 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-      nsum = sel3(k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
+      nsum = sel3(k0 + u < count, pf_add3(nsum, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * vol[u])), nsum);
     }
   }
   return make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-          nsum = sel3(!tail || k0 + u < count, pf_add3(nsum, pf_mul3(pf_spiky_grad(d, F.h, F.spiky), vol[u])), nsum);
+          nsum = sel3(!tail || k0 + u < count, pf_add3(nsum, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * vol[u])), nsum);
         }
       };
       const uint32_t full = count & ~3u;
@@ -1104,7 +1104,7 @@ static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, f
                                                    float& gss) {
   const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
   const float r1 = rho + mj * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
-  const float3 g = pf_mul3(pf_spiky_grad(d, F.h, F.spiky), mj);
+  const float3 g = pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * mj);
   const float3 gs1 = pf_add3(gs, g);
   const float gss1 = gss + pf_dot3(g, g) * wj;
   rho = ok ? r1 : rho;
@@ -1214,9 +1214,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
 // when ok; returns the kernel gradient for the boundary reaction.
 static __device__ __forceinline__ float3 delta_term(const FluidDev& F, bool ok, float3 xn, float li,
                                                     float4 xj, float mj, float3& sum) {
-  const float3 g = pf_spiky_grad(make_float3(xn.x - xj.x, xn.y - xj.y, xn.z - xj.z), F.h, F.spiky);
-  sum = sel3(ok, pf_add3(sum, pf_mul3(g, (li + xj.w) * mj)), sum);
-  return g;
+  const float3 d = make_float3(xn.x - xj.x, xn.y - xj.y, xn.z - xj.z);
+  const float sp = pf_spiky_scale(d, F.h, F.spiky);
+  sum = sel3(ok, pf_add3(sum, pf_mul3(d, sp * ((li + xj.w) * mj))), sum);
+  return pf_mul3(d, sp);
 }
 
 // The matching push on a dynamic boundary neighbour j of this fluid
@@ -1487,7 +1488,8 @@ static __device__ __forceinline__ void velocity_term(const FluidDev& F, bool ok,
                                                      float vol, float3& curl, float3& sm) {
   const float3 off = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
   if (VORT && vort)
-    curl = sel3(ok, pf_add3(curl, pf_mul3(pf_cross3(pf_sub3(vi, vj), pf_spiky_grad(off, F.h, F.spiky)), vol)), curl);
+    curl = sel3(ok, pf_add3(curl, pf_cross3(pf_sub3(vi, vj), pf_mul3(off, pf_spiky_scale(off, F.h, F.spiky) * vol))),
+                curl);
   if (VISC && visc)
     sm = sel3(ok, pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol)), sm);
 }
@@ -1595,8 +1597,8 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
 // One neighbour's term of the vorticity gradient (velocity.ts:80-88), added when ok.
 static __device__ __forceinline__ void eta_term(const FluidDev& F, bool ok, float3 xi, float oi,
                                                 float4 xj, float oj, float3& eta) {
-  const float3 g = pf_spiky_grad(make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z), F.h, F.spiky);
-  eta = sel3(ok, pf_add3(eta, pf_mul3(g, (oj - oi) * F.pv)), eta);
+  const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+  eta = sel3(ok, pf_add3(eta, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * ((oj - oi) * F.pv))), eta);
 }
 
 template <bool VORT, bool VISC, bool FRICTION>

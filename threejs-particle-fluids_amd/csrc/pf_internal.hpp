@@ -325,16 +325,21 @@ __device__ __forceinline__ float pf_poly6(float r2, float h2, float poly6) {
   return ((d * d) * d) * poly6;
 }
 
-// grad W = (r_vec / max(|r|, 1e-20)) * (-(t*t*spikyCoef)), t = max(h - |r|, 0);
-// the direction takes one rounded reciprocal and three products (within the
-// reference's 2.5-ulp WGSL division).
-__device__ __forceinline__ float3 pf_spiky_grad(float3 r, float h, float spiky) {
+// grad W = r_vec * sp, sp = -(t*t*spikyCoef) / max(|r|, 1e-20), t = max(h - |r|, 0)
+// (one rounded reciprocal: the reference's WGSL division is within 2.5 ulp).
+// The passes fold their per-neighbour factor into sp before it meets r_vec,
+// one product per component, as the oracle does.
+__device__ __forceinline__ float pf_spiky_scale(float3 r, float h, float spiky) {
   const float rsq = (r.x * r.x + r.y * r.y) + r.z * r.z;
   const float rl = pf_sqrt_rn(rsq);
   const float t = fmaxf(h - rl, 0.0f);
   const float inv = pf_rcp_rn(fmaxf(rl, 1e-20f));
   const float mag = -((t * t) * spiky);
-  return make_float3((r.x * inv) * mag, (r.y * inv) * mag, (r.z * inv) * mag);
+  return inv * mag;
+}
+__device__ __forceinline__ float3 pf_spiky_grad(float3 r, float h, float spiky) {
+  const float sp = pf_spiky_scale(r, h, spiky);
+  return make_float3(r.x * sp, r.y * sp, r.z * sp);
 }
 
 __device__ __forceinline__ float pf_dot3(float3 a, float3 b) {
