@@ -1291,8 +1291,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
     float3 sum = make_float3(0.f, 0.f, 0.f);
     if (staged) {
       Nl16Cursor cur(F.nl16, n, s, U);
-      // Whole groups of four need no per-entry test; the last partial group does.
-      auto group = [&](uint32_t k0, bool tail) {
+      // Whole groups of four need no per-entry test; the last partial group
+      // does. The boundary reaction has a walk of its own, so the common walk
+      // carries no per-neighbour branch.
+      auto group = [&](uint32_t k0, bool tail, bool with_reaction) {
         uint32_t idx[4];
         cur.next(k0, idx);
         float4 xj[4];
@@ -1306,12 +1308,17 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
         for (int u = 0; u < 4; ++u) {
           const bool ok = !tail || k0 + u < cnt;
           const float3 g = delta_term(F, ok, xn, li, xj[u], mj[u], sum);
-          if (react && ok) reaction_add(S, F, p, uslots[idx[u]], li, g);
+          if (with_reaction && ok) reaction_add(S, F, p, uslots[idx[u]], li, g);
         }
       };
       const uint32_t full = cnt & ~3u;
-      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
-      if (full < cnt) group(full, true);
+      if (react) {
+        for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
+        if (full < cnt) group(full, true, true);
+      } else {
+        for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
+        if (full < cnt) group(full, true, false);
+      }
     } else {
       NlCursor cur(F.nl, n, s, cnt);
       for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
