@@ -51,15 +51,20 @@ KERNEL_BYTES = {
 SUBSTEP_BYTES_PER_PARTICLE = lambda iters: 336 + 48 * iters  # noqa: E731
 
 # Measured HBM bytes per particle per launch (FETCH_SIZE + WRITE_SIZE, each
-# calibrated against a copy of known size) of the 1M dam break, from the
-# rocprofv3 counter summary profiles/dam_break_1m_hbm_traffic.md.
+# calibrated against a copy of known size) of the 1M dam break over the
+# bench's window (steps 5-25), from the rocprofv3 counter summary
+# profiles/dam_break_1m_hbm_traffic.md.
 HBM_TRAFFIC_B_PER_PARTICLE = {
-    "neighbors": 628.2,
-    "impulses": 390.5,
-    "lambda": 191.5,
-    "delta_walls": 371.9,
-    "velocity": 383.2,
-    "vorticity_friction": 267.0,
+    "neighbors": 682.4,
+    "impulses": 535.5,
+    "lambda": 191.7,
+    "delta_walls": 224.2,
+    "velocity": 213.2,
+    "vorticity_friction": 264.6,
+    "rank_permute": 228.4,
+    "hash_predict": 59.6,
+    "impulse_apply": 116.1,
+    "place": 29.7,
 }
 SCAN_BYTES_PER_BUCKET = 12
 
@@ -68,8 +73,22 @@ SCAN_BYTES_PER_BUCKET = 12
 # summary profiles/dam_break_1m_sq_counters.md. The VALU peak is the MI355X's
 # issue rate with packed FP32 off: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
 # instruction per SIMD every 4 cycles.
-VALU_INSTS_PER_PARTICLE = {}
+VALU_INSTS_PER_PARTICLE = {
+    "neighbors": 244.3,
+    "impulses": 125.4,
+    "lambda": 48.7,
+    "delta_walls": 55.2,
+    "velocity": 63.9,
+    "vorticity_friction": 42.6,
+    "rank_permute": 3.2,
+    "hash_predict": 2.8,
+}
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+# The whole substep's VALU instructions per particle, from the same summary:
+# the kernels once per substep (neighbour build 244.3, impulses 125.4 and
+# their apply 0.5, velocity 63.9, vorticity 42.6, the sort 6.9) and the
+# density and correction passes once per iteration (48.7 + 55.2).
+VALU_SUBSTEP_INSTS_PER_PARTICLE = lambda iters: 483.6 + 103.9 * iters  # noqa: E731
 
 # The isolated sort: 4,194,304 positions, cell 0.01, 2^22 buckets. Algorithmic
 # bytes per sort 32 N + 12 H: per point read x (16), write and read its key
@@ -557,8 +576,8 @@ def run_served(args, ctx, tile_union=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--budget", type=int, default=1 << 20)
     ap.add_argument("--substeps", type=int, default=2)
     ap.add_argument("--iterations", type=int, default=4)
@@ -681,6 +700,13 @@ def main():
     roofline_substep = {"bound": "hbm", "achieved": round(sub_bytes / sub_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(sub_bytes / sub_s / 1e9 / HBM_PEAK_GBS, 5),
                         "bytes_per_substep": sub_bytes}
+    roofline_valu_substep = None
+    if args.budget == 1 << 20:
+        insts = VALU_SUBSTEP_INSTS_PER_PARTICLE(args.iterations) * n
+        roofline_valu_substep = {"bound": "valu", "achieved": round(insts / sub_s / 1e9, 2),
+                                 "peak": round(VALU_PEAK_WAVE_INSTS / 1e9, 1), "unit": "G wave-instructions/s",
+                                 "frac": round(insts / sub_s / VALU_PEAK_WAVE_INSTS, 4),
+                                 "instructions_per_substep": round(insts)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -740,6 +766,7 @@ def main():
                    "iterations": args.iterations, "graph": not args.no_graph, "autotune": tuned},
         "roofline": roofline,
         "roofline_valu": roofline_valu,
+        "roofline_valu_substep": roofline_valu_substep,
         "roofline_substep": roofline_substep,
         "overflow": {"timed_window": dict(ov._asdict()), "profiled_twin": dict(twin_ov._asdict())},
         "cpu_baseline": cpu,
