@@ -530,9 +530,14 @@ def run_served(args, ctx, tile_union=None):
         ste = srv.stats()
         print(f"[bench] served, evicting: budget {budget / 2**30:.1f} GiB, {ste['evictions']} evictions at "
               f"admission, {ste['host_bytes'] / 2**30:.1f} GiB in host memory", file=sys.stderr, flush=True)
-        poisson_e = run_stream(args.serve_requests, args.serve_rate, args.serve_seed + 5)
+        # capacity first (every request queued at once), then a Poisson stream
+        # offered at half of it, so the latencies are not a queue's
+        saturated_e = run_stream(args.serve_evict_requests, 0, args.serve_seed + 5)
+        print(f"[bench] served, evicting: saturated {saturated_e}", file=sys.stderr, flush=True)
+        rate_e = max(1.0, round(0.5 * saturated_e["scene_steps_per_s"], 1))
+        poisson_e = run_stream(args.serve_evict_requests, rate_e, args.serve_seed + 6)
         st1 = srv.stats()
-        evicting = {"hbm_budget_gib": round(budget / 2**30, 2), "poisson": poisson_e,
+        evicting = {"hbm_budget_gib": round(budget / 2**30, 2), "saturated": saturated_e, "poisson": poisson_e,
                     "evictions": st1["evictions"], "restores": st1["restores"],
                     "evict_failures": st1["evict_failures"], "host_gib": round(st1["host_bytes"] / 2**30, 2),
                     "peak_resident_gib": round(st1["peak_resident_bytes"] / 2**30, 2)}
@@ -600,6 +605,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-seed", type=int, default=0x5E7E)
+    ap.add_argument("--serve-evict-requests", type=int, default=400,
+                    help="requests per stream of the evicting phase (saturated, then Poisson at half its rate)")
     ap.add_argument("--serve-evict-frac", type=float, default=0.6,
                     help="then serve the same scenes under this fraction of their resident bytes (0: skip)")
     ap.add_argument("--field-updates", type=int, default=20, help="surface-field updates timed (0: skip)")

@@ -1,10 +1,15 @@
 // Context, ParticleSystem and HashGrid behind the C ABI, plus the small
 // gather/scatter kernels that move particle data between original order
 // (the ABI) and cell-sorted order (the device).
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <cstdlib>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <unordered_set>
 
 #include "runtime.hpp"
@@ -103,14 +108,101 @@ Arena::~Arena() {
 // Evicted chunks live in pageable host memory; they move through a fixed
 // ring of two pinned staging buffers per device (allocated once), so the
 // pinned memory an eviction needs does not grow with the scenes evicted.
-// One piece's DMA overlaps the host copy of the piece before it.
+// One piece's DMA overlaps the host copy of the piece before it, and each
+// host copy is split over a few worker threads (a single thread's memcpy,
+// page faults included, runs well below the DMA's rate).
 namespace {
+
+// Host memory for an evicted segment: 2 MiB-aligned and marked for huge
+// pages where the kernel allows it, so writing it takes few page faults.
+char* host_alloc(size_t n) {
+  constexpr size_t kHuge = 2u << 20;
+  const size_t m = std::max<size_t>((n + kHuge - 1) / kHuge * kHuge, kHuge);
+  void* p = std::aligned_alloc(kHuge, m);
+  if (p) (void)madvise(p, m, MADV_HUGEPAGE);
+  return static_cast<char*>(p);
+}
+
+class CopyPool {
+ public:
+  static constexpr int kWorkers = 4;
+  CopyPool() {
+    for (int i = 0; i < kWorkers; ++i) threads_.emplace_back([this, i] { run(i); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  // memcpy(dst, src, n), the calling thread taking one share.
+  void copy(char* dst, const char* src, size_t n) {
+    if (n < (4u << 20)) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    const size_t share = (n / (kWorkers + 1) + 4095) / 4096 * 4096;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      dst_ = dst;
+      src_ = src;
+      n_ = n;
+      share_ = share;
+      pending_ = kWorkers;
+      ++gen_;
+    }
+    cv_.notify_all();
+    const size_t off = share * kWorkers;
+    if (off < n) std::memcpy(dst + off, src + off, n - off);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+  }
+
+ private:
+  void run(int i) {
+    uint64_t seen = 0;
+    for (;;) {
+      char* dst;
+      const char* src;
+      size_t n, share;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        dst = dst_;
+        src = src_;
+        n = n_;
+        share = share_;
+      }
+      const size_t off = share * (size_t)i;
+      if (off < n) std::memcpy(dst + off, src + off, std::min(share, n - off));
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (--pending_ == 0) done_.notify_one();
+      }
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> threads_;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t n_ = 0, share_ = 0;
+  int pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
 struct StagingRing {
   static constexpr size_t kBytes = 32u << 20;
   std::mutex mu;
   char* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipStream_t stream = nullptr;
+  std::unique_ptr<CopyPool> pool;
 
   void init() {
     if (stream) return;
@@ -119,6 +211,7 @@ struct StagingRing {
       PF_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
     }
     PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pool.reset(new CopyPool());
   }
   // Device -> pageable host.
   void download(char* dst, const char* src, size_t n) {
@@ -135,7 +228,7 @@ struct StagingRing {
       }
       if (pend >= 0) {
         PF_HIP(hipEventSynchronize(ev[pend]));
-        std::memcpy(dst + pend_off, buf[pend], pend_n);
+        pool->copy(dst + pend_off, buf[pend], pend_n);
       }
       pend = issued;
       pend_off = off;
@@ -150,7 +243,7 @@ struct StagingRing {
     for (size_t off = 0; off < n; off += kBytes) {
       const size_t m = std::min(kBytes, n - off);
       PF_HIP(hipEventSynchronize(ev[slot]));  // the copy that last read this buffer is done
-      std::memcpy(buf[slot], src + off, m);
+      pool->copy(buf[slot], src + off, m);
       PF_HIP(hipMemcpyAsync(dst + off, buf[slot], m, hipMemcpyHostToDevice, stream));
       PF_HIP(hipEventRecord(ev[slot], stream));
       slot ^= 1;
@@ -224,7 +317,7 @@ void Arena::evict() {
   std::vector<char*> copies(segs.size(), nullptr);
   try {
     for (size_t k = 0; k < segs.size(); ++k) {
-      copies[k] = static_cast<char*>(std::malloc(std::max<size_t>(segs[k].used, 1)));
+      copies[k] = host_alloc(segs[k].used);
       if (!copies[k]) fail(PF_ENOMEM, "Server: no host memory for an evicted scene's %zu bytes", segs[k].used);
     }
     StagingRing& R = ring_for(device);
