@@ -164,15 +164,18 @@ int pf_grid_sort_noalloc(pf_context* ctx, const float* positions4, uint32_t n, d
 
 /* ---- Device self-check (no counterpart in the reference) -------------------
  * The passes take reciprocals and square roots through short sequences that
- * start from the hardware approximations and round them exactly. This call
- * compares one of them bit for bit with the IEEE operation (1.0f / x, sqrtf)
- * for every float whose bit pattern lies in [first_bits, end_bits), on the
+ * start from the hardware approximations and round them exactly, and convert
+ * fixed-point ticks with the hardware's saturating conversion. This call
+ * compares one of them bit for bit with its reference form (1.0f / x, sqrtf,
+ * the branchy saturating conversion) for every float whose bit pattern lies
+ * in [first_bits, end_bits), on the
  * context's GPU, and returns the number of mismatches and the first one
  * (0xffffffff if none). It is for tests and for users who move the library
  * to another GPU. Blocks until done. */
 #define PF_EXACT_RCP 0        /* 1/x, exact for 2^-126 <= |x| < 2^126 */
 #define PF_EXACT_SQRT 1       /* sqrt(q), exact for every q >= 0 and -0 */
 #define PF_EXACT_SQRT_FAST 2  /* its branch-free part, exact for q = 0 and q >= 2^-96 */
+#define PF_EXACT_F2I 3        /* saturating f32 -> i32 (NaN to 0), one instruction, for every x */
 int pf_check_exact_math(pf_context* ctx, int function, uint32_t first_bits, uint32_t end_bits,
                         uint64_t* mismatches, uint32_t* first_mismatch);
 

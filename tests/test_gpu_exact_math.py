@@ -1,4 +1,5 @@
-"""The passes' correctly rounded reciprocal and square root, exhaustively.
+"""The passes' correctly rounded reciprocal and square root, and their
+saturating float-to-int conversion, exhaustively.
 
 pf_rcp_rn and pf_sqrt_rn start from the hardware approximations (v_rcp_f32,
 v_sqrt_f32, within 1 ulp) and round them exactly with fused multiply-adds.
@@ -16,7 +17,7 @@ from particle_fluids import _lib
 
 pytestmark = pytest.mark.gpu
 
-RCP, SQRT, SQRT_FAST = 0, 1, 2
+RCP, SQRT, SQRT_FAST, F2I = 0, 1, 2, 3
 
 
 def mismatches(function, lo, hi):
@@ -47,3 +48,9 @@ def test_branch_free_sqrt_range():
     # below 2^-96 it is not exact, which is why pf_sqrt_rn branches there
     n, first = mismatches(SQRT_FAST, 0x00000001, 0x0F800000)
     assert n > 0 and first != 0xFFFFFFFF
+
+
+def test_saturating_conversion_equals_the_branchy_form_for_every_float():
+    # v_cvt_i32_f32 against the explicit clamp and NaN test, every bit pattern
+    # but the last (a NaN; the range end is exclusive)
+    assert mismatches(F2I, 0x00000000, 0xFFFFFFFF) == (0, 0xFFFFFFFF)

@@ -385,8 +385,18 @@ __device__ __forceinline__ void pf_atomic_add_g(PF_G int* a, int v) {
 
 // Fixed-point accumulator add (accumulator.ts:62-68): i32(round(v * scale)),
 // round-half-to-even, saturating conversion. Sums wrap like WGSL atomicAdd.
+// WGSL's saturating f32 -> i32 conversion (NaN to 0) in one instruction:
+// v_cvt_i32_f32 clamps out-of-range inputs and maps NaN to 0, the same
+// function as pf_f2i_sat, which compiles to three branches;
+// pf_check_exact_math compares the two over every float.
+__device__ __forceinline__ int pf_f2i_sat_hw(float f) {
+  int r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
+}
+
 __device__ __forceinline__ int pf_ticks(float v, float scale) {
-  return pf_f2i_sat(rintf(v * scale));
+  return pf_f2i_sat_hw(rintf(v * scale));
 }
 __device__ __forceinline__ int pf_wrap_add(int a, int b) {
   return (int)((uint32_t)a + (uint32_t)b);

@@ -1,8 +1,9 @@
 // Exhaustive device check of the correctly rounded reciprocal and square root
-// the passes use (pf_rcp_rn, pf_sqrt_rn in pf_internal.hpp): each is compared
-// bit for bit with the compiler's IEEE sequence (1.0f / x, sqrtf) over a range
-// of float bit patterns, on the GPU that runs the passes. The sequences start
-// from the hardware approximations, so this is where their exactness is shown.
+// the passes use (pf_rcp_rn, pf_sqrt_rn in pf_internal.hpp), and of the
+// one-instruction saturating conversion (pf_f2i_sat_hw): each is compared bit
+// for bit with its reference form (1.0f / x, sqrtf, pf_f2i_sat) over a range
+// of float bit patterns, on the GPU that runs the passes. They rest on the
+// hardware's own instructions, so this is where their exactness is shown.
 #include <hip/hip_runtime.h>
 
 #include "runtime.hpp"
@@ -19,7 +20,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_exact_math(int function, uint32_t 
   for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * PF_BLOCK + threadIdx.x; b < hi; b += stride) {
     const float x = __uint_as_float((uint32_t)b);
     float ref, got;
-    if (function == PF_EXACT_RCP) {
+    if (function == PF_EXACT_F2I) {
+      ref = __int_as_float(pf_f2i_sat(x));
+      got = __int_as_float(pf_f2i_sat_hw(x));
+    } else if (function == PF_EXACT_RCP) {
       ref = 1.0f / x;
       got = pf_rcp_rn(x);
     } else if (function == PF_EXACT_SQRT) {
@@ -59,7 +63,8 @@ int pf_check_exact_math(pf_context* ctx, int function, uint32_t first_bits, uint
   using namespace pf;
   PF_API_BEGIN
   if (!ctx) fail(PF_EINVAL, "pf_check_exact_math: null handle");
-  if (function != PF_EXACT_RCP && function != PF_EXACT_SQRT && function != PF_EXACT_SQRT_FAST)
+  if (function != PF_EXACT_RCP && function != PF_EXACT_SQRT && function != PF_EXACT_SQRT_FAST &&
+      function != PF_EXACT_F2I)
     fail(PF_EINVAL, "pf_check_exact_math: unknown function %d", function);
   if (!mismatches) fail(PF_EINVAL, "pf_check_exact_math: mismatches is required");
   if (end_bits < first_bits) fail(PF_EINVAL, "pf_check_exact_math: empty range");
