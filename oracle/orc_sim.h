@@ -417,7 +417,8 @@ static void SFX(lambda_pass)(SFX(scene)* s, SFX(fluid)* F) {
       REAL sc = SFX(spiky_s)(d, F->h, F->spiky) * mj;
       g[0] = d[0] * sc; g[1] = d[1] * sc; g[2] = d[2] * sc;
       gs[0] = gs[0] + g[0]; gs[1] = gs[1] + g[1]; gs[2] = gs[2] + g[2];
-      gss = gss + SFX(dot)(g, g) * s->w[j];
+      /* |g|^2 = |d|^2 sc^2 */
+      gss = gss + ((SFX(dot)(d, d) * sc) * sc) * s->w[j];
     }
     s->rho[i] = rho;
     REAL inv = R(1.0) / F->rho0;

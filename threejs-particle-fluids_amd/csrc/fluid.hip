@@ -1103,10 +1103,12 @@ static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, f
                                                    float mj, float wj, float& rho, float3& gs,
                                                    float& gss) {
   const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-  const float r1 = rho + mj * pf_poly6(pf_dot3(d, d), F.h2, F.poly6);
-  const float3 g = pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * mj);
+  const float q = pf_dot3(d, d);
+  const float r1 = rho + mj * pf_poly6(q, F.h2, F.poly6);
+  const float sc = pf_spiky_scale(d, F.h, F.spiky) * mj;
+  const float3 g = pf_mul3(d, sc);
   const float3 gs1 = pf_add3(gs, g);
-  const float gss1 = gss + pf_dot3(g, g) * wj;
+  const float gss1 = gss + ((q * sc) * sc) * wj;  // |g|^2 = |d|^2 sc^2
   rho = ok ? r1 : rho;
   gs = sel3(ok, gs1, gs);
   gss = ok ? gss1 : gss;
@@ -1501,13 +1503,18 @@ static __device__ __forceinline__ void velocity_term(const FluidDev& F, bool ok,
     sm = sel3(ok, pf_add3(sm, pf_mul3(pf_mul3(pf_sub3(vj, vi), pf_poly6(pf_dot3(off, off), F.h2, F.poly6)), vol)), sm);
 }
 
-template <bool VORT, bool VISC>
+// BND: some scene of the launch has boundary particles, whose volume differs
+// from the fluid's; the union then holds each entry's volume (32 bytes an
+// entry), else the volume is the fluid's own and an entry takes 24 bytes,
+// which fits three workgroups per CU instead of two.
+template <bool VORT, bool VISC, bool BND>
 __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, uint32_t cap) {
   extern __shared__ float4 pf_lds[];
-  float4* u_x = pf_lds;        // position, neighbour volume
-  float4* u_v = pf_lds + cap;  // velocity
+  float4* u_x = pf_lds;        // position; the neighbour volume (BND) or velocity x
+  float4* u_v = pf_lds + cap;  // velocity (BND)
+  float2* u_yz = reinterpret_cast<float2*>(pf_lds + cap);  // velocity y, z (not BND)
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -1531,7 +1538,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
       vv[i] = V[js[i]];
       ov[i] = F.pv;
     }
-    if (S.has_boundary) {
+    if (BND && S.has_boundary) {
 #pragma unroll
       for (int i = 0; i < PF_TU_PER; ++i) ov[i] = neighbor_volume(S, F, p, js[i]);
     }
@@ -1539,8 +1546,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
     for (int i = 0; i < PF_TU_PER; ++i) {
       const uint32_t u = threadIdx.x + i * PF_BLOCK;
       if (u < U) {
-        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
-        u_v[u] = pf_f4(vv[i]);
+        if (BND) {
+          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
+          u_v[u] = pf_f4(vv[i]);
+        } else {
+          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, vv[i].x);
+          u_yz[u] = make_float2(vv[i].y, vv[i].z);
+        }
       }
     }
     __syncthreads();
@@ -1565,16 +1577,25 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
     auto group = [&](uint32_t k0, bool tail) {
       uint32_t idx[4];
       cur.next(k0, idx);
-      float4 xj[4], vj4[4];
+      float4 xj[4];
+      float3 vj[4];
+      float vol[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         xj[u] = u_x[idx[u]];
-        vj4[u] = u_v[idx[u]];
+        if (BND) {
+          const float4 v4 = u_v[idx[u]];
+          vj[u] = pf_xyz(v4);
+          vol[u] = xj[u].w;
+        } else {
+          const float2 yz = u_yz[idx[u]];
+          vj[u] = make_float3(xj[u].w, yz.x, yz.y);
+          vol[u] = F.pv;
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        velocity_term<VORT, VISC>(F, !tail || k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), xj[u].w,
-                                  curl, sm);
+        velocity_term<VORT, VISC>(F, !tail || k0 + u < cnt, vort, visc, xi, vi, xj[u], vj[u], vol[u], curl, sm);
     };
     const uint32_t full = cnt & ~3u;
     for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
@@ -1835,12 +1856,20 @@ void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t 
 }
 
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, bool vort, bool visc, uint32_t cap) {
+                     int p, bool vort, bool visc, bool boundary, uint32_t cap) {
   if (!ntiles || !(vort || visc)) return;
-  const size_t lds = (size_t)cap * 2 * sizeof(float4);
-  if (vort && visc) hipLaunchKernelGGL((k_velocity<true, true>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
-  else if (vort) hipLaunchKernelGGL((k_velocity<true, false>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
-  else hipLaunchKernelGGL((k_velocity<false, true>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap);
+  const size_t lds = (size_t)cap * (boundary ? 2 * sizeof(float4) : sizeof(float4) + sizeof(float2));
+#define PF_VK(a, b, c) hipLaunchKernelGGL((k_velocity<a, b, c>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap)
+  if (boundary) {
+    if (vort && visc) PF_VK(true, true, true);
+    else if (vort) PF_VK(true, false, true);
+    else PF_VK(false, true, true);
+  } else {
+    if (vort && visc) PF_VK(true, true, false);
+    else if (vort) PF_VK(true, false, false);
+    else PF_VK(false, true, false);
+  }
+#undef PF_VK
 }
 
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,

@@ -36,7 +36,7 @@ void launch_reaction_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, u
 void launch_boundary_volume(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
                             int c, int dynamic);
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
-                     int p, bool vort, bool visc, uint32_t cap);
+                     int p, bool vort, bool visc, bool boundary, uint32_t cap);
 void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool vort, bool visc, bool friction, int substep,
                       int touch_valid, uint32_t cap);

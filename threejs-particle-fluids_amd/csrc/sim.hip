@@ -216,7 +216,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       const bool lastf = f + 1 == sh.nfluids;
       if (vort || visc) {
         PF_LAUNCH(ctx, "velocity");
-        launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc, sh.tu_cap[f]);
+        launch_velocity(st, scenes, tiles, ntiles, (int)f, p, vort, visc, sh.any_boundary, sh.tu_cap[f]);
         ++nk;
       }
       // Friction follows every material's post-solve (loop.ts:241): it fuses
