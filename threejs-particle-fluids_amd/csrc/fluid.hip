@@ -874,13 +874,103 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
   }
 }
 
+// The walk of a staged tile, eight list entries per group: their ids decide
+// which entries are pairs this particle evaluates (a higher fluid id), and
+// only those are evaluated, one per iteration, so a lane spends its time on
+// the pairs it has instead of idling through the entries where it is the
+// upper end. Same sums as impulse_walk. s_slot is the tile union's sorted
+// slots, staged in LDS.
+static __device__ __forceinline__ void impulse_walk_staged(const SceneDev& S, const FluidDev& F, int p,
+                                                           const float4* __restrict__ xs, uint32_t s, uint32_t id,
+                                                           float3 xi, float3 ni, float rhoi, uint32_t cnt,
+                                                           bool st_pairs, bool adh_on, uint32_t U,
+                                                           const uint32_t* s_slot, int* acc_l, int tk[3]) {
+  const uint32_t n = S.n;
+  const uint32_t fend = F.start + F.count;
+  const uint32_t lim = U ? U - 1u : 0u;
+  const PF_G uint32_t* __restrict__ col = pf_g(F.nl16) + s;
+  const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
+  const PF_G float4* __restrict__ g_x = pf_g(xs);
+  const PF_G float4* __restrict__ g_nrm = pf_g(F.normal);
+  const PF_G float* __restrict__ g_rho = pf_g(S.rho[p]);
+  const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
+    const uint32_t r0 = k0 >> 1, rmax = PF_MAX_NEIGHBORS / 2 - 1;
+    const uint32_t w0 = col[(size_t)min(r0, rmax) * n], w1 = col[(size_t)min(r0 + 1u, rmax) * n];
+    const uint32_t w2 = col[(size_t)min(r0 + 2u, rmax) * n], w3 = col[(size_t)min(r0 + 3u, rmax) * n];
+    auto entry = [=](uint32_t u) {  // union index of entry k0 + u
+      const bool b1 = (u & 2u) != 0u, b2 = (u & 4u) != 0u;
+      const uint32_t lo = b1 ? w1 : w0, hi = b1 ? w3 : w2;
+      const uint32_t a = b2 ? hi : lo;
+      return min((a >> ((u & 1u) * 16u)) & (PF_TU_MAX - 1u), lim);
+    };
+    uint32_t m = 0, ma = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t j = s_slot[entry((uint32_t)u)];
+      const bool in = k0 + (uint32_t)u < cnt;
+      const uint32_t idj = g_id[j];
+      m |= (st_pairs && in && idj > id && idj < fend) ? (1u << u) : 0u;
+      if (adh_on) ma |= (in && g_vol[j] > 0.0f) ? (1u << u) : 0u;
+    }
+    // The next pair's loads are issued before this pair's arithmetic.
+    uint32_t ix = 0u;
+    float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), na = xa;
+    float ra = 0.0f;
+    bool have = m != 0u;
+    if (have) {
+      ix = entry((uint32_t)__builtin_ctz(m));
+      m &= m - 1u;
+      const uint32_t j = s_slot[ix];
+      xa = pf_f4(g_x[j]);
+      na = pf_f4(g_nrm[j]);
+      ra = g_rho[j];
+    }
+    while (have) {
+      const bool more = m != 0u;
+      uint32_t ixb = 0u;
+      float4 xb = xa, nb = na;
+      float rb = ra;
+      if (more) {
+        ixb = entry((uint32_t)__builtin_ctz(m));
+        m &= m - 1u;
+        const uint32_t j = s_slot[ixb];
+        xb = pf_f4(g_x[j]);
+        nb = pf_f4(g_nrm[j]);
+        rb = g_rho[j];
+      }
+      float3 dv;
+      if (pair_dv(F, S.dt, xi, pf_xyz(xa), ni, pf_xyz(na), rhoi, ra, &dv)) {
+        tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
+        tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
+        tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
+        // the partner is union entry ix: summed in LDS, flushed once per tile
+        atomicAdd(acc_l + 3 * ix + 0, pf_ticks(-dv.x, F.imp_scale));
+        atomicAdd(acc_l + 3 * ix + 1, pf_ticks(-dv.y, F.imp_scale));
+        atomicAdd(acc_l + 3 * ix + 2, pf_ticks(-dv.z, F.imp_scale));
+      }
+      have = more;
+      ix = ixb;
+      xa = xb;
+      na = nb;
+      ra = rb;
+    }
+    while (ma != 0u) {
+      const uint32_t j = s_slot[entry((uint32_t)__builtin_ctz(ma))];
+      ma &= ma - 1u;
+      adhesion_add(S, F, xi, pf_xyz(pf_f4(g_x[j])), g_vol[j], tk);
+    }
+  }
+}
+
 // One particle's surface tension and adhesion. With surface tension its own
 // sum is parked, as bits, in its slot of the output prediction buffer for
 // k_impulse_apply, which adds the halves its lower partners scattered; a
 // fluid with adhesion alone applies here.
 template <bool ST, bool ADH>
 static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const FluidDev& F, int p, int c,
-                                                        uint32_t s, uint32_t tile, int* acc_l, bool lds_acc) {
+                                                        uint32_t s, uint32_t tile, int* acc_l, bool lds_acc,
+                                                        const uint32_t* s_slot) {
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const uint32_t id = S.id[p][s];
@@ -909,6 +999,8 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
   if (S.H < 64u) {
     impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, tk);
+  } else if (lds_acc) {
+    impulse_walk_staged(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, s_slot, acc_l, tk);
   } else {
     impulse_walk(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, uslots, acc_l, lds_acc, tk);
   }
@@ -935,6 +1027,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, int c) {
   __shared__ int acc_l[ST ? 3 * PF_TU_MAX : 1];
+  __shared__ uint32_t s_slot[ST ? PF_TU_MAX : 1];
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -944,11 +1037,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
   const bool lds = ST && st_on && S.H >= 64u && U != PF_TU_GLOBAL;
   if (lds) {
+    const PF_G uint32_t* __restrict__ g_slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
     for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
+    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) s_slot[u] = g_slots[u];
     __syncthreads();
   }
   const uint32_t s = t.first + threadIdx.x;
-  if (s < S.n) impulse_particle<ST, ADH>(S, F, p, c, s, tile, acc_l, lds);
+  if (s < S.n) impulse_particle<ST, ADH>(S, F, p, c, s, tile, acc_l, lds, s_slot);
   if (lds) {
     __syncthreads();
     const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
