@@ -248,19 +248,206 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
 }
 
 // ---------------------------------------------------------------------------
+// Run walk: the neighbour lists of a wave's particles, one cell run at a time.
+//
+// The wave's active slots split into runs of equal query cell, cut at the
+// wave's edges (so a run holds at most 64 particles). For a run the wave
+// enumerates the candidates of the cell's 27 stencil buckets in walk order
+// (dz, dy, dx, then slot order inside a bucket), 64 per chunk, one candidate
+// per lane, and tests each chunk against the run's particles in turn: for one
+// particle the passing lanes are its next list entries in lane order, placed
+// by a ballot and a prefix count. Every lane does useful work whatever the
+// bucket sizes, and no lane loops over its own appends.
+//
+// A chunk's lanes find their bucket through marks: each non-empty bucket whose
+// first candidate falls in the chunk writes its key (stencil index + 1 in the
+// top bits, its entry offset below) at that lane in LDS, and an inclusive max
+// over the lanes, carried from chunk to chunk, gives each lane the last bucket
+// that opened at or before it.
+// ---------------------------------------------------------------------------
+
+// Inclusive max / sum over the wave's 64 lanes: row shifts inside each row of
+// 16 lanes, then the row broadcasts of lanes 15 and 31. Lanes a step does not
+// reach read 0, the identity of both.
+template <int CTRL, int ROWS>
+static __device__ __forceinline__ uint32_t dpp_or_zero(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+static __device__ __forceinline__ uint32_t wave_inclusive_max64(uint32_t v) {
+  v = max(v, dpp_or_zero<0x111, 0xf>(v));  // row_shr:1
+  v = max(v, dpp_or_zero<0x112, 0xf>(v));  // row_shr:2
+  v = max(v, dpp_or_zero<0x114, 0xf>(v));  // row_shr:4
+  v = max(v, dpp_or_zero<0x118, 0xf>(v));  // row_shr:8
+  v = max(v, dpp_or_zero<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+  v = max(v, dpp_or_zero<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+static __device__ __forceinline__ uint32_t wave_inclusive_sum64(uint32_t v) {
+  v += dpp_or_zero<0x111, 0xf>(v);
+  v += dpp_or_zero<0x112, 0xf>(v);
+  v += dpp_or_zero<0x114, 0xf>(v);
+  v += dpp_or_zero<0x118, 0xf>(v);
+  v += dpp_or_zero<0x142, 0xa>(v);
+  v += dpp_or_zero<0x143, 0xc>(v);
+  return v;
+}
+
+#define PF_RUN_CHUNKS 4  // chunks of 64 candidates held in registers at once
+
+// Builds the lists of the wave's active particles. Every lane of the wave
+// calls it. STAGED: candidates are the tile union's staged positions and the
+// entries union indices, written as 16-bit halves of F.nl16's words; else
+// candidates are read from xw and the entries are sorted slots in F.nl.
+// WIDE: row offsets need full 32-bit products (n >= 2^22; n <= 2^24 keeps
+// every offset within 32 bits). marks: 64 * PF_RUN_CHUNKS words of LDS owned
+// by the wave. Returns the lane's list length; *over is set where a particle
+// has more than PF_MAX_NEIGHBORS neighbours (its list keeps the first ones).
+template <bool STAGED, bool WIDE>
+static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const FluidDev& F, const float4* __restrict__ xw,
+                                     const float* u_px, const float* u_py, const float* u_pz,
+                                     const uint32_t* h_key, const uint32_t* h_off, uint32_t* marks,
+                                     bool active, bool leader, const int q[3], float3 xi, uint32_t s,
+                                     bool* over) {
+  constexpr uint32_t K = PF_RUN_CHUNKS;
+  const uint32_t lane = __lane_id();
+  const uint32_t n = S.n;
+  const float h2 = F.h2;
+  const uint32_t s0 = __builtin_amdgcn_readfirstlane(s - lane);  // slot of lane 0
+  const uint64_t actives = __builtin_amdgcn_ballot_w64(active);
+  const uint64_t leaders = __builtin_amdgcn_ballot_w64(active && (leader || lane == 0u));
+  // This lane's stencil offset (lanes 0..26, dz slowest, dx fastest).
+  const uint32_t sdx = lane % 3u - 1u, sdy = (lane / 3u) % 3u - 1u, sdz = lane / 9u - 1u;
+  PF_G uint32_t* __restrict__ nl = pf_g(F.nl);
+  PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
+  const PF_G float4* __restrict__ gxw = pf_g(xw);
+  uint32_t count = 0;
+  uint64_t overm = 0;
+  uint64_t todo = leaders;
+  while (todo != 0ull) {
+    const uint32_t a = (uint32_t)__builtin_ctzll(todo);
+    todo &= todo - 1ull;
+    const uint64_t above = a == 63u ? 0ull : ~0ull << (a + 1u);
+    const uint64_t stop = (leaders | ~actives) & above;
+    const uint32_t end = stop ? (uint32_t)__builtin_ctzll(stop) : 64u;
+    const uint64_t run = (end == 64u ? ~0ull : (1ull << end) - 1ull) & (~0ull << a);
+    const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane(q[0], a);
+    const uint32_t q1 = (uint32_t)__builtin_amdgcn_readlane(q[1], a);
+    const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane(q[2], a);
+    // The stencil buckets' entry ranges, their prefix over the walk order and
+    // the marks' keys.
+    uint32_t cn = 0, base = 0;
+    if (lane < 27u) {
+      const uint32_t b = pf_morton((int)(q0 + sdx), (int)(q1 + sdy), (int)(q2 + sdz)) & S.hash_mask;
+      if (STAGED) {
+        uint32_t h = tu_hash(b);
+        for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
+        if (h_key[h] == b + 1u) {  // always: the cell's leader inserted b
+          const uint32_t ho = h_off[h];
+          base = ho >> 16;
+          cn = ho & 0xffffu;
+        }
+      } else {
+        base = S.start[b];
+        cn = S.start[b + 1] - base;
+      }
+    }
+    const uint32_t inc = wave_inclusive_sum64(cn);
+    const uint32_t pre = inc - cn;
+    const uint32_t C = (uint32_t)__builtin_amdgcn_readlane(inc, 63);
+    const uint32_t key = cn ? ((lane + 1u) << 27) | (base - pre + (1u << 26)) : 0u;
+    uint64_t live = run;  // particles of the run whose list is still open
+    uint32_t carry = 0;
+    for (uint32_t k0 = 0; k0 < C && live != 0ull; k0 += 64u * K) {
+      float3 xj[K];
+      uint32_t ej[K];
+#pragma unroll
+      for (uint32_t j = 0; j < K; ++j) {
+        const uint32_t c0 = k0 + 64u * j;
+        uint32_t* mk = marks + 64u * j;
+        mk[lane] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (key != 0u && pre - c0 < 64u) mk[pre - c0] = key;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t v = max(wave_inclusive_max64(mk[lane]), carry);
+        carry = (uint32_t)__builtin_amdgcn_readlane(v, 63);
+        const uint32_t kk = c0 + lane;
+        const bool valid = kk < C;
+        const uint32_t e = valid ? kk + (v & 0x7ffffffu) - (1u << 26) : 0u;
+        ej[j] = e;
+        float3 p;
+        if (STAGED) {
+          p = make_float3(u_px[e], u_py[e], u_pz[e]);
+        } else {
+          const float4 g = pf_f4(gxw[e]);
+          p = make_float3(g.x, g.y, g.z);
+        }
+        // Lanes past the last candidate never pass the distance test.
+        xj[j] = valid ? p : make_float3(__builtin_inff(), __builtin_inff(), __builtin_inff());
+      }
+      uint64_t rem = live;
+      while (rem != 0ull) {
+        const uint32_t i = (uint32_t)__builtin_ctzll(rem);
+        rem &= rem - 1ull;
+        const float sx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xi.x), i));
+        const float sy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xi.y), i));
+        const float sz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xi.z), i));
+        uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane(count, i);
+        // The particle's list column, word rows n apart.
+        PF_G char* const col = STAGED ? (PF_G char*)(nl16 + s0 + i) : (PF_G char*)(nl + s0 + i);
+        // cnt counts every neighbour found; entries past PF_MAX_NEIGHBORS are
+        // not stored, and a list that passes it closes after this block.
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j) {
+          if (k0 + 64u * j >= C) break;
+          const float3 d = make_float3(sx - xj[j].x, sy - xj[j].y, sz - xj[j].z);
+          const bool hit = pf_dot3(d, d) < h2;
+          const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+          if (m != 0ull) {
+            // The lane's entry: the particle's cnt-th plus the hits below it.
+            const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, cnt));
+            if (hit & (pos < PF_MAX_NEIGHBORS)) {
+              if (STAGED) {  // word row pos >> 1, half pos & 1
+                const uint32_t off = (WIDE ? (pos >> 1) * (n * 4u) : __umul24(pos >> 1, n * 4u)) + ((pos & 1u) << 1);
+                *(PF_G uint16_t*)(col + off) = (uint16_t)ej[j];
+              } else {
+                const uint32_t off = WIDE ? pos * (n * 4u) : __umul24(pos, n * 4u);
+                *(PF_G uint32_t*)(col + off) = ej[j];
+              }
+            }
+          }
+          cnt += (uint32_t)__popcll(m);
+        }
+        count = lane == i ? min(cnt, (uint32_t)PF_MAX_NEIGHBORS) : count;
+        if (cnt > PF_MAX_NEIGHBORS) {
+          // The list is full and another neighbour exists: it overflows,
+          // nothing later in the walk can enter it.
+          live &= ~(1ull << i);
+          overm |= 1ull << i;
+        }
+      }
+    }
+  }
+  *over = ((overm >> lane) & 1ull) != 0ull;
+  return active ? count : 0u;
+}
+
+// ---------------------------------------------------------------------------
 // Neighbour list (+ colour-field normal when surface tension is on)
 //
 // Each 256-slot tile first gathers the set of buckets its fluid particles
 // walk (all 27 stencil buckets of each distinct query cell) in an LDS hash
 // set, stages those buckets' particles in LDS in one pass (the tile union),
-// and walks them there. The list keeps the reference's order and cap (walk
-// order dz->dy->dx, slot order inside a bucket, first 64 within h) and stores
-// 16-bit union indices, two per word. The colour-field normal is summed over
-// the finished list afterwards, in list order, so the divergent walk carries
-// only the distance test. A tile whose union does not fit (more than PF_TU_MAX
-// particles or PF_TU_HASH_LIMIT buckets), and every tile of a table smaller
-// than 64 buckets (where stencil buckets can coincide), walks global memory
-// and stores global slots instead. Both walks give the same list.
+// and walks them there, one cell run per wave at a time (walk_runs). The list
+// keeps the reference's order and cap (walk order dz->dy->dx, slot order
+// inside a bucket, first 64 within h) and stores 16-bit union indices, two per
+// word. The colour-field normal is summed over the finished list afterwards,
+// in list order. A tile whose union does not fit (more than PF_TU_MAX
+// particles or PF_TU_HASH_LIMIT buckets) walks global memory the same way and
+// stores global slots, then compacts them into a union when the listed
+// particles fit; every tile of a table smaller than 64 buckets (where stencil
+// buckets can coincide) walks global memory one particle per lane with the
+// reference's bucket dedup. All walks give the same list.
 // With `snap` set, a contact stabilisation has moved some predictions since
 // the sort: the query point is the current prediction (xs[c]) and candidates
 // are tested at the positions the grid was built from (xs[c ^ 1], the
@@ -298,23 +485,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   const float4* __restrict__ xw = snap ? S.xs[c ^ 1] : xs;  // candidate positions
   float3 xi = make_float3(0.f, 0.f, 0.f);
   int q[3] = {0, 0, 0};
-  // Distance from the particle to the faces of its own cell, to skip stencil
-  // cells that no particle within h can occupy (a wide margin covers the
-  // rounding of the cell assignment).
-  float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};
   if (active) {
     xi = pf_xyz(xs[s]);
     const float r[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float fl = floorf(r[a]);
-      q[a] = pf_f2i_sat(fl);
-      lo[a] = (r[a] - fl) * S.cell;
-      hi[a] = (1.0f - (r[a] - fl)) * S.cell;
-    }
+    for (int a = 0; a < 3; ++a) q[a] = pf_f2i_sat(floorf(r[a]));
   }
-  const float lim = F.h + 1e-4f * S.cell;
-  const float lim2 = lim * lim;
   const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
   uint32_t count = 0;
   bool over = false;
@@ -429,54 +605,22 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     __syncthreads();
   }
 
+  // h_start is free once the layout is read: each wave's run-walk marks.
+  uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
+  static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
   if (global_walk) {
+    count = n < (1u << 22)
+                ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over)
+                : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over);
     if (active) {
-    PF_G uint32_t* row = pf_g(F.nl) + s;  // the row of the next entry
-    const PF_G float4* __restrict__ gxw = pf_g(xw);
-    for (int dz = -1; dz <= 1 && !over; ++dz) {
-      const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
-      for (int dy = -1; dy <= 1 && !over; ++dy) {
-        const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
-        for (int dx = -1; dx <= 1 && !over; ++dx) {
-          const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
-          if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
-          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
-                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
-          const uint32_t st = S.start[b];
-          const uint32_t en = S.start[b + 1];
-          // Sixteen candidates (four cache lines) per round trip to memory.
-          for (uint32_t j0 = st; j0 < en && !over; j0 += 16) {
-            float4 xj[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) xj[u] = pf_f4(gxw[min(j0 + u, en - 1u)]);
-            // The distance tests first (most candidates fail), then the
-            // appends of the accepted ones in candidate order.
-            uint32_t m = 0;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-              const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              m |= (j0 + u < en && pf_dot3(d, d) < F.h2) ? (1u << u) : 0u;
-            }
-            while (m != 0u) {
-              // The list is full and another neighbour exists: it overflows,
-              // nothing later in the walk can enter it.
-              if (count >= PF_MAX_NEIGHBORS) {
-                over = true;
-                break;
-              }
-              const uint32_t j = j0 + (uint32_t)__builtin_ctz(m);
-              m &= m - 1u;
-              *row = j;
-              row += n;
-              ++count;
-            }
-          }
-        }
+      F.nl_count[s] = count;
+      if (want_normal) {
+        // The lane's list was written by the wave's other lanes.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
       }
-    }
-    F.nl_count[s] = count;
-    if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
-    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+      if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     }
     // The passes after this one only need the particles the lists name: if
     // those fit, they become the tile's union and the lists are rewritten as
@@ -552,69 +696,20 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   }
   __syncthreads();
 
-  // 4. Walk the union. Completed entry pairs go straight to their list row.
+  // 4. Walk the union.
   PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
+  count = n < (1u << 22)
+              ? walk_runs<true, false>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over)
+              : walk_runs<true, true>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over);
   if (active) {
-    uint32_t pend = 0;
-    PF_G uint32_t* row = nl16 + s;  // the row of the next entry pair
-    for (int dz = -1; dz <= 1 && !over; ++dz) {
-      const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
-      for (int dy = -1; dy <= 1 && !over; ++dy) {
-        const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
-        for (int dx = -1; dx <= 1 && !over; ++dx) {
-          const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
-          if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
-          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
-                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
-          uint32_t h = tu_hash(b);
-          for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
-          if (h_key[h] != b + 1u) continue;  // unreachable: the cell's leader inserted b
-          const uint32_t ho = h_off[h];
-          const uint32_t off = ho >> 16, cn = ho & 0xffffu;
-          // The distance tests of a group of eight first (most candidates
-          // fail), then the appends of the accepted ones in candidate order;
-          // only a completed entry pair stores. Whole groups need no bounds
-          // test; the last partial group does.
-          auto group = [&](uint32_t i0, bool tail) {
-            float3 xj[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const uint32_t e = off + (tail ? min(i0 + u, cn - 1u) : i0 + u);
-              xj[u] = make_float3(u_px[e], u_py[e], u_pz[e]);
-            }
-            uint32_t m = 0;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-              m |= ((!tail || i0 + u < cn) && pf_dot3(d, d) < F.h2) ? (1u << u) : 0u;
-            }
-            while (m != 0u) {
-              // The list is full and another neighbour exists: it overflows,
-              // nothing later in the walk can enter it.
-              if (count >= PF_MAX_NEIGHBORS) {
-                over = true;
-                break;
-              }
-              const uint32_t local = off + i0 + (uint32_t)__builtin_ctz(m);
-              m &= m - 1u;
-              if (count & 1u) {
-                *row = pend | (local << 16);
-                row += n;
-              }
-              pend = local;
-              ++count;
-            }
-          };
-          const uint32_t full = cn & ~7u;
-          for (uint32_t i0 = 0; i0 < full && !over; i0 += 8) group(i0, false);
-          if (full < cn && !over) group(full, true);
-        }
-      }
-    }
-    if (count & 1u) *row = pend;
+    // The upper half of an odd list's last word is zero.
+    if (count & 1u) *((PF_G uint16_t*)(nl16 + (size_t)(count >> 1) * n + s) + 1) = 0;
     F.nl_count[s] = count;
     if (want_normal) {
-      // Colour-field normal over the finished list, in list order.
+      // Colour-field normal over the finished list, in list order. The lane's
+      // list was written by the wave's other lanes.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       const PF_G uint32_t* __restrict__ uslots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
       const PF_G float4* __restrict__ gxs = pf_g(xs);
       float3 nsum = make_float3(0.f, 0.f, 0.f);

@@ -20,6 +20,7 @@
 
 #define PF_MAX_FLUIDS 4
 #define PF_MAX_NEIGHBORS 64          // neighborList.ts:14
+#define PF_FLUID_MAX_PARTICLES (1u << 24)  // particles of a system with fluids (32-bit list offsets)
 #define PF_MAX_BOUNDARIES 8
 #define PF_MAX_SUBSTEPS 64
 #define PF_MORTON_BIAS 512           // mortonHash.ts:25

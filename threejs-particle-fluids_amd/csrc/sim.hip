@@ -1427,6 +1427,9 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     g.host = S;
   }
   S.nfluids = (uint32_t)L->fluids.size();
+  if (S.nfluids && n > PF_FLUID_MAX_PARTICLES)
+    fail(PF_EINVAL, "SimLoop: a system with fluids holds at most %u particles (neighbour-list offsets are 32-bit), got %u",
+         (uint32_t)PF_FLUID_MAX_PARTICLES, n);
   for (uint32_t k = 0; k < S.nfluids; ++k) {
     pf_fluid* f = L->fluids[k];
     f->slot = (int)k;
