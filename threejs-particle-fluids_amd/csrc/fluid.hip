@@ -183,27 +183,29 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   __syncthreads();
   const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
   {
-    // Eight list entries loaded per round trip, then inserted.
+    // Eight list entries loaded per round trip; their home slots are claimed
+    // together and only a collision probes on (a list holds distinct slots).
     volatile uint32_t* give_up = &ctl[1];
     for (uint32_t k0 = 0; k0 < count && !*give_up; k0 += 8u) {
-      uint32_t keys[8];
+      uint32_t keys[8], h[8], prev[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) keys[u] = col[(size_t)min(k0 + u, count - 1u) * n] + 1u;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        if (k0 + u >= count) break;
-        const uint32_t key = keys[u];
-        uint32_t h = cu_home(key, hs);
-        for (uint32_t probe = 0; probe < hs; ++probe) {
-          const uint32_t prev = atomicCAS(&hk[h], 0u, key);
-          if (prev == 0u) {
-            if (atomicAdd(&ctl[0], 1u) + 1u > limit) *give_up = 1u;
-            break;
-          }
-          if (prev == key) break;
-          h = (h + 1u == hs) ? 0u : h + 1u;
-        }
+        h[u] = cu_home(keys[u], hs);
+        prev[u] = k0 + u < count ? atomicCAS(&hk[h[u]], 0u, keys[u]) : keys[u];
       }
+      uint32_t added = 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        uint32_t pv = prev[u], hh = h[u];
+        for (uint32_t probe = 1; probe < hs && pv != 0u && pv != keys[u]; ++probe) {
+          hh = (hh + 1u == hs) ? 0u : hh + 1u;
+          pv = atomicCAS(&hk[hh], 0u, keys[u]);
+        }
+        added += pv == 0u ? 1u : 0u;
+      }
+      if (added && atomicAdd(&ctl[0], added) + added > limit) *give_up = 1u;
     }
   }
   __syncthreads();
@@ -230,15 +232,21 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   // Eight entries loaded per round trip; entries past the count are zero.
   PF_G uint32_t* __restrict__ row = pf_g(F.nl16) + s;
   for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
-    uint32_t keys[8], idx[8];
+    uint32_t keys[8], h[8], v[8], idx[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) keys[u] = col[(size_t)min(k0 + u, count - 1u) * n] + 1u;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const uint32_t key = keys[u];
-      uint32_t h = cu_home(key, hs);
-      for (uint32_t probe = 0; probe < hs && hk[h] != key; ++probe) h = (h + 1u == hs) ? 0u : h + 1u;
-      idx[u] = k0 + u < count ? (uint32_t)hv[h] : 0u;
+      h[u] = cu_home(keys[u], hs);
+      v[u] = hk[h[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      for (uint32_t probe = 1; probe < hs && v[u] != keys[u]; ++probe) {
+        h[u] = (h[u] + 1u == hs) ? 0u : h[u] + 1u;
+        v[u] = hk[h[u]];
+      }
+      idx[u] = k0 + u < count ? (uint32_t)hv[h[u]] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < 8; u += 2)
