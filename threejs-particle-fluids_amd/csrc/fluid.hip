@@ -440,6 +440,15 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
   return active ? count : 0u;
 }
 
+// Surface tension's per-particle record, two float4s at F.normal + 2 s: the
+// prediction and density, then the colour-field normal. A pair gathers its
+// partner's record from one cache line.
+static __device__ __forceinline__ void store_record(const SceneDev& S, const FluidDev& F, int p, uint32_t s,
+                                                    float3 xi, float4 nrm) {
+  F.normal[2 * s] = make_float4(xi.x, xi.y, xi.z, S.rho[p][s]);
+  F.normal[2 * s + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour list (+ colour-field normal when surface tension is on)
 //
@@ -533,7 +542,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
           }
         }
     F.nl_count[s] = count;
-    if (want_normal) F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
+    if (want_normal) store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     return;
   }
@@ -626,7 +635,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
         // The lane's list was written by the wave's other lanes.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        F.normal[s] = color_normal_global(S, F, p, xs, xi, s, count);
+        store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
       }
       if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     }
@@ -745,7 +754,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       const uint32_t full = count & ~3u;
       for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
       if (full < count) group(full, true);
-      F.normal[s] = make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f);
+      store_record(S, F, p, s, xi, make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f));
     }
     if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
   }
@@ -880,7 +889,7 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
       const uint32_t idj = S.id[p][j];
       if (idj > id && idj < fend) {
         float3 dv;
-        if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[j]), rhoi, S.rho[p][j], &dv)) {
+        if (pair_dv(F, S.dt, xi, pf_xyz(xs[j]), ni, pf_xyz(F.normal[2 * j + 1]), rhoi, S.rho[p][j], &dv)) {
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
@@ -901,14 +910,14 @@ static __device__ void impulses_tiny(const SceneDev& S, const FluidDev& F, int p
 // list of its lower id, the change added to both ends, so a pair exists iff
 // the lower end's list holds it: this particle evaluates the pairs its list
 // holds with a higher fluid id, adds the change to its own ticks and the
-// negated change to the partner's (in LDS for a staged tile's union entry,
-// flushed once per tile; else straight to the accumulator). Pairs where it is
-// the upper end arrive from the partner the same way.
+// negated change straight to the partner's accumulator. Pairs where it is the
+// upper end arrive from the partner the same way. (A staged tile with surface
+// tension takes impulse_tile_flat instead.)
 static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const FluidDev& F, int p,
                                                     const float4* __restrict__ xs, uint32_t s, uint32_t id,
                                                     float3 xi, float3 ni, float rhoi, uint32_t cnt, bool st_pairs,
                                                     bool adh_on, uint32_t U, const uint32_t* __restrict__ uslots,
-                                                    int* acc_l, bool lds_acc, int tk[3]) {
+                                                    int tk[3]) {
   const uint32_t n = S.n;
   const uint32_t fend = F.start + F.count;
   const bool staged = U != PF_TU_GLOBAL;
@@ -922,8 +931,8 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
   const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
   for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
     uint32_t jj[4];
-    uint32_t idx[4] = {0u, 0u, 0u, 0u};
     if (staged) {
+      uint32_t idx[4];
       c16.next(k0, idx);
 #pragma unroll
       for (int u = 0; u < 4; ++u) jj[u] = g_slot[idx[u]];
@@ -949,7 +958,7 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
       rj[u] = 0.0f;
       if (lower[u] || (in && vol[u] > 0.0f)) xj[u] = pf_f4(g_x[jj[u]]);
       if (lower[u]) {
-        nj[u] = pf_f4(g_nrm[jj[u]]);
+        nj[u] = pf_f4(g_nrm[2 * jj[u] + 1]);
         rj[u] = g_rho[jj[u]];
       }
     }
@@ -962,106 +971,10 @@ static __device__ __forceinline__ void impulse_walk(const SceneDev& S, const Flu
           tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
           tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
           tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
-          if (lds_acc) {
-            // the partner is union entry idx: summed in LDS, flushed once per tile
-            atomicAdd(acc_l + 3 * idx[u] + 0, pf_ticks(-dv.x, F.imp_scale));
-            atomicAdd(acc_l + 3 * idx[u] + 1, pf_ticks(-dv.y, F.imp_scale));
-            atomicAdd(acc_l + 3 * idx[u] + 2, pf_ticks(-dv.z, F.imp_scale));
-          } else {
-            scatter_half(S.ct.acc, jj[u], dv, F.imp_scale);
-          }
+          scatter_half(S.ct.acc, jj[u], dv, F.imp_scale);
         }
       }
       if (adh_on && k0 + u < cnt && vol[u] > 0.0f) adhesion_add(S, F, xi, xjv, vol[u], tk);
-    }
-  }
-}
-
-// The walk of a staged tile, eight list entries per group: their ids decide
-// which entries are pairs this particle evaluates (a higher fluid id), and
-// only those are evaluated, one per iteration, so a lane spends its time on
-// the pairs it has instead of idling through the entries where it is the
-// upper end. Same sums as impulse_walk. s_slot is the tile union's sorted
-// slots, staged in LDS.
-static __device__ __forceinline__ void impulse_walk_staged(const SceneDev& S, const FluidDev& F, int p,
-                                                           const float4* __restrict__ xs, uint32_t s, uint32_t id,
-                                                           float3 xi, float3 ni, float rhoi, uint32_t cnt,
-                                                           bool st_pairs, bool adh_on, uint32_t U,
-                                                           const uint32_t* s_slot, int* acc_l, int tk[3]) {
-  const uint32_t n = S.n;
-  const uint32_t fend = F.start + F.count;
-  const uint32_t lim = U ? U - 1u : 0u;
-  const PF_G uint32_t* __restrict__ col = pf_g(F.nl16) + s;
-  const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
-  const PF_G float4* __restrict__ g_x = pf_g(xs);
-  const PF_G float4* __restrict__ g_nrm = pf_g(F.normal);
-  const PF_G float* __restrict__ g_rho = pf_g(S.rho[p]);
-  const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
-  for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
-    const uint32_t r0 = k0 >> 1, rmax = PF_MAX_NEIGHBORS / 2 - 1;
-    const uint32_t w0 = col[(size_t)min(r0, rmax) * n], w1 = col[(size_t)min(r0 + 1u, rmax) * n];
-    const uint32_t w2 = col[(size_t)min(r0 + 2u, rmax) * n], w3 = col[(size_t)min(r0 + 3u, rmax) * n];
-    auto entry = [=](uint32_t u) {  // union index of entry k0 + u
-      const bool b1 = (u & 2u) != 0u, b2 = (u & 4u) != 0u;
-      const uint32_t lo = b1 ? w1 : w0, hi = b1 ? w3 : w2;
-      const uint32_t a = b2 ? hi : lo;
-      return min((a >> ((u & 1u) * 16u)) & (PF_TU_MAX - 1u), lim);
-    };
-    uint32_t m = 0, ma = 0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const uint32_t j = s_slot[entry((uint32_t)u)];
-      const bool in = k0 + (uint32_t)u < cnt;
-      const uint32_t idj = g_id[j];
-      m |= (st_pairs && in && idj > id && idj < fend) ? (1u << u) : 0u;
-      if (adh_on) ma |= (in && g_vol[j] > 0.0f) ? (1u << u) : 0u;
-    }
-    // The next pair's loads are issued before this pair's arithmetic.
-    uint32_t ix = 0u;
-    float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), na = xa;
-    float ra = 0.0f;
-    bool have = m != 0u;
-    if (have) {
-      ix = entry((uint32_t)__builtin_ctz(m));
-      m &= m - 1u;
-      const uint32_t j = s_slot[ix];
-      xa = pf_f4(g_x[j]);
-      na = pf_f4(g_nrm[j]);
-      ra = g_rho[j];
-    }
-    while (have) {
-      const bool more = m != 0u;
-      uint32_t ixb = 0u;
-      float4 xb = xa, nb = na;
-      float rb = ra;
-      if (more) {
-        ixb = entry((uint32_t)__builtin_ctz(m));
-        m &= m - 1u;
-        const uint32_t j = s_slot[ixb];
-        xb = pf_f4(g_x[j]);
-        nb = pf_f4(g_nrm[j]);
-        rb = g_rho[j];
-      }
-      float3 dv;
-      if (pair_dv(F, S.dt, xi, pf_xyz(xa), ni, pf_xyz(na), rhoi, ra, &dv)) {
-        tk[0] = pf_wrap_add(tk[0], pf_ticks(dv.x, F.imp_scale));
-        tk[1] = pf_wrap_add(tk[1], pf_ticks(dv.y, F.imp_scale));
-        tk[2] = pf_wrap_add(tk[2], pf_ticks(dv.z, F.imp_scale));
-        // the partner is union entry ix: summed in LDS, flushed once per tile
-        atomicAdd(acc_l + 3 * ix + 0, pf_ticks(-dv.x, F.imp_scale));
-        atomicAdd(acc_l + 3 * ix + 1, pf_ticks(-dv.y, F.imp_scale));
-        atomicAdd(acc_l + 3 * ix + 2, pf_ticks(-dv.z, F.imp_scale));
-      }
-      have = more;
-      ix = ixb;
-      xa = xb;
-      na = nb;
-      ra = rb;
-    }
-    while (ma != 0u) {
-      const uint32_t j = s_slot[entry((uint32_t)__builtin_ctz(ma))];
-      ma &= ma - 1u;
-      adhesion_add(S, F, xi, pf_xyz(pf_f4(g_x[j])), g_vol[j], tk);
     }
   }
 }
@@ -1072,8 +985,7 @@ static __device__ __forceinline__ void impulse_walk_staged(const SceneDev& S, co
 // fluid with adhesion alone applies here.
 template <bool ST, bool ADH>
 static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const FluidDev& F, int p, int c,
-                                                        uint32_t s, uint32_t tile, int* acc_l, bool lds_acc,
-                                                        const uint32_t* s_slot) {
+                                                        uint32_t s, uint32_t tile) {
   const float4* __restrict__ xs = S.xs[c];
   const float4 xi4 = xs[s];
   const uint32_t id = S.id[p][s];
@@ -1095,17 +1007,15 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
     return;
   }
   if (st_on) {
-    ni = pf_xyz(F.normal[s]);
+    ni = pf_xyz(F.normal[2 * s + 1]);
     rhoi = S.rho[p][s];
   }
   const uint32_t U = F.tu_size[tile];
   const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
   if (S.H < 64u) {
     impulses_tiny(S, F, p, c, s, id, xi, ni, rhoi, cnt, st_on, adh_on, tk);
-  } else if (lds_acc) {
-    impulse_walk_staged(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, s_slot, acc_l, tk);
   } else {
-    impulse_walk(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, uslots, acc_l, lds_acc, tk);
+    impulse_walk(S, F, p, xs, s, id, xi, ni, rhoi, cnt, st_on, adh_on, U, uslots, tk);
   }
   if (st_on) {
     S.xs[c ^ 1][s] = make_float4(__int_as_float(tk[0]), __int_as_float(tk[1]), __int_as_float(tk[2]), xi4.w);
@@ -1122,6 +1032,150 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
   S.xs[c ^ 1][s] = make_float4(xi.x + ch.x * S.dt, xi.y + ch.y * S.dt, xi.z + ch.z * S.dt, xi4.w);
 }
 
+// Surface tension of a staged tile with its pairs spread over the wave. The
+// lanes read their lists eight entries at a time; each entry naming a pair the
+// particle evaluates (a higher fluid id) goes to the wave's pair queue in LDS
+// as (lane, union index), and the wave then evaluates the queue 64 pairs at a
+// time, one per lane, whoever's pairs they are. Both halves of a pair are
+// fixed-point sums (the particle's own in own_l, the partner's in acc_l), so
+// the order the pairs are evaluated in does not change them. s_id holds the
+// union's particle ids (the lower-end test reads LDS only); a pair reads its
+// partner's slot from the tile's union slots and its record from global
+// memory. Adhesion stays with its particle's lane. Every thread of the block
+// calls it.
+#define PF_PAIR_QUEUE (64 * 8)  // a round's pairs per wave: eight entries per lane
+template <bool ADH>
+static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, const FluidDev& F, int p, int c,
+                                                         uint32_t first, uint32_t tile, uint32_t U, int* acc_l,
+                                                         const uint32_t* s_id, uint32_t* q_l, int* own_l,
+                                                         float4* xi_l, float4* ni_l) {
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t s = first + threadIdx.x;
+  const uint32_t n = S.n;
+  const float4* __restrict__ xs = S.xs[c];
+  const uint32_t fend = F.start + F.count;
+  const uint32_t lim = U ? U - 1u : 0u;
+  const bool adh_on = ADH && (F.effects & PF_FX_ADHESION) && S.has_boundary;
+  uint32_t* q = q_l + wave * PF_PAIR_QUEUE;
+  int* own = own_l + wave * 64 * 3;
+  float4* xw = xi_l + wave * 64;
+  float4* nw = ni_l + wave * 64;
+  float4 xi4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t id = 0, cnt = 0;
+  bool act = false;
+  if (s < n) {
+    xi4 = xs[s];
+    id = S.id[p][s];
+    act = in_range(id, F.start, F.count);
+  }
+  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;  // the particle's record
+  if (act) {
+    cnt = F.nl_count[s];
+    r0 = F.normal[2 * s];
+    r1 = F.normal[2 * s + 1];
+  }
+  xw[lane] = r0;
+  nw[lane] = r1;
+  own[3 * lane + 0] = 0;
+  own[3 * lane + 1] = 0;
+  own[3 * lane + 2] = 0;
+  int tk[3] = {0, 0, 0};  // adhesion
+  const PF_G uint32_t* __restrict__ col = pf_g(F.nl16) + s;
+  const PF_G float4* __restrict__ g_rec = pf_g(F.normal);
+  const PF_G uint32_t* __restrict__ g_slot = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+  const PF_G float4* __restrict__ g_x = pf_g(xs);
+  const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
+  const float3 xi = pf_xyz(xi4);
+  for (uint32_t k0 = 0; __builtin_amdgcn_ballot_w64(k0 < cnt) != 0ull; k0 += 8u) {
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    uint32_t m = 0, ma = 0;
+    if (k0 < cnt) {
+      const uint32_t r0 = k0 >> 1, rmax = PF_MAX_NEIGHBORS / 2 - 1;
+      w0 = col[(size_t)min(r0, rmax) * n];
+      w1 = col[(size_t)min(r0 + 1u, rmax) * n];
+      w2 = col[(size_t)min(r0 + 2u, rmax) * n];
+      w3 = col[(size_t)min(r0 + 3u, rmax) * n];
+    }
+    auto entry = [=](uint32_t u) {  // union index of entry k0 + u
+      const bool b1 = (u & 2u) != 0u, b2 = (u & 4u) != 0u;
+      const uint32_t lo = b1 ? w1 : w0, hi = b1 ? w3 : w2;
+      const uint32_t a = b2 ? hi : lo;
+      return min((a >> ((u & 1u) * 16u)) & (PF_TU_MAX - 1u), lim);
+    };
+    if (k0 < cnt) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t ix = entry((uint32_t)u);
+        const bool in = k0 + (uint32_t)u < cnt;
+        const uint32_t idj = s_id[ix];
+        m |= (in && idj > id && idj < fend) ? (1u << u) : 0u;
+        if (adh_on) ma |= (in && g_vol[g_slot[ix]] > 0.0f) ? (1u << u) : 0u;
+      }
+    }
+    // The round's pairs, queued in lane order.
+    const uint32_t np = (uint32_t)__popc(m);
+    const uint32_t incl = wave_inclusive_sum64(np);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
+    uint32_t at = incl - np;
+    while (m != 0u) {
+      q[at++] = (lane << 11) | entry((uint32_t)__builtin_ctz(m));
+      m &= m - 1u;
+    }
+    static_assert(PF_TU_MAX <= (1u << 11), "union index must fit 11 bits");
+    __builtin_amdgcn_wave_barrier();
+    // 64 pairs per pass, the next pass's loads issued before this one's
+    // arithmetic.
+    uint32_t e = lane < total ? q[lane] : 0u;
+    uint32_t ix = e & 2047u, il = e >> 11;
+    float4 xa = pf_f4(xw[il]), na = pf_f4(nw[il]);
+    uint32_t j = g_slot[ix];
+    float4 xb = pf_f4(g_rec[2 * j]), nb = pf_f4(g_rec[2 * j + 1]);
+    for (uint32_t q0 = 0; q0 < total; q0 += 64u) {
+      const bool mine = q0 + lane < total;
+      const uint32_t qn = q0 + 64u + lane;
+      const uint32_t en = qn < total ? q[qn] : 0u;
+      const uint32_t ixn = en & 2047u, iln = en >> 11;
+      const float4 xan = pf_f4(xw[iln]), nan = pf_f4(nw[iln]);
+      const uint32_t jn = g_slot[ixn];
+      const float4 xbn = pf_f4(g_rec[2 * jn]), nbn = pf_f4(g_rec[2 * jn + 1]);
+      float3 dv;
+      if (mine && pair_dv(F, S.dt, pf_xyz(xa), pf_xyz(xb), pf_xyz(na), pf_xyz(nb), xa.w, xb.w, &dv)) {
+        atomicAdd(own + 3 * il + 0, pf_ticks(dv.x, F.imp_scale));
+        atomicAdd(own + 3 * il + 1, pf_ticks(dv.y, F.imp_scale));
+        atomicAdd(own + 3 * il + 2, pf_ticks(dv.z, F.imp_scale));
+        atomicAdd(acc_l + 3 * ix + 0, pf_ticks(-dv.x, F.imp_scale));
+        atomicAdd(acc_l + 3 * ix + 1, pf_ticks(-dv.y, F.imp_scale));
+        atomicAdd(acc_l + 3 * ix + 2, pf_ticks(-dv.z, F.imp_scale));
+      }
+      ix = ixn;
+      il = iln;
+      xa = xan;
+      na = nan;
+      xb = xbn;
+      nb = nbn;
+    }
+    while (ma != 0u) {
+      const uint32_t jj = g_slot[entry((uint32_t)__builtin_ctz(ma))];
+      ma &= ma - 1u;
+      adhesion_add(S, F, xi, pf_xyz(pf_f4(g_x[jj])), g_vol[jj], tk);
+    }
+    // The next round's queue overwrites this one's.
+    __builtin_amdgcn_wave_barrier();
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (s >= n) return;
+  if (!act) {
+    S.xs[c ^ 1][s] = xi4;
+    return;
+  }
+  // The particle's own sum, parked as bits for k_impulse_apply.
+  tk[0] = pf_wrap_add(tk[0], own[3 * lane + 0]);
+  tk[1] = pf_wrap_add(tk[1], own[3 * lane + 1]);
+  tk[2] = pf_wrap_add(tk[2], own[3 * lane + 2]);
+  S.xs[c ^ 1][s] = make_float4(__int_as_float(tk[0]), __int_as_float(tk[1]), __int_as_float(tk[2]), xi4.w);
+}
+
 // A staged tile sums the halves its particles scatter to its union's entries
 // in LDS and adds each touched entry's sums to the accumulator once at the
 // end; k_impulse_apply applies once every half has arrived.
@@ -1130,7 +1184,12 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, int c) {
   __shared__ int acc_l[ST ? 3 * PF_TU_MAX : 1];
-  __shared__ uint32_t s_slot[ST ? PF_TU_MAX : 1];
+  __shared__ uint32_t s_id[ST ? PF_TU_MAX : 1];
+  // The waves' pair queues, own sums and own records.
+  __shared__ uint32_t q_l[ST ? PF_BLOCK / 64 * PF_PAIR_QUEUE : 1];
+  __shared__ int own_l[ST ? PF_BLOCK * 3 : 1];
+  __shared__ float4 xi_l[ST ? PF_BLOCK : 1];
+  __shared__ float4 ni_l[ST ? PF_BLOCK : 1];
   const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
   const SceneDev& S = scenes[t.scene];
   const FluidDev& F = S.fluid[fi];
@@ -1140,13 +1199,26 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
   const bool lds = ST && st_on && S.H >= 64u && U != PF_TU_GLOBAL;
   if (lds) {
-    const PF_G uint32_t* __restrict__ g_slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
     for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
-    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) s_slot[u] = g_slots[u];
+    // The union's particle ids, every gather of a thread in flight at once.
+    const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
+    uint32_t js[PF_TU_PER], ids[PF_TU_PER];
+    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) ids[i] = g_id[js[i]];
+#pragma unroll
+    for (int i = 0; i < PF_TU_PER; ++i) {
+      const uint32_t u = threadIdx.x + i * PF_BLOCK;
+      if (u < U) s_id[u] = ids[i];
+    }
     __syncthreads();
   }
   const uint32_t s = t.first + threadIdx.x;
-  if (s < S.n) impulse_particle<ST, ADH>(S, F, p, c, s, tile, acc_l, lds, s_slot);
+  if (lds) {
+    if constexpr (ST) impulse_tile_flat<ADH>(S, F, p, c, t.first, tile, U, acc_l, s_id, q_l, own_l, xi_l, ni_l);
+  } else if (s < S.n) {
+    impulse_particle<ST, ADH>(S, F, p, c, s, tile);
+  }
   if (lds) {
     __syncthreads();
     const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;

@@ -81,7 +81,7 @@ struct FluidDev {
   uint32_t* tu_size;   // [tiles] union size of each 256-slot tile, or PF_TU_GLOBAL
   uint32_t* tu_slots;  // [tiles * PF_TU_MAX] sorted slot of each union entry
   uint32_t* nl_count;  // [n]
-  float4* normal;      // [n] colour-field normal
+  float4* normal;      // [2 n] pair records: (prediction, density), (colour-field normal, id bits)
   float4* omega;       // [n] vorticity (xyz) and |omega| (w)
   float4* dv;          // [n] XSPH velocity change
   int* reaction;       // [3n] fixed-point boundary reaction (only with boundaries)

@@ -1440,7 +1440,7 @@ int pf_loop_create(pf_particles* ps, const pf_loop_options* opt, const pf_materi
     f->nl16 = dalloc<uint32_t>((size_t)(PF_MAX_NEIGHBORS / 2) * n);
     f->tu_size = dalloc<uint32_t>(ntiles);
     f->tu_slots = dalloc<uint32_t>(ntiles * PF_TU_MAX);
-    if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(n);
+    if (f->present & PF_FLUID_SURFACE_TENSION) f->normal = dalloc<float4>(2 * (size_t)n);  // pair records
     if (f->present & PF_FLUID_VORTICITY) f->omega = dalloc<float4>(n);
     if (f->present & PF_FLUID_VISCOSITY) f->dv = dalloc<float4>(n);
     if (!f->boundaries.empty()) {
