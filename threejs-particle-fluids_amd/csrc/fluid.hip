@@ -17,6 +17,8 @@
 //   k_vorticity     vorticity gradient + velocity apply, optionally fused
 //                   kinetic friction (velocity.ts:74-119, PrimitiveSet.ts:366-391)
 // plus standalone k_walls / k_advect / k_friction for scenes that cannot fuse.
+#include <type_traits>
+
 #include "collide.hpp"
 #include "pf_internal.hpp"
 #include "pf_kernels.hpp"
@@ -778,17 +780,34 @@ struct Nl16Cursor {  // staged tile: 16-bit union indices, two per word
     w1 = col[n];
   }
   // Union indices of entries k0..k0+3; prefetches those of k0+4..k0+7.
+  // CLAMP: some of the four may lie past the count (stale words); entries
+  // inside the count are union indices already.
+  template <bool CLAMP = true>
   __device__ __forceinline__ void next(uint32_t k0, uint32_t idx[4]) {
     const uint32_t c0 = w0, c1 = w1;
     const uint32_t r = min((k0 >> 1) + 2u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 2));
     w0 = col[(size_t)r * n];
     w1 = col[(size_t)(r + 1) * n];
-    idx[0] = min(c0 & (PF_TU_MAX - 1), lim);
-    idx[1] = min((c0 >> 16) & (PF_TU_MAX - 1), lim);
-    idx[2] = min(c1 & (PF_TU_MAX - 1), lim);
-    idx[3] = min((c1 >> 16) & (PF_TU_MAX - 1), lim);
+    idx[0] = c0 & (PF_TU_MAX - 1);
+    idx[1] = (c0 >> 16) & (PF_TU_MAX - 1);
+    idx[2] = c1 & (PF_TU_MAX - 1);
+    idx[3] = (c1 >> 16) & (PF_TU_MAX - 1);
+    if (CLAMP) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) idx[u] = min(idx[u], lim);
+    }
   }
 };
+
+// Calls body(std::true_type) when no active lane has a square root argument
+// in the scaled range, else body(std::false_type); scaled is the OR of
+// ballot(pf_sqrt_scaled(q)) over the group's arguments, so the choice is
+// uniform over the wave.
+template <class Body>
+static __device__ __forceinline__ void with_sqrt_form(uint64_t scaled, Body&& body) {
+  if (scaled == 0ull) body(std::true_type{});
+  else body(std::false_type{});
+}
 
 struct NlCursor {  // global-gather tile: sorted slots
   const PF_G uint32_t* __restrict__ col;
@@ -1369,13 +1388,14 @@ __global__ __launch_bounds__(PF_BLOCK) void k_isv_apply(const SceneDev* __restri
 
 // One neighbour's terms of the density and lambda sums (lambda.ts:40-58),
 // added when ok.
+template <bool DIRECT = false>
 static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, float3 xi, float4 xj,
                                                    float mj, float wj, float& rho, float3& gs,
                                                    float& gss) {
   const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
   const float q = pf_dot3(d, d);
   const float r1 = rho + mj * pf_poly6(q, F.h2, F.poly6);
-  const float sc = pf_spiky_scale(d, F.h, F.spiky) * mj;
+  const float sc = pf_spiky_scale<DIRECT>(d, F.h, F.spiky) * mj;
   const float3 g = pf_mul3(d, sc);
   const float3 gs1 = pf_add3(gs, g);
   const float gss1 = gss + ((q * sc) * sc) * wj;  // |g|^2 = |d|^2 sc^2
@@ -1435,16 +1455,23 @@ __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict_
     // Whole groups of four need no per-entry test; the last partial group does.
     auto group = [&](uint32_t k0, bool tail) {
       uint32_t idx[4];
-      cur.next(k0, idx);
+      if (tail) cur.next<true>(k0, idx);
+      else cur.next<false>(k0, idx);
       float4 xj[4];
       float wj[4];
+      uint64_t scaled = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         xj[u] = u_x[idx[u]];
         wj[u] = u_w[idx[u]];
+        const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+        scaled |= __builtin_amdgcn_ballot_w64(pf_sqrt_scaled(pf_dot3(d, d)));
       }
+      with_sqrt_form(scaled, [&](auto direct) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) lambda_term(F, !tail || k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
+        for (int u = 0; u < 4; ++u)
+          lambda_term<decltype(direct)::value>(F, !tail || k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
+      });
     };
     const uint32_t full = cnt & ~3u;
     for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);

@@ -316,6 +316,14 @@ __device__ __forceinline__ float pf_sqrt_rn(float q) {
   return tiny ? s * 0x1p-32f : s;
 }
 
+// 0 < q < 2^-96, the one range of q >= 0 where pf_sqrt_rn_fast is not exact
+// and pf_sqrt_rn scales. Passes test a group of square roots with it and take
+// the direct form (DIRECT = true below) when no lane of the wave needs the
+// scaled one: two instructions per root instead of five.
+__device__ __forceinline__ bool pf_sqrt_scaled(float q) {
+  return __float_as_uint(q) - 1u < 0x0F800000u - 1u;  // 0x0F800000 = 2^-96
+}
+
 // ---------------------------------------------------------------------------
 // SPH kernels (kernels.ts:64-107), FP32, operand order as the reference
 // ---------------------------------------------------------------------------
@@ -329,10 +337,12 @@ __device__ __forceinline__ float pf_poly6(float r2, float h2, float poly6) {
 // grad W = r_vec * sp, sp = -(t*t*spikyCoef) / max(|r|, 1e-20), t = max(h - |r|, 0)
 // (one rounded reciprocal: the reference's WGSL division is within 2.5 ulp).
 // The passes fold their per-neighbour factor into sp before it meets r_vec,
-// one product per component, as the oracle does.
+// one product per component, as the oracle does. DIRECT: |r|^2 is known not
+// to need the scaled square root (pf_sqrt_scaled false).
+template <bool DIRECT = false>
 __device__ __forceinline__ float pf_spiky_scale(float3 r, float h, float spiky) {
   const float rsq = (r.x * r.x + r.y * r.y) + r.z * r.z;
-  const float rl = pf_sqrt_rn(rsq);
+  const float rl = DIRECT ? pf_sqrt_rn_fast(rsq) : pf_sqrt_rn(rsq);
   const float t = fmaxf(h - rl, 0.0f);
   const float inv = pf_rcp_rn(fmaxf(rl, 1e-20f));
   const float mag = -((t * t) * spiky);
