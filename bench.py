@@ -235,8 +235,14 @@ def run_batched(args, ctx):
         batch.step(dt)
     tuned = None
     if not args.no_autotune:
+        # the tile-union capacity, then the launch shape (one workgroup per
+        # tile, or the persistent form at 4, 8 or 16 workgroups per CU) at it
         best, times = batch.autotune(dt, steps=2)
-        tuned = {"tile_union_capacity": best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()}}
+        shape, shape_times = batch.autotuneLaunch(dt, candidates=(0, 1024, 2048, 4096), steps=2)
+        tuned = {"tile_union_capacity": best, "ms_per_step": {str(k): round(v, 3) for k, v in times.items()},
+                 "launch_shape": shape,
+                 "launch_ms_per_step": {("per_tile" if k == 0 else f"persistent_{k}"): round(v, 3)
+                                        for k, v in shape_times.items()}}
         print(f"[bench] batched autotune: {tuned}", file=sys.stderr, flush=True)
     for L in loops:  # the plain form's graphs, captured before timing
         L.step(dt)

@@ -41,6 +41,7 @@ typedef struct pf_loop_s pf_loop;
 typedef struct pf_batch_s pf_batch;
 typedef struct pf_viscosity_s pf_viscosity;
 typedef struct pf_fence_s pf_fence;
+typedef struct pf_server_s pf_server;
 
 /* ---- errors / version ---------------------------------------------------- */
 const char* pf_last_error(void);
@@ -445,6 +446,24 @@ int pf_loop_autotune(pf_loop* loop, double dt, const uint32_t* candidates, uint3
 int pf_batch_autotune(pf_batch* batch, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
                       uint32_t* best, float* ms_per_step);
 
+/* ---- Launch shape (no counterpart in the reference; the reference's passes
+ * are one compute dispatch each, src/core/loop.ts:276-278). workgroups = 0
+ * launches every tile kernel of a step with one 256-thread workgroup per
+ * 256-slot tile; a positive value (a multiple of 8, at most 2^20) is the
+ * persistent form: each launch has at most that many workgroups, and each
+ * walks the tile list grid-stride. Results do not depend on the shape. A
+ * loop's or batch's graph is re-captured; a server applies it to launches
+ * formed after the call. */
+int pf_loop_set_launch_shape(pf_loop* loop, uint32_t workgroups);
+int pf_batch_set_launch_shape(pf_batch* batch, uint32_t workgroups);
+int pf_server_set_launch_shape(pf_server* srv, uint32_t workgroups);
+/* The autotuner over launch shapes: times each candidate (0 or multiples of
+ * 8) like pf_loop_autotune / pf_batch_autotune and sets the fastest. */
+int pf_loop_autotune_launch(pf_loop* loop, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                            uint32_t* best, float* ms_per_step);
+int pf_batch_autotune_launch(pf_batch* batch, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                             uint32_t* best, float* ms_per_step);
+
 /* ---- Served form: a request loop over live scenes. No counterpart in the
  * reference. Step requests for scenes added to the server are queued as they
  * arrive; a worker thread launches every queued request whose scene is not
@@ -456,7 +475,6 @@ int pf_batch_autotune(pf_batch* batch, double dt, const uint32_t* candidates, ui
  * run in submission order, one step each. While a scene is served, the
  * ordinary calls on it (step, readback, upload, new materials) fail with
  * PF_ESTATE; pf_server_remove_scene hands it back, resident. */
-typedef struct pf_server_s pf_server;
 typedef struct pf_server_options {
   uint64_t hbm_budget_bytes;     /* resident scene bytes; 0 = no cap */
   uint32_t max_batch_scenes;     /* scenes per launch; 0 = 256 */

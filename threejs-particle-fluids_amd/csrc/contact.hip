@@ -54,70 +54,73 @@ static __device__ __forceinline__ void ct_scatter(int4* acc, uint32_t s, float3 
 // ---------------------------------------------------------------------------
 #define PF_CT_LDS 1024
 
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_contact_generate(const SceneDev* __restrict__ scenes,
                                                                const Tile* __restrict__ tiles,
                                                                uint32_t ntiles, int p, int c) {
   __shared__ uint2 buf[PF_CT_LDS];
   __shared__ uint32_t nbuf, base;
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  if (!S.ct.on) return;  // uniform over the block
-  const uint32_t tid = threadIdx.x;
-  if (tid == 0) nbuf = 0u;
-  __syncthreads();
-  const uint32_t s = t.first + tid;
-  if (s < S.n) {
-    const uint32_t I = S.id[p][s];
-    if (ct_emitter(S, I)) {
-      const float4* __restrict__ xs = S.xs[c];
-      const float3 xi = pf_xyz(xs[s]);
-      const uint32_t gi = S.grp[p][s];
-      const float wi = S.w[p][s];
-      const float r3[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
-      int q[3];
-      float lo[3], hi[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float fl = floorf(r3[a]);
-        q[a] = pf_f2i_sat(fl);
-        lo[a] = (r3[a] - fl) * S.cell;
-        hi[a] = (1.0f - (r3[a] - fl)) * S.cell;
-      }
-      const float lim = S.ct.cr + 1e-4f * S.cell;
-      const float lim2 = lim * lim;
-      uint32_t visited[27];
-      uint32_t nv = 0;
-      for (int dz = -1; dz <= 1; ++dz) {
-        const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
-        for (int dy = -1; dy <= 1; ++dy) {
-          const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
-          for (int dx = -1; dx <= 1; ++dx) {
-            const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
-            if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
-            const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
-                                         (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
-            bool seen = false;
-            for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
-            visited[nv++] = b;
-            if (seen) continue;
-            const uint32_t en = S.start[b + 1];
-            for (uint32_t j = S.start[b]; j < en; ++j) {
-              const float3 xj = pf_xyz(xs[j]);
-              const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-              if (pf_dot3(d, d) >= S.ct.cr2) continue;
-              const uint32_t J = S.id[p][j];
-              if (J == I) continue;
-              if (J < I && ct_emitter(S, J)) continue;
-              if (gi != 0u && gi == S.grp[p][j]) continue;
-              if (wi <= 0.0f && S.w[p][j] <= 0.0f) continue;
-              const uint32_t k = atomicAdd(&nbuf, 1u);
-              if (k < PF_CT_LDS) {
-                buf[k] = make_uint2(s, j);
-              } else {
-                const uint32_t g = atomicAdd(S.ct.count, 1u);
-                if (g < S.ct.max) {
-                  S.ct.pair[g] = make_uint2(s, j);
-                  S.ct.lam[g] = make_int2(0, 0);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    if (!S.ct.on) return;  // uniform over the block
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) nbuf = 0u;
+    __syncthreads();
+    const uint32_t s = t.first + tid;
+    if (s < S.n) {
+      const uint32_t I = S.id[p][s];
+      if (ct_emitter(S, I)) {
+        const float4* __restrict__ xs = S.xs[c];
+        const float3 xi = pf_xyz(xs[s]);
+        const uint32_t gi = S.grp[p][s];
+        const float wi = S.w[p][s];
+        const float r3[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
+        int q[3];
+        float lo[3], hi[3];
+  #pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float fl = floorf(r3[a]);
+          q[a] = pf_f2i_sat(fl);
+          lo[a] = (r3[a] - fl) * S.cell;
+          hi[a] = (1.0f - (r3[a] - fl)) * S.cell;
+        }
+        const float lim = S.ct.cr + 1e-4f * S.cell;
+        const float lim2 = lim * lim;
+        uint32_t visited[27];
+        uint32_t nv = 0;
+        for (int dz = -1; dz <= 1; ++dz) {
+          const float ez = dz < 0 ? lo[2] : (dz > 0 ? hi[2] : 0.0f);
+          for (int dy = -1; dy <= 1; ++dy) {
+            const float ey = dy < 0 ? lo[1] : (dy > 0 ? hi[1] : 0.0f);
+            for (int dx = -1; dx <= 1; ++dx) {
+              const float ex = dx < 0 ? lo[0] : (dx > 0 ? hi[0] : 0.0f);
+              if ((ex * ex + ey * ey) + ez * ez > lim2) continue;
+              const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                           (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+              bool seen = false;
+              for (uint32_t k = 0; k < nv; ++k) seen |= visited[k] == b;
+              visited[nv++] = b;
+              if (seen) continue;
+              const uint32_t en = S.start[b + 1];
+              for (uint32_t j = S.start[b]; j < en; ++j) {
+                const float3 xj = pf_xyz(xs[j]);
+                const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+                if (pf_dot3(d, d) >= S.ct.cr2) continue;
+                const uint32_t J = S.id[p][j];
+                if (J == I) continue;
+                if (J < I && ct_emitter(S, J)) continue;
+                if (gi != 0u && gi == S.grp[p][j]) continue;
+                if (wi <= 0.0f && S.w[p][j] <= 0.0f) continue;
+                const uint32_t k = atomicAdd(&nbuf, 1u);
+                if (k < PF_CT_LDS) {
+                  buf[k] = make_uint2(s, j);
+                } else {
+                  const uint32_t g = atomicAdd(S.ct.count, 1u);
+                  if (g < S.ct.max) {
+                    S.ct.pair[g] = make_uint2(s, j);
+                    S.ct.lam[g] = make_int2(0, 0);
+                  }
                 }
               }
             }
@@ -125,19 +128,19 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_generate(const SceneDev* _
         }
       }
     }
-  }
-  __syncthreads();
-  const uint32_t nb = min(nbuf, (uint32_t)PF_CT_LDS);
-  if (tid == 0 && nb) base = atomicAdd(S.ct.count, nb);
-  __syncthreads();
-  for (uint32_t k = tid; k < nb; k += PF_BLOCK) {
-    const uint32_t g = base + k;
-    if (g < S.ct.max) {
-      S.ct.pair[g] = buf[k];
-      // resetLambdaKernel (ContactBuffer.ts:67-71), for the records in use
-      S.ct.lam[g] = make_int2(0, 0);
+    __syncthreads();
+    const uint32_t nb = min(nbuf, (uint32_t)PF_CT_LDS);
+    if (tid == 0 && nb) base = atomicAdd(S.ct.count, nb);
+    __syncthreads();
+    for (uint32_t k = tid; k < nb; k += PF_BLOCK) {
+      const uint32_t g = base + k;
+      if (g < S.ct.max) {
+        S.ct.pair[g] = buf[k];
+        // resetLambdaKernel (ContactBuffer.ts:67-71), for the records in use
+        S.ct.lam[g] = make_int2(0, 0);
+      }
     }
-  }
+  });
 }
 
 // Records of this substep: contacts [t.first + tid, used) in steps of the
@@ -177,31 +180,34 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_stabilize(const SceneDev* 
 // the sorted copy the grid was built from stays behind for the fluid
 // neighbour build (neighborList.ts:59). Scenes without contacts in the same
 // launch only copy.
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_contact_stabilize_apply(const SceneDev* __restrict__ scenes,
                                                                       const Tile* __restrict__ tiles,
                                                                       uint32_t ntiles, int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  float4 xs = S.xs[c][s];
-  if (S.ct.on) {
-    const int4 a = S.ct.acc[s];
-    S.ct.acc[s] = make_int4(0, 0, 0, 0);
-    const int tk[3] = {a.x, a.y, a.z};
-    bool sat;
-    const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
-    if (sat) pf_flag_raise(S.flags + PF_FLAG_POSITIONS);
-    float4 x = S.x[p][s];
-    x.x = x.x + ch.x;
-    x.y = x.y + ch.y;
-    x.z = x.z + ch.z;
-    S.x[p][s] = x;
-    xs.x = xs.x + ch.x;
-    xs.y = xs.y + ch.y;
-    xs.z = xs.z + ch.z;
-  }
-  S.xs[c ^ 1][s] = xs;
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n) return;
+    float4 xs = S.xs[c][s];
+    if (S.ct.on) {
+      const int4 a = S.ct.acc[s];
+      S.ct.acc[s] = make_int4(0, 0, 0, 0);
+      const int tk[3] = {a.x, a.y, a.z};
+      bool sat;
+      const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
+      if (sat) pf_flag_raise(S.flags + PF_FLAG_POSITIONS);
+      float4 x = S.x[p][s];
+      x.x = x.x + ch.x;
+      x.y = x.y + ch.y;
+      x.z = x.z + ch.z;
+      S.x[p][s] = x;
+      xs.x = xs.x + ch.x;
+      xs.y = xs.y + ch.y;
+      xs.z = xs.z + ch.z;
+    }
+    S.xs[c ^ 1][s] = xs;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -283,13 +289,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_contact_friction(const SceneDev* _
 // ---------------------------------------------------------------------------
 void launch_contact_generate(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_contact_generate, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tl, ntiles, p, c);
+  launch_tiles(k_contact_generate<false>, k_contact_generate<true>, ntiles, 0, st, sc, tl, ntiles, p, c);
 }
 
 void launch_contact_stabilize(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles,
                               const Tile* ctl, uint32_t nctiles, int p, int c) {
   if (nctiles) hipLaunchKernelGGL(k_contact_stabilize, dim3(nctiles), dim3(PF_BLOCK), 0, st, sc, ctl, p);
-  if (ntiles) hipLaunchKernelGGL(k_contact_stabilize_apply, dim3(ntiles), dim3(PF_BLOCK), 0, st, sc, tl, ntiles, p, c);
+  launch_tiles(k_contact_stabilize_apply<false>, k_contact_stabilize_apply<true>, ntiles, 0, st, sc, tl, ntiles, p, c);
 }
 
 void launch_contact_solve(hipStream_t st, const SceneDev* sc, const Tile* ctl, uint32_t nctiles, int p, int c) {

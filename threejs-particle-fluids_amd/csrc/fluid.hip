@@ -473,7 +473,7 @@ static __device__ __forceinline__ void store_record(const SceneDev& S, const Flu
 // reference's sortedPredictedPositions, neighborList.ts:59), while the
 // colour-field normal reads current positions.
 // ---------------------------------------------------------------------------
-template <bool NORMAL>
+template <bool NORMAL, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
@@ -492,275 +492,277 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(u_words);
   uint32_t* a_sh = u_words + 3 * PF_BLOCK;
   __shared__ uint32_t ctl[4 + PF_BLOCK / 64];  // distinct buckets, give-up flag, -, -, scan
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t s = t.first + tid;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t n = S.n;
-  const bool active = s < n && in_range(S.id[p][s], F.start, F.count);
-  const float4* __restrict__ xs = S.xs[c];
-  const float4* __restrict__ xw = snap ? S.xs[c ^ 1] : xs;  // candidate positions
-  float3 xi = make_float3(0.f, 0.f, 0.f);
-  int q[3] = {0, 0, 0};
-  if (active) {
-    xi = pf_xyz(xs[s]);
-    const float r[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) q[a] = pf_f2i_sat(floorf(r[a]));
-  }
-  const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
-  uint32_t count = 0;
-  bool over = false;
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t s = t.first + tid;
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t n = S.n;
+    const bool active = s < n && in_range(S.id[p][s], F.start, F.count);
+    const float4* __restrict__ xs = S.xs[c];
+    const float4* __restrict__ xw = snap ? S.xs[c ^ 1] : xs;  // candidate positions
+    float3 xi = make_float3(0.f, 0.f, 0.f);
+    int q[3] = {0, 0, 0};
+    if (active) {
+      xi = pf_xyz(xs[s]);
+      const float r[3] = {(xi.x - S.ox) / S.cell, (xi.y - S.oy) / S.cell, (xi.z - S.oz) / S.cell};
+  #pragma unroll
+      for (int a = 0; a < 3; ++a) q[a] = pf_f2i_sat(floorf(r[a]));
+    }
+    const bool want_normal = NORMAL && (F.effects & PF_FX_SURFACE_TENSION);
+    uint32_t count = 0;
+    bool over = false;
 
-  // Tables below 64 buckets: global walk with the reference's bucket dedup.
-  // Such tables belong to scenes of at most 16 particles (H = nextPow2(2N)).
-  if (S.H < 64u) {
-    if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
-    if (!active) return;
-    uint32_t visited[27];
-    uint32_t nvisited = 0;
-    for (int dz = -1; dz <= 1 && !over; ++dz)
-      for (int dy = -1; dy <= 1 && !over; ++dy)
-        for (int dx = -1; dx <= 1 && !over; ++dx) {
-          const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
-                                       (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
-          bool seen = false;
-          for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == b);
-          visited[nvisited++] = b;
-          if (seen) continue;
-          for (uint32_t j = S.start[b]; j < S.start[b + 1] && !over; ++j) {
-            const float4 xj = xw[j];
-            const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-            if (pf_dot3(d, d) < F.h2) {
-              if (count < PF_MAX_NEIGHBORS) {
-                F.nl[(size_t)count * n + s] = j;
-                ++count;
-              } else {
-                over = true;
+    // Tables below 64 buckets: global walk with the reference's bucket dedup.
+    // Such tables belong to scenes of at most 16 particles (H = nextPow2(2N)).
+    if (S.H < 64u) {
+      if (tid == 0) F.tu_size[tile] = PF_TU_GLOBAL;
+      if (!active) return;
+      uint32_t visited[27];
+      uint32_t nvisited = 0;
+      for (int dz = -1; dz <= 1 && !over; ++dz)
+        for (int dy = -1; dy <= 1 && !over; ++dy)
+          for (int dx = -1; dx <= 1 && !over; ++dx) {
+            const uint32_t b = pf_morton((int)((uint32_t)q[0] + (uint32_t)dx), (int)((uint32_t)q[1] + (uint32_t)dy),
+                                         (int)((uint32_t)q[2] + (uint32_t)dz)) & S.hash_mask;
+            bool seen = false;
+            for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == b);
+            visited[nvisited++] = b;
+            if (seen) continue;
+            for (uint32_t j = S.start[b]; j < S.start[b + 1] && !over; ++j) {
+              const float4 xj = xw[j];
+              const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+              if (pf_dot3(d, d) < F.h2) {
+                if (count < PF_MAX_NEIGHBORS) {
+                  F.nl[(size_t)count * n + s] = j;
+                  ++count;
+                } else {
+                  over = true;
+                }
               }
             }
           }
-        }
-    F.nl_count[s] = count;
-    if (want_normal) store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
-    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
-    return;
-  }
+      F.nl_count[s] = count;
+      if (want_normal) store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
+      if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+      return;
+    }
 
-  // 1. Bucket set: the first particle of each run of equal query cells inserts
-  // its 27 stencil buckets.
-  for (uint32_t i = tid; i < PF_TU_HASH; i += PF_BLOCK) h_key[i] = 0u;
-  if (tid < 4) ctl[tid] = 0u;
-  q_sh[0][tid] = q[0];
-  q_sh[1][tid] = q[1];
-  q_sh[2][tid] = q[2];
-  a_sh[tid] = active ? 1 : 0;
-  __syncthreads();
-  const bool leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] ||
-                                 q_sh[1][tid - 1] != q[1] || q_sh[2][tid - 1] != q[2]);
-  // The leaders' cells, in lane order (h_start is free until the layout);
-  // then every thread inserts (cell, stencil) pairs, 27 per leader.
-  uint32_t nlead;
-  const uint32_t li = block_exclusive_scan(leader ? 1u : 0u, ctl + 4, &nlead);
-  if (leader) {
-    h_start[3 * li + 0] = (uint32_t)q[0];
-    h_start[3 * li + 1] = (uint32_t)q[1];
-    h_start[3 * li + 2] = (uint32_t)q[2];
-  }
-  __syncthreads();
-  {
-    volatile uint32_t* give_up = &ctl[1];
-    for (uint32_t task = tid; task < nlead * 27u && !*give_up; task += PF_BLOCK) {
-      const uint32_t ci = task / 27u, d = task - ci * 27u;
-      const uint32_t dx = d % 3u - 1u, dy = (d / 3u) % 3u - 1u, dz = d / 9u - 1u;
-      const uint32_t b = pf_morton((int)(h_start[3 * ci + 0] + dx), (int)(h_start[3 * ci + 1] + dy),
-                                   (int)(h_start[3 * ci + 2] + dz)) & S.hash_mask;
-      uint32_t h = tu_hash(b);
-      bool placed = false;
-      for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
-        const uint32_t prev = atomicCAS(&h_key[h], 0u, b + 1u);
-        if (prev == 0u) {
-          if (atomicAdd(&ctl[0], 1u) + 1u > PF_TU_HASH_LIMIT) *give_up = 1u;
-          placed = true;
-          break;
-        }
-        if (prev == b + 1u) {
-          placed = true;
-          break;
-        }
-        h = (h + 1u) & (PF_TU_HASH - 1u);
-      }
-      if (!placed) *give_up = 1u;
-    }
-  }
-  __syncthreads();
-  bool global_walk = ctl[1] != 0u;
-  uint32_t U = 0;
-  if (!global_walk) {
-    // 2. Union layout: buckets in hash-slot order, offsets by a block scan.
-    constexpr int PER = PF_TU_HASH / PF_BLOCK;
-    uint32_t cn[PER], st[PER], sum = 0;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t key = h_key[tid * PER + i];
-      cn[i] = 0u;
-      st[i] = 0u;
-      if (key) {
-        st[i] = S.start[key - 1u];
-        cn[i] = S.start[key] - st[i];
-      }
-      sum += cn[i];
-    }
-    uint32_t off = block_exclusive_scan(sum, ctl + 4, &U);
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      h_off[tid * PER + i] = (min(off, 0xffffu) << 16) | min(cn[i], 0xffffu);
-      h_start[tid * PER + i] = st[i];
-      off += cn[i];
-    }
-    global_walk = U > min(F.tu_limit, cap);
+    // 1. Bucket set: the first particle of each run of equal query cells inserts
+    // its 27 stencil buckets.
+    for (uint32_t i = tid; i < PF_TU_HASH; i += PF_BLOCK) h_key[i] = 0u;
+    if (tid < 4) ctl[tid] = 0u;
+    q_sh[0][tid] = q[0];
+    q_sh[1][tid] = q[1];
+    q_sh[2][tid] = q[2];
+    a_sh[tid] = active ? 1 : 0;
     __syncthreads();
-  }
+    const bool leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] ||
+                                   q_sh[1][tid - 1] != q[1] || q_sh[2][tid - 1] != q[2]);
+    // The leaders' cells, in lane order (h_start is free until the layout);
+    // then every thread inserts (cell, stencil) pairs, 27 per leader.
+    uint32_t nlead;
+    const uint32_t li = block_exclusive_scan(leader ? 1u : 0u, ctl + 4, &nlead);
+    if (leader) {
+      h_start[3 * li + 0] = (uint32_t)q[0];
+      h_start[3 * li + 1] = (uint32_t)q[1];
+      h_start[3 * li + 2] = (uint32_t)q[2];
+    }
+    __syncthreads();
+    {
+      volatile uint32_t* give_up = &ctl[1];
+      for (uint32_t task = tid; task < nlead * 27u && !*give_up; task += PF_BLOCK) {
+        const uint32_t ci = task / 27u, d = task - ci * 27u;
+        const uint32_t dx = d % 3u - 1u, dy = (d / 3u) % 3u - 1u, dz = d / 9u - 1u;
+        const uint32_t b = pf_morton((int)(h_start[3 * ci + 0] + dx), (int)(h_start[3 * ci + 1] + dy),
+                                     (int)(h_start[3 * ci + 2] + dz)) & S.hash_mask;
+        uint32_t h = tu_hash(b);
+        bool placed = false;
+        for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
+          const uint32_t prev = atomicCAS(&h_key[h], 0u, b + 1u);
+          if (prev == 0u) {
+            if (atomicAdd(&ctl[0], 1u) + 1u > PF_TU_HASH_LIMIT) *give_up = 1u;
+            placed = true;
+            break;
+          }
+          if (prev == b + 1u) {
+            placed = true;
+            break;
+          }
+          h = (h + 1u) & (PF_TU_HASH - 1u);
+        }
+        if (!placed) *give_up = 1u;
+      }
+    }
+    __syncthreads();
+    bool global_walk = ctl[1] != 0u;
+    uint32_t U = 0;
+    if (!global_walk) {
+      // 2. Union layout: buckets in hash-slot order, offsets by a block scan.
+      constexpr int PER = PF_TU_HASH / PF_BLOCK;
+      uint32_t cn[PER], st[PER], sum = 0;
+  #pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const uint32_t key = h_key[tid * PER + i];
+        cn[i] = 0u;
+        st[i] = 0u;
+        if (key) {
+          st[i] = S.start[key - 1u];
+          cn[i] = S.start[key] - st[i];
+        }
+        sum += cn[i];
+      }
+      uint32_t off = block_exclusive_scan(sum, ctl + 4, &U);
+  #pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        h_off[tid * PER + i] = (min(off, 0xffffu) << 16) | min(cn[i], 0xffffu);
+        h_start[tid * PER + i] = st[i];
+        off += cn[i];
+      }
+      global_walk = U > min(F.tu_limit, cap);
+      __syncthreads();
+    }
 
-  // h_start is free once the layout is read: each wave's run-walk marks.
-  uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
-  static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
-  if (global_walk) {
+    // h_start is free once the layout is read: each wave's run-walk marks.
+    uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
+    static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
+    if (global_walk) {
+      count = n < (1u << 22)
+                  ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over)
+                  : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over);
+      if (active) {
+        F.nl_count[s] = count;
+        if (want_normal) {
+          // The lane's list was written by the wave's other lanes.
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
+        }
+        if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
+      }
+      // The passes after this one only need the particles the lists name: if
+      // those fit, they become the tile's union and the lists are rewritten as
+      // 16-bit indices into it, so the tile is staged in LDS like the others.
+      const uint32_t Uc = compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl,
+                                        active ? count : 0u, s);
+      if (tid == 0) F.tu_size[tile] = Uc;
+      return;
+    }
+
+    // 3. Stage the union: entry u belongs to the last bucket whose union offset
+    // is not above u. Each bucket marks its offset with its hash slot + 1 and a
+    // chunked prefix max spreads the marks; the marks and the resolved slots
+    // use the union's LDS (4 entries per float4, so 8 per thread fit).
+    {
+      uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+      uint32_t* mark = u_words;
+      const uint32_t u8 = (U + 7u) & ~7u;
+      for (uint32_t u = tid; u < u8; u += PF_BLOCK) mark[u] = 0u;
+      __syncthreads();
+      constexpr int PER = PF_TU_HASH / PF_BLOCK;
+  #pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const uint32_t h = tid * PER + i;
+        const uint32_t ho = h_off[h];
+        if (ho & 0xffffu) mark[ho >> 16] = h + 1u;
+      }
+      __syncthreads();
+      const uint32_t u0 = tid * 8u;
+      uint32_t mk[8], run = 0;
+      if (u0 < U) {
+        const uint4 a = *reinterpret_cast<const uint4*>(mark + u0);
+        const uint4 b = *reinterpret_cast<const uint4*>(mark + u0 + 4u);
+        const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  #pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          run = max(run, m[k]);
+          mk[k] = run;
+        }
+      }
+      const uint32_t carry = block_exclusive_max(run, ctl + 4);
+      if (u0 < U) {
+        uint32_t r[8];
+  #pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t h = max(mk[k], carry) - 1u;  // entry 0 opens the first bucket
+          r[k] = h_start[h] + ((u0 + (uint32_t)k) - (h_off[h] >> 16));
+        }
+        *reinterpret_cast<uint4*>(mark + u0) = make_uint4(r[0], r[1], r[2], r[3]);
+        *reinterpret_cast<uint4*>(mark + u0 + 4u) = make_uint4(r[4], r[5], r[6], r[7]);
+      }
+      __syncthreads();
+      uint32_t js[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = tid + i * PF_BLOCK;
+        js[i] = u < U ? mark[u] : 0u;
+      }
+      __syncthreads();
+      float4 xv[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = tid + i * PF_BLOCK;
+        if (u < U) {
+          u_px[u] = xv[i].x;
+          u_py[u] = xv[i].y;
+          u_pz[u] = xv[i].z;
+          slots[u] = js[i];
+        }
+      }
+    }
+    __syncthreads();
+
+    // 4. Walk the union.
+    PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
     count = n < (1u << 22)
-                ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over)
-                : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over);
+                ? walk_runs<true, false>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over)
+                : walk_runs<true, true>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over);
     if (active) {
+      // The upper half of an odd list's last word is zero.
+      if (count & 1u) *((PF_G uint16_t*)(nl16 + (size_t)(count >> 1) * n + s) + 1) = 0;
       F.nl_count[s] = count;
       if (want_normal) {
-        // The lane's list was written by the wave's other lanes.
+        // Colour-field normal over the finished list, in list order. The lane's
+        // list was written by the wave's other lanes.
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        store_record(S, F, p, s, xi, color_normal_global(S, F, p, xs, xi, s, count));
+        const PF_G uint32_t* __restrict__ uslots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+        const PF_G float4* __restrict__ gxs = pf_g(xs);
+        float3 nsum = make_float3(0.f, 0.f, 0.f);
+        // Whole groups of four need no per-entry test; the last partial group does.
+        auto group = [&](uint32_t k0, bool tail) {
+          const uint32_t w0 = nl16[(size_t)(k0 >> 1) * n + s];
+          const uint32_t w1 = nl16[(size_t)min((k0 >> 1) + 1u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 1)) * n + s];
+          // entries past the count may hold stale indices: keep them inside the union
+          const uint32_t ul = U ? U - 1u : 0u;
+          const uint32_t idx[4] = {(uint32_t)min(w0 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w0 >> 16) & (PF_TU_MAX - 1u), ul),
+                                   (uint32_t)min(w1 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w1 >> 16) & (PF_TU_MAX - 1u), ul)};
+          float3 xj[4];
+          float vol[4];
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = snap ? pf_xyz(gxs[uslots[idx[u]]]) : make_float3(u_px[idx[u]], u_py[idx[u]], u_pz[idx[u]]);
+            vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
+          }
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+            nsum = sel3(!tail || k0 + u < count, pf_add3(nsum, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * vol[u])), nsum);
+          }
+        };
+        const uint32_t full = count & ~3u;
+        for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+        if (full < count) group(full, true);
+        store_record(S, F, p, s, xi, make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f));
       }
       if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
     }
-    // The passes after this one only need the particles the lists name: if
-    // those fit, they become the tile's union and the lists are rewritten as
-    // 16-bit indices into it, so the tile is staged in LDS like the others.
-    const uint32_t Uc = compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl,
-                                      active ? count : 0u, s);
-    if (tid == 0) F.tu_size[tile] = Uc;
-    return;
-  }
-
-  // 3. Stage the union: entry u belongs to the last bucket whose union offset
-  // is not above u. Each bucket marks its offset with its hash slot + 1 and a
-  // chunked prefix max spreads the marks; the marks and the resolved slots
-  // use the union's LDS (4 entries per float4, so 8 per thread fit).
-  {
-    uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-    uint32_t* mark = u_words;
-    const uint32_t u8 = (U + 7u) & ~7u;
-    for (uint32_t u = tid; u < u8; u += PF_BLOCK) mark[u] = 0u;
-    __syncthreads();
-    constexpr int PER = PF_TU_HASH / PF_BLOCK;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const uint32_t h = tid * PER + i;
-      const uint32_t ho = h_off[h];
-      if (ho & 0xffffu) mark[ho >> 16] = h + 1u;
-    }
-    __syncthreads();
-    const uint32_t u0 = tid * 8u;
-    uint32_t mk[8], run = 0;
-    if (u0 < U) {
-      const uint4 a = *reinterpret_cast<const uint4*>(mark + u0);
-      const uint4 b = *reinterpret_cast<const uint4*>(mark + u0 + 4u);
-      const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        run = max(run, m[k]);
-        mk[k] = run;
-      }
-    }
-    const uint32_t carry = block_exclusive_max(run, ctl + 4);
-    if (u0 < U) {
-      uint32_t r[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t h = max(mk[k], carry) - 1u;  // entry 0 opens the first bucket
-        r[k] = h_start[h] + ((u0 + (uint32_t)k) - (h_off[h] >> 16));
-      }
-      *reinterpret_cast<uint4*>(mark + u0) = make_uint4(r[0], r[1], r[2], r[3]);
-      *reinterpret_cast<uint4*>(mark + u0 + 4u) = make_uint4(r[4], r[5], r[6], r[7]);
-    }
-    __syncthreads();
-    uint32_t js[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = tid + i * PF_BLOCK;
-      js[i] = u < U ? mark[u] : 0u;
-    }
-    __syncthreads();
-    float4 xv[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = tid + i * PF_BLOCK;
-      if (u < U) {
-        u_px[u] = xv[i].x;
-        u_py[u] = xv[i].y;
-        u_pz[u] = xv[i].z;
-        slots[u] = js[i];
-      }
-    }
-  }
-  __syncthreads();
-
-  // 4. Walk the union.
-  PF_G uint32_t* __restrict__ nl16 = pf_g(F.nl16);
-  count = n < (1u << 22)
-              ? walk_runs<true, false>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over)
-              : walk_runs<true, true>(S, F, xw, u_px, u_py, u_pz, h_key, h_off, marks, active, leader, q, xi, s, &over);
-  if (active) {
-    // The upper half of an odd list's last word is zero.
-    if (count & 1u) *((PF_G uint16_t*)(nl16 + (size_t)(count >> 1) * n + s) + 1) = 0;
-    F.nl_count[s] = count;
-    if (want_normal) {
-      // Colour-field normal over the finished list, in list order. The lane's
-      // list was written by the wave's other lanes.
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const PF_G uint32_t* __restrict__ uslots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
-      const PF_G float4* __restrict__ gxs = pf_g(xs);
-      float3 nsum = make_float3(0.f, 0.f, 0.f);
-      // Whole groups of four need no per-entry test; the last partial group does.
-      auto group = [&](uint32_t k0, bool tail) {
-        const uint32_t w0 = nl16[(size_t)(k0 >> 1) * n + s];
-        const uint32_t w1 = nl16[(size_t)min((k0 >> 1) + 1u, (uint32_t)(PF_MAX_NEIGHBORS / 2 - 1)) * n + s];
-        // entries past the count may hold stale indices: keep them inside the union
-        const uint32_t ul = U ? U - 1u : 0u;
-        const uint32_t idx[4] = {(uint32_t)min(w0 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w0 >> 16) & (PF_TU_MAX - 1u), ul),
-                                 (uint32_t)min(w1 & (PF_TU_MAX - 1u), ul), (uint32_t)min((w1 >> 16) & (PF_TU_MAX - 1u), ul)};
-        float3 xj[4];
-        float vol[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          xj[u] = snap ? pf_xyz(gxs[uslots[idx[u]]]) : make_float3(u_px[idx[u]], u_py[idx[u]], u_pz[idx[u]]);
-          vol[u] = S.has_boundary ? neighbor_volume(S, F, p, uslots[idx[u]]) : F.pv;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-          nsum = sel3(!tail || k0 + u < count, pf_add3(nsum, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * vol[u])), nsum);
-        }
-      };
-      const uint32_t full = count & ~3u;
-      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
-      if (full < count) group(full, true);
-      store_record(S, F, p, s, xi, make_float4(nsum.x * F.h, nsum.y * F.h, nsum.z * F.h, 0.0f));
-    }
-    if (over) flag_set(S, p ^ 1, PF_FLAG_NEIGHBORS);
-  }
-  if (tid == 0) F.tu_size[tile] = U;
+    if (tid == 0) F.tu_size[tile] = U;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1198,7 +1200,7 @@ static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, cons
 // A staged tile sums the halves its particles scatter to its union's entries
 // in LDS and adds each touched entry's sums to the accumulator once at the
 // end; k_impulse_apply applies once every half has arrived.
-template <bool ST, bool ADH>
+template <bool ST, bool ADH, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, int c) {
@@ -1209,80 +1211,85 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   __shared__ int own_l[ST ? PF_BLOCK * 3 : 1];
   __shared__ float4 xi_l[ST ? PF_BLOCK : 1];
   __shared__ float4 ni_l[ST ? PF_BLOCK : 1];
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t tile = t.first / PF_BLOCK;
-  // Uniform over the block.
-  const bool st_on = ST && (F.effects & PF_FX_SURFACE_TENSION);
-  const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
-  const bool lds = ST && st_on && S.H >= 64u && U != PF_TU_GLOBAL;
-  if (lds) {
-    for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
-    // The union's particle ids, every gather of a thread in flight at once.
-    const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
-    uint32_t js[PF_TU_PER], ids[PF_TU_PER];
-    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) ids[i] = g_id[js[i]];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = threadIdx.x + i * PF_BLOCK;
-      if (u < U) s_id[u] = ids[i];
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t tile = t.first / PF_BLOCK;
+    // Uniform over the block.
+    const bool st_on = ST && (F.effects & PF_FX_SURFACE_TENSION);
+    const uint32_t U = st_on ? F.tu_size[tile] : PF_TU_GLOBAL;
+    const bool lds = ST && st_on && S.H >= 64u && U != PF_TU_GLOBAL;
+    if (lds) {
+      for (uint32_t u = threadIdx.x; u < 3 * U; u += PF_BLOCK) acc_l[u] = 0;
+      // The union's particle ids, every gather of a thread in flight at once.
+      const PF_G uint32_t* __restrict__ g_id = pf_g(S.id[p]);
+      uint32_t js[PF_TU_PER], ids[PF_TU_PER];
+      tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) ids[i] = g_id[js[i]];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = threadIdx.x + i * PF_BLOCK;
+        if (u < U) s_id[u] = ids[i];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  const uint32_t s = t.first + threadIdx.x;
-  if (lds) {
-    if constexpr (ST) impulse_tile_flat<ADH>(S, F, p, c, t.first, tile, U, acc_l, s_id, q_l, own_l, xi_l, ni_l);
-  } else if (s < S.n) {
-    impulse_particle<ST, ADH>(S, F, p, c, s, tile);
-  }
-  if (lds) {
-    __syncthreads();
-    const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-    for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) {
-      const int a0 = acc_l[3 * u], a1 = acc_l[3 * u + 1], a2 = acc_l[3 * u + 2];
-      if ((a0 | a1 | a2) != 0) {
-        PF_G int* a = pf_g(reinterpret_cast<int*>(S.ct.acc + uslots[u]));
-        pf_atomic_add_g(a + 0, a0);
-        pf_atomic_add_g(a + 1, a1);
-        pf_atomic_add_g(a + 2, a2);
+    const uint32_t s = t.first + threadIdx.x;
+    if (lds) {
+      if constexpr (ST) impulse_tile_flat<ADH>(S, F, p, c, t.first, tile, U, acc_l, s_id, q_l, own_l, xi_l, ni_l);
+    } else if (s < S.n) {
+      impulse_particle<ST, ADH>(S, F, p, c, s, tile);
+    }
+    if (lds) {
+      __syncthreads();
+      const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+      for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) {
+        const int a0 = acc_l[3 * u], a1 = acc_l[3 * u + 1], a2 = acc_l[3 * u + 2];
+        if ((a0 | a1 | a2) != 0) {
+          PF_G int* a = pf_g(reinterpret_cast<int*>(S.ct.acc + uslots[u]));
+          pf_atomic_add_g(a + 0, a0);
+          pf_atomic_add_g(a + 1, a1);
+          pf_atomic_add_g(a + 2, a2);
+        }
       }
     }
-  }
+  });
 }
 
 // The impulse apply (FluidSystem.ts:253-258, accumulator.ts:101-131) of the
 // fluid particles of scenes with surface tension: the particle's own sum
 // (parked in its output prediction slot) plus the halves scattered to it;
 // v += change, x* += change dt, sums cleared.
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_impulse_apply(const SceneDev* __restrict__ scenes,
                                                             const Tile* __restrict__ tiles, uint32_t ntiles,
                                                             int fi, int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  if (!(F.effects & PF_FX_SURFACE_TENSION)) return;
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || !in_range(S.id[p][s], F.start, F.count)) return;
-  int* a = reinterpret_cast<int*>(S.ct.acc + s);
-  const float4 own = S.xs[c ^ 1][s];
-  int tk[3] = {pf_wrap_add(a[0], __float_as_int(own.x)), pf_wrap_add(a[1], __float_as_int(own.y)),
-               pf_wrap_add(a[2], __float_as_int(own.z))};
-  a[0] = 0;
-  a[1] = 0;
-  a[2] = 0;
-  bool sat;
-  const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);
-  if (sat) flag_set(S, 0, PF_FLAG_IMPULSES);
-  const float4 xi4 = S.xs[c][s];
-  float4 v = S.v[p][s];
-  v.x = v.x + ch.x;
-  v.y = v.y + ch.y;
-  v.z = v.z + ch.z;
-  S.v[p][s] = v;
-  S.xs[c ^ 1][s] = make_float4(xi4.x + ch.x * S.dt, xi4.y + ch.y * S.dt, xi4.z + ch.z * S.dt, xi4.w);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    if (!(F.effects & PF_FX_SURFACE_TENSION)) return;
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n || !in_range(S.id[p][s], F.start, F.count)) return;
+    int* a = reinterpret_cast<int*>(S.ct.acc + s);
+    const float4 own = S.xs[c ^ 1][s];
+    int tk[3] = {pf_wrap_add(a[0], __float_as_int(own.x)), pf_wrap_add(a[1], __float_as_int(own.y)),
+                 pf_wrap_add(a[2], __float_as_int(own.z))};
+    a[0] = 0;
+    a[1] = 0;
+    a[2] = 0;
+    bool sat;
+    const float3 ch = ticks_to_change(tk, F.imp_inv, &sat);
+    if (sat) flag_set(S, 0, PF_FLAG_IMPULSES);
+    const float4 xi4 = S.xs[c][s];
+    float4 v = S.v[p][s];
+    v.x = v.x + ch.x;
+    v.y = v.y + ch.y;
+    v.z = v.z + ch.z;
+    S.v[p][s] = v;
+    S.xs[c ^ 1][s] = make_float4(xi4.x + ch.x * S.dt, xi4.y + ch.y * S.dt, xi4.z + ch.z * S.dt, xi4.w);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1291,95 +1298,104 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulse_apply(const SceneDev* __re
 // passes. The iterates live in slot order for one substep; slots outside the
 // fluid hold zero velocity in both, as the reference's id-indexed buffers do.
 // ---------------------------------------------------------------------------
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_isv_seed(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles, uint32_t ntiles,
                                                        int fi, int p) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || F.isv_iters == 0u) return;
-  if (in_range(S.id[p][s], F.start, F.count)) {
-    const float4 v = S.v[p][s];
-    F.isv_rhs[s] = v;
-    F.isv_a[s] = v;
-  } else {
-    F.isv_a[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    F.isv_b[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n || F.isv_iters == 0u) return;
+    if (in_range(S.id[p][s], F.start, F.count)) {
+      const float4 v = S.v[p][s];
+      F.isv_rhs[s] = v;
+      F.isv_a[s] = v;
+    } else {
+      F.isv_a[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      F.isv_b[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  });
 }
 
 // Sweep k reads iterate k & 1 and writes the other. A scene of a batch with
 // fewer sweeps copies its iterate through the extra ones.
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_isv_sweep(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles, uint32_t ntiles,
                                                         int fi, int p, uint32_t k) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  const uint32_t n = S.n;
-  if (s >= n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
-  const float4* __restrict__ src = (k & 1u) ? F.isv_b : F.isv_a;
-  float4* __restrict__ dst = (k & 1u) ? F.isv_a : F.isv_b;
-  if (k >= F.isv_iters) {
-    dst[s] = src[s];
-    return;
-  }
-  const float4* __restrict__ X = S.x[p];
-  const float3 xi = pf_xyz(X[s]);
-  const uint32_t cnt = F.nl_count[s];
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t U = F.tu_size[tile];
-  const bool staged = U != PF_TU_GLOBAL;
-  const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-  Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
-  NlCursor cg(F.nl, n, s, cnt);
-  float3 sum = make_float3(0.f, 0.f, 0.f);
-  float weight = 0.0f;
-  for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-    uint32_t jj[4];
-    if (staged) {
-      uint32_t idx[4];
-      c16.next(k0, idx);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
-    } else {
-      cg.next(k0, jj);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    const uint32_t n = S.n;
+    if (s >= n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
+    const float4* __restrict__ src = (k & 1u) ? F.isv_b : F.isv_a;
+    float4* __restrict__ dst = (k & 1u) ? F.isv_a : F.isv_b;
+    if (k >= F.isv_iters) {
+      dst[s] = src[s];
+      return;
     }
-    float4 xj[4], vj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      xj[u] = X[jj[u]];
-      vj[u] = src[jj[u]];
+    const float4* __restrict__ X = S.x[p];
+    const float3 xi = pf_xyz(X[s]);
+    const uint32_t cnt = F.nl_count[s];
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t U = F.tu_size[tile];
+    const bool staged = U != PF_TU_GLOBAL;
+    const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+    Nl16Cursor c16(F.nl16, n, s, staged ? U : 1u);
+    NlCursor cg(F.nl, n, s, cnt);
+    float3 sum = make_float3(0.f, 0.f, 0.f);
+    float weight = 0.0f;
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+      uint32_t jj[4];
+      if (staged) {
+        uint32_t idx[4];
+        c16.next(k0, idx);
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) jj[u] = uslots[idx[u]];
+      } else {
+        cg.next(k0, jj);
+      }
+      float4 xj[4], vj[4];
+  #pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = X[jj[u]];
+        vj[u] = src[jj[u]];
+      }
+  #pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+        const float w = pf_poly6(pf_dot3(d, d), F.h2, F.poly6) * F.pv;
+        const bool ok = k0 + u < cnt;
+        sum = sel3(ok, make_float3(sum.x + vj[u].x * w, sum.y + vj[u].y * w, sum.z + vj[u].z * w), sum);
+        weight = ok ? weight + w : weight;
+      }
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-      const float w = pf_poly6(pf_dot3(d, d), F.h2, F.poly6) * F.pv;
-      const bool ok = k0 + u < cnt;
-      sum = sel3(ok, make_float3(sum.x + vj[u].x * w, sum.y + vj[u].y * w, sum.z + vj[u].z * w), sum);
-      weight = ok ? weight + w : weight;
-    }
-  }
-  const float rate = (F.isv_nu * S.dt) * F.isv_c10;
-  const float4 r = F.isv_rhs[s];
-  const float den = weight * rate + 1.0f;
-  dst[s] = make_float4((r.x + sum.x * rate) / den, (r.y + sum.y * rate) / den, (r.z + sum.z * rate) / den, 0.0f);
+    const float rate = (F.isv_nu * S.dt) * F.isv_c10;
+    const float4 r = F.isv_rhs[s];
+    const float den = weight * rate + 1.0f;
+    dst[s] = make_float4((r.x + sum.x * rate) / den, (r.y + sum.y * rate) / den, (r.z + sum.z * rate) / den, 0.0f);
+  });
 }
 
 // The last iterate replaces the fluid's velocities (w kept).
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_isv_apply(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles, uint32_t ntiles,
                                                         int fi, int p, uint32_t sweeps) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
-  const float4 r = ((sweeps & 1u) ? F.isv_b : F.isv_a)[s];
-  const float4 v = S.v[p][s];
-  S.v[p][s] = make_float4(r.x, r.y, r.z, v.w);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n || F.isv_iters == 0u || !in_range(S.id[p][s], F.start, F.count)) return;
+    const float4 r = ((sweeps & 1u) ? F.isv_b : F.isv_a)[s];
+    const float4 v = S.v[p][s];
+    S.v[p][s] = make_float4(r.x, r.y, r.z, v.w);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1404,105 +1420,108 @@ static __device__ __forceinline__ void lambda_term(const FluidDev& F, bool ok, f
   gss = ok ? gss1 : gss;
 }
 
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_lambda(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, uint32_t ntiles,
                                                      int fi, int p, int c, uint32_t cap) {
   extern __shared__ float4 pf_lds[];
   float4* u_x = pf_lds;                                  // position, neighbour mass
   float* u_w = reinterpret_cast<float*>(pf_lds + cap);   // inverse mass
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t n = S.n;
-  float4* __restrict__ xs = S.xs[c];
-  const uint32_t U = F.tu_size[tile];
-  const bool staged = U != PF_TU_GLOBAL;
-  if (staged) {
-    uint32_t js[PF_TU_PER];
-    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
-    float4 xv[PF_TU_PER];
-    float mv[PF_TU_PER], wv[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      xv[i] = xs[js[i]];
-      mv[i] = S.mass[js[i]];
-      wv[i] = S.w[p][js[i]];
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t n = S.n;
+    float4* __restrict__ xs = S.xs[c];
+    const uint32_t U = F.tu_size[tile];
+    const bool staged = U != PF_TU_GLOBAL;
+    if (staged) {
+      uint32_t js[PF_TU_PER];
+      tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+      float4 xv[PF_TU_PER];
+      float mv[PF_TU_PER], wv[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        xv[i] = xs[js[i]];
+        mv[i] = S.mass[js[i]];
+        wv[i] = S.w[p][js[i]];
+      }
+      if (S.has_boundary) {
+  #pragma unroll
+        for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
+      }
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = threadIdx.x + i * PF_BLOCK;
+        if (u < U) {
+          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, mv[i]);
+          u_w[u] = wv[i];
+        }
+      }
+      __syncthreads();
     }
-    if (S.has_boundary) {
-#pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = threadIdx.x + i * PF_BLOCK;
-      if (u < U) {
-        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, mv[i]);
-        u_w[u] = wv[i];
+    if (s >= n || !in_range(S.id[p][s], F.start, F.count)) return;
+    const float3 xi = pf_xyz(xs[s]);
+    const uint32_t cnt = F.nl_count[s];
+    float rho = 0.0f, gss = 0.0f;
+    float3 gs = make_float3(0.f, 0.f, 0.f);
+    // Four neighbours' loads in flight, then the sums in list order.
+    if (staged) {
+      Nl16Cursor cur(F.nl16, n, s, U);
+      // Whole groups of four need no per-entry test; the last partial group does.
+      auto group = [&](uint32_t k0, bool tail) {
+        uint32_t idx[4];
+        if (tail) cur.next<true>(k0, idx);
+        else cur.next<false>(k0, idx);
+        float4 xj[4];
+        float wj[4];
+        uint64_t scaled = 0;
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = u_x[idx[u]];
+          wj[u] = u_w[idx[u]];
+          const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+          scaled |= __builtin_amdgcn_ballot_w64(pf_sqrt_scaled(pf_dot3(d, d)));
+        }
+        with_sqrt_form(scaled, [&](auto direct) {
+  #pragma unroll
+          for (int u = 0; u < 4; ++u)
+            lambda_term<decltype(direct)::value>(F, !tail || k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
+        });
+      };
+      const uint32_t full = cnt & ~3u;
+      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+      if (full < cnt) group(full, true);
+    } else {
+      NlCursor cur(F.nl, n, s, cnt);
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t jj[4];
+        cur.next(k0, jj);
+        float4 xj[4];
+        float mj[4], wj[4];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = xs[jj[u]];
+          mj[u] = neighbor_mass(S, F, p, jj[u]);
+          wj[u] = S.w[p][jj[u]];
+        }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) lambda_term(F, k0 + u < cnt, xi, xj[u], mj[u], wj[u], rho, gs, gss);
       }
     }
-    __syncthreads();
-  }
-  if (s >= n || !in_range(S.id[p][s], F.start, F.count)) return;
-  const float3 xi = pf_xyz(xs[s]);
-  const uint32_t cnt = F.nl_count[s];
-  float rho = 0.0f, gss = 0.0f;
-  float3 gs = make_float3(0.f, 0.f, 0.f);
-  // Four neighbours' loads in flight, then the sums in list order.
-  if (staged) {
-    Nl16Cursor cur(F.nl16, n, s, U);
-    // Whole groups of four need no per-entry test; the last partial group does.
-    auto group = [&](uint32_t k0, bool tail) {
-      uint32_t idx[4];
-      if (tail) cur.next<true>(k0, idx);
-      else cur.next<false>(k0, idx);
-      float4 xj[4];
-      float wj[4];
-      uint64_t scaled = 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = u_x[idx[u]];
-        wj[u] = u_w[idx[u]];
-        const float3 d = make_float3(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
-        scaled |= __builtin_amdgcn_ballot_w64(pf_sqrt_scaled(pf_dot3(d, d)));
-      }
-      with_sqrt_form(scaled, [&](auto direct) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          lambda_term<decltype(direct)::value>(F, !tail || k0 + u < cnt, xi, xj[u], xj[u].w, wj[u], rho, gs, gss);
-      });
-    };
-    const uint32_t full = cnt & ~3u;
-    for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
-    if (full < cnt) group(full, true);
-  } else {
-    NlCursor cur(F.nl, n, s, cnt);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-      uint32_t jj[4];
-      cur.next(k0, jj);
-      float4 xj[4];
-      float mj[4], wj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = xs[jj[u]];
-        mj[u] = neighbor_mass(S, F, p, jj[u]);
-        wj[u] = S.w[p][jj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) lambda_term(F, k0 + u < cnt, xi, xj[u], mj[u], wj[u], rho, gs, gss);
+    S.rho[p][s] = rho;
+    const float inv_rho0 = 1.0f / F.rho0;
+    const float C = rho * inv_rho0 - 1.0f;
+    float lambda = 0.0f;
+    if (!(C <= 0.0f)) {
+      const float gn = (pf_dot3(gs, gs) * S.w[p][s] + gss) * (inv_rho0 * inv_rho0);
+      const float at = F.compliance / (S.dt * S.dt);
+      lambda = -C / fmaxf(gn + at, 1e-20f);
     }
-  }
-  S.rho[p][s] = rho;
-  const float inv_rho0 = 1.0f / F.rho0;
-  const float C = rho * inv_rho0 - 1.0f;
-  float lambda = 0.0f;
-  if (!(C <= 0.0f)) {
-    const float gn = (pf_dot3(gs, gs) * S.w[p][s] + gss) * (inv_rho0 * inv_rho0);
-    const float at = F.compliance / (S.dt * S.dt);
-    lambda = -C / fmaxf(gn + at, 1e-20f);
-  }
-  xs[s].w = lambda;
+    xs[s].w = lambda;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1537,7 +1556,7 @@ static __device__ __forceinline__ void reaction_add(const SceneDev& S, const Flu
   atomicAdd(F.reaction + 3 * (size_t)j + 2, pf_ticks(g.z * rs, F.rea_scale));
 }
 
-template <bool WALLS, bool ADVECT>
+template <bool WALLS, bool ADVECT, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, uint32_t ntiles,
                                                     int fi, int p, int c, int first_iteration,
@@ -1545,107 +1564,226 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
   extern __shared__ float4 pf_lds[];
   float4* u_x = pf_lds;                                  // position, lambda
   float* u_m = reinterpret_cast<float*>(pf_lds + cap);   // neighbour mass
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t n = S.n;
-  const float4* __restrict__ xs = S.xs[c];
-  const uint32_t U = F.tu_size[tile];
-  const bool staged = U != PF_TU_GLOBAL;
-  if (staged) {
-    uint32_t js[PF_TU_PER];
-    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
-    float4 xv[PF_TU_PER];
-    float mv[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      xv[i] = xs[js[i]];
-      mv[i] = S.mass[js[i]];
-    }
-    if (S.has_boundary) {
-#pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = threadIdx.x + i * PF_BLOCK;
-      if (u < U) {
-        u_x[u] = pf_f4(xv[i]);
-        u_m[u] = mv[i];
-      }
-    }
-    __syncthreads();
-  }
-  if (s >= n) return;
-  const float4 xi4 = xs[s];
-  float3 xn = pf_xyz(xi4);
-  const float w = S.w[p][s];
-  if (in_range(S.id[p][s], F.start, F.count)) {
-    const uint32_t cnt = F.nl_count[s];
-    const float li = xi4.w;
-    const bool react = F.nboundary != 0u && li != 0.0f;
-    const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-    float3 sum = make_float3(0.f, 0.f, 0.f);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t n = S.n;
+    const float4* __restrict__ xs = S.xs[c];
+    const uint32_t U = F.tu_size[tile];
+    const bool staged = U != PF_TU_GLOBAL;
     if (staged) {
-      Nl16Cursor cur(F.nl16, n, s, U);
-      // Whole groups of four need no per-entry test; the last partial group
-      // does. The boundary reaction has a walk of its own, so the common walk
-      // carries no per-neighbour branch.
-      auto group = [&](uint32_t k0, bool tail, bool with_reaction) {
-        uint32_t idx[4];
-        cur.next(k0, idx);
-        float4 xj[4];
-        float mj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          xj[u] = u_x[idx[u]];
-          mj[u] = u_m[idx[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const bool ok = !tail || k0 + u < cnt;
-          const float3 g = delta_term(F, ok, xn, li, xj[u], mj[u], sum);
-          if (with_reaction && ok) reaction_add(S, F, p, uslots[idx[u]], li, g);
-        }
-      };
-      const uint32_t full = cnt & ~3u;
-      if (react) {
-        for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
-        if (full < cnt) group(full, true, true);
-      } else {
-        for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
-        if (full < cnt) group(full, true, false);
+      uint32_t js[PF_TU_PER];
+      tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+      float4 xv[PF_TU_PER];
+      float mv[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        xv[i] = xs[js[i]];
+        mv[i] = S.mass[js[i]];
       }
-    } else {
-      NlCursor cur(F.nl, n, s, cnt);
-      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-        uint32_t jj[4];
-        cur.next(k0, jj);
-        float4 xj[4];
-        float mj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          xj[u] = xs[jj[u]];
-          mj[u] = neighbor_mass(S, F, p, jj[u]);
+      if (S.has_boundary) {
+  #pragma unroll
+        for (int i = 0; i < PF_TU_PER; ++i) mv[i] = neighbor_mass(S, F, p, js[i]);
+      }
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = threadIdx.x + i * PF_BLOCK;
+        if (u < U) {
+          u_x[u] = pf_f4(xv[i]);
+          u_m[u] = mv[i];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float3 g = delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
-          if (react && k0 + u < cnt) reaction_add(S, F, p, jj[u], li, g);
+      }
+      __syncthreads();
+    }
+    if (s >= n) return;
+    const float4 xi4 = xs[s];
+    float3 xn = pf_xyz(xi4);
+    const float w = S.w[p][s];
+    if (in_range(S.id[p][s], F.start, F.count)) {
+      const uint32_t cnt = F.nl_count[s];
+      const float li = xi4.w;
+      const bool react = F.nboundary != 0u && li != 0.0f;
+      const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
+      float3 sum = make_float3(0.f, 0.f, 0.f);
+      if (staged) {
+        Nl16Cursor cur(F.nl16, n, s, U);
+        // Whole groups of four need no per-entry test; the last partial group
+        // does. The boundary reaction has a walk of its own, so the common walk
+        // carries no per-neighbour branch.
+        auto group = [&](uint32_t k0, bool tail, bool with_reaction) {
+          uint32_t idx[4];
+          cur.next(k0, idx);
+          float4 xj[4];
+          float mj[4];
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = u_x[idx[u]];
+            mj[u] = u_m[idx[u]];
+          }
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool ok = !tail || k0 + u < cnt;
+            const float3 g = delta_term(F, ok, xn, li, xj[u], mj[u], sum);
+            if (with_reaction && ok) reaction_add(S, F, p, uslots[idx[u]], li, g);
+          }
+        };
+        const uint32_t full = cnt & ~3u;
+        if (react) {
+          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
+          if (full < cnt) group(full, true, true);
+        } else {
+          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
+          if (full < cnt) group(full, true, false);
         }
+      } else {
+        NlCursor cur(F.nl, n, s, cnt);
+        for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+          uint32_t jj[4];
+          cur.next(k0, jj);
+          float4 xj[4];
+          float mj[4];
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            xj[u] = xs[jj[u]];
+            mj[u] = neighbor_mass(S, F, p, jj[u]);
+          }
+  #pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float3 g = delta_term(F, k0 + u < cnt, xn, li, xj[u], mj[u], sum);
+            if (react && k0 + u < cnt) reaction_add(S, F, p, jj[u], li, g);
+          }
+        }
+      }
+      xn = make_float3(xn.x + (sum.x / F.rho0) * w, xn.y + (sum.y / F.rho0) * w,
+                       xn.z + (sum.z / F.rho0) * w);
+    }
+    if (WALLS || ADVECT) {
+      float4 x = S.x[p][s];
+      if (WALLS) {
+        int tk[3] = {0, 0, 0};
+        walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
+        sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
+        bool sat;
+        const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
+        if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
+        xn = pf_add3(xn, ch);
+      }
+      if (ADVECT && w > 0.0f) {
+        S.v[p][s] = make_float4((xn.x - x.x) / S.dt, (xn.y - x.y) / S.dt, (xn.z - x.z) / S.dt, 0.0f);
+        S.x[p][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
       }
     }
-    xn = make_float3(xn.x + (sum.x / F.rho0) * w, xn.y + (sum.y / F.rho0) * w,
-                     xn.z + (sum.z / F.rho0) * w);
-  }
-  if (WALLS || ADVECT) {
-    float4 x = S.x[p][s];
-    if (WALLS) {
-      int tk[3] = {0, 0, 0};
-      walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
-      sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
+    S.xs[c ^ 1][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+  });
+}
+
+// Apply and clear the boundary reaction over the span covering the fluid's
+// boundary ranges (accumulator.ts:101-131 over coveringRange, FluidSystem.ts:280-287).
+template <bool PERSIST>
+__global__ __launch_bounds__(PF_BLOCK) void k_reaction_apply(const SceneDev* __restrict__ scenes,
+                                                             const Tile* __restrict__ tiles,
+                                                             uint32_t ntiles, int fi, int p, int c) {
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n || F.nboundary == 0u) return;
+    const uint32_t id = S.id[p][s];
+    if (!(id >= F.rcover_lo && id < F.rcover_hi)) return;
+    int* acc = F.reaction + 3 * (size_t)s;
+    const int tk[3] = {acc[0], acc[1], acc[2]};
+    bool sat;
+    const float3 ch = ticks_to_change(tk, F.rea_inv, &sat);
+    if (sat) flag_set(S, 0, PF_FLAG_REACTION);
+    float4 x = S.xs[c][s];
+    S.xs[c][s] = make_float4(x.x + ch.x, x.y + ch.y, x.z + ch.z, x.w);
+    acc[0] = 0;
+    acc[1] = 0;
+    acc[2] = 0;
+  });
+}
+
+// Akinci boundary volume V_i = 1 / max(sum_k W(x_i - x_k), 1e-12) over the
+// boundary's own range (boundaryVolume.ts:26-52), for the slots of this
+// fluid's dynamic boundaries (predicted positions, every substep) or static
+// boundaries (committed positions, once). The walk is the reference's
+// 27-bucket query with bucket dedup (query.ts:24-70).
+template <bool PERSIST>
+__global__ __launch_bounds__(PF_BLOCK) void k_boundary_volume(const SceneDev* __restrict__ scenes,
+                                                              const Tile* __restrict__ tiles,
+                                                              uint32_t ntiles, int fi, int p, int c,
+                                                              int dynamic) {
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n || F.nboundary == 0u) return;
+    const uint32_t id = S.id[p][s];
+    int b = -1;
+    for (uint32_t k = 0; k < F.nboundary; ++k)
+      if (in_range(id, F.bstart[k], F.bcount[k]) && (int)((F.bdynamic >> k) & 1u) == dynamic) b = (int)k;
+    if (b < 0) return;
+    const uint32_t lo = F.bstart[b], cnt = F.bcount[b];
+    const float4* __restrict__ pos = dynamic ? S.xs[c] : S.x[p];
+    const float3 xi = pf_xyz(pos[s]);
+    const int qx = pf_query_coord(xi.x, S.ox, S.cell);
+    const int qy = pf_query_coord(xi.y, S.oy, S.cell);
+    const int qz = pf_query_coord(xi.z, S.oz, S.cell);
+    uint32_t visited[27];
+    uint32_t nvisited = 0;
+    float sum = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const uint32_t bk = pf_morton((int)((uint32_t)qx + (uint32_t)dx), (int)((uint32_t)qy + (uint32_t)dy),
+                                        (int)((uint32_t)qz + (uint32_t)dz)) & S.hash_mask;
+          bool seen = false;
+          for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == bk);
+          visited[nvisited++] = bk;
+          if (seen) continue;
+          for (uint32_t j = S.start[bk]; j < S.start[bk + 1]; ++j) {
+            if (!in_range(S.id[p][j], lo, cnt)) continue;
+            const float3 xj = pf_xyz(pos[j]);
+            const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+            const float r2 = pf_dot3(d, d);
+            if (r2 >= F.h2) continue;
+            sum = sum + pf_poly6(r2, F.h2, F.poly6);
+          }
+        }
+    S.bvol[p][s] = 1.0f / fmaxf(sum, 1e-12f);
+  });
+}
+
+// Container contact + position accumulator apply (+ advect), in place.
+template <bool ADVECT, bool PERSIST>
+__global__ __launch_bounds__(PF_BLOCK) void k_walls(const SceneDev* __restrict__ scenes,
+                                                    const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                    int p, int c, int first_iteration, int substep) {
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n) return;
+    const float4 xi4 = S.xs[c][s];
+    float3 xn = pf_xyz(xi4);
+    const float w = S.w[p][s];
+    const float4 x = S.x[p][s];
+    int tk[3] = {0, 0, 0};
+    if (S.ct.on) {  // the contact solve's sums share the accumulator (loop.ts:183-236)
+      const int4 a = S.ct.acc[s];
+      S.ct.acc[s] = make_int4(0, 0, 0, 0);
+      tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+    }
+    walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
+    sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
+    // A scene in the same launch with neither colliders nor contacts has no
+    // accumulator apply in its own schedule.
+    if (S.nprim || S.nsdf || S.ct.on) {
       bool sat;
       const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
       if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
@@ -1655,133 +1793,27 @@ __global__ __launch_bounds__(PF_BLOCK) void k_delta(const SceneDev* __restrict__
       S.v[p][s] = make_float4((xn.x - x.x) / S.dt, (xn.y - x.y) / S.dt, (xn.z - x.z) / S.dt, 0.0f);
       S.x[p][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
     }
-  }
-  S.xs[c ^ 1][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
-}
-
-// Apply and clear the boundary reaction over the span covering the fluid's
-// boundary ranges (accumulator.ts:101-131 over coveringRange, FluidSystem.ts:280-287).
-__global__ __launch_bounds__(PF_BLOCK) void k_reaction_apply(const SceneDev* __restrict__ scenes,
-                                                             const Tile* __restrict__ tiles,
-                                                             uint32_t ntiles, int fi, int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || F.nboundary == 0u) return;
-  const uint32_t id = S.id[p][s];
-  if (!(id >= F.rcover_lo && id < F.rcover_hi)) return;
-  int* acc = F.reaction + 3 * (size_t)s;
-  const int tk[3] = {acc[0], acc[1], acc[2]};
-  bool sat;
-  const float3 ch = ticks_to_change(tk, F.rea_inv, &sat);
-  if (sat) flag_set(S, 0, PF_FLAG_REACTION);
-  float4 x = S.xs[c][s];
-  S.xs[c][s] = make_float4(x.x + ch.x, x.y + ch.y, x.z + ch.z, x.w);
-  acc[0] = 0;
-  acc[1] = 0;
-  acc[2] = 0;
-}
-
-// Akinci boundary volume V_i = 1 / max(sum_k W(x_i - x_k), 1e-12) over the
-// boundary's own range (boundaryVolume.ts:26-52), for the slots of this
-// fluid's dynamic boundaries (predicted positions, every substep) or static
-// boundaries (committed positions, once). The walk is the reference's
-// 27-bucket query with bucket dedup (query.ts:24-70).
-__global__ __launch_bounds__(PF_BLOCK) void k_boundary_volume(const SceneDev* __restrict__ scenes,
-                                                              const Tile* __restrict__ tiles,
-                                                              uint32_t ntiles, int fi, int p, int c,
-                                                              int dynamic) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n || F.nboundary == 0u) return;
-  const uint32_t id = S.id[p][s];
-  int b = -1;
-  for (uint32_t k = 0; k < F.nboundary; ++k)
-    if (in_range(id, F.bstart[k], F.bcount[k]) && (int)((F.bdynamic >> k) & 1u) == dynamic) b = (int)k;
-  if (b < 0) return;
-  const uint32_t lo = F.bstart[b], cnt = F.bcount[b];
-  const float4* __restrict__ pos = dynamic ? S.xs[c] : S.x[p];
-  const float3 xi = pf_xyz(pos[s]);
-  const int qx = pf_query_coord(xi.x, S.ox, S.cell);
-  const int qy = pf_query_coord(xi.y, S.oy, S.cell);
-  const int qz = pf_query_coord(xi.z, S.oz, S.cell);
-  uint32_t visited[27];
-  uint32_t nvisited = 0;
-  float sum = 0.0f;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const uint32_t bk = pf_morton((int)((uint32_t)qx + (uint32_t)dx), (int)((uint32_t)qy + (uint32_t)dy),
-                                      (int)((uint32_t)qz + (uint32_t)dz)) & S.hash_mask;
-        bool seen = false;
-        for (uint32_t k = 0; k < nvisited; ++k) seen |= (visited[k] == bk);
-        visited[nvisited++] = bk;
-        if (seen) continue;
-        for (uint32_t j = S.start[bk]; j < S.start[bk + 1]; ++j) {
-          if (!in_range(S.id[p][j], lo, cnt)) continue;
-          const float3 xj = pf_xyz(pos[j]);
-          const float3 d = make_float3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
-          const float r2 = pf_dot3(d, d);
-          if (r2 >= F.h2) continue;
-          sum = sum + pf_poly6(r2, F.h2, F.poly6);
-        }
-      }
-  S.bvol[p][s] = 1.0f / fmaxf(sum, 1e-12f);
-}
-
-// Container contact + position accumulator apply (+ advect), in place.
-template <bool ADVECT>
-__global__ __launch_bounds__(PF_BLOCK) void k_walls(const SceneDev* __restrict__ scenes,
-                                                    const Tile* __restrict__ tiles, uint32_t ntiles,
-                                                    int p, int c, int first_iteration, int substep) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  const float4 xi4 = S.xs[c][s];
-  float3 xn = pf_xyz(xi4);
-  const float w = S.w[p][s];
-  const float4 x = S.x[p][s];
-  int tk[3] = {0, 0, 0};
-  if (S.ct.on) {  // the contact solve's sums share the accumulator (loop.ts:183-236)
-    const int4 a = S.ct.acc[s];
-    S.ct.acc[s] = make_int4(0, 0, 0, 0);
-    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
-  }
-  walls_solve(S, s, xn, pf_xyz(x), w, S.clock[substep], first_iteration != 0, tk);
-  sdf_solve(S, s, xn, pf_xyz(x), w, first_iteration != 0, tk);
-  // A scene in the same launch with neither colliders nor contacts has no
-  // accumulator apply in its own schedule.
-  if (S.nprim || S.nsdf || S.ct.on) {
-    bool sat;
-    const float3 ch = ticks_to_change(tk, S.pos_inv, &sat);
-    if (sat) flag_set(S, 0, PF_FLAG_POSITIONS);
-    xn = pf_add3(xn, ch);
-  }
-  if (ADVECT && w > 0.0f) {
-    S.v[p][s] = make_float4((xn.x - x.x) / S.dt, (xn.y - x.y) / S.dt, (xn.z - x.z) / S.dt, 0.0f);
-    S.x[p][s] = make_float4(xn.x, xn.y, xn.z, 0.0f);
-  }
-  S.xs[c][s] = make_float4(xn.x, xn.y, xn.z, xi4.w);
+    S.xs[c][s] = make_float4(xn.x, xn.y, xn.z, xi4.w);
+  });
 }
 
 // Advect only (integrate.ts:36-44), for scenes without colliders or fluid.
+template <bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_advect(const SceneDev* __restrict__ scenes,
                                                      const Tile* __restrict__ tiles, uint32_t ntiles,
                                                      int p, int c) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  const float w = S.w[p][s];
-  if (!(w > 0.0f)) return;
-  const float4 xsv = S.xs[c][s];
-  const float4 x = S.x[p][s];
-  S.v[p][s] = make_float4((xsv.x - x.x) / S.dt, (xsv.y - x.y) / S.dt, (xsv.z - x.z) / S.dt, 0.0f);
-  S.x[p][s] = make_float4(xsv.x, xsv.y, xsv.z, 0.0f);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n) return;
+    const float w = S.w[p][s];
+    if (!(w > 0.0f)) return;
+    const float4 xsv = S.xs[c][s];
+    const float4 x = S.x[p][s];
+    S.v[p][s] = make_float4((xsv.x - x.x) / S.dt, (xsv.y - x.y) / S.dt, (xsv.z - x.z) / S.dt, 0.0f);
+    S.x[p][s] = make_float4(xsv.x, xsv.y, xsv.z, 0.0f);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1804,7 +1836,7 @@ static __device__ __forceinline__ void velocity_term(const FluidDev& F, bool ok,
 // from the fluid's; the union then holds each entry's volume (32 bytes an
 // entry), else the volume is the fluid's own and an entry takes 24 bytes,
 // which fits three workgroups per CU instead of two.
-template <bool VORT, bool VISC, bool BND>
+template <bool VORT, bool VISC, bool BND, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restrict__ scenes,
                                                        const Tile* __restrict__ tiles,
                                                        uint32_t ntiles, int fi, int p, uint32_t cap) {
@@ -1812,111 +1844,113 @@ __global__ __launch_bounds__(PF_BLOCK) void k_velocity(const SceneDev* __restric
   float4* u_x = pf_lds;        // position; the neighbour volume (BND) or velocity x
   float4* u_v = pf_lds + cap;  // velocity (BND)
   float2* u_yz = reinterpret_cast<float2*>(pf_lds + cap);  // velocity y, z (not BND)
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t n = S.n;
-  const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
-  const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
-  const float4* __restrict__ X = S.x[p];
-  const float4* __restrict__ V = S.v[p];
-  const uint32_t U = F.tu_size[tile];
-  const bool staged = (vort || visc) && U != PF_TU_GLOBAL;
-  if (staged) {
-    uint32_t js[PF_TU_PER];
-    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
-    float4 xv[PF_TU_PER], vv[PF_TU_PER];
-    float ov[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      xv[i] = X[js[i]];
-      vv[i] = V[js[i]];
-      ov[i] = F.pv;
-    }
-    if (BND && S.has_boundary) {
-#pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) ov[i] = neighbor_volume(S, F, p, js[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = threadIdx.x + i * PF_BLOCK;
-      if (u < U) {
-        if (BND) {
-          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
-          u_v[u] = pf_f4(vv[i]);
-        } else {
-          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, vv[i].x);
-          u_yz[u] = make_float2(vv[i].y, vv[i].z);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t n = S.n;
+    const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
+    const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
+    const float4* __restrict__ X = S.x[p];
+    const float4* __restrict__ V = S.v[p];
+    const uint32_t U = F.tu_size[tile];
+    const bool staged = (vort || visc) && U != PF_TU_GLOBAL;
+    if (staged) {
+      uint32_t js[PF_TU_PER];
+      tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+      float4 xv[PF_TU_PER], vv[PF_TU_PER];
+      float ov[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        xv[i] = X[js[i]];
+        vv[i] = V[js[i]];
+        ov[i] = F.pv;
+      }
+      if (BND && S.has_boundary) {
+  #pragma unroll
+        for (int i = 0; i < PF_TU_PER; ++i) ov[i] = neighbor_volume(S, F, p, js[i]);
+      }
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = threadIdx.x + i * PF_BLOCK;
+        if (u < U) {
+          if (BND) {
+            u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
+            u_v[u] = pf_f4(vv[i]);
+          } else {
+            u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, vv[i].x);
+            u_yz[u] = make_float2(vv[i].y, vv[i].z);
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if (s >= n) return;
-  if (!in_range(S.id[p][s], F.start, F.count)) {
-    // |omega| of particles outside the fluid reads as zero in the gradient
-    // pass (the reference's per-fluid buffer is zero there); slots change
-    // particles every substep, so clear it.
-    if (vort) F.omega[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  if (!(vort || visc)) return;
-  const float3 xi = pf_xyz(X[s]);
-  const float3 vi = pf_xyz(V[s]);
-  const uint32_t cnt = F.nl_count[s];
-  float3 curl = make_float3(0.f, 0.f, 0.f);
-  float3 sm = make_float3(0.f, 0.f, 0.f);
-  if (staged) {
-    Nl16Cursor cur(F.nl16, n, s, U);
-    // Whole groups of four need no per-entry test; the last partial group does.
-    auto group = [&](uint32_t k0, bool tail) {
-      uint32_t idx[4];
-      cur.next(k0, idx);
-      float4 xj[4];
-      float3 vj[4];
-      float vol[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = u_x[idx[u]];
-        if (BND) {
-          const float4 v4 = u_v[idx[u]];
-          vj[u] = pf_xyz(v4);
-          vol[u] = xj[u].w;
-        } else {
-          const float2 yz = u_yz[idx[u]];
-          vj[u] = make_float3(xj[u].w, yz.x, yz.y);
-          vol[u] = F.pv;
+    if (s >= n) return;
+    if (!in_range(S.id[p][s], F.start, F.count)) {
+      // |omega| of particles outside the fluid reads as zero in the gradient
+      // pass (the reference's per-fluid buffer is zero there); slots change
+      // particles every substep, so clear it.
+      if (vort) F.omega[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      return;
+    }
+    if (!(vort || visc)) return;
+    const float3 xi = pf_xyz(X[s]);
+    const float3 vi = pf_xyz(V[s]);
+    const uint32_t cnt = F.nl_count[s];
+    float3 curl = make_float3(0.f, 0.f, 0.f);
+    float3 sm = make_float3(0.f, 0.f, 0.f);
+    if (staged) {
+      Nl16Cursor cur(F.nl16, n, s, U);
+      // Whole groups of four need no per-entry test; the last partial group does.
+      auto group = [&](uint32_t k0, bool tail) {
+        uint32_t idx[4];
+        cur.next(k0, idx);
+        float4 xj[4];
+        float3 vj[4];
+        float vol[4];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = u_x[idx[u]];
+          if (BND) {
+            const float4 v4 = u_v[idx[u]];
+            vj[u] = pf_xyz(v4);
+            vol[u] = xj[u].w;
+          } else {
+            const float2 yz = u_yz[idx[u]];
+            vj[u] = make_float3(xj[u].w, yz.x, yz.y);
+            vol[u] = F.pv;
+          }
         }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u)
+          velocity_term<VORT, VISC>(F, !tail || k0 + u < cnt, vort, visc, xi, vi, xj[u], vj[u], vol[u], curl, sm);
+      };
+      const uint32_t full = cnt & ~3u;
+      for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
+      if (full < cnt) group(full, true);
+    } else {
+      NlCursor cur(F.nl, n, s, cnt);
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t jj[4];
+        cur.next(k0, jj);
+        float4 xj[4], vj4[4];
+        float vol[4];
+  #pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xj[u] = X[jj[u]];
+          vj4[u] = V[jj[u]];
+          vol[u] = neighbor_volume(S, F, p, jj[u]);
+        }
+  #pragma unroll
+        for (int u = 0; u < 4; ++u)
+          velocity_term<VORT, VISC>(F, k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), vol[u], curl, sm);
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        velocity_term<VORT, VISC>(F, !tail || k0 + u < cnt, vort, visc, xi, vi, xj[u], vj[u], vol[u], curl, sm);
-    };
-    const uint32_t full = cnt & ~3u;
-    for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false);
-    if (full < cnt) group(full, true);
-  } else {
-    NlCursor cur(F.nl, n, s, cnt);
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-      uint32_t jj[4];
-      cur.next(k0, jj);
-      float4 xj[4], vj4[4];
-      float vol[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        xj[u] = X[jj[u]];
-        vj4[u] = V[jj[u]];
-        vol[u] = neighbor_volume(S, F, p, jj[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        velocity_term<VORT, VISC>(F, k0 + u < cnt, vort, visc, xi, vi, xj[u], pf_xyz(vj4[u]), vol[u], curl, sm);
     }
-  }
-  if (vort) F.omega[s] = make_float4(curl.x, curl.y, curl.z, pf_len3(curl));
-  if (visc) F.dv[s] = make_float4(sm.x * F.visc, sm.y * F.visc, sm.z * F.visc, 0.0f);
+    if (vort) F.omega[s] = make_float4(curl.x, curl.y, curl.z, pf_len3(curl));
+    if (visc) F.dv[s] = make_float4(sm.x * F.visc, sm.y * F.visc, sm.z * F.visc, 0.0f);
+  });
 }
 
 // One neighbour's term of the vorticity gradient (velocity.ts:80-88), added when ok.
@@ -1926,7 +1960,7 @@ static __device__ __forceinline__ void eta_term(const FluidDev& F, bool ok, floa
   eta = sel3(ok, pf_add3(eta, pf_mul3(d, pf_spiky_scale(d, F.h, F.spiky) * ((oj - oi) * F.pv))), eta);
 }
 
-template <bool VORT, bool VISC, bool FRICTION>
+template <bool VORT, bool VISC, bool FRICTION, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restrict__ scenes,
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
@@ -1934,109 +1968,141 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
   extern __shared__ float4 pf_lds[];
   float4* u_x = pf_lds;                                       // position, |omega|
   uint8_t* u_skip = reinterpret_cast<uint8_t*>(pf_lds + cap);  // boundary particle (left out of the gradient)
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const FluidDev& F = S.fluid[fi];
-  const uint32_t s = t.first + threadIdx.x;
-  const uint32_t tile = t.first / PF_BLOCK;
-  const uint32_t n = S.n;
-  const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
-  const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
-  const float4* __restrict__ X = S.x[p];
-  const uint32_t U = F.tu_size[tile];
-  const bool staged = vort && U != PF_TU_GLOBAL;
-  if (staged) {
-    uint32_t js[PF_TU_PER];
-    tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
-    float4 xv[PF_TU_PER];
-    float ov[PF_TU_PER];
-    uint8_t kv[PF_TU_PER];
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      xv[i] = X[js[i]];
-      ov[i] = F.omega[js[i]].w;
-      kv[i] = 0;
-    }
-    if (S.has_boundary) {
-#pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) kv[i] = S.bvol[p][js[i]] > 0.0f ? 1 : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < PF_TU_PER; ++i) {
-      const uint32_t u = threadIdx.x + i * PF_BLOCK;
-      if (u < U) {
-        u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
-        u_skip[u] = kv[i];
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const FluidDev& F = S.fluid[fi];
+    const uint32_t s = t.first + threadIdx.x;
+    const uint32_t tile = t.first / PF_BLOCK;
+    const uint32_t n = S.n;
+    const bool vort = VORT && (F.effects & PF_FX_VORTICITY);
+    const bool visc = VISC && (F.effects & PF_FX_VISCOSITY);
+    const float4* __restrict__ X = S.x[p];
+    const uint32_t U = F.tu_size[tile];
+    const bool staged = vort && U != PF_TU_GLOBAL;
+    if (staged) {
+      uint32_t js[PF_TU_PER];
+      tu_load_slots(F.tu_slots + (size_t)tile * PF_TU_MAX, U, js);
+      float4 xv[PF_TU_PER];
+      float ov[PF_TU_PER];
+      uint8_t kv[PF_TU_PER];
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        xv[i] = X[js[i]];
+        ov[i] = F.omega[js[i]].w;
+        kv[i] = 0;
       }
+      if (S.has_boundary) {
+  #pragma unroll
+        for (int i = 0; i < PF_TU_PER; ++i) kv[i] = S.bvol[p][js[i]] > 0.0f ? 1 : 0;
+      }
+  #pragma unroll
+      for (int i = 0; i < PF_TU_PER; ++i) {
+        const uint32_t u = threadIdx.x + i * PF_BLOCK;
+        if (u < U) {
+          u_x[u] = make_float4(xv[i].x, xv[i].y, xv[i].z, ov[i]);
+          u_skip[u] = kv[i];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if (s >= n) return;
-  float4 v = S.v[p][s];
-  if ((vort || visc) && in_range(S.id[p][s], F.start, F.count)) {
-    float3 change = make_float3(0.f, 0.f, 0.f);
-    if (vort) {
-      const float3 xi = pf_xyz(X[s]);
-      const float4 oi4 = F.omega[s];
-      const float oi = oi4.w;
-      const uint32_t cnt = F.nl_count[s];
-      float3 eta = make_float3(0.f, 0.f, 0.f);
-      if (staged) {
-        Nl16Cursor cur(F.nl16, n, s, U);
-        // Whole groups of four in a scene without boundary particles need no
-        // per-entry test; the last partial group, or skipped boundary
-        // neighbours, do.
-        auto group = [&](uint32_t k0, bool tail, bool skips) {
-          uint32_t idx[4];
-          cur.next(k0, idx);
-          float4 xj[4];
-          bool ok[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            xj[u] = u_x[idx[u]];
-            ok[u] = (!tail || k0 + u < cnt) && !(skips && u_skip[idx[u]]);
+    if (s >= n) return;
+    float4 v = S.v[p][s];
+    if ((vort || visc) && in_range(S.id[p][s], F.start, F.count)) {
+      float3 change = make_float3(0.f, 0.f, 0.f);
+      if (vort) {
+        const float3 xi = pf_xyz(X[s]);
+        const float4 oi4 = F.omega[s];
+        const float oi = oi4.w;
+        const uint32_t cnt = F.nl_count[s];
+        float3 eta = make_float3(0.f, 0.f, 0.f);
+        if (staged) {
+          Nl16Cursor cur(F.nl16, n, s, U);
+          // Whole groups of four in a scene without boundary particles need no
+          // per-entry test; the last partial group, or skipped boundary
+          // neighbours, do.
+          auto group = [&](uint32_t k0, bool tail, bool skips) {
+            uint32_t idx[4];
+            cur.next(k0, idx);
+            float4 xj[4];
+            bool ok[4];
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              xj[u] = u_x[idx[u]];
+              ok[u] = (!tail || k0 + u < cnt) && !(skips && u_skip[idx[u]]);
+            }
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], xj[u].w, eta);
+          };
+          const uint32_t full = cnt & ~3u;
+          if (S.has_boundary) {
+            for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
+          } else {
+            for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
           }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], xj[u].w, eta);
-        };
-        const uint32_t full = cnt & ~3u;
-        if (S.has_boundary) {
-          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, true);
+          if (full < cnt) group(full, true, S.has_boundary != 0);
         } else {
-          for (uint32_t k0 = 0; k0 < full; k0 += 4) group(k0, false, false);
-        }
-        if (full < cnt) group(full, true, S.has_boundary != 0);
-      } else {
-        NlCursor cur(F.nl, n, s, cnt);
-        for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
-          uint32_t jj[4];
-          cur.next(k0, jj);
-          float4 xj[4];
-          float oj[4];
-          bool ok[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            xj[u] = X[jj[u]];
-            oj[u] = F.omega[jj[u]].w;
-            ok[u] = (k0 + u < cnt) && !(S.has_boundary && S.bvol[p][jj[u]] > 0.0f);
+          NlCursor cur(F.nl, n, s, cnt);
+          for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+            uint32_t jj[4];
+            cur.next(k0, jj);
+            float4 xj[4];
+            float oj[4];
+            bool ok[4];
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              xj[u] = X[jj[u]];
+              oj[u] = F.omega[jj[u]].w;
+              ok[u] = (k0 + u < cnt) && !(S.has_boundary && S.bvol[p][jj[u]] > 0.0f);
+            }
+  #pragma unroll
+            for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], oj[u], eta);
           }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) eta_term(F, ok[u], xi, oi, xj[u], oj[u], eta);
+        }
+        const float el = pf_len3(eta);
+        if (el >= 1e-12f) {
+          const float3 force = pf_cross3(make_float3(eta.x / el, eta.y / el, eta.z / el), pf_xyz(oi4));
+          change = pf_add3(change, pf_mul3(pf_mul3(force, F.vort), S.dt));
         }
       }
-      const float el = pf_len3(eta);
-      if (el >= 1e-12f) {
-        const float3 force = pf_cross3(make_float3(eta.x / el, eta.y / el, eta.z / el), pf_xyz(oi4));
-        change = pf_add3(change, pf_mul3(pf_mul3(force, F.vort), S.dt));
-      }
+      if (visc) change = pf_add3(change, pf_xyz(F.dv[s]));
+      v.x = v.x + change.x;
+      v.y = v.y + change.y;
+      v.z = v.z + change.z;
     }
-    if (visc) change = pf_add3(change, pf_xyz(F.dv[s]));
-    v.x = v.x + change.x;
-    v.y = v.y + change.y;
-    v.z = v.z + change.z;
-  }
-  if (FRICTION && touch_valid) {
+    if (FRICTION && touch_valid) {
+      int tk[3] = {0, 0, 0};
+      walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
+      sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
+      bool sat;
+      const float3 ch = ticks_to_change(tk, S.vel_inv, &sat);
+      if (sat) flag_set(S, 0, PF_FLAG_VELOCITIES);
+      v.x = v.x + ch.x;
+      v.y = v.y + ch.y;
+      v.z = v.z + ch.z;
+    }
+    S.v[p][s] = v;
+  });
+}
+
+// Friction + velocity accumulator apply on its own (no fluid to fuse with).
+template <bool PERSIST>
+__global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restrict__ scenes,
+                                                       const Tile* __restrict__ tiles,
+                                                       uint32_t ntiles, int p, int c, int substep) {
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n) return;
+    if (!(S.nprim || S.nsdf || S.ct.on)) return;  // nothing to apply in this scene
+    float4 v = S.v[p][s];
     int tk[3] = {0, 0, 0};
+    if (S.ct.on) {  // the contacts' kinetic friction sums (loop.ts:226-230)
+      const int4 a = S.ct.acc[s];
+      S.ct.acc[s] = make_int4(0, 0, 0, 0);
+      tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
+    }
     walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
     sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
     bool sat;
@@ -2045,63 +2111,38 @@ __global__ __launch_bounds__(PF_BLOCK) void k_vorticity(const SceneDev* __restri
     v.x = v.x + ch.x;
     v.y = v.y + ch.y;
     v.z = v.z + ch.z;
-  }
-  S.v[p][s] = v;
-}
-
-// Friction + velocity accumulator apply on its own (no fluid to fuse with).
-__global__ __launch_bounds__(PF_BLOCK) void k_friction(const SceneDev* __restrict__ scenes,
-                                                       const Tile* __restrict__ tiles,
-                                                       uint32_t ntiles, int p, int c, int substep) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  if (!(S.nprim || S.nsdf || S.ct.on)) return;  // nothing to apply in this scene
-  float4 v = S.v[p][s];
-  int tk[3] = {0, 0, 0};
-  if (S.ct.on) {  // the contacts' kinetic friction sums (loop.ts:226-230)
-    const int4 a = S.ct.acc[s];
-    S.ct.acc[s] = make_int4(0, 0, 0, 0);
-    tk[0] = a.x; tk[1] = a.y; tk[2] = a.z;
-  }
-  walls_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], S.clock[substep], tk);
-  sdf_friction(S, s, pf_xyz(S.xs[c][s]), pf_xyz(v), S.w[p][s], tk);
-  bool sat;
-  const float3 ch = ticks_to_change(tk, S.vel_inv, &sat);
-  if (sat) flag_set(S, 0, PF_FLAG_VELOCITIES);
-  v.x = v.x + ch.x;
-  v.y = v.y + ch.y;
-  v.z = v.z + ch.z;
-  S.v[p][s] = v;
+    S.v[p][s] = v;
+  });
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-#define PF_GRID dim3(ntiles), dim3(PF_BLOCK), 0, st
-// Launch with `lds` bytes of dynamic LDS (the tile-union arrays).
-#define PF_GRID_LDS(lds) dim3(ntiles), dim3(PF_BLOCK), (lds), st
+// A tile kernel K (with its leading template arguments TARGS, parenthesised)
+// over the tiles, in the launch shape's form (launch_tiles).
+#define PF_UNPAREN(...) __VA_ARGS__
+#define PF_TILES(K, TARGS, lds, ...) \
+  launch_tiles(K<PF_UNPAREN TARGS false>, K<PF_UNPAREN TARGS true>, ntiles, (lds), st, __VA_ARGS__)
 
 void launch_neighbors(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                       int p, int c, bool normal, uint32_t cap, int snap) {
   if (!ntiles) return;
   // x, y, z of cap union entries; at least the leader test's 4 words per thread
   const size_t lds = std::max<size_t>((size_t)cap * 3 * sizeof(float), 4 * PF_BLOCK * sizeof(uint32_t));
-  if (normal) hipLaunchKernelGGL(k_neighbors<true>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
-  else hipLaunchKernelGGL(k_neighbors<false>, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap, snap);
+  if (normal) PF_TILES(k_neighbors, (true,), lds, sc, tl, ntiles, fi, p, c, cap, snap);
+  else PF_TILES(k_neighbors, (false,), lds, sc, tl, ntiles, fi, p, c, cap, snap);
 }
 
 void launch_impulse_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
                           int c) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_impulse_apply, PF_GRID, sc, tl, ntiles, fi, p, c);
+  PF_TILES(k_impulse_apply, (), 0, sc, tl, ntiles, fi, p, c);
 }
 
 void launch_impulses(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, int c, bool surface_tension, bool adhesion) {
   if (!ntiles) return;
-#define PF_IK(a, b) hipLaunchKernelGGL((k_impulses<a, b>), PF_GRID, sc, tl, ntiles, fi, p, c)
+#define PF_IK(a, b) PF_TILES(k_impulses, (a, b,), 0, sc, tl, ntiles, fi, p, c)
   if (surface_tension && adhesion) PF_IK(true, true);
   else if (surface_tension) PF_IK(true, false);
   else if (adhesion) PF_IK(false, true);
@@ -2113,14 +2154,14 @@ void launch_lambda(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t 
                    int p, int c, uint32_t cap) {
   if (!ntiles) return;
   const size_t lds = (size_t)cap * (sizeof(float4) + sizeof(float));
-  hipLaunchKernelGGL(k_lambda, PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, cap);
+  PF_TILES(k_lambda, (), lds, sc, tl, ntiles, fi, p, c, cap);
 }
 
 void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                   int p, int c, bool walls, bool advect, int first_iteration, int substep, uint32_t cap) {
   if (!ntiles) return;
   const size_t lds = (size_t)cap * (sizeof(float4) + sizeof(float));
-#define PF_DK(w, a) hipLaunchKernelGGL((k_delta<w, a>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, first_iteration, substep, cap)
+#define PF_DK(w, a) PF_TILES(k_delta, (w, a,), lds, sc, tl, ntiles, fi, p, c, first_iteration, substep, cap)
   if (walls && advect) PF_DK(true, true);
   else if (walls) PF_DK(true, false);
   else if (advect) PF_DK(false, true);
@@ -2131,32 +2172,32 @@ void launch_delta(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t n
 void launch_walls(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c,
                   bool advect, int first_iteration, int substep) {
   if (!ntiles) return;
-  if (advect) hipLaunchKernelGGL(k_walls<true>, PF_GRID, sc, tl, ntiles, p, c, first_iteration, substep);
-  else hipLaunchKernelGGL(k_walls<false>, PF_GRID, sc, tl, ntiles, p, c, first_iteration, substep);
+  if (advect) PF_TILES(k_walls, (true,), 0, sc, tl, ntiles, p, c, first_iteration, substep);
+  else PF_TILES(k_walls, (false,), 0, sc, tl, ntiles, p, c, first_iteration, substep);
 }
 
 void launch_reaction_apply(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
                            int c) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_reaction_apply, PF_GRID, sc, tl, ntiles, fi, p, c);
+  PF_TILES(k_reaction_apply, (), 0, sc, tl, ntiles, fi, p, c);
 }
 
 void launch_boundary_volume(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi, int p,
                             int c, int dynamic) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_boundary_volume, PF_GRID, sc, tl, ntiles, fi, p, c, dynamic);
+  PF_TILES(k_boundary_volume, (), 0, sc, tl, ntiles, fi, p, c, dynamic);
 }
 
 void launch_advect(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p, int c) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_advect, PF_GRID, sc, tl, ntiles, p, c);
+  PF_TILES(k_advect, (), 0, sc, tl, ntiles, p, c);
 }
 
 void launch_velocity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                      int p, bool vort, bool visc, bool boundary, uint32_t cap) {
   if (!ntiles || !(vort || visc)) return;
   const size_t lds = (size_t)cap * (boundary ? 2 * sizeof(float4) : sizeof(float4) + sizeof(float2));
-#define PF_VK(a, b, c) hipLaunchKernelGGL((k_velocity<a, b, c>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, cap)
+#define PF_VK(a, b, c) PF_TILES(k_velocity, (a, b, c,), lds, sc, tl, ntiles, fi, p, cap)
   if (boundary) {
     if (vort && visc) PF_VK(true, true, true);
     else if (vort) PF_VK(true, false, true);
@@ -2174,7 +2215,7 @@ void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
                       int touch_valid, uint32_t cap) {
   if (!ntiles) return;
   const size_t lds = (size_t)cap * sizeof(float4) + ((cap + 15u) & ~15u);
-#define PF_VK(a, b, f) hipLaunchKernelGGL((k_vorticity<a, b, f>), PF_GRID_LDS(lds), sc, tl, ntiles, fi, p, c, substep, touch_valid, cap)
+#define PF_VK(a, b, f) PF_TILES(k_vorticity, (a, b, f,), lds, sc, tl, ntiles, fi, p, c, substep, touch_valid, cap)
   if (friction) {
     if (vort && visc) PF_VK(true, true, true);
     else if (vort) PF_VK(true, false, true);
@@ -2191,15 +2232,15 @@ void launch_vorticity(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32
 void launch_viscosity_solve(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int fi,
                             int p, uint32_t sweeps) {
   if (!ntiles || !sweeps) return;
-  hipLaunchKernelGGL(k_isv_seed, PF_GRID, sc, tl, ntiles, fi, p);
-  for (uint32_t k = 0; k < sweeps; ++k) hipLaunchKernelGGL(k_isv_sweep, PF_GRID, sc, tl, ntiles, fi, p, k);
-  hipLaunchKernelGGL(k_isv_apply, PF_GRID, sc, tl, ntiles, fi, p, sweeps);
+  PF_TILES(k_isv_seed, (), 0, sc, tl, ntiles, fi, p);
+  for (uint32_t k = 0; k < sweeps; ++k) PF_TILES(k_isv_sweep, (), 0, sc, tl, ntiles, fi, p, k);
+  PF_TILES(k_isv_apply, (), 0, sc, tl, ntiles, fi, p, sweeps);
 }
 
 void launch_friction(hipStream_t st, const SceneDev* sc, const Tile* tl, uint32_t ntiles, int p,
                      int c, int substep) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_friction, PF_GRID, sc, tl, ntiles, p, c, substep);
+  PF_TILES(k_friction, (), 0, sc, tl, ntiles, p, c, substep);
 }
 
 }  // namespace pf

@@ -53,57 +53,59 @@ static __device__ __forceinline__ void predict(const SceneDev& S, float4 x, floa
 
 // MODE 0: standalone grid over x[0] positions (ids = slots).
 // MODE 1: scene substep; predicts first, reads state buffers of `parity`.
-template <int MODE>
+template <int MODE, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_hash(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, uint32_t ntiles,
                                                    int parity, int xs_prev, int reset_step_flags) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t i = t.first + threadIdx.x;
-  const bool valid = i < S.n;
-  uint32_t over = 0;
-  uint32_t b = 0xffffffffu;
-  if (MODE == 1 && reset_step_flags && t.first == 0 && threadIdx.x == 0) {
-    // Step start: position/velocity saturation flags cover a whole step (loop.ts:206-210).
-    __hip_atomic_store(flag_word(S, 0, PF_FLAG_POSITIONS), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(flag_word(S, 0, PF_FLAG_VELOCITIES), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (MODE == 1 && S.ct.on && t.first == 0 && threadIdx.x == 0) {
-    // Substep start: the contact counter (resetCounterKernel, ContactBuffer.ts:62-65).
-    __hip_atomic_store(S.ct.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (valid) {
-    float4 p;
-    if (MODE == 0) {
-      p = S.x[0][i];
-    } else {
-      const float w = S.w[parity][i];
-      float4 vel;
-      const float4 xsp = (w > 0.0f) ? make_float4(0.f, 0.f, 0.f, 0.f) : S.xs[xs_prev][i];
-      predict(S, S.x[parity][i], S.v[parity][i], w, xsp, &vel, &p);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    const bool valid = i < S.n;
+    uint32_t over = 0;
+    uint32_t b = 0xffffffffu;
+    if (MODE == 1 && reset_step_flags && t.first == 0 && threadIdx.x == 0) {
+      // Step start: position/velocity saturation flags cover a whole step (loop.ts:206-210).
+      __hip_atomic_store(flag_word(S, 0, PF_FLAG_POSITIONS), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(flag_word(S, 0, PF_FLAG_VELOCITIES), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    b = bucket_of(S, p.x, p.y, p.z, &over);
-    S.key[i] = b;
-  }
-  // Wave-level aggregation: runs of equal keys in adjacent lanes (the common
-  // case once particles are cell-sorted) issue one atomic per run.
-  const uint32_t lane = __lane_id();
-  const uint32_t prevb = __shfl_up(b, 1);
-  const bool head = valid && (lane == 0 || prevb != b);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long validm = __ballot(valid);
-  const uint32_t nvalid = __popcll(validm);
-  const unsigned long long below = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const uint32_t head_lane = valid ? (63u - __clzll(heads & below)) : 0u;
-  uint32_t base = 0;
-  if (head) {
-    const unsigned long long above = heads & ~below;
-    const uint32_t run_end = above ? (uint32_t)__ffsll((long long)above) - 1u : nvalid;
-    base = atomicAdd(&S.counts[b], run_end - lane);
-  }
-  const uint32_t hb = __shfl(base, (int)head_lane);
-  if (valid) S.rank[i] = hb + (lane - head_lane);
-  if (over) pf_flag_raise(flag_word(S, MODE == 1 ? parity : 0, PF_FLAG_GRID));
+    if (MODE == 1 && S.ct.on && t.first == 0 && threadIdx.x == 0) {
+      // Substep start: the contact counter (resetCounterKernel, ContactBuffer.ts:62-65).
+      __hip_atomic_store(S.ct.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (valid) {
+      float4 p;
+      if (MODE == 0) {
+        p = S.x[0][i];
+      } else {
+        const float w = S.w[parity][i];
+        float4 vel;
+        const float4 xsp = (w > 0.0f) ? make_float4(0.f, 0.f, 0.f, 0.f) : S.xs[xs_prev][i];
+        predict(S, S.x[parity][i], S.v[parity][i], w, xsp, &vel, &p);
+      }
+      b = bucket_of(S, p.x, p.y, p.z, &over);
+      S.key[i] = b;
+    }
+    // Wave-level aggregation: runs of equal keys in adjacent lanes (the common
+    // case once particles are cell-sorted) issue one atomic per run.
+    const uint32_t lane = __lane_id();
+    const uint32_t prevb = __shfl_up(b, 1);
+    const bool head = valid && (lane == 0 || prevb != b);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long validm = __ballot(valid);
+    const uint32_t nvalid = __popcll(validm);
+    const unsigned long long below = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t head_lane = valid ? (63u - __clzll(heads & below)) : 0u;
+    uint32_t base = 0;
+    if (head) {
+      const unsigned long long above = heads & ~below;
+      const uint32_t run_end = above ? (uint32_t)__ffsll((long long)above) - 1u : nvalid;
+      base = atomicAdd(&S.counts[b], run_end - lane);
+    }
+    const uint32_t hb = __shfl(base, (int)head_lane);
+    if (valid) S.rank[i] = hb + (lane - head_lane);
+    if (over) pf_flag_raise(flag_word(S, MODE == 1 ? parity : 0, PF_FLAG_GRID));
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -262,83 +264,87 @@ __global__ __launch_bounds__(PF_BLOCK) void k_scan(const SceneDev* __restrict__ 
 }
 
 // Bucket position from start + arrival rank (the reference's atomic scatter).
-template <int MODE>
+template <int MODE, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_place(const SceneDev* __restrict__ scenes,
                                                     const Tile* __restrict__ tiles, uint32_t ntiles,
                                                     int parity) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t i = t.first + threadIdx.x;
-  if (i >= S.n) return;
-  const uint32_t b = S.key[i];
-  const uint32_t slot = S.start[b] + S.rank[i];
-  S.tmp_src[slot] = i;
-  S.tmp_key[slot] = b;
-  if (MODE == 1) S.tmp_id[slot] = S.id[parity][i];
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t i = t.first + threadIdx.x;
+    if (i >= S.n) return;
+    const uint32_t b = S.key[i];
+    const uint32_t slot = S.start[b] + S.rank[i];
+    S.tmp_src[slot] = i;
+    S.tmp_key[slot] = b;
+    if (MODE == 1) S.tmp_id[slot] = S.id[parity][i];
+  });
 }
 
 // Stable rank inside the bucket, then gather into cell-sorted order.
 // MODE 0 writes the permutation / sorted keys (/ sorted positions).
 // MODE 1 gathers the particle state of `parity` into `parity ^ 1`, predicting
 // velocities and positions (integrate.ts:25-34) on the way.
-template <int MODE>
+template <int MODE, bool PERSIST>
 __global__ __launch_bounds__(PF_BLOCK) void k_rank(const SceneDev* __restrict__ scenes,
                                                    const Tile* __restrict__ tiles, uint32_t ntiles,
                                                    int parity, int xs_prev, int xs_out,
                                                    uint32_t* __restrict__ out_perm,
                                                    uint32_t* __restrict__ out_skey,
                                                    float4* __restrict__ out_spos) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  if (MODE == 1 && t.first == 0 && threadIdx.x == 0) {
-    // Per-rebuild flags for the next substep start cleared (HashGrid.ts:195-204,
-    // neighborList.ts:47): the next substep writes the other parity's words.
-    uint32_t* nf = S.flags + (parity ^ 1) * PF_NUM_FLAGS;
-    nf[PF_FLAG_GRID] = 0u;
-    nf[PF_FLAG_NEIGHBORS] = 0u;
-    nf[PF_FLAG_REACTION] = 0u;
-    nf[PF_FLAG_LONG_BUCKET] = 0u;
-  }
-  const uint32_t s = t.first + threadIdx.x;
-  if (s >= S.n) return;
-  const uint32_t b = S.tmp_key[s];
-  const uint32_t st = S.start[b];
-  const uint32_t en = S.start[b + 1];
-  const uint32_t* cmp = (MODE == 0) ? S.tmp_src : S.tmp_id;
-  const uint32_t my = cmp[s];
-  const uint32_t src = S.tmp_src[s];
-  uint32_t dest;
-  if (en - st <= (uint32_t)PF_STABLE_SEGMENT_LIMIT) {
-    uint32_t r = 0;
-    for (uint32_t q = st; q < en; ++q) r += (cmp[q] < my) ? 1u : 0u;
-    dest = st + r;
-  } else {
-    dest = s;
-    pf_flag_raise(S.flags + (MODE == 1 ? parity : 0) * PF_NUM_FLAGS + PF_FLAG_LONG_BUCKET);
-  }
-  if (MODE == 0) {
-    out_perm[dest] = src;
-    out_skey[s] = b;
-    if (out_spos) out_spos[dest] = S.x[0][src];
-  } else {
-    const int q = parity ^ 1;
-    const float4 x = S.x[parity][src];
-    const float4 v = S.v[parity][src];
-    const float w = S.w[parity][src];
-    const float4 xsp = (w > 0.0f) ? make_float4(0.f, 0.f, 0.f, 0.f) : S.xs[xs_prev][src];
-    float4 vel, xs;
-    predict(S, x, v, w, xsp, &vel, &xs);
-    S.x[q][dest] = x;
-    S.v[q][dest] = vel;
-    S.w[q][dest] = w;
-    S.mass[dest] = (w > 0.0f) ? (1.0f / w) : 0.0f;  // shared.ts:59
-    S.id[q][dest] = my;
-    S.slot_of[my] = dest;
-    S.rho[q][dest] = S.rho[parity][src];
-    S.bvol[q][dest] = S.bvol[parity][src];
-    S.grp[q][dest] = S.grp[parity][src];
-    S.xs[xs_out][dest] = xs;
-  }
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    if (MODE == 1 && t.first == 0 && threadIdx.x == 0) {
+      // Per-rebuild flags for the next substep start cleared (HashGrid.ts:195-204,
+      // neighborList.ts:47): the next substep writes the other parity's words.
+      uint32_t* nf = S.flags + (parity ^ 1) * PF_NUM_FLAGS;
+      nf[PF_FLAG_GRID] = 0u;
+      nf[PF_FLAG_NEIGHBORS] = 0u;
+      nf[PF_FLAG_REACTION] = 0u;
+      nf[PF_FLAG_LONG_BUCKET] = 0u;
+    }
+    const uint32_t s = t.first + threadIdx.x;
+    if (s >= S.n) return;
+    const uint32_t b = S.tmp_key[s];
+    const uint32_t st = S.start[b];
+    const uint32_t en = S.start[b + 1];
+    const uint32_t* cmp = (MODE == 0) ? S.tmp_src : S.tmp_id;
+    const uint32_t my = cmp[s];
+    const uint32_t src = S.tmp_src[s];
+    uint32_t dest;
+    if (en - st <= (uint32_t)PF_STABLE_SEGMENT_LIMIT) {
+      uint32_t r = 0;
+      for (uint32_t q = st; q < en; ++q) r += (cmp[q] < my) ? 1u : 0u;
+      dest = st + r;
+    } else {
+      dest = s;
+      pf_flag_raise(S.flags + (MODE == 1 ? parity : 0) * PF_NUM_FLAGS + PF_FLAG_LONG_BUCKET);
+    }
+    if (MODE == 0) {
+      out_perm[dest] = src;
+      out_skey[s] = b;
+      if (out_spos) out_spos[dest] = S.x[0][src];
+    } else {
+      const int q = parity ^ 1;
+      const float4 x = S.x[parity][src];
+      const float4 v = S.v[parity][src];
+      const float w = S.w[parity][src];
+      const float4 xsp = (w > 0.0f) ? make_float4(0.f, 0.f, 0.f, 0.f) : S.xs[xs_prev][src];
+      float4 vel, xs;
+      predict(S, x, v, w, xsp, &vel, &xs);
+      S.x[q][dest] = x;
+      S.v[q][dest] = vel;
+      S.w[q][dest] = w;
+      S.mass[dest] = (w > 0.0f) ? (1.0f / w) : 0.0f;  // shared.ts:59
+      S.id[q][dest] = my;
+      S.slot_of[my] = dest;
+      S.rho[q][dest] = S.rho[parity][src];
+      S.bvol[q][dest] = S.bvol[parity][src];
+      S.grp[q][dest] = S.grp[parity][src];
+      S.xs[xs_out][dest] = xs;
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -348,11 +354,11 @@ void launch_hash(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint
                  int mode, int parity, int xs_prev, int reset_step_flags) {
   if (ntiles == 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_hash<0>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity,
-                       xs_prev, reset_step_flags);
+    launch_tiles(k_hash<0, false>, k_hash<0, true>, ntiles, 0, st, scenes, tiles, ntiles, parity,
+                 xs_prev, reset_step_flags);
   else
-    hipLaunchKernelGGL(k_hash<1>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity,
-                       xs_prev, reset_step_flags);
+    launch_tiles(k_hash<1, false>, k_hash<1, true>, ntiles, 0, st, scenes, tiles, ntiles, parity,
+                 xs_prev, reset_step_flags);
 }
 
 void launch_scan(hipStream_t st, const SceneDev* scenes, const uint2* stiles, uint32_t nstiles,
@@ -366,9 +372,9 @@ void launch_place(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uin
                   int mode, int parity) {
   if (ntiles == 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_place<0>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity);
+    launch_tiles(k_place<0, false>, k_place<0, true>, ntiles, 0, st, scenes, tiles, ntiles, parity);
   else
-    hipLaunchKernelGGL(k_place<1>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity);
+    launch_tiles(k_place<1, false>, k_place<1, true>, ntiles, 0, st, scenes, tiles, ntiles, parity);
 }
 
 void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
@@ -376,11 +382,11 @@ void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint
                  uint32_t* out_skey, float4* out_spos) {
   if (ntiles == 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_rank<0>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity,
-                       xs_prev, xs_out, out_perm, out_skey, out_spos);
+    launch_tiles(k_rank<0, false>, k_rank<0, true>, ntiles, 0, st, scenes, tiles, ntiles, parity,
+                 xs_prev, xs_out, out_perm, out_skey, out_spos);
   else
-    hipLaunchKernelGGL(k_rank<1>, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, parity,
-                       xs_prev, xs_out, out_perm, out_skey, out_spos);
+    launch_tiles(k_rank<1, false>, k_rank<1, true>, ntiles, 0, st, scenes, tiles, ntiles, parity,
+                 xs_prev, xs_out, out_perm, out_skey, out_spos);
 }
 
 }  // namespace pf

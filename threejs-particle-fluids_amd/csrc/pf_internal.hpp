@@ -434,3 +434,26 @@ __device__ __forceinline__ uint32_t pf_xcd_remap(uint32_t bid, uint32_t nblocks)
   if (bid >= full) return bid;              // remainder keeps identity
   return (bid % nxcd) * per + bid / nxcd;
 }
+
+// The tiles of this workgroup, each passed to body as its index in the tile
+// list (XCD-remapped). Every tile kernel is instantiated twice. PERSIST false
+// is the launch of one workgroup per tile: body runs once. PERSIST true is the
+// persistent form (fewer workgroups than tiles, a multiple of the 8 XCDs): the
+// workgroup walks the list grid-stride, b, b + G, b + 2G, ..., tiles that keep
+// b's XCD under the remap. Separate instantiations keep the loop's live
+// ranges out of the one-tile form's register allocation. body returns to end
+// its tile; a barrier between tiles keeps one tile's LDS staging from
+// overwriting what a slower wave still reads, so no body may place a barrier
+// after a return that a whole wave can take (every body here returns per
+// lane, or block-uniformly).
+template <bool PERSIST, class Body>
+__device__ __forceinline__ void pf_for_tiles(uint32_t ntiles, Body&& body) {
+  if constexpr (!PERSIST) {
+    body(pf_xcd_remap(blockIdx.x, ntiles));
+  } else {
+    for (uint32_t vb = blockIdx.x; vb < ntiles; vb += gridDim.x) {
+      body(pf_xcd_remap(vb, ntiles));
+      if (vb + gridDim.x < ntiles) __syncthreads();
+    }
+  }
+}

@@ -1,8 +1,47 @@
 // Host-side launchers of the device kernels (one per kernel family).
 #pragma once
+#include <tuple>
+
 #include "pf_internal.hpp"
 
 namespace pf {
+
+// Launch shape of the tile kernels (those that take their tiles through
+// pf_for_tiles). 0 launches one workgroup per tile; a positive cap (a multiple
+// of the 8 XCDs) launches at most that many workgroups, each walking the tile
+// list grid-stride: the persistent form. enqueue_step sets it from its plan
+// for the launches it makes on the calling thread; other callers get 0.
+extern thread_local uint32_t tile_grid_cap;
+inline uint32_t tile_blocks(uint32_t ntiles) {
+  return (tile_grid_cap != 0u && tile_grid_cap < ntiles) ? tile_grid_cap : ntiles;
+}
+struct TileGridScope {
+  uint32_t saved;
+  explicit TileGridScope(uint32_t cap) : saved(tile_grid_cap) { tile_grid_cap = cap; }
+  ~TileGridScope() { tile_grid_cap = saved; }
+};
+
+// Launches a tile kernel over ntiles tiles with `lds` bytes of dynamic LDS:
+// `one` (its PERSIST = false instantiation) with a workgroup per tile, or
+// `many` (PERSIST = true) with tile_blocks(ntiles) workgroups when the launch
+// shape caps the grid below the tile count. Launch errors surface through
+// hipGetLastError like a triple-chevron launch's.
+template <typename... P, typename... A>
+void launch_tiles(void (*one)(P...), void (*many)(P...), uint32_t ntiles, size_t lds, hipStream_t st,
+                  A&&... args) {
+  static_assert(sizeof...(P) == sizeof...(A), "launch_tiles: argument count differs from the kernel's");
+  if (ntiles == 0) return;
+  const uint32_t blocks = tile_blocks(ntiles);
+  std::tuple<P...> vals(static_cast<P>(args)...);
+  std::apply(
+      [&](auto&... v) {
+        void* argv[] = {static_cast<void*>(&v)...};
+        (void)hipLaunchKernel(blocks < ntiles ? reinterpret_cast<const void*>(many)
+                                              : reinterpret_cast<const void*>(one),
+                              dim3(blocks), dim3(PF_BLOCK), argv, lds, st);
+      },
+      vals);
+}
 
 // grid.hip
 void launch_hash(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,

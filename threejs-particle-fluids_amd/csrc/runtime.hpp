@@ -38,6 +38,15 @@ struct PfError {
 // Throw an argument error with the reference's "<Owner>: <reason>" wording.
 [[noreturn]] void fail(int code, const char* fmt, ...);
 
+// A launch shape for the tile kernels: 0 (one workgroup per tile) or a
+// persistent cap, a multiple of the 8 XCDs so each workgroup keeps its XCD's
+// tiles, up to 2^20 workgroups.
+inline void check_launch_shape(uint32_t workgroups, const char* owner) {
+  if (workgroups % 8u != 0u || workgroups > (1u << 20))
+    fail(PF_EINVAL, "%s: launch shape must be 0 (one workgroup per tile) or a multiple of 8 up to %u workgroups, "
+         "got %u", owner, 1u << 20, workgroups);
+}
+
 // Device memory of one evictable scene. Each chunk is a range of reserved
 // device addresses backed by one physical allocation; eviction copies the
 // chunk's used bytes to pinned host memory and releases the physical memory,
@@ -295,6 +304,7 @@ struct PlanShape {
   bool soft = false;        // any scene has soft bodies
   bool grid = false;        // the scenes sort by hash bucket every substep (fluids or contacts)
   bool contact = false;     // any scene has particle contacts
+  uint32_t grid_cap = 0;    // tile kernels' launch shape: 0 one workgroup per tile, else the persistent cap
 };
 
 struct pf_loop_s {
@@ -319,6 +329,7 @@ struct pf_loop_s {
   Tile* ctiles = nullptr;    // (scene, first contact) tiles of the per-contact passes
   uint32_t nctiles = 0;
   bool contact = false;      // SimLoop contact option (loop.ts:23-49)
+  uint32_t grid_cap = 0;     // launch shape of the tile kernels (pf_loop_set_launch_shape)
   bool static_boundaries = false;  // a fluid has a static boundary (volumes computed on the first step)
   double last_dt = -1;
   uint32_t last_flag_half = 0;
@@ -346,7 +357,7 @@ struct pf_batch_s {
   Tile* ctiles = nullptr;
   uint32_t nctiles = 0;
   void* scan_ctl = nullptr;
-  PlanShape shape;
+  PlanShape shape;  // shape.grid_cap: launch shape of the tile kernels (pf_batch_set_launch_shape)
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
   bool graph_valid = false;

@@ -60,6 +60,7 @@ struct pf_server_s {
   Context* ctx = nullptr;
   pf_server_options opt{};
   uint32_t substeps = 0, iterations = 0;
+  uint32_t grid_cap = 0;  // launch shape of the tile kernels (pf_server_set_launch_shape)
   int has_fluid = -1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -258,6 +259,7 @@ struct pf_server_s {
     PlanShape sh;
     sh.substeps = substeps;
     sh.iterations = iterations;
+    sh.grid_cap = grid_cap;
     std::vector<SceneDev> descs;
     std::vector<Tile> tl, ctl;
     std::vector<uint2> stl, btl, ltl;
@@ -577,6 +579,15 @@ int pf_server_stats_get(pf_server* S, pf_server_stats* out) {
 }
 
 double pf_server_now(pf_server* S) { return S ? S->now() : 0.0; }
+
+int pf_server_set_launch_shape(pf_server* S, uint32_t workgroups) {
+  PF_API_BEGIN
+  if (!S) fail(PF_EINVAL, "Server: null handle");
+  check_launch_shape(workgroups, "Server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->grid_cap = workgroups;  // launches formed from now on
+  PF_API_END
+}
 
 int pf_server_evict_scene(pf_server* S, uint32_t id) {
   PF_API_BEGIN

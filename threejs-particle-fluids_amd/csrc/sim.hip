@@ -7,6 +7,8 @@
 #include "runtime.hpp"
 
 namespace pf {
+thread_local uint32_t tile_grid_cap = 0;
+
 void particles_scatter(pf_particles* ps, uint32_t start, uint32_t count, const std::vector<float4>* pos,
                        const std::vector<float4>* vel, const std::vector<float>* f,
                        const std::vector<uint32_t>* grp, uint32_t mask);
@@ -45,26 +47,29 @@ static uint16_t half_from_float(float v) {
 
 // Predict without a grid (integrate.ts:25-34), for scenes with no neighbour
 // queries: the state keeps its order and the prediction goes to xs[c ^ 1].
+template <bool PERSIST>
 __global__ void k_predict(const SceneDev* __restrict__ scenes, const Tile* __restrict__ tiles,
                           uint32_t ntiles, int p, int c, int reset_step_flags) {
-  const Tile t = tiles[pf_xcd_remap(blockIdx.x, ntiles)];
-  const SceneDev& S = scenes[t.scene];
-  const uint32_t s = t.first + threadIdx.x;
-  if (reset_step_flags && t.first == 0 && threadIdx.x == 0) {
-    S.flags[PF_FLAG_POSITIONS] = 0u;
-    S.flags[PF_FLAG_VELOCITIES] = 0u;
-  }
-  if (s >= S.n) return;
-  const float w = S.w[p][s];
-  float4 xs = S.xs[c][s];
-  if (w > 0.0f) {
-    const float4 v = S.v[p][s];
-    const float vx = v.x + S.gx * S.dt, vy = v.y + S.gy * S.dt, vz = v.z + S.gz * S.dt;
-    S.v[p][s] = make_float4(vx, vy, vz, 0.0f);
-    const float4 x = S.x[p][s];
-    xs = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
-  }
-  S.xs[c ^ 1][s] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+  pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
+    const Tile t = tiles[ti];
+    const SceneDev& S = scenes[t.scene];
+    const uint32_t s = t.first + threadIdx.x;
+    if (reset_step_flags && t.first == 0 && threadIdx.x == 0) {
+      S.flags[PF_FLAG_POSITIONS] = 0u;
+      S.flags[PF_FLAG_VELOCITIES] = 0u;
+    }
+    if (s >= S.n) return;
+    const float w = S.w[p][s];
+    float4 xs = S.xs[c][s];
+    if (w > 0.0f) {
+      const float4 v = S.v[p][s];
+      const float vx = v.x + S.gx * S.dt, vy = v.y + S.gy * S.dt, vz = v.z + S.gz * S.dt;
+      S.v[p][s] = make_float4(vx, vy, vz, 0.0f);
+      const float4 x = S.x[p][s];
+      xs = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
+    }
+    S.xs[c ^ 1][s] = make_float4(xs.x, xs.y, xs.z, 0.0f);
+  });
 }
 
 // The substep schedule (loop.ts:175-241, FluidSystem.ts:200-309), fused as
@@ -76,6 +81,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
                       uint32_t nbtiles, const uint2* ltiles, uint32_t nltiles, const Tile* ctiles,
                       uint32_t nctiles, void* scan_ctl) {
   hipStream_t st = ctx->stream;
+  const TileGridScope launch_shape(sh.grid_cap);
   uint32_t nk = 0;
   int c = 0;
   const bool grid = sh.grid;
@@ -117,7 +123,7 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
       }
     } else {
       PF_LAUNCH(ctx, "predict");
-      hipLaunchKernelGGL(k_predict, dim3(ntiles), dim3(PF_BLOCK), 0, st, scenes, tiles, ntiles, 0, c, k == 0);
+      launch_tiles(k_predict<false>, k_predict<true>, ntiles, 0, st, scenes, tiles, ntiles, 0, c, k == 0);
       ++nk;
       c ^= 1;
     }
@@ -1274,6 +1280,7 @@ static PlanShape loop_shape(const pf_loop* L) {
     sh.tu_cap[f] = std::min<uint32_t>(std::max<uint32_t>(L->ps->host.fluid[f].tu_limit, PF_BLOCK), PF_TU_MAX);
     sh.isv[f] = L->ps->host.fluid[f].isv_iters;
   }
+  sh.grid_cap = L->grid_cap;
   return sh;
 }
 
@@ -1932,6 +1939,15 @@ int pf_loop_set_graph(pf_loop* L, int enabled) {
   PF_API_END
 }
 
+int pf_loop_set_launch_shape(pf_loop* L, uint32_t workgroups) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop");
+  check_launch_shape(workgroups, "SimLoop");
+  if (workgroups != L->grid_cap) loop_invalidate_graphs(L);
+  L->grid_cap = workgroups;
+  PF_API_END
+}
+
 int pf_loop_destroy(pf_loop* L) {
   PF_API_BEGIN
   if (!L || L->disposed) return PF_OK;
@@ -2114,13 +2130,12 @@ static void tune_set_limit(const std::vector<pf_loop*>& loops, uint32_t cap) {
       if (pf_fluid_set_tile_union_limit(f, cap) != PF_OK) throw PfError{PF_EINVAL};
 }
 
-template <typename StepFn>
-static uint32_t tune_run(Context* ctx, const std::vector<pf_loop*>& loops, const uint32_t* cand, uint32_t n,
-                         uint32_t steps, float* ms_out, StepFn step) {
-  if (!cand || n == 0) fail(PF_EINVAL, "Autotune: at least one candidate is required");
-  for (uint32_t i = 0; i < n; ++i)
-    if (cand[i] < PF_BLOCK || cand[i] > PF_TU_MAX)
-      fail(PF_EINVAL, "Autotune: tile-union capacity %u outside [%u, %u]", cand[i], PF_BLOCK, PF_TU_MAX);
+// Times each candidate setting over `steps` steps (after one untimed step
+// that captures its graph), in a forward then a backward sweep, and keeps the
+// fastest: set(candidate) applies a setting, step() enqueues one step.
+template <typename SetFn, typename StepFn>
+static uint32_t tune_run(Context* ctx, const uint32_t* cand, uint32_t n, uint32_t steps, float* ms_out, SetFn set,
+                         StepFn step) {
   if (steps == 0) steps = 1;
   std::vector<double> total(n, 0.0);
   hipEvent_t e0, e1;
@@ -2130,7 +2145,7 @@ static uint32_t tune_run(Context* ctx, const std::vector<pf_loop*>& loops, const
     for (int pass = 0; pass < 2; ++pass) {
       for (uint32_t k = 0; k < n; ++k) {
         const uint32_t i = pass == 0 ? k : n - 1 - k;
-        tune_set_limit(loops, cand[i]);
+        set(cand[i]);
         step();  // captures the candidate's graph
         PF_HIP(hipEventRecord(e0, ctx->stream));
         for (uint32_t t = 0; t < steps; ++t) step();
@@ -2153,8 +2168,20 @@ static uint32_t tune_run(Context* ctx, const std::vector<pf_loop*>& loops, const
     if (ms_out) ms_out[i] = (float)(total[i] / 2.0);
     if (total[i] < total[best]) best = i;
   }
-  tune_set_limit(loops, cand[best]);
+  set(cand[best]);
   return cand[best];
+}
+
+static void tune_check_limits(const uint32_t* cand, uint32_t n) {
+  if (!cand || n == 0) fail(PF_EINVAL, "Autotune: at least one candidate is required");
+  for (uint32_t i = 0; i < n; ++i)
+    if (cand[i] < PF_BLOCK || cand[i] > PF_TU_MAX)
+      fail(PF_EINVAL, "Autotune: tile-union capacity %u outside [%u, %u]", cand[i], PF_BLOCK, PF_TU_MAX);
+}
+
+static void tune_check_shapes(const uint32_t* cand, uint32_t n) {
+  if (!cand || n == 0) fail(PF_EINVAL, "Autotune: at least one candidate is required");
+  for (uint32_t i = 0; i < n; ++i) check_launch_shape(cand[i], "Autotune");
 }
 
 extern "C" {
@@ -2164,10 +2191,13 @@ int pf_loop_autotune(pf_loop* L, double dt, const uint32_t* candidates, uint32_t
   PF_API_BEGIN
   loop_alive(L, "SimLoop.autotune");
   if (L->fluids.empty()) fail(PF_EINVAL, "SimLoop.autotune: the loop has no fluid to tune");
+  tune_check_limits(candidates, n);
   std::vector<pf_loop*> loops{L};
-  const uint32_t b = tune_run(L->ctx, loops, candidates, n, steps, ms_per_step, [&] {
-    if (pf_loop_step(L, dt) != PF_OK) throw PfError{PF_EINVAL};
-  });
+  const uint32_t b = tune_run(
+      L->ctx, candidates, n, steps, ms_per_step, [&](uint32_t c) { tune_set_limit(loops, c); },
+      [&] {
+        if (pf_loop_step(L, dt) != PF_OK) throw PfError{PF_EINVAL};
+      });
   if (best) *best = b;
   PF_API_END
 }
@@ -2177,10 +2207,61 @@ int pf_batch_autotune(pf_batch* B, double dt, const uint32_t* candidates, uint32
   PF_API_BEGIN
   if (!B) fail(PF_EINVAL, "Batch: null handle");
   if (B->shape.nfluids == 0) fail(PF_EINVAL, "Batch.autotune: the batch has no fluid to tune");
-  const uint32_t b = tune_run(B->ctx, B->loops, candidates, n, steps, ms_per_step, [&] {
-    if (pf_batch_step(B, dt) != PF_OK) throw PfError{PF_EINVAL};
-  });
+  tune_check_limits(candidates, n);
+  const uint32_t b = tune_run(
+      B->ctx, candidates, n, steps, ms_per_step, [&](uint32_t c) { tune_set_limit(B->loops, c); },
+      [&] {
+        if (pf_batch_step(B, dt) != PF_OK) throw PfError{PF_EINVAL};
+      });
   if (best) *best = b;
+  PF_API_END
+}
+
+int pf_loop_autotune_launch(pf_loop* L, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                            uint32_t* best, float* ms_per_step) {
+  PF_API_BEGIN
+  loop_alive(L, "SimLoop.autotuneLaunch");
+  tune_check_shapes(candidates, n);
+  const uint32_t b = tune_run(
+      L->ctx, candidates, n, steps, ms_per_step,
+      [&](uint32_t w) {
+        if (pf_loop_set_launch_shape(L, w) != PF_OK) throw PfError{PF_EINVAL};
+      },
+      [&] {
+        if (pf_loop_step(L, dt) != PF_OK) throw PfError{PF_EINVAL};
+      });
+  if (best) *best = b;
+  PF_API_END
+}
+
+int pf_batch_autotune_launch(pf_batch* B, double dt, const uint32_t* candidates, uint32_t n, uint32_t steps,
+                             uint32_t* best, float* ms_per_step) {
+  PF_API_BEGIN
+  if (!B) fail(PF_EINVAL, "Batch: null handle");
+  tune_check_shapes(candidates, n);
+  const uint32_t b = tune_run(
+      B->ctx, candidates, n, steps, ms_per_step,
+      [&](uint32_t w) {
+        if (pf_batch_set_launch_shape(B, w) != PF_OK) throw PfError{PF_EINVAL};
+      },
+      [&] {
+        if (pf_batch_step(B, dt) != PF_OK) throw PfError{PF_EINVAL};
+      });
+  if (best) *best = b;
+  PF_API_END
+}
+
+int pf_batch_set_launch_shape(pf_batch* B, uint32_t workgroups) {
+  PF_API_BEGIN
+  if (!B) fail(PF_EINVAL, "Batch: null handle");
+  check_launch_shape(workgroups, "Batch");
+  if (workgroups != B->shape.grid_cap && B->graph) {
+    PF_HIP(hipStreamSynchronize(B->ctx->stream));  // not destroyed while a replay runs
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  if (workgroups != B->shape.grid_cap) B->graph_valid = false;
+  B->shape.grid_cap = workgroups;
   PF_API_END
 }
 

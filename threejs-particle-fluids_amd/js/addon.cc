@@ -1197,6 +1197,30 @@ napi_value ServerEvictScene(napi_env env, napi_callback_info info) {
   PF_CALL(pf_server_evict_scene((pf_server*)handle(env, a.argv[0]), u32(env, a.argv[1])));
   return nullptr;
 }
+// Launch shapes (no reference counterpart): 0 or a persistent workgroup cap.
+napi_value LoopSetLaunchShape(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_loop_set_launch_shape((pf_loop*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value BatchSetLaunchShape(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_batch_set_launch_shape((pf_batch*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value ServerSetLaunchShape(napi_env env, napi_callback_info info) {
+  Args a = get_args(env, info);
+  PF_CALL(pf_server_set_launch_shape((pf_server*)handle(env, a.argv[0]), u32(env, a.argv[1])));
+  return nullptr;
+}
+napi_value LoopAutotuneLaunch(napi_env env, napi_callback_info info) {
+  return autotune(env, info, [](void* h, double dt, const uint32_t* c, uint32_t n, uint32_t st, uint32_t* b,
+                                float* ms) { return pf_loop_autotune_launch((pf_loop*)h, dt, c, n, st, b, ms); });
+}
+napi_value BatchAutotuneLaunch(napi_env env, napi_callback_info info) {
+  return autotune(env, info, [](void* h, double dt, const uint32_t* c, uint32_t n, uint32_t st, uint32_t* b,
+                                float* ms) { return pf_batch_autotune_launch((pf_batch*)h, dt, c, n, st, b, ms); });
+}
 napi_value ServerDestroy(napi_env env, napi_callback_info info) {
   Args a = get_args(env, info);
   PF_CALL(pf_server_destroy((pf_server*)handle(env, a.argv[0])));
@@ -1266,6 +1290,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"serverCreate", ServerCreate}, {"serverAddScene", ServerAddScene}, {"serverRemoveScene", ServerRemoveScene},
       {"serverSubmit", ServerSubmit}, {"serverWait", ServerWait}, {"serverDrain", ServerDrain},
       {"serverStats", ServerStats}, {"serverEvictScene", ServerEvictScene}, {"serverDestroy", ServerDestroy},
+      {"loopSetLaunchShape", LoopSetLaunchShape}, {"batchSetLaunchShape", BatchSetLaunchShape},
+      {"serverSetLaunchShape", ServerSetLaunchShape}, {"loopAutotuneLaunch", LoopAutotuneLaunch},
+      {"batchAutotuneLaunch", BatchAutotuneLaunch},
       {"layoutFillBox", LayoutFillBox}, {"layoutFitBudget", LayoutFitBudget}, {"abiVersion", AbiVersion},
   };
   for (const Fn& f : fns) {

@@ -425,6 +425,15 @@ class SimLoop {
   autotune(dt, candidates = TILE_UNION_CANDIDATES, steps = 2) {
     return native.loopAutotune(this.handle, dt, candidates, steps);
   }
+  /** 0: one workgroup per tile; else the persistent form, at most that many
+   * workgroups (a multiple of 8) per launch. Results do not depend on it. */
+  setLaunchShape(workgroups) {
+    native.loopSetLaunchShape(this.handle, workgroups);
+  }
+  /** Times each launch shape on this loop's workload and keeps the fastest. */
+  autotuneLaunch(dt, candidates = LAUNCH_SHAPE_CANDIDATES, steps = 2) {
+    return native.loopAutotuneLaunch(this.handle, dt, candidates, steps);
+  }
   get hashGrid() {
     return new HashGrid(this.particles, null, native.loopGrid(this.handle));
   }
@@ -438,6 +447,8 @@ class SimLoop {
 }
 
 const TILE_UNION_CANDIDATES = [768, 1024, 1280, 1536, 2048];
+// 0: one workgroup per tile; then 1, 2, 4 and 8 persistent workgroups per CU of 256.
+const LAUNCH_SHAPE_CANDIDATES = [0, 256, 512, 1024, 2048];
 
 /** Many independent SimLoops advanced by one launch sequence (no reference counterpart). */
 class Batch {
@@ -455,6 +466,12 @@ class Batch {
   }
   autotune(dt, candidates = TILE_UNION_CANDIDATES, steps = 2) {
     return native.batchAutotune(this.handle, dt, candidates, steps);
+  }
+  setLaunchShape(workgroups) {
+    native.batchSetLaunchShape(this.handle, workgroups);
+  }
+  autotuneLaunch(dt, candidates = LAUNCH_SHAPE_CANDIDATES, steps = 2) {
+    return native.batchAutotuneLaunch(this.handle, dt, candidates, steps);
   }
   dispose() {
     if (this.handle) native.batchDestroy(this.handle);
@@ -495,6 +512,10 @@ class Server {
   }
   evictScene(id) {
     native.serverEvictScene(this.handle, id);
+  }
+  /** Launch shape of the launches formed from now on (SimLoop.setLaunchShape). */
+  setLaunchShape(workgroups) {
+    native.serverSetLaunchShape(this.handle, workgroups);
   }
   get stats() {
     return native.serverStats(this.handle);

@@ -620,6 +620,9 @@ SimLoopOverflow = namedtuple("SimLoopOverflow", "positions velocities grid conta
 
 
 TILE_UNION_CANDIDATES = (768, 1024, 1280, 1536, 2048)
+# Launch shapes: 0 is one workgroup per tile; the others are the persistent
+# form's workgroup caps, 1, 2, 4 and 8 workgroups per CU of the MI355X's 256.
+LAUNCH_SHAPE_CANDIDATES = (0, 256, 512, 1024, 2048)
 
 
 def _autotune(fn, handle, dt, candidates, steps):
@@ -742,6 +745,17 @@ class SimLoop:
         the fastest; returns (best, {capacity: ms per step}). Advances the scene."""
         return _autotune("pf_loop_autotune", self.handle, dt, candidates, steps)
 
+    def setLaunchShape(self, workgroups):
+        """0: one workgroup per tile; else the persistent form, at most
+        `workgroups` (a multiple of 8) workgroups per launch, each walking
+        the tile list. Results do not depend on it."""
+        call("pf_loop_set_launch_shape", self.handle, int(workgroups))
+
+    def autotuneLaunch(self, dt, candidates=LAUNCH_SHAPE_CANDIDATES, steps=2):
+        """Time each launch shape on this loop's workload and keep the
+        fastest; returns (best, {workgroups: ms per step}). Advances the scene."""
+        return _autotune("pf_loop_autotune_launch", self.handle, dt, candidates, steps)
+
     @property
     def hashGrid(self):
         if self._grid is None:
@@ -777,6 +791,15 @@ class Batch:
         """Time each tile-union LDS capacity on the batch's workload and keep
         the fastest; returns (best, {capacity: ms per step}). Advances the scenes."""
         return _autotune("pf_batch_autotune", self.handle, dt, candidates, steps)
+
+    def setLaunchShape(self, workgroups):
+        """0: one workgroup per tile; else the persistent form (SimLoop.setLaunchShape)."""
+        call("pf_batch_set_launch_shape", self.handle, int(workgroups))
+
+    def autotuneLaunch(self, dt, candidates=LAUNCH_SHAPE_CANDIDATES, steps=2):
+        """Time each launch shape on the batch's workload and keep the
+        fastest; returns (best, {workgroups: ms per step}). Advances the scenes."""
+        return _autotune("pf_batch_autotune_launch", self.handle, dt, candidates, steps)
 
     def dispose(self):
         if self.handle:
@@ -828,6 +851,10 @@ class Server:
 
     def evictScene(self, sceneId):
         call("pf_server_evict_scene", self.handle, int(sceneId))
+
+    def setLaunchShape(self, workgroups):
+        """Launch shape of the launches formed from now on (SimLoop.setLaunchShape)."""
+        call("pf_server_set_launch_shape", self.handle, int(workgroups))
 
     def now(self):
         return _lib.load().pf_server_now(self.handle)

@@ -247,6 +247,11 @@ SIGNATURES = {
     "pf_batch_destroy": [vp],
     "pf_loop_autotune": [vp, f64, P(u32), u32, u32, P(u32), P(f32)],
     "pf_batch_autotune": [vp, f64, P(u32), u32, u32, P(u32), P(f32)],
+    "pf_loop_set_launch_shape": [vp, u32],
+    "pf_batch_set_launch_shape": [vp, u32],
+    "pf_server_set_launch_shape": [vp, u32],
+    "pf_loop_autotune_launch": [vp, f64, P(u32), u32, u32, P(u32), P(f32)],
+    "pf_batch_autotune_launch": [vp, f64, P(u32), u32, u32, P(u32), P(f32)],
     "pf_server_create": [vp, P(ServerOptions), P(vp)],
     "pf_server_add_scene": [vp, vp, P(u32)],
     "pf_server_remove_scene": [vp, u32],
