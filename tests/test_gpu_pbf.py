@@ -223,15 +223,19 @@ def test_truncated_neighbour_lists_match_fp32_oracle(effects):
     ps.dispose()
 
 
-def test_fast_truncated_dam_break_matches_fp32_oracle():
+@pytest.mark.parametrize("limit", [None, 512])
+def test_fast_truncated_dam_break_matches_fp32_oracle(limit):
     """The dam break at a 60,000 budget under the bench's step (2 substeps x
     4 iterations): lists are cut at 64 from step 10 while the flow reaches
     10 m/s, so constraint corrections carry particles across cells between
     the list build and surface tension. Every step is bit-identical with the
-    oracle."""
+    oracle. A tile-union limit of 512 sends most tiles through the big-union
+    neighbour build, its compaction and its global-slot lists."""
     sc = scenes.build_dam_break(60000)
+    if limit is not None:
+        sc["fluid"].setTileUnionLimit(limit)
     n = sc["ps"].capacity
-    print(f"[pbf] fast truncated dam break, {n} particles", flush=True)
+    print(f"[pbf] fast truncated dam break, {n} particles, tile-union limit {limit}", flush=True)
     o = oracle_dam_break(sc["positions"], sc["radius"], 2, 4, "f32")
     cut = []
     for step in range(16):

@@ -451,6 +451,165 @@ static __device__ __forceinline__ void store_record(const SceneDev& S, const Flu
   F.normal[2 * s + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
 }
 
+// Steps 1 and 2 of a tile's neighbour build. 1, the bucket set: the first
+// particle of each run of equal query cells inserts its 27 stencil buckets in
+// the LDS hash set h_key. 2, the union layout: buckets in hash-slot order,
+// offsets by a block scan, into h_off ((union offset << 16) | count) and
+// h_start (first sorted slot). *leader: the thread's particle opens a run.
+// Returns the union size U, or PF_TU_GLOBAL when the set overflows (more
+// than PF_TU_HASH_LIMIT buckets); uniform over the block. u_words holds the
+// threads' query cells for the leader test (4 * PF_BLOCK words).
+static __device__ __forceinline__ uint32_t nb_layout(const SceneDev& S, bool active, const int q[3], bool* leader,
+                                                     uint32_t* h_key, uint32_t* h_off, uint32_t* h_start,
+                                                     uint32_t* ctl, uint32_t* u_words) {
+  const uint32_t tid = threadIdx.x;
+  int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(u_words);
+  uint32_t* a_sh = u_words + 3 * PF_BLOCK;
+  for (uint32_t i = tid; i < PF_TU_HASH; i += PF_BLOCK) h_key[i] = 0u;
+  if (tid < 4) ctl[tid] = 0u;
+  q_sh[0][tid] = q[0];
+  q_sh[1][tid] = q[1];
+  q_sh[2][tid] = q[2];
+  a_sh[tid] = active ? 1 : 0;
+  __syncthreads();
+  *leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] || q_sh[1][tid - 1] != q[1] ||
+                       q_sh[2][tid - 1] != q[2]);
+  // The leaders' cells, in lane order (h_start is free until the layout);
+  // then every thread inserts (cell, stencil) pairs, 27 per leader.
+  uint32_t nlead;
+  const uint32_t li = block_exclusive_scan(*leader ? 1u : 0u, ctl + 4, &nlead);
+  if (*leader) {
+    h_start[3 * li + 0] = (uint32_t)q[0];
+    h_start[3 * li + 1] = (uint32_t)q[1];
+    h_start[3 * li + 2] = (uint32_t)q[2];
+  }
+  __syncthreads();
+  {
+    volatile uint32_t* give_up = &ctl[1];
+    for (uint32_t task = tid; task < nlead * 27u && !*give_up; task += PF_BLOCK) {
+      const uint32_t ci = task / 27u, d = task - ci * 27u;
+      const uint32_t dx = d % 3u - 1u, dy = (d / 3u) % 3u - 1u, dz = d / 9u - 1u;
+      const uint32_t b = pf_morton((int)(h_start[3 * ci + 0] + dx), (int)(h_start[3 * ci + 1] + dy),
+                                   (int)(h_start[3 * ci + 2] + dz)) & S.hash_mask;
+      uint32_t h = tu_hash(b);
+      bool placed = false;
+      for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
+        const uint32_t prev = atomicCAS(&h_key[h], 0u, b + 1u);
+        if (prev == 0u) {
+          if (atomicAdd(&ctl[0], 1u) + 1u > PF_TU_HASH_LIMIT) *give_up = 1u;
+          placed = true;
+          break;
+        }
+        if (prev == b + 1u) {
+          placed = true;
+          break;
+        }
+        h = (h + 1u) & (PF_TU_HASH - 1u);
+      }
+      if (!placed) *give_up = 1u;
+    }
+  }
+  __syncthreads();
+  if (ctl[1] != 0u) return PF_TU_GLOBAL;
+  constexpr int PER = PF_TU_HASH / PF_BLOCK;
+  uint32_t cn[PER], st[PER], sum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t key = h_key[tid * PER + i];
+    cn[i] = 0u;
+    st[i] = 0u;
+    if (key) {
+      st[i] = S.start[key - 1u];
+      cn[i] = S.start[key] - st[i];
+    }
+    sum += cn[i];
+  }
+  uint32_t U = 0;
+  uint32_t off = block_exclusive_scan(sum, ctl + 4, &U);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    h_off[tid * PER + i] = (min(off, 0xffffu) << 16) | min(cn[i], 0xffffu);
+    h_start[tid * PER + i] = st[i];
+    off += cn[i];
+  }
+  __syncthreads();
+  return U;
+}
+
+// Step 3, staging the union's U entries (at most PER * PF_BLOCK): entry u
+// belongs to the last bucket whose union offset is not above u. Each bucket
+// marks its offset with its hash slot + 1 and a chunked prefix max spreads the
+// marks; the marks and the resolved slots use u_words (the positions' LDS,
+// PER entries per thread). Positions go to u_px/u_py/u_pz, sorted slots to
+// slots[u] (global or LDS). Every thread of the block calls it.
+template <int PER>
+static __device__ __forceinline__ void nb_stage(uint32_t U, const float4* __restrict__ xw, const uint32_t* h_off,
+                                                const uint32_t* h_start, uint32_t* ctl, uint32_t* u_words,
+                                                float* u_px, float* u_py, float* u_pz, uint32_t* slots) {
+  static_assert(PER % 4 == 0, "nb_stage reads marks four at a time");
+  const uint32_t tid = threadIdx.x;
+  uint32_t* mark = u_words;
+  const uint32_t uall = (U + PER - 1u) / PER * PER;
+  for (uint32_t u = tid; u < uall; u += PF_BLOCK) mark[u] = 0u;
+  __syncthreads();
+  constexpr int HPER = PF_TU_HASH / PF_BLOCK;
+#pragma unroll
+  for (int i = 0; i < HPER; ++i) {
+    const uint32_t h = tid * HPER + i;
+    const uint32_t ho = h_off[h];
+    if (ho & 0xffffu) mark[ho >> 16] = h + 1u;
+  }
+  __syncthreads();
+  const uint32_t u0 = tid * (uint32_t)PER;
+  uint32_t mk[PER], run = 0;
+  if (u0 < U) {
+#pragma unroll
+    for (int k4 = 0; k4 < PER; k4 += 4) {
+      const uint4 a = *reinterpret_cast<const uint4*>(mark + u0 + k4);
+      const uint32_t m[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        run = max(run, m[k]);
+        mk[k4 + k] = run;
+      }
+    }
+  }
+  const uint32_t carry = block_exclusive_max(run, ctl + 4);
+  if (u0 < U) {
+#pragma unroll
+    for (int k4 = 0; k4 < PER; k4 += 4) {
+      uint32_t r[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t h = max(mk[k4 + k], carry) - 1u;  // entry 0 opens the first bucket
+        r[k] = h_start[h] + ((u0 + (uint32_t)(k4 + k)) - (h_off[h] >> 16));
+      }
+      *reinterpret_cast<uint4*>(mark + u0 + k4) = make_uint4(r[0], r[1], r[2], r[3]);
+    }
+  }
+  __syncthreads();
+  uint32_t js[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t u = tid + i * PF_BLOCK;
+    js[i] = u < U ? mark[u] : 0u;
+  }
+  __syncthreads();
+  float4 xv[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) xv[i] = xw[js[i]];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const uint32_t u = tid + i * PF_BLOCK;
+    if (u < U) {
+      u_px[u] = xv[i].x;
+      u_py[u] = xv[i].y;
+      u_pz[u] = xv[i].z;
+      slots[u] = js[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour list (+ colour-field normal when surface tension is on)
 //
@@ -489,8 +648,6 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
   float* u_px = reinterpret_cast<float*>(pf_lds);
   float* u_py = u_px + cap;
   float* u_pz = u_py + cap;
-  int (*q_sh)[PF_BLOCK] = reinterpret_cast<int (*)[PF_BLOCK]>(u_words);
-  uint32_t* a_sh = u_words + 3 * PF_BLOCK;
   __shared__ uint32_t ctl[4 + PF_BLOCK / 64];  // distinct buckets, give-up flag, -, -, scan
   pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
     const Tile t = tiles[ti];
@@ -550,80 +707,11 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       return;
     }
 
-    // 1. Bucket set: the first particle of each run of equal query cells inserts
-    // its 27 stencil buckets.
-    for (uint32_t i = tid; i < PF_TU_HASH; i += PF_BLOCK) h_key[i] = 0u;
-    if (tid < 4) ctl[tid] = 0u;
-    q_sh[0][tid] = q[0];
-    q_sh[1][tid] = q[1];
-    q_sh[2][tid] = q[2];
-    a_sh[tid] = active ? 1 : 0;
-    __syncthreads();
-    const bool leader = active && (tid == 0 || !a_sh[tid - 1] || q_sh[0][tid - 1] != q[0] ||
-                                   q_sh[1][tid - 1] != q[1] || q_sh[2][tid - 1] != q[2]);
-    // The leaders' cells, in lane order (h_start is free until the layout);
-    // then every thread inserts (cell, stencil) pairs, 27 per leader.
-    uint32_t nlead;
-    const uint32_t li = block_exclusive_scan(leader ? 1u : 0u, ctl + 4, &nlead);
-    if (leader) {
-      h_start[3 * li + 0] = (uint32_t)q[0];
-      h_start[3 * li + 1] = (uint32_t)q[1];
-      h_start[3 * li + 2] = (uint32_t)q[2];
-    }
-    __syncthreads();
-    {
-      volatile uint32_t* give_up = &ctl[1];
-      for (uint32_t task = tid; task < nlead * 27u && !*give_up; task += PF_BLOCK) {
-        const uint32_t ci = task / 27u, d = task - ci * 27u;
-        const uint32_t dx = d % 3u - 1u, dy = (d / 3u) % 3u - 1u, dz = d / 9u - 1u;
-        const uint32_t b = pf_morton((int)(h_start[3 * ci + 0] + dx), (int)(h_start[3 * ci + 1] + dy),
-                                     (int)(h_start[3 * ci + 2] + dz)) & S.hash_mask;
-        uint32_t h = tu_hash(b);
-        bool placed = false;
-        for (uint32_t probe = 0; probe < PF_TU_HASH; ++probe) {
-          const uint32_t prev = atomicCAS(&h_key[h], 0u, b + 1u);
-          if (prev == 0u) {
-            if (atomicAdd(&ctl[0], 1u) + 1u > PF_TU_HASH_LIMIT) *give_up = 1u;
-            placed = true;
-            break;
-          }
-          if (prev == b + 1u) {
-            placed = true;
-            break;
-          }
-          h = (h + 1u) & (PF_TU_HASH - 1u);
-        }
-        if (!placed) *give_up = 1u;
-      }
-    }
-    __syncthreads();
-    bool global_walk = ctl[1] != 0u;
-    uint32_t U = 0;
-    if (!global_walk) {
-      // 2. Union layout: buckets in hash-slot order, offsets by a block scan.
-      constexpr int PER = PF_TU_HASH / PF_BLOCK;
-      uint32_t cn[PER], st[PER], sum = 0;
-  #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const uint32_t key = h_key[tid * PER + i];
-        cn[i] = 0u;
-        st[i] = 0u;
-        if (key) {
-          st[i] = S.start[key - 1u];
-          cn[i] = S.start[key] - st[i];
-        }
-        sum += cn[i];
-      }
-      uint32_t off = block_exclusive_scan(sum, ctl + 4, &U);
-  #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        h_off[tid * PER + i] = (min(off, 0xffffu) << 16) | min(cn[i], 0xffffu);
-        h_start[tid * PER + i] = st[i];
-        off += cn[i];
-      }
-      global_walk = U > min(F.tu_limit, cap);
-      __syncthreads();
-    }
+    bool leader = false;
+    const uint32_t U = nb_layout(S, active, q, &leader, h_key, h_off, h_start, ctl, u_words);
+    // A bucket set that overflowed (U == PF_TU_GLOBAL) or a union above the
+    // capacity walks global memory.
+    const bool global_walk = U > min(F.tu_limit, cap);
 
     // h_start is free once the layout is read: each wave's run-walk marks.
     uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
@@ -651,69 +739,9 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       return;
     }
 
-    // 3. Stage the union: entry u belongs to the last bucket whose union offset
-    // is not above u. Each bucket marks its offset with its hash slot + 1 and a
-    // chunked prefix max spreads the marks; the marks and the resolved slots
-    // use the union's LDS (4 entries per float4, so 8 per thread fit).
-    {
-      uint32_t* __restrict__ slots = F.tu_slots + (size_t)tile * PF_TU_MAX;
-      uint32_t* mark = u_words;
-      const uint32_t u8 = (U + 7u) & ~7u;
-      for (uint32_t u = tid; u < u8; u += PF_BLOCK) mark[u] = 0u;
-      __syncthreads();
-      constexpr int PER = PF_TU_HASH / PF_BLOCK;
-  #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const uint32_t h = tid * PER + i;
-        const uint32_t ho = h_off[h];
-        if (ho & 0xffffu) mark[ho >> 16] = h + 1u;
-      }
-      __syncthreads();
-      const uint32_t u0 = tid * 8u;
-      uint32_t mk[8], run = 0;
-      if (u0 < U) {
-        const uint4 a = *reinterpret_cast<const uint4*>(mark + u0);
-        const uint4 b = *reinterpret_cast<const uint4*>(mark + u0 + 4u);
-        const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          run = max(run, m[k]);
-          mk[k] = run;
-        }
-      }
-      const uint32_t carry = block_exclusive_max(run, ctl + 4);
-      if (u0 < U) {
-        uint32_t r[8];
-  #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const uint32_t h = max(mk[k], carry) - 1u;  // entry 0 opens the first bucket
-          r[k] = h_start[h] + ((u0 + (uint32_t)k) - (h_off[h] >> 16));
-        }
-        *reinterpret_cast<uint4*>(mark + u0) = make_uint4(r[0], r[1], r[2], r[3]);
-        *reinterpret_cast<uint4*>(mark + u0 + 4u) = make_uint4(r[4], r[5], r[6], r[7]);
-      }
-      __syncthreads();
-      uint32_t js[PF_TU_PER];
-  #pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) {
-        const uint32_t u = tid + i * PF_BLOCK;
-        js[i] = u < U ? mark[u] : 0u;
-      }
-      __syncthreads();
-      float4 xv[PF_TU_PER];
-  #pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) xv[i] = xw[js[i]];
-  #pragma unroll
-      for (int i = 0; i < PF_TU_PER; ++i) {
-        const uint32_t u = tid + i * PF_BLOCK;
-        if (u < U) {
-          u_px[u] = xv[i].x;
-          u_py[u] = xv[i].y;
-          u_pz[u] = xv[i].z;
-          slots[u] = js[i];
-        }
-      }
-    }
+    // 3. Stage the union.
+    nb_stage<PF_TU_MAX / PF_BLOCK>(U, xw, h_off, h_start, ctl, u_words, u_px, u_py, u_pz,
+                                   F.tu_slots + (size_t)tile * PF_TU_MAX);
     __syncthreads();
 
     // 4. Walk the union.
@@ -1054,22 +1082,25 @@ static __device__ __forceinline__ void impulse_particle(const SceneDev& S, const
 }
 
 // Surface tension of a staged tile with its pairs spread over the wave. The
-// lanes read their lists eight entries at a time; each entry naming a pair the
-// particle evaluates (a higher fluid id) goes to the wave's pair queue in LDS
-// as (lane, union index), and the wave then evaluates the queue 64 pairs at a
-// time, one per lane, whoever's pairs they are. Both halves of a pair are
+// lanes read their lists eight entries at a time (the next eight already in
+// flight); each entry naming a pair the particle evaluates (a higher fluid id)
+// goes to the wave's pair queue in LDS as (lane, union index), and the wave
+// then evaluates the queue 64 pairs at a time, one per lane, whoever's pairs
+// they are. The queue holds every lane's first pair, then every lane's second
+// and so on, so the 64 pairs of a pass mostly belong to different particles
+// and their own-sum atomics do not collide. Both halves of a pair are
 // fixed-point sums (the particle's own in own_l, the partner's in acc_l), so
-// the order the pairs are evaluated in does not change them. s_id holds the
-// union's particle ids (the lower-end test reads LDS only); a pair reads its
-// partner's slot from the tile's union slots and its record from global
-// memory. Adhesion stays with its particle's lane. Every thread of the block
-// calls it.
+// the order the pairs are evaluated in does not change them. s_id and s_slot
+// hold the union's particle ids and sorted slots: the lower-end test and a
+// pair's partner slot read LDS only, and a pass's records (its own and its
+// partner's, from global memory) are loaded a pass ahead. Adhesion stays with
+// its particle's lane. Every thread of the block calls it.
 #define PF_PAIR_QUEUE (64 * 8)  // a round's pairs per wave: eight entries per lane
 template <bool ADH>
 static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, const FluidDev& F, int p, int c,
-                                                         uint32_t first, uint32_t tile, uint32_t U, int* acc_l,
-                                                         const uint32_t* s_id, uint32_t* q_l, int* own_l,
-                                                         float4* xi_l, float4* ni_l) {
+                                                         uint32_t first, uint32_t U, int* acc_l,
+                                                         const uint32_t* s_id, const uint32_t* s_slot,
+                                                         uint32_t* q_l, int* own_l) {
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;
   const uint32_t s = first + threadIdx.x;
@@ -1080,8 +1111,7 @@ static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, cons
   const bool adh_on = ADH && (F.effects & PF_FX_ADHESION) && S.has_boundary;
   uint32_t* q = q_l + wave * PF_PAIR_QUEUE;
   int* own = own_l + wave * 64 * 3;
-  float4* xw = xi_l + wave * 64;
-  float4* nw = ni_l + wave * 64;
+  const uint32_t s0 = first + wave * 64u;  // slot of the wave's lane 0 (< n whenever a lane has a list)
   float4 xi4 = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t id = 0, cnt = 0;
   bool act = false;
@@ -1090,34 +1120,29 @@ static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, cons
     id = S.id[p][s];
     act = in_range(id, F.start, F.count);
   }
-  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;  // the particle's record
-  if (act) {
-    cnt = F.nl_count[s];
-    r0 = F.normal[2 * s];
-    r1 = F.normal[2 * s + 1];
-  }
-  xw[lane] = r0;
-  nw[lane] = r1;
+  if (act) cnt = F.nl_count[s];
   own[3 * lane + 0] = 0;
   own[3 * lane + 1] = 0;
   own[3 * lane + 2] = 0;
   int tk[3] = {0, 0, 0};  // adhesion
   const PF_G uint32_t* __restrict__ col = pf_g(F.nl16) + s;
   const PF_G float4* __restrict__ g_rec = pf_g(F.normal);
-  const PF_G uint32_t* __restrict__ g_slot = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
   const PF_G float4* __restrict__ g_x = pf_g(xs);
   const PF_G float* __restrict__ g_vol = pf_g(S.bvol[p]);
   const float3 xi = pf_xyz(xi4);
+  constexpr uint32_t rmax = PF_MAX_NEIGHBORS / 2 - 1;
+  // The list words of a round (entries k0..k0+7), rows clamped inside the list.
+  auto load_words = [&](uint32_t k0, uint32_t w[4]) {
+    const uint32_t r0 = k0 >> 1;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) w[i] = k0 < cnt ? col[(size_t)min(r0 + i, rmax) * n] : 0u;
+  };
+  uint32_t wn[4];
+  load_words(0, wn);
   for (uint32_t k0 = 0; __builtin_amdgcn_ballot_w64(k0 < cnt) != 0ull; k0 += 8u) {
-    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    const uint32_t w0 = wn[0], w1 = wn[1], w2 = wn[2], w3 = wn[3];
+    load_words(k0 + 8u, wn);  // the next round's words, in flight during this round
     uint32_t m = 0, ma = 0;
-    if (k0 < cnt) {
-      const uint32_t r0 = k0 >> 1, rmax = PF_MAX_NEIGHBORS / 2 - 1;
-      w0 = col[(size_t)min(r0, rmax) * n];
-      w1 = col[(size_t)min(r0 + 1u, rmax) * n];
-      w2 = col[(size_t)min(r0 + 2u, rmax) * n];
-      w3 = col[(size_t)min(r0 + 3u, rmax) * n];
-    }
     auto entry = [=](uint32_t u) {  // union index of entry k0 + u
       const bool b1 = (u & 2u) != 0u, b2 = (u & 4u) != 0u;
       const uint32_t lo = b1 ? w1 : w0, hi = b1 ? w3 : w2;
@@ -1131,53 +1156,69 @@ static __device__ __forceinline__ void impulse_tile_flat(const SceneDev& S, cons
         const bool in = k0 + (uint32_t)u < cnt;
         const uint32_t idj = s_id[ix];
         m |= (in && idj > id && idj < fend) ? (1u << u) : 0u;
-        if (adh_on) ma |= (in && g_vol[g_slot[ix]] > 0.0f) ? (1u << u) : 0u;
+        if (adh_on) ma |= (in && g_vol[s_slot[ix]] > 0.0f) ? (1u << u) : 0u;
       }
     }
-    // The round's pairs, queued in lane order.
+    // The round's pairs, queued round-robin over the lanes: every lane's
+    // first pair in lane order, then every second one, and so on.
     const uint32_t np = (uint32_t)__popc(m);
-    const uint32_t incl = wave_inclusive_sum64(np);
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane(incl, 63);
-    uint32_t at = incl - np;
-    while (m != 0u) {
-      q[at++] = (lane << 11) | entry((uint32_t)__builtin_ctz(m));
-      m &= m - 1u;
+    uint32_t total = 0;
+    {
+      uint32_t mm = m;
+#pragma unroll
+      for (uint32_t r = 0; r < 8u; ++r) {
+        const bool has = np > r;
+        const uint64_t b = __builtin_amdgcn_ballot_w64(has);
+        if (b == 0ull) break;
+        if (has) {
+          const uint32_t at = total + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+          q[at] = (lane << 11) | entry((uint32_t)__builtin_ctz(mm));
+          mm &= mm - 1u;
+        }
+        total += (uint32_t)__popcll(b);
+      }
     }
     static_assert(PF_TU_MAX <= (1u << 11), "union index must fit 11 bits");
     __builtin_amdgcn_wave_barrier();
-    // 64 pairs per pass, the next pass's loads issued before this one's
-    // arithmetic.
-    uint32_t e = lane < total ? q[lane] : 0u;
-    uint32_t ix = e & 2047u, il = e >> 11;
-    float4 xa = pf_f4(xw[il]), na = pf_f4(nw[il]);
-    uint32_t j = g_slot[ix];
-    float4 xb = pf_f4(g_rec[2 * j]), nb = pf_f4(g_rec[2 * j + 1]);
+    // 64 pairs per pass, the next pass's records loaded before this one's
+    // arithmetic. Lanes past the queue's end read pair 0's entry (union index
+    // 0 exists: a list holds its own particle).
+    struct Pass {
+      uint32_t ix, il;
+      float4 xa, na, xb, nb;
+    };
+    auto load_pass = [&](uint32_t qi) {
+      Pass d;
+      const uint32_t e = qi < total ? q[qi] : 0u;
+      d.ix = e & 2047u;
+      d.il = e >> 11;
+      const uint32_t j = s_slot[d.ix];
+      const uint32_t so = min(s0 + d.il, n - 1u);
+      d.xa = pf_f4(g_rec[2 * so]);
+      d.na = pf_f4(g_rec[2 * so + 1]);
+      d.xb = pf_f4(g_rec[2 * j]);
+      d.nb = pf_f4(g_rec[2 * j + 1]);
+      return d;
+    };
+    Pass cur = load_pass(lane);
     for (uint32_t q0 = 0; q0 < total; q0 += 64u) {
       const bool mine = q0 + lane < total;
-      const uint32_t qn = q0 + 64u + lane;
-      const uint32_t en = qn < total ? q[qn] : 0u;
-      const uint32_t ixn = en & 2047u, iln = en >> 11;
-      const float4 xan = pf_f4(xw[iln]), nan = pf_f4(nw[iln]);
-      const uint32_t jn = g_slot[ixn];
-      const float4 xbn = pf_f4(g_rec[2 * jn]), nbn = pf_f4(g_rec[2 * jn + 1]);
+      const Pass nxt = load_pass(q0 + 64u + lane);
       float3 dv;
-      if (mine && pair_dv(F, S.dt, pf_xyz(xa), pf_xyz(xb), pf_xyz(na), pf_xyz(nb), xa.w, xb.w, &dv)) {
-        atomicAdd(own + 3 * il + 0, pf_ticks(dv.x, F.imp_scale));
-        atomicAdd(own + 3 * il + 1, pf_ticks(dv.y, F.imp_scale));
-        atomicAdd(own + 3 * il + 2, pf_ticks(dv.z, F.imp_scale));
-        atomicAdd(acc_l + 3 * ix + 0, pf_ticks(-dv.x, F.imp_scale));
-        atomicAdd(acc_l + 3 * ix + 1, pf_ticks(-dv.y, F.imp_scale));
-        atomicAdd(acc_l + 3 * ix + 2, pf_ticks(-dv.z, F.imp_scale));
+      if (mine && pair_dv(F, S.dt, pf_xyz(cur.xa), pf_xyz(cur.xb), pf_xyz(cur.na), pf_xyz(cur.nb), cur.xa.w,
+                          cur.xb.w, &dv)) {
+        atomicAdd(own + 3 * cur.il + 0, pf_ticks(dv.x, F.imp_scale));
+        atomicAdd(own + 3 * cur.il + 1, pf_ticks(dv.y, F.imp_scale));
+        atomicAdd(own + 3 * cur.il + 2, pf_ticks(dv.z, F.imp_scale));
+        atomicAdd(acc_l + 3 * cur.ix + 0, pf_ticks(-dv.x, F.imp_scale));
+        atomicAdd(acc_l + 3 * cur.ix + 1, pf_ticks(-dv.y, F.imp_scale));
+        atomicAdd(acc_l + 3 * cur.ix + 2, pf_ticks(-dv.z, F.imp_scale));
       }
-      ix = ixn;
-      il = iln;
-      xa = xan;
-      na = nan;
-      xb = xbn;
-      nb = nbn;
+      cur = nxt;
     }
     while (ma != 0u) {
-      const uint32_t jj = g_slot[entry((uint32_t)__builtin_ctz(ma))];
+      const uint32_t jj = s_slot[entry((uint32_t)__builtin_ctz(ma))];
       ma &= ma - 1u;
       adhesion_add(S, F, xi, pf_xyz(pf_f4(g_x[jj])), g_vol[jj], tk);
     }
@@ -1206,11 +1247,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
                                                        uint32_t ntiles, int fi, int p, int c) {
   __shared__ int acc_l[ST ? 3 * PF_TU_MAX : 1];
   __shared__ uint32_t s_id[ST ? PF_TU_MAX : 1];
-  // The waves' pair queues, own sums and own records.
+  __shared__ uint32_t s_slot[ST ? PF_TU_MAX : 1];
+  // The waves' pair queues and own sums.
   __shared__ uint32_t q_l[ST ? PF_BLOCK / 64 * PF_PAIR_QUEUE : 1];
   __shared__ int own_l[ST ? PF_BLOCK * 3 : 1];
-  __shared__ float4 xi_l[ST ? PF_BLOCK : 1];
-  __shared__ float4 ni_l[ST ? PF_BLOCK : 1];
   pf_for_tiles<PERSIST>(ntiles, [&](uint32_t ti) {
     const Tile t = tiles[ti];
     const SceneDev& S = scenes[t.scene];
@@ -1231,23 +1271,25 @@ __global__ __launch_bounds__(PF_BLOCK) void k_impulses(const SceneDev* __restric
   #pragma unroll
       for (int i = 0; i < PF_TU_PER; ++i) {
         const uint32_t u = threadIdx.x + i * PF_BLOCK;
-        if (u < U) s_id[u] = ids[i];
+        if (u < U) {
+          s_id[u] = ids[i];
+          s_slot[u] = js[i];
+        }
       }
       __syncthreads();
     }
     const uint32_t s = t.first + threadIdx.x;
     if (lds) {
-      if constexpr (ST) impulse_tile_flat<ADH>(S, F, p, c, t.first, tile, U, acc_l, s_id, q_l, own_l, xi_l, ni_l);
+      if constexpr (ST) impulse_tile_flat<ADH>(S, F, p, c, t.first, U, acc_l, s_id, s_slot, q_l, own_l);
     } else if (s < S.n) {
       impulse_particle<ST, ADH>(S, F, p, c, s, tile);
     }
     if (lds) {
       __syncthreads();
-      const uint32_t* __restrict__ uslots = F.tu_slots + (size_t)tile * PF_TU_MAX;
       for (uint32_t u = threadIdx.x; u < U; u += PF_BLOCK) {
         const int a0 = acc_l[3 * u], a1 = acc_l[3 * u + 1], a2 = acc_l[3 * u + 2];
         if ((a0 | a1 | a2) != 0) {
-          PF_G int* a = pf_g(reinterpret_cast<int*>(S.ct.acc + uslots[u]));
+          PF_G int* a = pf_g(reinterpret_cast<int*>(S.ct.acc + s_slot[u]));
           pf_atomic_add_g(a + 0, a0);
           pf_atomic_add_g(a + 1, a1);
           pf_atomic_add_g(a + 2, a2);
