@@ -166,14 +166,16 @@ static __device__ __forceinline__ uint32_t cu_home(uint32_t key, uint32_t hs) {
 // Union of the particles named by a global-walk tile's finished lists (global
 // slots, F.nl). Every thread of the block calls it; `count` is the calling
 // thread's list length (0 for threads without a list). When at most `limit`
-// distinct slots appear, they become the tile's union (F.tu_slots, in
-// table order), each list is rewritten as 16-bit union indices (F.nl16) and
+// distinct slots appear, they become the tile's union (F.tu_slots, slots
+// ascending), each list is rewritten as 16-bit union indices (F.nl16) and
 // the union size is returned; otherwise PF_TU_GLOBAL. The table lives in the
 // union's LDS, 3 * cap words: 2 * cap keys (slot + 1) and as many 16-bit
-// dense indices.
+// dense indices; `dense` (limit words of LDS) sorts the keys. Ascending
+// slots make consecutive union entries mostly consecutive slots, so the
+// passes that stage the union read it coalesced.
 static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* lds,
                                          uint32_t cap, uint32_t limit, uint32_t* ctl, uint32_t count,
-                                         uint32_t s) {
+                                         uint32_t s, uint32_t* dense) {
   const uint32_t tid = threadIdx.x;
   const uint32_t hs = 2u * cap;
   uint32_t* hk = lds;                                        // [hs] slot + 1, 0 = empty
@@ -213,21 +215,47 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   __syncthreads();
   // Uniform over the block.
   if (ctl[1] != 0u || ctl[0] > limit || limit == 0u) return PF_TU_GLOBAL;
-  // Dense indices in table order: each thread numbers a contiguous chunk.
+  // The keys, gathered in table order (each thread a contiguous chunk).
   const uint32_t per = (hs + PF_BLOCK - 1u) / PF_BLOCK;
   const uint32_t b0 = min(tid * per, hs), b1 = min(b0 + per, hs);
   uint32_t mine = 0;
   for (uint32_t i = b0; i < b1; ++i) mine += hk[i] ? 1u : 0u;
   uint32_t U = 0;
   uint32_t at = block_exclusive_scan(mine, ctl + 4, &U);
-  PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
   for (uint32_t i = b0; i < b1; ++i) {
     const uint32_t key = hk[i];
-    if (key) {
-      hv[i] = (uint16_t)at;
-      slots[at] = key - 1u;
-      ++at;
+    if (key) dense[at++] = key;
+  }
+  __syncthreads();
+  // Sorted ascending by a bitonic network over the next power of two P, in
+  // its all-ascending form (each merge opens with the mirrored comparison):
+  // entries past U stand for +inf and never move, so only pairs inside U swap.
+  uint32_t P = 1;
+  while (P < U) P <<= 1;
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t cidx = tid; cidx < P / 2; cidx += PF_BLOCK) {
+        const uint32_t a = (cidx / j) * 2u * j + cidx % j;
+        const uint32_t b = j == (k >> 1) ? a ^ (k - 1u) : a + j;
+        if (b < U) {
+          const uint32_t x = dense[a], y = dense[b];
+          if (x > y) {
+            dense[a] = y;
+            dense[b] = x;
+          }
+        }
+      }
+      __syncthreads();
     }
+  }
+  // Each key's table entry takes its rank.
+  PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+  for (uint32_t i = tid; i < U; i += PF_BLOCK) {
+    const uint32_t key = dense[i];
+    uint32_t h = cu_home(key, hs);
+    for (uint32_t probe = 1; probe < hs && hk[h] != key; ++probe) h = (h + 1u == hs) ? 0u : h + 1u;
+    hv[h] = (uint16_t)i;
+    slots[i] = key - 1u;
   }
   __syncthreads();
   // The lists as pairs of union indices, column-major like the staged walk's.
@@ -253,6 +281,93 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
 #pragma unroll
     for (int u = 0; u < 8; u += 2)
       if (k0 + u < count) row[(size_t)((k0 + u) >> 1) * n] = idx[u] | (idx[u + 1] << 16);
+  }
+  return U;
+}
+
+// Union of the particles named by a global-walk tile's finished lists when
+// the walk also wrote them as 16-bit indices into the tile's bucket layout of
+// ulay entries (F.nl16; walk_runs ULIST). Every thread of the block calls it;
+// `count` is the thread's list length (0 without a list). The listed entries
+// are marked in an LDS bitmap over the layout (bitmap: ulay bits, at most
+// 2^16) and numbered in layout order by a prefix of the words' popcounts
+// (pre16: a 16-bit prefix per word; claim: PF_TU_MAX bits marking the union
+// entries whose slot is written), so the union is in bucket order, slots
+// ascending inside each bucket, and the passes that stage it read it
+// coalesced. When at most `limit` entries are listed they become the tile's
+// union (F.tu_slots) and each list is rewritten as 16-bit union indices; the
+// union size is returned. Otherwise the lists keep their global slots (F.nl)
+// and PF_TU_GLOBAL is returned.
+static __device__ uint32_t compact_layout(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* bitmap,
+                                          uint16_t* pre16, uint32_t* claim, uint32_t ulay, uint32_t limit,
+                                          uint32_t* ctl, uint32_t count, uint32_t s) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n = S.n;
+  const uint32_t nw = (ulay + 31u) >> 5;
+  constexpr uint32_t WPT = 2048u / PF_BLOCK;  // bitmap words per thread (2^16 bits)
+  __syncthreads();  // every wave has left the walk (its marks share the bitmap's words)
+  for (uint32_t w = tid; w < nw; w += PF_BLOCK) bitmap[w] = 0u;
+  if (tid < PF_TU_MAX / 32) claim[tid] = 0u;
+  __syncthreads();
+  const PF_G uint32_t* __restrict__ row16 = pf_g(F.nl16) + s;
+  constexpr uint32_t rmax = PF_MAX_NEIGHBORS / 2 - 1;
+  const uint32_t ul = ulay ? ulay - 1u : 0u;
+  // Four list words (eight entries) per round trip; entries past the count skipped.
+  for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
+    uint32_t w[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) w[i] = row16[(size_t)min((k0 >> 1) + i, rmax) * n];
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      if (k0 + e < count) {
+        const uint32_t u = min((w[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu, ul);
+        atomicOr(bitmap + (u >> 5), 1u << (u & 31u));
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t w0 = tid * WPT;
+  uint32_t pc[WPT], sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < WPT; ++i) {
+    pc[i] = w0 + i < nw ? (uint32_t)__popc(bitmap[w0 + i]) : 0u;
+    sum += pc[i];
+  }
+  uint32_t U = 0;
+  uint32_t at = block_exclusive_scan(sum, ctl + 4, &U);
+  // Uniform over the block.
+  if (U > limit || limit == 0u) return PF_TU_GLOBAL;
+#pragma unroll
+  for (uint32_t i = 0; i < WPT; ++i) {
+    if (w0 + i < nw) pre16[w0 + i] = (uint16_t)at;
+    at += pc[i];
+  }
+  __syncthreads();
+  // Each entry's union index; the first list entry to claim a union entry
+  // writes its slot.
+  PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
+  const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
+  PF_G uint32_t* __restrict__ out16 = pf_g(F.nl16) + s;
+  for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
+    uint32_t w[4], js[8];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) w[i] = row16[(size_t)min((k0 >> 1) + i, rmax) * n];
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) js[e] = col[(size_t)min(k0 + e, count - 1u) * n];
+    uint32_t idx[8];
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      const uint32_t u = min((w[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu, ul);
+      const uint32_t below = bitmap[u >> 5] & ((1u << (u & 31u)) - 1u);
+      idx[e] = k0 + e < count ? (uint32_t)pre16[u >> 5] + (uint32_t)__popc(below) : 0u;
+      if (k0 + e < count && idx[e] < PF_TU_MAX) {
+        const uint32_t bit = 1u << (idx[e] & 31u);
+        if (!(atomicOr(claim + (idx[e] >> 5), bit) & bit)) slots[idx[e]] = js[e];
+      }
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 8; e += 2)
+      if (k0 + e < count) out16[(size_t)((k0 + e) >> 1) * n] = idx[e] | (idx[e + 1] << 16);
   }
   return U;
 }
@@ -312,7 +427,9 @@ static __device__ __forceinline__ uint32_t wave_inclusive_sum64(uint32_t v) {
 // every offset within 32 bits). marks: 64 * PF_RUN_CHUNKS words of LDS owned
 // by the wave. Returns the lane's list length; *over is set where a particle
 // has more than PF_MAX_NEIGHBORS neighbours (its list keeps the first ones).
-template <bool STAGED, bool WIDE>
+// ULIST (global walk of a tile whose bucket layout is valid, h_key/h_off):
+// every entry is also written as its 16-bit layout index into F.nl16.
+template <bool STAGED, bool WIDE, bool ULIST = false>
 static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const FluidDev& F, const float4* __restrict__ xw,
                                      const float* u_px, const float* u_py, const float* u_pz,
                                      const uint32_t* h_key, const uint32_t* h_off, uint32_t* marks,
@@ -345,10 +462,10 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
     const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane(q[2], a);
     // The stencil buckets' entry ranges, their prefix over the walk order and
     // the marks' keys.
-    uint32_t cn = 0, base = 0;
+    uint32_t cn = 0, base = 0, udelta = 0;
     if (lane < 27u) {
       const uint32_t b = pf_morton((int)(q0 + sdx), (int)(q1 + sdy), (int)(q2 + sdz)) & S.hash_mask;
-      if (STAGED) {
+      if (STAGED || ULIST) {
         uint32_t h = tu_hash(b);
         for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
         if (h_key[h] == b + 1u) {  // always: the cell's leader inserted b
@@ -356,9 +473,12 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
           base = ho >> 16;
           cn = ho & 0xffffu;
         }
-      } else {
-        base = S.start[b];
-        cn = S.start[b + 1] - base;
+      }
+      if (!STAGED) {
+        const uint32_t first = S.start[b];
+        if (ULIST) udelta = base - first;  // layout index minus sorted slot, for the bucket's entries
+        base = first;
+        cn = S.start[b + 1] - first;
       }
     }
     const uint32_t inc = wave_inclusive_sum64(cn);
@@ -369,7 +489,7 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
     uint32_t carry = 0;
     for (uint32_t k0 = 0; k0 < C && live != 0ull; k0 += 64u * K) {
       float3 xj[K];
-      uint32_t ej[K];
+      uint32_t ej[K], uj[K];
 #pragma unroll
       for (uint32_t j = 0; j < K; ++j) {
         const uint32_t c0 = k0 + 64u * j;
@@ -384,6 +504,11 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
         const bool valid = kk < C;
         const uint32_t e = valid ? kk + (v & 0x7ffffffu) - (1u << 26) : 0u;
         ej[j] = e;
+        if (ULIST) {
+          // The candidate's bucket is stencil lane (v >> 27) - 1.
+          const uint32_t st = ((v >> 27) - 1u) & 63u;
+          uj[j] = e + (uint32_t)__builtin_amdgcn_ds_bpermute((int)(st << 2), (int)udelta);
+        }
         float3 p;
         if (STAGED) {
           p = make_float3(u_px[e], u_py[e], u_pz[e]);
@@ -404,6 +529,7 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
         uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane(count, i);
         // The particle's list column, word rows n apart.
         PF_G char* const col = STAGED ? (PF_G char*)(nl16 + s0 + i) : (PF_G char*)(nl + s0 + i);
+        PF_G char* const col16 = (PF_G char*)(nl16 + s0 + i);
         // cnt counts every neighbour found; entries past PF_MAX_NEIGHBORS are
         // not stored, and a list that passes it closes after this block.
 #pragma unroll
@@ -423,6 +549,11 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
               } else {
                 const uint32_t off = WIDE ? pos * (n * 4u) : __umul24(pos, n * 4u);
                 *(PF_G uint32_t*)(col + off) = ej[j];
+                if (ULIST) {
+                  const uint32_t off16 =
+                      (WIDE ? (pos >> 1) * (n * 4u) : __umul24(pos >> 1, n * 4u)) + ((pos & 1u) << 1);
+                  *(PF_G uint16_t*)(col16 + off16) = (uint16_t)uj[j];
+                }
               }
             }
           }
@@ -637,9 +768,10 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
                                                         const Tile* __restrict__ tiles,
                                                         uint32_t ntiles, int fi, int p, int c,
                                                         uint32_t cap, int snap) {
-  __shared__ uint32_t h_key[PF_TU_HASH];    // bucket + 1, 0 = empty
-  __shared__ uint32_t h_off[PF_TU_HASH];    // (union offset << 16) | bucket count
-  __shared__ uint32_t h_start[PF_TU_HASH];  // first sorted slot of the bucket
+  __shared__ uint32_t h_tab[3 * PF_TU_HASH];
+  uint32_t* h_key = h_tab;                   // bucket + 1, 0 = empty
+  uint32_t* h_off = h_tab + PF_TU_HASH;      // (union offset << 16) | bucket count
+  uint32_t* h_start = h_tab + 2 * PF_TU_HASH;  // first sorted slot of the bucket
   // The staged union's positions, x, y and z arrays of cap entries (dynamic
   // LDS, 12 bytes per entry and at least 4 KB); before staging the same words
   // hold each thread's query cell and activity for the leader test.
@@ -717,9 +849,20 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
     static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
     if (global_walk) {
-      count = n < (1u << 22)
-                  ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over)
-                  : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active, leader, q, xi, s, &over);
+      // A valid layout (no bucket-set overflow) of at most 2^16 entries holds
+      // every particle the lists can name: the walk also writes their layout
+      // indices, and the compaction numbers them in layout order.
+      const bool ulist = U <= 0xffffu;  // uniform over the block
+      if (ulist)
+        count = n < (1u << 22) ? walk_runs<false, false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks,
+                                                               active, leader, q, xi, s, &over)
+                               : walk_runs<false, true, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks,
+                                                              active, leader, q, xi, s, &over);
+      else
+        count = n < (1u << 22) ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active,
+                                                         leader, q, xi, s, &over)
+                               : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active,
+                                                        leader, q, xi, s, &over);
       if (active) {
         F.nl_count[s] = count;
         if (want_normal) {
@@ -733,8 +876,13 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       // The passes after this one only need the particles the lists name: if
       // those fit, they become the tile's union and the lists are rewritten as
       // 16-bit indices into it, so the tile is staged in LDS like the others.
-      const uint32_t Uc = compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl,
-                                        active ? count : 0u, s);
+      // h_off and h_start (2 * PF_TU_HASH words: 2^16 bits, or PF_TU_MAX
+      // sort keys) are free after the walk.
+      static_assert(2 * PF_TU_HASH >= PF_TU_MAX && 2 * PF_TU_HASH * 32 >= 65536, "compaction space");
+      const uint32_t Uc =
+          ulist ? compact_layout(S, F, tile, h_off, reinterpret_cast<uint16_t*>(u_words), h_key, U,
+                                 min(F.tu_limit, cap), ctl, active ? count : 0u, s)
+                : compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl, active ? count : 0u, s, h_off);
       if (tid == 0) F.tu_size[tile] = Uc;
       return;
     }
