@@ -285,47 +285,67 @@ static __device__ uint32_t compact_union(const SceneDev& S, const FluidDev& F, u
   return U;
 }
 
-// Union of the particles named by a global-walk tile's finished lists when
-// the walk also wrote them as 16-bit indices into the tile's bucket layout of
-// ulay entries (F.nl16; walk_runs ULIST). Every thread of the block calls it;
-// `count` is the thread's list length (0 without a list). The listed entries
-// are marked in an LDS bitmap over the layout (bitmap: ulay bits, at most
-// 2^16) and numbered in layout order by a prefix of the words' popcounts
-// (pre16: a 16-bit prefix per word; claim: PF_TU_MAX bits marking the union
-// entries whose slot is written), so the union is in bucket order, slots
-// ascending inside each bucket, and the passes that stage it read it
-// coalesced. When at most `limit` entries are listed they become the tile's
-// union (F.tu_slots) and each list is rewritten as 16-bit union indices; the
-// union size is returned. Otherwise the lists keep their global slots (F.nl)
-// and PF_TU_GLOBAL is returned.
-static __device__ uint32_t compact_layout(const SceneDev& S, const FluidDev& F, uint32_t tile, uint32_t* bitmap,
-                                          uint16_t* pre16, uint32_t* claim, uint32_t ulay, uint32_t limit,
-                                          uint32_t* ctl, uint32_t count, uint32_t s) {
+// Union of the particles named by a global-walk tile's finished lists (global
+// slots, F.nl) when the tile's bucket layout survived the walk (h_key, h_off,
+// h_start; ulay <= 2^16 entries): a listed slot j lies in bucket tmp_key[j],
+// at layout index (its union offset) + (j - its first slot). Every thread of
+// the block calls it; `count` is the thread's list length (0 without a
+// list). The listed entries are marked in an LDS bitmap over the layout and
+// numbered in layout order by a prefix of the words' popcounts, so the union
+// is in bucket order, slots ascending inside each bucket, and the passes that
+// stage it read it coalesced. lds holds the bitmap (nw words), a 16-bit
+// prefix per word and PF_TU_MAX claim bits (compact_layout_words). When at
+// most `limit` entries are listed they become the tile's union (F.tu_slots,
+// each slot written by the first list entry to claim it) and each list is
+// rewritten as 16-bit union indices (F.nl16); the union size is returned.
+// Otherwise the lists keep their global slots and PF_TU_GLOBAL is returned.
+static __device__ __forceinline__ uint32_t compact_layout_words(uint32_t ulay) {
+  const uint32_t nw = (ulay + 31u) >> 5;
+  return nw + (nw + 1u) / 2u + PF_TU_MAX / 32u;
+}
+static __device__ uint32_t compact_layout(const SceneDev& S, const FluidDev& F, uint32_t tile, const uint32_t* h_key,
+                                          const uint32_t* h_off, const uint32_t* h_start, uint32_t ulay,
+                                          uint32_t* lds, uint32_t limit, uint32_t* ctl, uint32_t count, uint32_t s) {
   const uint32_t tid = threadIdx.x;
   const uint32_t n = S.n;
   const uint32_t nw = (ulay + 31u) >> 5;
-  constexpr uint32_t WPT = 2048u / PF_BLOCK;  // bitmap words per thread (2^16 bits)
-  __syncthreads();  // every wave has left the walk (its marks share the bitmap's words)
+  uint32_t* bitmap = lds;
+  uint16_t* pre16 = reinterpret_cast<uint16_t*>(lds + nw);
+  uint32_t* claim = lds + nw + (nw + 1u) / 2u;
+  const uint32_t ul = ulay ? ulay - 1u : 0u;
+  __syncthreads();  // every wave has left the walk (its marks share these words)
   for (uint32_t w = tid; w < nw; w += PF_BLOCK) bitmap[w] = 0u;
   if (tid < PF_TU_MAX / 32) claim[tid] = 0u;
   __syncthreads();
-  const PF_G uint32_t* __restrict__ row16 = pf_g(F.nl16) + s;
-  constexpr uint32_t rmax = PF_MAX_NEIGHBORS / 2 - 1;
-  const uint32_t ul = ulay ? ulay - 1u : 0u;
-  // Four list words (eight entries) per round trip; entries past the count skipped.
+  const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
+  const PF_G uint32_t* __restrict__ g_key = pf_g(S.tmp_key);
+  // Layout index of a listed slot.
+  auto layout_index = [&](uint32_t j, uint32_t b) {
+    uint32_t h = tu_hash(b);
+    for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
+    return min((h_off[h] >> 16) + (j - h_start[h]), ul);  // the walk's buckets are all in the set
+  };
+  // Eight list entries per round trip, their buckets' loads in flight
+  // together; the layout indices are kept in the thread's own F.nl16 column
+  // (rows written across the wave's lanes at once) for the numbering pass.
+  PF_G uint32_t* __restrict__ out16 = pf_g(F.nl16) + s;
   for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
-    uint32_t w[4];
+    uint32_t js[8], bs[8], us[8];
 #pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) w[i] = row16[(size_t)min((k0 >> 1) + i, rmax) * n];
+    for (uint32_t e = 0; e < 8; ++e) js[e] = col[(size_t)min(k0 + e, count - 1u) * n];
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) bs[e] = g_key[js[e]];
 #pragma unroll
     for (uint32_t e = 0; e < 8; ++e) {
-      if (k0 + e < count) {
-        const uint32_t u = min((w[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu, ul);
-        atomicOr(bitmap + (u >> 5), 1u << (u & 31u));
-      }
+      us[e] = layout_index(js[e], bs[e]);
+      if (k0 + e < count) atomicOr(bitmap + (us[e] >> 5), 1u << (us[e] & 31u));
     }
+#pragma unroll
+    for (uint32_t e = 0; e < 8; e += 2)
+      if (k0 + e < count) out16[(size_t)((k0 + e) >> 1) * n] = us[e] | (us[e + 1] << 16);
   }
   __syncthreads();
+  constexpr uint32_t WPT = 2048u / PF_BLOCK;  // bitmap words per thread (2^16 bits)
   const uint32_t w0 = tid * WPT;
   uint32_t pc[WPT], sum = 0;
 #pragma unroll
@@ -346,15 +366,13 @@ static __device__ uint32_t compact_layout(const SceneDev& S, const FluidDev& F, 
   // Each entry's union index; the first list entry to claim a union entry
   // writes its slot.
   PF_G uint32_t* __restrict__ slots = pf_g(F.tu_slots) + (size_t)tile * PF_TU_MAX;
-  const PF_G uint32_t* __restrict__ col = pf_g(F.nl) + s;
-  PF_G uint32_t* __restrict__ out16 = pf_g(F.nl16) + s;
+  constexpr uint32_t rmax = PF_MAX_NEIGHBORS / 2 - 1;
   for (uint32_t k0 = 0; k0 < count; k0 += 8u) {
-    uint32_t w[4], js[8];
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) w[i] = row16[(size_t)min((k0 >> 1) + i, rmax) * n];
+    uint32_t js[8], w[4], idx[8];
 #pragma unroll
     for (uint32_t e = 0; e < 8; ++e) js[e] = col[(size_t)min(k0 + e, count - 1u) * n];
-    uint32_t idx[8];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) w[i] = out16[(size_t)min((k0 >> 1) + i, rmax) * n];
 #pragma unroll
     for (uint32_t e = 0; e < 8; ++e) {
       const uint32_t u = min((w[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu, ul);
@@ -427,9 +445,7 @@ static __device__ __forceinline__ uint32_t wave_inclusive_sum64(uint32_t v) {
 // every offset within 32 bits). marks: 64 * PF_RUN_CHUNKS words of LDS owned
 // by the wave. Returns the lane's list length; *over is set where a particle
 // has more than PF_MAX_NEIGHBORS neighbours (its list keeps the first ones).
-// ULIST (global walk of a tile whose bucket layout is valid, h_key/h_off):
-// every entry is also written as its 16-bit layout index into F.nl16.
-template <bool STAGED, bool WIDE, bool ULIST = false>
+template <bool STAGED, bool WIDE>
 static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const FluidDev& F, const float4* __restrict__ xw,
                                      const float* u_px, const float* u_py, const float* u_pz,
                                      const uint32_t* h_key, const uint32_t* h_off, uint32_t* marks,
@@ -462,10 +478,10 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
     const uint32_t q2 = (uint32_t)__builtin_amdgcn_readlane(q[2], a);
     // The stencil buckets' entry ranges, their prefix over the walk order and
     // the marks' keys.
-    uint32_t cn = 0, base = 0, udelta = 0;
+    uint32_t cn = 0, base = 0;
     if (lane < 27u) {
       const uint32_t b = pf_morton((int)(q0 + sdx), (int)(q1 + sdy), (int)(q2 + sdz)) & S.hash_mask;
-      if (STAGED || ULIST) {
+      if (STAGED) {
         uint32_t h = tu_hash(b);
         for (uint32_t probe = 0; probe < PF_TU_HASH && h_key[h] != b + 1u; ++probe) h = (h + 1u) & (PF_TU_HASH - 1u);
         if (h_key[h] == b + 1u) {  // always: the cell's leader inserted b
@@ -473,12 +489,9 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
           base = ho >> 16;
           cn = ho & 0xffffu;
         }
-      }
-      if (!STAGED) {
-        const uint32_t first = S.start[b];
-        if (ULIST) udelta = base - first;  // layout index minus sorted slot, for the bucket's entries
-        base = first;
-        cn = S.start[b + 1] - first;
+      } else {
+        base = S.start[b];
+        cn = S.start[b + 1] - base;
       }
     }
     const uint32_t inc = wave_inclusive_sum64(cn);
@@ -489,7 +502,7 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
     uint32_t carry = 0;
     for (uint32_t k0 = 0; k0 < C && live != 0ull; k0 += 64u * K) {
       float3 xj[K];
-      uint32_t ej[K], uj[K];
+      uint32_t ej[K];
 #pragma unroll
       for (uint32_t j = 0; j < K; ++j) {
         const uint32_t c0 = k0 + 64u * j;
@@ -504,11 +517,6 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
         const bool valid = kk < C;
         const uint32_t e = valid ? kk + (v & 0x7ffffffu) - (1u << 26) : 0u;
         ej[j] = e;
-        if (ULIST) {
-          // The candidate's bucket is stencil lane (v >> 27) - 1.
-          const uint32_t st = ((v >> 27) - 1u) & 63u;
-          uj[j] = e + (uint32_t)__builtin_amdgcn_ds_bpermute((int)(st << 2), (int)udelta);
-        }
         float3 p;
         if (STAGED) {
           p = make_float3(u_px[e], u_py[e], u_pz[e]);
@@ -529,7 +537,6 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
         uint32_t cnt = (uint32_t)__builtin_amdgcn_readlane(count, i);
         // The particle's list column, word rows n apart.
         PF_G char* const col = STAGED ? (PF_G char*)(nl16 + s0 + i) : (PF_G char*)(nl + s0 + i);
-        PF_G char* const col16 = (PF_G char*)(nl16 + s0 + i);
         // cnt counts every neighbour found; entries past PF_MAX_NEIGHBORS are
         // not stored, and a list that passes it closes after this block.
 #pragma unroll
@@ -549,11 +556,6 @@ static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const Fl
               } else {
                 const uint32_t off = WIDE ? pos * (n * 4u) : __umul24(pos, n * 4u);
                 *(PF_G uint32_t*)(col + off) = ej[j];
-                if (ULIST) {
-                  const uint32_t off16 =
-                      (WIDE ? (pos >> 1) * (n * 4u) : __umul24(pos >> 1, n * 4u)) + ((pos & 1u) << 1);
-                  *(PF_G uint16_t*)(col16 + off16) = (uint16_t)uj[j];
-                }
               }
             }
           }
@@ -845,24 +847,16 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     // capacity walks global memory.
     const bool global_walk = U > min(F.tu_limit, cap);
 
-    // h_start is free once the layout is read: each wave's run-walk marks.
+    // The run-walk marks, per wave: in h_start (free once the layout is read)
+    // for the staged walk; in the union's LDS for the global walk, which keeps
+    // the layout for its compaction.
     uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
     static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
     if (global_walk) {
-      // A valid layout (no bucket-set overflow) of at most 2^16 entries holds
-      // every particle the lists can name: the walk also writes their layout
-      // indices, and the compaction numbers them in layout order.
-      const bool ulist = U <= 0xffffu;  // uniform over the block
-      if (ulist)
-        count = n < (1u << 22) ? walk_runs<false, false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks,
-                                                               active, leader, q, xi, s, &over)
-                               : walk_runs<false, true, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks,
-                                                              active, leader, q, xi, s, &over);
-      else
-        count = n < (1u << 22) ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active,
-                                                         leader, q, xi, s, &over)
-                               : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, marks, active,
-                                                        leader, q, xi, s, &over);
+      uint32_t* gmarks = u_words + (tid >> 6) * (64u * PF_RUN_CHUNKS);  // the union's LDS holds 4 KB at least
+      count = n < (1u << 22)
+                  ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active, leader, q, xi, s, &over)
+                  : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active, leader, q, xi, s, &over);
       if (active) {
         F.nl_count[s] = count;
         if (want_normal) {
@@ -876,13 +870,16 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
       // The passes after this one only need the particles the lists name: if
       // those fit, they become the tile's union and the lists are rewritten as
       // 16-bit indices into it, so the tile is staged in LDS like the others.
-      // h_off and h_start (2 * PF_TU_HASH words: 2^16 bits, or PF_TU_MAX
-      // sort keys) are free after the walk.
-      static_assert(2 * PF_TU_HASH >= PF_TU_MAX && 2 * PF_TU_HASH * 32 >= 65536, "compaction space");
+      // A valid layout numbers them in layout order; a bucket set that
+      // overflowed, or a layout too large for the LDS, sorts a hash table's.
+      const uint32_t lds_words = (uint32_t)max((size_t)cap * 3u, (size_t)PF_BLOCK * 4u);
+      const bool by_layout = U <= 0xffffu && compact_layout_words(U) <= lds_words;  // uniform
+      // h_off and h_start (2 * PF_TU_HASH >= PF_TU_MAX words) sort the hash table's keys.
+      static_assert(2 * PF_TU_HASH >= PF_TU_MAX, "compaction sort space");
       const uint32_t Uc =
-          ulist ? compact_layout(S, F, tile, h_off, reinterpret_cast<uint16_t*>(u_words), h_key, U,
-                                 min(F.tu_limit, cap), ctl, active ? count : 0u, s)
-                : compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl, active ? count : 0u, s, h_off);
+          by_layout ? compact_layout(S, F, tile, h_key, h_off, h_start, U, u_words, min(F.tu_limit, cap), ctl,
+                                     active ? count : 0u, s)
+                    : compact_union(S, F, tile, u_words, cap, min(F.tu_limit, cap), ctl, active ? count : 0u, s, h_off);
       if (tid == 0) F.tu_size[tile] = Uc;
       return;
     }
