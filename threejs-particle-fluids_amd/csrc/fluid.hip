@@ -442,16 +442,16 @@ static __device__ __forceinline__ uint32_t wave_inclusive_sum64(uint32_t v) {
 // entries union indices, written as 16-bit halves of F.nl16's words; else
 // candidates are read from xw and the entries are sorted slots in F.nl.
 // WIDE: row offsets need full 32-bit products (n >= 2^22; n <= 2^24 keeps
-// every offset within 32 bits). marks: 64 * PF_RUN_CHUNKS words of LDS owned
+// every offset within 32 bits). K: chunks of 64 candidates loaded at once.
+// marks: 64 * K words of LDS owned
 // by the wave. Returns the lane's list length; *over is set where a particle
 // has more than PF_MAX_NEIGHBORS neighbours (its list keeps the first ones).
-template <bool STAGED, bool WIDE>
+template <bool STAGED, bool WIDE, uint32_t K = PF_RUN_CHUNKS>
 static __device__ __forceinline__ uint32_t walk_runs(const SceneDev& S, const FluidDev& F, const float4* __restrict__ xw,
                                      const float* u_px, const float* u_py, const float* u_pz,
                                      const uint32_t* h_key, const uint32_t* h_off, uint32_t* marks,
                                      bool active, bool leader, const int q[3], float3 xi, uint32_t s,
                                      bool* over) {
-  constexpr uint32_t K = PF_RUN_CHUNKS;
   const uint32_t lane = __lane_id();
   const uint32_t n = S.n;
   const float h2 = F.h2;
@@ -853,10 +853,21 @@ __global__ __launch_bounds__(PF_BLOCK) void k_neighbors(const SceneDev* __restri
     uint32_t* marks = h_start + (tid >> 6) * (64u * PF_RUN_CHUNKS);
     static_assert(PF_BLOCK / 64 * 64 * PF_RUN_CHUNKS <= PF_TU_HASH, "run-walk marks exceed h_start");
     if (global_walk) {
-      uint32_t* gmarks = u_words + (tid >> 6) * (64u * PF_RUN_CHUNKS);  // the union's LDS holds 4 KB at least
-      count = n < (1u << 22)
-                  ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active, leader, q, xi, s, &over)
-                  : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active, leader, q, xi, s, &over);
+      // Global candidates arrive with a global load's latency: twice the
+      // chunks in flight where the union's LDS holds their marks (from a
+      // capacity of 683; 4 KB, the least it holds, fits the default chunks).
+      const bool deep = cap * 3u >= (uint32_t)(PF_BLOCK / 64) * 64u * (2u * PF_RUN_CHUNKS);  // uniform
+      uint32_t* gmarks = u_words + (tid >> 6) * (64u * (deep ? 2u * PF_RUN_CHUNKS : PF_RUN_CHUNKS));
+      if (deep)
+        count = n < (1u << 22) ? walk_runs<false, false, 2 * PF_RUN_CHUNKS>(S, F, xw, nullptr, nullptr, nullptr, h_key,
+                                                                            h_off, gmarks, active, leader, q, xi, s, &over)
+                               : walk_runs<false, true, 2 * PF_RUN_CHUNKS>(S, F, xw, nullptr, nullptr, nullptr, h_key,
+                                                                           h_off, gmarks, active, leader, q, xi, s, &over);
+      else
+        count = n < (1u << 22) ? walk_runs<false, false>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active,
+                                                         leader, q, xi, s, &over)
+                               : walk_runs<false, true>(S, F, xw, nullptr, nullptr, nullptr, h_key, h_off, gmarks, active,
+                                                        leader, q, xi, s, &over);
       if (active) {
         F.nl_count[s] = count;
         if (want_normal) {
