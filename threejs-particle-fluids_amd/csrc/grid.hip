@@ -307,15 +307,32 @@ __global__ __launch_bounds__(PF_BLOCK) void k_rank(const SceneDev* __restrict__ 
     const uint32_t s = t.first + threadIdx.x;
     if (s >= S.n) return;
     const uint32_t b = S.tmp_key[s];
-    const uint32_t st = S.start[b];
-    const uint32_t en = S.start[b + 1];
     const uint32_t* cmp = (MODE == 0) ? S.tmp_src : S.tmp_id;
     const uint32_t my = cmp[s];
     const uint32_t src = S.tmp_src[s];
+    // The particle's state is gathered while the rank is counted.
+    float4 x, v, xsp;
+    float w = 0.0f;
+    if (MODE == 1) {
+      x = S.x[parity][src];
+      v = S.v[parity][src];
+      w = S.w[parity][src];
+      xsp = S.xs[xs_prev][src];
+    }
+    const uint32_t st = S.start[b];
+    const uint32_t en = S.start[b + 1];
     uint32_t dest;
     if (en - st <= (uint32_t)PF_STABLE_SEGMENT_LIMIT) {
+      // Eight of the bucket's keys in flight per round trip.
+      const PF_G uint32_t* __restrict__ g_cmp = pf_g(cmp);
       uint32_t r = 0;
-      for (uint32_t q = st; q < en; ++q) r += (cmp[q] < my) ? 1u : 0u;
+      for (uint32_t q0 = st; q0 < en; q0 += 8u) {
+        uint32_t k[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) k[u] = g_cmp[min(q0 + u, en - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) r += (q0 + u < en && k[u] < my) ? 1u : 0u;
+      }
       dest = st + r;
     } else {
       dest = s;
@@ -327,10 +344,7 @@ __global__ __launch_bounds__(PF_BLOCK) void k_rank(const SceneDev* __restrict__ 
       if (out_spos) out_spos[dest] = S.x[0][src];
     } else {
       const int q = parity ^ 1;
-      const float4 x = S.x[parity][src];
-      const float4 v = S.v[parity][src];
-      const float w = S.w[parity][src];
-      const float4 xsp = (w > 0.0f) ? make_float4(0.f, 0.f, 0.f, 0.f) : S.xs[xs_prev][src];
+      if (w > 0.0f) xsp = make_float4(0.f, 0.f, 0.f, 0.f);
       float4 vel, xs;
       predict(S, x, v, w, xsp, &vel, &xs);
       S.x[q][dest] = x;
