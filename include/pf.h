@@ -157,7 +157,7 @@ int pf_grid_destroy(pf_grid* grid);
  * with PF_EINVAL. Results equal the standalone grid's bit for bit. */
 #define PF_NOALLOC_MAX_PARTICLES (1u << 22)
 #define PF_NOALLOC_MAX_BUCKETS (1u << 22)
-#define PF_NOALLOC_SCRATCH_BYTES 50482688u
+#define PF_NOALLOC_SCRATCH_BYTES 51389184u
 int pf_grid_sort_noalloc(pf_context* ctx, const float* positions4, uint32_t n, double cell_size,
                          uint32_t hash_table_size, const double origin[3], uint32_t* cell_index,
                          uint32_t* counts, uint32_t* cell_start, uint32_t* cell_end,

@@ -55,7 +55,8 @@ def test_sort_small_tables_and_ragged_sizes():
 
 
 def test_sort_dense_bucket_and_repeated_rebuilds():
-    # every point in one cell: a 3000-long bucket, then 50 rebuilds identical (determinism tier 1)
+    # every point in one cell: a 3000-long bucket, stable (ascending index) like
+    # every other bucket, then 50 rebuilds identical (determinism tier 1)
     pos = np.zeros((3000, 4), np.float32)
     pos[:, :3] = 0.005 + scenes.lcg_uniform(9, 9000).reshape(3000, 3).astype(np.float32) * 0.004
     ref = orc.grid_build(pos, 0.01, 4096)
@@ -68,6 +69,8 @@ def test_sort_dense_bucket_and_repeated_rebuilds():
         if first is None:
             first = s
             assert np.array_equal(s.cellStart, ref["cellStart"])
+            assert np.array_equal(s.sortedIndices, ref["sortedIndices"])
+        assert np.array_equal(s.sortedIndices, first.sortedIndices)
         assert np.array_equal(s.cellStart, first.cellStart)
         assert np.array_equal(s.counts, first.counts)
         assert np.array_equal(np.sort(s.sortedIndices), np.arange(3000))

@@ -1,16 +1,16 @@
-// Hash-grid counting sort that allocates nothing: pf_grid_sort_noalloc.
+// Hash-grid sort that allocates nothing: pf_grid_sort_noalloc.
 //
-// The same four sort kernels as pf_grid_rebuild over a standalone grid
-// (k_hash, k_scan, k_place, k_rank in grid.hip), for callers that cannot
-// allocate device memory (code under HIP graph capture, tight serving loops).
-// Inputs and outputs are the caller's device buffers; everything else the
-// kernels need (the grid descriptor, the work tiles, arrival ranks, the
-// arrival-order scatter, the bucket offsets, the scan's status words and
-// ticket, the overflow flags) lives in one file-scope device array,
-// `scratch`, of fixed size. Each GPU holds its own copy of that array until
-// the process ends; its address on a device is taken once, with that device
-// current, and every launch is given that address. Calls that overlap on one
-// GPU share the array and race; successive calls on one stream do not.
+// The same radix-sort kernels as pf_grid_rebuild over a standalone grid
+// (launch_grid_radix in grid.hip), for callers that cannot allocate device
+// memory (code under HIP graph capture, tight serving loops). Inputs and
+// outputs are the caller's device buffers; everything else the kernels need
+// (the grid descriptor, the overflow flags, the passes' other key and index
+// buffers, the bucket offsets, the digit histograms and range bounds) lives
+// in one file-scope device array, `scratch`, of fixed size. Each GPU holds
+// its own copy of that array until the process ends; its address on a device
+// is taken once, with that device current, and every launch is given that
+// address. Calls that overlap on one GPU share the array and race; successive
+// calls on one stream do not.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -25,21 +25,17 @@ constexpr uint32_t kMaxH = PF_NOALLOC_MAX_BUCKETS;
 constexpr size_t kAlign = 256;
 constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
-constexpr uint32_t kMaxTiles = kMaxN / PF_BLOCK;
-constexpr uint32_t kMaxScanTiles = kMaxH / PF_SCAN_TILE;
-
 // Byte offsets inside the array.
 constexpr size_t kDesc = 0;
-constexpr size_t kCtl = align_up(kDesc + sizeof(SceneDev));  // scan ticket (16 B), scan_ctl (8 B), flags
-constexpr size_t kCtlBytes = 64 + 4 * 2 * PF_NUM_FLAGS;
-constexpr size_t kStatus = align_up(kCtl + kCtlBytes);
-constexpr size_t kTiles = align_up(kStatus + 8 * (size_t)kMaxScanTiles);
-constexpr size_t kSTiles = align_up(kTiles + sizeof(Tile) * (size_t)kMaxTiles);
-constexpr size_t kRank = align_up(kSTiles + sizeof(uint2) * (size_t)kMaxScanTiles);
-constexpr size_t kSrc = align_up(kRank + 4 * (size_t)kMaxN);
-constexpr size_t kStart = align_up(kSrc + 4 * (size_t)kMaxN);
-constexpr size_t kTotal = align_up(kStart + 4 * ((size_t)kMaxH + 1));
+constexpr size_t kFlags = align_up(kDesc + sizeof(SceneDev));
+constexpr size_t kTmpKey = align_up(kFlags + 4 * 2 * PF_NUM_FLAGS);
+constexpr size_t kTmpSrc = align_up(kTmpKey + 4 * (size_t)kMaxN);
+constexpr size_t kStart = align_up(kTmpSrc + 4 * (size_t)kMaxN);
+constexpr size_t kRadix = align_up(kStart + 4 * ((size_t)kMaxH + 1));
+constexpr size_t kTotal = align_up(kRadix + 4 * grid_radix_scratch_words(kMaxN, kMaxH));
 static_assert(kTotal == PF_NOALLOC_SCRATCH_BYTES, "scratch size and its documented value differ");
+// the descriptor travels by value as a kernel argument (at most 4 KiB of them)
+static_assert(sizeof(SceneDev) + sizeof(void*) <= 4096, "descriptor too large for a kernel argument");
 
 }  // namespace
 
@@ -47,52 +43,12 @@ static __device__ __align__(256) unsigned char scratch[kTotal];
 
 namespace {
 
-// Writes the grid descriptor (block 0) and the work tiles, and clears the
-// histogram. One launch ahead of the sort kernels on the same stream.
-__global__ __launch_bounds__(PF_BLOCK) void k_noalloc_setup(unsigned char* base, const float4* positions,
-                                                            uint32_t n, uint32_t H, float cell, float ox,
-                                                            float oy, float oz, uint32_t* key,
-                                                            uint32_t* counts, uint32_t* sorted_keys) {
-  const uint32_t ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
-  const uint32_t nstiles = (H + PF_SCAN_TILE - 1) / PF_SCAN_TILE;
-  if (blockIdx.x == 0) {
-    uint32_t* words = reinterpret_cast<uint32_t*>(base + kDesc);
-    for (uint32_t i = threadIdx.x; i < sizeof(SceneDev) / 4; i += PF_BLOCK) words[i] = 0u;
-    uint32_t* flags = reinterpret_cast<uint32_t*>(base + kCtl + 64);
-    for (uint32_t i = threadIdx.x; i < 2 * PF_NUM_FLAGS; i += PF_BLOCK) flags[i] = 0u;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      SceneDev* S = reinterpret_cast<SceneDev*>(base + kDesc);
-      S->n = n;
-      S->H = H;
-      S->hash_mask = H - 1u;
-      S->nscan_tiles = nstiles;
-      S->cell = cell;
-      S->ox = ox;
-      S->oy = oy;
-      S->oz = oz;
-      S->x[0] = const_cast<float4*>(positions);
-      S->key = key;
-      S->rank = reinterpret_cast<uint32_t*>(base + kRank);
-      S->counts = counts;
-      S->start = reinterpret_cast<uint32_t*>(base + kStart);
-      S->scan_status = reinterpret_cast<unsigned long long*>(base + kStatus);
-      S->scan_ctl = reinterpret_cast<uint32_t*>(base + kCtl + 16);
-      S->tmp_src = reinterpret_cast<uint32_t*>(base + kSrc);
-      S->tmp_key = sorted_keys;  // k_rank reads entry s and writes the sorted key s in place
-      S->flags = flags;
-    }
-  }
-  Tile* tiles = reinterpret_cast<Tile*>(base + kTiles);
-  uint2* stiles = reinterpret_cast<uint2*>(base + kSTiles);
-  const uint32_t stride = gridDim.x * PF_BLOCK;
-  for (uint32_t i = blockIdx.x * PF_BLOCK + threadIdx.x; i < H; i += stride) {
-    counts[i] = 0u;
-    if (i < ntiles) tiles[i] = Tile{0u, i * PF_BLOCK};
-    if (i < nstiles) stiles[i] = make_uint2(0u, i);
-  }
-  for (uint32_t i = H + blockIdx.x * PF_BLOCK + threadIdx.x; i < ntiles; i += stride)
-    tiles[i] = Tile{0u, i * PF_BLOCK};
+// The descriptor's device copy, with every pointer into the caller's buffers
+// or the array, and cleared overflow flags. One launch ahead of the sort.
+__global__ __launch_bounds__(PF_BLOCK) void k_noalloc_setup(unsigned char* base, SceneDev desc) {
+  uint32_t* flags = reinterpret_cast<uint32_t*>(base + kFlags);
+  for (uint32_t i = threadIdx.x; i < 2 * PF_NUM_FLAGS; i += PF_BLOCK) flags[i] = 0u;
+  if (threadIdx.x == 0) *reinterpret_cast<SceneDev*>(base + kDesc) = desc;
 }
 
 // HashGridSnapshot ranges from the bucket offsets (HashGrid.ts:55-64,
@@ -191,19 +147,23 @@ int pf_grid_sort_noalloc(pf_context* ctx, const float* positions4, uint32_t n, d
   } restore{ctx->device, prev};
   unsigned char* base = scratch_on(ctx->device);
   hipStream_t st = ctx->stream;
+  SceneDev desc{};
+  desc.n = n;
+  desc.H = H;
+  desc.hash_mask = H - 1u;
+  desc.cell = (float)cell_size;
+  desc.ox = (float)o[0];
+  desc.oy = (float)o[1];
+  desc.oz = (float)o[2];
+  desc.x[0] = const_cast<float4*>(reinterpret_cast<const float4*>(positions4));
+  desc.key = cell_index;
+  desc.start = reinterpret_cast<uint32_t*>(base + kStart);
+  desc.tmp_key = reinterpret_cast<uint32_t*>(base + kTmpKey);
+  desc.tmp_src = reinterpret_cast<uint32_t*>(base + kTmpSrc);
+  desc.flags = reinterpret_cast<uint32_t*>(base + kFlags);
   const SceneDev* S = reinterpret_cast<const SceneDev*>(base + kDesc);
-  Tile* tiles = reinterpret_cast<Tile*>(base + kTiles);
-  uint2* stiles = reinterpret_cast<uint2*>(base + kSTiles);
-  const uint32_t ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
-  const uint32_t nstiles = (H + PF_SCAN_TILE - 1) / PF_SCAN_TILE;
-  const uint32_t fill_blocks = std::min<uint32_t>((std::max(H, ntiles) + PF_BLOCK - 1) / PF_BLOCK, 2048u);
-  hipLaunchKernelGGL(k_noalloc_setup, dim3(fill_blocks), dim3(PF_BLOCK), 0, st, base,
-                     reinterpret_cast<const float4*>(positions4), n, H, (float)cell_size, (float)o[0],
-                     (float)o[1], (float)o[2], cell_index, counts, sorted_keys);
-  launch_hash(st, S, tiles, ntiles, 0, 0, 0, 0);
-  launch_scan(st, S, stiles, nstiles, base + kCtl);
-  launch_place(st, S, tiles, ntiles, 0, 0);
-  launch_rank(st, S, tiles, ntiles, 0, 0, 0, 0, sorted_indices, sorted_keys, nullptr);
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(PF_BLOCK), 0, st, base, desc);
+  launch_grid_radix(st, S, desc, sorted_keys, sorted_indices, nullptr, reinterpret_cast<uint32_t*>(base + kRadix));
   hipLaunchKernelGGL(k_noalloc_ranges, dim3(std::min<uint32_t>((Hpad + PF_BLOCK - 1) / PF_BLOCK, 2048u)),
                      dim3(PF_BLOCK), 0, st, base, Hpad, counts, cell_start, cell_end, overflow);
   PF_HIP(hipGetLastError());

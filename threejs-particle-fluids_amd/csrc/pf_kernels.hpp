@@ -43,16 +43,29 @@ void launch_tiles(void (*one)(P...), void (*many)(P...), uint32_t ntiles, size_t
       vals);
 }
 
-// grid.hip
-void launch_hash(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
-                 int mode, int parity, int xs_prev, int reset_step_flags);
+// grid.hip: a scene substep's counting sort (predict + hash, scan, place,
+// stable rank + state gather from parity into parity ^ 1)
+void launch_hash(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles, int parity,
+                 int xs_prev, int reset_step_flags);
 void launch_scan(hipStream_t st, const SceneDev* scenes, const uint2* stiles, uint32_t nstiles,
                  void* ctl);
-void launch_place(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
-                  int mode, int parity);
-void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles,
-                 int mode, int parity, int xs_prev, int xs_out, uint32_t* out_perm,
-                 uint32_t* out_skey, float4* out_spos);
+void launch_place(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles, int parity);
+void launch_rank(hipStream_t st, const SceneDev* scenes, const Tile* tiles, uint32_t ntiles, int parity,
+                 int xs_prev, int xs_out);
+// The standalone grid's rebuild as an LSD radix sort of its x[0] positions:
+// key (cell index), start[0..H], sorted keys, permutation and (spos non-null)
+// sorted positions; tmp_key/tmp_src are the passes' other buffers. `scratch`
+// holds grid_radix_scratch_words(n, H) words: the per-block digit histograms
+// and the first sorted position of each offsets range.
+constexpr uint32_t RS_ITEMS = 16;                  // points per thread
+constexpr uint32_t RS_TILE = PF_BLOCK * RS_ITEMS;  // points per block (4096)
+constexpr uint32_t RS_BINS = 256;                  // 8-bit digits
+constexpr uint32_t RS_RANGE = 4096;                // buckets per offsets workgroup
+constexpr size_t grid_radix_scratch_words(uint32_t n, uint32_t H) {
+  return (size_t)RS_BINS * ((n + RS_TILE - 1) / RS_TILE) + RS_BINS + ((size_t)H + RS_RANGE - 1) / RS_RANGE + 1;
+}
+void launch_grid_radix(hipStream_t st, const SceneDev* dev, const SceneDev& host, uint32_t* skey,
+                       uint32_t* perm, float4* spos, uint32_t* scratch);
 
 // fluid.hip. `cap`: tile-union entries of dynamic LDS per workgroup
 // (PF_BLOCK..PF_TU_MAX); the neighbour build keeps unions within it.

@@ -949,7 +949,6 @@ static void grid_init(pf_grid* g, pf::Context* ctx, uint32_t n, double cell, uin
   S.n = n;
   S.H = H;
   S.hash_mask = H - 1;
-  S.nscan_tiles = (H + PF_SCAN_TILE - 1) / PF_SCAN_TILE;
   S.cell = (float)cell;
   S.ox = (float)g->origin[0];
   S.oy = (float)g->origin[1];
@@ -957,37 +956,23 @@ static void grid_init(pf_grid* g, pf::Context* ctx, uint32_t n, double cell, uin
   g->positions = dalloc<float4>(n);
   S.x[0] = g->positions;
   S.key = dalloc<uint32_t>(n);
-  S.rank = dalloc<uint32_t>(n);
-  S.counts = dalloc<uint32_t>(H);
   S.start = dalloc<uint32_t>((size_t)H + 1);
-  S.scan_status = dalloc<unsigned long long>(S.nscan_tiles);
-  S.scan_ctl = dalloc<uint32_t>(2);
   S.tmp_src = dalloc<uint32_t>(n);
   S.tmp_key = dalloc<uint32_t>(n);
   S.flags = dalloc<uint32_t>(2 * PF_NUM_FLAGS);
   g->perm = dalloc<uint32_t>(n);
   g->skey = dalloc<uint32_t>(n);
   g->spos = dalloc<float4>(n);
+  g->radix = dalloc<uint32_t>(grid_radix_scratch_words(n, H));
   g->dev = dalloc<SceneDev>(1);
   PF_HIP(hipMemcpy(g->dev, &S, sizeof(SceneDev), hipMemcpyHostToDevice));
-  g->ntiles = (n + PF_BLOCK - 1) / PF_BLOCK;
-  std::vector<Tile> tl(g->ntiles);
-  for (uint32_t i = 0; i < g->ntiles; ++i) tl[i] = Tile{0u, i * PF_BLOCK};
-  g->tiles = dalloc<Tile>(g->ntiles);
-  PF_HIP(hipMemcpy(g->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  g->nstiles = S.nscan_tiles;
-  std::vector<uint2> stl(g->nstiles);
-  for (uint32_t i = 0; i < g->nstiles; ++i) stl[i] = make_uint2(0u, i);
-  g->stiles = dalloc<uint2>(g->nstiles);
-  PF_HIP(hipMemcpy(g->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
 }
 
 static void grid_free(pf_grid* g) {
   if (g->owned_by_loop) return;
   SceneDev& S = g->host;
-  dfree(g->positions); dfree(S.key); dfree(S.rank); dfree(S.counts); dfree(S.start);
-  dfree(S.scan_status); dfree(S.scan_ctl); dfree(S.tmp_src); dfree(S.tmp_key); dfree(S.flags);
-  dfree(g->perm); dfree(g->skey); dfree(g->spos); dfree(g->dev); dfree(g->tiles); dfree(g->stiles);
+  dfree(g->positions); dfree(S.key); dfree(S.start); dfree(S.tmp_src); dfree(S.tmp_key); dfree(S.flags);
+  dfree(g->perm); dfree(g->skey); dfree(g->spos); dfree(g->radix); dfree(g->dev);
 }
 
 int pf_grid_create(pf_particles* ps, double cell_size, uint32_t hash_table_size,
@@ -1064,21 +1049,9 @@ int pf_grid_rebuild(pf_grid* g) {
     PF_HIP(hipMemsetAsync(g->host.flags, 0, sizeof(uint32_t) * 2 * PF_NUM_FLAGS, st));
   }
   {
-    PF_LAUNCH(ctx, "grid_hash");
-    launch_hash(st, g->dev, g->tiles, g->ntiles, 0, 0, 0, 0);
-  }
-  {
-    PF_LAUNCH(ctx, "grid_scan");
-    launch_scan(st, g->dev, g->stiles, g->nstiles, ctx->scan_ctl);
-  }
-  {
-    PF_LAUNCH(ctx, "grid_place");
-    launch_place(st, g->dev, g->tiles, g->ntiles, 0, 0);
-  }
-  {
-    PF_LAUNCH(ctx, "grid_rank");
-    launch_rank(st, g->dev, g->tiles, g->ntiles, 0, 0, 0, 0, g->perm, g->skey,
-                g->sorted_positions ? g->spos : nullptr);
+    // keys, radix passes and bucket offsets, timed as one sort
+    PF_LAUNCH(ctx, "grid_sort");
+    launch_grid_radix(st, g->dev, g->host, g->skey, g->perm, g->sorted_positions ? g->spos : nullptr, g->radix);
   }
   PF_HIP(hipGetLastError());
   g->built = true;

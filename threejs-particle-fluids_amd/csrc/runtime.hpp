@@ -179,10 +179,7 @@ struct pf_grid_s {
   bool built = false;
   SceneDev host{};
   SceneDev* dev = nullptr;
-  Tile* tiles = nullptr;
-  uint32_t ntiles = 0;
-  uint2* stiles = nullptr;
-  uint32_t nstiles = 0;
+  uint32_t* radix = nullptr;     // radix-sort block histograms and range bounds
   float4* positions = nullptr;   // [n] input positions (original order)
   uint32_t* perm = nullptr;      // [n] sorted indices
   uint32_t* skey = nullptr;      // [n] sorted keys

@@ -103,10 +103,10 @@ uint32_t enqueue_step(Context* ctx, const SceneDev* scenes, const PlanShape& sh,
     int p = 0;
     if (grid) {
       p = (int)(k & 1u);
-      { PF_LAUNCH(ctx, "hash_predict"); launch_hash(st, scenes, tiles, ntiles, 1, p, c, k == 0); ++nk; }
+      { PF_LAUNCH(ctx, "hash_predict"); launch_hash(st, scenes, tiles, ntiles, p, c, k == 0); ++nk; }
       { PF_LAUNCH(ctx, "scan"); launch_scan(st, scenes, stiles, nstiles, scan_ctl); ++nk; }
-      { PF_LAUNCH(ctx, "place"); launch_place(st, scenes, tiles, ntiles, 1, p); ++nk; }
-      { PF_LAUNCH(ctx, "rank_permute"); launch_rank(st, scenes, tiles, ntiles, 1, p, c, c ^ 1, nullptr, nullptr, nullptr); ++nk; }
+      { PF_LAUNCH(ctx, "place"); launch_place(st, scenes, tiles, ntiles, p); ++nk; }
+      { PF_LAUNCH(ctx, "rank_permute"); launch_rank(st, scenes, tiles, ntiles, p, c, c ^ 1); ++nk; }
       c ^= 1;
       p ^= 1;
       if (sh.contact) {
@@ -1645,10 +1645,10 @@ static void loop_static_boundaries(pf_loop* L) {
   ps->dirty = true;
   ps->upload_desc();
   const SceneDev* d = ps->cur();
-  launch_hash(st, d, L->tiles, L->ntiles, 1, 0, 0, 0);
+  launch_hash(st, d, L->tiles, L->ntiles, 0, 0, 0);
   launch_scan(st, d, L->stiles, L->nstiles, ctx->scan_ctl);
-  launch_place(st, d, L->tiles, L->ntiles, 1, 0);
-  launch_rank(st, d, L->tiles, L->ntiles, 1, 0, 0, 1, nullptr, nullptr, nullptr);
+  launch_place(st, d, L->tiles, L->ntiles, 0);
+  launch_rank(st, d, L->tiles, L->ntiles, 0, 0, 1);
   for (uint32_t f = 0; f < (uint32_t)L->fluids.size(); ++f)
     if (S.fluid[f].nboundary) launch_boundary_volume(st, d, L->tiles, L->ntiles, (int)f, 1, 1, 0);
   PF_HIP(hipGetLastError());
