@@ -122,13 +122,13 @@ def cpu_model():
 
 
 def run_sort(args, ctx):
-    """The hash-grid counting sort in isolation (HashGrid.rebuild over a
+    """The hash-grid sort in isolation (HashGrid.rebuild over a
     standalone grid): positions resident in HBM, outputs the keys, counts,
     offsets, sorted keys and permutation; both seeded distributions."""
     import particle_fluids as pf
     from particle_fluids import scenes
 
-    out = {"workload": f"hash-grid counting sort, {SORT_N} positions, cell {SORT_CELL}, {SORT_H} buckets",
+    out = {"workload": f"hash-grid sort (standalone rebuild), {SORT_N} positions, cell {SORT_CELL}, {SORT_H} buckets",
            "particles": SORT_N, "hash_table_size": SORT_H, "bytes_per_sort": SORT_BYTES, "reps": args.sort_reps}
     for dist in ("uniform", "blobs"):
         pos = scenes.sort_points(SORT_N, dist)

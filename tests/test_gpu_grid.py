@@ -1,4 +1,4 @@
-"""GPU hash-grid counting sort vs the oracle, bit-exact (keys, counts, offsets, sorted keys, permutation)."""
+"""GPU hash-grid sort vs the oracle, bit-exact (keys, counts, offsets, sorted keys, permutation)."""
 import numpy as np
 import pytest
 
