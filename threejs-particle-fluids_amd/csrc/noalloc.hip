@@ -48,7 +48,10 @@ namespace {
 __global__ __launch_bounds__(PF_BLOCK) void k_noalloc_setup(unsigned char* base, SceneDev desc) {
   uint32_t* flags = reinterpret_cast<uint32_t*>(base + kFlags);
   for (uint32_t i = threadIdx.x; i < 2 * PF_NUM_FLAGS; i += PF_BLOCK) flags[i] = 0u;
-  if (threadIdx.x == 0) *reinterpret_cast<SceneDev*>(base + kDesc) = desc;
+  static_assert(sizeof(SceneDev) % 4 == 0, "descriptor copied as words");
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&desc);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(base + kDesc);
+  for (uint32_t i = threadIdx.x; i < sizeof(SceneDev) / 4; i += PF_BLOCK) dst[i] = src[i];
 }
 
 // HashGridSnapshot ranges from the bucket offsets (HashGrid.ts:55-64,

@@ -339,10 +339,13 @@ struct pf_loop_s {
   bool disposed = false;
 };
 
-struct pf_batch_s {
-  pf::Context* ctx = nullptr;
-  std::vector<pf_loop_s*> loops;
-  SceneDev* dev = nullptr;
+// Scenes of a batch that share one substep schedule (their PlanShapes equal
+// but for the union capacity and launch shape), enqueued over their own tile
+// lists: a plain fluid scene keeps the fused schedule beside scenes whose soft
+// bodies or contacts unfuse theirs.
+struct BatchGroup {
+  std::vector<uint32_t> scenes;  // indices into pf_batch_s::loops
+  PlanShape shape;
   Tile* tiles = nullptr;
   uint32_t ntiles = 0;
   uint2* stiles = nullptr;
@@ -353,8 +356,15 @@ struct pf_batch_s {
   uint32_t nltiles = 0;
   Tile* ctiles = nullptr;
   uint32_t nctiles = 0;
+};
+
+struct pf_batch_s {
+  pf::Context* ctx = nullptr;
+  std::vector<pf_loop_s*> loops;
+  SceneDev* dev = nullptr;
+  std::vector<BatchGroup> groups;
   void* scan_ctl = nullptr;
-  PlanShape shape;  // shape.grid_cap: launch shape of the tile kernels (pf_batch_set_launch_shape)
+  PlanShape shape;  // every scene's merged; shape.grid_cap: launch shape of the tile kernels (pf_batch_set_launch_shape)
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
   bool graph_valid = false;

@@ -1988,19 +1988,53 @@ int pf_loop_destroy(pf_loop* L) {
   PF_API_END
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
 // Batched form
 // ---------------------------------------------------------------------------
+
+// Scenes whose substep schedules are the same (union capacity and launch
+// shape aside) share a batch group.
+static bool same_schedule(const PlanShape& a, const PlanShape& b) {
+  if (a.nfluids != b.nfluids || a.any_boundary != b.any_boundary || a.local_soft != b.local_soft ||
+      a.walls != b.walls || a.soft != b.soft || a.grid != b.grid || a.contact != b.contact)
+    return false;
+  for (uint32_t f = 0; f < PF_MAX_FLUIDS; ++f)
+    if (a.fx[f] != b.fx[f] || a.nb[f] != b.nb[f] || a.bdyn[f] != b.bdyn[f] || a.isv[f] != b.isv[f]) return false;
+  return true;
+}
+
+template <typename T>
+static T* upload_list(const std::vector<T>& v, uint32_t* count) {
+  *count = (uint32_t)v.size();
+  if (v.empty()) return nullptr;
+  T* d = dalloc<T>(v.size());
+  PF_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+static void batch_group_free(BatchGroup& g) {
+  dfree(g.tiles); dfree(g.stiles); dfree(g.btiles); dfree(g.ltiles); dfree(g.ctiles);
+  g = BatchGroup{};
+}
+
+extern "C" {
+
 int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch** out) {
   PF_API_BEGIN
   if (!ctx || !loops || n == 0) fail(PF_EINVAL, "Batch: at least one loop is required");
-  auto B = std::make_unique<pf_batch_s>();
+  auto B = std::unique_ptr<pf_batch_s, void (*)(pf_batch_s*)>(new pf_batch_s(), [](pf_batch_s* b) {
+    for (auto& g : b->groups) batch_group_free(g);
+    dfree(b->dev);
+    dfree(b->scan_ctl);
+    delete b;
+  });
   B->ctx = ctx;
   PlanShape sh;
   sh.substeps = loops[0]->substeps;
   sh.iterations = loops[0]->iterations;
-  std::vector<Tile> tl, ctl;
-  std::vector<uint2> stl, btl, ltl;
+  std::vector<PlanShape> keys;  // each group's first scene's shape
   for (uint32_t s = 0; s < n; ++s) {
     pf_loop* L = loops[s];
     loop_alive(L, "Batch");
@@ -2009,36 +2043,43 @@ int pf_batch_create(pf_context* ctx, pf_loop* const* loops, uint32_t n, pf_batch
       fail(PF_EINVAL, "Batch: every loop must share substeps and iterations");
     for (uint32_t t = 0; t < s; ++t)
       if (loops[t] == L) fail(PF_EINVAL, "Batch: a loop appears twice");
-    plan_merge(sh, loop_shape(L));
-    loop_append_tiles(L, s, tl, stl, btl, ltl, ctl);
+    const PlanShape own = loop_shape(L);
+    plan_merge(sh, own);
+    size_t g = 0;
+    while (g < keys.size() && !same_schedule(keys[g], own)) ++g;
+    if (g == keys.size()) {
+      keys.push_back(own);
+      BatchGroup grp;
+      grp.shape.substeps = sh.substeps;
+      grp.shape.iterations = sh.iterations;
+      B->groups.push_back(grp);
+    }
+    B->groups[g].scenes.push_back(s);
+    plan_merge(B->groups[g].shape, own);
     B->loops.push_back(L);
   }
   for (auto* L : B->loops)
     if (loop_has_grid(L) != sh.grid)
       fail(PF_EINVAL, "Batch: mixing scenes with and without a neighbour grid (fluids or contacts) is not supported");
   B->shape = sh;
-  B->ntiles = (uint32_t)tl.size();
-  B->nstiles = (uint32_t)stl.size();
-  B->nbtiles = (uint32_t)btl.size();
-  B->nltiles = (uint32_t)ltl.size();
-  if (B->nltiles) {
-    B->ltiles = dalloc<uint2>(B->nltiles);
-    PF_HIP(hipMemcpy(B->ltiles, ltl.data(), ltl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  // Each group adds its own launches: many different schedules run as one
+  // group under the merged schedule instead.
+  constexpr size_t kMaxGroups = 4;
+  if (B->groups.size() > kMaxGroups) {
+    BatchGroup all;
+    all.shape = sh;
+    for (uint32_t s = 0; s < n; ++s) all.scenes.push_back(s);
+    B->groups.assign(1, all);
   }
-  B->tiles = dalloc<Tile>(B->ntiles);
-  PF_HIP(hipMemcpy(B->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  B->nctiles = (uint32_t)ctl.size();
-  if (B->nctiles) {
-    B->ctiles = dalloc<Tile>(B->nctiles);
-    PF_HIP(hipMemcpy(B->ctiles, ctl.data(), ctl.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  }
-  if (B->nstiles) {
-    B->stiles = dalloc<uint2>(B->nstiles);
-    PF_HIP(hipMemcpy(B->stiles, stl.data(), stl.size() * sizeof(uint2), hipMemcpyHostToDevice));
-  }
-  if (B->nbtiles) {
-    B->btiles = dalloc<uint2>(B->nbtiles);
-    PF_HIP(hipMemcpy(B->btiles, btl.data(), btl.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  for (auto& g : B->groups) {
+    std::vector<Tile> tl, ctl;
+    std::vector<uint2> stl, btl, ltl;
+    for (uint32_t s : g.scenes) loop_append_tiles(B->loops[s], s, tl, stl, btl, ltl, ctl);
+    g.tiles = upload_list(tl, &g.ntiles);
+    g.stiles = upload_list(stl, &g.nstiles);
+    g.btiles = upload_list(btl, &g.nbtiles);
+    g.ltiles = upload_list(ltl, &g.nltiles);
+    g.ctiles = upload_list(ctl, &g.nctiles);
   }
   B->dev = dalloc<SceneDev>(n);
   B->scan_ctl = dalloc<uint32_t>(4);
@@ -2052,17 +2093,21 @@ int pf_batch_step(pf_batch* B, double dt) {
   if (!(dt > 0) || !std::isfinite(dt)) fail(PF_EINVAL, "Batch: step dt must be a positive finite number, got %g", dt);
   Context* ctx = B->ctx;
   bool changed = false;
-  PlanShape now;
-  now.substeps = B->shape.substeps;
-  now.iterations = B->shape.iterations;
   for (auto* L : B->loops) {
     loop_alive(L, "Batch");
     changed = loop_prepare_step(L, dt) || changed;
-    plan_merge(now, loop_shape(L));
   }
   // A changed tile-union limit changes the launches' LDS capacity.
-  if (std::memcmp(now.tu_cap, B->shape.tu_cap, sizeof now.tu_cap) != 0) {
-    std::memcpy(B->shape.tu_cap, now.tu_cap, sizeof now.tu_cap);
+  bool cap_changed = false;
+  for (auto& g : B->groups) {
+    PlanShape now;
+    for (uint32_t s : g.scenes) plan_merge(now, loop_shape(B->loops[s]));
+    if (std::memcmp(now.tu_cap, g.shape.tu_cap, sizeof now.tu_cap) != 0) {
+      std::memcpy(g.shape.tu_cap, now.tu_cap, sizeof now.tu_cap);
+      cap_changed = true;
+    }
+  }
+  if (cap_changed) {
     if (B->graph) {
       PF_HIP(hipStreamSynchronize(ctx->stream));  // not destroyed while a replay runs
       (void)hipGraphExecDestroy(B->graph);
@@ -2080,14 +2125,22 @@ int pf_batch_step(pf_batch* B, double dt) {
     PF_HIP(hipMemcpy(B->dev, descs.data(), descs.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
     B->desc_dirty = false;
   }
+  // The groups' steps one after the other on the stream (one graph).
+  auto enqueue_groups = [&] {
+    for (const auto& g : B->groups) {
+      PlanShape sh = g.shape;
+      sh.grid_cap = B->shape.grid_cap;
+      enqueue_step(ctx, B->dev, sh, g.tiles, g.ntiles, g.stiles, g.nstiles, g.btiles, g.nbtiles, g.ltiles,
+                   g.nltiles, g.ctiles, g.nctiles, B->scan_ctl);
+    }
+  };
   if (B->use_graph) {
     if (!B->graph_valid) {
       hipGraph_t g;
       PF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       ctx->capturing = true;
       try {
-        enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles,
-                     B->nltiles, B->ctiles, B->nctiles, B->scan_ctl);
+        enqueue_groups();
       } catch (...) {
         ctx->capturing = false;
         hipGraph_t dummy;
@@ -2102,15 +2155,16 @@ int pf_batch_step(pf_batch* B, double dt) {
     }
     PF_HIP(hipGraphLaunch(B->graph, ctx->stream));
   } else {
-    enqueue_step(ctx, B->dev, B->shape, B->tiles, B->ntiles, B->stiles, B->nstiles, B->btiles, B->nbtiles, B->ltiles,
-                     B->nltiles, B->ctiles, B->nctiles, B->scan_ctl);
+    enqueue_groups();
   }
   PF_HIP(hipGetLastError());
-  const int flips = step_variant_flips(B->shape);
-  for (auto* L : B->loops) loop_finish_step(L, flips);
-  // The graph reads the descriptor array at run time; swapped buffer pairs
-  // only need the array re-uploaded before the next step.
-  if (flips) B->desc_dirty = true;
+  for (const auto& g : B->groups) {
+    const int flips = step_variant_flips(g.shape);
+    for (uint32_t s : g.scenes) loop_finish_step(B->loops[s], flips);
+    // The graph reads the descriptor array at run time; swapped buffer pairs
+    // only need the array re-uploaded before the next step.
+    if (flips) B->desc_dirty = true;
+  }
   PF_API_END
 }
 
@@ -2283,7 +2337,8 @@ int pf_batch_destroy(pf_batch* B) {
   if (!B) return PF_OK;
   (void)hipStreamSynchronize(B->ctx->stream);
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
-  dfree(B->dev); dfree(B->tiles); dfree(B->stiles); dfree(B->btiles); dfree(B->ltiles); dfree(B->ctiles);
+  for (auto& g : B->groups) batch_group_free(g);
+  dfree(B->dev);
   dfree(B->scan_ctl);
   delete B;
   PF_API_END

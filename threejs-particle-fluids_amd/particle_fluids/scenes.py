@@ -156,25 +156,27 @@ SIMULATION_CONTACT = {"muS": 0.3, "muK": 0.2}
 
 
 def build_mixed_scene(fluid_budget, bodies, seed, substeps=2, iterations=4, context=None, use_graph=False,
-                      evictable=False, contact=None):
+                      evictable=False, contact=None, fluid_options=None):
     """One scene of the batched / served mixes on the GPU: the dam break with
-    Simulation's default fluid effects, `bodies` globally shape-matched cubes
-    and the five planes of the (extended) tank. `evictable` places every
-    device buffer in one arena so a Server can move the scene to host memory.
-    `contact` is SimLoop's option; None turns contacts on as Simulation does:
-    with SIMULATION_CONTACT whenever the scene has cubes."""
+    Simulation's default fluid effects (or `fluid_options`), `bodies` globally
+    shape-matched cubes and the five planes of the (extended) tank.
+    `evictable` places every device buffer in one arena so a Server can move
+    the scene to host memory. `contact` is SimLoop's option; None turns
+    contacts on as Simulation does: with SIMULATION_CONTACT whenever the scene
+    has cubes."""
     lay = mixed_layout(fluid_budget, bodies, seed)
-    return build_layout_scene(lay, substeps, iterations, context, use_graph, evictable, contact)
+    return build_layout_scene(lay, substeps, iterations, context, use_graph, evictable, contact, fluid_options)
 
 
-def build_layout_scene(lay, substeps=2, iterations=4, context=None, use_graph=False, evictable=False, contact=None):
+def build_layout_scene(lay, substeps=2, iterations=4, context=None, use_graph=False, evictable=False, contact=None,
+                       fluid_options=None):
     """The GPU scene of a mixed_layout-shaped dict (positions, inv_mass,
     fluid_count, bodies, radius, container); see build_mixed_scene."""
     pos, nf, r = lay["positions"], lay["fluid_count"], lay["radius"]
     bodies = len(lay["bodies"])
     ps = ParticleSystem(len(pos), r, context=context, evictable=evictable)
     ps.uploadParticles(pos, invMass=lay["inv_mass"])
-    fluid = FluidSystem(ps, range=(0, nf), **FLUID_DEFAULTS)
+    fluid = FluidSystem(ps, range=(0, nf), **(FLUID_DEFAULTS if fluid_options is None else fluid_options))
     mats = [fluid]
     soft = None
     if bodies:
