@@ -55,16 +55,16 @@ SUBSTEP_BYTES_PER_PARTICLE = lambda iters: 336 + 48 * iters  # noqa: E731
 # bench's window (steps 5-25), from the rocprofv3 counter summary
 # profiles/dam_break_1m_hbm_traffic.md.
 HBM_TRAFFIC_B_PER_PARTICLE = {
-    "neighbors": 646.8,
-    "impulses": 402.1,
+    "neighbors": 667.2,
+    "impulses": 402.2,
     "lambda": 192.5,
-    "delta_walls": 224.4,
-    "velocity": 213.7,
+    "delta_walls": 224.5,
+    "velocity": 213.8,
     "vorticity_friction": 264.9,
-    "rank_permute": 229.3,
-    "hash_predict": 59.7,
+    "rank_permute": 238.7,
+    "hash_predict": 59.6,
     "impulse_apply": 116.1,
-    "place": 29.7,
+    "place": 29.8,
 }
 SCAN_BYTES_PER_BUCKET = 12
 
@@ -74,21 +74,21 @@ SCAN_BYTES_PER_BUCKET = 12
 # issue rate with packed FP32 off: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
 # instruction per SIMD every 4 cycles.
 VALU_INSTS_PER_PARTICLE = {
-    "neighbors": 176.5,
+    "neighbors": 179.4,
     "impulses": 73.0,
     "lambda": 43.7,
     "delta_walls": 55.1,
     "velocity": 63.2,
     "vorticity_friction": 42.6,
-    "rank_permute": 3.2,
+    "rank_permute": 3.9,
     "hash_predict": 2.8,
 }
 VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 # The whole substep's VALU instructions per particle, from the same summary:
-# the kernels once per substep (neighbour build 176.5, surface tension 73.0
-# and its apply 0.5, velocity 63.2, vorticity 42.6, the sort 6.3) and the
+# the kernels once per substep (neighbour build 179.4, surface tension 73.0
+# and its apply 0.5, velocity 63.2, vorticity 42.6, the sort 7.0) and the
 # density and correction passes once per iteration (43.7 + 55.1).
-VALU_SUBSTEP_INSTS_PER_PARTICLE = lambda iters: 362.1 + 98.8 * iters  # noqa: E731
+VALU_SUBSTEP_INSTS_PER_PARTICLE = lambda iters: 365.7 + 98.8 * iters  # noqa: E731
 
 # The isolated sort: 4,194,304 positions, cell 0.01, 2^22 buckets. Algorithmic
 # bytes per sort 32 N + 12 H: per point read x (16), write and read its key
